@@ -1,0 +1,262 @@
+/*
+ * sl_gpu.h -- C ABI of the MI355X (gfx950) SoftLight triangle pipeline.
+ *
+ * This is the drop-in boundary for SoftLight's `SL_Context::draw` path.  Every
+ * entry point below replaces one member of the reference's C++ API; the
+ * citation on each names the reference declaration/definition it stands in
+ * for (paths relative to the SoftLight source tree).  Resource handles keep the
+ * reference's semantics: dense `size_t` indices returned by `sl_create_*`.
+ * Error codes keep the reference's integer conventions (0 = ok, negative =
+ * error) and add SL_ERR_* codes where the reference had undefined behaviour
+ * (bad handles).
+ *
+ * Host function-pointer shaders (SL_Shader.hpp:58-98) cannot run on the GPU;
+ * shaders select a built-in device program by `programId` instead, and take a
+ * POD uniform block (no host pointers).  See sl_std_uniforms below.
+ *
+ * Execution model: draw and clear calls are recorded into a per-context
+ * command list and executed as one batched frame on the context's HIP stream
+ * at the next synchronisation point (sl_finish, a readback, a resource upload,
+ * or a draw into a different framebuffer).  Uniform blocks are snapshotted at
+ * draw time, so updating a uniform buffer between draws behaves as in the
+ * reference's synchronous `draw`.
+ */
+#ifndef SL_GPU_H
+#define SL_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * Limits (SL_Config.hpp:36-58, SL_VertexArray.hpp:18, SL_UniformBuffer.hpp:12)
+ * ------------------------------------------------------------------------- */
+#define SL_SHADER_MAX_VARYING_VECTORS 4
+#define SL_SHADER_MAX_FRAG_OUTPUTS    4
+#define SL_VAO_MAX_BINDINGS           8
+#define SL_MAX_UNIFORM_BUFFER_SIZE    1024
+#define SL_INVALID_ID                 ((size_t)-1)
+#define SL_NO_RESOURCE                0xFFFFFFFFu
+
+/* ---------------------------------------------------------------------------
+ * Error codes.  Reference init() calls return 0 / small negative numbers; the
+ * codes below are used where the reference had no error path.
+ * ------------------------------------------------------------------------- */
+#define SL_OK                  0
+#define SL_ERR_INVALID_HANDLE  (-100)
+#define SL_ERR_INVALID_ARG     (-101)
+#define SL_ERR_OUT_OF_MEMORY   (-102)
+#define SL_ERR_DEVICE          (-103)
+#define SL_ERR_UNSUPPORTED     (-104)
+#define SL_ERR_CAPACITY        (-105)
+#define SL_ERR_STATE           (-106)
+
+/* ---------------------------------------------------------------------------
+ * Enumerations, values identical to the reference.
+ * ------------------------------------------------------------------------- */
+/* SL_PipelineState.hpp: SL_CullMode */
+enum { SL_CULL_BACK_FACE = 0, SL_CULL_FRONT_FACE = 1, SL_CULL_OFF = 2 };
+/* SL_PipelineState.hpp: SL_DepthTest */
+enum {
+    SL_DEPTH_TEST_OFF = 0, SL_DEPTH_TEST_LESS_THAN, SL_DEPTH_TEST_LESS_EQUAL,
+    SL_DEPTH_TEST_GREATER_THAN, SL_DEPTH_TEST_GREATER_EQUAL,
+    SL_DEPTH_TEST_EQUAL, SL_DEPTH_TEST_NOT_EQUAL
+};
+/* SL_PipelineState.hpp: SL_DepthMask */
+enum { SL_DEPTH_MASK_OFF = 0, SL_DEPTH_MASK_ON = 1 };
+/* SL_PipelineState.hpp: SL_BlendMode */
+enum {
+    SL_BLEND_OFF = 0, SL_BLEND_ALPHA, SL_BLEND_PREMULTIPLED_ALPHA,
+    SL_BLEND_ADDITIVE, SL_BLEND_SCREEN
+};
+/* SL_Mesh.hpp:13-23: SL_RenderMode */
+enum {
+    RENDER_MODE_POINTS = 0x01, RENDER_MODE_INDEXED_POINTS = 0x03,
+    RENDER_MODE_LINES = 0x04, RENDER_MODE_INDEXED_LINES = 0x0C,
+    RENDER_MODE_TRIANGLES = 0x10, RENDER_MODE_INDEXED_TRIANGLES = 0x30,
+    RENDER_MODE_TRI_WIRE = 0x50, RENDER_MODE_INDEXED_TRI_WIRE = 0x70
+};
+/* SL_Geometry.hpp:17-39: SL_Dimension, SL_DataType */
+enum { VERTEX_DIMENSION_1 = 0, VERTEX_DIMENSION_2, VERTEX_DIMENSION_3, VERTEX_DIMENSION_4 };
+enum {
+    VERTEX_DATA_BYTE = 0, VERTEX_DATA_SHORT, VERTEX_DATA_INT, VERTEX_DATA_LONG,
+    VERTEX_DATA_FLOAT, VERTEX_DATA_DOUBLE, VERTEX_DATA_INVALID
+};
+/* SL_Color.hpp:31-73: SL_ColorDataType (the formats this build supports are
+ * R_8U, RGBA_8U, R_32U, R_FLOAT, RGBA_FLOAT; the others are rejected). */
+enum {
+    SL_COLOR_R_8U = 0, SL_COLOR_R_16U, SL_COLOR_R_32U, SL_COLOR_R_64U,
+    SL_COLOR_R_HALF, SL_COLOR_R_FLOAT, SL_COLOR_R_DOUBLE,
+    SL_COLOR_RG_8U, SL_COLOR_RG_16U, SL_COLOR_RG_32U, SL_COLOR_RG_64U,
+    SL_COLOR_RG_HALF, SL_COLOR_RG_FLOAT, SL_COLOR_RG_DOUBLE,
+    SL_COLOR_RGB_8U, SL_COLOR_RGB_16U, SL_COLOR_RGB_32U, SL_COLOR_RGB_64U,
+    SL_COLOR_RGB_HALF, SL_COLOR_RGB_FLOAT, SL_COLOR_RGB_DOUBLE,
+    SL_COLOR_RGBA_8U, SL_COLOR_RGBA_16U, SL_COLOR_RGBA_32U, SL_COLOR_RGBA_64U,
+    SL_COLOR_RGBA_HALF, SL_COLOR_RGBA_FLOAT, SL_COLOR_RGBA_DOUBLE
+};
+
+/* ---------------------------------------------------------------------------
+ * Built-in device shader programs (replace the host function pointers of
+ * SL_VertexShader::shader / SL_FragmentShader::shader, SL_Shader.hpp:58-98).
+ *
+ * SL_PROGRAM_PASSTHROUGH_GOURAUD
+ *   VS: position = binding 0 (float, 3 or 4 components; w = 1 when 3);
+ *       varying0 = binding 1 colour (4 x u8 scaled by 1/255 as
+ *       color_cast<float,uint8_t>, SL_Color.hpp:666-669, or 4 x float).
+ *   FS: output0 = varying0.  (Pattern of tests/sl_draw_test.cpp:36-81.)
+ * SL_PROGRAM_LAMBERT_TEXTURED
+ *   VBO vertex = {float pos[3]; float normal[3]; float uv[2]} (bindings 0,1,2).
+ *   VS: model = uniforms.model, or the instance's mat4 from uniforms.instance_vbo;
+ *       world = model*(pos,1); n = model*(normal,0); clip = uniforms.mvp*world.
+ *       varying0 = (u, v, 0, 0), varying1 = (n, 0).
+ *   FS: albedo = bilinear REPEAT sample of an RGBA8 texture (SL_Sampler.hpp:186-218);
+ *       lambert = max(0, -dot(normalize(n), light_dir));
+ *       rgb = min(albedo.rgb * (ambient + diffuse*lambert) * tint.rgb, 1);
+ *       a = albedo.a * tint.a.  (Pattern of tests/sl_instancing_test.cpp:105-178.)
+ * SL_PROGRAM_COLOR_TEXTURED
+ *   VBO vertex = {float pos[3]; float rgb[3]; float uv[2]}.
+ *   VS: clip = uniforms.mvp*(model*(pos,1)); varying0 = (rgb, 1); varying1 = (u, v, 0, 0).
+ *   FS: output0 = varying0 * bilinear REPEAT sample of an RGBA32F texture.
+ *       This is the program of the differentiable mode.
+ * ------------------------------------------------------------------------- */
+enum {
+    SL_PROGRAM_NONE = 0,
+    SL_PROGRAM_PASSTHROUGH_GOURAUD = 1,
+    SL_PROGRAM_LAMBERT_TEXTURED = 2,
+    SL_PROGRAM_COLOR_TEXTURED = 3
+};
+
+/* POD uniform block read by the built-in programs (lives in an SL_UniformBuffer,
+ * SL_UniformBuffer.hpp:12; matrices column-major like ls::math::mat4). */
+typedef struct sl_std_uniforms {
+    float mvp[16];
+    float model[16];
+    float light_dir[4];
+    float ambient[4];
+    float diffuse[4];
+    float tint[4];
+    uint32_t texture;       /* texture handle or SL_NO_RESOURCE */
+    uint32_t instance_vbo;  /* vbo holding one mat4 per instance, or SL_NO_RESOURCE */
+    uint32_t flags;
+    uint32_t reserved;
+} sl_std_uniforms;
+
+/* SL_Mesh.hpp:30-39 */
+typedef struct sl_mesh {
+    size_t vaoId;
+    size_t elementBegin;
+    size_t elementEnd;
+    uint32_t mode;
+    uint32_t materialId;
+} sl_mesh;
+
+/* SL_Shader.hpp:41-64: SL_VertexShader, with the host pointer replaced by a program id */
+typedef struct sl_vertex_shader {
+    uint8_t  numVaryings;
+    uint8_t  cullMode;
+    uint32_t programId;
+} sl_vertex_shader;
+
+/* SL_Shader.hpp:74-98: SL_FragmentShader, with the host pointer replaced by a program id */
+typedef struct sl_fragment_shader {
+    uint8_t  numVaryings;
+    uint8_t  numOutputs;
+    uint8_t  blend;
+    uint8_t  depthTest;
+    uint8_t  depthMask;
+    uint32_t programId;
+} sl_fragment_shader;
+
+/* Per-frame counters.  Not in the reference (it has no counters); they carry
+ * the quantities the parity tests and the benchmark report. */
+typedef struct sl_frame_stats {
+    uint64_t triangles_submitted;   /* triangles fetched by the vertex stage */
+    uint64_t triangles_binned;      /* setup records that reached binning (after cull/clip/bbox reject) */
+    uint64_t tile_pairs;            /* (tile, record) bin entries */
+    uint64_t fragments_tested;      /* covered samples that were depth tested */
+    uint64_t fragments_shaded;      /* samples that passed the depth test and ran the fragment program */
+    uint64_t frames;                /* batched frames executed */
+    double   ms_vertex;             /* vertex + setup stage (device time) */
+    double   ms_bin;                /* tile binning (device time) */
+    double   ms_raster;             /* raster + fragment stage (device time) */
+    double   ms_total;              /* whole batched frame (device time) */
+} sl_frame_stats;
+
+typedef struct sl_context sl_context;
+
+/* ---- context (SL_Context.hpp:59-306) ------------------------------------ */
+int         sl_context_create(sl_context** outCtx);            /* SL_Context::SL_Context(), SL_Context.cpp */
+void        sl_context_destroy(sl_context* ctx);               /* SL_Context::~SL_Context / terminate(), SL_Context.cpp */
+const char* sl_last_error(const sl_context* ctx);
+unsigned    sl_num_threads(const sl_context* ctx);             /* SL_Context::num_threads(), SL_Context.cpp:1135 */
+unsigned    sl_set_num_threads(sl_context* ctx, unsigned n);   /* SL_Context::num_threads(unsigned), SL_Context.cpp:1145 */
+int         sl_set_viewport(sl_context* ctx, int32_t x, int32_t y, int32_t w, int32_t h); /* SL_ViewportState::viewport, SL_ViewportState.hpp:150 */
+int         sl_set_scissor(sl_context* ctx, int32_t x, int32_t y, int32_t w, int32_t h);  /* SL_ViewportState::scissor, SL_ViewportState.hpp:175 */
+
+/* ---- vertex buffers (SL_VertexBuffer.hpp / .cpp) ------------------------ */
+size_t sl_create_vbo(sl_context* ctx);                                                      /* SL_Context::create_vbo */
+int    sl_vbo_init(sl_context* ctx, size_t vbo, size_t numBytes, const void* data);         /* SL_VertexBuffer::init, SL_VertexBuffer.cpp:96 */
+int    sl_vbo_assign(sl_context* ctx, size_t vbo, const void* data, size_t offset, size_t numBytes); /* SL_VertexBuffer::assign, SL_VertexBuffer.hpp */
+
+/* ---- index buffers (SL_IndexBuffer.hpp / .cpp) -------------------------- */
+size_t sl_create_ibo(sl_context* ctx);                                                      /* SL_Context::create_ibo */
+int    sl_ibo_init(sl_context* ctx, size_t ibo, uint32_t numElements, int dataType, const void* data); /* SL_IndexBuffer::init, SL_IndexBuffer.cpp:138 */
+
+/* ---- vertex arrays (SL_VertexArray.hpp / .cpp) -------------------------- */
+size_t sl_create_vao(sl_context* ctx);                                                      /* SL_Context::create_vao, SL_Context.cpp:181 */
+int    sl_vao_set_vertex_buffer(sl_context* ctx, size_t vao, size_t vbo);                   /* SL_VertexArray::set_vertex_buffer */
+int    sl_vao_set_index_buffer(sl_context* ctx, size_t vao, size_t ibo);                    /* SL_VertexArray::set_index_buffer */
+int    sl_vao_set_num_bindings(sl_context* ctx, size_t vao, size_t numBindings);            /* SL_VertexArray::set_num_bindings, SL_VertexArray.cpp:162 */
+int    sl_vao_set_binding(sl_context* ctx, size_t vao, size_t bindId, ptrdiff_t offset,
+                          ptrdiff_t stride, int dimens, int dataType);                     /* SL_VertexArray::set_binding, SL_VertexArray.cpp:205 */
+
+/* ---- textures (SL_Texture.hpp / .cpp) ----------------------------------- */
+size_t sl_create_texture(sl_context* ctx);                                                  /* SL_Context::create_texture, SL_Context.cpp:232 */
+int    sl_texture_init(sl_context* ctx, size_t tex, int colorType, uint16_t w, uint16_t h, uint16_t d); /* SL_Texture::init, SL_Texture.cpp:276 */
+int    sl_texture_set_texels(sl_context* ctx, size_t tex, const void* data, size_t numBytes);           /* SL_Texture::set_texels (ordered layout) */
+int    sl_texture_read(sl_context* ctx, size_t tex, void* out, size_t numBytes);            /* SL_Texture::data() readback */
+
+/* ---- framebuffers (SL_Framebuffer.hpp / .cpp) --------------------------- */
+size_t sl_create_framebuffer(sl_context* ctx);                                              /* SL_Context::create_framebuffer */
+int    sl_fbo_reserve_color_buffers(sl_context* ctx, size_t fbo, unsigned n);              /* SL_Framebuffer::reserve_color_buffers, SL_Framebuffer.cpp:823 */
+int    sl_fbo_attach_color_buffer(sl_context* ctx, size_t fbo, unsigned index, size_t tex); /* SL_Framebuffer::attach_color_buffer, SL_Framebuffer.cpp:855 */
+int    sl_fbo_attach_depth_buffer(sl_context* ctx, size_t fbo, size_t tex);                 /* SL_Framebuffer::attach_depth_buffer, SL_Framebuffer.cpp:890 */
+int    sl_fbo_valid(sl_context* ctx, size_t fbo);                                           /* SL_Framebuffer::valid, SL_Framebuffer.cpp:930-990 */
+/* Optional parity attachments (no counterpart in the reference): per pixel,
+ * the 1-based submission number of the last triangle that passed the depth
+ * test, and the number of depth-test passes.  Both must be R_32U textures of
+ * the framebuffer's size; pass SL_INVALID_ID to detach. */
+int    sl_fbo_attach_debug_buffers(sl_context* ctx, size_t fbo, size_t lastPrimTex, size_t passCountTex);
+
+/* ---- uniform buffers (SL_UniformBuffer.hpp) ----------------------------- */
+size_t sl_create_ubo(sl_context* ctx);                                                      /* SL_Context::create_ubo */
+int    sl_ubo_assign(sl_context* ctx, size_t ubo, const void* data, size_t offset, size_t numBytes); /* SL_UniformBuffer::assign */
+
+/* ---- shaders (SL_Context.cpp:488-604) ------------------------------------ */
+size_t sl_create_shader(sl_context* ctx, const sl_vertex_shader* vs,
+                        const sl_fragment_shader* fs, size_t uboId);   /* SL_Context::create_shader; uboId may be SL_INVALID_ID */
+
+/* ---- drawing (SL_Context.cpp:714-740) ------------------------------------ */
+int sl_draw(sl_context* ctx, const sl_mesh* m, size_t shaderId, size_t fboId);                        /* SL_Context::draw */
+int sl_draw_multiple(sl_context* ctx, const sl_mesh* m, size_t numMeshes, size_t shaderId, size_t fboId); /* SL_Context::draw_multiple */
+int sl_draw_instanced(sl_context* ctx, const sl_mesh* m, size_t numInstances, size_t shaderId, size_t fboId); /* SL_Context::draw_instanced */
+
+/* ---- clears (SL_Context.cpp:881-964) ------------------------------------- */
+int sl_clear_color_buffer(sl_context* ctx, size_t fbo, unsigned attachment, const double rgba[4]);      /* SL_Context::clear_color_buffer */
+int sl_clear_depth_buffer(sl_context* ctx, size_t fbo, double depth);                                  /* SL_Context::clear_depth_buffer */
+int sl_clear_framebuffer(sl_context* ctx, size_t fbo, unsigned attachment, const double rgba[4], double depth); /* SL_Context::clear_framebuffer */
+
+/* ---- synchronisation and counters (no reference counterpart: the
+ *      reference's draw is synchronous, SL_ProcessorPool.cpp:326) ---------- */
+int sl_finish(sl_context* ctx);
+int sl_get_stats(const sl_context* ctx, sl_frame_stats* out);
+int sl_reset_stats(sl_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SL_GPU_H */

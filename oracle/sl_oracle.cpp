@@ -1,0 +1,1364 @@
+/*
+ * sl_oracle.cpp -- CPU restatement of SoftLight's triangle draw path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the MI355X
+ * pipeline in softlight_amd/: tests, the smoke check and the benchmark's CPU
+ * baseline load it; the product never does.  It exports the same C API as
+ * include/sl_gpu.h with the prefix `slo_` instead of `sl_`.
+ *
+ * It restates the reference's portable scalar path (the `#else` branches of
+ * the ISA-specific code) with every floating-point operation written out in a
+ * fixed order, because the reference's own results depend on its absent
+ * LightMath submodule and on the host ISA.  The pinned choices (each cited
+ * where it is used):
+ *   rcp(x)        = 1.0f/x (IEEE)                    SL_TriProcessor.cpp:281,440
+ *   fmadd(a,b,c)  = fmaf(a,b,c) (fused)              SL_PostVertexTransform.hpp:516-523, SL_TriRasterizer.cpp:857
+ *   dot/sum       = left to right, unfused           SL_TriProcessor.cpp:115, SL_FragmentProcessor.cpp:491
+ *   mat4*vec4     = ((c0*x + c1*y) + c2*z) + c3*w     SL_TriProcessor.cpp:615-617
+ *   mix(a,b,t)    = a + t*(b-a)                      SL_TriProcessor.cpp:410,441
+ *   float->int    = truncate; NaN/out of range -> INT32_MIN (x86 cvttss2si)
+ *   max(a,b)      = (a < b) ? b : a                  SL_PostVertexTransform.hpp:520-523
+ * The build runs with -ffp-contract=off so that no other operation fuses.
+ *
+ * Deviations from the reference (undefined behaviour there):
+ *   - a triangle whose snapped screen x or y is not finite is dropped;
+ *   - a trailing partial triangle (elementEnd-elementBegin not a multiple of 3)
+ *     is skipped instead of reading past elementEnd;
+ *   - vertex reads beyond the end of a vertex buffer return zeros.
+ *
+ * Parity status of each part and the checks that pin it are listed at the top
+ * of tests/test_oracle.py.
+ */
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../include/sl_gpu.h"
+
+namespace {
+
+/*-----------------------------------------------------------------------------
+ * Pinned scalar arithmetic
+-----------------------------------------------------------------------------*/
+struct V4 { float v[4]; };
+
+inline float rcpf(float x) { return 1.0f / x; }
+
+inline int32_t f2i(float f)
+{
+    if (!(f >= -2147483648.0f && f < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)f;
+}
+
+inline float maxf_ref(float a, float b) { return (a < b) ? b : a; }
+
+inline V4 v4(float a, float b, float c, float d) { V4 r; r.v[0] = a; r.v[1] = b; r.v[2] = c; r.v[3] = d; return r; }
+
+// ls::math::mat4 * vec4, column-major, SL_TriProcessor.cpp:615-617
+inline V4 mat_mul(const float* m, const V4& p)
+{
+    V4 r;
+    for (int i = 0; i < 4; ++i)
+    {
+        float a = m[0 * 4 + i] * p.v[0];
+        float b = m[1 * 4 + i] * p.v[1];
+        float c = m[2 * 4 + i] * p.v[2];
+        float d = m[3 * 4 + i] * p.v[3];
+        r.v[i] = ((a + b) + c) + d;
+    }
+    return r;
+}
+
+inline float dot4(const V4& a, const V4& b)
+{
+    float p0 = a.v[0] * b.v[0];
+    float p1 = a.v[1] * b.v[1];
+    float p2 = a.v[2] * b.v[2];
+    float p3 = a.v[3] * b.v[3];
+    return ((p0 + p1) + p2) + p3;
+}
+
+inline V4 mix4(const V4& a, const V4& b, float t)
+{
+    V4 r;
+    for (int i = 0; i < 4; ++i) r.v[i] = a.v[i] + t * (b.v[i] - a.v[i]);
+    return r;
+}
+
+/*-----------------------------------------------------------------------------
+ * Resources
+-----------------------------------------------------------------------------*/
+struct Vbo { bool valid = false; std::vector<uint8_t> bytes; };
+
+struct Ibo { bool valid = false; int type = VERTEX_DATA_INT; uint32_t count = 0; uint32_t bpe = 4; std::vector<uint8_t> bytes; };
+
+struct Binding { int dims = VERTEX_DIMENSION_4; int type = VERTEX_DATA_FLOAT; ptrdiff_t offset = 0; ptrdiff_t stride = 0; };
+
+struct Vao { size_t vbo = SL_INVALID_ID; size_t ibo = SL_INVALID_ID; size_t nb = 0; Binding b[SL_VAO_MAX_BINDINGS]; };
+
+struct Tex
+{
+    bool valid = false;
+    int type = SL_COLOR_RGBA_8U;
+    uint16_t w = 0, h = 0, d = 0;
+    uint32_t bpt = 0;
+    std::vector<uint8_t> data;   // padded + zeroed as SL_Texture.cpp:28-55
+};
+
+struct Fbo
+{
+    std::vector<size_t> colors;
+    size_t depth = SL_INVALID_ID;
+    size_t dbgLast = SL_INVALID_ID;
+    size_t dbgCount = SL_INVALID_ID;
+};
+
+struct Ubo { uint8_t bytes[SL_MAX_UNIFORM_BUFFER_SIZE]; };
+
+struct Shader
+{
+    uint8_t cull, dtest, dmask, blend, nvary, nout;
+    uint32_t vsProg, fsProg;
+    size_t ubo;
+};
+
+} // namespace
+
+struct sl_context
+{
+    std::vector<Vbo> vbos;
+    std::vector<Ibo> ibos;
+    std::vector<Vao> vaos;
+    std::vector<Tex> texs;
+    std::vector<Fbo> fbos;
+    std::vector<Ubo> ubos;
+    std::vector<Shader> shaders;
+    int32_t viewport[4] = {0, 0, 65535, 65535};   // SL_ViewportState.hpp:80-81
+    int32_t scissor[4]  = {0, 0, 65535, 65535};
+    unsigned numThreads = 1;
+    sl_frame_stats stats{};
+    uint32_t triSeq = 0;                          // submitted-triangle counter (debug buffers)
+    std::string err;
+};
+
+namespace {
+
+uint32_t bytes_per_color(int type)
+{
+    switch (type)
+    {
+        case SL_COLOR_R_8U: return 1;
+        case SL_COLOR_R_32U: return 4;
+        case SL_COLOR_R_FLOAT: return 4;
+        case SL_COLOR_RGBA_8U: return 4;
+        case SL_COLOR_RGBA_FLOAT: return 16;
+        default: return 0;
+    }
+}
+
+uint32_t bytes_per_index(int type)
+{
+    switch (type)
+    {
+        case VERTEX_DATA_BYTE: return 1;
+        case VERTEX_DATA_SHORT: return 2;
+        case VERTEX_DATA_INT: return 4;
+        default: return 0;
+    }
+}
+
+/*-----------------------------------------------------------------------------
+ * Per-draw state
+-----------------------------------------------------------------------------*/
+struct TVert { V4 pos; V4 var[SL_SHADER_MAX_VARYING_VECTORS]; };
+
+struct Bin
+{
+    V4 p[3];                   // screen x, y (snapped), ndc z, 1/w
+    V4 bx, by, bz;             // barycentric planes ddx*denom, ddy*denom, ddz*denom
+    V4 var[3][SL_SHADER_MAX_VARYING_VECTORS];
+    uint32_t seq;
+};
+
+struct Target
+{
+    int W, H;
+    float* depth;
+    uint8_t* color8;          // RGBA8 attachment 0, or null
+    float* colorF;            // RGBA32F attachment 0, or null
+    uint32_t* dbgLast;
+    uint32_t* dbgCount;
+};
+
+struct Draw
+{
+    sl_context* ctx;
+    const Shader* sh;
+    const uint8_t* uni;       // uniform snapshot (may be null)
+    Target tgt;
+    float scissorMat[16];
+    float vpDims[4];
+    const Vao* vao;
+    const Vbo* vbo;
+    const Ibo* ibo;
+    uint64_t tested = 0, shaded = 0, binned = 0;
+};
+
+const sl_std_uniforms* uniforms(const Draw& d)
+{
+    return reinterpret_cast<const sl_std_uniforms*>(d.uni);
+}
+
+/*-----------------------------------------------------------------------------
+ * Textures and samplers (SL_Sampler.hpp:51-218, SL_Texture.hpp:190)
+-----------------------------------------------------------------------------*/
+// SL_WrapModeRepeat (SL_Sampler.hpp:111-114): (u < 0 ? 1 : 0) + fmod_1(u),
+// fmod_1 taken as the sign-preserving fractional part u - trunc(u).
+inline float wrap_repeat(float u)
+{
+    return (u < 0.f ? 1.f : 0.f) + (u - truncf(u));
+}
+
+inline uint16_t to_u16(float f) { return (uint16_t)(uint32_t)f2i(f); }
+
+// sl_sample_bilinear<SL_ColorRGBA8, REPEAT> (SL_Sampler.hpp:186-218): the
+// x+1/y+1 neighbours clamp to W/H, so x == W aliases the next row's first
+// texel and y == H reads the zeroed allocation padding.
+void sample_bilinear_rgba8(const Tex& t, float u, float v, uint8_t out[4])
+{
+    const float xf = wrap_repeat(u) * (float)t.w;
+    const float yf = wrap_repeat(v) * (float)t.h;
+    const uint16_t xi0 = to_u16(xf);
+    const uint16_t yi0 = to_u16(yf);
+    const uint16_t xi1 = (uint16_t)std::min<uint32_t>((uint32_t)xi0 + 1u, t.w);
+    const uint16_t yi1 = (uint16_t)std::min<uint32_t>((uint32_t)yi0 + 1u, t.h);
+    const float dx = xf - (float)xi0;
+    const float dy = yf - (float)yi0;
+    const float omdx = 1.f - dx;
+    const float omdy = 1.f - dy;
+    const uint8_t* px = t.data.data();
+    const size_t i0 = (size_t)xi0 + (size_t)t.w * yi0;
+    const size_t i1 = (size_t)xi0 + (size_t)t.w * yi1;
+    const size_t i2 = (size_t)xi1 + (size_t)t.w * yi0;
+    const size_t i3 = (size_t)xi1 + (size_t)t.w * yi1;
+    for (int c = 0; c < 4; ++c)
+    {
+        const float p0 = (float)px[i0 * 4 + c] * 0.00392156862745f;
+        const float p1 = (float)px[i1 * 4 + c] * 0.00392156862745f;
+        const float p2 = (float)px[i2 * 4 + c] * 0.00392156862745f;
+        const float p3 = (float)px[i3 * 4 + c] * 0.00392156862745f;
+        const float w0 = (p0 * omdx) * omdy;
+        const float w1 = (p1 * omdx) * dy;
+        const float w2 = (p2 * dx) * omdy;
+        const float w3 = (p3 * dx) * dy;
+        const float s = ((w0 + w1) + w2) + w3;
+        out[c] = (uint8_t)(uint32_t)f2i(s * 255.f);   // color_cast<uint8_t,float>, SL_Color.hpp:621-626
+    }
+}
+
+// Same sampler for an RGBA32F texture: color_cast<float,float> is the identity.
+void sample_bilinear_rgbaf(const Tex& t, float u, float v, float out[4])
+{
+    const float xf = wrap_repeat(u) * (float)t.w;
+    const float yf = wrap_repeat(v) * (float)t.h;
+    const uint16_t xi0 = to_u16(xf);
+    const uint16_t yi0 = to_u16(yf);
+    const uint16_t xi1 = (uint16_t)std::min<uint32_t>((uint32_t)xi0 + 1u, t.w);
+    const uint16_t yi1 = (uint16_t)std::min<uint32_t>((uint32_t)yi0 + 1u, t.h);
+    const float dx = xf - (float)xi0;
+    const float dy = yf - (float)yi0;
+    const float omdx = 1.f - dx;
+    const float omdy = 1.f - dy;
+    const float* px = reinterpret_cast<const float*>(t.data.data());
+    const size_t i0 = (size_t)xi0 + (size_t)t.w * yi0;
+    const size_t i1 = (size_t)xi0 + (size_t)t.w * yi1;
+    const size_t i2 = (size_t)xi1 + (size_t)t.w * yi0;
+    const size_t i3 = (size_t)xi1 + (size_t)t.w * yi1;
+    for (int c = 0; c < 4; ++c)
+    {
+        const float w0 = (px[i0 * 4 + c] * omdx) * omdy;
+        const float w1 = (px[i1 * 4 + c] * omdx) * dy;
+        const float w2 = (px[i2 * 4 + c] * dx) * omdy;
+        const float w3 = (px[i3 * 4 + c] * dx) * dy;
+        out[c] = ((w0 + w1) + w2) + w3;
+    }
+}
+
+/*-----------------------------------------------------------------------------
+ * Built-in shader programs (the device programs of include/sl_gpu.h)
+-----------------------------------------------------------------------------*/
+// SL_VertexArray::offset(bind, vert) = offset + stride*vert (SL_VertexArray.hpp:120)
+void read_attr(const Draw& d, int bind, size_t vertId, float out[4], float defW)
+{
+    out[0] = out[1] = out[2] = 0.f;
+    out[3] = defW;
+    if ((size_t)bind >= d.vao->nb || d.vbo == nullptr) return;
+    const Binding& b = d.vao->b[bind];
+    const ptrdiff_t off = b.offset + b.stride * (ptrdiff_t)vertId;
+    const int n = b.dims + 1;
+    const uint8_t* base = d.vbo->bytes.data();
+    const size_t size = d.vbo->bytes.size();
+    if (b.type == VERTEX_DATA_FLOAT)
+    {
+        if (off < 0 || (size_t)off + 4u * n > size) return;
+        for (int i = 0; i < n; ++i) std::memcpy(&out[i], base + off + 4 * i, 4);
+    }
+    else if (b.type == VERTEX_DATA_BYTE)
+    {
+        if (off < 0 || (size_t)off + n > size) return;
+        for (int i = 0; i < n; ++i) out[i] = (float)base[off + i] * 0.00392156862745f;  // SL_Color.hpp:666-669
+    }
+}
+
+void model_matrix(const Draw& d, size_t instanceId, float m[16])
+{
+    const sl_std_uniforms* u = uniforms(d);
+    if (u->instance_vbo != SL_NO_RESOURCE && u->instance_vbo < d.ctx->vbos.size())
+    {
+        const Vbo& ib = d.ctx->vbos[u->instance_vbo];
+        const size_t off = instanceId * 64u;
+        if (off + 64u <= ib.bytes.size())
+        {
+            std::memcpy(m, ib.bytes.data() + off, 64);
+            return;
+        }
+        std::memset(m, 0, 64);
+        return;
+    }
+    std::memcpy(m, u->model, 64);
+}
+
+V4 run_vertex_program(const Draw& d, size_t vertId, size_t instanceId, V4* var)
+{
+    switch (d.sh->vsProg)
+    {
+        case SL_PROGRAM_PASSTHROUGH_GOURAUD:
+        {
+            float p[4], c[4];
+            read_attr(d, 0, vertId, p, 1.f);
+            read_attr(d, 1, vertId, c, 1.f);
+            var[0] = v4(c[0], c[1], c[2], c[3]);
+            return v4(p[0], p[1], p[2], p[3]);
+        }
+        case SL_PROGRAM_LAMBERT_TEXTURED:
+        case SL_PROGRAM_COLOR_TEXTURED:
+        {
+            float p[4], a[4], uv[4], m[16];
+            read_attr(d, 0, vertId, p, 1.f);
+            read_attr(d, 1, vertId, a, 0.f);
+            read_attr(d, 2, vertId, uv, 0.f);
+            model_matrix(d, instanceId, m);
+            const V4 world = mat_mul(m, v4(p[0], p[1], p[2], 1.f));
+            if (d.sh->vsProg == SL_PROGRAM_LAMBERT_TEXTURED)
+            {
+                const V4 n = mat_mul(m, v4(a[0], a[1], a[2], 0.f));
+                var[0] = v4(uv[0], uv[1], 0.f, 0.f);
+                var[1] = v4(n.v[0], n.v[1], n.v[2], 0.f);
+            }
+            else
+            {
+                var[0] = v4(a[0], a[1], a[2], 1.f);
+                var[1] = v4(uv[0], uv[1], 0.f, 0.f);
+            }
+            return mat_mul(uniforms(d)->mvp, world);
+        }
+        default:
+            return v4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+bool run_fragment_program(const Draw& d, const V4* var, V4& out)
+{
+    switch (d.sh->fsProg)
+    {
+        case SL_PROGRAM_PASSTHROUGH_GOURAUD:
+            out = var[0];
+            return true;
+
+        case SL_PROGRAM_LAMBERT_TEXTURED:
+        {
+            const sl_std_uniforms* u = uniforms(d);
+            uint8_t texel[4] = {0, 0, 0, 0};
+            if (u->texture < d.ctx->texs.size() && d.ctx->texs[u->texture].type == SL_COLOR_RGBA_8U)
+                sample_bilinear_rgba8(d.ctx->texs[u->texture], var[0].v[0], var[0].v[1], texel);
+            float albedo[4];
+            for (int c = 0; c < 4; ++c) albedo[c] = (float)texel[c] * 0.00392156862745f;
+            const float nx = var[1].v[0], ny = var[1].v[1], nz = var[1].v[2];
+            const float len2 = ((nx * nx) + (ny * ny)) + (nz * nz);
+            const float inv = 1.0f / sqrtf(len2);
+            const float mx = nx * inv, my = ny * inv, mz = nz * inv;
+            const float ndl = -(((mx * u->light_dir[0]) + (my * u->light_dir[1])) + (mz * u->light_dir[2]));
+            const float lam = (ndl > 0.f) ? ndl : 0.f;
+            for (int c = 0; c < 3; ++c)
+            {
+                const float light = u->ambient[c] + (u->diffuse[c] * lam);
+                const float v = (albedo[c] * light) * u->tint[c];
+                out.v[c] = (v < 1.f) ? v : 1.f;
+            }
+            out.v[3] = albedo[3] * u->tint[3];
+            return true;
+        }
+
+        case SL_PROGRAM_COLOR_TEXTURED:
+        {
+            const sl_std_uniforms* u = uniforms(d);
+            float t[4] = {1.f, 1.f, 1.f, 1.f};
+            if (u->texture < d.ctx->texs.size() && d.ctx->texs[u->texture].type == SL_COLOR_RGBA_FLOAT)
+                sample_bilinear_rgbaf(d.ctx->texs[u->texture], var[1].v[0], var[1].v[1], t);
+            for (int c = 0; c < 4; ++c) out.v[c] = var[0].v[c] * t[c];
+            return true;
+        }
+
+        default:
+            out = v4(0.f, 0.f, 0.f, 0.f);
+            return true;
+    }
+}
+
+/*-----------------------------------------------------------------------------
+ * Output merger (SL_Framebuffer.cpp:44-80,153-166,270-368; SL_Color.hpp:600-670)
+-----------------------------------------------------------------------------*/
+inline float clamp01(float x) { return (x < 0.f) ? 0.f : ((x > 1.f) ? 1.f : x); }
+
+void blend_rgba(int mode, const V4& s, float d[4])
+{
+    const float as = s.v[3];
+    const float mod = 1.f - as;
+    if (mode == SL_BLEND_ALPHA)                       // SL_Framebuffer.cpp:324-330
+    {
+        const float dstMod = mod * d[3];
+        const float dstAlpha = dstMod + as;
+        const float r = rcpf(dstAlpha);
+        float o[4];
+        for (int c = 0; c < 3; ++c) o[c] = fmaf(s.v[c], as, d[c] * dstMod) * r;
+        o[3] = dstAlpha;
+        for (int c = 0; c < 4; ++c) d[c] = o[c];
+    }
+    else if (mode == SL_BLEND_PREMULTIPLED_ALPHA)     // :331-334
+    {
+        for (int c = 0; c < 4; ++c) d[c] = fmaf(d[c], mod, s.v[c]);
+    }
+    else if (mode == SL_BLEND_ADDITIVE)               // :335-338
+    {
+        for (int c = 0; c < 4; ++c) d[c] = fmaf(s.v[c], as, d[c]);
+    }
+    else if (mode == SL_BLEND_SCREEN)                 // :339-342
+    {
+        for (int c = 0; c < 4; ++c) d[c] = (s.v[c] * as) + (d[c] * mod);
+    }
+    for (int c = 0; c < 4; ++c) d[c] = clamp01(d[c]);
+}
+
+void place_color(const Draw& d, int x, int y, const V4& c)
+{
+    const Target& t = d.tgt;
+    const size_t idx = (size_t)x + (size_t)t.W * (size_t)y;
+    const int blend = d.sh->blend;
+    if (t.color8)
+    {
+        uint8_t* px = t.color8 + idx * 4;
+        if (blend == SL_BLEND_OFF)
+        {
+            for (int k = 0; k < 4; ++k) px[k] = (uint8_t)(uint32_t)f2i(c.v[k] * 255.f);
+        }
+        else
+        {
+            float dst[4];
+            for (int k = 0; k < 4; ++k) dst[k] = (float)px[k] * 0.00392156862745f;
+            blend_rgba(blend, c, dst);
+            for (int k = 0; k < 4; ++k) px[k] = (uint8_t)(uint32_t)f2i(dst[k] * 255.f);
+        }
+    }
+    else if (t.colorF)
+    {
+        float* px = t.colorF + idx * 4;
+        if (blend == SL_BLEND_OFF)
+        {
+            for (int k = 0; k < 4; ++k) px[k] = c.v[k];
+        }
+        else
+        {
+            blend_rgba(blend, c, px);
+        }
+    }
+}
+
+/*-----------------------------------------------------------------------------
+ * Raster + fragment stage (SL_ScanlineBounds.hpp:137-166,310-322;
+ * SL_TriRasterizer.cpp:806-923; SL_FragmentProcessor.cpp:315-330,416-531)
+-----------------------------------------------------------------------------*/
+inline bool depth_func(int f, float a, float b)      // SL_ShaderUtil.hpp:240-507, scalar operators
+{
+    switch (f)
+    {
+        case SL_DEPTH_TEST_OFF: return true;
+        case SL_DEPTH_TEST_LESS_THAN: return a < b;
+        case SL_DEPTH_TEST_LESS_EQUAL: return a <= b;
+        case SL_DEPTH_TEST_GREATER_THAN: return a > b;
+        case SL_DEPTH_TEST_GREATER_EQUAL: return a >= b;
+        case SL_DEPTH_TEST_EQUAL: return a == b;
+        case SL_DEPTH_TEST_NOT_EQUAL: return a != b;
+        default: return false;
+    }
+}
+
+struct Scanline { float v0y, v1y, v0x, v1x, p20y, p20x, p21xy, p10xy; };
+
+// SL_ScanlineBounds::init, generic branch (:143-166)
+Scanline scanline_init(V4 p0, V4 p1, V4 p2)
+{
+    if (p0.v[1] < p1.v[1]) std::swap(p0, p1);
+    if (p0.v[1] < p2.v[1]) std::swap(p0, p2);
+    if (p1.v[1] < p2.v[1]) std::swap(p1, p2);
+    Scanline s;
+    s.v0y = p0.v[1];
+    s.v1y = p1.v[1];
+    s.v0x = p0.v[0];
+    s.v1x = p1.v[0];
+    s.p20y = rcpf(p2.v[1] - p0.v[1]);
+    s.p20x = p2.v[0] - p0.v[0];
+    s.p21xy = (p2.v[0] - p1.v[0]) * rcpf(p2.v[1] - p1.v[1]);
+    s.p10xy = (p1.v[0] - p0.v[0]) * rcpf(p1.v[1] - p0.v[1]);
+    return s;
+}
+
+// SL_ScanlineBounds::step, generic branch (:310-322).  Only the selected edge
+// is converted to int: the other one may be +-inf or NaN.
+inline void scanline_step(const Scanline& s, float yf, int32_t& xMin, int32_t& xMax)
+{
+    const float d0 = yf - s.v0y;
+    const float d1 = yf - s.v1y;
+    const float alpha = d0 * s.p20y;
+    const int32_t lo = f2i((s.p20x * alpha) + s.v0x);
+    const int32_t hi = (d1 < 0.f) ? f2i((s.p21xy * d1) + s.v1x) : f2i((s.p10xy * d0) + s.v0x);
+    xMin = (lo >= hi) ? hi : lo;
+    xMax = (lo >= hi) ? lo : hi;
+}
+
+void shade_fragment(Draw& d, const Bin& b, int x, int y, float z, const float bc[4])
+{
+    // perspective correction, generic branch SL_FragmentProcessor.cpp:486-494
+    const float h0 = b.p[0].v[3], h1 = b.p[1].v[3], h2 = b.p[2].v[3];
+    const float t0 = bc[0] * h0, t1 = bc[1] * h1, t2 = bc[2] * h2, t3 = bc[3] * 0.f;
+    const float s = rcpf(((t0 + t1) + t2) + t3);
+    const float c0 = t0 * s, c1 = t1 * s, c2 = t2 * s;
+
+    // interpolate_tri_varyings, generic branch :315-330
+    V4 var[SL_SHADER_MAX_VARYING_VECTORS];
+    for (unsigned i = 0; i < d.sh->nvary; ++i)
+        for (int k = 0; k < 4; ++k)
+            var[i].v[k] = ((b.var[0][i].v[k] * c0) + (b.var[1][i].v[k] * c1)) + (b.var[2][i].v[k] * c2);
+    for (unsigned i = d.sh->nvary; i < SL_SHADER_MAX_VARYING_VECTORS; ++i) var[i] = v4(0.f, 0.f, 0.f, 0.f);
+
+    V4 out;
+    const bool haveOutputs = run_fragment_program(d, var, out);
+    if (haveOutputs && d.sh->nout > 0) place_color(d, x, y, out);
+
+    const size_t idx = (size_t)x + (size_t)d.tgt.W * (size_t)y;
+    if (d.sh->dmask == SL_DEPTH_MASK_ON) d.tgt.depth[idx] = z;   // :526-529, regardless of the shader's return
+    if (d.tgt.dbgLast) d.tgt.dbgLast[idx] = b.seq;
+    if (d.tgt.dbgCount) d.tgt.dbgCount[idx] += 1u;
+    ++d.shaded;
+}
+
+// SL_TriRasterizer::render_triangle_simd, generic branch (:806-923), for the
+// rows y = minY + offset + k*step owned by one thread.
+void raster_bin(Draw& d, const Bin& b, int32_t rowThread, int32_t numRowThreads)
+{
+    const V4* P = b.p;
+    const int32_t bboxMinY = f2i(std::min(std::min(P[0].v[1], P[1].v[1]), P[2].v[1]));
+    const int32_t bboxMaxY = f2i(std::max(std::max(P[0].v[1], P[1].v[1]), P[2].v[1]));
+    const int32_t n1 = numRowThreads - 1;
+    const int32_t offset = n1 - ((bboxMinY + (n1 - rowThread)) % numRowThreads);  // sl_scanline_offset, SL_ScanlineBounds.hpp:31-50
+    int32_t y = bboxMinY + offset;
+    if (y >= bboxMaxY) return;
+
+    const float dep[3] = {P[0].v[2], P[1].v[2], P[2].v[2]};
+    const Scanline sc = scanline_init(P[0], P[1], P[2]);
+    const int dfunc = d.sh->dtest;
+    const int W = d.tgt.W;
+
+    do
+    {
+        const float yf = (float)y;
+        int32_t xMin, xMax;
+        scanline_step(sc, yf, xMin, xMax);
+        if ((uint32_t)xMin < (uint32_t)xMax)
+        {
+            float bcY[4], bc[4][4], bcX[4];
+            for (int k = 0; k < 4; ++k)
+            {
+                bcY[k] = fmaf(b.by.v[k], yf, b.bz.v[k]);
+                bcX[k] = b.bx.v[k] * 4.f;
+            }
+            for (int i = 0; i < 4; ++i)
+                for (int k = 0; k < 4; ++k)
+                    bc[i][k] = ((float)(xMin + i) * b.bx.v[k]) + bcY[k];
+
+            int32_t x0 = xMin;
+            do
+            {
+                for (int i = 0; i < 4; ++i)
+                {
+                    const int32_t x = x0 + i;
+                    if (x < xMax)
+                    {
+                        // z = depth * bc (SL_TriRasterizer.cpp:868), 4th term = 0 * bc[3]
+                        const float z = (((dep[0] * bc[i][0]) + (dep[1] * bc[i][1])) + (dep[2] * bc[i][2])) + (0.f * bc[i][3]);
+                        const float dv = d.tgt.depth[(size_t)x + (size_t)W * (size_t)y];
+                        ++d.tested;
+                        if (depth_func(dfunc, z, dv)) shade_fragment(d, b, x, y, z, bc[i]);
+                    }
+                }
+                for (int i = 0; i < 4; ++i)
+                    for (int k = 0; k < 4; ++k)
+                        bc[i][k] = bc[i][k] + bcX[k];
+                x0 += 4;
+            }
+            while (x0 < xMax);
+        }
+        y += numRowThreads;
+    }
+    while (y < bboxMaxY);
+}
+
+/*-----------------------------------------------------------------------------
+ * Vertex stage (SL_TriProcessor.cpp:31-118,248-342,370-519,527-640)
+-----------------------------------------------------------------------------*/
+// face_determinant, generic branch (:101-116)
+float face_determinant(const V4& p0, const V4& p1, const V4& p2)
+{
+    const float w0 = rcpf(p0.v[3]);
+    const float w1 = rcpf(p1.v[3]);
+    const float w2 = rcpf(p2.v[3]);
+    V4 v0, v1, v2;
+    for (int k = 0; k < 4; ++k) { v0.v[k] = p0.v[k] * w0; v1.v[k] = p1.v[k] * w1; v2.v[k] = p2.v[k] * w2; }
+    const V4 xABC = v4(v0.v[0], v1.v[0], v2.v[0], 0.f);
+    const V4 yBCA = v4(v1.v[1], v2.v[1], v0.v[1], 0.f);
+    const V4 yCAB = v4(v2.v[1], v0.v[1], v1.v[1], 0.f);
+    V4 diff;
+    for (int k = 0; k < 4; ++k) diff.v[k] = yBCA.v[k] - yCAB.v[k];
+    return dot4(xABC, diff);
+}
+
+// sl_ndc_clip_status, generic branch (SL_PostVertexTransform.hpp:653-673)
+int clip_status(const V4& c0, const V4& c1, const V4& c2)
+{
+    auto inside = [](const V4& c) {
+        const float w = c.v[3];
+        bool le = true, ge = true;
+        for (int k = 0; k < 4; ++k) { le = le && (c.v[k] <= w); ge = ge && (c.v[k] >= -w); }
+        return le && ge;
+    };
+    const int vis = (inside(c0) && inside(c1) && inside(c2)) ? 3 : 0;
+    const int part = (c0.v[3] > 0.f || c1.v[3] > 0.f || c2.v[3] > 0.f) ? 1 : 0;
+    return vis | part;
+}
+
+// sl_perspective_divide (:239-249) + sl_ndc_to_screen_coords (:508-524)
+void divide_and_snap(V4& p, const float vp[4])
+{
+    const float wInv = rcpf(p.v[3]);
+    const float x = p.v[0] * wInv, y = p.v[1] * wInv, z = p.v[2] * wInv;
+    const float hw = vp[2] * 0.5f;
+    const float hh = vp[3] * 0.5f;
+    p.v[0] = maxf_ref(floorf(fmaf(x + 1.f, hw, vp[0])), 0.f);
+    p.v[1] = maxf_ref(floorf(fmaf(y + 1.f, hh, vp[1])), 0.f);
+    p.v[2] = z;
+    p.v[3] = wInv;
+}
+
+struct VertexStage
+{
+    Draw& d;
+    std::vector<Bin>* out;     // when non-null, bins are collected instead of rasterised
+    explicit VertexStage(Draw& dr) : d(dr), out(nullptr) {}
+
+    void emit(Bin& b)
+    {
+        ++d.binned;
+        if (out) out->push_back(b);
+        else raster_bin(d, b, 0, 1);
+    }
+
+    // SL_TriProcessor::push_bin (:248-342)
+    void push_bin(uint32_t seq, const TVert& a, const TVert& b, const TVert& c)
+    {
+        const V4& p0 = a.pos;
+        const V4& p1 = b.pos;
+        const V4& p2 = c.pos;
+        for (int k = 0; k < 2; ++k)
+            if (!std::isfinite(p0.v[k]) || !std::isfinite(p1.v[k]) || !std::isfinite(p2.v[k])) return;
+
+        const float minX = std::min(std::min(p0.v[0], p1.v[0]), p2.v[0]);
+        const float maxX = std::max(std::max(p0.v[0], p1.v[0]), p2.v[0]);
+        const float minY = std::min(std::min(p0.v[1], p1.v[1]), p2.v[1]);
+        const float maxY = std::max(std::max(p0.v[1], p1.v[1]), p2.v[1]);
+        if ((maxX - minX < 1.f) || (maxY - minY < 1.f)) return;
+
+        // 2D cross of (p2-p1) and (p0-p1) (:271-281)
+        const float zyx = p2.v[0] - p1.v[0], zyy = p2.v[1] - p1.v[1];
+        const float xyx = p0.v[0] - p1.v[0], xyy = p0.v[1] - p1.v[1];
+        const float denom = rcpf((zyx * xyy) - (zyy * xyx));
+
+        // ddx = cross(Y, 1), ddy = cross(1, X), ddz = cross(X, Y) (:285-287)
+        const float X[3] = {p0.v[0], p1.v[0], p2.v[0]};
+        const float Y[3] = {p0.v[1], p1.v[1], p2.v[1]};
+        const float ddx[4] = {(Y[1] * 1.f) - (Y[2] * 1.f), (Y[2] * 1.f) - (Y[0] * 1.f), (Y[0] * 1.f) - (Y[1] * 1.f), 0.f};
+        const float ddy[4] = {(1.f * X[2]) - (1.f * X[1]), (1.f * X[0]) - (1.f * X[2]), (1.f * X[1]) - (1.f * X[0]), 0.f};
+        const float ddz[4] = {(X[1] * Y[2]) - (X[2] * Y[1]), (X[2] * Y[0]) - (X[0] * Y[2]), (X[0] * Y[1]) - (X[1] * Y[0]), 0.f};
+
+        Bin bin;
+        bin.p[0] = p0; bin.p[1] = p1; bin.p[2] = p2;
+        for (int k = 0; k < 4; ++k)
+        {
+            bin.bx.v[k] = ddx[k] * denom;
+            bin.by.v[k] = ddy[k] * denom;
+            bin.bz.v[k] = ddz[k] * denom;
+        }
+        const unsigned nv = d.sh->nvary;
+        for (unsigned i = 0; i < nv; ++i)
+        {
+            bin.var[0][i] = a.var[i];
+            bin.var[1][i] = b.var[i];
+            bin.var[2][i] = c.var[i];
+        }
+        bin.seq = seq;
+        emit(bin);
+    }
+
+    // SL_TriProcessor::clip_and_process_tris (:370-519)
+    void clip_and_process(uint32_t seq, const TVert& a, const TVert& b, const TVert& c)
+    {
+        const unsigned nv = d.sh->nvary;
+        const V4 edges[6] = {
+            v4(1.f, 0.f, 0.f, 1.f), v4(-1.f, 0.f, 0.f, 1.f),
+            v4(0.f, 1.f, 0.f, 1.f), v4(0.f, -1.f, 0.f, 1.f),
+            v4(0.f, 0.f, 1.f, 1.f), v4(0.f, 0.f, -1.f, 1.f)};
+        unsigned n = 3;
+        V4 nverts[9], tverts[9];
+        V4 nvar[9][SL_SHADER_MAX_VARYING_VECTORS], tvar[9][SL_SHADER_MAX_VARYING_VECTORS];
+        nverts[0] = a.pos; nverts[1] = b.pos; nverts[2] = c.pos;
+        for (unsigned i = 0; i < nv; ++i) { nvar[0][i] = a.var[i]; nvar[1][i] = b.var[i]; nvar[2][i] = c.var[i]; }
+
+        for (const V4& edge : edges)
+        {
+            unsigned nn = 0;
+            unsigned j = n - 1;
+            V4 q0 = nverts[n - 1];
+            float t0 = dot4(q0, edge);
+            bool vis0 = !std::signbit(t0);
+            for (unsigned k = 0; k < n; ++k)
+            {
+                const V4 q1 = nverts[k];
+                const float t1 = dot4(q1, edge);
+                const bool vis1 = !std::signbit(t1);
+                if (vis0 != vis1)
+                {
+                    const float t = t0 * rcpf(t0 - t1);
+                    tverts[nn] = mix4(q0, q1, t);
+                    for (unsigned i = 0; i < nv; ++i) tvar[nn][i] = mix4(nvar[j][i], nvar[k][i], t);
+                    ++nn;
+                }
+                if (vis1)
+                {
+                    tverts[nn] = q1;
+                    for (unsigned i = 0; i < nv; ++i) tvar[nn][i] = nvar[k][i];
+                    ++nn;
+                }
+                j = k; q0 = q1; t0 = t1; vis0 = vis1;
+            }
+            if (!nn) return;
+            n = nn;
+            for (unsigned k = 0; k < n; ++k)
+            {
+                nverts[k] = tverts[k];
+                for (unsigned i = 0; i < nv; ++i) nvar[k][i] = tvar[k][i];
+            }
+        }
+        if (n < 3) return;
+
+        for (unsigned k = 0; k < n; ++k) divide_and_snap(nverts[k], d.vpDims);
+
+        TVert f0, f1, f2;
+        f0.pos = nverts[0];
+        for (unsigned i = 0; i < nv; ++i) f0.var[i] = nvar[0][i];
+        for (unsigned i = n - 2; i--;)                // reverse fan sharing one primIndex (:505-518)
+        {
+            f1.pos = nverts[i + 1];
+            f2.pos = nverts[i + 2];
+            for (unsigned v = 0; v < nv; ++v) { f1.var[v] = nvar[i + 1][v]; f2.var[v] = nvar[i + 2][v]; }
+            push_bin(seq, f0, f1, f2);
+        }
+    }
+
+    void process_triangle(uint32_t seq, size_t vid0, size_t vid1, size_t vid2, size_t instanceId)
+    {
+        TVert tv[3];
+        const size_t ids[3] = {vid0, vid1, vid2};
+        for (int k = 0; k < 3; ++k)
+        {
+            for (auto& v : tv[k].var) v = v4(0.f, 0.f, 0.f, 0.f);
+            tv[k].pos = run_vertex_program(d, ids[k], instanceId, tv[k].var);
+        }
+        const int cull = d.sh->cull;
+        if (cull != SL_CULL_OFF)                       // (:597-612)
+        {
+            const float det = face_determinant(tv[0].pos, tv[1].pos, tv[2].pos);
+            const bool culled = (cull == SL_CULL_FRONT_FACE) ^ (bool)std::signbit(det);
+            if (culled) return;
+        }
+        for (int k = 0; k < 3; ++k) tv[k].pos = mat_mul(d.scissorMat, tv[k].pos);   // (:615-617)
+
+        const int status = clip_status(tv[0].pos, tv[1].pos, tv[2].pos);
+        if (status == 3)
+        {
+            for (int k = 0; k < 3; ++k) divide_and_snap(tv[k].pos, d.vpDims);
+            push_bin(seq, tv[0], tv[1], tv[2]);
+        }
+        else if (status == 1)
+        {
+            clip_and_process(seq, tv[0], tv[1], tv[2]);
+        }
+    }
+};
+
+size_t index_at(const Ibo& ibo, size_t i)
+{
+    if (i >= ibo.count) return 0;
+    const uint8_t* p = ibo.bytes.data() + i * ibo.bpe;
+    switch (ibo.type)
+    {
+        case VERTEX_DATA_BYTE: return p[0];
+        case VERTEX_DATA_SHORT: { uint16_t v; std::memcpy(&v, p, 2); return v; }
+        default: { uint32_t v; std::memcpy(&v, p, 4); return v; }
+    }
+}
+
+// SL_ViewportState::scissor_matrix / viewport_rect (SL_ViewportState.cpp:17-111)
+void viewport_setup(const sl_context* c, float fboW, float fboH, float mat[16], float vp[4])
+{
+    const float sc[4] = {(float)c->scissor[0], (float)c->scissor[1], (float)c->scissor[2], (float)c->scissor[3]};
+    float v[4] = {(float)c->viewport[0], (float)c->viewport[1], (float)c->viewport[2], (float)c->viewport[3]};
+    v[0] = std::min(std::max(v[0] + sc[0], 0.f), fboW);
+    v[1] = std::min(std::max(v[1] + sc[1], 0.f), fboH);
+    v[2] = std::min(std::min(v[2], (sc[0] + sc[2]) - v[0]), fboW - v[0]);
+    v[3] = std::min(std::min(v[3], (sc[1] + sc[3]) - v[1]), fboH - v[1]);
+    std::memcpy(vp, v, sizeof(v));
+
+    const float inv[4] = {rcpf(fboW), rcpf(fboH), rcpf(fboW), rcpf(fboH)};
+    float x = sc[0] * inv[0], y = sc[1] * inv[1], w = sc[2] * inv[2], h = sc[3] * inv[3];
+    if (x < 0.f) { w += x; x = 0.f; }
+    if (y < 0.f) { h += y; y = 0.f; }
+    w = std::min(1.f - x, w);
+    h = std::min(1.f - y, h);
+    const float nm00 = rcpf(w), nm11 = rcpf(h);
+    const float nm30 = nm00 - 1.f, nm31 = nm11 - 1.f;
+    const float m30 = fmaf(x, (-2.f * nm00), nm30);
+    const float m31 = fmaf(y, (-2.f * nm11), nm31);
+    const float m[16] = {nm00, 0.f, 0.f, 0.f, 0.f, nm11, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, m30, m31, 0.f, 1.f};
+    std::memcpy(mat, m, sizeof(m));
+}
+
+int setup_draw(sl_context* c, size_t shaderId, size_t fboId, Draw& d)
+{
+    if (shaderId >= c->shaders.size() || fboId >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    const Shader& sh = c->shaders[shaderId];
+    Fbo& f = c->fbos[fboId];
+    if (f.depth >= c->texs.size() || !c->texs[f.depth].valid) return SL_ERR_STATE;
+    Tex& dt = c->texs[f.depth];
+    if (dt.type != SL_COLOR_R_FLOAT) return SL_ERR_UNSUPPORTED;
+    d.ctx = c;
+    d.sh = &sh;
+    d.uni = (sh.ubo < c->ubos.size()) ? c->ubos[sh.ubo].bytes : nullptr;
+    if (d.uni == nullptr && (sh.vsProg != SL_PROGRAM_PASSTHROUGH_GOURAUD || sh.fsProg != SL_PROGRAM_PASSTHROUGH_GOURAUD)) return SL_ERR_STATE;
+    d.tgt.W = dt.w;
+    d.tgt.H = dt.h;
+    d.tgt.depth = reinterpret_cast<float*>(dt.data.data());
+    d.tgt.color8 = nullptr;
+    d.tgt.colorF = nullptr;
+    if (sh.nout > 1) return SL_ERR_UNSUPPORTED;
+    if (sh.nout == 1)
+    {
+        if (f.colors.empty() || f.colors[0] >= c->texs.size() || !c->texs[f.colors[0]].valid) return SL_ERR_STATE;
+        Tex& ct = c->texs[f.colors[0]];
+        if (ct.w != dt.w || ct.h != dt.h) return SL_ERR_STATE;
+        if (ct.type == SL_COLOR_RGBA_8U) d.tgt.color8 = ct.data.data();
+        else if (ct.type == SL_COLOR_RGBA_FLOAT) d.tgt.colorF = reinterpret_cast<float*>(ct.data.data());
+        else return SL_ERR_UNSUPPORTED;
+    }
+    d.tgt.dbgLast = (f.dbgLast < c->texs.size()) ? reinterpret_cast<uint32_t*>(c->texs[f.dbgLast].data.data()) : nullptr;
+    d.tgt.dbgCount = (f.dbgCount < c->texs.size()) ? reinterpret_cast<uint32_t*>(c->texs[f.dbgCount].data.data()) : nullptr;
+    viewport_setup(c, (float)dt.w, (float)dt.h, d.scissorMat, d.vpDims);
+    return SL_OK;
+}
+
+int bind_mesh(sl_context* c, const sl_mesh& m, Draw& d)
+{
+    if (m.vaoId >= c->vaos.size()) return SL_ERR_INVALID_HANDLE;
+    d.vao = &c->vaos[m.vaoId];
+    d.vbo = (d.vao->vbo < c->vbos.size()) ? &c->vbos[d.vao->vbo] : nullptr;
+    d.ibo = (d.vao->ibo < c->ibos.size()) ? &c->ibos[d.vao->ibo] : nullptr;
+    if (m.mode != RENDER_MODE_TRIANGLES && m.mode != RENDER_MODE_INDEXED_TRIANGLES) return SL_ERR_UNSUPPORTED;
+    if (m.mode == RENDER_MODE_INDEXED_TRIANGLES && d.ibo == nullptr) return SL_ERR_STATE;
+    return SL_OK;
+}
+
+/*-----------------------------------------------------------------------------
+ * Multithreaded timing mode: the SL_ProcessorPool scheme (strided vertex work,
+ * row-interleaved rasterisation, a barrier per batch of bins,
+ * SL_ProcessorPool.cpp:280-327, SL_VertexProcessor.cpp:84-181).  Bins are
+ * merged in submission order before rasterisation, so the output equals the
+ * single-threaded one.
+-----------------------------------------------------------------------------*/
+struct TriRef { size_t v0, v1, v2, inst; uint32_t seq; };
+
+void run_draw_threads(sl_context* c, Draw& proto, const std::vector<TriRef>& tris)
+{
+    const unsigned N = std::max(1u, c->numThreads);
+    const size_t window = 1024 * (size_t)N;
+    std::vector<std::vector<Bin>> perThread(N);
+    std::vector<Bin> merged;
+    std::vector<uint64_t> tested(N, 0), shaded(N, 0), binned(N, 0);
+    for (size_t w0 = 0; w0 < tris.size(); w0 += window)
+    {
+        const size_t w1 = std::min(tris.size(), w0 + window);
+        auto vwork = [&](unsigned t) {
+            Draw d = proto;
+            VertexStage vs(d);
+            perThread[t].clear();
+            vs.out = &perThread[t];
+            for (size_t i = w0 + t; i < w1; i += N) vs.process_triangle(tris[i].seq, tris[i].v0, tris[i].v1, tris[i].v2, tris[i].inst);
+            binned[t] += d.binned;
+        };
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < N; ++t) th.emplace_back(vwork, t);
+        vwork(0);
+        for (auto& x : th) x.join();
+        th.clear();
+
+        // merge: bins of one triangle are contiguous in its thread's list and
+        // carry its sequence number; walk triangles in order.
+        merged.clear();
+        std::vector<size_t> pos(N, 0);
+        for (size_t i = w0; i < w1; ++i)
+        {
+            const unsigned t = (unsigned)((i - w0) % N);
+            std::vector<Bin>& L = perThread[t];
+            while (pos[t] < L.size() && L[pos[t]].seq == tris[i].seq) merged.push_back(L[pos[t]++]);
+        }
+        auto rwork = [&](unsigned t) {
+            Draw d = proto;
+            for (const Bin& b : merged) raster_bin(d, b, (int32_t)t, (int32_t)N);
+            tested[t] += d.tested;
+            shaded[t] += d.shaded;
+        };
+        for (unsigned t = 1; t < N; ++t) th.emplace_back(rwork, t);
+        rwork(0);
+        for (auto& x : th) x.join();
+    }
+    for (unsigned t = 0; t < N; ++t) { proto.tested += tested[t]; proto.shaded += shaded[t]; proto.binned += binned[t]; }
+}
+
+int run_draws(sl_context* c, const sl_mesh* meshes, size_t numMeshes, size_t numInstances, size_t shaderId, size_t fboId)
+{
+    Draw proto;
+    int rc = setup_draw(c, shaderId, fboId, proto);
+    if (rc != SL_OK) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    const bool threaded = c->numThreads > 1;
+    std::vector<TriRef> tris;
+
+    const size_t nIter = (numInstances == 1) ? numMeshes : numInstances;
+    for (size_t it = 0; it < nIter; ++it)
+    {
+        const sl_mesh& m = (numInstances == 1) ? meshes[it] : meshes[0];
+        const size_t instanceId = (numInstances == 1) ? 0 : it;
+        rc = bind_mesh(c, m, proto);
+        if (rc != SL_OK) return rc;
+        const bool indexed = (m.mode == RENDER_MODE_INDEXED_TRIANGLES);
+        if (threaded) tris.clear();
+        Draw d = proto;
+        VertexStage vs(d);
+        for (size_t i = m.elementBegin; i + 3 <= m.elementEnd; i += 3)   // (:569-574)
+        {
+            const uint32_t seq = ++c->triSeq;
+            ++c->stats.triangles_submitted;
+            size_t a = i, b = i + 1, e = i + 2;
+            if (indexed) { a = index_at(*d.ibo, i); b = index_at(*d.ibo, i + 1); e = index_at(*d.ibo, i + 2); }
+            if (threaded) tris.push_back(TriRef{a, b, e, instanceId, seq});
+            else vs.process_triangle(seq, a, b, e, instanceId);
+        }
+        if (threaded)
+        {
+            run_draw_threads(c, d, tris);
+        }
+        c->stats.fragments_tested += d.tested;
+        c->stats.fragments_shaded += d.shaded;
+        c->stats.triangles_binned += d.binned;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    c->stats.ms_total += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    c->stats.frames += 1;
+    return SL_OK;
+}
+
+void clear_texture_bytes(Tex& t, const void* value, uint32_t bpt)
+{
+    const size_t n = (size_t)t.w * t.h * std::max<uint16_t>(t.d, 1);
+    uint8_t* p = t.data.data();
+    for (size_t i = 0; i < n; ++i) std::memcpy(p + i * bpt, value, bpt);
+}
+
+int clear_color(sl_context* c, size_t fbo, unsigned att, const double rgba[4])
+{
+    if (fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    Fbo& f = c->fbos[fbo];
+    if (att >= f.colors.size() || f.colors[att] >= c->texs.size()) return SL_ERR_INVALID_HANDLE;
+    Tex& t = c->texs[f.colors[att]];
+    if (t.type == SL_COLOR_RGBA_8U)
+    {
+        uint8_t v[4];
+        for (int k = 0; k < 4; ++k) v[k] = (uint8_t)(int64_t)(rgba[k] * 255.0);   // color_cast<uint8_t,double>, SL_Color.cpp:137
+        clear_texture_bytes(t, v, 4);
+    }
+    else if (t.type == SL_COLOR_RGBA_FLOAT)
+    {
+        float v[4];
+        for (int k = 0; k < 4; ++k) v[k] = (float)rgba[k];
+        clear_texture_bytes(t, v, 16);
+    }
+    else return SL_ERR_UNSUPPORTED;
+    return SL_OK;
+}
+
+int clear_depth(sl_context* c, size_t fbo, double depth)
+{
+    if (fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    Fbo& f = c->fbos[fbo];
+    if (f.depth >= c->texs.size()) return SL_ERR_STATE;
+    Tex& t = c->texs[f.depth];
+    if (t.type != SL_COLOR_R_FLOAT) return SL_ERR_UNSUPPORTED;
+    const float v = (float)depth;
+    clear_texture_bytes(t, &v, 4);
+    if (f.dbgLast < c->texs.size()) { const uint32_t z = 0; clear_texture_bytes(c->texs[f.dbgLast], &z, 4); }
+    if (f.dbgCount < c->texs.size()) { const uint32_t z = 0; clear_texture_bytes(c->texs[f.dbgCount], &z, 4); }
+    return SL_OK;
+}
+
+} // namespace
+
+/*-----------------------------------------------------------------------------
+ * C API (prefix slo_), one function per entry point of include/sl_gpu.h
+-----------------------------------------------------------------------------*/
+extern "C" {
+
+int slo_context_create(sl_context** out)
+{
+    if (!out) return SL_ERR_INVALID_ARG;
+    *out = new sl_context();
+    return SL_OK;
+}
+
+void slo_context_destroy(sl_context* c) { delete c; }
+
+const char* slo_last_error(const sl_context* c) { return c ? c->err.c_str() : "null context"; }
+
+unsigned slo_num_threads(const sl_context* c) { return c ? c->numThreads : 0u; }
+
+unsigned slo_set_num_threads(sl_context* c, unsigned n)
+{
+    if (!c) return 0;
+    c->numThreads = std::max(1u, n);
+    return c->numThreads;
+}
+
+int slo_set_viewport(sl_context* c, int32_t x, int32_t y, int32_t w, int32_t h)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    // SL_ViewportState::viewport (SL_ViewportState.hpp:148-156)
+    c->viewport[0] = std::min(std::max(x, -65536), 65535);
+    c->viewport[1] = std::min(std::max(y, -65536), 65535);
+    c->viewport[2] = std::min(w, 65535 - x);
+    c->viewport[3] = std::min(h, 65535 - y);
+    return SL_OK;
+}
+
+int slo_set_scissor(sl_context* c, int32_t x, int32_t y, int32_t w, int32_t h)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    c->scissor[0] = std::min(std::max(x, -65536), 65535);
+    c->scissor[1] = std::min(std::max(y, -65536), 65535);
+    c->scissor[2] = std::min(w, 65535 - x);
+    c->scissor[3] = std::min(h, 65535 - y);
+    return SL_OK;
+}
+
+size_t slo_create_vbo(sl_context* c) { c->vbos.emplace_back(); return c->vbos.size() - 1; }
+
+int slo_vbo_init(sl_context* c, size_t vbo, size_t numBytes, const void* data)
+{
+    if (vbo >= c->vbos.size()) return SL_ERR_INVALID_HANDLE;
+    if (!numBytes) return -1;
+    Vbo& v = c->vbos[vbo];
+    v.bytes.assign(numBytes, 0);
+    if (data) std::memcpy(v.bytes.data(), data, numBytes);
+    v.valid = true;
+    return SL_OK;
+}
+
+int slo_vbo_assign(sl_context* c, size_t vbo, const void* data, size_t offset, size_t numBytes)
+{
+    if (vbo >= c->vbos.size()) return SL_ERR_INVALID_HANDLE;
+    Vbo& v = c->vbos[vbo];
+    if (offset + numBytes > v.bytes.size()) return SL_ERR_INVALID_ARG;
+    std::memcpy(v.bytes.data() + offset, data, numBytes);
+    return SL_OK;
+}
+
+size_t slo_create_ibo(sl_context* c) { c->ibos.emplace_back(); return c->ibos.size() - 1; }
+
+int slo_ibo_init(sl_context* c, size_t ibo, uint32_t numElements, int type, const void* data)
+{
+    if (ibo >= c->ibos.size()) return SL_ERR_INVALID_HANDLE;
+    if (!numElements) return -1;
+    const uint32_t bpe = bytes_per_index(type);
+    if (!bpe) return SL_ERR_INVALID_ARG;
+    Ibo& b = c->ibos[ibo];
+    b.type = type;
+    b.bpe = bpe;
+    b.count = numElements;
+    b.bytes.assign((size_t)numElements * bpe + 16, 0);   // padded like SL_IndexBuffer.cpp:155
+    if (data) std::memcpy(b.bytes.data(), data, (size_t)numElements * bpe);
+    b.valid = true;
+    return SL_OK;
+}
+
+size_t slo_create_vao(sl_context* c) { c->vaos.emplace_back(); return c->vaos.size() - 1; }
+
+int slo_vao_set_vertex_buffer(sl_context* c, size_t vao, size_t vbo)
+{
+    if (vao >= c->vaos.size()) return SL_ERR_INVALID_HANDLE;
+    c->vaos[vao].vbo = vbo;
+    return SL_OK;
+}
+
+int slo_vao_set_index_buffer(sl_context* c, size_t vao, size_t ibo)
+{
+    if (vao >= c->vaos.size()) return SL_ERR_INVALID_HANDLE;
+    c->vaos[vao].ibo = ibo;
+    return SL_OK;
+}
+
+int slo_vao_set_num_bindings(sl_context* c, size_t vao, size_t n)
+{
+    if (vao >= c->vaos.size()) return SL_ERR_INVALID_HANDLE;
+    if (n >= SL_VAO_MAX_BINDINGS) return -1;                   // SL_VertexArray.cpp:164-167
+    Vao& v = c->vaos[vao];
+    for (size_t i = v.nb; i < n; ++i) v.b[i] = Binding{};
+    v.nb = n;
+    return (int)n;
+}
+
+int slo_vao_set_binding(sl_context* c, size_t vao, size_t bindId, ptrdiff_t offset, ptrdiff_t stride, int dims, int type)
+{
+    if (vao >= c->vaos.size()) return SL_ERR_INVALID_HANDLE;
+    Vao& v = c->vaos[vao];
+    if (bindId >= v.nb) return SL_ERR_INVALID_ARG;
+    v.b[bindId].offset = offset;
+    v.b[bindId].stride = stride;
+    v.b[bindId].dims = dims;
+    v.b[bindId].type = type;
+    return SL_OK;
+}
+
+size_t slo_create_texture(sl_context* c) { c->texs.emplace_back(); return c->texs.size() - 1; }
+
+int slo_texture_init(sl_context* c, size_t tex, int type, uint16_t w, uint16_t h, uint16_t d)
+{
+    if (tex >= c->texs.size()) return SL_ERR_INVALID_HANDLE;
+    const uint32_t bpt = bytes_per_color(type);
+    if (!bpt) return SL_ERR_UNSUPPORTED;
+    if (!w || !h || !d) return SL_ERR_INVALID_ARG;
+    Tex& t = c->texs[tex];
+    t.type = type; t.w = w; t.h = h; t.d = d; t.bpt = bpt;
+    // SL_Texture.cpp:28-55: each dimension padded by (4 - dim % 4), zero filled
+    const size_t pw = (size_t)w + (4 - w % 4), ph = (size_t)h + (4 - h % 4), pd = (size_t)d + (4 - d % 4);
+    t.data.assign(pw * ph * pd * bpt + 4 * bpt, 0);
+    t.valid = true;
+    return SL_OK;
+}
+
+int slo_texture_set_texels(sl_context* c, size_t tex, const void* data, size_t numBytes)
+{
+    if (tex >= c->texs.size() || !c->texs[tex].valid) return SL_ERR_INVALID_HANDLE;
+    Tex& t = c->texs[tex];
+    const size_t n = (size_t)t.w * t.h * t.d * t.bpt;
+    if (numBytes != n) return SL_ERR_INVALID_ARG;
+    std::memcpy(t.data.data(), data, n);
+    return SL_OK;
+}
+
+int slo_texture_read(sl_context* c, size_t tex, void* out, size_t numBytes)
+{
+    if (tex >= c->texs.size() || !c->texs[tex].valid) return SL_ERR_INVALID_HANDLE;
+    Tex& t = c->texs[tex];
+    const size_t n = (size_t)t.w * t.h * t.d * t.bpt;
+    if (numBytes != n) return SL_ERR_INVALID_ARG;
+    std::memcpy(out, t.data.data(), n);
+    return SL_OK;
+}
+
+size_t slo_create_framebuffer(sl_context* c) { c->fbos.emplace_back(); return c->fbos.size() - 1; }
+
+int slo_fbo_reserve_color_buffers(sl_context* c, size_t fbo, unsigned n)
+{
+    if (fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    if (n > SL_SHADER_MAX_FRAG_OUTPUTS) return SL_ERR_INVALID_ARG;
+    c->fbos[fbo].colors.assign(n, SL_INVALID_ID);
+    return SL_OK;
+}
+
+int slo_fbo_attach_color_buffer(sl_context* c, size_t fbo, unsigned idx, size_t tex)
+{
+    if (fbo >= c->fbos.size() || tex >= c->texs.size()) return SL_ERR_INVALID_HANDLE;
+    Fbo& f = c->fbos[fbo];
+    if (idx >= f.colors.size()) return SL_ERR_INVALID_ARG;
+    f.colors[idx] = tex;
+    return SL_OK;
+}
+
+int slo_fbo_attach_depth_buffer(sl_context* c, size_t fbo, size_t tex)
+{
+    if (fbo >= c->fbos.size() || tex >= c->texs.size()) return SL_ERR_INVALID_HANDLE;
+    c->fbos[fbo].depth = tex;
+    return SL_OK;
+}
+
+int slo_fbo_attach_debug_buffers(sl_context* c, size_t fbo, size_t lastTex, size_t countTex)
+{
+    if (fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    Fbo& f = c->fbos[fbo];
+    for (size_t t : {lastTex, countTex})
+        if (t != SL_INVALID_ID && (t >= c->texs.size() || c->texs[t].type != SL_COLOR_R_32U)) return SL_ERR_INVALID_ARG;
+    f.dbgLast = lastTex;
+    f.dbgCount = countTex;
+    return SL_OK;
+}
+
+int slo_fbo_valid(sl_context* c, size_t fbo)
+{
+    if (fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    const Fbo& f = c->fbos[fbo];
+    if (f.colors.empty()) return -1;                            // SL_Framebuffer.cpp:930-990
+    for (size_t t : f.colors) if (t >= c->texs.size() || !c->texs[t].valid) return -2;
+    const Tex& c0 = c->texs[f.colors[0]];
+    for (size_t t : f.colors)
+    {
+        if (c->texs[t].w != c0.w) return -3;
+        if (c->texs[t].h != c0.h) return -4;
+        if (c->texs[t].d != c0.d) return -5;
+    }
+    if (f.depth >= c->texs.size() || !c->texs[f.depth].valid) return -6;
+    const Tex& dt = c->texs[f.depth];
+    if (dt.w != c0.w) return -7;
+    if (dt.h != c0.h) return -8;
+    if (dt.d > 1) return -9;
+    if (dt.type != SL_COLOR_R_HALF && dt.type != SL_COLOR_R_FLOAT && dt.type != SL_COLOR_R_DOUBLE) return -10;
+    return 0;
+}
+
+size_t slo_create_ubo(sl_context* c)
+{
+    c->ubos.emplace_back();
+    std::memset(c->ubos.back().bytes, 0, SL_MAX_UNIFORM_BUFFER_SIZE);
+    return c->ubos.size() - 1;
+}
+
+int slo_ubo_assign(sl_context* c, size_t ubo, const void* data, size_t offset, size_t n)
+{
+    if (ubo >= c->ubos.size()) return SL_ERR_INVALID_HANDLE;
+    if (offset + n > SL_MAX_UNIFORM_BUFFER_SIZE) return SL_ERR_INVALID_ARG;
+    std::memcpy(c->ubos[ubo].bytes + offset, data, n);
+    return SL_OK;
+}
+
+size_t slo_create_shader(sl_context* c, const sl_vertex_shader* vs, const sl_fragment_shader* fs, size_t ubo)
+{
+    // SL_Context::create_shader (SL_Context.cpp:547-604)
+    if (!vs || !fs) return SL_INVALID_ID;
+    if (ubo != SL_INVALID_ID && ubo >= c->ubos.size()) return SL_INVALID_ID;
+    if (vs->numVaryings < fs->numVaryings) return SL_INVALID_ID;
+    if (vs->numVaryings > SL_SHADER_MAX_VARYING_VECTORS) return SL_INVALID_ID;
+    if (fs->numVaryings > SL_SHADER_MAX_VARYING_VECTORS) return SL_INVALID_ID;
+    if (fs->numOutputs > SL_SHADER_MAX_FRAG_OUTPUTS) return SL_INVALID_ID;
+    Shader s;
+    s.cull = vs->cullMode;
+    s.dtest = fs->depthTest;
+    s.dmask = fs->depthMask;
+    s.blend = fs->blend;
+    s.nvary = fs->numVaryings;          // the pipeline's varying count comes from the fragment shader
+    s.nout = fs->numOutputs;
+    s.vsProg = vs->programId;
+    s.fsProg = fs->programId;
+    s.ubo = ubo;
+    c->shaders.push_back(s);
+    return c->shaders.size() - 1;
+}
+
+int slo_draw(sl_context* c, const sl_mesh* m, size_t shader, size_t fbo)
+{
+    if (!c || !m) return SL_ERR_INVALID_ARG;
+    return run_draws(c, m, 1, 1, shader, fbo);
+}
+
+int slo_draw_multiple(sl_context* c, const sl_mesh* m, size_t n, size_t shader, size_t fbo)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    if (m == nullptr || n == 0) return SL_OK;                     // SL_Context.cpp:726-730
+    return run_draws(c, m, n, 1, shader, fbo);
+}
+
+int slo_draw_instanced(sl_context* c, const sl_mesh* m, size_t numInstances, size_t shader, size_t fbo)
+{
+    if (!c || !m) return SL_ERR_INVALID_ARG;
+    if (numInstances == 0) return SL_OK;
+    return run_draws(c, m, 1, numInstances, shader, fbo);
+}
+
+int slo_clear_color_buffer(sl_context* c, size_t fbo, unsigned att, const double rgba[4]) { return clear_color(c, fbo, att, rgba); }
+
+int slo_clear_depth_buffer(sl_context* c, size_t fbo, double depth) { return clear_depth(c, fbo, depth); }
+
+int slo_clear_framebuffer(sl_context* c, size_t fbo, unsigned att, const double rgba[4], double depth)
+{
+    int rc = clear_color(c, fbo, att, rgba);
+    if (rc != SL_OK) return rc;
+    return clear_depth(c, fbo, depth);
+}
+
+int slo_finish(sl_context*) { return SL_OK; }
+
+int slo_get_stats(const sl_context* c, sl_frame_stats* out)
+{
+    if (!c || !out) return SL_ERR_INVALID_ARG;
+    *out = c->stats;
+    return SL_OK;
+}
+
+int slo_reset_stats(sl_context* c)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    c->stats = sl_frame_stats{};
+    c->triSeq = 0;
+    return SL_OK;
+}
+
+} // extern "C"

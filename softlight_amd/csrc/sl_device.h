@@ -1,0 +1,166 @@
+// sl_device.h -- data layouts and pinned arithmetic shared by the gfx950
+// kernels of the SoftLight draw path and their host launch code.
+//
+// All kernels are built with -ffp-contract=off: every multiply and add below
+// rounds on its own unless fmaf() is written out, which is what makes the
+// coverage, depth and order bit-identical to the reference's scalar path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sl_gpu.h"
+
+#define SL_HD __host__ __device__ __forceinline__
+
+namespace slg {
+
+// ---------------------------------------------------------------------------
+// Fixed geometry of the pipeline
+// ---------------------------------------------------------------------------
+constexpr int kSetupBlock = 256;       // triangles per setup workgroup (one per lane)
+constexpr int kMaxFan = 7;             // a 9-gon from 6-plane clipping fans into 7 triangles
+
+// Tile raster: one wave per tile.  Lane = (row, phase): 16 rows x 4 x-phases.
+constexpr int kTileW = 32;
+constexpr int kTileH = 16;
+constexpr int kRasterThreads = 64;
+
+// ---------------------------------------------------------------------------
+// Pinned scalar arithmetic (mirrors the reference's generic branches)
+// ---------------------------------------------------------------------------
+// float -> int32 with truncation; NaN and out-of-range give INT32_MIN, the
+// value x86 cvttss2si produces for the reference's (int32_t) casts.
+SL_HD int32_t f2i(float f)
+{
+    return (f >= -2147483648.0f && f < 2147483648.0f) ? (int32_t)f : INT32_MIN;
+}
+
+SL_HD float rcpf(float x) { return 1.0f / x; }
+
+// ls::math::max as std::max: (a < b) ? b : a (NaN in a propagates).
+SL_HD float max_ref(float a, float b) { return (a < b) ? b : a; }
+
+// ---------------------------------------------------------------------------
+// Draw table entry: everything a kernel needs about one draw call, with
+// handles already resolved to device pointers on the host.
+// ---------------------------------------------------------------------------
+struct BindingDesc
+{
+    int64_t offset;
+    int64_t stride;
+    int32_t dims;      // SL_Dimension (components - 1)
+    int32_t type;      // SL_DataType
+};
+
+struct DrawDesc
+{
+    uint64_t triBase;          // first global triangle of this draw
+    uint64_t numTris;          // trisPerInstance * numInstances
+    uint64_t trisPerInstance;
+    uint64_t elementBegin;
+    const uint8_t* vbo;
+    uint64_t vboBytes;
+    const uint8_t* ibo;
+    uint32_t iboType;          // SL_DataType of indices
+    uint32_t iboCount;
+    const uint8_t* instVbo;    // per-instance mat4 array (or null)
+    uint64_t instVboBytes;
+    const uint8_t* tex;        // sampled texture texels (or null)
+    uint32_t texType;
+    uint32_t texW, texH;
+    uint32_t uniOffset;        // byte offset of the uniform snapshot in the arena
+    uint32_t seqBase;          // debug sequence number of the draw's first triangle
+    uint32_t numBindings;
+    uint32_t vsProg, fsProg;
+    uint8_t  indexed, cull, dtest, dmask, blend, nvary, nout, hasUniforms;
+    BindingDesc bind[SL_VAO_MAX_BINDINGS];
+    float scissor[16];         // SL_ViewportState::scissor_matrix at draw time
+    float vp[4];               // SL_ViewportState::viewport_rect at draw time
+};
+
+// Screen-space triangle after setup: SL_FragmentBin (SL_ShaderUtil.hpp:627-647)
+// without the varyings, plus the per-triangle scanline constants of
+// SL_ScanlineBounds::init (SL_ScanlineBounds.hpp:137-166).
+struct SetupRec
+{
+    float x[3], y[3];          // snapped screen coordinates
+    float z[3];                // NDC depth
+    float hw[3];               // 1/w
+    float bx[3], by[3], bz[3]; // barycentric planes (ddx, ddy, ddz) * denom
+    float bc3;                 // 4th barycentric component times 0 (+-0, or NaN when denom is not finite)
+    float v0y, v1y, v0x, v1x;  // scanline constants
+    float p20y, p20x, p21xy, p10xy;
+    int32_t minY, maxY;
+    uint32_t draw;
+    uint32_t seq;
+    uint32_t pad[2];
+};
+static_assert(sizeof(SetupRec) == 144, "SetupRec layout");
+
+struct ClipJob
+{
+    uint32_t tri;              // global triangle index
+    uint32_t recOff;           // first output record
+    uint32_t pairOff;          // first output (tile, record) pair
+    uint32_t nrec;             // records reserved by k_setup
+};
+
+// Device-side frame control block.
+struct FrameCtrl
+{
+    uint32_t ticket;           // setup chunk ticket (ordered scheduling)
+    uint32_t clipJobs;         // number of clip jobs appended
+    uint32_t totalRecs;
+    uint32_t totalPairs;
+    uint32_t overflow;         // capacity exceeded (records, pairs or clip jobs)
+    uint32_t error;            // look-back spin limit hit
+    uint32_t pad[2];
+    unsigned long long tested; // fragments depth-tested
+    unsigned long long shaded; // fragments that passed and were shaded
+};
+
+struct SetupParams
+{
+    const DrawDesc* draws;
+    uint32_t numDraws;
+    uint64_t totalTris;
+    const uint8_t* uniforms;
+    int32_t W, H;
+    int32_t tilesX, tilesY;
+    SetupRec* recs;
+    float4* vars;              // 3 * maxVary float4 per record
+    uint32_t varStride;        // float4 per record
+    uint32_t recCap;
+    uint32_t* pairKeys;
+    uint32_t* pairVals;
+    uint32_t pairCap;
+    ClipJob* clipJobs;
+    uint32_t clipCap;
+    unsigned long long* status;   // look-back status per chunk
+    FrameCtrl* ctrl;
+};
+
+struct RasterParams
+{
+    const DrawDesc* draws;
+    const uint8_t* uniforms;
+    const SetupRec* recs;
+    const float4* vars;
+    uint32_t varStride;
+    const uint32_t* tileRecs;     // record ids sorted by tile
+    const uint2* tileRange;       // [start, end) into tileRecs per tile
+    int32_t W, H, tilesX, tilesY;
+    uint8_t* color;               // attachment 0 texels (RGBA8 or RGBA32F) or null
+    int32_t colorType;
+    float* depth;
+    uint32_t* dbgLast;
+    uint32_t* dbgCount;
+    int32_t clearColor, clearDepth;   // fused clear of this batch
+    uint32_t clearRGBA8;
+    float clearRGBAF[4];
+    float clearDepthValue;
+    FrameCtrl* ctrl;
+};
+
+}  // namespace slg

@@ -1,0 +1,686 @@
+// sl_kernels.hip -- gfx950 kernels of the SoftLight triangle pipeline.
+//
+//   k_setup    vertex programs, culling, clip classification, perspective
+//              divide, viewport snap, bounding-box reject and plane setup for
+//              one triangle per lane; ordered compaction of the surviving
+//              records and their (tile, record) bin entries through a
+//              single-pass decoupled look-back over 256-triangle chunks.
+//              Triangles that need clipping get their record/pair slots
+//              reserved here (position-only clipping to count them) and are
+//              finished by k_clip.
+//   k_clip     6-plane Sutherland-Hodgman clipping with varyings and the
+//              reversed fan, writing into the reserved slots.
+//   k_ranges   [start, end) of each tile's entries in the tile-sorted list.
+//   k_raster   one wave per screen tile: walks the tile's records in
+//              submission order; lane = (row, x-phase) replays the
+//              reference's scanline span and its sequential barycentric
+//              recurrence, depth-tests, shades and blends with the tile's
+//              colour/depth held in LDS.
+//
+// Reference routines reproduced: SL_TriProcessor::process_verts / push_bin /
+// clip_and_process_tris (SL_TriProcessor.cpp:248-640), SL_TriRasterizer::
+// render_triangle_simd generic branch (SL_TriRasterizer.cpp:806-923),
+// SL_FragmentProcessor::flush_tri_fragments (SL_FragmentProcessor.cpp:416-531).
+#include <hipcub/hipcub.hpp>
+
+#include "sl_stages.h"
+#include "sl_kernels.h"
+
+namespace slg {
+
+constexpr uint64_t kFlagAgg = 1ull << 62;
+constexpr uint64_t kFlagInc = 2ull << 62;
+constexpr uint64_t kValMask = (1ull << 62) - 1;
+constexpr uint32_t kRecBits = 30;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+// ---------------------------------------------------------------------------
+// small utilities
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, unsigned long long v, uint64_t n)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__device__ __forceinline__ uint32_t find_draw(const DrawDesc* __restrict__ draws, uint32_t n, uint64_t g)
+{
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi)
+    {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (draws[mid].triBase <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint64_t fetch_index(const DrawDesc& d, uint64_t e)
+{
+    if (e >= d.iboCount) return 0;
+    switch (d.iboType)
+    {
+        case VERTEX_DATA_BYTE: return d.ibo[e];
+        case VERTEX_DATA_SHORT: return reinterpret_cast<const uint16_t*>(d.ibo)[e];
+        default: return reinterpret_cast<const uint32_t*>(d.ibo)[e];
+    }
+}
+
+struct TriIn
+{
+    float4 pos[3];
+    Vary4 var[3];
+    uint32_t draw;
+    uint32_t seq;
+    int status;               // 0 culled/invisible, 1 partial, 3 full
+};
+
+// Fetch, vertex programs, cull, scissor matrix and clip status of one triangle
+// (SL_TriProcessor::process_verts, SL_TriProcessor.cpp:574-631).
+__device__ __forceinline__ void load_triangle(const SetupParams& P, uint64_t g, TriIn& T)
+{
+    const uint32_t di = find_draw(P.draws, P.numDraws, g);
+    const DrawDesc& D = P.draws[di];
+    const uint64_t t = g - D.triBase;
+    const uint64_t inst = t / D.trisPerInstance;
+    const uint64_t ti = t - inst * D.trisPerInstance;
+    const uint64_t e = D.elementBegin + 3u * ti;
+    uint64_t vid[3] = {e, e + 1, e + 2};
+    if (D.indexed)
+    {
+        vid[0] = fetch_index(D, e);
+        vid[1] = fetch_index(D, e + 1);
+        vid[2] = fetch_index(D, e + 2);
+    }
+    T.draw = di;
+    T.seq = D.seqBase + 1u + (uint32_t)t;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) T.pos[k] = vertex_program(D, P.uniforms, vid[k], inst, T.var[k]);
+
+    if (D.cull != SL_CULL_OFF)
+    {
+        const float det = face_determinant(T.pos[0], T.pos[1], T.pos[2]);
+        const bool culled = (D.cull == SL_CULL_FRONT_FACE) ^ (bool)signbit(det);
+        if (culled) { T.status = 0; return; }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) T.pos[k] = mat_mul(D.scissor, T.pos[k]);
+    T.status = clip_status(T.pos[0], T.pos[1], T.pos[2]);
+}
+
+__device__ __forceinline__ float4 clip_edge(int e)
+{
+    switch (e)
+    {
+        case 0: return f4(1.f, 0.f, 0.f, 1.f);
+        case 1: return f4(-1.f, 0.f, 0.f, 1.f);
+        case 2: return f4(0.f, 1.f, 0.f, 1.f);
+        case 3: return f4(0.f, -1.f, 0.f, 1.f);
+        case 4: return f4(0.f, 0.f, 1.f, 1.f);
+        default: return f4(0.f, 0.f, -1.f, 1.f);
+    }
+}
+
+// Exact shortcut of clip_and_process_tris for the common case: planes are
+// visited in the reference's order; a plane that keeps all three vertices
+// leaves the triangle unchanged, the first plane that rejects all three ends
+// it with no output.  Returns 0 = rejected, 1 = needs the full clipper.
+__device__ __forceinline__ int clip_quick(const float4* p)
+{
+    for (int e = 0; e < 6; ++e)
+    {
+        const float4 E = clip_edge(e);
+        const bool v0 = !signbit(dot4(p[0], E));
+        const bool v1 = !signbit(dot4(p[1], E));
+        const bool v2 = !signbit(dot4(p[2], E));
+        if (v0 && v1 && v2) continue;
+        if (!v0 && !v1 && !v2) return 0;
+        return 1;
+    }
+    return 1;
+}
+
+// SL_TriProcessor::clip_and_process_tris (SL_TriProcessor.cpp:370-499) on
+// positions and (optionally) varyings.  Returns the polygon size (0 if it was
+// clipped away or has fewer than 3 vertices); vertices are left in clip space.
+template <bool kVary>
+__device__ int clip_polygon(const float4* inPos, const Vary4* inVar, int nv, float4* outPos, Vary4* outVar)
+{
+    float4 tpos[9];
+    Vary4 tvar[kVary ? 9 : 1];
+    int n = 3;
+    for (int k = 0; k < 3; ++k)
+    {
+        outPos[k] = inPos[k];
+        if (kVary) outVar[k] = inVar[k];
+    }
+    for (int e = 0; e < 6; ++e)
+    {
+        const float4 E = clip_edge(e);
+        int nn = 0;
+        int j = n - 1;
+        float4 q0 = outPos[n - 1];
+        float t0 = dot4(q0, E);
+        bool vis0 = !signbit(t0);
+        for (int k = 0; k < n; ++k)
+        {
+            const float4 q1 = outPos[k];
+            const float t1 = dot4(q1, E);
+            const bool vis1 = !signbit(t1);
+            if (vis0 != vis1)
+            {
+                const float t = t0 * rcpf(t0 - t1);
+                tpos[nn] = mix4(q0, q1, t);
+                if (kVary)
+                    for (int i = 0; i < nv; ++i) tvar[nn].v[i] = mix4(outVar[j].v[i], outVar[k].v[i], t);
+                ++nn;
+            }
+            if (vis1)
+            {
+                tpos[nn] = q1;
+                if (kVary) tvar[nn] = outVar[k];
+                ++nn;
+            }
+            j = k; q0 = q1; t0 = t1; vis0 = vis1;
+        }
+        if (nn == 0) return 0;
+        n = nn;
+        for (int k = 0; k < n; ++k)
+        {
+            outPos[k] = tpos[k];
+            if (kVary) outVar[k] = tvar[k];
+        }
+    }
+    return n < 3 ? 0 : n;
+}
+
+__device__ __forceinline__ void write_record(const SetupParams& P, uint32_t recId, uint32_t pairOff,
+                                             float4 p0, float4 p1, float4 p2, const Vary4& a, const Vary4& b,
+                                             const Vary4& c, int nv, uint32_t draw, uint32_t seq, const TileRect& tr)
+{
+    SetupRec r;
+    setup_record(p0, p1, p2, draw, seq, r);
+    P.recs[recId] = r;
+    float4* V = P.vars + (uint64_t)recId * P.varStride;
+    const uint32_t nvMax = P.varStride / 3;
+    for (int i = 0; i < nv; ++i)
+    {
+        V[0 * nvMax + i] = a.v[i];
+        V[1 * nvMax + i] = b.v[i];
+        V[2 * nvMax + i] = c.v[i];
+    }
+    uint32_t o = pairOff;
+    for (int ty = tr.ty0; ty <= tr.ty1; ++ty)
+        for (int tx = tr.tx0; tx <= tr.tx1; ++tx)
+        {
+            P.pairKeys[o] = (uint32_t)(ty * P.tilesX + tx);
+            P.pairVals[o] = recId;
+            ++o;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// k_setup
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long wave_incl_scan(unsigned long long v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        const unsigned long long o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(kSetupBlock) void k_setup(SetupParams P)
+{
+    __shared__ uint32_t sChunk;
+    __shared__ unsigned long long sWave[kSetupBlock / 64];
+    __shared__ unsigned long long sPrefix;
+
+    if (threadIdx.x == 0) sChunk = atomicAdd(&P.ctrl->ticket, 1u);
+    __syncthreads();
+    const uint32_t chunk = sChunk;
+    const uint64_t g = (uint64_t)chunk * kSetupBlock + threadIdx.x;
+
+    TriIn T;
+    T.status = 0;
+    T.draw = 0;
+    T.seq = 0;
+    uint32_t nrec = 0, npair = 0;
+    bool clipped = false;
+    float4 sp[3];
+    TileRect tr{0, -1, 0, -1};
+    if (g < P.totalTris)
+    {
+        load_triangle(P, g, T);
+        if (T.status == 3)
+        {
+            const DrawDesc& D = P.draws[T.draw];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) sp[k] = divide_and_snap(T.pos[k], D.vp);
+            if (bin_accept(sp[0], sp[1], sp[2]))
+            {
+                tr = tile_rect(sp[0], sp[1], sp[2], P.W, P.H, P.tilesX, P.tilesY);
+                nrec = 1;
+                npair = tile_count(tr);
+            }
+        }
+        else if (T.status == 1 && clip_quick(T.pos))
+        {
+            const DrawDesc& D = P.draws[T.draw];
+            float4 poly[9];
+            const int n = clip_polygon<false>(T.pos, nullptr, 0, poly, nullptr);
+            for (int k = 0; k < n; ++k) poly[k] = divide_and_snap(poly[k], D.vp);
+            for (int i = n - 3; i >= 0; --i)
+            {
+                if (bin_accept(poly[0], poly[i + 1], poly[i + 2]))
+                {
+                    const TileRect r = tile_rect(poly[0], poly[i + 1], poly[i + 2], P.W, P.H, P.tilesX, P.tilesY);
+                    ++nrec;
+                    npair += tile_count(r);
+                }
+            }
+            clipped = nrec > 0;
+        }
+    }
+
+    // block-wide exclusive scan of the packed (pairs << 30 | records) counts
+    const unsigned long long mine = ((unsigned long long)npair << kRecBits) | nrec;
+    const unsigned long long incl = wave_incl_scan(mine);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 63) sWave[wave] = incl;
+    __syncthreads();
+    unsigned long long waveBase = 0, blockTotal = 0;
+#pragma unroll
+    for (int w = 0; w < kSetupBlock / 64; ++w)
+    {
+        if (w < wave) waveBase += sWave[w];
+        blockTotal += sWave[w];
+    }
+
+    // decoupled look-back (ticket order guarantees every predecessor runs)
+    if (threadIdx.x == 0)
+    {
+        unsigned long long prefix = 0;
+        if (chunk == 0)
+        {
+            __hip_atomic_store(&P.status[0], kFlagInc | blockTotal, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        else
+        {
+            __hip_atomic_store(&P.status[chunk], kFlagAgg | blockTotal, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j = (int64_t)chunk - 1;
+            uint32_t spins = 0;
+            while (j >= 0)
+            {
+                const unsigned long long s = __hip_atomic_load(&P.status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long flag = s & ~kValMask;
+                if (flag == 0)
+                {
+                    if (++spins > kSpinLimit)
+                    {
+                        atomicExch(&P.ctrl->error, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                prefix += s & kValMask;
+                if (flag == kFlagInc) break;
+                --j;
+            }
+            __hip_atomic_store(&P.status[chunk], kFlagInc | (prefix + blockTotal), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        sPrefix = prefix;
+        if (chunk == gridDim.x - 1)
+        {
+            const unsigned long long tot = prefix + blockTotal;
+            P.ctrl->totalRecs = (uint32_t)(tot & ((1ull << kRecBits) - 1));
+            P.ctrl->totalPairs = (uint32_t)(tot >> kRecBits);
+        }
+    }
+    __syncthreads();
+
+    const unsigned long long base = sPrefix + waveBase + (incl - mine);
+    const uint32_t recOff = (uint32_t)(base & ((1ull << kRecBits) - 1));
+    const uint64_t pairOff = base >> kRecBits;
+    if (nrec == 0) return;
+    if ((uint64_t)recOff + nrec > P.recCap || pairOff + npair > P.pairCap)
+    {
+        atomicExch(&P.ctrl->overflow, 1u);
+        return;
+    }
+    if (!clipped)
+    {
+        const DrawDesc& D = P.draws[T.draw];
+        write_record(P, recOff, (uint32_t)pairOff, sp[0], sp[1], sp[2], T.var[0], T.var[1], T.var[2], D.nvary,
+                     T.draw, T.seq, tr);
+    }
+    else
+    {
+        const uint32_t job = atomicAdd(&P.ctrl->clipJobs, 1u);
+        if (job < P.clipCap)
+        {
+            ClipJob J;
+            J.tri = (uint32_t)g;
+            J.recOff = recOff;
+            J.pairOff = (uint32_t)pairOff;
+            J.nrec = nrec;
+            P.clipJobs[job] = J;
+        }
+        else
+        {
+            atomicExch(&P.ctrl->overflow, 1u);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_clip: finish the clipped triangles into their reserved slots
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_clip(SetupParams P)
+{
+    const uint32_t njobs = min(P.ctrl->clipJobs, P.clipCap);
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < njobs; j += gridDim.x * blockDim.x)
+    {
+        const ClipJob J = P.clipJobs[j];
+        TriIn T;
+        load_triangle(P, J.tri, T);
+        const DrawDesc& D = P.draws[T.draw];
+        const int nv = D.nvary;
+        float4 poly[9];
+        Vary4 pvar[9];
+        const int n = clip_polygon<true>(T.pos, T.var, nv, poly, pvar);
+        for (int k = 0; k < n; ++k) poly[k] = divide_and_snap(poly[k], D.vp);
+        uint32_t rec = J.recOff, pair = J.pairOff;
+        const uint32_t recEnd = J.recOff + J.nrec;
+        for (int i = n - 3; i >= 0 && rec < recEnd; --i)
+        {
+            if (!bin_accept(poly[0], poly[i + 1], poly[i + 2])) continue;
+            const TileRect r = tile_rect(poly[0], poly[i + 1], poly[i + 2], P.W, P.H, P.tilesX, P.tilesY);
+            write_record(P, rec, pair, poly[0], poly[i + 1], poly[i + 2], pvar[0], pvar[i + 1], pvar[i + 2], nv,
+                         T.draw, T.seq, r);
+            ++rec;
+            pair += tile_count(r);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_ranges: per-tile [start, end) in the tile-sorted entry list
+// ---------------------------------------------------------------------------
+__global__ void k_ranges(const uint32_t* __restrict__ keys, uint32_t n, uint2* __restrict__ range)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const uint32_t k = keys[i];
+        if (i == 0 || keys[i - 1] != k) range[k].x = i;
+        if (i == n - 1 || keys[i + 1] != k) range[k].y = i + 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_raster: one wave per tile
+// ---------------------------------------------------------------------------
+template <int CF, bool DBG>
+__global__ __launch_bounds__(kRasterThreads) void k_raster(RasterParams P)
+{
+    __shared__ float sDepth[kTileH][kTileW];
+    __shared__ uint32_t sCol8[CF == 1 ? kTileH : 1][kTileW];
+    __shared__ float4 sColF[CF == 2 ? kTileH : 1][kTileW];
+    __shared__ uint32_t sLast[DBG ? kTileH : 1][kTileW];
+    __shared__ uint32_t sCount[DBG ? kTileH : 1][kTileW];
+
+    const int tile = blockIdx.x;
+    const int tx = tile % P.tilesX, ty = tile / P.tilesX;
+    const int x0 = tx * kTileW, y0 = ty * kTileH;
+    const int lane = threadIdx.x;
+
+    // tile init: fused clear or load
+    for (int k = lane; k < kTileW * kTileH; k += kRasterThreads)
+    {
+        const int r = k / kTileW, c = k % kTileW;
+        const int gx = x0 + c, gy = y0 + r;
+        const bool inb = gx < P.W && gy < P.H;
+        const uint64_t gi = (uint64_t)gy * P.W + gx;
+        sDepth[r][c] = P.clearDepth ? P.clearDepthValue : (inb ? P.depth[gi] : 0.f);
+        if (CF == 1) sCol8[r][c] = P.clearColor ? P.clearRGBA8 : (inb ? reinterpret_cast<const uint32_t*>(P.color)[gi] : 0u);
+        if (CF == 2)
+            sColF[r][c] = P.clearColor ? make_float4(P.clearRGBAF[0], P.clearRGBAF[1], P.clearRGBAF[2], P.clearRGBAF[3])
+                                       : (inb ? reinterpret_cast<const float4*>(P.color)[gi] : make_float4(0.f, 0.f, 0.f, 0.f));
+        if (DBG)
+        {
+            sLast[r][c] = P.clearDepth ? 0u : (inb ? P.dbgLast[gi] : 0u);
+            sCount[r][c] = P.clearDepth ? 0u : (inb ? P.dbgCount[gi] : 0u);
+        }
+    }
+    __syncthreads();
+
+    const int row = lane >> 2, phase = lane & 3;
+    const int y = y0 + row;
+    const float yf = (float)y;
+    const uint2 range = P.tileRange[tile];
+    const uint32_t nvMax = P.varStride / 3;
+    uint32_t tested = 0, shaded = 0;
+
+    for (uint32_t it = range.x; it < range.y; ++it)
+    {
+        const uint32_t rid = __builtin_amdgcn_readfirstlane(P.tileRecs[it]);
+        const SetupRec& R = P.recs[rid];
+        if (y >= R.minY && y < R.maxY && y < P.H)
+        {
+            int32_t xMin, xMax;
+            scanline_step(R, yf, xMin, xMax);
+            if ((uint32_t)xMin < (uint32_t)xMax)
+            {
+                const int xs = max(xMin, x0), xe = min(min(xMax, x0 + kTileW), P.W);
+                const int xf = xMin + phase;
+                const int g0 = (xs > xf) ? ((xs - xf + 3) >> 2) : 0;
+                int x = xf + 4 * g0;
+                if (x < xe)
+                {
+                    const DrawDesc& D = P.draws[R.draw];
+                    // bc(x) = fl(fl(x*dx) + fma(dy, y, c)), then += 4*dx per group of 4 pixels
+                    const float fx = (float)xf;
+                    float s0 = (fx * R.bx[0]) + fmaf(R.by[0], yf, R.bz[0]);
+                    float s1 = (fx * R.bx[1]) + fmaf(R.by[1], yf, R.bz[1]);
+                    float s2 = (fx * R.bx[2]) + fmaf(R.by[2], yf, R.bz[2]);
+                    const float d0 = R.bx[0] * 4.f, d1 = R.bx[1] * 4.f, d2 = R.bx[2] * 4.f;
+                    for (int gi = 0; gi < g0; ++gi)
+                    {
+                        s0 = s0 + d0;
+                        s1 = s1 + d1;
+                        s2 = s2 + d2;
+                    }
+                    for (; x < xe; x += 4)
+                    {
+                        const float z = (((R.z[0] * s0) + (R.z[1] * s1)) + (R.z[2] * s2)) + R.bc3;
+                        const int c = x - x0;
+                        ++tested;
+                        if (depth_test(D.dtest, z, sDepth[row][c]))
+                        {
+                            if (D.nout > 0)
+                            {
+                                // perspective correction (SL_FragmentProcessor.cpp:486-494)
+                                const float t0 = s0 * R.hw[0], t1 = s1 * R.hw[1], t2 = s2 * R.hw[2];
+                                const float sinv = rcpf(((t0 + t1) + t2) + R.bc3);
+                                const float c0 = t0 * sinv, c1 = t1 * sinv, c2 = t2 * sinv;
+                                const float4* V = P.vars + (uint64_t)rid * P.varStride;
+                                Vary4 var;
+                                for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+                                {
+                                    if (i < D.nvary)
+                                    {
+                                        const float4 a = V[i], b = V[nvMax + i], e = V[2 * nvMax + i];
+                                        var.v[i] = f4(((a.x * c0) + (b.x * c1)) + (e.x * c2), ((a.y * c0) + (b.y * c1)) + (e.y * c2),
+                                                      ((a.z * c0) + (b.z * c1)) + (e.z * c2), ((a.w * c0) + (b.w * c1)) + (e.w * c2));
+                                    }
+                                    else
+                                    {
+                                        var.v[i] = f4(0.f, 0.f, 0.f, 0.f);
+                                    }
+                                }
+                                float4 out;
+                                const bool have = fragment_program(D, P.uniforms, var, out);
+                                if (have)
+                                {
+                                    if (CF == 1)
+                                    {
+                                        if (D.blend == SL_BLEND_OFF)
+                                        {
+                                            const float o[4] = {out.x, out.y, out.z, out.w};
+                                            sCol8[row][c] = pack_rgba8(o);
+                                        }
+                                        else
+                                        {
+                                            float dcol[4];
+                                            unpack_rgba8(sCol8[row][c], dcol);
+                                            blend(D.blend, out, dcol);
+                                            sCol8[row][c] = pack_rgba8(dcol);
+                                        }
+                                    }
+                                    if (CF == 2)
+                                    {
+                                        if (D.blend == SL_BLEND_OFF)
+                                        {
+                                            sColF[row][c] = out;
+                                        }
+                                        else
+                                        {
+                                            const float4 cur = sColF[row][c];
+                                            float dcol[4] = {cur.x, cur.y, cur.z, cur.w};
+                                            blend(D.blend, out, dcol);
+                                            sColF[row][c] = make_float4(dcol[0], dcol[1], dcol[2], dcol[3]);
+                                        }
+                                    }
+                                }
+                            }
+                            if (D.dmask == SL_DEPTH_MASK_ON) sDepth[row][c] = z;
+                            if (DBG)
+                            {
+                                sLast[row][c] = R.seq;
+                                sCount[row][c] += 1u;
+                            }
+                            ++shaded;
+                        }
+                        s0 = s0 + d0;
+                        s1 = s1 + d1;
+                        s2 = s2 + d2;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // tile store
+    for (int k = lane; k < kTileW * kTileH; k += kRasterThreads)
+    {
+        const int r = k / kTileW, c = k % kTileW;
+        const int gx = x0 + c, gy = y0 + r;
+        if (gx < P.W && gy < P.H)
+        {
+            const uint64_t gi = (uint64_t)gy * P.W + gx;
+            P.depth[gi] = sDepth[r][c];
+            if (CF == 1) reinterpret_cast<uint32_t*>(P.color)[gi] = sCol8[r][c];
+            if (CF == 2) reinterpret_cast<float4*>(P.color)[gi] = sColF[r][c];
+            if (DBG)
+            {
+                P.dbgLast[gi] = sLast[r][c];
+                P.dbgCount[gi] = sCount[r][c];
+            }
+        }
+    }
+
+    // counters
+    unsigned long long t = tested, s = shaded;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        t += __shfl_xor(t, off, 64);
+        s += __shfl_xor(s, off, 64);
+    }
+    if (lane == 0)
+    {
+        atomicAdd(&P.ctrl->tested, t);
+        atomicAdd(&P.ctrl->shaded, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static inline int grid_for(uint64_t n, int block, int cap = 4096)
+{
+    uint64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > (uint64_t)cap) g = cap;
+    return (int)g;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, uint64_t n, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_setup(const SetupParams& p, uint64_t chunks, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_setup, dim3((unsigned)chunks), dim3(kSetupBlock), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_clip(const SetupParams& p, uint32_t jobs, hipStream_t s)
+{
+    if (jobs == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clip, dim3(grid_for(jobs, 64, 8192)), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_ranges(const uint32_t* keys, uint32_t n, uint2* range, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 256)), dim3(256), 0, s, keys, n, range);
+    return hipGetLastError();
+}
+
+hipError_t sort_pairs(void* temp, size_t& tempBytes, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
+                      uint32_t* vout, uint32_t n, int endBit, hipStream_t s)
+{
+    return hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, kin, kout, vin, vout, (int)n, 0, endBit, s);
+}
+
+hipError_t launch_raster(const RasterParams& p, int numTiles, bool debug, hipStream_t s)
+{
+    const int cf = (p.color == nullptr) ? 0 : (p.colorType == SL_COLOR_RGBA_FLOAT ? 2 : 1);
+    const dim3 g(numTiles), b(kRasterThreads);
+    if (debug)
+    {
+        if (cf == 0) hipLaunchKernelGGL((k_raster<0, true>), g, b, 0, s, p);
+        else if (cf == 1) hipLaunchKernelGGL((k_raster<1, true>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_raster<2, true>), g, b, 0, s, p);
+    }
+    else
+    {
+        if (cf == 0) hipLaunchKernelGGL((k_raster<0, false>), g, b, 0, s, p);
+        else if (cf == 1) hipLaunchKernelGGL((k_raster<1, false>), g, b, 0, s, p);
+        else hipLaunchKernelGGL((k_raster<2, false>), g, b, 0, s, p);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace slg

@@ -1,0 +1,928 @@
+// sl_runtime.hip -- host runtime of the gfx950 draw path: the C ABI of
+// include/sl_gpu.h, device-resident resources, and the batched frame
+// (setup -> clip -> tile sort -> raster) on the context's HIP stream.
+//
+// The reference's SL_Context owns resources in vectors indexed by size_t
+// handles (SL_Context.hpp:66-82) and runs each draw synchronously through its
+// thread pool (SL_ProcessorPool.cpp:280-327).  Here draws and clears are
+// recorded and executed as one batch per framebuffer at the next
+// synchronisation point; uniform blocks are snapshotted at draw time, so the
+// result equals drawing each call in order.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sl_device.h"
+#include "sl_kernels.h"
+
+using namespace slg;
+
+namespace {
+
+struct DevBuf
+{
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Vbo { bool valid = false; DevBuf buf; size_t bytes = 0; };
+struct Ibo { bool valid = false; DevBuf buf; int type = VERTEX_DATA_INT; uint32_t count = 0; uint32_t bpe = 4; };
+struct Binding { int dims = VERTEX_DIMENSION_4; int type = VERTEX_DATA_FLOAT; ptrdiff_t offset = 0; ptrdiff_t stride = 0; };
+struct Vao { size_t vbo = SL_INVALID_ID; size_t ibo = SL_INVALID_ID; size_t nb = 0; Binding b[SL_VAO_MAX_BINDINGS]; };
+struct Tex { bool valid = false; DevBuf buf; int type = SL_COLOR_RGBA_8U; uint16_t w = 0, h = 0, d = 0; uint32_t bpt = 0; };
+struct Fbo { std::vector<size_t> colors; size_t depth = SL_INVALID_ID; size_t dbgLast = SL_INVALID_ID; size_t dbgCount = SL_INVALID_ID; };
+struct Ubo { uint8_t bytes[SL_MAX_UNIFORM_BUFFER_SIZE]; };
+struct Shader { uint8_t cull, dtest, dmask, blend, nvary, nout; uint32_t vsProg, fsProg; size_t ubo; };
+
+struct Batch
+{
+    size_t fbo = SL_INVALID_ID;
+    bool clearColor = false, clearDepth = false;
+    double rgba[4] = {0, 0, 0, 0};
+    double depth = 1.0;
+    std::vector<DrawDesc> draws;
+    std::vector<uint8_t> uni;
+    uint64_t totalTris = 0;
+    uint32_t maxVary = 1;
+
+    bool empty() const { return !clearColor && !clearDepth && draws.empty(); }
+    void reset() { *this = Batch{}; }
+};
+
+uint32_t bytes_per_color(int type)
+{
+    switch (type)
+    {
+        case SL_COLOR_R_8U: return 1;
+        case SL_COLOR_R_32U: return 4;
+        case SL_COLOR_R_FLOAT: return 4;
+        case SL_COLOR_RGBA_8U: return 4;
+        case SL_COLOR_RGBA_FLOAT: return 16;
+        default: return 0;
+    }
+}
+
+uint32_t bytes_per_index(int type)
+{
+    switch (type)
+    {
+        case VERTEX_DATA_BYTE: return 1;
+        case VERTEX_DATA_SHORT: return 2;
+        case VERTEX_DATA_INT: return 4;
+        default: return 0;
+    }
+}
+
+}  // namespace
+
+struct sl_context
+{
+    hipStream_t stream = nullptr;
+    std::vector<Vbo> vbos;
+    std::vector<Ibo> ibos;
+    std::vector<Vao> vaos;
+    std::vector<Tex> texs;
+    std::vector<Fbo> fbos;
+    std::vector<Ubo> ubos;
+    std::vector<Shader> shaders;
+    int32_t viewport[4] = {0, 0, 65535, 65535};   // SL_ViewportState.hpp:80-81
+    int32_t scissor[4] = {0, 0, 65535, 65535};
+    unsigned numThreads = 1;
+    uint32_t triSeq = 0;
+    sl_frame_stats stats{};
+    std::string err;
+
+    Batch batch;
+
+    // frame scratch (grow-only)
+    DevBuf dDraws, dUni, dRecs, dVars, dKeys[2], dVals[2], dClip, dStatus, dRanges, dCtrl, dSortTemp;
+    uint32_t recCap = 0, pairCap = 0, clipCap = 0, varStride = 0;
+    FrameCtrl* hCtrl = nullptr;                    // pinned readback of the control block
+    hipEvent_t ev[8] = {};
+};
+
+namespace {
+
+int fail(sl_context* c, int code, const char* msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+int check(sl_context* c, hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return SL_OK;
+    if (c)
+    {
+        c->err = std::string(what) + ": " + hipGetErrorString(e);
+    }
+    return SL_ERR_DEVICE;
+}
+
+int ensure(sl_context* c, DevBuf& b, size_t bytes)
+{
+    if (bytes <= b.bytes) return SL_OK;
+    if (b.p)
+    {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return check(c, e, "hipStreamSynchronize");
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(c, SL_ERR_OUT_OF_MEMORY, "hipMalloc failed");
+    b.p = p;
+    b.bytes = bytes;
+    return SL_OK;
+}
+
+void release(DevBuf& b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+// SL_ViewportState::scissor_matrix / viewport_rect (SL_ViewportState.cpp:17-111)
+void viewport_setup(const sl_context* c, float fboW, float fboH, float mat[16], float vp[4])
+{
+    const float sc[4] = {(float)c->scissor[0], (float)c->scissor[1], (float)c->scissor[2], (float)c->scissor[3]};
+    float v[4] = {(float)c->viewport[0], (float)c->viewport[1], (float)c->viewport[2], (float)c->viewport[3]};
+    v[0] = std::min(std::max(v[0] + sc[0], 0.f), fboW);
+    v[1] = std::min(std::max(v[1] + sc[1], 0.f), fboH);
+    v[2] = std::min(std::min(v[2], (sc[0] + sc[2]) - v[0]), fboW - v[0]);
+    v[3] = std::min(std::min(v[3], (sc[1] + sc[3]) - v[1]), fboH - v[1]);
+    std::memcpy(vp, v, sizeof(v));
+    const float inv0 = 1.0f / fboW, inv1 = 1.0f / fboH;
+    float x = sc[0] * inv0, y = sc[1] * inv1, w = sc[2] * inv0, h = sc[3] * inv1;
+    if (x < 0.f) { w += x; x = 0.f; }
+    if (y < 0.f) { h += y; y = 0.f; }
+    w = std::min(1.f - x, w);
+    h = std::min(1.f - y, h);
+    const float nm00 = 1.0f / w, nm11 = 1.0f / h;
+    const float nm30 = nm00 - 1.f, nm31 = nm11 - 1.f;
+    const float m30 = std::fma(x, (-2.f * nm00), nm30);
+    const float m31 = std::fma(y, (-2.f * nm11), nm31);
+    const float m[16] = {nm00, 0.f, 0.f, 0.f, 0.f, nm11, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, m30, m31, 0.f, 1.f};
+    std::memcpy(mat, m, sizeof(m));
+}
+
+Tex* tex_at(sl_context* c, size_t id)
+{
+    return (id < c->texs.size() && c->texs[id].valid) ? &c->texs[id] : nullptr;
+}
+
+int flush(sl_context* c);
+
+// ---------------------------------------------------------------------------
+// batch execution
+// ---------------------------------------------------------------------------
+int execute_batch(sl_context* c)
+{
+    Batch& B = c->batch;
+    if (B.empty()) return SL_OK;
+    if (B.fbo >= c->fbos.size()) return fail(c, SL_ERR_INVALID_HANDLE, "batch framebuffer");
+    Fbo& F = c->fbos[B.fbo];
+    Tex* dt = tex_at(c, F.depth);
+    if (!dt || dt->type != SL_COLOR_R_FLOAT) return fail(c, SL_ERR_STATE, "framebuffer needs an R_FLOAT depth attachment");
+    Tex* ct = F.colors.empty() ? nullptr : tex_at(c, F.colors[0]);
+    Tex* dbgL = tex_at(c, F.dbgLast);
+    Tex* dbgC = tex_at(c, F.dbgCount);
+    const bool debug = dbgL && dbgC;
+
+    const int W = dt->w, H = dt->h;
+    const int tilesX = (W + kTileW - 1) / kTileW, tilesY = (H + kTileH - 1) / kTileH;
+    const int numTiles = tilesX * tilesY;
+    hipStream_t s = c->stream;
+    int rc;
+
+    if ((rc = ensure(c, c->dCtrl, sizeof(FrameCtrl))) != SL_OK) return rc;
+    if ((rc = ensure(c, c->dRanges, sizeof(uint2) * (size_t)numTiles)) != SL_OK) return rc;
+    FrameCtrl* ctrl = reinterpret_cast<FrameCtrl*>(c->dCtrl.p);
+    uint2* ranges = reinterpret_cast<uint2*>(c->dRanges.p);
+    if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(ctrl), 0u, sizeof(FrameCtrl) / 4, s), "fill ctrl")) != SL_OK) return rc;
+    if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(ranges), 0u, 2ull * numTiles, s), "fill ranges")) != SL_OK) return rc;
+
+    float msVertex = 0.f, msBin = 0.f, msRaster = 0.f;
+    const uint32_t* tileRecs = nullptr;
+    uint32_t totalRecs = 0, totalPairs = 0;
+
+    SetupParams sp{};
+    if (B.totalTris > 0)
+    {
+        if (B.totalTris >= (1ull << 32)) return fail(c, SL_ERR_CAPACITY, "more than 2^32 triangles in one batch");
+        const size_t nd = B.draws.size();
+        if ((rc = ensure(c, c->dDraws, sizeof(DrawDesc) * nd)) != SL_OK) return rc;
+        if ((rc = ensure(c, c->dUni, std::max<size_t>(B.uni.size(), 16))) != SL_OK) return rc;
+        if ((rc = check(c, hipMemcpyAsync(c->dDraws.p, B.draws.data(), sizeof(DrawDesc) * nd, hipMemcpyHostToDevice, s), "upload draws")) != SL_OK) return rc;
+        if (!B.uni.empty() && (rc = check(c, hipMemcpyAsync(c->dUni.p, B.uni.data(), B.uni.size(), hipMemcpyHostToDevice, s), "upload uniforms")) != SL_OK) return rc;
+
+        const uint64_t chunks = (B.totalTris + kSetupBlock - 1) / kSetupBlock;
+        if ((rc = ensure(c, c->dStatus, sizeof(unsigned long long) * chunks)) != SL_OK) return rc;
+
+        const uint32_t varStride = 3u * std::max<uint32_t>(B.maxVary, 1u);
+        if (c->recCap == 0) { c->recCap = 1u << 18; c->pairCap = 1u << 20; c->clipCap = 1u << 16; }
+        for (int attempt = 0; attempt < 4; ++attempt)
+        {
+            if ((rc = ensure(c, c->dRecs, sizeof(SetupRec) * (size_t)c->recCap)) != SL_OK) return rc;
+            if ((rc = ensure(c, c->dVars, sizeof(float4) * (size_t)varStride * c->recCap)) != SL_OK) return rc;
+            for (int k = 0; k < 2; ++k)
+            {
+                if ((rc = ensure(c, c->dKeys[k], sizeof(uint32_t) * (size_t)c->pairCap)) != SL_OK) return rc;
+                if ((rc = ensure(c, c->dVals[k], sizeof(uint32_t) * (size_t)c->pairCap)) != SL_OK) return rc;
+            }
+            if ((rc = ensure(c, c->dClip, sizeof(ClipJob) * (size_t)c->clipCap)) != SL_OK) return rc;
+
+            sp.draws = reinterpret_cast<const DrawDesc*>(c->dDraws.p);
+            sp.numDraws = (uint32_t)nd;
+            sp.totalTris = B.totalTris;
+            sp.uniforms = reinterpret_cast<const uint8_t*>(c->dUni.p);
+            sp.W = W; sp.H = H; sp.tilesX = tilesX; sp.tilesY = tilesY;
+            sp.recs = reinterpret_cast<SetupRec*>(c->dRecs.p);
+            sp.vars = reinterpret_cast<float4*>(c->dVars.p);
+            sp.varStride = varStride;
+            sp.recCap = c->recCap;
+            sp.pairKeys = reinterpret_cast<uint32_t*>(c->dKeys[0].p);
+            sp.pairVals = reinterpret_cast<uint32_t*>(c->dVals[0].p);
+            sp.pairCap = c->pairCap;
+            sp.clipJobs = reinterpret_cast<ClipJob*>(c->dClip.p);
+            sp.clipCap = c->clipCap;
+            sp.status = reinterpret_cast<unsigned long long*>(c->dStatus.p);
+            sp.ctrl = ctrl;
+
+            if ((rc = check(c, launch_fill_u64(sp.status, 0ull, chunks, s), "fill status")) != SL_OK) return rc;
+            if (attempt > 0 && (rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(ctrl), 0u, sizeof(FrameCtrl) / 4, s), "fill ctrl")) != SL_OK) return rc;
+            (void)hipEventRecord(c->ev[0], s);
+            if ((rc = check(c, launch_setup(sp, chunks, s), "k_setup")) != SL_OK) return rc;
+            (void)hipEventRecord(c->ev[1], s);
+            if ((rc = check(c, hipMemcpyAsync(c->hCtrl, ctrl, sizeof(FrameCtrl), hipMemcpyDeviceToHost, s), "read ctrl")) != SL_OK) return rc;
+            if ((rc = check(c, hipStreamSynchronize(s), "k_setup sync")) != SL_OK) return rc;
+            if (c->hCtrl->error) return fail(c, SL_ERR_DEVICE, "setup look-back did not complete");
+            if (!c->hCtrl->overflow) break;
+            // grow to the exact requirement (+25%) and run setup again; the
+            // framebuffer has not been touched yet.
+            const uint64_t needR = c->hCtrl->totalRecs, needP = c->hCtrl->totalPairs, needC = c->hCtrl->clipJobs;
+            auto grow = [](uint64_t need, uint32_t cur) -> uint64_t { return std::max<uint64_t>(cur, need + need / 4 + 1024); };
+            const uint64_t nr = grow(needR, c->recCap), np = grow(needP, c->pairCap), nc = grow(needC, c->clipCap);
+            if (nr >= (1ull << 30) || np >= (1ull << 31) || nc >= (1ull << 31)) return fail(c, SL_ERR_CAPACITY, "frame exceeds bin capacity");
+            c->recCap = (uint32_t)nr; c->pairCap = (uint32_t)np; c->clipCap = (uint32_t)nc;
+            if (attempt == 3) return fail(c, SL_ERR_CAPACITY, "bin capacity growth did not converge");
+        }
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, c->ev[0], c->ev[1]);
+        msVertex += t;
+        totalRecs = c->hCtrl->totalRecs;
+        totalPairs = c->hCtrl->totalPairs;
+        const uint32_t jobs = c->hCtrl->clipJobs;
+
+        (void)hipEventRecord(c->ev[2], s);
+        if ((rc = check(c, launch_clip(sp, jobs, s), "k_clip")) != SL_OK) return rc;
+        (void)hipEventRecord(c->ev[3], s);
+
+        if (totalPairs > 0)
+        {
+            int endBit = 1;
+            while ((1 << endBit) < numTiles) ++endBit;
+            size_t tempBytes = 0;
+            if ((rc = check(c, sort_pairs(nullptr, tempBytes, sp.pairKeys, reinterpret_cast<uint32_t*>(c->dKeys[1].p), sp.pairVals,
+                                          reinterpret_cast<uint32_t*>(c->dVals[1].p), totalPairs, endBit, s), "sort size")) != SL_OK) return rc;
+            if ((rc = ensure(c, c->dSortTemp, std::max<size_t>(tempBytes, 16))) != SL_OK) return rc;
+            tempBytes = c->dSortTemp.bytes;
+            if ((rc = check(c, sort_pairs(c->dSortTemp.p, tempBytes, sp.pairKeys, reinterpret_cast<uint32_t*>(c->dKeys[1].p), sp.pairVals,
+                                          reinterpret_cast<uint32_t*>(c->dVals[1].p), totalPairs, endBit, s), "sort")) != SL_OK) return rc;
+            if ((rc = check(c, launch_ranges(reinterpret_cast<const uint32_t*>(c->dKeys[1].p), totalPairs, ranges, s), "k_ranges")) != SL_OK) return rc;
+            tileRecs = reinterpret_cast<const uint32_t*>(c->dVals[1].p);
+        }
+        (void)hipEventRecord(c->ev[4], s);
+    }
+    else
+    {
+        (void)hipEventRecord(c->ev[4], s);
+    }
+
+    RasterParams rp{};
+    rp.draws = reinterpret_cast<const DrawDesc*>(c->dDraws.p);
+    rp.uniforms = reinterpret_cast<const uint8_t*>(c->dUni.p);
+    rp.recs = reinterpret_cast<const SetupRec*>(c->dRecs.p);
+    rp.vars = reinterpret_cast<const float4*>(c->dVars.p);
+    rp.varStride = sp.varStride ? sp.varStride : 3;
+    rp.tileRecs = tileRecs;
+    rp.tileRange = ranges;
+    rp.W = W; rp.H = H; rp.tilesX = tilesX; rp.tilesY = tilesY;
+    rp.color = ct ? reinterpret_cast<uint8_t*>(ct->buf.p) : nullptr;
+    rp.colorType = ct ? ct->type : SL_COLOR_RGBA_8U;
+    if (ct && ct->type != SL_COLOR_RGBA_8U && ct->type != SL_COLOR_RGBA_FLOAT) return fail(c, SL_ERR_UNSUPPORTED, "colour attachment format");
+    if (ct && (ct->w != W || ct->h != H)) return fail(c, SL_ERR_STATE, "colour and depth attachment sizes differ");
+    rp.depth = reinterpret_cast<float*>(dt->buf.p);
+    rp.dbgLast = debug ? reinterpret_cast<uint32_t*>(dbgL->buf.p) : nullptr;
+    rp.dbgCount = debug ? reinterpret_cast<uint32_t*>(dbgC->buf.p) : nullptr;
+    rp.clearColor = B.clearColor ? 1 : 0;
+    rp.clearDepth = B.clearDepth ? 1 : 0;
+    uint32_t packed = 0;
+    for (int k = 0; k < 4; ++k)
+    {
+        packed |= ((uint32_t)(uint8_t)(int64_t)(B.rgba[k] * 255.0)) << (8 * k);   // color_cast<uint8_t,double>
+        rp.clearRGBAF[k] = (float)B.rgba[k];
+    }
+    rp.clearRGBA8 = packed;
+    rp.clearDepthValue = (float)B.depth;
+    rp.ctrl = ctrl;
+    if ((rc = check(c, launch_raster(rp, numTiles, debug, s), "k_raster")) != SL_OK) return rc;
+    (void)hipEventRecord(c->ev[5], s);
+    if ((rc = check(c, hipMemcpyAsync(c->hCtrl, ctrl, sizeof(FrameCtrl), hipMemcpyDeviceToHost, s), "read ctrl")) != SL_OK) return rc;
+    if ((rc = check(c, hipStreamSynchronize(s), "frame sync")) != SL_OK) return rc;
+
+    float t = 0.f;
+    if (B.totalTris > 0)
+    {
+        (void)hipEventElapsedTime(&t, c->ev[2], c->ev[3]);
+        msVertex += t;
+        (void)hipEventElapsedTime(&t, c->ev[3], c->ev[4]);
+        msBin += t;
+    }
+    (void)hipEventElapsedTime(&t, c->ev[4], c->ev[5]);
+    msRaster += t;
+    c->stats.triangles_binned += totalRecs;
+    c->stats.tile_pairs += totalPairs;
+    c->stats.fragments_tested += c->hCtrl->tested;
+    c->stats.fragments_shaded += c->hCtrl->shaded;
+    c->stats.frames += 1;
+    c->stats.ms_vertex += msVertex;
+    c->stats.ms_bin += msBin;
+    c->stats.ms_raster += msRaster;
+    c->stats.ms_total += msVertex + msBin + msRaster;
+    return SL_OK;
+}
+
+int flush(sl_context* c)
+{
+    if (c->batch.empty()) return SL_OK;
+    const int rc = execute_batch(c);
+    c->batch.reset();
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// draw recording
+// ---------------------------------------------------------------------------
+int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t shaderId, size_t fboId)
+{
+    if (shaderId >= c->shaders.size()) return fail(c, SL_ERR_INVALID_HANDLE, "shader handle");
+    if (fboId >= c->fbos.size()) return fail(c, SL_ERR_INVALID_HANDLE, "framebuffer handle");
+    if (m.vaoId >= c->vaos.size()) return fail(c, SL_ERR_INVALID_HANDLE, "vao handle");
+    if (m.mode != RENDER_MODE_TRIANGLES && m.mode != RENDER_MODE_INDEXED_TRIANGLES) return fail(c, SL_ERR_UNSUPPORTED, "render mode");
+    const Shader& sh = c->shaders[shaderId];
+    if (sh.nout > 1) return fail(c, SL_ERR_UNSUPPORTED, "more than one fragment output");
+    Fbo& F = c->fbos[fboId];
+    Tex* dt = tex_at(c, F.depth);
+    if (!dt || dt->type != SL_COLOR_R_FLOAT) return fail(c, SL_ERR_STATE, "framebuffer needs an R_FLOAT depth attachment");
+    if (sh.nout == 1)
+    {
+        Tex* ct = F.colors.empty() ? nullptr : tex_at(c, F.colors[0]);
+        if (!ct) return fail(c, SL_ERR_STATE, "framebuffer has no colour attachment 0");
+    }
+    const Vao& vao = c->vaos[m.vaoId];
+    const Vbo* vbo = (vao.vbo < c->vbos.size() && c->vbos[vao.vbo].valid) ? &c->vbos[vao.vbo] : nullptr;
+    const Ibo* ibo = (vao.ibo < c->ibos.size() && c->ibos[vao.ibo].valid) ? &c->ibos[vao.ibo] : nullptr;
+    const bool indexed = m.mode == RENDER_MODE_INDEXED_TRIANGLES;
+    if (indexed && !ibo) return fail(c, SL_ERR_STATE, "indexed draw without an index buffer");
+    const bool needUni = sh.vsProg != SL_PROGRAM_PASSTHROUGH_GOURAUD || sh.fsProg != SL_PROGRAM_PASSTHROUGH_GOURAUD;
+    if (needUni && sh.ubo >= c->ubos.size()) return fail(c, SL_ERR_STATE, "program needs a uniform buffer");
+
+    if (c->batch.fbo != fboId && !c->batch.empty())
+    {
+        const int rc = flush(c);
+        if (rc != SL_OK) return rc;
+    }
+    Batch& B = c->batch;
+    B.fbo = fboId;
+
+    const uint64_t count = (m.elementEnd > m.elementBegin) ? (m.elementEnd - m.elementBegin) : 0;
+    const uint64_t tpi = count / 3;
+    const uint64_t numTris = tpi * numInstances;
+    c->stats.triangles_submitted += numTris;
+    if (numTris == 0) return SL_OK;
+
+    DrawDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.triBase = B.totalTris;
+    d.numTris = numTris;
+    d.trisPerInstance = tpi;
+    d.elementBegin = m.elementBegin;
+    d.vbo = vbo ? reinterpret_cast<const uint8_t*>(vbo->buf.p) : nullptr;
+    d.vboBytes = vbo ? vbo->bytes : 0;
+    d.ibo = ibo ? reinterpret_cast<const uint8_t*>(ibo->buf.p) : nullptr;
+    d.iboType = ibo ? (uint32_t)ibo->type : 0u;
+    d.iboCount = ibo ? ibo->count : 0u;
+    d.indexed = indexed ? 1 : 0;
+    d.numBindings = (uint32_t)vao.nb;
+    for (size_t i = 0; i < vao.nb; ++i)
+    {
+        d.bind[i].offset = vao.b[i].offset;
+        d.bind[i].stride = vao.b[i].stride;
+        d.bind[i].dims = vao.b[i].dims;
+        d.bind[i].type = vao.b[i].type;
+    }
+    d.vsProg = sh.vsProg;
+    d.fsProg = sh.fsProg;
+    d.cull = sh.cull;
+    d.dtest = sh.dtest;
+    d.dmask = sh.dmask;
+    d.blend = sh.blend;
+    d.nvary = sh.nvary;
+    d.nout = sh.nout;
+    d.seqBase = c->triSeq;
+    c->triSeq += (uint32_t)numTris;
+    viewport_setup(c, (float)dt->w, (float)dt->h, d.scissor, d.vp);
+    if (sh.ubo < c->ubos.size())
+    {
+        const size_t off = B.uni.size();
+        B.uni.resize(off + SL_MAX_UNIFORM_BUFFER_SIZE);
+        std::memcpy(B.uni.data() + off, c->ubos[sh.ubo].bytes, SL_MAX_UNIFORM_BUFFER_SIZE);
+        d.uniOffset = (uint32_t)off;
+        d.hasUniforms = 1;
+        sl_std_uniforms u;
+        std::memcpy(&u, c->ubos[sh.ubo].bytes, sizeof(u));
+        if (u.texture != SL_NO_RESOURCE && u.texture < c->texs.size() && c->texs[u.texture].valid)
+        {
+            const Tex& t = c->texs[u.texture];
+            d.tex = reinterpret_cast<const uint8_t*>(t.buf.p);
+            d.texType = (uint32_t)t.type;
+            d.texW = t.w;
+            d.texH = t.h;
+        }
+        if (u.instance_vbo != SL_NO_RESOURCE && u.instance_vbo < c->vbos.size() && c->vbos[u.instance_vbo].valid)
+        {
+            d.instVbo = reinterpret_cast<const uint8_t*>(c->vbos[u.instance_vbo].buf.p);
+            d.instVboBytes = c->vbos[u.instance_vbo].bytes;
+        }
+    }
+    B.draws.push_back(d);
+    B.totalTris += numTris;
+    B.maxVary = std::max<uint32_t>(B.maxVary, sh.nvary);
+    return SL_OK;
+}
+
+int fill_texture(sl_context* c, Tex& t, const void* value, uint32_t bpt)
+{
+    const size_t n = (size_t)t.w * t.h * std::max<uint16_t>(t.d, 1);
+    if (bpt == 4)
+    {
+        uint32_t v;
+        std::memcpy(&v, value, 4);
+        return check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(t.buf.p), v, n, c->stream), "fill texture");
+    }
+    if (bpt == 16)
+    {
+        uint32_t v[4];
+        std::memcpy(v, value, 16);
+        if (v[0] == v[1] && v[1] == v[2] && v[2] == v[3])
+            return check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(t.buf.p), v[0], 4 * n, c->stream), "fill texture");
+        std::vector<uint8_t> host(n * 16);
+        for (size_t i = 0; i < n; ++i) std::memcpy(host.data() + 16 * i, value, 16);
+        hipError_t e = hipMemcpy(t.buf.p, host.data(), host.size(), hipMemcpyHostToDevice);
+        return check(c, e, "fill texture");
+    }
+    return fail(c, SL_ERR_UNSUPPORTED, "clear of this texture format");
+}
+
+int record_clear(sl_context* c, size_t fboId, int att, const double* rgba, bool depth, double dval)
+{
+    if (fboId >= c->fbos.size()) return fail(c, SL_ERR_INVALID_HANDLE, "framebuffer handle");
+    Fbo& F = c->fbos[fboId];
+    if (rgba)
+    {
+        if (att < 0 || (size_t)att >= F.colors.size() || !tex_at(c, F.colors[att])) return fail(c, SL_ERR_INVALID_HANDLE, "colour attachment");
+        const int ty = c->texs[F.colors[att]].type;
+        if (ty != SL_COLOR_RGBA_8U && ty != SL_COLOR_RGBA_FLOAT) return fail(c, SL_ERR_UNSUPPORTED, "clear of this colour format");
+    }
+    if (depth)
+    {
+        Tex* dt = tex_at(c, F.depth);
+        if (!dt) return fail(c, SL_ERR_STATE, "no depth attachment");
+        if (dt->type != SL_COLOR_R_FLOAT) return fail(c, SL_ERR_UNSUPPORTED, "depth format");
+    }
+    // a clear after draws, or of another framebuffer, ends the current batch
+    if (!c->batch.empty() && (c->batch.fbo != fboId || !c->batch.draws.empty()))
+    {
+        const int rc = flush(c);
+        if (rc != SL_OK) return rc;
+    }
+    if (rgba && att != 0)
+    {
+        // attachments other than 0 are cleared directly
+        Tex& t = c->texs[F.colors[att]];
+        if (t.type == SL_COLOR_RGBA_8U)
+        {
+            uint8_t v[4];
+            for (int k = 0; k < 4; ++k) v[k] = (uint8_t)(int64_t)(rgba[k] * 255.0);
+            return fill_texture(c, t, v, 4);
+        }
+        float v[4];
+        for (int k = 0; k < 4; ++k) v[k] = (float)rgba[k];
+        return fill_texture(c, t, v, 16);
+    }
+    Batch& B = c->batch;
+    B.fbo = fboId;
+    if (rgba)
+    {
+        B.clearColor = true;
+        for (int k = 0; k < 4; ++k) B.rgba[k] = rgba[k];
+    }
+    if (depth)
+    {
+        B.clearDepth = true;
+        B.depth = dval;
+    }
+    return SL_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int sl_context_create(sl_context** out)
+{
+    if (!out) return SL_ERR_INVALID_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SL_ERR_DEVICE;
+    sl_context* c = new sl_context();
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return SL_ERR_DEVICE; }
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) { delete c; return SL_ERR_DEVICE; }
+    if (hipHostMalloc(reinterpret_cast<void**>(&c->hCtrl), sizeof(FrameCtrl), hipHostMallocDefault) != hipSuccess) { delete c; return SL_ERR_DEVICE; }
+    std::memset(c->hCtrl, 0, sizeof(FrameCtrl));
+    *out = c;
+    return SL_OK;
+}
+
+void sl_context_destroy(sl_context* c)
+{
+    if (!c) return;
+    (void)flush(c);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& v : c->vbos) release(v.buf);
+    for (auto& v : c->ibos) release(v.buf);
+    for (auto& t : c->texs) release(t.buf);
+    for (DevBuf* b : {&c->dDraws, &c->dUni, &c->dRecs, &c->dVars, &c->dKeys[0], &c->dKeys[1], &c->dVals[0], &c->dVals[1],
+                      &c->dClip, &c->dStatus, &c->dRanges, &c->dCtrl, &c->dSortTemp}) release(*b);
+    for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->hCtrl) (void)hipHostFree(c->hCtrl);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* sl_last_error(const sl_context* c) { return c ? c->err.c_str() : "null context"; }
+
+unsigned sl_num_threads(const sl_context* c) { return c ? c->numThreads : 0u; }
+
+unsigned sl_set_num_threads(sl_context* c, unsigned n)
+{
+    if (!c) return 0;
+    c->numThreads = std::max(1u, n);   // the GPU path has no host worker pool; kept for API parity
+    return c->numThreads;
+}
+
+int sl_set_viewport(sl_context* c, int32_t x, int32_t y, int32_t w, int32_t h)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    c->viewport[0] = std::min(std::max(x, -65536), 65535);   // SL_ViewportState.hpp:148-156
+    c->viewport[1] = std::min(std::max(y, -65536), 65535);
+    c->viewport[2] = std::min(w, 65535 - x);
+    c->viewport[3] = std::min(h, 65535 - y);
+    return SL_OK;
+}
+
+int sl_set_scissor(sl_context* c, int32_t x, int32_t y, int32_t w, int32_t h)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    c->scissor[0] = std::min(std::max(x, -65536), 65535);
+    c->scissor[1] = std::min(std::max(y, -65536), 65535);
+    c->scissor[2] = std::min(w, 65535 - x);
+    c->scissor[3] = std::min(h, 65535 - y);
+    return SL_OK;
+}
+
+size_t sl_create_vbo(sl_context* c) { c->vbos.emplace_back(); return c->vbos.size() - 1; }
+
+int sl_vbo_init(sl_context* c, size_t vbo, size_t numBytes, const void* data)
+{
+    if (!c || vbo >= c->vbos.size()) return SL_ERR_INVALID_HANDLE;
+    if (!numBytes) return -1;                                   // SL_VertexBuffer.cpp:98-101
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    Vbo& v = c->vbos[vbo];
+    if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
+    release(v.buf);
+    if ((rc = ensure(c, v.buf, numBytes)) != SL_OK) return rc;
+    v.bytes = numBytes;
+    v.valid = true;
+    if (data) return check(c, hipMemcpy(v.buf.p, data, numBytes, hipMemcpyHostToDevice), "vbo upload");
+    return check(c, hipMemset(v.buf.p, 0, numBytes), "vbo zero");
+}
+
+int sl_vbo_assign(sl_context* c, size_t vbo, const void* data, size_t offset, size_t numBytes)
+{
+    if (!c || vbo >= c->vbos.size() || !c->vbos[vbo].valid) return SL_ERR_INVALID_HANDLE;
+    Vbo& v = c->vbos[vbo];
+    if (offset + numBytes > v.bytes || !data) return SL_ERR_INVALID_ARG;
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    return check(c, hipMemcpy(reinterpret_cast<uint8_t*>(v.buf.p) + offset, data, numBytes, hipMemcpyHostToDevice), "vbo assign");
+}
+
+size_t sl_create_ibo(sl_context* c) { c->ibos.emplace_back(); return c->ibos.size() - 1; }
+
+int sl_ibo_init(sl_context* c, size_t ibo, uint32_t numElements, int type, const void* data)
+{
+    if (!c || ibo >= c->ibos.size()) return SL_ERR_INVALID_HANDLE;
+    if (!numElements) return -1;                                // SL_IndexBuffer.cpp:144-147
+    const uint32_t bpe = bytes_per_index(type);
+    if (!bpe) return SL_ERR_INVALID_ARG;
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
+    Ibo& b = c->ibos[ibo];
+    release(b.buf);
+    const size_t bytes = (size_t)numElements * bpe;
+    if ((rc = ensure(c, b.buf, bytes + 16)) != SL_OK) return rc;   // padded like SL_IndexBuffer.cpp:155
+    if ((rc = check(c, hipMemset(b.buf.p, 0, bytes + 16), "ibo zero")) != SL_OK) return rc;
+    if (data && (rc = check(c, hipMemcpy(b.buf.p, data, bytes, hipMemcpyHostToDevice), "ibo upload")) != SL_OK) return rc;
+    b.type = type;
+    b.bpe = bpe;
+    b.count = numElements;
+    b.valid = true;
+    return SL_OK;
+}
+
+size_t sl_create_vao(sl_context* c) { c->vaos.emplace_back(); return c->vaos.size() - 1; }
+
+int sl_vao_set_vertex_buffer(sl_context* c, size_t vao, size_t vbo)
+{
+    if (!c || vao >= c->vaos.size()) return SL_ERR_INVALID_HANDLE;
+    c->vaos[vao].vbo = vbo;
+    return SL_OK;
+}
+
+int sl_vao_set_index_buffer(sl_context* c, size_t vao, size_t ibo)
+{
+    if (!c || vao >= c->vaos.size()) return SL_ERR_INVALID_HANDLE;
+    c->vaos[vao].ibo = ibo;
+    return SL_OK;
+}
+
+int sl_vao_set_num_bindings(sl_context* c, size_t vao, size_t n)
+{
+    if (!c || vao >= c->vaos.size()) return SL_ERR_INVALID_HANDLE;
+    if (n >= SL_VAO_MAX_BINDINGS) return -1;                    // SL_VertexArray.cpp:164-167
+    Vao& v = c->vaos[vao];
+    for (size_t i = v.nb; i < n; ++i) v.b[i] = Binding{};
+    v.nb = n;
+    return (int)n;
+}
+
+int sl_vao_set_binding(sl_context* c, size_t vao, size_t bindId, ptrdiff_t offset, ptrdiff_t stride, int dims, int type)
+{
+    if (!c || vao >= c->vaos.size()) return SL_ERR_INVALID_HANDLE;
+    Vao& v = c->vaos[vao];
+    if (bindId >= v.nb) return SL_ERR_INVALID_ARG;
+    if (dims < VERTEX_DIMENSION_1 || dims > VERTEX_DIMENSION_4) return SL_ERR_INVALID_ARG;
+    v.b[bindId].offset = offset;
+    v.b[bindId].stride = stride;
+    v.b[bindId].dims = dims;
+    v.b[bindId].type = type;
+    return SL_OK;
+}
+
+size_t sl_create_texture(sl_context* c) { c->texs.emplace_back(); return c->texs.size() - 1; }
+
+int sl_texture_init(sl_context* c, size_t tex, int type, uint16_t w, uint16_t h, uint16_t d)
+{
+    if (!c || tex >= c->texs.size()) return SL_ERR_INVALID_HANDLE;
+    const uint32_t bpt = bytes_per_color(type);
+    if (!bpt) return SL_ERR_UNSUPPORTED;
+    if (!w || !h || !d) return SL_ERR_INVALID_ARG;
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
+    Tex& t = c->texs[tex];
+    release(t.buf);
+    // SL_Texture.cpp:28-55: each dimension padded by (4 - dim % 4), zero filled,
+    // so the bilinear sampler's x = W / y = H neighbours read zeros.
+    const size_t pw = (size_t)w + (4 - w % 4), ph = (size_t)h + (4 - h % 4), pd = (size_t)d + (4 - d % 4);
+    const size_t bytes = pw * ph * pd * bpt + 4 * bpt;
+    if ((rc = ensure(c, t.buf, bytes)) != SL_OK) return rc;
+    if ((rc = check(c, hipMemset(t.buf.p, 0, bytes), "texture zero")) != SL_OK) return rc;
+    t.type = type; t.w = w; t.h = h; t.d = d; t.bpt = bpt;
+    t.valid = true;
+    return SL_OK;
+}
+
+int sl_texture_set_texels(sl_context* c, size_t tex, const void* data, size_t numBytes)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    Tex* t = tex_at(c, tex);
+    if (!t) return SL_ERR_INVALID_HANDLE;
+    const size_t n = (size_t)t->w * t->h * t->d * t->bpt;
+    if (numBytes != n || !data) return SL_ERR_INVALID_ARG;
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    return check(c, hipMemcpy(t->buf.p, data, n, hipMemcpyHostToDevice), "texture upload");
+}
+
+int sl_texture_read(sl_context* c, size_t tex, void* out, size_t numBytes)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    Tex* t = tex_at(c, tex);
+    if (!t) return SL_ERR_INVALID_HANDLE;
+    const size_t n = (size_t)t->w * t->h * t->d * t->bpt;
+    if (numBytes != n || !out) return SL_ERR_INVALID_ARG;
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
+    return check(c, hipMemcpy(out, t->buf.p, n, hipMemcpyDeviceToHost), "texture readback");
+}
+
+size_t sl_create_framebuffer(sl_context* c) { c->fbos.emplace_back(); return c->fbos.size() - 1; }
+
+int sl_fbo_reserve_color_buffers(sl_context* c, size_t fbo, unsigned n)
+{
+    if (!c || fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    if (n > SL_SHADER_MAX_FRAG_OUTPUTS) return SL_ERR_INVALID_ARG;
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    c->fbos[fbo].colors.assign(n, SL_INVALID_ID);
+    return SL_OK;
+}
+
+int sl_fbo_attach_color_buffer(sl_context* c, size_t fbo, unsigned idx, size_t tex)
+{
+    if (!c || fbo >= c->fbos.size() || tex >= c->texs.size()) return SL_ERR_INVALID_HANDLE;
+    Fbo& f = c->fbos[fbo];
+    if (idx >= f.colors.size()) return SL_ERR_INVALID_ARG;
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    f.colors[idx] = tex;
+    return SL_OK;
+}
+
+int sl_fbo_attach_depth_buffer(sl_context* c, size_t fbo, size_t tex)
+{
+    if (!c || fbo >= c->fbos.size() || tex >= c->texs.size()) return SL_ERR_INVALID_HANDLE;
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    c->fbos[fbo].depth = tex;
+    return SL_OK;
+}
+
+int sl_fbo_attach_debug_buffers(sl_context* c, size_t fbo, size_t lastTex, size_t countTex)
+{
+    if (!c || fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    for (size_t t : {lastTex, countTex})
+        if (t != SL_INVALID_ID && (t >= c->texs.size() || c->texs[t].type != SL_COLOR_R_32U)) return SL_ERR_INVALID_ARG;
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    c->fbos[fbo].dbgLast = lastTex;
+    c->fbos[fbo].dbgCount = countTex;
+    return SL_OK;
+}
+
+int sl_fbo_valid(sl_context* c, size_t fbo)
+{
+    if (!c || fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    const Fbo& f = c->fbos[fbo];
+    if (f.colors.empty()) return -1;                            // SL_Framebuffer.cpp:930-990
+    for (size_t t : f.colors) if (!tex_at(c, t)) return -2;
+    const Tex& c0 = c->texs[f.colors[0]];
+    for (size_t t : f.colors)
+    {
+        if (c->texs[t].w != c0.w) return -3;
+        if (c->texs[t].h != c0.h) return -4;
+        if (c->texs[t].d != c0.d) return -5;
+    }
+    const Tex* dt = tex_at(c, f.depth);
+    if (!dt) return -6;
+    if (dt->w != c0.w) return -7;
+    if (dt->h != c0.h) return -8;
+    if (dt->d > 1) return -9;
+    if (dt->type != SL_COLOR_R_HALF && dt->type != SL_COLOR_R_FLOAT && dt->type != SL_COLOR_R_DOUBLE) return -10;
+    return 0;
+}
+
+size_t sl_create_ubo(sl_context* c)
+{
+    c->ubos.emplace_back();
+    std::memset(c->ubos.back().bytes, 0, SL_MAX_UNIFORM_BUFFER_SIZE);
+    return c->ubos.size() - 1;
+}
+
+int sl_ubo_assign(sl_context* c, size_t ubo, const void* data, size_t offset, size_t n)
+{
+    if (!c || ubo >= c->ubos.size()) return SL_ERR_INVALID_HANDLE;
+    if (offset + n > SL_MAX_UNIFORM_BUFFER_SIZE || !data) return SL_ERR_INVALID_ARG;
+    std::memcpy(c->ubos[ubo].bytes + offset, data, n);         // snapshotted by later draws
+    return SL_OK;
+}
+
+size_t sl_create_shader(sl_context* c, const sl_vertex_shader* vs, const sl_fragment_shader* fs, size_t ubo)
+{
+    // SL_Context::create_shader (SL_Context.cpp:547-604)
+    if (!c || !vs || !fs) return SL_INVALID_ID;
+    if (ubo != SL_INVALID_ID && ubo >= c->ubos.size()) return SL_INVALID_ID;
+    if (vs->numVaryings < fs->numVaryings) return SL_INVALID_ID;
+    if (vs->numVaryings > SL_SHADER_MAX_VARYING_VECTORS) return SL_INVALID_ID;
+    if (fs->numVaryings > SL_SHADER_MAX_VARYING_VECTORS) return SL_INVALID_ID;
+    if (fs->numOutputs > SL_SHADER_MAX_FRAG_OUTPUTS) return SL_INVALID_ID;
+    if (vs->cullMode > SL_CULL_OFF || fs->depthTest > SL_DEPTH_TEST_NOT_EQUAL || fs->blend > SL_BLEND_SCREEN) return SL_INVALID_ID;
+    Shader s;
+    s.cull = vs->cullMode;
+    s.dtest = fs->depthTest;
+    s.dmask = fs->depthMask;
+    s.blend = fs->blend;
+    s.nvary = fs->numVaryings;
+    s.nout = fs->numOutputs;
+    s.vsProg = vs->programId;
+    s.fsProg = fs->programId;
+    s.ubo = ubo;
+    c->shaders.push_back(s);
+    return c->shaders.size() - 1;
+}
+
+int sl_draw(sl_context* c, const sl_mesh* m, size_t shader, size_t fbo)
+{
+    if (!c || !m) return SL_ERR_INVALID_ARG;
+    return record_draw(c, *m, 1, shader, fbo);
+}
+
+int sl_draw_multiple(sl_context* c, const sl_mesh* m, size_t n, size_t shader, size_t fbo)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    if (m == nullptr || n == 0) return SL_OK;                   // SL_Context.cpp:726-730
+    for (size_t i = 0; i < n; ++i)
+    {
+        const int rc = record_draw(c, m[i], 1, shader, fbo);
+        if (rc != SL_OK) return rc;
+    }
+    return SL_OK;
+}
+
+int sl_draw_instanced(sl_context* c, const sl_mesh* m, size_t numInstances, size_t shader, size_t fbo)
+{
+    if (!c || !m) return SL_ERR_INVALID_ARG;
+    if (numInstances == 0) return SL_OK;
+    return record_draw(c, *m, numInstances, shader, fbo);
+}
+
+int sl_clear_color_buffer(sl_context* c, size_t fbo, unsigned att, const double rgba[4])
+{
+    if (!c || !rgba) return SL_ERR_INVALID_ARG;
+    return record_clear(c, fbo, (int)att, rgba, false, 0.0);
+}
+
+int sl_clear_depth_buffer(sl_context* c, size_t fbo, double depth)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    return record_clear(c, fbo, -1, nullptr, true, depth);
+}
+
+int sl_clear_framebuffer(sl_context* c, size_t fbo, unsigned att, const double rgba[4], double depth)
+{
+    if (!c || !rgba) return SL_ERR_INVALID_ARG;
+    const int rc = record_clear(c, fbo, (int)att, rgba, att == 0, depth);
+    if (rc != SL_OK || att == 0) return rc;
+    return record_clear(c, fbo, -1, nullptr, true, depth);
+}
+
+int sl_finish(sl_context* c)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    const int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    return check(c, hipStreamSynchronize(c->stream), "finish");
+}
+
+int sl_get_stats(const sl_context* c, sl_frame_stats* out)
+{
+    if (!c || !out) return SL_ERR_INVALID_ARG;
+    *out = c->stats;
+    return SL_OK;
+}
+
+int sl_reset_stats(sl_context* c)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    const int rc = flush(c);
+    c->stats = sl_frame_stats{};
+    c->triSeq = 0;
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,467 @@
+// sl_stages.h -- per-primitive and per-fragment stage functions of the gfx950
+// draw path (vertex programs, clipping, triangle setup, scanline spans,
+// samplers, fragment programs, output merger).
+//
+// Each function names the reference routine whose results it reproduces; the
+// arithmetic order follows the reference's portable scalar branch, with
+// rcp = IEEE 1/x and fmadd = fused, and everything else unfused.
+#pragma once
+
+#include "sl_device.h"
+
+namespace slg {
+
+struct Vary4 { float4 v[SL_SHADER_MAX_VARYING_VECTORS]; };
+
+__device__ __forceinline__ float4 f4(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
+
+// ls::math::mat4 * vec4, column-major: ((c0*x + c1*y) + c2*z) + c3*w
+__device__ __forceinline__ float4 mat_mul(const float* m, float4 p)
+{
+    float r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+        const float a = m[0 * 4 + i] * p.x;
+        const float b = m[1 * 4 + i] * p.y;
+        const float c = m[2 * 4 + i] * p.z;
+        const float d = m[3 * 4 + i] * p.w;
+        r[i] = ((a + b) + c) + d;
+    }
+    return f4(r[0], r[1], r[2], r[3]);
+}
+
+__device__ __forceinline__ float dot4(float4 a, float4 b)
+{
+    return (((a.x * b.x) + (a.y * b.y)) + (a.z * b.z)) + (a.w * b.w);
+}
+
+__device__ __forceinline__ float4 mix4(float4 a, float4 b, float t)
+{
+    return f4(a.x + t * (b.x - a.x), a.y + t * (b.y - a.y), a.z + t * (b.z - a.z), a.w + t * (b.w - a.w));
+}
+
+__device__ __forceinline__ float comp(float4 v, int i)
+{
+    return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
+}
+
+// ---------------------------------------------------------------------------
+// Vertex fetch: SL_VertexArray::offset(bind, vert) = offset + stride*vert
+// (SL_VertexArray.hpp:120).  Reads past the buffer end return zeros.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float4 read_attr(const DrawDesc& d, int bind, uint64_t vertId, float defW)
+{
+    float o[4] = {0.f, 0.f, 0.f, defW};
+    if ((uint32_t)bind >= d.numBindings || d.vbo == nullptr) return f4(o[0], o[1], o[2], o[3]);
+    const BindingDesc& b = d.bind[bind];
+    const int64_t off = b.offset + b.stride * (int64_t)vertId;
+    const int n = b.dims + 1;
+    if (b.type == VERTEX_DATA_FLOAT)
+    {
+        if (off >= 0 && (uint64_t)off + 4u * (uint64_t)n <= d.vboBytes && (off & 3) == 0)
+        {
+            const float* p = reinterpret_cast<const float*>(d.vbo + off);
+            for (int i = 0; i < n; ++i) o[i] = p[i];
+        }
+    }
+    else if (b.type == VERTEX_DATA_BYTE)
+    {
+        if (off >= 0 && (uint64_t)off + (uint64_t)n <= d.vboBytes)
+        {
+            const uint8_t* p = d.vbo + off;
+            for (int i = 0; i < n; ++i) o[i] = (float)p[i] * 0.00392156862745f;   // color_cast<float,uint8_t>, SL_Color.hpp:666-669
+        }
+    }
+    return f4(o[0], o[1], o[2], o[3]);
+}
+
+__device__ __forceinline__ void model_matrix(const DrawDesc& d, const sl_std_uniforms* u, uint64_t inst, float m[16])
+{
+    if (d.instVbo != nullptr)
+    {
+        const uint64_t off = inst * 64u;
+        if (off + 64u <= d.instVboBytes)
+        {
+            const float4* p = reinterpret_cast<const float4*>(d.instVbo + off);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+            {
+                const float4 col = p[c];
+                m[c * 4 + 0] = col.x; m[c * 4 + 1] = col.y; m[c * 4 + 2] = col.z; m[c * 4 + 3] = col.w;
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) m[i] = 0.f;
+        }
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) m[i] = u->model[i];
+}
+
+// Built-in vertex programs (replace SL_VertexShader::shader, SL_Shader.hpp:58-64)
+__device__ __forceinline__ float4 vertex_program(const DrawDesc& d, const uint8_t* uniArena, uint64_t vertId,
+                                                 uint64_t inst, Vary4& out)
+{
+#pragma unroll
+    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i) out.v[i] = f4(0.f, 0.f, 0.f, 0.f);
+    if (d.vsProg == SL_PROGRAM_PASSTHROUGH_GOURAUD)
+    {
+        out.v[0] = read_attr(d, 1, vertId, 1.f);
+        return read_attr(d, 0, vertId, 1.f);
+    }
+    if (d.vsProg == SL_PROGRAM_LAMBERT_TEXTURED || d.vsProg == SL_PROGRAM_COLOR_TEXTURED)
+    {
+        const sl_std_uniforms* u = reinterpret_cast<const sl_std_uniforms*>(uniArena + d.uniOffset);
+        const float4 p = read_attr(d, 0, vertId, 1.f);
+        const float4 a = read_attr(d, 1, vertId, 0.f);
+        const float4 uv = read_attr(d, 2, vertId, 0.f);
+        float m[16];
+        model_matrix(d, u, inst, m);
+        const float4 world = mat_mul(m, f4(p.x, p.y, p.z, 1.f));
+        if (d.vsProg == SL_PROGRAM_LAMBERT_TEXTURED)
+        {
+            const float4 n = mat_mul(m, f4(a.x, a.y, a.z, 0.f));
+            out.v[0] = f4(uv.x, uv.y, 0.f, 0.f);
+            out.v[1] = f4(n.x, n.y, n.z, 0.f);
+        }
+        else
+        {
+            out.v[0] = f4(a.x, a.y, a.z, 1.f);
+            out.v[1] = f4(uv.x, uv.y, 0.f, 0.f);
+        }
+        return mat_mul(u->mvp, world);
+    }
+    return f4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// Vertex stage helpers (SL_TriProcessor.cpp, SL_PostVertexTransform.hpp)
+// ---------------------------------------------------------------------------
+// face_determinant, generic branch (SL_TriProcessor.cpp:101-116)
+__device__ __forceinline__ float face_determinant(float4 p0, float4 p1, float4 p2)
+{
+    const float w0 = rcpf(p0.w), w1 = rcpf(p1.w), w2 = rcpf(p2.w);
+    const float x0 = p0.x * w0, x1 = p1.x * w1, x2 = p2.x * w2;
+    const float y0 = p0.y * w0, y1 = p1.y * w1, y2 = p2.y * w2;
+    const float4 xABC = f4(x0, x1, x2, 0.f);
+    const float4 dy = f4(y1 - y2, y2 - y0, y0 - y1, 0.f - 0.f);
+    return dot4(xABC, dy);
+}
+
+// sl_ndc_clip_status, generic branch (SL_PostVertexTransform.hpp:653-673)
+__device__ __forceinline__ bool clip_inside(float4 c)
+{
+    const float w = c.w, nw = -c.w;
+    return (c.x <= w) && (c.y <= w) && (c.z <= w) && (c.w <= w) &&
+           (c.x >= nw) && (c.y >= nw) && (c.z >= nw) && (c.w >= nw);
+}
+
+__device__ __forceinline__ int clip_status(float4 c0, float4 c1, float4 c2)
+{
+    const int vis = (clip_inside(c0) && clip_inside(c1) && clip_inside(c2)) ? 3 : 0;
+    const int part = (c0.w > 0.f || c1.w > 0.f || c2.w > 0.f) ? 1 : 0;
+    return vis | part;
+}
+
+// sl_perspective_divide (:239-249) + sl_ndc_to_screen_coords (:508-524)
+__device__ __forceinline__ float4 divide_and_snap(float4 p, const float vp[4])
+{
+    const float wInv = rcpf(p.w);
+    const float x = p.x * wInv, y = p.y * wInv, z = p.z * wInv;
+    const float hw = vp[2] * 0.5f, hh = vp[3] * 0.5f;
+    return f4(max_ref(floorf(fmaf(x + 1.f, hw, vp[0])), 0.f),
+              max_ref(floorf(fmaf(y + 1.f, hh, vp[1])), 0.f), z, wInv);
+}
+
+// Bounding-box reject of SL_TriProcessor::push_bin (:256-264) plus the
+// non-finite drop; returns false when the triangle produces no bin.
+__device__ __forceinline__ bool bin_accept(float4 p0, float4 p1, float4 p2)
+{
+    if (!isfinite(p0.x) || !isfinite(p1.x) || !isfinite(p2.x) ||
+        !isfinite(p0.y) || !isfinite(p1.y) || !isfinite(p2.y)) return false;
+    const float minX = fminf(fminf(p0.x, p1.x), p2.x), maxX = fmaxf(fmaxf(p0.x, p1.x), p2.x);
+    const float minY = fminf(fminf(p0.y, p1.y), p2.y), maxY = fmaxf(fmaxf(p0.y, p1.y), p2.y);
+    return !((maxX - minX < 1.f) || (maxY - minY < 1.f));
+}
+
+// Tiles a binned triangle can cover: rows [minY, maxY), columns [minX-1, maxX-1]
+// (a truncated span end may fall one pixel left of the snapped bounding box).
+struct TileRect { int tx0, tx1, ty0, ty1; };
+
+__device__ __forceinline__ TileRect tile_rect(float4 p0, float4 p1, float4 p2, int W, int H, int tilesX, int tilesY)
+{
+    const int minX = (int)fminf(fminf(p0.x, p1.x), p2.x);
+    const int maxX = (int)fmaxf(fmaxf(p0.x, p1.x), p2.x);
+    const int minY = (int)fminf(fminf(p0.y, p1.y), p2.y);
+    const int maxY = (int)fmaxf(fmaxf(p0.y, p1.y), p2.y);
+    TileRect r;
+    const int x0 = max(minX - 1, 0), x1 = min(maxX, W - 1);
+    const int y0 = max(minY, 0), y1 = min(maxY - 1, H - 1);
+    if (x0 > x1 || y0 > y1) { r.tx0 = 0; r.tx1 = -1; r.ty0 = 0; r.ty1 = -1; return r; }
+    r.tx0 = x0 / kTileW; r.tx1 = min(x1 / kTileW, tilesX - 1);
+    r.ty0 = y0 / kTileH; r.ty1 = min(y1 / kTileH, tilesY - 1);
+    return r;
+}
+
+__device__ __forceinline__ uint32_t tile_count(const TileRect& r)
+{
+    return (r.tx1 < r.tx0 || r.ty1 < r.ty0) ? 0u : (uint32_t)((r.tx1 - r.tx0 + 1) * (r.ty1 - r.ty0 + 1));
+}
+
+// push_bin's plane setup (SL_TriProcessor.cpp:271-314) and the scanline
+// constants of SL_ScanlineBounds::init, generic branch (SL_ScanlineBounds.hpp:143-166).
+__device__ __forceinline__ void setup_record(float4 p0, float4 p1, float4 p2, uint32_t draw, uint32_t seq, SetupRec& r)
+{
+    const float zyx = p2.x - p1.x, zyy = p2.y - p1.y;
+    const float xyx = p0.x - p1.x, xyy = p0.y - p1.y;
+    const float denom = rcpf((zyx * xyy) - (zyy * xyx));
+    const float X0 = p0.x, X1 = p1.x, X2 = p2.x, Y0 = p0.y, Y1 = p1.y, Y2 = p2.y;
+    const float ddx[3] = {(Y1 * 1.f) - (Y2 * 1.f), (Y2 * 1.f) - (Y0 * 1.f), (Y0 * 1.f) - (Y1 * 1.f)};
+    const float ddy[3] = {(1.f * X2) - (1.f * X1), (1.f * X0) - (1.f * X2), (1.f * X1) - (1.f * X0)};
+    const float ddz[3] = {(X1 * Y2) - (X2 * Y1), (X2 * Y0) - (X0 * Y2), (X0 * Y1) - (X1 * Y0)};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        r.bx[k] = ddx[k] * denom;
+        r.by[k] = ddy[k] * denom;
+        r.bz[k] = ddz[k] * denom;
+    }
+    // 4th barycentric component: planes (0*denom), so bc[3] is a signed zero
+    // (or NaN when denom is not finite); the depth and perspective sums add
+    // it times 0.
+    {
+        const float b3x = 0.f * denom, b3y = 0.f * denom, b3z = 0.f * denom;
+        // value at an arbitrary covered pixel: sign/NaN only depend on the planes
+        const float bcY = fmaf(b3y, 1.f, b3z);
+        const float bc = (1.f * b3x) + bcY;
+        r.bc3 = 0.f * bc;
+    }
+    r.x[0] = p0.x; r.x[1] = p1.x; r.x[2] = p2.x;
+    r.y[0] = p0.y; r.y[1] = p1.y; r.y[2] = p2.y;
+    r.z[0] = p0.z; r.z[1] = p1.z; r.z[2] = p2.z;
+    r.hw[0] = p0.w; r.hw[1] = p1.w; r.hw[2] = p2.w;
+    r.minY = f2i(fminf(fminf(p0.y, p1.y), p2.y));
+    r.maxY = f2i(fmaxf(fmaxf(p0.y, p1.y), p2.y));
+    r.draw = draw;
+    r.seq = seq;
+    r.pad[0] = r.pad[1] = 0;
+
+    // SL_ScanlineBounds::init: sort by y descending with the reference's swaps
+    float ax = p0.x, ay = p0.y, bxx = p1.x, byy = p1.y, cx = p2.x, cy = p2.y, t;
+    if (ay < byy) { t = ax; ax = bxx; bxx = t; t = ay; ay = byy; byy = t; }
+    if (ay < cy) { t = ax; ax = cx; cx = t; t = ay; ay = cy; cy = t; }
+    if (byy < cy) { t = bxx; bxx = cx; cx = t; t = byy; byy = cy; cy = t; }
+    r.v0y = ay; r.v1y = byy; r.v0x = ax; r.v1x = bxx;
+    r.p20y = rcpf(cy - ay);
+    r.p20x = cx - ax;
+    r.p21xy = (cx - bxx) * rcpf(cy - byy);
+    r.p10xy = (bxx - ax) * rcpf(byy - ay);
+}
+
+// SL_ScanlineBounds::step, generic branch (:310-322); only the selected edge
+// is converted (the other may be +-inf or NaN).
+__device__ __forceinline__ void scanline_step(const SetupRec& r, float yf, int32_t& xMin, int32_t& xMax)
+{
+    const float d0 = yf - r.v0y;
+    const float d1 = yf - r.v1y;
+    const float alpha = d0 * r.p20y;
+    const int32_t lo = f2i((r.p20x * alpha) + r.v0x);
+    const float he = (d1 < 0.f) ? ((r.p21xy * d1) + r.v1x) : ((r.p10xy * d0) + r.v0x);
+    const int32_t hi = f2i(he);
+    xMin = (lo >= hi) ? hi : lo;
+    xMax = (lo >= hi) ? lo : hi;
+}
+
+// ---------------------------------------------------------------------------
+// Samplers (SL_Sampler.hpp:51-218): REPEAT wrap, bilinear with the x+1/y+1
+// neighbours clamped to W/H (not W-1/H-1), ordered texel layout x + W*y.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wrap_repeat(float u)
+{
+    return (u < 0.f ? 1.f : 0.f) + (u - truncf(u));
+}
+
+__device__ __forceinline__ uint32_t to_u16(float f) { return ((uint32_t)f2i(f)) & 0xFFFFu; }
+
+struct BilinearTaps { uint32_t i0, i1, i2, i3; float dx, dy, omdx, omdy; };
+
+__device__ __forceinline__ BilinearTaps bilinear_taps(uint32_t W, uint32_t H, float u, float v)
+{
+    BilinearTaps t;
+    const float xf = wrap_repeat(u) * (float)W;
+    const float yf = wrap_repeat(v) * (float)H;
+    const uint32_t xi0 = to_u16(xf), yi0 = to_u16(yf);
+    const uint32_t xi1 = min(xi0 + 1u, W), yi1 = min(yi0 + 1u, H);
+    t.dx = xf - (float)xi0;
+    t.dy = yf - (float)yi0;
+    t.omdx = 1.f - t.dx;
+    t.omdy = 1.f - t.dy;
+    t.i0 = xi0 + W * yi0;
+    t.i1 = xi0 + W * yi1;
+    t.i2 = xi1 + W * yi0;
+    t.i3 = xi1 + W * yi1;
+    return t;
+}
+
+__device__ __forceinline__ float bilerp(const BilinearTaps& t, float p0, float p1, float p2, float p3)
+{
+    const float w0 = (p0 * t.omdx) * t.omdy;
+    const float w1 = (p1 * t.omdx) * t.dy;
+    const float w2 = (p2 * t.dx) * t.omdy;
+    const float w3 = (p3 * t.dx) * t.dy;
+    return ((w0 + w1) + w2) + w3;
+}
+
+// sl_sample_bilinear<RGBA8, REPEAT>: result cast back to u8 by truncation
+__device__ __forceinline__ uint32_t sample_bilinear_rgba8(const uint8_t* tex, uint32_t W, uint32_t H, float u, float v)
+{
+    const BilinearTaps t = bilinear_taps(W, H, u, v);
+    const uint32_t* px = reinterpret_cast<const uint32_t*>(tex);
+    const uint32_t q0 = px[t.i0], q1 = px[t.i1], q2 = px[t.i2], q3 = px[t.i3];
+    uint32_t out = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+    {
+        const int s = 8 * c;
+        const float s0 = (float)((q0 >> s) & 0xFFu) * 0.00392156862745f;
+        const float s1 = (float)((q1 >> s) & 0xFFu) * 0.00392156862745f;
+        const float s2 = (float)((q2 >> s) & 0xFFu) * 0.00392156862745f;
+        const float s3 = (float)((q3 >> s) & 0xFFu) * 0.00392156862745f;
+        const uint32_t b = ((uint32_t)f2i(bilerp(t, s0, s1, s2, s3) * 255.f)) & 0xFFu;
+        out |= b << s;
+    }
+    return out;
+}
+
+__device__ __forceinline__ float4 sample_bilinear_rgbaf(const uint8_t* tex, uint32_t W, uint32_t H, float u, float v)
+{
+    const BilinearTaps t = bilinear_taps(W, H, u, v);
+    const float4* px = reinterpret_cast<const float4*>(tex);
+    const float4 a = px[t.i0], b = px[t.i1], c = px[t.i2], d = px[t.i3];
+    return f4(bilerp(t, a.x, b.x, c.x, d.x), bilerp(t, a.y, b.y, c.y, d.y),
+              bilerp(t, a.z, b.z, c.z, d.z), bilerp(t, a.w, b.w, c.w, d.w));
+}
+
+// Built-in fragment programs (replace SL_FragmentShader::shader, SL_Shader.hpp:89-98)
+__device__ __forceinline__ bool fragment_program(const DrawDesc& d, const uint8_t* uniArena, const Vary4& var, float4& out)
+{
+    if (d.fsProg == SL_PROGRAM_PASSTHROUGH_GOURAUD)
+    {
+        out = var.v[0];
+        return true;
+    }
+    if (d.fsProg == SL_PROGRAM_LAMBERT_TEXTURED)
+    {
+        const sl_std_uniforms* u = reinterpret_cast<const sl_std_uniforms*>(uniArena + d.uniOffset);
+        uint32_t texel = 0;
+        if (d.tex != nullptr && d.texType == SL_COLOR_RGBA_8U)
+            texel = sample_bilinear_rgba8(d.tex, d.texW, d.texH, var.v[0].x, var.v[0].y);
+        const float a0 = (float)(texel & 0xFFu) * 0.00392156862745f;
+        const float a1 = (float)((texel >> 8) & 0xFFu) * 0.00392156862745f;
+        const float a2 = (float)((texel >> 16) & 0xFFu) * 0.00392156862745f;
+        const float a3 = (float)((texel >> 24) & 0xFFu) * 0.00392156862745f;
+        const float nx = var.v[1].x, ny = var.v[1].y, nz = var.v[1].z;
+        const float len2 = ((nx * nx) + (ny * ny)) + (nz * nz);
+        const float inv = 1.0f / sqrtf(len2);
+        const float mx = nx * inv, my = ny * inv, mz = nz * inv;
+        const float ndl = -(((mx * u->light_dir[0]) + (my * u->light_dir[1])) + (mz * u->light_dir[2]));
+        const float lam = (ndl > 0.f) ? ndl : 0.f;
+        const float alb[3] = {a0, a1, a2};
+        float o[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+            const float light = u->ambient[c] + (u->diffuse[c] * lam);
+            const float v = (alb[c] * light) * u->tint[c];
+            o[c] = (v < 1.f) ? v : 1.f;
+        }
+        out = f4(o[0], o[1], o[2], a3 * u->tint[3]);
+        return true;
+    }
+    if (d.fsProg == SL_PROGRAM_COLOR_TEXTURED)
+    {
+        float4 t = f4(1.f, 1.f, 1.f, 1.f);
+        if (d.tex != nullptr && d.texType == SL_COLOR_RGBA_FLOAT)
+            t = sample_bilinear_rgbaf(d.tex, d.texW, d.texH, var.v[1].x, var.v[1].y);
+        out = f4(var.v[0].x * t.x, var.v[0].y * t.y, var.v[0].z * t.z, var.v[0].w * t.w);
+        return true;
+    }
+    out = f4(0.f, 0.f, 0.f, 0.f);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Output merger (SL_Framebuffer.cpp:44-80,153-166,270-368)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float clamp01(float x) { return (x < 0.f) ? 0.f : ((x > 1.f) ? 1.f : x); }
+
+__device__ __forceinline__ void blend(int mode, float4 s, float d[4])
+{
+    const float as = s.w;
+    const float mod = 1.f - as;
+    const float sv[4] = {s.x, s.y, s.z, s.w};
+    if (mode == SL_BLEND_ALPHA)
+    {
+        const float dstMod = mod * d[3];
+        const float dstAlpha = dstMod + as;
+        const float r = rcpf(dstAlpha);
+        float o[4];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o[c] = fmaf(sv[c], as, d[c] * dstMod) * r;
+        o[3] = dstAlpha;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) d[c] = o[c];
+    }
+    else if (mode == SL_BLEND_PREMULTIPLED_ALPHA)
+    {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) d[c] = fmaf(d[c], mod, sv[c]);
+    }
+    else if (mode == SL_BLEND_ADDITIVE)
+    {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) d[c] = fmaf(sv[c], as, d[c]);
+    }
+    else if (mode == SL_BLEND_SCREEN)
+    {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) d[c] = (sv[c] * as) + (d[c] * mod);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) d[c] = clamp01(d[c]);
+}
+
+__device__ __forceinline__ uint32_t pack_rgba8(const float c[4])
+{
+    uint32_t o = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o |= (((uint32_t)f2i(c[k] * 255.f)) & 0xFFu) << (8 * k);
+    return o;
+}
+
+__device__ __forceinline__ void unpack_rgba8(uint32_t p, float d[4])
+{
+#pragma unroll
+    for (int k = 0; k < 4; ++k) d[k] = (float)((p >> (8 * k)) & 0xFFu) * 0.00392156862745f;
+}
+
+__device__ __forceinline__ bool depth_test(int f, float a, float b)
+{
+    switch (f)
+    {
+        case SL_DEPTH_TEST_OFF: return true;
+        case SL_DEPTH_TEST_LESS_THAN: return a < b;
+        case SL_DEPTH_TEST_LESS_EQUAL: return a <= b;
+        case SL_DEPTH_TEST_GREATER_THAN: return a > b;
+        case SL_DEPTH_TEST_GREATER_EQUAL: return a >= b;
+        case SL_DEPTH_TEST_EQUAL: return a == b;
+        case SL_DEPTH_TEST_NOT_EQUAL: return a != b;
+        default: return false;
+    }
+}
+
+}  // namespace slg

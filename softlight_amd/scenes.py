@@ -1,0 +1,276 @@
+"""Seeded synthetic workloads for the draw path.
+
+All randomness comes from a SplitMix64 stream turned into floats through the
+top 24 bits, so every backend receives bit-identical buffers.  The builders
+only use the SL_Context-shaped API (`SLContext`), so the same call sequence
+runs on any library that implements include/sl_gpu.h.
+
+Workloads (sizes are the defaults):
+  * `build_random_triangles` -- 4096 independent non-indexed triangles,
+    clip-space xyz ~ U[-1,1]^3, w = 1, RGBA8 vertex colours ~ U{0..255},
+    pass-through vertex program + Gouraud fragment program, 640x480 RGBA8 +
+    R32F targets cleared to 0 / 1.0, depth LESS, no blending, culling off.
+  * `build_sphere_scene` -- a 256x256-segment UV sphere (130,560 indexed u32
+    triangles, 66,049 vertices of {pos, normal, uv} f32) drawn as 64
+    instances at random positions in a 40-unit cube, 60 deg perspective
+    (near 0.1, far 100) from the origin looking down -z, a 1024x1024 RGBA8
+    value-noise texture sampled bilinearly with REPEAT, Lambert lighting from
+    one directional light, 1920x1080 RGBA8 + R32F, depth LESS, back-face
+    culling.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import api as A
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed: int, n: int, stream: int = 0) -> np.ndarray:
+    """n outputs of SplitMix64 started from (seed, stream)."""
+    with np.errstate(over="ignore"):
+        base = np.uint64((seed * 0x9E3779B97F4A7C15 + stream * 0xD1B54A32D192ED03) & 0xFFFFFFFFFFFFFFFF)
+        z = base + (np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed: int, n: int, stream: int = 0) -> np.ndarray:
+    """float32 in [0, 1) with 24 random mantissa bits (exactly representable)."""
+    r = splitmix64(seed, n, stream) >> np.uint64(40)
+    return (r.astype(np.float64) * (1.0 / 16777216.0)).astype(np.float32)
+
+
+def uniform(seed: int, n: int, lo: float, hi: float, stream: int = 0) -> np.ndarray:
+    u = uniform01(seed, n, stream)
+    return (np.float32(lo) + u * np.float32(hi - lo)).astype(np.float32)
+
+
+def uniform_u8(seed: int, n: int, stream: int = 0) -> np.ndarray:
+    return (splitmix64(seed, n, stream) >> np.uint64(56)).astype(np.uint8)
+
+
+# --------------------------------------------------------------------------
+# matrices (column-major, like ls::math::mat4)
+# --------------------------------------------------------------------------
+def perspective(fovy_deg: float, aspect: float, near: float, far: float) -> np.ndarray:
+    f = 1.0 / math.tan(math.radians(fovy_deg) * 0.5)
+    m = np.zeros((4, 4), dtype=np.float64)        # m[col][row]
+    m[0][0] = f / aspect
+    m[1][1] = f
+    m[2][2] = (far + near) / (near - far)
+    m[2][3] = -1.0
+    m[3][2] = 2.0 * far * near / (near - far)
+    return m.astype(np.float32)
+
+
+def translate_scale(tx, ty, tz, s) -> np.ndarray:
+    m = np.zeros((4, 4), dtype=np.float32)
+    m[0][0] = m[1][1] = m[2][2] = s
+    m[3][0], m[3][1], m[3][2], m[3][3] = tx, ty, tz, 1.0
+    return m
+
+
+def identity() -> np.ndarray:
+    return np.eye(4, dtype=np.float32)
+
+
+# --------------------------------------------------------------------------
+# geometry
+# --------------------------------------------------------------------------
+def uv_sphere(stacks: int, sectors: int, radius: float = 1.0):
+    """UV sphere with counter-clockwise outward faces (as seen from outside).
+
+    Returns (vertices float32[(stacks+1)*(sectors+1), 8] = pos, normal, uv;
+    indices uint32[ntri*3]).  Pole bands contribute one triangle per sector,
+    the other bands two: ntri = 2*stacks*sectors - 2*sectors.
+    """
+    i = np.arange(stacks + 1, dtype=np.float64)[:, None]
+    j = np.arange(sectors + 1, dtype=np.float64)[None, :]
+    phi = math.pi * 0.5 - math.pi * i / stacks          # +pi/2 (north) .. -pi/2
+    theta = 2.0 * math.pi * j / sectors
+    x = np.cos(phi) * np.cos(theta)
+    y = np.sin(phi) * np.ones_like(theta)
+    z = -np.cos(phi) * np.sin(theta)
+    nrm = np.stack([x, y, z], axis=-1).reshape(-1, 3)
+    pos = nrm * radius
+    uv = np.stack([np.broadcast_to(j / sectors, x.shape), np.broadcast_to(i / stacks, x.shape)], axis=-1).reshape(-1, 2)
+    verts = np.concatenate([pos, nrm, uv], axis=1).astype(np.float32)
+
+    k1 = (np.arange(stacks)[:, None] * (sectors + 1) + np.arange(sectors)[None, :])
+    k2 = k1 + (sectors + 1)
+    tris = []
+    top = np.stack([k1, k2, k1 + 1], axis=-1)        # skipped on the north pole band
+    bot = np.stack([k1 + 1, k2, k2 + 1], axis=-1)    # skipped on the south pole band
+    for s in range(stacks):
+        if s != 0:
+            tris.append(top[s])
+        if s != stacks - 1:
+            tris.append(bot[s])
+    idx = np.concatenate(tris, axis=0).astype(np.uint32).reshape(-1)
+    return verts, idx
+
+
+def value_noise_rgba8(size: int, seed: int, cells: int = 16) -> np.ndarray:
+    """Smooth value noise, one lattice of cells x cells per channel, RGBA8."""
+    out = np.empty((size, size, 4), dtype=np.uint8)
+    t = (np.arange(size, dtype=np.float64) + 0.5) * cells / size
+    i0 = np.floor(t).astype(np.int64) % cells
+    i1 = (i0 + 1) % cells
+    f = t - np.floor(t)
+    f = f * f * (3.0 - 2.0 * f)
+    for ch in range(4):
+        lat = uniform01(seed, cells * cells, stream=100 + ch).astype(np.float64).reshape(cells, cells)
+        if ch == 3:
+            lat = 0.5 + 0.5 * lat
+        a = lat[i0[:, None], i0[None, :]]
+        b = lat[i0[:, None], i1[None, :]]
+        c = lat[i1[:, None], i0[None, :]]
+        d = lat[i1[:, None], i1[None, :]]
+        fx = f[None, :]
+        fy = f[:, None]
+        v = (a * (1 - fx) + b * fx) * (1 - fy) + (c * (1 - fx) + d * fx) * fy
+        out[..., ch] = np.clip(v * 255.0, 0, 255).astype(np.uint8)
+    return out
+
+
+# --------------------------------------------------------------------------
+# scenes
+# --------------------------------------------------------------------------
+@dataclass
+class Scene:
+    ctx: A.SLContext
+    W: int
+    H: int
+    fbo: int
+    color: int
+    depth: int
+    shader: int
+    draws: list = field(default_factory=list)      # (kind, mesh, n) tuples
+    clear_rgba: tuple = (0.0, 0.0, 0.0, 0.0)
+    clear_depth: float = 1.0
+    n_triangles: int = 0
+    debug: tuple | None = None
+    info: dict = field(default_factory=dict)
+
+    def render(self, clear: bool = True):
+        if clear:
+            self.ctx.clear_framebuffer(self.fbo, 0, self.clear_rgba, self.clear_depth)
+        for kind, mesh, n in self.draws:
+            if kind == "draw":
+                self.ctx.draw(mesh, self.shader, self.fbo)
+            elif kind == "instanced":
+                self.ctx.draw_instanced(mesh, n, self.shader, self.fbo)
+            elif kind == "multiple":
+                self.ctx.draw_multiple(mesh, self.shader, self.fbo)
+
+    def read(self):
+        self.ctx.finish()
+        col = self.ctx.texture_read(self.color)
+        dep = self.ctx.texture_read(self.depth)
+        out = {"color": col, "depth": dep}
+        if self.debug is not None:
+            out["last"] = self.ctx.texture_read(self.debug[0])
+            out["count"] = self.ctx.texture_read(self.debug[1])
+        return out
+
+
+def make_target(ctx: A.SLContext, W: int, H: int, debug: bool, color_type=A.SL_COLOR_RGBA_8U):
+    color = ctx.create_texture(color_type, W, H)
+    depth = ctx.create_texture(A.SL_COLOR_R_FLOAT, W, H)
+    fbo = ctx.create_framebuffer([color], depth)
+    dbg = None
+    if debug:
+        last = ctx.create_texture(A.SL_COLOR_R_32U, W, H)
+        cnt = ctx.create_texture(A.SL_COLOR_R_32U, W, H)
+        ctx.attach_debug_buffers(fbo, last, cnt)
+        dbg = (last, cnt)
+    return color, depth, fbo, dbg
+
+
+def gouraud_shader(ctx, cull=A.SL_CULL_OFF, depth_test=A.SL_DEPTH_TEST_LESS_THAN,
+                   depth_mask=A.SL_DEPTH_MASK_ON, blend=A.SL_BLEND_OFF):
+    vs = A.sl_vertex_shader(1, cull, A.SL_PROGRAM_PASSTHROUGH_GOURAUD)
+    fs = A.sl_fragment_shader(1, 1, blend, depth_test, depth_mask, A.SL_PROGRAM_PASSTHROUGH_GOURAUD)
+    return ctx.create_shader(vs, fs)
+
+
+def random_triangle_vertices(seed: int, ntri: int) -> np.ndarray:
+    """Vertex records {f32 x, y, z; u8 r, g, b, a} for ntri independent triangles."""
+    nv = ntri * 3
+    xyz = uniform(seed, nv * 3, -1.0, 1.0, stream=1).reshape(nv, 3)
+    rgba = uniform_u8(seed, nv * 4, stream=2).reshape(nv, 4)
+    rec = np.zeros((nv, 16), dtype=np.uint8)
+    rec[:, 0:12] = xyz.view(np.uint8).reshape(nv, 12)
+    rec[:, 12:16] = rgba
+    return rec
+
+
+def build_from_vertex_records(ctx, rec: np.ndarray, W: int, H: int, debug=False, shader_kw=None,
+                              color_type=A.SL_COLOR_RGBA_8U) -> Scene:
+    nv = rec.shape[0]
+    vbo = ctx.create_vbo(rec)
+    vao = ctx.create_vao(vbo, None, [(0, 16, A.VERTEX_DIMENSION_3, A.VERTEX_DATA_FLOAT),
+                                     (12, 16, A.VERTEX_DIMENSION_4, A.VERTEX_DATA_BYTE)])
+    color, depth, fbo, dbg = make_target(ctx, W, H, debug, color_type)
+    shader = gouraud_shader(ctx, **(shader_kw or {}))
+    mesh = A.sl_mesh(vao, 0, nv, A.RENDER_MODE_TRIANGLES, 0)
+    return Scene(ctx, W, H, fbo, color, depth, shader, [("draw", mesh, 1)], n_triangles=nv // 3, debug=dbg)
+
+
+def build_random_triangles(ctx, seed=0, ntri=4096, W=640, H=480, debug=False) -> Scene:
+    s = build_from_vertex_records(ctx, random_triangle_vertices(seed, ntri), W, H, debug)
+    s.info.update(workload="random_triangles", seed=seed, triangles=ntri, W=W, H=H)
+    return s
+
+
+def sphere_instances(seed: int, n: int, extent: float = 20.0, radius: float = 2.0) -> np.ndarray:
+    pos = uniform(seed, n * 3, -extent, extent, stream=7).reshape(n, 3)
+    mats = np.stack([translate_scale(p[0], p[1], p[2], radius) for p in pos]).astype(np.float32)
+    return mats
+
+
+def build_sphere_scene(ctx, seed=1, n_instances=64, stacks=256, sectors=256, W=1920, H=1080,
+                       tex_size=1024, radius=2.0, extent=20.0, debug=False, cull=A.SL_CULL_BACK_FACE,
+                       blend=A.SL_BLEND_OFF, alpha=1.0) -> Scene:
+    verts, idx = uv_sphere(stacks, sectors)
+    vbo = ctx.create_vbo(verts)
+    ibo = ctx.create_ibo(idx)
+    vao = ctx.create_vao(vbo, ibo, [(0, 32, A.VERTEX_DIMENSION_3, A.VERTEX_DATA_FLOAT),
+                                    (12, 32, A.VERTEX_DIMENSION_3, A.VERTEX_DATA_FLOAT),
+                                    (24, 32, A.VERTEX_DIMENSION_2, A.VERTEX_DATA_FLOAT)])
+    tex = ctx.create_texture(A.SL_COLOR_RGBA_8U, tex_size, tex_size, 1, value_noise_rgba8(tex_size, seed))
+    inst = sphere_instances(seed, n_instances, extent, radius)
+    inst_vbo = ctx.create_vbo(inst)
+    u = A.sl_std_uniforms()
+    P = perspective(60.0, W / H, 0.1, 100.0)
+    u.mvp[:] = P.reshape(-1).tolist()
+    u.model[:] = identity().reshape(-1).tolist()
+    L = np.array([-1.0, -1.0, -1.0])
+    L = (L / np.linalg.norm(L)).astype(np.float32)
+    u.light_dir[:] = [float(L[0]), float(L[1]), float(L[2]), 0.0]
+    u.ambient[:] = [0.2, 0.2, 0.2, 0.0]
+    u.diffuse[:] = [0.8, 0.8, 0.8, 0.0]
+    u.tint[:] = [1.0, 1.0, 1.0, alpha]
+    u.texture = tex
+    u.instance_vbo = inst_vbo
+    u.flags = 0
+    ubo = ctx.create_ubo(u)
+    vs = A.sl_vertex_shader(2, cull, A.SL_PROGRAM_LAMBERT_TEXTURED)
+    fs = A.sl_fragment_shader(2, 1, blend, A.SL_DEPTH_TEST_LESS_THAN, A.SL_DEPTH_MASK_ON, A.SL_PROGRAM_LAMBERT_TEXTURED)
+    shader = ctx.create_shader(vs, fs, ubo)
+    color, depth, fbo, dbg = make_target(ctx, W, H, debug)
+    mesh = A.sl_mesh(vao, 0, idx.size, A.RENDER_MODE_INDEXED_TRIANGLES, 0)
+    s = Scene(ctx, W, H, fbo, color, depth, shader, [("instanced", mesh, n_instances)],
+              n_triangles=(idx.size // 3) * n_instances, debug=dbg)
+    s.info.update(workload="textured_spheres", seed=seed, instances=n_instances, stacks=stacks, sectors=sectors,
+                  W=W, H=H, tex_size=tex_size, radius=radius, extent=extent,
+                  triangles_per_instance=idx.size // 3, vertices_per_instance=verts.shape[0])
+    s.info["_keep"] = (verts, idx, inst)
+    return s

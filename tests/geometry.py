@@ -1,0 +1,112 @@
+"""Test-side helpers: exact (rational) statements of the reference's coverage
+rule and interpolation, and builders for screen-space test triangles.
+
+The coverage rule restated here is geometric, not a copy of the oracle's
+float code: for a triangle with integer (snapped) vertices, row y in
+[minY, maxY) covers x in [min(trunc(xa), trunc(xb)), max(trunc(xa), trunc(xb)))
+where xa, xb are the exact x positions of the triangle's two edges at height y
+(SL_ScanlineBounds.hpp:137-166,310-322; SL_TriRasterizer.cpp:826-916).
+"""
+from __future__ import annotations
+
+import math
+from fractions import Fraction as Fr
+
+import numpy as np
+
+from softlight_amd import api as A
+from softlight_amd import scenes
+
+
+def ndc_for_pixel(X: int, size: int) -> float:
+    """NDC coordinate whose viewport snap floor(fma(x+1, size/2, 0)) is X
+    (X == size maps to NDC 1.0, the edge of the clip volume)."""
+    if X >= size:
+        return 1.0
+    return (2.0 * X + 1.0) / size - 1.0
+
+
+def vertex_records(tris, W, H, z=None, w=None, colors=None) -> np.ndarray:
+    """Records {f32 x, y, z; u8 rgba} for triangles given in integer screen
+    coordinates [(X0, Y0), (X1, Y1), (X2, Y2)] (w = 1 unless given; clip
+    coordinates are scaled by w so the snapped position is unchanged)."""
+    n = len(tris)
+    rec = np.zeros((n * 3, 16), dtype=np.uint8)
+    pos = np.zeros((n * 3, 3), dtype=np.float32)
+    for t, tri in enumerate(tris):
+        for k, (X, Y) in enumerate(tri):
+            zz = 0.0 if z is None else z[t][k] if np.ndim(z[t]) else z[t]
+            pos[t * 3 + k] = (ndc_for_pixel(X, W), ndc_for_pixel(Y, H), zz)
+    rec[:, 0:12] = pos.view(np.uint8).reshape(-1, 12)
+    if colors is None:
+        rec[:, 12:16] = 255
+    else:
+        rec[:, 12:16] = np.asarray(colors, dtype=np.uint8).reshape(-1, 4)
+    return rec
+
+
+def build_screen_triangles(ctx, tris, W, H, z=None, colors=None, depth_test=A.SL_DEPTH_TEST_OFF,
+                           depth_mask=A.SL_DEPTH_MASK_ON, cull=A.SL_CULL_OFF, blend=A.SL_BLEND_OFF,
+                           color_type=A.SL_COLOR_RGBA_8U):
+    rec = vertex_records(tris, W, H, z=z, colors=colors)
+    return scenes.build_from_vertex_records(
+        ctx, rec, W, H, debug=True, color_type=color_type,
+        shader_kw=dict(cull=cull, depth_test=depth_test, depth_mask=depth_mask, blend=blend))
+
+
+def edge_x_at(p, q, y):
+    """Exact x of segment p-q at height y (None if horizontal)."""
+    (x0, y0), (x1, y1) = p, q
+    if y0 == y1:
+        return None
+    return Fr(x0) + Fr(x1 - x0) * Fr(y - y0, y1 - y0)
+
+
+def exact_coverage(tri, W, H, margin=Fr(1, 1000)):
+    """Covered pixels of one snapped triangle under the span rule, as a set, and
+    the set of 'ambiguous' pixels whose row end lies within `margin` of an
+    integer (where float rounding of the edge position may move the end)."""
+    xs = [p[0] for p in tri]
+    ys = [p[1] for p in tri]
+    if max(xs) - min(xs) < 1 or max(ys) - min(ys) < 1:
+        return set(), set()
+    covered, amb = set(), set()
+    for y in range(min(ys), max(ys)):
+        hits = []
+        for a, b in ((0, 1), (1, 2), (2, 0)):
+            p, q = tri[a], tri[b]
+            lo, hi = sorted((p[1], q[1]))
+            if lo <= y <= hi:
+                x = edge_x_at(p, q, y)
+                if x is not None:
+                    hits.append(x)
+        if len(hits) < 2:
+            continue
+        xa, xb = min(hits), max(hits)
+        ends = (math.trunc(xa), math.trunc(xb))
+        x0, x1 = min(ends), max(ends)
+        near = any(abs(v - round(v)) < margin for v in (xa, xb))
+        for x in range(max(x0, 0), min(x1, W)):
+            if 0 <= y < H:
+                covered.add((x, y))
+        if near:
+            for x in (x0 - 1, x0, x1 - 1, x1):
+                amb.add((x, y))
+    return covered, amb
+
+
+def exact_barycentric(tri, x, y):
+    """Exact screen-space barycentrics of pixel corner (x, y)."""
+    (x0, y0), (x1, y1), (x2, y2) = tri
+    den = Fr((x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0))
+    b1 = Fr((x - x0) * (y2 - y0) - (x2 - x0) * (y - y0)) / den
+    b2 = Fr((x1 - x0) * (y - y0) - (x - x0) * (y1 - y0)) / den
+    return 1 - b1 - b2, b1, b2
+
+
+def render(ctx, scene):
+    ctx.reset_stats()
+    scene.render()
+    out = scene.read()
+    out["stats"] = ctx.stats()
+    return out
