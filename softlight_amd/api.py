@@ -73,7 +73,8 @@ class sl_frame_stats(C.Structure):
                 ("tile_pairs", C.c_uint64), ("fragments_tested", C.c_uint64),
                 ("fragments_shaded", C.c_uint64), ("frames", C.c_uint64),
                 ("ms_vertex", C.c_double), ("ms_bin", C.c_double), ("ms_raster", C.c_double),
-                ("ms_total", C.c_double)]
+                ("ms_total", C.c_double), ("ms_setup_kernel", C.c_double), ("ms_clip_kernel", C.c_double),
+                ("setup_launches", C.c_uint64), ("raster_launches", C.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -51,6 +51,13 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    # the C++ facade example: a host program written against
+    # include/softlight_amd/SL_Context.hpp, linked to the library
+    ex = os.path.join(OUT, "facade_draw")
+    _run(["g++", "-std=c++17", "-O1", "-I", os.path.join(HERE, "..", "include"), "-o", ex + ".tmp",
+          os.path.join(HERE, "examples", "facade_draw.cpp"), "-L", OUT, "-lsoftlight_amd",
+          f"-Wl,-rpath,$ORIGIN"])
+    os.replace(ex + ".tmp", ex)
     return LIB
 
 

@@ -23,6 +23,7 @@
 // SL_FragmentProcessor::flush_tri_fragments (SL_FragmentProcessor.cpp:416-531).
 #include <hipcub/hipcub.hpp>
 
+#include "sl_recurrence.h"
 #include "sl_stages.h"
 #include "sl_kernels.h"
 
@@ -492,12 +493,10 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(RasterParams P)
                     float s1 = (fx * R.bx[1]) + fmaf(R.by[1], yf, R.bz[1]);
                     float s2 = (fx * R.bx[2]) + fmaf(R.by[2], yf, R.bz[2]);
                     const float d0 = R.bx[0] * 4.f, d1 = R.bx[1] * 4.f, d2 = R.bx[2] * 4.f;
-                    for (int gi = 0; gi < g0; ++gi)
-                    {
-                        s0 = s0 + d0;
-                        s1 = s1 + d1;
-                        s2 = s2 + d2;
-                    }
+                    // advance to the tile's first group: exact k-step jump
+                    s0 = sl_recur_advance(s0, d0, g0);
+                    s1 = sl_recur_advance(s1, d1, g0);
+                    s2 = sl_recur_advance(s2, d2, g0);
                     for (; x < xe; x += 4)
                     {
                         const float z = (((R.z[0] * s0) + (R.z[1] * s1)) + (R.z[2] * s2)) + R.bc3;

@@ -259,6 +259,7 @@ int execute_batch(sl_context* c)
             (void)hipEventRecord(c->ev[0], s);
             if ((rc = check(c, launch_setup(sp, chunks, s), "k_setup")) != SL_OK) return rc;
             (void)hipEventRecord(c->ev[1], s);
+            c->stats.setup_launches += 1;
             if ((rc = check(c, hipMemcpyAsync(c->hCtrl, ctrl, sizeof(FrameCtrl), hipMemcpyDeviceToHost, s), "read ctrl")) != SL_OK) return rc;
             if ((rc = check(c, hipStreamSynchronize(s), "k_setup sync")) != SL_OK) return rc;
             if (c->hCtrl->error) return fail(c, SL_ERR_DEVICE, "setup look-back did not complete");
@@ -275,6 +276,7 @@ int execute_batch(sl_context* c)
         float t = 0.f;
         (void)hipEventElapsedTime(&t, c->ev[0], c->ev[1]);
         msVertex += t;
+        c->stats.ms_setup_kernel += t;
         totalRecs = c->hCtrl->totalRecs;
         totalPairs = c->hCtrl->totalPairs;
         const uint32_t jobs = c->hCtrl->clipJobs;
@@ -341,6 +343,7 @@ int execute_batch(sl_context* c)
     {
         (void)hipEventElapsedTime(&t, c->ev[2], c->ev[3]);
         msVertex += t;
+        c->stats.ms_clip_kernel += t;
         (void)hipEventElapsedTime(&t, c->ev[3], c->ev[4]);
         msBin += t;
     }
@@ -351,6 +354,7 @@ int execute_batch(sl_context* c)
     c->stats.fragments_tested += c->hCtrl->tested;
     c->stats.fragments_shaded += c->hCtrl->shaded;
     c->stats.frames += 1;
+    c->stats.raster_launches += 1;
     c->stats.ms_vertex += msVertex;
     c->stats.ms_bin += msBin;
     c->stats.ms_raster += msRaster;
