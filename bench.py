@@ -34,6 +34,27 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 WORKLOAD = dict(seed=1, n_instances=64, stacks=256, sectors=256, W=1920, H=1080, tex_size=1024, radius=2.0,
                 extent=20.0)
+RANDOM_WORKLOAD = dict(seed=0, ntri=4096, W=640, H=480)
+CALIBRATION_BYTES = 256 << 20
+
+
+def calibration_copy(repeats=3):
+    """A plain device-to-device copy of CALIBRATION_BYTES (the copy kernel the
+    HBM byte counters are scaled against)."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    n = ctypes.c_size_t(CALIBRATION_BYTES)
+    assert hip.hipMalloc(ctypes.byref(a), n) == 0 and hip.hipMalloc(ctypes.byref(b), n) == 0
+    hip.hipMemset(a, 1, n)
+    for _ in range(repeats):
+        assert hip.hipMemcpy(b, a, n, 3) == 0          # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
 
 
 def algorithmic_bytes_per_triangle(info) -> float:
@@ -42,17 +63,20 @@ def algorithmic_bytes_per_triangle(info) -> float:
     return 12.0 + 32.0 * vpi / tpi
 
 
-def frag_bytes(tested, shaded) -> float:
-    # depth read per tested sample; depth write + RGBA8 write + 4 RGBA8 texels per shaded sample
-    return 4.0 * tested + (4.0 + 4.0 + 16.0) * shaded
+def frag_bytes(tested, shaded, texel_bytes=16.0) -> float:
+    # depth read per tested sample; depth write + RGBA8 write (+ 4 RGBA8 texels) per shaded sample
+    return 4.0 * tested + (4.0 + 4.0 + texel_bytes) * shaded
 
 
-def run_gpu(steps, warmup):
+def run_gpu(steps, warmup, workload="spheres"):
     import softlight_amd
     from softlight_amd import api, scenes
 
     ctx = api.SLContext(softlight_amd.load_library())
-    scene = scenes.build_sphere_scene(ctx, **WORKLOAD)
+    if workload == "spheres":
+        scene = scenes.build_sphere_scene(ctx, **WORKLOAD)
+    else:
+        scene = scenes.build_random_triangles(ctx, **RANDOM_WORKLOAD)
     for _ in range(warmup):
         scene.render()
         ctx.finish()
@@ -76,6 +100,7 @@ def run_cpu_baseline(frames=3):
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         pass
+    cores = min(cores, 16)               # the host-core share of one GPU
     ctx = api.SLContext(oracle.load())
     ctx.num_threads(cores)
     scene = scenes.build_sphere_scene(ctx, **WORKLOAD)
@@ -103,13 +128,18 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--workload", choices=["spheres", "random"], default="spheres",
+                    help="spheres: the 1080p textured-sphere frame (default); random: 4096 random triangles at 640x480")
+    ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (a single-GPU box)",
               file=sys.stderr)
         sys.exit(2)
 
-    scene, st, elapsed = run_gpu(args.steps, args.warmup)
+    if args.calibrate:
+        calibration_copy()
+    scene, st, elapsed = run_gpu(args.steps, args.warmup, args.workload)
     tris = scene.n_triangles
     frames = max(st["frames"], 1)
     ms_step = elapsed * 1e3 / args.steps
@@ -122,9 +152,11 @@ def main():
     shaded = st["fragments_shaded"] / frames
     tested = st["fragments_tested"] / frames
 
-    bpt = algorithmic_bytes_per_triangle(scene.info)
+    bpt = algorithmic_bytes_per_triangle(scene.info) if args.workload == "spheres" else 48.0
     achieved = tris * bpt / (ms_setup * 1e-3) / 1e9        # GB/s
-    frag_gbs = frag_bytes(tested, shaded) / (ms_raster * 1e-3) / 1e9
+    texel_bytes = 16.0 if args.workload == "spheres" else 0.0
+    fbytes = frag_bytes(tested, shaded, texel_bytes)
+    frag_gbs = fbytes / (ms_raster * 1e-3) / 1e9
 
     out = {
         "metric": "Mtri/s + shaded Gfrag/s at 1080p; fwd+bwd views/s vs plain fwd, 1 MI355X",
@@ -140,10 +172,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded UV spheres, value-noise texture)",
         "config": {
-            "workload": "textured UV-sphere scene: 64 instances x 130,560 indexed triangles, 1920x1080 RGBA8+R32F, "
-                        "1024^2 RGBA8 bilinear REPEAT, Lambert, depth LESS, back-face cull",
+            "workload": ("textured UV-sphere scene: 64 instances x 130,560 indexed triangles, 1920x1080 RGBA8+R32F, "
+                         "1024^2 RGBA8 bilinear REPEAT, Lambert, depth LESS, back-face cull")
+            if args.workload == "spheres" else
+            "4096 random non-indexed triangles, 640x480 RGBA8+R32F, Gouraud, depth LESS, no cull",
             "triangles_per_frame": tris,
-            "W": 1920, "H": 1080,
+            "W": scene.W, "H": scene.H,
         },
         "mtri_per_s_stage": round(tris / ((ms_vertex + ms_bin) * 1e-3) / 1e6, 3),
         "shaded_gfrag_per_s": round(shaded / (ms_raster * 1e-3) / 1e9, 4),
@@ -152,8 +186,9 @@ def main():
         "triangles_binned_per_frame": int(st["triangles_binned"] / frames),
         "stage_ms": {"vertex_setup": round(ms_vertex, 4), "binning": round(ms_bin, 4), "raster": round(ms_raster, 4),
                      "setup_kernel": round(ms_setup, 4)},
-        "fragment_roofline": {"bound": "hbm", "achieved": round(frag_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": round(frag_gbs / HBM_PEAK_GBS, 5)},
+        "fragment_roofline": {"kernel": "k_raster", "bound": "hbm", "achieved": round(frag_gbs, 2), "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": round(frag_gbs / HBM_PEAK_GBS, 5),
+                              "bytes_per_frame": fbytes},
         "roofline": {
             "kernel": "k_setup",
             "bound": "hbm",
@@ -165,7 +200,7 @@ def main():
             "bytes_per_triangle": round(bpt, 3),
         },
     }
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "spheres":
         out["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(out), flush=True)
 

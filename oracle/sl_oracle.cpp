@@ -912,58 +912,111 @@ int bind_mesh(sl_context* c, const sl_mesh& m, Draw& d)
 }
 
 /*-----------------------------------------------------------------------------
- * Multithreaded timing mode: the SL_ProcessorPool scheme (strided vertex work,
- * row-interleaved rasterisation, a barrier per batch of bins,
- * SL_ProcessorPool.cpp:280-327, SL_VertexProcessor.cpp:84-181).  Bins are
- * merged in submission order before rasterisation, so the output equals the
- * single-threaded one.
+ * Multithreaded timing mode: the SL_ProcessorPool scheme (a persistent pool,
+ * strided vertex work, row-interleaved rasterisation, barriers between the
+ * phases of each batch, SL_ProcessorPool.cpp:280-327,
+ * SL_VertexProcessor.cpp:84-181).  Bins are merged in submission order before
+ * rasterisation, so the output equals the single-threaded one.
 -----------------------------------------------------------------------------*/
-struct TriRef { size_t v0, v1, v2, inst; uint32_t seq; };
+// One (mesh, instance) pass of a draw call, as the reference loops them in
+// SL_TriProcessor::execute (SL_TriProcessor.cpp:677-713).
+struct Pass { const sl_mesh* mesh; size_t instance; size_t triBase; size_t numTris; uint32_t seqBase; };
 
-void run_draw_threads(sl_context* c, Draw& proto, const std::vector<TriRef>& tris)
+size_t index_at(const Ibo& ibo, size_t i);
+int bind_mesh(sl_context* c, const sl_mesh& m, Draw& d);
+
+class SpinBarrier
+{
+  public:
+    explicit SpinBarrier(unsigned n) : mN(n) {}
+    void wait()
+    {
+        const unsigned gen = mGen.load(std::memory_order_acquire);
+        if (mCount.fetch_add(1, std::memory_order_acq_rel) + 1 == mN)
+        {
+            mCount.store(0, std::memory_order_relaxed);
+            mGen.fetch_add(1, std::memory_order_acq_rel);
+        }
+        else
+        {
+            while (mGen.load(std::memory_order_acquire) == gen) std::this_thread::yield();
+        }
+    }
+  private:
+    const unsigned mN;
+    std::atomic<unsigned> mCount{0};
+    std::atomic<unsigned> mGen{0};
+};
+
+void run_draw_threads(sl_context* c, Draw& proto, const std::vector<Pass>& passes, size_t totalTris)
 {
     const unsigned N = std::max(1u, c->numThreads);
-    const size_t window = 1024 * (size_t)N;
+    const size_t window = 16384 * (size_t)N;
     std::vector<std::vector<Bin>> perThread(N);
-    std::vector<Bin> merged;
+    std::vector<const Bin*> merged;
     std::vector<uint64_t> tested(N, 0), shaded(N, 0), binned(N, 0);
-    for (size_t w0 = 0; w0 < tris.size(); w0 += window)
-    {
-        const size_t w1 = std::min(tris.size(), w0 + window);
-        auto vwork = [&](unsigned t) {
-            Draw d = proto;
-            VertexStage vs(d);
-            perThread[t].clear();
-            vs.out = &perThread[t];
-            for (size_t i = w0 + t; i < w1; i += N) vs.process_triangle(tris[i].seq, tris[i].v0, tris[i].v1, tris[i].v2, tris[i].inst);
-            binned[t] += d.binned;
-        };
-        std::vector<std::thread> th;
-        for (unsigned t = 1; t < N; ++t) th.emplace_back(vwork, t);
-        vwork(0);
-        for (auto& x : th) x.join();
-        th.clear();
+    SpinBarrier bar(N);
 
-        // merge: bins of one triangle are contiguous in its thread's list and
-        // carry its sequence number; walk triangles in order.
-        merged.clear();
-        std::vector<size_t> pos(N, 0);
-        for (size_t i = w0; i < w1; ++i)
+    // triangle g of the draw -> its pass (passes are in order, triBase ascending)
+    auto pass_of = [&](size_t g, size_t hint) {
+        while (hint + 1 < passes.size() && passes[hint + 1].triBase <= g) ++hint;
+        return hint;
+    };
+
+    auto worker = [&](unsigned t) {
+        Draw d = proto;
+        size_t hint = 0;
+        for (size_t w0 = 0; w0 < totalTris; w0 += window)
         {
-            const unsigned t = (unsigned)((i - w0) % N);
-            std::vector<Bin>& L = perThread[t];
-            while (pos[t] < L.size() && L[pos[t]].seq == tris[i].seq) merged.push_back(L[pos[t]++]);
+            const size_t w1 = std::min(totalTris, w0 + window);
+            {
+                perThread[t].clear();
+                for (size_t g = w0 + t; g < w1; g += N)
+                {
+                    hint = pass_of(g, hint);
+                    const Pass& P = passes[hint];
+                    bind_mesh(c, *P.mesh, d);
+                    VertexStage vs(d);
+                    vs.out = &perThread[t];
+                    const size_t i = P.mesh->elementBegin + 3 * (g - P.triBase);
+                    size_t a = i, b = i + 1, e = i + 2;
+                    if (P.mesh->mode == RENDER_MODE_INDEXED_TRIANGLES)
+                    {
+                        a = index_at(*d.ibo, i); b = index_at(*d.ibo, i + 1); e = index_at(*d.ibo, i + 2);
+                    }
+                    vs.process_triangle(P.seqBase + (uint32_t)(g - P.triBase) + 1u, a, b, e, P.instance);
+                }
+            }
+            bar.wait();
+            if (t == 0)
+            {
+                // merge: bins of one triangle are contiguous in its thread's
+                // list and carry its sequence number; walk triangles in order.
+                merged.clear();
+                std::vector<size_t> pos(N, 0);
+                size_t mh = 0;
+                for (size_t g = w0; g < w1; ++g)
+                {
+                    const unsigned o = (unsigned)((g - w0) % N);
+                    const std::vector<Bin>& L = perThread[o];
+                    if (pos[o] >= L.size()) continue;
+                    mh = pass_of(g, mh);
+                    const uint32_t sq = passes[mh].seqBase + (uint32_t)(g - passes[mh].triBase) + 1u;
+                    while (pos[o] < L.size() && L[pos[o]].seq == sq) merged.push_back(&L[pos[o]++]);
+                }
+            }
+            bar.wait();
+            for (const Bin* b : merged) raster_bin(d, *b, (int32_t)t, (int32_t)N);
+            bar.wait();
         }
-        auto rwork = [&](unsigned t) {
-            Draw d = proto;
-            for (const Bin& b : merged) raster_bin(d, b, (int32_t)t, (int32_t)N);
-            tested[t] += d.tested;
-            shaded[t] += d.shaded;
-        };
-        for (unsigned t = 1; t < N; ++t) th.emplace_back(rwork, t);
-        rwork(0);
-        for (auto& x : th) x.join();
-    }
+        tested[t] = d.tested;
+        shaded[t] = d.shaded;
+        binned[t] = d.binned;
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < N; ++t) th.emplace_back(worker, t);
+    worker(0);
+    for (auto& x : th) x.join();
     for (unsigned t = 0; t < N; ++t) { proto.tested += tested[t]; proto.shaded += shaded[t]; proto.binned += binned[t]; }
 }
 
@@ -973,37 +1026,43 @@ int run_draws(sl_context* c, const sl_mesh* meshes, size_t numMeshes, size_t num
     int rc = setup_draw(c, shaderId, fboId, proto);
     if (rc != SL_OK) return rc;
     auto t0 = std::chrono::steady_clock::now();
-    const bool threaded = c->numThreads > 1;
-    std::vector<TriRef> tris;
-
     const size_t nIter = (numInstances == 1) ? numMeshes : numInstances;
+    std::vector<Pass> passes;
+    size_t total = 0;
     for (size_t it = 0; it < nIter; ++it)
     {
         const sl_mesh& m = (numInstances == 1) ? meshes[it] : meshes[0];
-        const size_t instanceId = (numInstances == 1) ? 0 : it;
         rc = bind_mesh(c, m, proto);
         if (rc != SL_OK) return rc;
-        const bool indexed = (m.mode == RENDER_MODE_INDEXED_TRIANGLES);
-        if (threaded) tris.clear();
-        Draw d = proto;
-        VertexStage vs(d);
-        for (size_t i = m.elementBegin; i + 3 <= m.elementEnd; i += 3)   // (:569-574)
-        {
-            const uint32_t seq = ++c->triSeq;
-            ++c->stats.triangles_submitted;
-            size_t a = i, b = i + 1, e = i + 2;
-            if (indexed) { a = index_at(*d.ibo, i); b = index_at(*d.ibo, i + 1); e = index_at(*d.ibo, i + 2); }
-            if (threaded) tris.push_back(TriRef{a, b, e, instanceId, seq});
-            else vs.process_triangle(seq, a, b, e, instanceId);
-        }
-        if (threaded)
-        {
-            run_draw_threads(c, d, tris);
-        }
-        c->stats.fragments_tested += d.tested;
-        c->stats.fragments_shaded += d.shaded;
-        c->stats.triangles_binned += d.binned;
+        const size_t n = (m.elementEnd > m.elementBegin) ? (m.elementEnd - m.elementBegin) / 3 : 0;   // (:569-574)
+        passes.push_back(Pass{&m, (numInstances == 1) ? 0 : it, total, n, c->triSeq});
+        c->triSeq += (uint32_t)n;
+        c->stats.triangles_submitted += n;
+        total += n;
     }
+    if (c->numThreads > 1)
+    {
+        run_draw_threads(c, proto, passes, total);
+    }
+    else
+    {
+        for (const Pass& P : passes)
+        {
+            bind_mesh(c, *P.mesh, proto);
+            VertexStage vs(proto);
+            const bool indexed = (P.mesh->mode == RENDER_MODE_INDEXED_TRIANGLES);
+            for (size_t t = 0; t < P.numTris; ++t)
+            {
+                const size_t i = P.mesh->elementBegin + 3 * t;
+                size_t a = i, b = i + 1, e = i + 2;
+                if (indexed) { a = index_at(*proto.ibo, i); b = index_at(*proto.ibo, i + 1); e = index_at(*proto.ibo, i + 2); }
+                vs.process_triangle(P.seqBase + (uint32_t)t + 1u, a, b, e, P.instance);
+            }
+        }
+    }
+    c->stats.fragments_tested += proto.tested;
+    c->stats.fragments_shaded += proto.shaded;
+    c->stats.triangles_binned += proto.binned;
     auto t1 = std::chrono::steady_clock::now();
     c->stats.ms_total += std::chrono::duration<double, std::milli>(t1 - t0).count();
     c->stats.frames += 1;
