@@ -68,15 +68,31 @@ def frag_bytes(tested, shaded, texel_bytes=16.0) -> float:
     return 4.0 * tested + (4.0 + 4.0 + texel_bytes) * shaded
 
 
-def run_gpu(steps, warmup, workload="spheres"):
+def run_gpu(steps, warmup, workload="spheres", raster_config=None, tune=False):
     import softlight_amd
-    from softlight_amd import api, scenes
+    from softlight_amd import api, autotune, scenes
 
     ctx = api.SLContext(softlight_amd.load_library())
     if workload == "spheres":
         scene = scenes.build_sphere_scene(ctx, **WORKLOAD)
     else:
         scene = scenes.build_random_triangles(ctx, **RANDOM_WORKLOAD)
+    signature = f"{workload}:{scene.W}x{scene.H}:{scene.n_triangles}"
+
+    def frame():
+        scene.render()
+        ctx.finish()
+
+    if tune:
+        res = autotune.tune(ctx, frame, signature, path=os.environ.get("SL_TUNE_CACHE", autotune.DEFAULT_CACHE))
+        print(json.dumps({"autotune": signature, **res}), file=sys.stderr, flush=True)
+    elif raster_config is not None:
+        ctx.set_raster_config(raster_config)
+    else:
+        cached = autotune.lookup(signature, os.environ.get("SL_TUNE_CACHE", autotune.DEFAULT_CACHE))
+        if cached is not None:
+            ctx.set_raster_config(cached)
+    scene.info["raster_config"] = ctx.raster_config()
     for _ in range(warmup):
         scene.render()
         ctx.finish()
@@ -131,6 +147,8 @@ def main():
     ap.add_argument("--workload", choices=["spheres", "random"], default="spheres",
                     help="spheres: the 1080p textured-sphere frame (default); random: 4096 random triangles at 640x480")
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter calibration)")
+    ap.add_argument("--raster-config", type=int, default=None, help="raster tile configuration 0..7 (default: tuned)")
+    ap.add_argument("--tune", action="store_true", help="time all raster configurations first and cache the best")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (a single-GPU box)",
@@ -139,7 +157,7 @@ def main():
 
     if args.calibrate:
         calibration_copy()
-    scene, st, elapsed = run_gpu(args.steps, args.warmup, args.workload)
+    scene, st, elapsed = run_gpu(args.steps, args.warmup, args.workload, args.raster_config, args.tune)
     tris = scene.n_triangles
     frames = max(st["frames"], 1)
     ms_step = elapsed * 1e3 / args.steps
@@ -178,6 +196,8 @@ def main():
             "4096 random non-indexed triangles, 640x480 RGBA8+R32F, Gouraud, depth LESS, no cull",
             "triangles_per_frame": tris,
             "W": scene.W, "H": scene.H,
+            "raster_tile": {"config": scene.info["raster_config"][0], "tile_w": scene.info["raster_config"][1][0],
+                            "tile_h": scene.info["raster_config"][1][1], "segments": scene.info["raster_config"][1][2]},
         },
         "mtri_per_s_stage": round(tris / ((ms_vertex + ms_bin) * 1e-3) / 1e6, 3),
         "shaded_gfrag_per_s": round(shaded / (ms_raster * 1e-3) / 1e9, 4),

@@ -200,6 +200,15 @@ unsigned    sl_set_num_threads(sl_context* ctx, unsigned n);   /* SL_Context::nu
 int         sl_set_viewport(sl_context* ctx, int32_t x, int32_t y, int32_t w, int32_t h); /* SL_ViewportState::viewport, SL_ViewportState.hpp:150 */
 int         sl_set_scissor(sl_context* ctx, int32_t x, int32_t y, int32_t w, int32_t h);  /* SL_ViewportState::scissor, SL_ViewportState.hpp:175 */
 
+/* Raster tile configuration (no counterpart in the reference, whose raster
+ * splits scanlines across CPU threads, SL_ScanlineBounds.hpp:31-50).
+ * Configurations 0..7 select the screen tile each GPU workgroup owns and how
+ * many column segments a tile row is split into; see sl_get_raster_config.
+ * sl_get_raster_config(ctx, -1, ...) reports the current one and returns its
+ * index; results do not depend on the configuration. */
+int         sl_set_raster_config(sl_context* ctx, int config);
+int         sl_get_raster_config(const sl_context* ctx, int config, int* tileW, int* tileH, int* segments);
+
 /* ---- vertex buffers (SL_VertexBuffer.hpp / .cpp) ------------------------ */
 size_t sl_create_vbo(sl_context* ctx);                                                      /* SL_Context::create_vbo */
 int    sl_vbo_init(sl_context* ctx, size_t vbo, size_t numBytes, const void* data);         /* SL_VertexBuffer::init, SL_VertexBuffer.cpp:96 */

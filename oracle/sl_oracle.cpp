@@ -1139,6 +1139,22 @@ unsigned slo_set_num_threads(sl_context* c, unsigned n)
     return c->numThreads;
 }
 
+// The oracle has no tiles: the raster configuration is accepted and ignored.
+int slo_set_raster_config(sl_context* c, int config)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    return (config >= 0 && config < 8) ? SL_OK : SL_ERR_INVALID_ARG;
+}
+
+int slo_get_raster_config(const sl_context* c, int config, int* tileW, int* tileH, int* segments)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    if (tileW) *tileW = 0;
+    if (tileH) *tileH = 0;
+    if (segments) *segments = 0;
+    return config < 0 ? 0 : config;
+}
+
 int slo_set_viewport(sl_context* c, int32_t x, int32_t y, int32_t w, int32_t h)
 {
     if (!c) return SL_ERR_INVALID_ARG;

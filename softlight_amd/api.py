@@ -90,6 +90,8 @@ _SIGS = {
     "last_error": (C.c_char_p, [_P]),
     "num_threads": (_U, [_P]),
     "set_num_threads": (_U, [_P, _U]),
+    "set_raster_config": (_I, [_P, _I]),
+    "get_raster_config": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "set_viewport": (_I, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "set_scissor": (_I, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "create_vbo": (_SZ, [_P]),
@@ -194,6 +196,14 @@ class SLContext:
         if n is None:
             return self.lib.num_threads(self.h)
         return self.lib.set_num_threads(self.h, n)
+
+    def set_raster_config(self, config):
+        return self._check(self.lib.set_raster_config(self.h, config), "set_raster_config")
+
+    def raster_config(self, config=-1):
+        w, h, s = _I(), _I(), _I()
+        idx = self.lib.get_raster_config(self.h, config, C.byref(w), C.byref(h), C.byref(s))
+        return idx, (w.value, h.value, s.value)
 
     def set_viewport(self, x, y, w, h):
         return self._check(self.lib.set_viewport(self.h, x, y, w, h), "set_viewport")

@@ -1,0 +1,69 @@
+"""Raster tile-configuration autotuner.
+
+Times each of the library's raster tile configurations (sl_get_raster_config
+0..7: tile size and column segments per row) on a caller-supplied frame and
+keeps the fastest.  Results are cached in a JSON file keyed by the device
+name and a workload signature, so later runs reuse them without timing.
+The choice never changes results, only speed (the parity tests run every
+configuration).
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+
+DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_configs.json")
+NUM_CONFIGS = 8
+
+
+def device_name() -> str:
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        try:
+            hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+        except OSError:
+            return "unknown"
+    buf = ctypes.create_string_buffer(256)
+    if hip.hipDeviceGetName(buf, 256, 0) != 0:
+        return "unknown"
+    return buf.value.decode(errors="replace") or "unknown"
+
+
+def load_cache(path: str = DEFAULT_CACHE) -> dict:
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def lookup(signature: str, path: str = DEFAULT_CACHE):
+    entry = load_cache(path).get(device_name(), {}).get(signature)
+    return None if entry is None else int(entry["config"])
+
+
+def tune(ctx, render_frame, signature: str, frames: int = 3, path: str | None = DEFAULT_CACHE) -> dict:
+    """Time render_frame() under each raster configuration; return the
+    per-configuration raster milliseconds and the best index (also cached)."""
+    timings = {}
+    for cfg in range(NUM_CONFIGS):
+        ctx.set_raster_config(cfg)
+        render_frame()                     # warm-up (buffer growth, code load)
+        ctx.reset_stats()
+        for _ in range(frames):
+            render_frame()
+        st = ctx.stats()
+        timings[cfg] = {"raster_ms": st["ms_raster"] / max(st["frames"], 1),
+                        "frame_ms": st["ms_total"] / max(st["frames"], 1),
+                        "tile": ctx.raster_config(cfg)[1]}
+    best = min(timings, key=lambda k: timings[k]["frame_ms"])
+    ctx.set_raster_config(best)
+    result = {"config": best, "tile": timings[best]["tile"], "timings": timings}
+    if path:
+        cache = load_cache(path)
+        cache.setdefault(device_name(), {})[signature] = {"config": best, "tile": timings[best]["tile"],
+                                                          "frame_ms": timings[best]["frame_ms"]}
+        with open(path, "w") as f:
+            json.dump(cache, f, indent=1, sort_keys=True)
+    return result

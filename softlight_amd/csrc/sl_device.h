@@ -21,10 +21,25 @@ namespace slg {
 constexpr int kSetupBlock = 256;       // triangles per setup workgroup (one per lane)
 constexpr int kMaxFan = 7;             // a 9-gon from 6-plane clipping fans into 7 triangles
 
-// Tile raster: one wave per tile.  Lane = (row, phase): 16 rows x 4 x-phases.
-constexpr int kTileW = 32;
-constexpr int kTileH = 16;
-constexpr int kRasterThreads = 64;
+// Tile raster configurations.  A workgroup owns one tileW x tileH screen
+// tile; lane = (row, column segment, x-phase): each lane walks the pixels
+// x = xMin + phase + 4k of its row that fall in its segment of tileW/segs
+// columns (the reference steps its barycentrics in groups of 4 pixels).
+struct TileConfig { int tileW, tileH, segs; };
+constexpr int kNumTileConfigs = 8;
+constexpr TileConfig kTileConfigs[kNumTileConfigs] = {
+    {32, 16, 1},   // 0: 64 lanes
+    {32, 8, 2},    // 1: 64 lanes
+    {16, 16, 1},   // 2: 64 lanes
+    {16, 8, 2},    // 3: 64 lanes
+    {64, 16, 2},   // 4: 128 lanes
+    {32, 32, 1},   // 5: 128 lanes
+    {64, 8, 4},    // 6: 128 lanes
+    {64, 32, 2},   // 7: 256 lanes
+};
+constexpr int kDefaultTileConfig = 0;
+
+SL_HD int ilog2i(int v) { int r = 0; while ((1 << r) < v) ++r; return r; }
 
 // ---------------------------------------------------------------------------
 // Pinned scalar arithmetic (mirrors the reference's generic branches)
@@ -128,6 +143,7 @@ struct SetupParams
     const uint8_t* uniforms;
     int32_t W, H;
     int32_t tilesX, tilesY;
+    int32_t tileShiftX, tileShiftY;   // log2 of the tile size
     SetupRec* recs;
     float4* vars;              // 3 * maxVary float4 per record
     uint32_t varStride;        // float4 per record

@@ -81,10 +81,8 @@ struct TriIn
 
 // Fetch, vertex programs, cull, scissor matrix and clip status of one triangle
 // (SL_TriProcessor::process_verts, SL_TriProcessor.cpp:574-631).
-__device__ __forceinline__ void load_triangle(const SetupParams& P, uint64_t g, TriIn& T)
+__device__ __forceinline__ void load_triangle_in(const SetupParams& P, uint64_t g, const DrawDesc& D, uint32_t di, TriIn& T)
 {
-    const uint32_t di = find_draw(P.draws, P.numDraws, g);
-    const DrawDesc& D = P.draws[di];
     const uint64_t t = g - D.triBase;
     const uint64_t inst = t / D.trisPerInstance;
     const uint64_t ti = t - inst * D.trisPerInstance;
@@ -110,6 +108,18 @@ __device__ __forceinline__ void load_triangle(const SetupParams& P, uint64_t g, 
 #pragma unroll
     for (int k = 0; k < 3; ++k) T.pos[k] = mat_mul(D.scissor, T.pos[k]);
     T.status = clip_status(T.pos[0], T.pos[1], T.pos[2]);
+}
+
+// Most waves lie inside one draw: then the draw's descriptor is read through a
+// wave-uniform index (scalar loads); otherwise per lane.
+__device__ __forceinline__ void load_triangle(const SetupParams& P, uint64_t g, TriIn& T)
+{
+    const uint32_t di = find_draw(P.draws, P.numDraws, g);
+    const uint32_t di0 = __builtin_amdgcn_readfirstlane(di);
+    if (__all(di == di0))
+        load_triangle_in(P, g, P.draws[di0], di0, T);
+    else
+        load_triangle_in(P, g, P.draws[di], di, T);
 }
 
 __device__ __forceinline__ float4 clip_edge(int e)
@@ -176,7 +186,11 @@ __device__ int clip_polygon(const float4* inPos, const Vary4* inVar, int nv, flo
                 const float t = t0 * rcpf(t0 - t1);
                 tpos[nn] = mix4(q0, q1, t);
                 if (kVary)
-                    for (int i = 0; i < nv; ++i) tvar[nn].v[i] = mix4(outVar[j].v[i], outVar[k].v[i], t);
+                {
+#pragma unroll
+                    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+                        if (i < nv) tvar[nn].v[i] = mix4(outVar[j].v[i], outVar[k].v[i], t);
+                }
                 ++nn;
             }
             if (vis1)
@@ -207,11 +221,15 @@ __device__ __forceinline__ void write_record(const SetupParams& P, uint32_t recI
     P.recs[recId] = r;
     float4* V = P.vars + (uint64_t)recId * P.varStride;
     const uint32_t nvMax = P.varStride / 3;
-    for (int i = 0; i < nv; ++i)
+#pragma unroll
+    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)   // static indices keep the varyings in registers
     {
-        V[0 * nvMax + i] = a.v[i];
-        V[1 * nvMax + i] = b.v[i];
-        V[2 * nvMax + i] = c.v[i];
+        if (i < nv)
+        {
+            V[0 * nvMax + i] = a.v[i];
+            V[1 * nvMax + i] = b.v[i];
+            V[2 * nvMax + i] = c.v[i];
+        }
     }
     uint32_t o = pairOff;
     for (int ty = tr.ty0; ty <= tr.ty1; ++ty)
@@ -267,7 +285,7 @@ __global__ __launch_bounds__(kSetupBlock) void k_setup(SetupParams P)
             for (int k = 0; k < 3; ++k) sp[k] = divide_and_snap(T.pos[k], D.vp);
             if (bin_accept(sp[0], sp[1], sp[2]))
             {
-                tr = tile_rect(sp[0], sp[1], sp[2], P.W, P.H, P.tilesX, P.tilesY);
+                tr = tile_rect(sp[0], sp[1], sp[2], P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
                 nrec = 1;
                 npair = tile_count(tr);
             }
@@ -282,7 +300,7 @@ __global__ __launch_bounds__(kSetupBlock) void k_setup(SetupParams P)
             {
                 if (bin_accept(poly[0], poly[i + 1], poly[i + 2]))
                 {
-                    const TileRect r = tile_rect(poly[0], poly[i + 1], poly[i + 2], P.W, P.H, P.tilesX, P.tilesY);
+                    const TileRect r = tile_rect(poly[0], poly[i + 1], poly[i + 2], P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
                     ++nrec;
                     npair += tile_count(r);
                 }
@@ -305,45 +323,63 @@ __global__ __launch_bounds__(kSetupBlock) void k_setup(SetupParams P)
         blockTotal += sWave[w];
     }
 
-    // decoupled look-back (ticket order guarantees every predecessor runs)
-    if (threadIdx.x == 0)
+    // decoupled look-back by the first wave: each round reads the statuses of
+    // the 64 nearest unread predecessors at once (ticket order guarantees every
+    // predecessor is running, so every wait ends)
+    if (threadIdx.x < 64)
     {
+        const int lane = threadIdx.x;
         unsigned long long prefix = 0;
         if (chunk == 0)
         {
-            __hip_atomic_store(&P.status[0], kFlagInc | blockTotal, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&P.status[0], kFlagInc | blockTotal, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
         else
         {
-            __hip_atomic_store(&P.status[chunk], kFlagAgg | blockTotal, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            int64_t j = (int64_t)chunk - 1;
+            if (lane == 0)
+                __hip_atomic_store(&P.status[chunk], kFlagAgg | blockTotal, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j = (int64_t)chunk - 1;          // nearest predecessor not yet accumulated
             uint32_t spins = 0;
-            while (j >= 0)
+            while (true)
             {
-                const unsigned long long s = __hip_atomic_load(&P.status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned long long flag = s & ~kValMask;
-                if (flag == 0)
+                const int64_t idx = j - lane;
+                unsigned long long st = kFlagInc;    // before chunk 0: an inclusive zero
+                if (idx >= 0) st = __hip_atomic_load(&P.status[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long flag = st & ~kValMask;
+                const unsigned long long incMask = __ballot(flag == kFlagInc);
+                const unsigned long long zeroMask = __ballot(flag == 0);
+                const int firstInc = incMask ? (__ffsll((long long)incMask) - 1) : 64;
+                const unsigned long long below = (firstInc >= 64) ? ~0ull : ((2ull << firstInc) - 1ull);
+                if (zeroMask & below)
                 {
                     if (++spins > kSpinLimit)
                     {
-                        atomicExch(&P.ctrl->error, 1u);
+                        if (lane == 0) atomicExch(&P.ctrl->error, 1u);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                prefix += s & kValMask;
-                if (flag == kFlagInc) break;
-                --j;
+                unsigned long long v = (lane <= firstInc) ? (st & kValMask) : 0ull;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+                prefix += v;
+                if (firstInc < 64) break;
+                j -= 64;
             }
-            __hip_atomic_store(&P.status[chunk], kFlagInc | (prefix + blockTotal), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&P.status[chunk], kFlagInc | (prefix + blockTotal), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
-        sPrefix = prefix;
-        if (chunk == gridDim.x - 1)
+        if (lane == 0)
         {
-            const unsigned long long tot = prefix + blockTotal;
-            P.ctrl->totalRecs = (uint32_t)(tot & ((1ull << kRecBits) - 1));
-            P.ctrl->totalPairs = (uint32_t)(tot >> kRecBits);
+            sPrefix = prefix;
+            if (chunk == gridDim.x - 1)
+            {
+                const unsigned long long tot = prefix + blockTotal;
+                P.ctrl->totalRecs = (uint32_t)(tot & ((1ull << kRecBits) - 1));
+                P.ctrl->totalPairs = (uint32_t)(tot >> kRecBits);
+            }
         }
     }
     __syncthreads();
@@ -404,7 +440,7 @@ __global__ __launch_bounds__(64) void k_clip(SetupParams P)
         for (int i = n - 3; i >= 0 && rec < recEnd; --i)
         {
             if (!bin_accept(poly[0], poly[i + 1], poly[i + 2])) continue;
-            const TileRect r = tile_rect(poly[0], poly[i + 1], poly[i + 2], P.W, P.H, P.tilesX, P.tilesY);
+            const TileRect r = tile_rect(poly[0], poly[i + 1], poly[i + 2], P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
             write_record(P, rec, pair, poly[0], poly[i + 1], poly[i + 2], pvar[0], pvar[i + 1], pvar[i + 2], nv,
                          T.draw, T.seq, r);
             ++rec;
@@ -429,24 +465,45 @@ __global__ void k_ranges(const uint32_t* __restrict__ keys, uint32_t n, uint2* _
 // ---------------------------------------------------------------------------
 // k_raster: one wave per tile
 // ---------------------------------------------------------------------------
-template <int CF, bool DBG>
-__global__ __launch_bounds__(kRasterThreads) void k_raster(RasterParams P)
+// Orders one record's LDS updates before the next record's: a wave barrier
+// plus compiler fences when the tile is one wave (LDS operations of a wave
+// execute in order), a workgroup barrier otherwise.
+template <int NT>
+__device__ __forceinline__ void record_barrier()
 {
-    __shared__ float sDepth[kTileH][kTileW];
-    __shared__ uint32_t sCol8[CF == 1 ? kTileH : 1][kTileW];
-    __shared__ float4 sColF[CF == 2 ? kTileH : 1][kTileW];
-    __shared__ uint32_t sLast[DBG ? kTileH : 1][kTileW];
-    __shared__ uint32_t sCount[DBG ? kTileH : 1][kTileW];
+    if (NT == 64)
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    else
+    {
+        __syncthreads();
+    }
+}
+
+template <int CF, bool DBG, int TW, int TH, int SEGS>
+__global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
+{
+    constexpr int NT = TH * 4 * SEGS;       // lanes: rows x segments x 4 phases
+    constexpr int SW = TW / SEGS;           // segment width (columns)
+    static_assert(SW >= 4 && SW % 4 == 0, "segment width");
+    __shared__ float sDepth[TH][TW];
+    __shared__ uint32_t sCol8[CF == 1 ? TH : 1][TW];
+    __shared__ float4 sColF[CF == 2 ? TH : 1][TW];
+    __shared__ uint32_t sLast[DBG ? TH : 1][TW];
+    __shared__ uint32_t sCount[DBG ? TH : 1][TW];
 
     const int tile = blockIdx.x;
     const int tx = tile % P.tilesX, ty = tile / P.tilesX;
-    const int x0 = tx * kTileW, y0 = ty * kTileH;
+    const int x0 = tx * TW, y0 = ty * TH;
     const int lane = threadIdx.x;
 
     // tile init: fused clear or load
-    for (int k = lane; k < kTileW * kTileH; k += kRasterThreads)
+    for (int k = lane; k < TW * TH; k += NT)
     {
-        const int r = k / kTileW, c = k % kTileW;
+        const int r = k / TW, c = k % TW;
         const int gx = x0 + c, gy = y0 + r;
         const bool inb = gx < P.W && gy < P.H;
         const uint64_t gi = (uint64_t)gy * P.W + gx;
@@ -463,30 +520,51 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(RasterParams P)
     }
     __syncthreads();
 
-    const int row = lane >> 2, phase = lane & 3;
+    const int row = lane / (4 * SEGS);
+    const int phase = lane & 3;
+    const int seg = (lane >> 2) % SEGS;
+    const int sx0 = x0 + seg * SW;
     const int y = y0 + row;
     const float yf = (float)y;
     const uint2 range = P.tileRange[tile];
     const uint32_t nvMax = P.varStride / 3;
     uint32_t tested = 0, shaded = 0;
 
-    for (uint32_t it = range.x; it < range.y; ++it)
+    // Records are staged into LDS kBatch at a time by the whole workgroup, so
+    // the dependent global loads (entry -> record) are paid once per batch.
+    constexpr int kBatch = 16;
+    constexpr int kRecWords = sizeof(SetupRec) / 4;
+    __shared__ uint32_t sRid[kBatch];
+    __shared__ SetupRec sRec[kBatch];
+    for (uint32_t base = range.x; base < range.y; base += kBatch)
     {
-        const uint32_t rid = __builtin_amdgcn_readfirstlane(P.tileRecs[it]);
-        const SetupRec& R = P.recs[rid];
+        const int nb = (int)min((uint32_t)kBatch, range.y - base);
+        if (lane < nb) sRid[lane] = P.tileRecs[base + lane];
+        __syncthreads();
+        for (int q = lane; q < nb * kRecWords; q += NT)
+        {
+            const int r = q / kRecWords, w = q - r * kRecWords;
+            reinterpret_cast<uint32_t*>(sRec)[q] = reinterpret_cast<const uint32_t*>(P.recs + sRid[r])[w];
+        }
+        __syncthreads();
+    for (int bi = 0; bi < nb; ++bi)
+    {
+        const uint32_t rid = __builtin_amdgcn_readfirstlane(sRid[bi]);
+        const SetupRec& R = sRec[bi];
+        const uint32_t drawIdx = __builtin_amdgcn_readfirstlane(R.draw);
         if (y >= R.minY && y < R.maxY && y < P.H)
         {
             int32_t xMin, xMax;
             scanline_step(R, yf, xMin, xMax);
             if ((uint32_t)xMin < (uint32_t)xMax)
             {
-                const int xs = max(xMin, x0), xe = min(min(xMax, x0 + kTileW), P.W);
+                const int xs = max(xMin, sx0), xe = min(min(xMax, sx0 + SW), P.W);
                 const int xf = xMin + phase;
                 const int g0 = (xs > xf) ? ((xs - xf + 3) >> 2) : 0;
                 int x = xf + 4 * g0;
                 if (x < xe)
                 {
-                    const DrawDesc& D = P.draws[R.draw];
+                    const DrawDesc& D = P.draws[drawIdx];
                     // bc(x) = fl(fl(x*dx) + fma(dy, y, c)), then += 4*dx per group of 4 pixels
                     const float fx = (float)xf;
                     float s0 = (fx * R.bx[0]) + fmaf(R.by[0], yf, R.bz[0]);
@@ -512,6 +590,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(RasterParams P)
                                 const float c0 = t0 * sinv, c1 = t1 * sinv, c2 = t2 * sinv;
                                 const float4* V = P.vars + (uint64_t)rid * P.varStride;
                                 Vary4 var;
+#pragma unroll
                                 for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
                                 {
                                     if (i < D.nvary)
@@ -575,13 +654,16 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(RasterParams P)
                 }
             }
         }
+        record_barrier<NT>();
+    }
         __syncthreads();
     }
+    __syncthreads();
 
     // tile store
-    for (int k = lane; k < kTileW * kTileH; k += kRasterThreads)
+    for (int k = lane; k < TW * TH; k += NT)
     {
-        const int r = k / kTileW, c = k % kTileW;
+        const int r = k / TW, c = k % TW;
         const int gx = x0 + c, gy = y0 + r;
         if (gx < P.W && gy < P.H)
         {
@@ -605,7 +687,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(RasterParams P)
         t += __shfl_xor(t, off, 64);
         s += __shfl_xor(s, off, 64);
     }
-    if (lane == 0)
+    if ((lane & 63) == 0)
     {
         atomicAdd(&P.ctrl->tested, t);
         atomicAdd(&P.ctrl->shaded, s);
@@ -663,21 +745,37 @@ hipError_t sort_pairs(void* temp, size_t& tempBytes, const uint32_t* kin, uint32
     return hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, kin, kout, vin, vout, (int)n, 0, endBit, s);
 }
 
-hipError_t launch_raster(const RasterParams& p, int numTiles, bool debug, hipStream_t s)
+template <int CF, bool DBG>
+static void launch_raster_cfg(const RasterParams& p, int numTiles, int cfg, hipStream_t s)
+{
+    const dim3 g(numTiles);
+    switch (cfg)
+    {
+        case 1: hipLaunchKernelGGL((k_raster<CF, DBG, 32, 8, 2>), g, dim3(64), 0, s, p); break;
+        case 2: hipLaunchKernelGGL((k_raster<CF, DBG, 16, 16, 1>), g, dim3(64), 0, s, p); break;
+        case 3: hipLaunchKernelGGL((k_raster<CF, DBG, 16, 8, 2>), g, dim3(64), 0, s, p); break;
+        case 4: hipLaunchKernelGGL((k_raster<CF, DBG, 64, 16, 2>), g, dim3(128), 0, s, p); break;
+        case 5: hipLaunchKernelGGL((k_raster<CF, DBG, 32, 32, 1>), g, dim3(128), 0, s, p); break;
+        case 6: hipLaunchKernelGGL((k_raster<CF, DBG, 64, 8, 4>), g, dim3(128), 0, s, p); break;
+        case 7: hipLaunchKernelGGL((k_raster<CF, DBG, 64, 32, 2>), g, dim3(256), 0, s, p); break;
+        default: hipLaunchKernelGGL((k_raster<CF, DBG, 32, 16, 1>), g, dim3(64), 0, s, p); break;
+    }
+}
+
+hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s)
 {
     const int cf = (p.color == nullptr) ? 0 : (p.colorType == SL_COLOR_RGBA_FLOAT ? 2 : 1);
-    const dim3 g(numTiles), b(kRasterThreads);
     if (debug)
     {
-        if (cf == 0) hipLaunchKernelGGL((k_raster<0, true>), g, b, 0, s, p);
-        else if (cf == 1) hipLaunchKernelGGL((k_raster<1, true>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((k_raster<2, true>), g, b, 0, s, p);
+        if (cf == 0) launch_raster_cfg<0, true>(p, numTiles, cfg, s);
+        else if (cf == 1) launch_raster_cfg<1, true>(p, numTiles, cfg, s);
+        else launch_raster_cfg<2, true>(p, numTiles, cfg, s);
     }
     else
     {
-        if (cf == 0) hipLaunchKernelGGL((k_raster<0, false>), g, b, 0, s, p);
-        else if (cf == 1) hipLaunchKernelGGL((k_raster<1, false>), g, b, 0, s, p);
-        else hipLaunchKernelGGL((k_raster<2, false>), g, b, 0, s, p);
+        if (cf == 0) launch_raster_cfg<0, false>(p, numTiles, cfg, s);
+        else if (cf == 1) launch_raster_cfg<1, false>(p, numTiles, cfg, s);
+        else launch_raster_cfg<2, false>(p, numTiles, cfg, s);
     }
     return hipGetLastError();
 }

@@ -90,6 +90,7 @@ struct sl_context
     int32_t viewport[4] = {0, 0, 65535, 65535};   // SL_ViewportState.hpp:80-81
     int32_t scissor[4] = {0, 0, 65535, 65535};
     unsigned numThreads = 1;
+    int tileConfig = kDefaultTileConfig;
     uint32_t triSeq = 0;
     sl_frame_stats stats{};
     std::string err;
@@ -195,7 +196,9 @@ int execute_batch(sl_context* c)
     const bool debug = dbgL && dbgC;
 
     const int W = dt->w, H = dt->h;
-    const int tilesX = (W + kTileW - 1) / kTileW, tilesY = (H + kTileH - 1) / kTileH;
+    const int cfg = (c->tileConfig >= 0 && c->tileConfig < kNumTileConfigs) ? c->tileConfig : kDefaultTileConfig;
+    const TileConfig tc = kTileConfigs[cfg];
+    const int tilesX = (W + tc.tileW - 1) / tc.tileW, tilesY = (H + tc.tileH - 1) / tc.tileH;
     const int numTiles = tilesX * tilesY;
     hipStream_t s = c->stream;
     int rc;
@@ -242,6 +245,8 @@ int execute_batch(sl_context* c)
             sp.totalTris = B.totalTris;
             sp.uniforms = reinterpret_cast<const uint8_t*>(c->dUni.p);
             sp.W = W; sp.H = H; sp.tilesX = tilesX; sp.tilesY = tilesY;
+            sp.tileShiftX = ilog2i(tc.tileW);
+            sp.tileShiftY = ilog2i(tc.tileH);
             sp.recs = reinterpret_cast<SetupRec*>(c->dRecs.p);
             sp.vars = reinterpret_cast<float4*>(c->dVars.p);
             sp.varStride = varStride;
@@ -333,7 +338,7 @@ int execute_batch(sl_context* c)
     rp.clearRGBA8 = packed;
     rp.clearDepthValue = (float)B.depth;
     rp.ctrl = ctrl;
-    if ((rc = check(c, launch_raster(rp, numTiles, debug, s), "k_raster")) != SL_OK) return rc;
+    if ((rc = check(c, launch_raster(rp, numTiles, cfg, debug, s), "k_raster")) != SL_OK) return rc;
     (void)hipEventRecord(c->ev[5], s);
     if ((rc = check(c, hipMemcpyAsync(c->hCtrl, ctrl, sizeof(FrameCtrl), hipMemcpyDeviceToHost, s), "read ctrl")) != SL_OK) return rc;
     if ((rc = check(c, hipStreamSynchronize(s), "frame sync")) != SL_OK) return rc;
@@ -593,6 +598,26 @@ unsigned sl_set_num_threads(sl_context* c, unsigned n)
     if (!c) return 0;
     c->numThreads = std::max(1u, n);   // the GPU path has no host worker pool; kept for API parity
     return c->numThreads;
+}
+
+int sl_set_raster_config(sl_context* c, int config)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    if (config < 0 || config >= kNumTileConfigs) return SL_ERR_INVALID_ARG;
+    const int rc = flush(c);
+    c->tileConfig = config;
+    return rc;
+}
+
+int sl_get_raster_config(const sl_context* c, int config, int* tileW, int* tileH, int* segments)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    if (config < 0) config = c->tileConfig;
+    if (config >= kNumTileConfigs) return SL_ERR_INVALID_ARG;
+    if (tileW) *tileW = kTileConfigs[config].tileW;
+    if (tileH) *tileH = kTileConfigs[config].tileH;
+    if (segments) *segments = kTileConfigs[config].segs;
+    return config;
 }
 
 int sl_set_viewport(sl_context* c, int32_t x, int32_t y, int32_t w, int32_t h)

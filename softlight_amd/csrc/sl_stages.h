@@ -59,10 +59,17 @@ __device__ __forceinline__ float4 read_attr(const DrawDesc& d, int bind, uint64_
     const int n = b.dims + 1;
     if (b.type == VERTEX_DATA_FLOAT)
     {
-        if (off >= 0 && (uint64_t)off + 4u * (uint64_t)n <= d.vboBytes && (off & 3) == 0)
+        if (off >= 0 && (uint64_t)off + 4u * (uint64_t)n <= d.vboBytes)
         {
-            const float* p = reinterpret_cast<const float*>(d.vbo + off);
-            for (int i = 0; i < n; ++i) o[i] = p[i];
+            if ((off & 3) == 0)
+            {
+                const float* p = reinterpret_cast<const float*>(d.vbo + off);
+                for (int i = 0; i < n; ++i) o[i] = p[i];
+            }
+            else
+            {
+                for (int i = 0; i < n; ++i) __builtin_memcpy(&o[i], d.vbo + off + 4 * i, 4);
+            }
         }
     }
     else if (b.type == VERTEX_DATA_BYTE)
@@ -192,7 +199,8 @@ __device__ __forceinline__ bool bin_accept(float4 p0, float4 p1, float4 p2)
 // (a truncated span end may fall one pixel left of the snapped bounding box).
 struct TileRect { int tx0, tx1, ty0, ty1; };
 
-__device__ __forceinline__ TileRect tile_rect(float4 p0, float4 p1, float4 p2, int W, int H, int tilesX, int tilesY)
+__device__ __forceinline__ TileRect tile_rect(float4 p0, float4 p1, float4 p2, int W, int H, int tilesX, int tilesY,
+                                              int shx, int shy)
 {
     const int minX = (int)fminf(fminf(p0.x, p1.x), p2.x);
     const int maxX = (int)fmaxf(fmaxf(p0.x, p1.x), p2.x);
@@ -202,8 +210,8 @@ __device__ __forceinline__ TileRect tile_rect(float4 p0, float4 p1, float4 p2, i
     const int x0 = max(minX - 1, 0), x1 = min(maxX, W - 1);
     const int y0 = max(minY, 0), y1 = min(maxY - 1, H - 1);
     if (x0 > x1 || y0 > y1) { r.tx0 = 0; r.tx1 = -1; r.ty0 = 0; r.ty1 = -1; return r; }
-    r.tx0 = x0 / kTileW; r.tx1 = min(x1 / kTileW, tilesX - 1);
-    r.ty0 = y0 / kTileH; r.ty1 = min(y1 / kTileH, tilesY - 1);
+    r.tx0 = x0 >> shx; r.tx1 = min(x1 >> shx, tilesX - 1);
+    r.ty0 = y0 >> shy; r.ty1 = min(y1 >> shy, tilesY - 1);
     return r;
 }
 
