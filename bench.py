@@ -8,7 +8,9 @@ R32F targets, depth LESS, back-face culling; inputs resident in HBM before
 timing starts.  Prints one JSON line:
 
   value      submitted triangles per second of whole frames (Mtri/s, wall
-             clock around `steps` frames, each ending in a device sync)
+             clock around `steps` back-to-back replays of the frame captured as
+             a HIP graph, synchronised on both sides); eager frames (each with
+             a host round trip after setup) give the per-stage device times
   mtri_per_s_stage    submitted triangles / (vertex+setup+binning device time)
   shaded_gfrag_per_s  shaded fragments / fragment-stage device time
   roofline   the setup kernel (the dominant kernel) against the 8 TB/s HBM
@@ -35,6 +37,7 @@ HBM_PEAK_GBS = 8000.0
 WORKLOAD = dict(seed=1, n_instances=64, stacks=256, sectors=256, W=1920, H=1080, tex_size=1024, radius=2.0,
                 extent=20.0)
 RANDOM_WORKLOAD = dict(seed=0, ntri=4096, W=640, H=480)
+BATCHED_WORKLOAD = dict(seed=3, n_meshes=1000)
 CALIBRATION_BYTES = 256 << 20
 
 
@@ -75,6 +78,8 @@ def run_gpu(steps, warmup, workload="spheres", raster_config=None, tune=False):
     ctx = api.SLContext(softlight_amd.load_library())
     if workload == "spheres":
         scene = scenes.build_sphere_scene(ctx, **WORKLOAD)
+    elif workload == "batched":
+        scene = scenes.build_batched_scene(ctx, **BATCHED_WORKLOAD)
     else:
         scene = scenes.build_random_triangles(ctx, **RANDOM_WORKLOAD)
     signature = f"{workload}:{scene.W}x{scene.H}:{scene.n_triangles}"
@@ -94,17 +99,29 @@ def run_gpu(steps, warmup, workload="spheres", raster_config=None, tune=False):
             ctx.set_raster_config(cached)
     scene.info["raster_config"] = ctx.raster_config()
     for _ in range(warmup):
-        scene.render()
-        ctx.finish()
+        frame()
+    # eager frames: per-stage device times
     ctx.reset_stats()
     ctx.finish()
     t0 = time.perf_counter()
     for _ in range(steps):
-        scene.render()
-        ctx.finish()
+        frame()
     t1 = time.perf_counter()
     st = ctx.stats()
-    return scene, st, (t1 - t0)
+    eager = (t1 - t0)
+    # captured frames: the timed measurement
+    scene.render()
+    ctx.frame_graph_capture()
+    ctx.finish()
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.frame_graph_launch()
+    ctx.finish()
+    t1 = time.perf_counter()
+    gst = ctx.stats()
+    ctx.frame_graph_release()
+    return scene, st, eager, gst, (t1 - t0)
 
 
 def run_cpu_baseline(frames=3):
@@ -144,8 +161,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--workload", choices=["spheres", "random"], default="spheres",
-                    help="spheres: the 1080p textured-sphere frame (default); random: 4096 random triangles at 640x480")
+    ap.add_argument("--workload", choices=["spheres", "random", "batched"], default="spheres",
+                    help="spheres: the 1080p textured-sphere frame (default); random: 4096 random triangles at "
+                         "640x480; batched: 1000 meshes with mixed opaque/blended materials at 1080p")
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter calibration)")
     ap.add_argument("--raster-config", type=int, default=None, help="raster tile configuration 0..7 (default: tuned)")
     ap.add_argument("--tune", action="store_true", help="time all raster configurations first and cache the best")
@@ -157,7 +175,7 @@ def main():
 
     if args.calibrate:
         calibration_copy()
-    scene, st, elapsed = run_gpu(args.steps, args.warmup, args.workload, args.raster_config, args.tune)
+    scene, st, eager_s, gst, elapsed = run_gpu(args.steps, args.warmup, args.workload, args.raster_config, args.tune)
     tris = scene.n_triangles
     frames = max(st["frames"], 1)
     ms_step = elapsed * 1e3 / args.steps
@@ -170,9 +188,14 @@ def main():
     shaded = st["fragments_shaded"] / frames
     tested = st["fragments_tested"] / frames
 
-    bpt = algorithmic_bytes_per_triangle(scene.info) if args.workload == "spheres" else 48.0
+    if args.workload == "spheres":
+        bpt = algorithmic_bytes_per_triangle(scene.info)
+    elif args.workload == "batched":
+        bpt = 12.0 + 32.0 * 0.5          # u32 indices + ~1/2 vertex (32 B) per triangle of an icosphere
+    else:
+        bpt = 48.0
     achieved = tris * bpt / (ms_setup * 1e-3) / 1e9        # GB/s
-    texel_bytes = 16.0 if args.workload == "spheres" else 0.0
+    texel_bytes = 0.0 if args.workload == "random" else 16.0
     fbytes = frag_bytes(tested, shaded, texel_bytes)
     frag_gbs = fbytes / (ms_raster * 1e-3) / 1e9
 
@@ -193,12 +216,17 @@ def main():
             "workload": ("textured UV-sphere scene: 64 instances x 130,560 indexed triangles, 1920x1080 RGBA8+R32F, "
                          "1024^2 RGBA8 bilinear REPEAT, Lambert, depth LESS, back-face cull")
             if args.workload == "spheres" else
-            "4096 random non-indexed triangles, 640x480 RGBA8+R32F, Gouraud, depth LESS, no cull",
+            ("1000 displaced icospheres, log-normal triangle counts (median 2000), 20% alpha-blended, "
+             "16 RGBA8 textures, 1920x1080, one batch" if args.workload == "batched" else
+             "4096 random non-indexed triangles, 640x480 RGBA8+R32F, Gouraud, depth LESS, no cull"),
             "triangles_per_frame": tris,
             "W": scene.W, "H": scene.H,
             "raster_tile": {"config": scene.info["raster_config"][0], "tile_w": scene.info["raster_config"][1][0],
                             "tile_h": scene.info["raster_config"][1][1], "segments": scene.info["raster_config"][1][2]},
         },
+        "frame_mode": "hipGraph replay of the captured frame (clear + draws), back to back",
+        "eager_ms_per_frame": round(eager_s * 1e3 / args.steps, 4),
+        "graph_device_ms_per_frame": round(gst["ms_total"] / max(gst["frames"], 1), 4),
         "mtri_per_s_stage": round(tris / ((ms_vertex + ms_bin) * 1e-3) / 1e6, 3),
         "shaded_gfrag_per_s": round(shaded / (ms_raster * 1e-3) / 1e9, 4),
         "fragments_shaded_per_frame": int(shaded),

@@ -262,6 +262,18 @@ int sl_clear_color_buffer(sl_context* ctx, size_t fbo, unsigned attachment, cons
 int sl_clear_depth_buffer(sl_context* ctx, size_t fbo, double depth);                                  /* SL_Context::clear_depth_buffer */
 int sl_clear_framebuffer(sl_context* ctx, size_t fbo, unsigned attachment, const double rgba[4], double depth); /* SL_Context::clear_framebuffer */
 
+/* ---- HIP-graph frames (no reference counterpart) -------------------------
+ * sl_frame_graph_capture executes the pending batch (the clears and draws
+ * recorded since the last synchronisation point) once, then captures the same
+ * frame -- uniform snapshots included -- into a HIP graph with private
+ * buffers sized from that run.  sl_frame_graph_launch replays it on the
+ * context's stream without host round trips; sl_finish waits and adds the
+ * replays to the counters (SL_ERR_CAPACITY if a replay outgrew the captured
+ * capacity).  One graph per context; capturing again replaces it. */
+int sl_frame_graph_capture(sl_context* ctx);
+int sl_frame_graph_launch(sl_context* ctx);
+int sl_frame_graph_release(sl_context* ctx);
+
 /* ---- synchronisation and counters (no reference counterpart: the
  *      reference's draw is synchronous, SL_ProcessorPool.cpp:326) ---------- */
 int sl_finish(sl_context* ctx);

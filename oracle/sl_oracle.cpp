@@ -1421,6 +1421,11 @@ int slo_clear_framebuffer(sl_context* c, size_t fbo, unsigned att, const double 
 
 int slo_finish(sl_context*) { return SL_OK; }
 
+// The oracle executes every draw immediately; there is nothing to capture.
+int slo_frame_graph_capture(sl_context*) { return SL_OK; }
+int slo_frame_graph_launch(sl_context*) { return SL_ERR_UNSUPPORTED; }
+int slo_frame_graph_release(sl_context*) { return SL_OK; }
+
 int slo_get_stats(const sl_context* c, sl_frame_stats* out)
 {
     if (!c || !out) return SL_ERR_INVALID_ARG;

@@ -124,6 +124,9 @@ _SIGS = {
     "clear_depth_buffer": (_I, [_P, _SZ, C.c_double]),
     "clear_framebuffer": (_I, [_P, _SZ, _U, C.POINTER(C.c_double), C.c_double]),
     "finish": (_I, [_P]),
+    "frame_graph_capture": (_I, [_P]),
+    "frame_graph_launch": (_I, [_P]),
+    "frame_graph_release": (_I, [_P]),
     "get_stats": (_I, [_P, C.POINTER(sl_frame_stats)]),
     "reset_stats": (_I, [_P]),
 }
@@ -326,6 +329,15 @@ class SLContext:
 
     def finish(self):
         return self._check(self.lib.finish(self.h), "finish")
+
+    def frame_graph_capture(self):
+        return self._check(self.lib.frame_graph_capture(self.h), "frame_graph_capture")
+
+    def frame_graph_launch(self):
+        return self._check(self.lib.frame_graph_launch(self.h), "frame_graph_launch")
+
+    def frame_graph_release(self):
+        return self._check(self.lib.frame_graph_release(self.h), "frame_graph_release")
 
     def stats(self):
         s = sl_frame_stats()

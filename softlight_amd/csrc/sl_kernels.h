@@ -9,9 +9,10 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, uint64_t n, hipStream_t s);
 hipError_t launch_setup(const SetupParams& p, uint64_t chunks, hipStream_t s);
 hipError_t launch_clip(const SetupParams& p, uint32_t jobs, hipStream_t s);
-hipError_t launch_ranges(const uint32_t* keys, uint32_t n, uint2* range, hipStream_t s);
-hipError_t sort_pairs(void* temp, size_t& tempBytes, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
-                      uint32_t* vout, uint32_t n, int endBit, hipStream_t s);
+uint32_t bin_blocks(uint32_t entries);
+hipError_t launch_binning(const uint32_t* keys, const uint32_t* vals, const FrameCtrl* ctrl, uint32_t cap,
+                          uint32_t gridBlocks, uint32_t* counts, uint32_t* totals, uint2* range, uint32_t* out,
+                          int numTiles, hipStream_t s);
 hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s);
 
 }  // namespace slg

@@ -10,7 +10,7 @@
 //              finished by k_clip.
 //   k_clip     6-plane Sutherland-Hodgman clipping with varyings and the
 //              reversed fan, writing into the reserved slots.
-//   k_ranges   [start, end) of each tile's entries in the tile-sorted list.
+//   k_bin_*    ordered binning: stable counting sort of the bin entries by tile.
 //   k_raster   one wave per screen tile: walks the tile's records in
 //              submission order; lane = (row, x-phase) replays the
 //              reference's scanline span and its sequential barycentric
@@ -21,8 +21,7 @@
 // clip_and_process_tris (SL_TriProcessor.cpp:248-640), SL_TriRasterizer::
 // render_triangle_simd generic branch (SL_TriRasterizer.cpp:806-923),
 // SL_FragmentProcessor::flush_tri_fragments (SL_FragmentProcessor.cpp:416-531).
-#include <hipcub/hipcub.hpp>
-
+#include "sl_device.h"
 #include "sl_recurrence.h"
 #include "sl_stages.h"
 #include "sl_kernels.h"
@@ -450,15 +449,128 @@ __global__ __launch_bounds__(64) void k_clip(SetupParams P)
 }
 
 // ---------------------------------------------------------------------------
-// k_ranges: per-tile [start, end) in the tile-sorted entry list
+// Ordered binning: a stable counting sort of the (tile, record) entries, which
+// k_setup/k_clip emit in record order, into per-tile lists that keep that
+// order.  Entries are split into blocks of kBinBlock; k_bin_count histograms
+// each block per tile, k_bin_colscan turns every tile's column of block counts
+// into block offsets and tile totals, k_bin_tilescan scans the totals into
+// tile ranges, and k_bin_scatter (one wave per block) places each block's
+// entries with ranks from wave ballots, chunk by chunk, so equal tiles keep
+// their entry order.  Entry and block counts are read from the device, so no
+// host round trip is needed between setup and raster.
 // ---------------------------------------------------------------------------
-__global__ void k_ranges(const uint32_t* __restrict__ keys, uint32_t n, uint2* __restrict__ range)
+constexpr uint32_t kBinBlock = 8192;
+
+__device__ __forceinline__ uint32_t bin_entries(const FrameCtrl* ctrl, uint32_t cap)
 {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    return min(ctrl->totalPairs, cap);
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(const uint32_t* __restrict__ keys, const FrameCtrl* __restrict__ ctrl,
+                                                   uint32_t cap, uint32_t* __restrict__ counts, int numTiles)
+{
+    extern __shared__ uint32_t sCnt[];
+    const uint32_t n = bin_entries(ctrl, cap);
+    const uint32_t start = blockIdx.x * kBinBlock;
+    if (start >= n) return;
+    const uint32_t end = min(start + kBinBlock, n);
+    for (int t = threadIdx.x; t < numTiles; t += blockDim.x) sCnt[t] = 0u;
+    __syncthreads();
+    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) atomicAdd(&sCnt[keys[i]], 1u);
+    __syncthreads();
+    uint32_t* row = counts + (uint64_t)blockIdx.x * numTiles;
+    for (int t = threadIdx.x; t < numTiles; t += blockDim.x) row[t] = sCnt[t];
+}
+
+__global__ __launch_bounds__(256) void k_bin_colscan(const FrameCtrl* __restrict__ ctrl, uint32_t cap,
+                                                     uint32_t* __restrict__ counts, uint32_t* __restrict__ totals, int numTiles)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= numTiles) return;
+    const uint32_t n = bin_entries(ctrl, cap);
+    const uint32_t blocks = (n + kBinBlock - 1) / kBinBlock;
+    uint32_t run = 0;
+    for (uint32_t b = 0; b < blocks; ++b)
     {
-        const uint32_t k = keys[i];
-        if (i == 0 || keys[i - 1] != k) range[k].x = i;
-        if (i == n - 1 || keys[i + 1] != k) range[k].y = i + 1;
+        uint32_t* c = counts + (uint64_t)b * numTiles + t;
+        const uint32_t v = *c;
+        *c = run;
+        run += v;
+    }
+    totals[t] = run;
+}
+
+__global__ __launch_bounds__(1024) void k_bin_tilescan(const uint32_t* __restrict__ totals, uint2* __restrict__ range, int numTiles)
+{
+    __shared__ uint32_t sWave[16];
+    __shared__ uint32_t sCarry;
+    if (threadIdx.x == 0) sCarry = 0;
+    __syncthreads();
+    for (int base = 0; base < numTiles; base += 1024)
+    {
+        const int t = base + threadIdx.x;
+        const uint32_t v = (t < numTiles) ? totals[t] : 0u;
+        uint32_t incl = v;
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1)
+        {
+            const uint32_t o = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += o;
+        }
+        if (lane == 63) sWave[wave] = incl;
+        __syncthreads();
+        uint32_t waveBase = 0, chunkTotal = 0;
+        for (int w = 0; w < 16; ++w)
+        {
+            if (w < wave) waveBase += sWave[w];
+            chunkTotal += sWave[w];
+        }
+        const uint32_t excl = sCarry + waveBase + incl - v;
+        if (t < numTiles) range[t] = make_uint2(excl, excl + v);
+        __syncthreads();
+        if (threadIdx.x == 0) sCarry += chunkTotal;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(64) void k_bin_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                    const FrameCtrl* __restrict__ ctrl, uint32_t cap,
+                                                    const uint32_t* __restrict__ counts, const uint2* __restrict__ range,
+                                                    uint32_t* __restrict__ out, int numTiles)
+{
+    extern __shared__ uint32_t sBase[];
+    const uint32_t n = bin_entries(ctrl, cap);
+    const uint32_t start = blockIdx.x * kBinBlock;
+    if (start >= n) return;
+    const uint32_t end = min(start + kBinBlock, n);
+    const uint32_t* row = counts + (uint64_t)blockIdx.x * numTiles;
+    for (int t = threadIdx.x; t < numTiles; t += 64) sBase[t] = range[t].x + row[t];
+    __syncthreads();
+    const int lane = threadIdx.x;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (uint32_t c = start; c < end; c += 64)
+    {
+        const uint32_t i = c + lane;
+        const bool valid = i < end;
+        const uint32_t key = valid ? keys[i] : 0xFFFFFFFFu;
+        const uint32_t val = valid ? vals[i] : 0u;
+        unsigned long long remaining = __ballot(valid);
+        while (remaining)
+        {
+            const int leader = __ffsll((long long)remaining) - 1;
+            const uint32_t k = __shfl(key, leader, 64);
+            const unsigned long long m = __ballot(key == k) & remaining;
+            const uint32_t b = sBase[k];
+            if ((m >> lane) & 1ull) out[b + __popcll(m & below)] = val;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (lane == leader) sBase[k] = b + (uint32_t)__popcll(m);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            remaining &= ~m;
+        }
     }
 }
 
@@ -732,17 +844,19 @@ hipError_t launch_clip(const SetupParams& p, uint32_t jobs, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_ranges(const uint32_t* keys, uint32_t n, uint2* range, hipStream_t s)
-{
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ranges, dim3(grid_for(n, 256)), dim3(256), 0, s, keys, n, range);
-    return hipGetLastError();
-}
+uint32_t bin_blocks(uint32_t entries) { return (entries + kBinBlock - 1) / kBinBlock; }
 
-hipError_t sort_pairs(void* temp, size_t& tempBytes, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
-                      uint32_t* vout, uint32_t n, int endBit, hipStream_t s)
+hipError_t launch_binning(const uint32_t* keys, const uint32_t* vals, const FrameCtrl* ctrl, uint32_t cap,
+                          uint32_t gridBlocks, uint32_t* counts, uint32_t* totals, uint2* range, uint32_t* out,
+                          int numTiles, hipStream_t s)
 {
-    return hipcub::DeviceRadixSort::SortPairs(temp, tempBytes, kin, kout, vin, vout, (int)n, 0, endBit, s);
+    if (gridBlocks == 0) gridBlocks = 1;
+    const size_t lds = sizeof(uint32_t) * (size_t)numTiles;
+    hipLaunchKernelGGL(k_bin_count, dim3(gridBlocks), dim3(256), lds, s, keys, ctrl, cap, counts, numTiles);
+    hipLaunchKernelGGL(k_bin_colscan, dim3((numTiles + 255) / 256), dim3(256), 0, s, ctrl, cap, counts, totals, numTiles);
+    hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), 0, s, totals, range, numTiles);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(gridBlocks), dim3(64), lds, s, keys, vals, ctrl, cap, counts, range, out, numTiles);
+    return hipGetLastError();
 }
 
 template <int CF, bool DBG>

@@ -36,6 +36,24 @@ struct Fbo { std::vector<size_t> colors; size_t depth = SL_INVALID_ID; size_t db
 struct Ubo { uint8_t bytes[SL_MAX_UNIFORM_BUFFER_SIZE]; };
 struct Shader { uint8_t cull, dtest, dmask, blend, nvary, nout; uint32_t vsProg, fsProg; size_t ubo; };
 
+// Device buffers of one frame.
+struct FrameRes
+{
+    DevBuf draws, uni, recs, vars, keys, vals, sorted, clip, status, ranges, ctrl, binCounts, binTotals;
+    uint32_t recCap = 0, pairCap = 0, clipCap = 0;
+};
+
+struct FrameGraph
+{
+    bool valid = false;
+    FrameRes res;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    FrameCtrl* hCtrl = nullptr;
+    uint64_t tris = 0;
+    uint64_t pending = 0;                          // replays since the last collection
+};
+
 struct Batch
 {
     size_t fbo = SL_INVALID_ID;
@@ -97,9 +115,9 @@ struct sl_context
 
     Batch batch;
 
-    // frame scratch (grow-only)
-    DevBuf dDraws, dUni, dRecs, dVars, dKeys[2], dVals[2], dClip, dStatus, dRanges, dCtrl, dSortTemp;
-    uint32_t recCap = 0, pairCap = 0, clipCap = 0, varStride = 0;
+    FrameRes res;                                  // eager-frame scratch (grow-only)
+    FrameGraph graph;                              // captured frame
+    uint64_t lastTotals[3] = {0, 0, 0};            // records, bin entries, clip jobs of the last eager frame
     FrameCtrl* hCtrl = nullptr;                    // pinned readback of the control block
     hipEvent_t ev[8] = {};
 };
@@ -182,151 +200,108 @@ int flush(sl_context* c);
 // ---------------------------------------------------------------------------
 // batch execution
 // ---------------------------------------------------------------------------
-int execute_batch(sl_context* c)
+// Everything derived from a batch and its target framebuffer.
+struct Plan
 {
-    Batch& B = c->batch;
-    if (B.empty()) return SL_OK;
+    int W, H, cfg, tilesX, tilesY, numTiles;
+    TileConfig tc;
+    Tex* dt;
+    Tex* ct;
+    Tex* dbgL;
+    Tex* dbgC;
+    bool debug;
+    uint64_t chunks;
+    uint32_t varStride;
+};
+
+int make_plan(sl_context* c, const Batch& B, Plan& p)
+{
     if (B.fbo >= c->fbos.size()) return fail(c, SL_ERR_INVALID_HANDLE, "batch framebuffer");
     Fbo& F = c->fbos[B.fbo];
-    Tex* dt = tex_at(c, F.depth);
-    if (!dt || dt->type != SL_COLOR_R_FLOAT) return fail(c, SL_ERR_STATE, "framebuffer needs an R_FLOAT depth attachment");
-    Tex* ct = F.colors.empty() ? nullptr : tex_at(c, F.colors[0]);
-    Tex* dbgL = tex_at(c, F.dbgLast);
-    Tex* dbgC = tex_at(c, F.dbgCount);
-    const bool debug = dbgL && dbgC;
+    p.dt = tex_at(c, F.depth);
+    if (!p.dt || p.dt->type != SL_COLOR_R_FLOAT) return fail(c, SL_ERR_STATE, "framebuffer needs an R_FLOAT depth attachment");
+    p.ct = F.colors.empty() ? nullptr : tex_at(c, F.colors[0]);
+    if (p.ct && p.ct->type != SL_COLOR_RGBA_8U && p.ct->type != SL_COLOR_RGBA_FLOAT) return fail(c, SL_ERR_UNSUPPORTED, "colour attachment format");
+    p.dbgL = tex_at(c, F.dbgLast);
+    p.dbgC = tex_at(c, F.dbgCount);
+    p.debug = p.dbgL && p.dbgC;
+    p.W = p.dt->w;
+    p.H = p.dt->h;
+    if (p.ct && (p.ct->w != p.W || p.ct->h != p.H)) return fail(c, SL_ERR_STATE, "colour and depth attachment sizes differ");
+    p.cfg = (c->tileConfig >= 0 && c->tileConfig < kNumTileConfigs) ? c->tileConfig : kDefaultTileConfig;
+    p.tc = kTileConfigs[p.cfg];
+    p.tilesX = (p.W + p.tc.tileW - 1) / p.tc.tileW;
+    p.tilesY = (p.H + p.tc.tileH - 1) / p.tc.tileH;
+    p.numTiles = p.tilesX * p.tilesY;
+    if (p.numTiles > 16384) return fail(c, SL_ERR_UNSUPPORTED, "more than 16384 raster tiles: choose a larger tile configuration");
+    if (B.totalTris >= (1ull << 32)) return fail(c, SL_ERR_CAPACITY, "more than 2^32 triangles in one batch");
+    p.chunks = (B.totalTris + kSetupBlock - 1) / kSetupBlock;
+    p.varStride = 3u * std::max<uint32_t>(B.maxVary, 1u);
+    return SL_OK;
+}
 
-    const int W = dt->w, H = dt->h;
-    const int cfg = (c->tileConfig >= 0 && c->tileConfig < kNumTileConfigs) ? c->tileConfig : kDefaultTileConfig;
-    const TileConfig tc = kTileConfigs[cfg];
-    const int tilesX = (W + tc.tileW - 1) / tc.tileW, tilesY = (H + tc.tileH - 1) / tc.tileH;
-    const int numTiles = tilesX * tilesY;
-    hipStream_t s = c->stream;
+// Allocates a frame's buffers at the resource set's current capacities.
+int ensure_frame(sl_context* c, FrameRes& R, const Plan& p, const Batch& B)
+{
     int rc;
+    if ((rc = ensure(c, R.ctrl, sizeof(FrameCtrl))) != SL_OK) return rc;
+    if ((rc = ensure(c, R.ranges, sizeof(uint2) * (size_t)p.numTiles)) != SL_OK) return rc;
+    if (B.totalTris == 0) return SL_OK;
+    if (R.recCap == 0) { R.recCap = 1u << 18; R.pairCap = 1u << 20; R.clipCap = 1u << 16; }
+    if ((rc = ensure(c, R.draws, sizeof(DrawDesc) * B.draws.size())) != SL_OK) return rc;
+    if ((rc = ensure(c, R.uni, std::max<size_t>(B.uni.size(), 16))) != SL_OK) return rc;
+    if ((rc = ensure(c, R.status, sizeof(unsigned long long) * p.chunks)) != SL_OK) return rc;
+    if ((rc = ensure(c, R.recs, sizeof(SetupRec) * (size_t)R.recCap)) != SL_OK) return rc;
+    if ((rc = ensure(c, R.vars, sizeof(float4) * (size_t)p.varStride * R.recCap)) != SL_OK) return rc;
+    if ((rc = ensure(c, R.keys, sizeof(uint32_t) * (size_t)R.pairCap)) != SL_OK) return rc;
+    if ((rc = ensure(c, R.vals, sizeof(uint32_t) * (size_t)R.pairCap)) != SL_OK) return rc;
+    if ((rc = ensure(c, R.sorted, sizeof(uint32_t) * (size_t)R.pairCap)) != SL_OK) return rc;
+    if ((rc = ensure(c, R.clip, sizeof(ClipJob) * (size_t)R.clipCap)) != SL_OK) return rc;
+    if ((rc = ensure(c, R.binCounts, sizeof(uint32_t) * (size_t)bin_blocks(R.pairCap) * p.numTiles)) != SL_OK) return rc;
+    if ((rc = ensure(c, R.binTotals, sizeof(uint32_t) * (size_t)p.numTiles)) != SL_OK) return rc;
+    return SL_OK;
+}
 
-    if ((rc = ensure(c, c->dCtrl, sizeof(FrameCtrl))) != SL_OK) return rc;
-    if ((rc = ensure(c, c->dRanges, sizeof(uint2) * (size_t)numTiles)) != SL_OK) return rc;
-    FrameCtrl* ctrl = reinterpret_cast<FrameCtrl*>(c->dCtrl.p);
-    uint2* ranges = reinterpret_cast<uint2*>(c->dRanges.p);
-    if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(ctrl), 0u, sizeof(FrameCtrl) / 4, s), "fill ctrl")) != SL_OK) return rc;
-    if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(ranges), 0u, 2ull * numTiles, s), "fill ranges")) != SL_OK) return rc;
-
-    float msVertex = 0.f, msBin = 0.f, msRaster = 0.f;
-    const uint32_t* tileRecs = nullptr;
-    uint32_t totalRecs = 0, totalPairs = 0;
-
+SetupParams setup_params(const Plan& p, const Batch& B, FrameRes& R)
+{
     SetupParams sp{};
-    if (B.totalTris > 0)
-    {
-        if (B.totalTris >= (1ull << 32)) return fail(c, SL_ERR_CAPACITY, "more than 2^32 triangles in one batch");
-        const size_t nd = B.draws.size();
-        if ((rc = ensure(c, c->dDraws, sizeof(DrawDesc) * nd)) != SL_OK) return rc;
-        if ((rc = ensure(c, c->dUni, std::max<size_t>(B.uni.size(), 16))) != SL_OK) return rc;
-        if ((rc = check(c, hipMemcpyAsync(c->dDraws.p, B.draws.data(), sizeof(DrawDesc) * nd, hipMemcpyHostToDevice, s), "upload draws")) != SL_OK) return rc;
-        if (!B.uni.empty() && (rc = check(c, hipMemcpyAsync(c->dUni.p, B.uni.data(), B.uni.size(), hipMemcpyHostToDevice, s), "upload uniforms")) != SL_OK) return rc;
+    sp.draws = reinterpret_cast<const DrawDesc*>(R.draws.p);
+    sp.numDraws = (uint32_t)B.draws.size();
+    sp.totalTris = B.totalTris;
+    sp.uniforms = reinterpret_cast<const uint8_t*>(R.uni.p);
+    sp.W = p.W; sp.H = p.H; sp.tilesX = p.tilesX; sp.tilesY = p.tilesY;
+    sp.tileShiftX = ilog2i(p.tc.tileW);
+    sp.tileShiftY = ilog2i(p.tc.tileH);
+    sp.recs = reinterpret_cast<SetupRec*>(R.recs.p);
+    sp.vars = reinterpret_cast<float4*>(R.vars.p);
+    sp.varStride = p.varStride;
+    sp.recCap = R.recCap;
+    sp.pairKeys = reinterpret_cast<uint32_t*>(R.keys.p);
+    sp.pairVals = reinterpret_cast<uint32_t*>(R.vals.p);
+    sp.pairCap = R.pairCap;
+    sp.clipJobs = reinterpret_cast<ClipJob*>(R.clip.p);
+    sp.clipCap = R.clipCap;
+    sp.status = reinterpret_cast<unsigned long long*>(R.status.p);
+    sp.ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
+    return sp;
+}
 
-        const uint64_t chunks = (B.totalTris + kSetupBlock - 1) / kSetupBlock;
-        if ((rc = ensure(c, c->dStatus, sizeof(unsigned long long) * chunks)) != SL_OK) return rc;
-
-        const uint32_t varStride = 3u * std::max<uint32_t>(B.maxVary, 1u);
-        if (c->recCap == 0) { c->recCap = 1u << 18; c->pairCap = 1u << 20; c->clipCap = 1u << 16; }
-        for (int attempt = 0; attempt < 4; ++attempt)
-        {
-            if ((rc = ensure(c, c->dRecs, sizeof(SetupRec) * (size_t)c->recCap)) != SL_OK) return rc;
-            if ((rc = ensure(c, c->dVars, sizeof(float4) * (size_t)varStride * c->recCap)) != SL_OK) return rc;
-            for (int k = 0; k < 2; ++k)
-            {
-                if ((rc = ensure(c, c->dKeys[k], sizeof(uint32_t) * (size_t)c->pairCap)) != SL_OK) return rc;
-                if ((rc = ensure(c, c->dVals[k], sizeof(uint32_t) * (size_t)c->pairCap)) != SL_OK) return rc;
-            }
-            if ((rc = ensure(c, c->dClip, sizeof(ClipJob) * (size_t)c->clipCap)) != SL_OK) return rc;
-
-            sp.draws = reinterpret_cast<const DrawDesc*>(c->dDraws.p);
-            sp.numDraws = (uint32_t)nd;
-            sp.totalTris = B.totalTris;
-            sp.uniforms = reinterpret_cast<const uint8_t*>(c->dUni.p);
-            sp.W = W; sp.H = H; sp.tilesX = tilesX; sp.tilesY = tilesY;
-            sp.tileShiftX = ilog2i(tc.tileW);
-            sp.tileShiftY = ilog2i(tc.tileH);
-            sp.recs = reinterpret_cast<SetupRec*>(c->dRecs.p);
-            sp.vars = reinterpret_cast<float4*>(c->dVars.p);
-            sp.varStride = varStride;
-            sp.recCap = c->recCap;
-            sp.pairKeys = reinterpret_cast<uint32_t*>(c->dKeys[0].p);
-            sp.pairVals = reinterpret_cast<uint32_t*>(c->dVals[0].p);
-            sp.pairCap = c->pairCap;
-            sp.clipJobs = reinterpret_cast<ClipJob*>(c->dClip.p);
-            sp.clipCap = c->clipCap;
-            sp.status = reinterpret_cast<unsigned long long*>(c->dStatus.p);
-            sp.ctrl = ctrl;
-
-            if ((rc = check(c, launch_fill_u64(sp.status, 0ull, chunks, s), "fill status")) != SL_OK) return rc;
-            if (attempt > 0 && (rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(ctrl), 0u, sizeof(FrameCtrl) / 4, s), "fill ctrl")) != SL_OK) return rc;
-            (void)hipEventRecord(c->ev[0], s);
-            if ((rc = check(c, launch_setup(sp, chunks, s), "k_setup")) != SL_OK) return rc;
-            (void)hipEventRecord(c->ev[1], s);
-            c->stats.setup_launches += 1;
-            if ((rc = check(c, hipMemcpyAsync(c->hCtrl, ctrl, sizeof(FrameCtrl), hipMemcpyDeviceToHost, s), "read ctrl")) != SL_OK) return rc;
-            if ((rc = check(c, hipStreamSynchronize(s), "k_setup sync")) != SL_OK) return rc;
-            if (c->hCtrl->error) return fail(c, SL_ERR_DEVICE, "setup look-back did not complete");
-            if (!c->hCtrl->overflow) break;
-            // grow to the exact requirement (+25%) and run setup again; the
-            // framebuffer has not been touched yet.
-            const uint64_t needR = c->hCtrl->totalRecs, needP = c->hCtrl->totalPairs, needC = c->hCtrl->clipJobs;
-            auto grow = [](uint64_t need, uint32_t cur) -> uint64_t { return std::max<uint64_t>(cur, need + need / 4 + 1024); };
-            const uint64_t nr = grow(needR, c->recCap), np = grow(needP, c->pairCap), nc = grow(needC, c->clipCap);
-            if (nr >= (1ull << 30) || np >= (1ull << 31) || nc >= (1ull << 31)) return fail(c, SL_ERR_CAPACITY, "frame exceeds bin capacity");
-            c->recCap = (uint32_t)nr; c->pairCap = (uint32_t)np; c->clipCap = (uint32_t)nc;
-            if (attempt == 3) return fail(c, SL_ERR_CAPACITY, "bin capacity growth did not converge");
-        }
-        float t = 0.f;
-        (void)hipEventElapsedTime(&t, c->ev[0], c->ev[1]);
-        msVertex += t;
-        c->stats.ms_setup_kernel += t;
-        totalRecs = c->hCtrl->totalRecs;
-        totalPairs = c->hCtrl->totalPairs;
-        const uint32_t jobs = c->hCtrl->clipJobs;
-
-        (void)hipEventRecord(c->ev[2], s);
-        if ((rc = check(c, launch_clip(sp, jobs, s), "k_clip")) != SL_OK) return rc;
-        (void)hipEventRecord(c->ev[3], s);
-
-        if (totalPairs > 0)
-        {
-            int endBit = 1;
-            while ((1 << endBit) < numTiles) ++endBit;
-            size_t tempBytes = 0;
-            if ((rc = check(c, sort_pairs(nullptr, tempBytes, sp.pairKeys, reinterpret_cast<uint32_t*>(c->dKeys[1].p), sp.pairVals,
-                                          reinterpret_cast<uint32_t*>(c->dVals[1].p), totalPairs, endBit, s), "sort size")) != SL_OK) return rc;
-            if ((rc = ensure(c, c->dSortTemp, std::max<size_t>(tempBytes, 16))) != SL_OK) return rc;
-            tempBytes = c->dSortTemp.bytes;
-            if ((rc = check(c, sort_pairs(c->dSortTemp.p, tempBytes, sp.pairKeys, reinterpret_cast<uint32_t*>(c->dKeys[1].p), sp.pairVals,
-                                          reinterpret_cast<uint32_t*>(c->dVals[1].p), totalPairs, endBit, s), "sort")) != SL_OK) return rc;
-            if ((rc = check(c, launch_ranges(reinterpret_cast<const uint32_t*>(c->dKeys[1].p), totalPairs, ranges, s), "k_ranges")) != SL_OK) return rc;
-            tileRecs = reinterpret_cast<const uint32_t*>(c->dVals[1].p);
-        }
-        (void)hipEventRecord(c->ev[4], s);
-    }
-    else
-    {
-        (void)hipEventRecord(c->ev[4], s);
-    }
-
+RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool haveEntries)
+{
     RasterParams rp{};
-    rp.draws = reinterpret_cast<const DrawDesc*>(c->dDraws.p);
-    rp.uniforms = reinterpret_cast<const uint8_t*>(c->dUni.p);
-    rp.recs = reinterpret_cast<const SetupRec*>(c->dRecs.p);
-    rp.vars = reinterpret_cast<const float4*>(c->dVars.p);
-    rp.varStride = sp.varStride ? sp.varStride : 3;
-    rp.tileRecs = tileRecs;
-    rp.tileRange = ranges;
-    rp.W = W; rp.H = H; rp.tilesX = tilesX; rp.tilesY = tilesY;
-    rp.color = ct ? reinterpret_cast<uint8_t*>(ct->buf.p) : nullptr;
-    rp.colorType = ct ? ct->type : SL_COLOR_RGBA_8U;
-    if (ct && ct->type != SL_COLOR_RGBA_8U && ct->type != SL_COLOR_RGBA_FLOAT) return fail(c, SL_ERR_UNSUPPORTED, "colour attachment format");
-    if (ct && (ct->w != W || ct->h != H)) return fail(c, SL_ERR_STATE, "colour and depth attachment sizes differ");
-    rp.depth = reinterpret_cast<float*>(dt->buf.p);
-    rp.dbgLast = debug ? reinterpret_cast<uint32_t*>(dbgL->buf.p) : nullptr;
-    rp.dbgCount = debug ? reinterpret_cast<uint32_t*>(dbgC->buf.p) : nullptr;
+    rp.draws = reinterpret_cast<const DrawDesc*>(R.draws.p);
+    rp.uniforms = reinterpret_cast<const uint8_t*>(R.uni.p);
+    rp.recs = reinterpret_cast<const SetupRec*>(R.recs.p);
+    rp.vars = reinterpret_cast<const float4*>(R.vars.p);
+    rp.varStride = p.varStride;
+    rp.tileRecs = haveEntries ? reinterpret_cast<const uint32_t*>(R.sorted.p) : nullptr;
+    rp.tileRange = reinterpret_cast<const uint2*>(R.ranges.p);
+    rp.W = p.W; rp.H = p.H; rp.tilesX = p.tilesX; rp.tilesY = p.tilesY;
+    rp.color = p.ct ? reinterpret_cast<uint8_t*>(p.ct->buf.p) : nullptr;
+    rp.colorType = p.ct ? p.ct->type : SL_COLOR_RGBA_8U;
+    rp.depth = reinterpret_cast<float*>(p.dt->buf.p);
+    rp.dbgLast = p.debug ? reinterpret_cast<uint32_t*>(p.dbgL->buf.p) : nullptr;
+    rp.dbgCount = p.debug ? reinterpret_cast<uint32_t*>(p.dbgC->buf.p) : nullptr;
     rp.clearColor = B.clearColor ? 1 : 0;
     rp.clearDepth = B.clearDepth ? 1 : 0;
     uint32_t packed = 0;
@@ -337,8 +312,113 @@ int execute_batch(sl_context* c)
     }
     rp.clearRGBA8 = packed;
     rp.clearDepthValue = (float)B.depth;
-    rp.ctrl = ctrl;
-    if ((rc = check(c, launch_raster(rp, numTiles, cfg, debug, s), "k_raster")) != SL_OK) return rc;
+    rp.ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
+    return rp;
+}
+
+// Control-block reset, look-back status reset and the setup kernel.
+int enqueue_front(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, hipStream_t s)
+{
+    int rc;
+    FrameCtrl* ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
+    if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(ctrl), 0u, sizeof(FrameCtrl) / 4, s), "fill ctrl")) != SL_OK) return rc;
+    if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(R.ranges.p), 0u, 2ull * p.numTiles, s), "fill ranges")) != SL_OK) return rc;
+    if (B.totalTris == 0) return SL_OK;
+    const SetupParams sp = setup_params(p, B, R);
+    if ((rc = check(c, launch_fill_u64(sp.status, 0ull, p.chunks, s), "fill status")) != SL_OK) return rc;
+    return check(c, launch_setup(sp, p.chunks, s), "k_setup");
+}
+
+// Clip, binning and raster.  clipJobs / entries size the grids (a graph uses
+// the capacities; the kernels read the actual counts from the control block).
+int enqueue_back(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, uint32_t clipJobs, uint32_t entries,
+                 hipStream_t s, hipEvent_t evClip0, hipEvent_t evClip1, hipEvent_t evBin1)
+{
+    int rc;
+    if (B.totalTris > 0)
+    {
+        const SetupParams sp = setup_params(p, B, R);
+        if (evClip0) (void)hipEventRecord(evClip0, s);
+        if ((rc = check(c, launch_clip(sp, clipJobs, s), "k_clip")) != SL_OK) return rc;
+        if (evClip1) (void)hipEventRecord(evClip1, s);
+        if (entries > 0)
+        {
+            if ((rc = check(c, launch_binning(sp.pairKeys, sp.pairVals, sp.ctrl, R.pairCap, bin_blocks(entries),
+                                              reinterpret_cast<uint32_t*>(R.binCounts.p), reinterpret_cast<uint32_t*>(R.binTotals.p),
+                                              reinterpret_cast<uint2*>(R.ranges.p), reinterpret_cast<uint32_t*>(R.sorted.p),
+                                              p.numTiles, s), "binning")) != SL_OK) return rc;
+        }
+    }
+    if (evBin1) (void)hipEventRecord(evBin1, s);
+    const RasterParams rp = raster_params(p, B, R, B.totalTris > 0 && entries > 0);
+    return check(c, launch_raster(rp, p.numTiles, p.cfg, p.debug, s), "k_raster");
+}
+
+int upload_batch(sl_context* c, FrameRes& R, const Batch& B, hipStream_t s)
+{
+    int rc;
+    if (B.draws.empty()) return SL_OK;
+    if ((rc = check(c, hipMemcpyAsync(R.draws.p, B.draws.data(), sizeof(DrawDesc) * B.draws.size(), hipMemcpyHostToDevice, s), "upload draws")) != SL_OK) return rc;
+    if (!B.uni.empty() && (rc = check(c, hipMemcpyAsync(R.uni.p, B.uni.data(), B.uni.size(), hipMemcpyHostToDevice, s), "upload uniforms")) != SL_OK) return rc;
+    return SL_OK;
+}
+
+int execute_batch(sl_context* c)
+{
+    Batch& B = c->batch;
+    if (B.empty()) return SL_OK;
+    Plan p;
+    int rc = make_plan(c, B, p);
+    if (rc != SL_OK) return rc;
+    hipStream_t s = c->stream;
+    FrameRes& R = c->res;
+    FrameCtrl* ctrl = nullptr;
+    float msVertex = 0.f, msBin = 0.f, msRaster = 0.f;
+    uint32_t totalRecs = 0, totalPairs = 0, jobs = 0;
+
+    for (int attempt = 0; attempt < 4; ++attempt)
+    {
+        if ((rc = ensure_frame(c, R, p, B)) != SL_OK) return rc;
+        ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
+        if (attempt == 0 && (rc = upload_batch(c, R, B, s)) != SL_OK) return rc;
+        if (B.totalTris == 0)
+        {
+            if ((rc = enqueue_front(c, R, p, B, s)) != SL_OK) return rc;
+            break;
+        }
+        (void)hipEventRecord(c->ev[0], s);
+        if ((rc = enqueue_front(c, R, p, B, s)) != SL_OK) return rc;
+        (void)hipEventRecord(c->ev[1], s);
+        c->stats.setup_launches += 1;
+        if ((rc = check(c, hipMemcpyAsync(c->hCtrl, ctrl, sizeof(FrameCtrl), hipMemcpyDeviceToHost, s), "read ctrl")) != SL_OK) return rc;
+        if ((rc = check(c, hipStreamSynchronize(s), "k_setup sync")) != SL_OK) return rc;
+        if (c->hCtrl->error) return fail(c, SL_ERR_DEVICE, "setup look-back did not complete");
+        if (!c->hCtrl->overflow) break;
+        // grow to the exact requirement (+25%) and run setup again; the
+        // framebuffer has not been touched yet.
+        auto grow = [](uint64_t need, uint32_t cur) -> uint64_t { return std::max<uint64_t>(cur, need + need / 4 + 1024); };
+        const uint64_t nr = grow(c->hCtrl->totalRecs, R.recCap), np = grow(c->hCtrl->totalPairs, R.pairCap),
+                       nc = grow(c->hCtrl->clipJobs, R.clipCap);
+        if (nr >= (1ull << 30) || np >= (1ull << 31) || nc >= (1ull << 31)) return fail(c, SL_ERR_CAPACITY, "frame exceeds bin capacity");
+        R.recCap = (uint32_t)nr; R.pairCap = (uint32_t)np; R.clipCap = (uint32_t)nc;
+        if (attempt == 3) return fail(c, SL_ERR_CAPACITY, "bin capacity growth did not converge");
+    }
+    if (B.totalTris > 0)
+    {
+        float t = 0.f;
+        (void)hipEventElapsedTime(&t, c->ev[0], c->ev[1]);
+        msVertex += t;
+        c->stats.ms_setup_kernel += t;
+        totalRecs = c->hCtrl->totalRecs;
+        totalPairs = c->hCtrl->totalPairs;
+        jobs = c->hCtrl->clipJobs;
+    }
+    else
+    {
+        (void)hipEventRecord(c->ev[3], s);
+    }
+    if ((rc = enqueue_back(c, R, p, B, jobs, totalPairs, s, B.totalTris ? c->ev[2] : nullptr,
+                           B.totalTris ? c->ev[3] : nullptr, c->ev[4])) != SL_OK) return rc;
     (void)hipEventRecord(c->ev[5], s);
     if ((rc = check(c, hipMemcpyAsync(c->hCtrl, ctrl, sizeof(FrameCtrl), hipMemcpyDeviceToHost, s), "read ctrl")) != SL_OK) return rc;
     if ((rc = check(c, hipStreamSynchronize(s), "frame sync")) != SL_OK) return rc;
@@ -364,6 +444,88 @@ int execute_batch(sl_context* c)
     c->stats.ms_bin += msBin;
     c->stats.ms_raster += msRaster;
     c->stats.ms_total += msVertex + msBin + msRaster;
+    c->lastTotals[0] = totalRecs;
+    c->lastTotals[1] = totalPairs;
+    c->lastTotals[2] = jobs;
+    return SL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// HIP-graph frame: the pending batch, captured with private buffers sized from
+// an eager run of the same batch, replayed without host round trips.
+// ---------------------------------------------------------------------------
+void graph_release(sl_context* c)
+{
+    FrameGraph& G = c->graph;
+    if (G.exec) (void)hipGraphExecDestroy(G.exec);
+    if (G.graph) (void)hipGraphDestroy(G.graph);
+    G.exec = nullptr;
+    G.graph = nullptr;
+    for (DevBuf* b : {&G.res.draws, &G.res.uni, &G.res.recs, &G.res.vars, &G.res.keys, &G.res.vals, &G.res.sorted,
+                      &G.res.clip, &G.res.status, &G.res.ranges, &G.res.ctrl, &G.res.binCounts, &G.res.binTotals}) release(*b);
+    G.res = FrameRes{};
+    if (G.hCtrl) (void)hipHostFree(G.hCtrl);
+    G.hCtrl = nullptr;
+    G.valid = false;
+    G.pending = 0;
+}
+
+int graph_capture(sl_context* c)
+{
+    Batch B = c->batch;                      // keep a copy: executing the batch resets it
+    if (B.empty()) return fail(c, SL_ERR_STATE, "no pending draws to capture");
+    int rc = flush(c);                       // eager run: sizes buffers, produces the frame
+    if (rc != SL_OK) return rc;
+    graph_release(c);
+    FrameGraph& G = c->graph;
+    Plan p;
+    if ((rc = make_plan(c, B, p)) != SL_OK) return rc;
+    hipStream_t s = c->stream;
+    auto grow = [](uint64_t need) -> uint32_t { return (uint32_t)std::min<uint64_t>(need + need / 4 + 1024, (1ull << 30)); };
+    G.res.recCap = grow(c->lastTotals[0]);
+    G.res.pairCap = grow(c->lastTotals[1]);
+    G.res.clipCap = grow(c->lastTotals[2]);
+    if ((rc = ensure_frame(c, G.res, p, B)) != SL_OK) { graph_release(c); return rc; }
+    if ((rc = upload_batch(c, G.res, B, s)) != SL_OK) { graph_release(c); return rc; }
+    if (hipHostMalloc(reinterpret_cast<void**>(&G.hCtrl), sizeof(FrameCtrl), hipHostMallocDefault) != hipSuccess)
+    {
+        graph_release(c);
+        return fail(c, SL_ERR_OUT_OF_MEMORY, "pinned control block");
+    }
+    if ((rc = check(c, hipStreamSynchronize(s), "sync")) != SL_OK) { graph_release(c); return rc; }
+    if ((rc = check(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture")) != SL_OK) { graph_release(c); return rc; }
+    int rc1 = enqueue_front(c, G.res, p, B, s);
+    int rc2 = (rc1 == SL_OK) ? enqueue_back(c, G.res, p, B, G.res.clipCap, G.res.pairCap, s, nullptr, nullptr, nullptr) : rc1;
+    int rc3 = (rc2 == SL_OK) ? check(c, hipMemcpyAsync(G.hCtrl, G.res.ctrl.p, sizeof(FrameCtrl), hipMemcpyDeviceToHost, s), "read ctrl") : rc2;
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (rc3 != SL_OK) { if (graph) (void)hipGraphDestroy(graph); graph_release(c); return rc3; }
+    if ((rc = check(c, e, "end capture")) != SL_OK) { graph_release(c); return rc; }
+    G.graph = graph;
+    if ((rc = check(c, hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0), "instantiate graph")) != SL_OK) { graph_release(c); return rc; }
+    G.tris = B.totalTris;
+    G.valid = true;
+    return SL_OK;
+}
+
+int graph_collect(sl_context* c)
+{
+    FrameGraph& G = c->graph;
+    if (!G.valid || G.pending == 0) return SL_OK;
+    int rc = check(c, hipStreamSynchronize(c->stream), "graph sync");
+    if (rc != SL_OK) return rc;
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, c->ev[6], c->ev[7]);
+    c->stats.ms_total += t;
+    c->stats.frames += G.pending;
+    c->stats.triangles_submitted += G.tris * G.pending;
+    c->stats.triangles_binned += (uint64_t)G.hCtrl->totalRecs * G.pending;
+    c->stats.tile_pairs += (uint64_t)G.hCtrl->totalPairs * G.pending;
+    c->stats.fragments_tested += G.hCtrl->tested * G.pending;
+    c->stats.fragments_shaded += G.hCtrl->shaded * G.pending;
+    G.pending = 0;
+    if (G.hCtrl->error) return fail(c, SL_ERR_DEVICE, "graph frame: setup look-back did not complete");
+    if (G.hCtrl->overflow) return fail(c, SL_ERR_CAPACITY, "graph frame exceeded its captured capacity; capture again");
     return SL_OK;
 }
 
@@ -581,8 +743,9 @@ void sl_context_destroy(sl_context* c)
     for (auto& v : c->vbos) release(v.buf);
     for (auto& v : c->ibos) release(v.buf);
     for (auto& t : c->texs) release(t.buf);
-    for (DevBuf* b : {&c->dDraws, &c->dUni, &c->dRecs, &c->dVars, &c->dKeys[0], &c->dKeys[1], &c->dVals[0], &c->dVals[1],
-                      &c->dClip, &c->dStatus, &c->dRanges, &c->dCtrl, &c->dSortTemp}) release(*b);
+    graph_release(c);
+    for (DevBuf* b : {&c->res.draws, &c->res.uni, &c->res.recs, &c->res.vars, &c->res.keys, &c->res.vals, &c->res.sorted,
+                      &c->res.clip, &c->res.status, &c->res.ranges, &c->res.ctrl, &c->res.binCounts, &c->res.binTotals}) release(*b);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->hCtrl) (void)hipHostFree(c->hCtrl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -648,6 +811,7 @@ int sl_vbo_init(sl_context* c, size_t vbo, size_t numBytes, const void* data)
     if (!numBytes) return -1;                                   // SL_VertexBuffer.cpp:98-101
     int rc = flush(c);
     if (rc != SL_OK) return rc;
+    if (c->graph.valid) { (void)graph_collect(c); graph_release(c); }   // the graph points at the old buffer
     Vbo& v = c->vbos[vbo];
     if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     release(v.buf);
@@ -678,6 +842,7 @@ int sl_ibo_init(sl_context* c, size_t ibo, uint32_t numElements, int type, const
     if (!bpe) return SL_ERR_INVALID_ARG;
     int rc = flush(c);
     if (rc != SL_OK) return rc;
+    if (c->graph.valid) { (void)graph_collect(c); graph_release(c); }
     if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     Ibo& b = c->ibos[ibo];
     release(b.buf);
@@ -741,6 +906,7 @@ int sl_texture_init(sl_context* c, size_t tex, int type, uint16_t w, uint16_t h,
     if (!w || !h || !d) return SL_ERR_INVALID_ARG;
     int rc = flush(c);
     if (rc != SL_OK) return rc;
+    if (c->graph.valid) { (void)graph_collect(c); graph_release(c); }
     if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     Tex& t = c->texs[tex];
     release(t.buf);
@@ -933,9 +1099,41 @@ int sl_clear_framebuffer(sl_context* c, size_t fbo, unsigned att, const double r
 int sl_finish(sl_context* c)
 {
     if (!c) return SL_ERR_INVALID_ARG;
-    const int rc = flush(c);
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    rc = graph_collect(c);
     if (rc != SL_OK) return rc;
     return check(c, hipStreamSynchronize(c->stream), "finish");
+}
+
+int sl_frame_graph_capture(sl_context* c)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    int rc = graph_collect(c);
+    if (rc != SL_OK) return rc;
+    return graph_capture(c);
+}
+
+int sl_frame_graph_launch(sl_context* c)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    if (!c->graph.valid) return fail(c, SL_ERR_STATE, "no captured frame graph");
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    if (c->graph.pending == 0) (void)hipEventRecord(c->ev[6], c->stream);
+    rc = check(c, hipGraphLaunch(c->graph.exec, c->stream), "graph launch");
+    if (rc != SL_OK) return rc;
+    (void)hipEventRecord(c->ev[7], c->stream);
+    c->graph.pending += 1;
+    return SL_OK;
+}
+
+int sl_frame_graph_release(sl_context* c)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    const int rc = graph_collect(c);
+    graph_release(c);
+    return rc;
 }
 
 int sl_get_stats(const sl_context* c, sl_frame_stats* out)

@@ -163,7 +163,9 @@ class Scene:
         if clear:
             self.ctx.clear_framebuffer(self.fbo, 0, self.clear_rgba, self.clear_depth)
         for kind, mesh, n in self.draws:
-            if kind == "draw":
+            if kind == "draw_shader":
+                self.ctx.draw(mesh, n, self.fbo)
+            elif kind == "draw":
                 self.ctx.draw(mesh, self.shader, self.fbo)
             elif kind == "instanced":
                 self.ctx.draw_instanced(mesh, n, self.shader, self.fbo)
@@ -273,4 +275,126 @@ def build_sphere_scene(ctx, seed=1, n_instances=64, stacks=256, sectors=256, W=1
                   W=W, H=H, tex_size=tex_size, radius=radius, extent=extent,
                   triangles_per_instance=idx.size // 3, vertices_per_instance=verts.shape[0])
     s.info["_keep"] = (verts, idx, inst)
+    return s
+
+
+# --------------------------------------------------------------------------
+# batched multi-material frame
+# --------------------------------------------------------------------------
+_ICO_FACES = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+                       (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+                       (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)], dtype=np.int64)
+_ico_cache = {}
+
+
+def icosphere(level: int):
+    """Unit icosphere subdivided `level` times: (positions float64[V,3],
+    faces int64[20*4^level, 3]) with counter-clockwise outward faces."""
+    if level in _ico_cache:
+        return _ico_cache[level]
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+                  (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)], dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = _ICO_FACES.copy()
+    for _ in range(level):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]], axis=0)
+        key = np.minimum(e[:, 0], e[:, 1]) * (1 << 32) + np.maximum(e[:, 0], e[:, 1])
+        uniq, inv = np.unique(key, return_inverse=True)
+        a, b = uniq >> 32, uniq & 0xFFFFFFFF
+        mid = (v[a] + v[b]) * 0.5
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = v.shape[0]
+        v = np.concatenate([v, mid], axis=0)
+        n = f.shape[0]
+        ab, bc, ca = base + inv[:n], base + inv[n:2 * n], base + inv[2 * n:]
+        f = np.concatenate([np.stack([f[:, 0], ab, ca], 1), np.stack([f[:, 1], bc, ab], 1),
+                            np.stack([f[:, 2], ca, bc], 1), np.stack([ab, bc, ca], 1)], axis=0)
+    _ico_cache[level] = (v, f)
+    return v, f
+
+
+def rotation(axis, angle):
+    axis = np.asarray(axis, dtype=np.float64)
+    axis = axis / np.linalg.norm(axis)
+    x, y, z = axis
+    c, s, C = np.cos(angle), np.sin(angle), 1 - np.cos(angle)
+    R = np.array([[c + x * x * C, x * y * C - z * s, x * z * C + y * s],
+                  [y * x * C + z * s, c + y * y * C, y * z * C - x * s],
+                  [z * x * C - y * s, z * y * C + x * s, c + z * z * C]])
+    return R
+
+
+def build_batched_scene(ctx, seed=3, n_meshes=1000, median_tris=2000, sigma=1.0, min_tris=12, max_tris=200_000,
+                        W=1920, H=1080, n_textures=16, tex_size=512, blend_fraction=0.2, debug=False):
+    """1000 displaced icospheres (log-normal triangle counts), random affine
+    model matrices, 80% opaque / 20% alpha-blended Lambert materials over
+    sixteen RGBA8 noise textures; one draw call per mesh, in index order."""
+    rng_u = uniform01(seed, n_meshes * 16, stream=30).astype(np.float64).reshape(n_meshes, 16)
+    # log-normal counts via Box-Muller on the seeded stream
+    u1, u2 = np.clip(rng_u[:, 0], 1e-7, 1.0), rng_u[:, 1]
+    normal = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
+    counts = np.clip(np.round(median_tris * np.exp(sigma * normal)), min_tris, max_tris).astype(np.int64)
+    texs = [ctx.create_texture(A.SL_COLOR_RGBA_8U, tex_size, tex_size, 1, value_noise_rgba8(tex_size, seed * 97 + i, cells=8))
+            for i in range(n_textures)]
+    P = perspective(60.0, W / H, 0.1, 100.0)
+    L = np.array([-0.4, -1.0, -0.6])
+    L = (L / np.linalg.norm(L)).astype(np.float32)
+    vs_op = A.sl_vertex_shader(2, A.SL_CULL_BACK_FACE, A.SL_PROGRAM_LAMBERT_TEXTURED)
+    fs_op = A.sl_fragment_shader(2, 1, A.SL_BLEND_OFF, A.SL_DEPTH_TEST_LESS_THAN, A.SL_DEPTH_MASK_ON, A.SL_PROGRAM_LAMBERT_TEXTURED)
+    fs_bl = A.sl_fragment_shader(2, 1, A.SL_BLEND_ALPHA, A.SL_DEPTH_TEST_LESS_THAN, A.SL_DEPTH_MASK_OFF, A.SL_PROGRAM_LAMBERT_TEXTURED)
+    color, depth, fbo, dbg = make_target(ctx, W, H, debug)
+    draws = []
+    shaders = []
+    keep = []
+    level_ibo = {}
+    total = 0
+    n_blend = 0
+    for m in range(n_meshes):
+        n = int(counts[m])
+        level = 0
+        while 20 * 4 ** level < n:
+            level += 1
+        base_v, faces = icosphere(level)
+        if level not in level_ibo:
+            level_ibo[level] = faces.astype(np.uint32).reshape(-1)
+        disp = 1.0 + 0.15 * (uniform01(seed * 7919 + m, base_v.shape[0], stream=31).astype(np.float64) - 0.5)
+        pos = base_v * disp[:, None]
+        uv = np.stack([np.arctan2(base_v[:, 2], base_v[:, 0]) / (2 * np.pi) + 0.5,
+                       np.arcsin(np.clip(base_v[:, 1], -1, 1)) / np.pi + 0.5], axis=1)
+        verts = np.concatenate([pos, base_v, uv], axis=1).astype(np.float32)
+        vbo = ctx.create_vbo(verts)
+        ibo = ctx.create_ibo(level_ibo[level])
+        vao = ctx.create_vao(vbo, ibo, [(0, 32, A.VERTEX_DIMENSION_3, A.VERTEX_DATA_FLOAT),
+                                        (12, 32, A.VERTEX_DIMENSION_3, A.VERTEX_DATA_FLOAT),
+                                        (24, 32, A.VERTEX_DIMENSION_2, A.VERTEX_DATA_FLOAT)])
+        r = rng_u[m]
+        R = rotation([r[2] - 0.5, r[3] - 0.5, r[4] - 0.5 + 1e-3], r[5] * 2 * np.pi)
+        S = np.diag([0.4 + 1.6 * r[6], 0.4 + 1.6 * r[7], 0.4 + 1.6 * r[8]])
+        M = np.eye(4)
+        M[:3, :3] = R @ S
+        M[:3, 3] = [(r[9] - 0.5) * 36.0, (r[10] - 0.5) * 20.0, -6.0 - 40.0 * r[11]]
+        u = A.sl_std_uniforms()
+        u.mvp[:] = P.reshape(-1).tolist()
+        u.model[:] = M.T.astype(np.float32).reshape(-1).tolist()      # column-major
+        u.light_dir[:] = [float(L[0]), float(L[1]), float(L[2]), 0.0]
+        u.ambient[:] = [0.25, 0.25, 0.25, 0.0]
+        u.diffuse[:] = [0.75, 0.75, 0.75, 0.0]
+        blended = r[12] < blend_fraction
+        alpha = 0.2 + 0.6 * r[13] if blended else 1.0
+        u.tint[:] = [1.0, 1.0, 1.0, float(alpha)]
+        u.texture = texs[m % n_textures]
+        u.instance_vbo = A.SL_NO_RESOURCE
+        ubo = ctx.create_ubo(u)
+        sh = ctx.create_shader(vs_op, fs_bl if blended else fs_op, ubo)
+        shaders.append(sh)
+        n_blend += int(blended)
+        mesh = A.sl_mesh(vao, 0, 3 * n, A.RENDER_MODE_INDEXED_TRIANGLES, m)
+        draws.append(("draw_shader", mesh, sh))
+        total += n
+        keep.append(verts)
+    s = Scene(ctx, W, H, fbo, color, depth, shaders[0] if shaders else 0, draws, n_triangles=total, debug=dbg)
+    s.info.update(workload="batched_meshes", seed=seed, meshes=n_meshes, blended=n_blend, W=W, H=H,
+                  triangles=total, median_tris=median_tris, sigma=sigma)
+    s.info["_keep"] = keep
     return s
