@@ -63,9 +63,9 @@ __device__ __forceinline__ uint64_t fetch_index(const DrawDesc& d, uint64_t e)
     if (e >= d.iboCount) return 0;
     switch (d.iboType)
     {
-        case VERTEX_DATA_BYTE: return d.ibo[e];
-        case VERTEX_DATA_SHORT: return reinterpret_cast<const uint16_t*>(d.ibo)[e];
-        default: return reinterpret_cast<const uint32_t*>(d.ibo)[e];
+        case VERTEX_DATA_BYTE: return gptr(d.ibo)[e];
+        case VERTEX_DATA_SHORT: return gptr(reinterpret_cast<const uint16_t*>(d.ibo))[e];
+        default: return gptr(reinterpret_cast<const uint32_t*>(d.ibo))[e];
     }
 }
 
@@ -109,16 +109,11 @@ __device__ __forceinline__ void load_triangle_in(const SetupParams& P, uint64_t 
     T.status = clip_status(T.pos[0], T.pos[1], T.pos[2]);
 }
 
-// Most waves lie inside one draw: then the draw's descriptor is read through a
-// wave-uniform index (scalar loads); otherwise per lane.
+// Per-lane draw lookup (the clip kernel's jobs come from any draw).
 __device__ __forceinline__ void load_triangle(const SetupParams& P, uint64_t g, TriIn& T)
 {
     const uint32_t di = find_draw(P.draws, P.numDraws, g);
-    const uint32_t di0 = __builtin_amdgcn_readfirstlane(di);
-    if (__all(di == di0))
-        load_triangle_in(P, g, P.draws[di0], di0, T);
-    else
-        load_triangle_in(P, g, P.draws[di], di, T);
+    load_triangle_in(P, g, P.draws[di], di, T);
 }
 
 __device__ __forceinline__ float4 clip_edge(int e)
@@ -261,10 +256,23 @@ __global__ __launch_bounds__(kSetupBlock) void k_setup(SetupParams P)
     __shared__ unsigned long long sWave[kSetupBlock / 64];
     __shared__ unsigned long long sPrefix;
 
-    if (threadIdx.x == 0) sChunk = atomicAdd(&P.ctrl->ticket, 1u);
+    __shared__ uint32_t sDraw;
+    if (threadIdx.x == 0)
+    {
+        sChunk = atomicAdd(&P.ctrl->ticket, 1u);
+        // draws start on chunk boundaries, so one draw covers the whole chunk
+        sDraw = find_draw(P.draws, P.numDraws, (uint64_t)sChunk * kSetupBlock);
+    }
     __syncthreads();
     const uint32_t chunk = sChunk;
+    const uint32_t drawIdx = __builtin_amdgcn_readfirstlane(sDraw);
     const uint64_t g = (uint64_t)chunk * kSetupBlock + threadIdx.x;
+    // the chunk's draw descriptor, staged once in LDS for all lanes
+    __shared__ DrawDesc sD;
+    static_assert(sizeof(DrawDesc) % 4 == 0 && sizeof(DrawDesc) / 4 <= kSetupBlock, "draw descriptor staging");
+    if (threadIdx.x < sizeof(DrawDesc) / 4)
+        reinterpret_cast<uint32_t*>(&sD)[threadIdx.x] = reinterpret_cast<const uint32_t*>(P.draws + drawIdx)[threadIdx.x];
+    __syncthreads();
 
     TriIn T;
     T.status = 0;
@@ -274,12 +282,13 @@ __global__ __launch_bounds__(kSetupBlock) void k_setup(SetupParams P)
     bool clipped = false;
     float4 sp[3];
     TileRect tr{0, -1, 0, -1};
-    if (g < P.totalTris)
+    const DrawDesc& DU = sD;
+    if (g < P.totalTris && g - DU.triBase < DU.numTris)
     {
-        load_triangle(P, g, T);
+        load_triangle_in(P, g, DU, drawIdx, T);
         if (T.status == 3)
         {
-            const DrawDesc& D = P.draws[T.draw];
+            const DrawDesc& D = DU;
 #pragma unroll
             for (int k = 0; k < 3; ++k) sp[k] = divide_and_snap(T.pos[k], D.vp);
             if (bin_accept(sp[0], sp[1], sp[2]))
@@ -291,7 +300,7 @@ __global__ __launch_bounds__(kSetupBlock) void k_setup(SetupParams P)
         }
         else if (T.status == 1 && clip_quick(T.pos))
         {
-            const DrawDesc& D = P.draws[T.draw];
+            const DrawDesc& D = DU;
             float4 poly[9];
             const int n = clip_polygon<false>(T.pos, nullptr, 0, poly, nullptr);
             for (int k = 0; k < n; ++k) poly[k] = divide_and_snap(poly[k], D.vp);
@@ -394,8 +403,7 @@ __global__ __launch_bounds__(kSetupBlock) void k_setup(SetupParams P)
     }
     if (!clipped)
     {
-        const DrawDesc& D = P.draws[T.draw];
-        write_record(P, recOff, (uint32_t)pairOff, sp[0], sp[1], sp[2], T.var[0], T.var[1], T.var[2], D.nvary,
+        write_record(P, recOff, (uint32_t)pairOff, sp[0], sp[1], sp[2], T.var[0], T.var[1], T.var[2], DU.nvary,
                      T.draw, T.seq, tr);
     }
     else
@@ -476,7 +484,11 @@ __global__ __launch_bounds__(256) void k_bin_count(const uint32_t* __restrict__ 
     const uint32_t end = min(start + kBinBlock, n);
     for (int t = threadIdx.x; t < numTiles; t += blockDim.x) sCnt[t] = 0u;
     __syncthreads();
-    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) atomicAdd(&sCnt[keys[i]], 1u);
+    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x)
+    {
+        const uint32_t k = keys[i];
+        if (k < (uint32_t)numTiles) atomicAdd(&sCnt[k], 1u);
+    }
     __syncthreads();
     uint32_t* row = counts + (uint64_t)blockIdx.x * numTiles;
     for (int t = threadIdx.x; t < numTiles; t += blockDim.x) row[t] = sCnt[t];
@@ -552,8 +564,8 @@ __global__ __launch_bounds__(64) void k_bin_scatter(const uint32_t* __restrict__
     for (uint32_t c = start; c < end; c += 64)
     {
         const uint32_t i = c + lane;
-        const bool valid = i < end;
-        const uint32_t key = valid ? keys[i] : 0xFFFFFFFFu;
+        const uint32_t key = (i < end) ? keys[i] : 0xFFFFFFFFu;
+        const bool valid = key < (uint32_t)numTiles;
         const uint32_t val = valid ? vals[i] : 0u;
         unsigned long long remaining = __ballot(valid);
         while (remaining)
@@ -577,6 +589,15 @@ __global__ __launch_bounds__(64) void k_bin_scatter(const uint32_t* __restrict__
 // ---------------------------------------------------------------------------
 // k_raster: one wave per tile
 // ---------------------------------------------------------------------------
+// Catch-up distances up to kStepLimit groups are stepped; longer ones use the
+// binade jump (one out-of-line copy for the three barycentric components).
+constexpr int kStepLimit = 48;
+
+__device__ __noinline__ float3 recur_advance3(float3 s, float3 d, int k)
+{
+    return make_float3(sl_recur_advance(s.x, d.x, k), sl_recur_advance(s.y, d.y, k), sl_recur_advance(s.z, d.z, k));
+}
+
 // Orders one record's LDS updates before the next record's: a wave barrier
 // plus compiler fences when the tile is one wave (LDS operations of a wave
 // execute in order), a workgroup barrier otherwise.
@@ -683,10 +704,24 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                     float s1 = (fx * R.bx[1]) + fmaf(R.by[1], yf, R.bz[1]);
                     float s2 = (fx * R.bx[2]) + fmaf(R.by[2], yf, R.bz[2]);
                     const float d0 = R.bx[0] * 4.f, d1 = R.bx[1] * 4.f, d2 = R.bx[2] * 4.f;
-                    // advance to the tile's first group: exact k-step jump
-                    s0 = sl_recur_advance(s0, d0, g0);
-                    s1 = sl_recur_advance(s1, d1, g0);
-                    s2 = sl_recur_advance(s2, d2, g0);
+                    // advance to the segment's first group: plain steps when
+                    // few, the exact k-step jump when many
+                    if (g0 <= kStepLimit)
+                    {
+                        for (int gi = 0; gi < g0; ++gi)
+                        {
+                            s0 = s0 + d0;
+                            s1 = s1 + d1;
+                            s2 = s2 + d2;
+                        }
+                    }
+                    else
+                    {
+                        const float3 a = recur_advance3(make_float3(s0, s1, s2), make_float3(d0, d1, d2), g0);
+                        s0 = a.x;
+                        s1 = a.y;
+                        s2 = a.z;
+                    }
                     for (; x < xe; x += 4)
                     {
                         const float z = (((R.z[0] * s0) + (R.z[1] * s1)) + (R.z[2] * s2)) + R.bc3;

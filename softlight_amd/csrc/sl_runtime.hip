@@ -580,6 +580,9 @@ int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t s
 
     DrawDesc d;
     std::memset(&d, 0, sizeof(d));
+    // each draw starts on a setup-chunk boundary, so a setup workgroup only
+    // ever sees one draw
+    B.totalTris = (B.totalTris + kSetupBlock - 1) / kSetupBlock * kSetupBlock;
     d.triBase = B.totalTris;
     d.numTris = numTris;
     d.trisPerInstance = tpi;

@@ -13,6 +13,15 @@ namespace slg {
 
 struct Vary4 { float4 v[SL_SHADER_MAX_VARYING_VECTORS]; };
 
+// Pointers read from the draw table point to device global memory; casting
+// them to the global address space lets loads through them use global (not
+// flat) instructions.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gptr(const T* p)
+{
+    return (const __attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ float4 f4(float a, float b, float c, float d) { return make_float4(a, b, c, d); }
 
 // ls::math::mat4 * vec4, column-major: ((c0*x + c1*y) + c2*z) + c3*w
@@ -63,12 +72,18 @@ __device__ __forceinline__ float4 read_attr(const DrawDesc& d, int bind, uint64_
         {
             if ((off & 3) == 0)
             {
-                const float* p = reinterpret_cast<const float*>(d.vbo + off);
+                const auto p = gptr(reinterpret_cast<const float*>(d.vbo + off));
                 for (int i = 0; i < n; ++i) o[i] = p[i];
             }
             else
             {
-                for (int i = 0; i < n; ++i) __builtin_memcpy(&o[i], d.vbo + off + 4 * i, 4);
+                const auto p = gptr(d.vbo + off);
+                for (int i = 0; i < n; ++i)
+                {
+                    const uint32_t b = (uint32_t)p[4 * i] | ((uint32_t)p[4 * i + 1] << 8) | ((uint32_t)p[4 * i + 2] << 16) |
+                                       ((uint32_t)p[4 * i + 3] << 24);
+                    o[i] = __uint_as_float(b);
+                }
             }
         }
     }
@@ -76,7 +91,7 @@ __device__ __forceinline__ float4 read_attr(const DrawDesc& d, int bind, uint64_
     {
         if (off >= 0 && (uint64_t)off + (uint64_t)n <= d.vboBytes)
         {
-            const uint8_t* p = d.vbo + off;
+            const auto p = gptr(d.vbo + off);
             for (int i = 0; i < n; ++i) o[i] = (float)p[i] * 0.00392156862745f;   // color_cast<float,uint8_t>, SL_Color.hpp:666-669
         }
     }
@@ -90,13 +105,9 @@ __device__ __forceinline__ void model_matrix(const DrawDesc& d, const sl_std_uni
         const uint64_t off = inst * 64u;
         if (off + 64u <= d.instVboBytes)
         {
-            const float4* p = reinterpret_cast<const float4*>(d.instVbo + off);
+            const auto p = gptr(reinterpret_cast<const float*>(d.instVbo + off));
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-            {
-                const float4 col = p[c];
-                m[c * 4 + 0] = col.x; m[c * 4 + 1] = col.y; m[c * 4 + 2] = col.z; m[c * 4 + 3] = col.w;
-            }
+            for (int i = 0; i < 16; ++i) m[i] = p[i];
         }
         else
         {
@@ -328,7 +339,7 @@ __device__ __forceinline__ float bilerp(const BilinearTaps& t, float p0, float p
 __device__ __forceinline__ uint32_t sample_bilinear_rgba8(const uint8_t* tex, uint32_t W, uint32_t H, float u, float v)
 {
     const BilinearTaps t = bilinear_taps(W, H, u, v);
-    const uint32_t* px = reinterpret_cast<const uint32_t*>(tex);
+    const auto px = gptr(reinterpret_cast<const uint32_t*>(tex));
     const uint32_t q0 = px[t.i0], q1 = px[t.i1], q2 = px[t.i2], q3 = px[t.i3];
     uint32_t out = 0;
 #pragma unroll
@@ -348,10 +359,12 @@ __device__ __forceinline__ uint32_t sample_bilinear_rgba8(const uint8_t* tex, ui
 __device__ __forceinline__ float4 sample_bilinear_rgbaf(const uint8_t* tex, uint32_t W, uint32_t H, float u, float v)
 {
     const BilinearTaps t = bilinear_taps(W, H, u, v);
-    const float4* px = reinterpret_cast<const float4*>(tex);
-    const float4 a = px[t.i0], b = px[t.i1], c = px[t.i2], d = px[t.i3];
-    return f4(bilerp(t, a.x, b.x, c.x, d.x), bilerp(t, a.y, b.y, c.y, d.y),
-              bilerp(t, a.z, b.z, c.z, d.z), bilerp(t, a.w, b.w, c.w, d.w));
+    const auto px = gptr(reinterpret_cast<const float*>(tex));
+    float o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        o[k] = bilerp(t, px[4 * t.i0 + k], px[4 * t.i1 + k], px[4 * t.i2 + k], px[4 * t.i3 + k]);
+    return f4(o[0], o[1], o[2], o[3]);
 }
 
 // Built-in fragment programs (replace SL_FragmentShader::shader, SL_Shader.hpp:89-98)
