@@ -187,6 +187,8 @@ typedef struct sl_frame_stats {
     double   ms_clip_kernel;        /* the clip kernel alone (part of ms_vertex) */
     uint64_t setup_launches;        /* setup kernel launches (re-runs after a capacity grow included) */
     uint64_t raster_launches;       /* raster kernel launches */
+    double   ms_backward;           /* backward kernels (device time, counted at the next synchronisation) */
+    uint64_t backward_launches;     /* backward passes executed (counted likewise) */
 } sl_frame_stats;
 
 typedef struct sl_context sl_context;
@@ -213,6 +215,7 @@ int         sl_get_raster_config(const sl_context* ctx, int config, int* tileW, 
 size_t sl_create_vbo(sl_context* ctx);                                                      /* SL_Context::create_vbo */
 int    sl_vbo_init(sl_context* ctx, size_t vbo, size_t numBytes, const void* data);         /* SL_VertexBuffer::init, SL_VertexBuffer.cpp:96 */
 int    sl_vbo_assign(sl_context* ctx, size_t vbo, const void* data, size_t offset, size_t numBytes); /* SL_VertexBuffer::assign, SL_VertexBuffer.hpp */
+int    sl_vbo_read(sl_context* ctx, size_t vbo, void* out, size_t offset, size_t numBytes);         /* SL_VertexBuffer::data, SL_VertexBuffer.hpp:45-47,95-110 (read back) */
 
 /* ---- index buffers (SL_IndexBuffer.hpp / .cpp) -------------------------- */
 size_t sl_create_ibo(sl_context* ctx);                                                      /* SL_Context::create_ibo */
@@ -273,6 +276,42 @@ int sl_clear_framebuffer(sl_context* ctx, size_t fbo, unsigned attachment, const
 int sl_frame_graph_capture(sl_context* ctx);
 int sl_frame_graph_launch(sl_context* ctx);
 int sl_frame_graph_release(sl_context* ctx);
+
+/* ---- differentiable draws (no reference counterpart) ---------------------
+ * Fixed-visibility adjoint of the draw path for the programs whose output is
+ * smooth in their inputs: SL_PROGRAM_COLOR_TEXTURED (gradients to the vertex
+ * colours and to the RGBA32F texels) and SL_PROGRAM_PASSTHROUGH_GOURAUD
+ * (gradients to the vertex colours).
+ *
+ * sl_framebuffer_attach_visibility: an RGBA32F texture of the framebuffer's
+ *   size.  While attached, every fragment that passes the depth test writes
+ *   {seq, b0, b1, b2}: seq = the 1-based submitted-triangle number as the bit
+ *   pattern of the red channel (0 = no fragment), b = the fragment's
+ *   perspective-correct weights of the source triangle's three vertices,
+ *   carried through clipping like a varying.  Needs numVaryings <= 3 in the
+ *   shaders drawn (the weights use the next varying slot).  Clears write 0.
+ *   texId = SL_NO_RESOURCE detaches.
+ * sl_vbo_attach_gradient: a VBO receiving dL/d(colour attribute) as one
+ *   float4 per vertex id (binding 1 of the draws that read the VBO).
+ * sl_texture_attach_gradient: an RGBA32F texture of the same size receiving
+ *   dL/d(texel); bilinear taps at x = W / y = H land where the forward read
+ *   them (the next row's first texel / the padding).
+ * sl_gradients_zero: zeroes every attached gradient buffer.
+ * sl_loss_l2_grad: grad = scale * 2 * (color - target) for two RGBA32F
+ *   textures of one size (the gradient of scale * sum((color - target)^2)).
+ * sl_framebuffer_backward: adds the gradient of the framebuffer's last
+ *   executed batch, given dL/d(colour attachment 0) as an RGBA32F texture,
+ *   into the attached gradient buffers.  The batch must have run with a
+ *   visibility attachment, blending off, and resources unchanged since
+ *   (re-initialising a VBO, IBO or texture drops the saved batch).  Like
+ *   sl_loss_l2_grad and sl_gradients_zero it is queued on the context's
+ *   stream; readbacks and sl_finish wait for it. */
+int sl_framebuffer_attach_visibility(sl_context* ctx, size_t fboId, size_t texId);
+int sl_vbo_attach_gradient(sl_context* ctx, size_t vboId, size_t gradVboId);
+int sl_texture_attach_gradient(sl_context* ctx, size_t texId, size_t gradTexId);
+int sl_gradients_zero(sl_context* ctx);
+int sl_loss_l2_grad(sl_context* ctx, size_t colorTex, size_t targetTex, size_t gradImageTex, float scale);
+int sl_framebuffer_backward(sl_context* ctx, size_t fboId, size_t gradImageTex);
 
 /* ---- synchronisation and counters (no reference counterpart: the
  *      reference's draw is synchronous, SL_ProcessorPool.cpp:326) ---------- */

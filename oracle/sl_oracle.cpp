@@ -26,8 +26,18 @@
  *     is skipped instead of reading past elementEnd;
  *   - vertex reads beyond the end of a vertex buffer return zeros.
  *
- * Parity status of each part and the checks that pin it are listed at the top
- * of tests/test_oracle.py.
+ * Each part is pinned by a closed-form, exact-arithmetic or invariant check
+ * in tests/test_closed_forms.py and tests/test_recurrence.py (list at the top
+ * of test_closed_forms.py); the reference ships no golden images.
+ *
+ * Parity unpinned (no reference output and no mathematical fact fixes them;
+ * the pins listed above decide them, and the GPU path shares the choice):
+ *   - summation order and fusion inside LightMath's vec4*mat4, dot, sum, mix
+ *     and sum_inv (the LightMath submodule is absent);
+ *   - truncation (generic build) rather than rounding (AVX build) of span ends;
+ *   - rcp being the exact IEEE reciprocal;
+ *   - float->u8 conversion of values outside [0,255] (low byte of the
+ *     truncated int).
  */
 #include <algorithm>
 #include <atomic>
@@ -95,7 +105,7 @@ inline V4 mix4(const V4& a, const V4& b, float t)
 /*-----------------------------------------------------------------------------
  * Resources
 -----------------------------------------------------------------------------*/
-struct Vbo { bool valid = false; std::vector<uint8_t> bytes; };
+struct Vbo { bool valid = false; std::vector<uint8_t> bytes; size_t grad = SL_INVALID_ID; };
 
 struct Ibo { bool valid = false; int type = VERTEX_DATA_INT; uint32_t count = 0; uint32_t bpe = 4; std::vector<uint8_t> bytes; };
 
@@ -110,6 +120,19 @@ struct Tex
     uint16_t w = 0, h = 0, d = 0;
     uint32_t bpt = 0;
     std::vector<uint8_t> data;   // padded + zeroed as SL_Texture.cpp:28-55
+    size_t grad = SL_INVALID_ID;
+};
+
+// One (mesh, instance) pass drawn with a visibility attachment, kept for the
+// backward pass.
+struct SavedPass
+{
+    sl_mesh mesh;
+    size_t instance;
+    size_t numTris;
+    uint32_t seqBase;
+    size_t shader;
+    size_t tex;               // texture bound through the uniforms at draw time
 };
 
 struct Fbo
@@ -118,6 +141,10 @@ struct Fbo
     size_t depth = SL_INVALID_ID;
     size_t dbgLast = SL_INVALID_ID;
     size_t dbgCount = SL_INVALID_ID;
+    size_t vis = SL_INVALID_ID;
+    std::vector<SavedPass> saved;   // passes since the depth attachment was last cleared
+    bool savedStale = false;
+    bool savedBlend = false;
 };
 
 struct Ubo { uint8_t bytes[SL_MAX_UNIFORM_BUFFER_SIZE]; };
@@ -195,6 +222,7 @@ struct Target
     float* colorF;            // RGBA32F attachment 0, or null
     uint32_t* dbgLast;
     uint32_t* dbgCount;
+    float* vis;               // visibility attachment {seq bits, b0, b1, b2}, or null
 };
 
 struct Draw
@@ -208,6 +236,8 @@ struct Draw
     const Vao* vao;
     const Vbo* vbo;
     const Ibo* ibo;
+    unsigned nvary;           // the shader's varyings, plus the weight slot with a visibility attachment
+    int visSlot;              // varying slot of the source-vertex weights, or -1
     uint64_t tested = 0, shaded = 0, binned = 0;
 };
 
@@ -552,10 +582,10 @@ void shade_fragment(Draw& d, const Bin& b, int x, int y, float z, const float bc
 
     // interpolate_tri_varyings, generic branch :315-330
     V4 var[SL_SHADER_MAX_VARYING_VECTORS];
-    for (unsigned i = 0; i < d.sh->nvary; ++i)
+    for (unsigned i = 0; i < d.nvary; ++i)
         for (int k = 0; k < 4; ++k)
             var[i].v[k] = ((b.var[0][i].v[k] * c0) + (b.var[1][i].v[k] * c1)) + (b.var[2][i].v[k] * c2);
-    for (unsigned i = d.sh->nvary; i < SL_SHADER_MAX_VARYING_VECTORS; ++i) var[i] = v4(0.f, 0.f, 0.f, 0.f);
+    for (unsigned i = d.nvary; i < SL_SHADER_MAX_VARYING_VECTORS; ++i) var[i] = v4(0.f, 0.f, 0.f, 0.f);
 
     V4 out;
     const bool haveOutputs = run_fragment_program(d, var, out);
@@ -565,6 +595,13 @@ void shade_fragment(Draw& d, const Bin& b, int x, int y, float z, const float bc
     if (d.sh->dmask == SL_DEPTH_MASK_ON) d.tgt.depth[idx] = z;   // :526-529, regardless of the shader's return
     if (d.tgt.dbgLast) d.tgt.dbgLast[idx] = b.seq;
     if (d.tgt.dbgCount) d.tgt.dbgCount[idx] += 1u;
+    if (d.tgt.vis)
+    {
+        // the source triangle's vertex weights, interpolated like a varying
+        float* v = d.tgt.vis + idx * 4;
+        std::memcpy(&v[0], &b.seq, 4);
+        for (int k = 0; k < 3; ++k) v[1 + k] = var[d.visSlot].v[k];
+    }
     ++d.shaded;
 }
 
@@ -723,7 +760,7 @@ struct VertexStage
             bin.by.v[k] = ddy[k] * denom;
             bin.bz.v[k] = ddz[k] * denom;
         }
-        const unsigned nv = d.sh->nvary;
+        const unsigned nv = d.nvary;
         for (unsigned i = 0; i < nv; ++i)
         {
             bin.var[0][i] = a.var[i];
@@ -737,7 +774,7 @@ struct VertexStage
     // SL_TriProcessor::clip_and_process_tris (:370-519)
     void clip_and_process(uint32_t seq, const TVert& a, const TVert& b, const TVert& c)
     {
-        const unsigned nv = d.sh->nvary;
+        const unsigned nv = d.nvary;
         const V4 edges[6] = {
             v4(1.f, 0.f, 0.f, 1.f), v4(-1.f, 0.f, 0.f, 1.f),
             v4(0.f, 1.f, 0.f, 1.f), v4(0.f, -1.f, 0.f, 1.f),
@@ -807,6 +844,7 @@ struct VertexStage
         {
             for (auto& v : tv[k].var) v = v4(0.f, 0.f, 0.f, 0.f);
             tv[k].pos = run_vertex_program(d, ids[k], instanceId, tv[k].var);
+            if (d.visSlot >= 0) tv[k].var[d.visSlot] = v4(k == 0 ? 1.f : 0.f, k == 1 ? 1.f : 0.f, k == 2 ? 1.f : 0.f, 0.f);
         }
         const int cull = d.sh->cull;
         if (cull != SL_CULL_OFF)                       // (:597-612)
@@ -896,6 +934,19 @@ int setup_draw(sl_context* c, size_t shaderId, size_t fboId, Draw& d)
     }
     d.tgt.dbgLast = (f.dbgLast < c->texs.size()) ? reinterpret_cast<uint32_t*>(c->texs[f.dbgLast].data.data()) : nullptr;
     d.tgt.dbgCount = (f.dbgCount < c->texs.size()) ? reinterpret_cast<uint32_t*>(c->texs[f.dbgCount].data.data()) : nullptr;
+    d.tgt.vis = nullptr;
+    d.nvary = sh.nvary;
+    d.visSlot = -1;
+    if (f.vis < c->texs.size() && c->texs[f.vis].valid)
+    {
+        Tex& vt = c->texs[f.vis];
+        if (vt.type != SL_COLOR_RGBA_FLOAT || vt.w != dt.w || vt.h != dt.h) return SL_ERR_STATE;
+        if (d.tgt.colorF == nullptr && sh.nout > 0) return SL_ERR_UNSUPPORTED;
+        if (sh.nvary >= SL_SHADER_MAX_VARYING_VECTORS) return SL_ERR_UNSUPPORTED;
+        d.tgt.vis = reinterpret_cast<float*>(vt.data.data());
+        d.visSlot = sh.nvary;
+        d.nvary = sh.nvary + 1u;
+    }
     viewport_setup(c, (float)dt.w, (float)dt.h, d.scissorMat, d.vpDims);
     return SL_OK;
 }
@@ -1036,6 +1087,14 @@ int run_draws(sl_context* c, const sl_mesh* meshes, size_t numMeshes, size_t num
         if (rc != SL_OK) return rc;
         const size_t n = (m.elementEnd > m.elementBegin) ? (m.elementEnd - m.elementBegin) / 3 : 0;   // (:569-574)
         passes.push_back(Pass{&m, (numInstances == 1) ? 0 : it, total, n, c->triSeq});
+        if (proto.tgt.vis && n > 0)
+        {
+            Fbo& f = c->fbos[fboId];
+            size_t tex = SL_INVALID_ID;
+            if (proto.uni) tex = reinterpret_cast<const sl_std_uniforms*>(proto.uni)->texture;
+            f.saved.push_back(SavedPass{m, (numInstances == 1) ? 0 : it, n, c->triSeq, shaderId, tex});
+            if (proto.sh->blend != SL_BLEND_OFF) f.savedBlend = true;
+        }
         c->triSeq += (uint32_t)n;
         c->stats.triangles_submitted += n;
         total += n;
@@ -1109,6 +1168,180 @@ int clear_depth(sl_context* c, size_t fbo, double depth)
     clear_texture_bytes(t, &v, 4);
     if (f.dbgLast < c->texs.size()) { const uint32_t z = 0; clear_texture_bytes(c->texs[f.dbgLast], &z, 4); }
     if (f.dbgCount < c->texs.size()) { const uint32_t z = 0; clear_texture_bytes(c->texs[f.dbgCount], &z, 4); }
+    if (f.vis < c->texs.size()) { const float z[4] = {0.f, 0.f, 0.f, 0.f}; clear_texture_bytes(c->texs[f.vis], z, 16); }
+    f.saved.clear();
+    f.savedStale = false;
+    f.savedBlend = false;
+    return SL_OK;
+}
+
+// Resources changed under the saved passes: the backward pass refuses them
+// (the GPU library keeps device pointers in its saved draws).
+void stale_saved(sl_context* c)
+{
+    for (Fbo& f : c->fbos)
+        if (!f.saved.empty()) f.savedStale = true;
+}
+
+/*-----------------------------------------------------------------------------
+ * Fixed-visibility adjoint (no reference counterpart: the reference renders
+ * forward only).  The scalar restatement of the backward kernels: for every
+ * pixel whose visibility record names a triangle, rebuild the winning
+ * fragment's varyings from the stored vertex weights b in the forward's
+ * operation order, then add
+ *   dL/d colour[v_k] += b_k * (dL/dC . texel)
+ *   dL/d texel[tap_j] += w_j * (dL/dC . colour)
+ * Each product is formed in double from the float forward values and summed
+ * in double in pixel order; the sums are added to the float gradient buffers
+ * at the end.
+-----------------------------------------------------------------------------*/
+struct Taps { size_t i[4]; float w[4]; };
+
+Taps bilinear_taps(const Tex& t, float u, float v)
+{
+    // same arithmetic as sample_bilinear_rgbaf (SL_Sampler.hpp:186-218)
+    const float xf = wrap_repeat(u) * (float)t.w;
+    const float yf = wrap_repeat(v) * (float)t.h;
+    const uint16_t xi0 = to_u16(xf);
+    const uint16_t yi0 = to_u16(yf);
+    const uint16_t xi1 = (uint16_t)std::min<uint32_t>((uint32_t)xi0 + 1u, t.w);
+    const uint16_t yi1 = (uint16_t)std::min<uint32_t>((uint32_t)yi0 + 1u, t.h);
+    const float dx = xf - (float)xi0;
+    const float dy = yf - (float)yi0;
+    const float omdx = 1.f - dx;
+    const float omdy = 1.f - dy;
+    Taps r;
+    r.i[0] = (size_t)xi0 + (size_t)t.w * yi0;
+    r.i[1] = (size_t)xi0 + (size_t)t.w * yi1;
+    r.i[2] = (size_t)xi1 + (size_t)t.w * yi0;
+    r.i[3] = (size_t)xi1 + (size_t)t.w * yi1;
+    r.w[0] = omdx; r.w[1] = omdx; r.w[2] = dx; r.w[3] = dx;      // x factor of each tap
+    (void)omdy; (void)dy;
+    return r;
+}
+
+int backward(sl_context* c, size_t fboId, size_t gradImageTex)
+{
+    if (fboId >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    Fbo& f = c->fbos[fboId];
+    if (f.vis >= c->texs.size() || !c->texs[f.vis].valid) return SL_ERR_STATE;
+    const Tex& vt = c->texs[f.vis];
+    if (gradImageTex >= c->texs.size() || !c->texs[gradImageTex].valid) return SL_ERR_INVALID_HANDLE;
+    const Tex& gt = c->texs[gradImageTex];
+    if (gt.type != SL_COLOR_RGBA_FLOAT || gt.w != vt.w || gt.h != vt.h) return SL_ERR_INVALID_ARG;
+    if (f.savedStale) return SL_ERR_STATE;
+    if (f.savedBlend) return SL_ERR_UNSUPPORTED;
+    if (f.saved.empty()) return SL_OK;
+    for (const SavedPass& sp : f.saved)
+    {
+        const Shader& sh = c->shaders[sp.shader];
+        const bool ct = sh.vsProg == SL_PROGRAM_COLOR_TEXTURED && sh.fsProg == SL_PROGRAM_COLOR_TEXTURED;
+        const bool pg = sh.vsProg == SL_PROGRAM_PASSTHROUGH_GOURAUD && sh.fsProg == SL_PROGRAM_PASSTHROUGH_GOURAUD;
+        if (!ct && !pg) return SL_ERR_UNSUPPORTED;
+    }
+    // double accumulators, one per destination buffer
+    std::vector<std::vector<double>> colAcc(c->vbos.size()), texAcc(c->texs.size());
+    const float* vis = reinterpret_cast<const float*>(vt.data.data());
+    const float* gimg = reinterpret_cast<const float*>(gt.data.data());
+    const size_t npx = (size_t)vt.w * vt.h;
+    for (size_t p = 0; p < npx; ++p)
+    {
+        uint32_t seq;
+        std::memcpy(&seq, &vis[4 * p], 4);
+        if (seq == 0) continue;
+        // last pass whose first sequence number is below seq
+        size_t lo = 0, hi = f.saved.size();
+        while (hi - lo > 1)
+        {
+            const size_t mid = (lo + hi) / 2;
+            if (f.saved[mid].seqBase < seq) lo = mid;
+            else hi = mid;
+        }
+        const SavedPass& sp = f.saved[lo];
+        if (seq <= sp.seqBase) continue;
+        const size_t t = (size_t)(seq - 1u - sp.seqBase);
+        if (t >= sp.numTris) continue;
+        Draw d;
+        d.ctx = c;
+        d.sh = &c->shaders[sp.shader];
+        if (bind_mesh(c, sp.mesh, d) != SL_OK) continue;
+        const size_t e = sp.mesh.elementBegin + 3 * t;
+        size_t vid[3] = {e, e + 1, e + 2};
+        if (sp.mesh.mode == RENDER_MODE_INDEXED_TRIANGLES)
+            for (int k = 0; k < 3; ++k) vid[k] = index_at(*d.ibo, e + (size_t)k);
+        const float b[3] = {vis[4 * p + 1], vis[4 * p + 2], vis[4 * p + 3]};
+        const float* g = &gimg[4 * p];
+        const bool ct = d.sh->fsProg == SL_PROGRAM_COLOR_TEXTURED;
+
+        float a[3][4], uv[3][4];
+        for (int k = 0; k < 3; ++k)
+        {
+            read_attr(d, 1, vid[k], a[k], ct ? 0.f : 1.f);
+            if (ct) a[k][3] = 1.f;                                 // varying0 = (rgb, 1)
+            if (ct) read_attr(d, 2, vid[k], uv[k], 0.f);
+        }
+        float col[4];
+        for (int ch = 0; ch < 4; ++ch) col[ch] = ((a[0][ch] * b[0]) + (a[1][ch] * b[1])) + (a[2][ch] * b[2]);
+        float tex[4] = {1.f, 1.f, 1.f, 1.f};
+        const Tex* T = nullptr;
+        Taps tp{};
+        float dy = 0.f;
+        if (ct && sp.tex < c->texs.size() && c->texs[sp.tex].valid && c->texs[sp.tex].type == SL_COLOR_RGBA_FLOAT)
+        {
+            T = &c->texs[sp.tex];
+            const float u = ((uv[0][0] * b[0]) + (uv[1][0] * b[1])) + (uv[2][0] * b[2]);
+            const float v = ((uv[0][1] * b[0]) + (uv[1][1] * b[1])) + (uv[2][1] * b[2]);
+            sample_bilinear_rgbaf(*T, u, v, tex);
+            tp = bilinear_taps(*T, u, v);
+            dy = wrap_repeat(v) * (float)T->h;
+            dy = dy - (float)to_u16(dy);
+        }
+        // colour attribute gradients
+        const Vbo& V = c->vbos[c->vaos[sp.mesh.vaoId].vbo];
+        if (V.grad < c->vbos.size() && c->vbos[V.grad].valid)
+        {
+            const size_t gv = V.grad;
+            const size_t cnt = c->vbos[gv].bytes.size() / 16;
+            std::vector<double>& acc = colAcc[gv];
+            if (acc.empty()) acc.assign(cnt * 4, 0.0);
+            for (int k = 0; k < 3; ++k)
+            {
+                if (vid[k] >= cnt) continue;
+                for (int ch = 0; ch < (ct ? 3 : 4); ++ch)
+                    acc[vid[k] * 4 + ch] += (double)b[k] * ((double)g[ch] * (double)tex[ch]);
+            }
+        }
+        // texel gradients
+        if (T && T->grad < c->texs.size() && c->texs[T->grad].valid)
+        {
+            const size_t gi = T->grad;
+            const size_t cnt = c->texs[gi].data.size() / 16;
+            std::vector<double>& acc = texAcc[gi];
+            if (acc.empty()) acc.assign(cnt * 4, 0.0);
+            const double omdy = 1.0 - (double)dy;
+            const double wy[4] = {omdy, (double)dy, omdy, (double)dy};
+            for (int j = 0; j < 4; ++j)
+            {
+                if (tp.i[j] >= cnt) continue;
+                const double wj = (double)tp.w[j] * wy[j];
+                for (int ch = 0; ch < 4; ++ch)
+                    acc[tp.i[j] * 4 + ch] += wj * ((double)g[ch] * (double)col[ch]);
+            }
+        }
+    }
+    for (size_t i = 0; i < colAcc.size(); ++i)
+    {
+        if (colAcc[i].empty()) continue;
+        float* dst = reinterpret_cast<float*>(c->vbos[i].bytes.data());
+        for (size_t k = 0; k < colAcc[i].size(); ++k) dst[k] += (float)colAcc[i][k];
+    }
+    for (size_t i = 0; i < texAcc.size(); ++i)
+    {
+        if (texAcc[i].empty()) continue;
+        float* dst = reinterpret_cast<float*>(c->texs[i].data.data());
+        for (size_t k = 0; k < texAcc[i].size(); ++k) dst[k] += (float)texAcc[i][k];
+    }
+    c->stats.backward_launches += 1;
     return SL_OK;
 }
 
@@ -1182,6 +1415,7 @@ int slo_vbo_init(sl_context* c, size_t vbo, size_t numBytes, const void* data)
 {
     if (vbo >= c->vbos.size()) return SL_ERR_INVALID_HANDLE;
     if (!numBytes) return -1;
+    stale_saved(c);
     Vbo& v = c->vbos[vbo];
     v.bytes.assign(numBytes, 0);
     if (data) std::memcpy(v.bytes.data(), data, numBytes);
@@ -1198,12 +1432,22 @@ int slo_vbo_assign(sl_context* c, size_t vbo, const void* data, size_t offset, s
     return SL_OK;
 }
 
+int slo_vbo_read(sl_context* c, size_t vbo, void* out, size_t offset, size_t numBytes)
+{
+    if (vbo >= c->vbos.size() || !c->vbos[vbo].valid) return SL_ERR_INVALID_HANDLE;
+    const Vbo& v = c->vbos[vbo];
+    if (offset + numBytes > v.bytes.size() || !out) return SL_ERR_INVALID_ARG;
+    std::memcpy(out, v.bytes.data() + offset, numBytes);
+    return SL_OK;
+}
+
 size_t slo_create_ibo(sl_context* c) { c->ibos.emplace_back(); return c->ibos.size() - 1; }
 
 int slo_ibo_init(sl_context* c, size_t ibo, uint32_t numElements, int type, const void* data)
 {
     if (ibo >= c->ibos.size()) return SL_ERR_INVALID_HANDLE;
     if (!numElements) return -1;
+    stale_saved(c);
     const uint32_t bpe = bytes_per_index(type);
     if (!bpe) return SL_ERR_INVALID_ARG;
     Ibo& b = c->ibos[ibo];
@@ -1262,6 +1506,7 @@ int slo_texture_init(sl_context* c, size_t tex, int type, uint16_t w, uint16_t h
     const uint32_t bpt = bytes_per_color(type);
     if (!bpt) return SL_ERR_UNSUPPORTED;
     if (!w || !h || !d) return SL_ERR_INVALID_ARG;
+    stale_saved(c);
     Tex& t = c->texs[tex];
     t.type = type; t.w = w; t.h = h; t.d = d; t.bpt = bpt;
     // SL_Texture.cpp:28-55: each dimension padded by (4 - dim % 4), zero filled
@@ -1425,6 +1670,82 @@ int slo_finish(sl_context*) { return SL_OK; }
 int slo_frame_graph_capture(sl_context*) { return SL_OK; }
 int slo_frame_graph_launch(sl_context*) { return SL_ERR_UNSUPPORTED; }
 int slo_frame_graph_release(sl_context*) { return SL_OK; }
+
+/* differentiable draws (no reference counterpart) */
+int slo_framebuffer_attach_visibility(sl_context* c, size_t fbo, size_t tex)
+{
+    if (!c || fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    if (tex != SL_INVALID_ID)
+    {
+        if (tex >= c->texs.size() || !c->texs[tex].valid) return SL_ERR_INVALID_HANDLE;
+        if (c->texs[tex].type != SL_COLOR_RGBA_FLOAT) return SL_ERR_INVALID_ARG;
+    }
+    Fbo& f = c->fbos[fbo];
+    f.vis = tex;
+    f.saved.clear();
+    f.savedStale = false;
+    f.savedBlend = false;
+    return SL_OK;
+}
+
+int slo_vbo_attach_gradient(sl_context* c, size_t vbo, size_t grad)
+{
+    if (!c || vbo >= c->vbos.size()) return SL_ERR_INVALID_HANDLE;
+    if (grad != SL_INVALID_ID && (grad >= c->vbos.size() || grad == vbo)) return SL_ERR_INVALID_HANDLE;
+    c->vbos[vbo].grad = grad;
+    return SL_OK;
+}
+
+int slo_texture_attach_gradient(sl_context* c, size_t tex, size_t grad)
+{
+    if (!c || tex >= c->texs.size()) return SL_ERR_INVALID_HANDLE;
+    if (grad != SL_INVALID_ID)
+    {
+        if (grad >= c->texs.size() || grad == tex || !c->texs[grad].valid || !c->texs[tex].valid) return SL_ERR_INVALID_HANDLE;
+        const Tex& g = c->texs[grad];
+        const Tex& t = c->texs[tex];
+        if (g.type != SL_COLOR_RGBA_FLOAT || g.w != t.w || g.h != t.h || g.d != t.d) return SL_ERR_INVALID_ARG;
+    }
+    c->texs[tex].grad = grad;
+    return SL_OK;
+}
+
+int slo_gradients_zero(sl_context* c)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    for (const Vbo& v : c->vbos)
+        if (v.valid && v.grad < c->vbos.size() && c->vbos[v.grad].valid)
+            std::fill(c->vbos[v.grad].bytes.begin(), c->vbos[v.grad].bytes.end(), 0);
+    for (const Tex& t : c->texs)
+        if (t.valid && t.grad < c->texs.size() && c->texs[t.grad].valid)
+            std::fill(c->texs[t.grad].data.begin(), c->texs[t.grad].data.end(), 0);
+    return SL_OK;
+}
+
+int slo_loss_l2_grad(sl_context* c, size_t colorTex, size_t targetTex, size_t gradTex, float scale)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    for (size_t id : {colorTex, targetTex, gradTex})
+        if (id >= c->texs.size() || !c->texs[id].valid) return SL_ERR_INVALID_HANDLE;
+    const Tex& a = c->texs[colorTex];
+    const Tex& b = c->texs[targetTex];
+    Tex& g = c->texs[gradTex];
+    for (const Tex* t : {&a, &b, (const Tex*)&g})
+        if (t->type != SL_COLOR_RGBA_FLOAT || t->w != a.w || t->h != a.h || t->d != a.d) return SL_ERR_INVALID_ARG;
+    const size_t n = (size_t)a.w * a.h * std::max<uint16_t>(a.d, 1) * 4;
+    const float* pa = reinterpret_cast<const float*>(a.data.data());
+    const float* pb = reinterpret_cast<const float*>(b.data.data());
+    float* pg = reinterpret_cast<float*>(g.data.data());
+    const float s2 = scale * 2.f;
+    for (size_t i = 0; i < n; ++i) pg[i] = s2 * (pa[i] - pb[i]);
+    return SL_OK;
+}
+
+int slo_framebuffer_backward(sl_context* c, size_t fbo, size_t gradImageTex)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    return backward(c, fbo, gradImageTex);
+}
 
 int slo_get_stats(const sl_context* c, sl_frame_stats* out)
 {

@@ -74,7 +74,8 @@ class sl_frame_stats(C.Structure):
                 ("fragments_shaded", C.c_uint64), ("frames", C.c_uint64),
                 ("ms_vertex", C.c_double), ("ms_bin", C.c_double), ("ms_raster", C.c_double),
                 ("ms_total", C.c_double), ("ms_setup_kernel", C.c_double), ("ms_clip_kernel", C.c_double),
-                ("setup_launches", C.c_uint64), ("raster_launches", C.c_uint64)]
+                ("setup_launches", C.c_uint64), ("raster_launches", C.c_uint64),
+                ("ms_backward", C.c_double), ("backward_launches", C.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -97,6 +98,7 @@ _SIGS = {
     "create_vbo": (_SZ, [_P]),
     "vbo_init": (_I, [_P, _SZ, _SZ, _P]),
     "vbo_assign": (_I, [_P, _SZ, _P, _SZ, _SZ]),
+    "vbo_read": (_I, [_P, _SZ, _P, _SZ, _SZ]),
     "create_ibo": (_SZ, [_P]),
     "ibo_init": (_I, [_P, _SZ, C.c_uint32, _I, _P]),
     "create_vao": (_SZ, [_P]),
@@ -127,6 +129,12 @@ _SIGS = {
     "frame_graph_capture": (_I, [_P]),
     "frame_graph_launch": (_I, [_P]),
     "frame_graph_release": (_I, [_P]),
+    "framebuffer_attach_visibility": (_I, [_P, _SZ, _SZ]),
+    "vbo_attach_gradient": (_I, [_P, _SZ, _SZ]),
+    "texture_attach_gradient": (_I, [_P, _SZ, _SZ]),
+    "gradients_zero": (_I, [_P]),
+    "loss_l2_grad": (_I, [_P, _SZ, _SZ, _SZ, C.c_float]),
+    "framebuffer_backward": (_I, [_P, _SZ, _SZ]),
     "get_stats": (_I, [_P, C.POINTER(sl_frame_stats)]),
     "reset_stats": (_I, [_P]),
 }
@@ -222,6 +230,11 @@ class SLContext:
             self._check(self.lib.vbo_init(self.h, vbo, arr.nbytes, p), "vbo_init")
         return vbo
 
+    def vbo_read(self, vbo, nbytes, offset=0, dtype=np.uint8):
+        out = np.empty(nbytes, dtype=np.uint8)
+        self._check(self.lib.vbo_read(self.h, vbo, out.ctypes.data_as(C.c_void_p), offset, nbytes), "vbo_read")
+        return out.view(dtype)
+
     def vbo_assign(self, vbo, data, offset=0):
         arr, p = _buf(data)
         return self._check(self.lib.vbo_assign(self.h, vbo, p, offset, arr.nbytes), "vbo_assign")
@@ -286,6 +299,25 @@ class SLContext:
     def attach_debug_buffers(self, fbo, last_tex, count_tex):
         return self._check(self.lib.fbo_attach_debug_buffers(self.h, fbo, last_tex, count_tex),
                            "fbo_attach_debug_buffers")
+
+    # ---- differentiable draws ----
+    def attach_visibility(self, fbo, tex):
+        return self._check(self.lib.framebuffer_attach_visibility(self.h, fbo, tex), "framebuffer_attach_visibility")
+
+    def vbo_attach_gradient(self, vbo, grad_vbo):
+        return self._check(self.lib.vbo_attach_gradient(self.h, vbo, grad_vbo), "vbo_attach_gradient")
+
+    def texture_attach_gradient(self, tex, grad_tex):
+        return self._check(self.lib.texture_attach_gradient(self.h, tex, grad_tex), "texture_attach_gradient")
+
+    def gradients_zero(self):
+        return self._check(self.lib.gradients_zero(self.h), "gradients_zero")
+
+    def loss_l2_grad(self, color_tex, target_tex, grad_tex, scale=1.0):
+        return self._check(self.lib.loss_l2_grad(self.h, color_tex, target_tex, grad_tex, scale), "loss_l2_grad")
+
+    def backward(self, fbo, grad_image_tex):
+        return self._check(self.lib.framebuffer_backward(self.h, fbo, grad_image_tex), "framebuffer_backward")
 
     # ---- uniforms / shaders ----
     def create_ubo(self, data=None):

@@ -68,6 +68,9 @@ struct BindingDesc
     int32_t type;      // SL_DataType
 };
 
+// DrawDesc::visSlot when the target has no visibility attachment.
+constexpr uint32_t kNoVisSlot = 0xFFu;
+
 struct DrawDesc
 {
     uint64_t triBase;          // first global triangle of this draw
@@ -88,6 +91,7 @@ struct DrawDesc
     uint32_t seqBase;          // debug sequence number of the draw's first triangle
     uint32_t numBindings;
     uint32_t vsProg, fsProg;
+    uint32_t visSlot;          // varying slot carrying the vertex weights, or kNoVisSlot
     uint8_t  indexed, cull, dtest, dmask, blend, nvary, nout, hasUniforms;
     BindingDesc bind[SL_VAO_MAX_BINDINGS];
     float scissor[16];         // SL_ViewportState::scissor_matrix at draw time
@@ -172,11 +176,34 @@ struct RasterParams
     float* depth;
     uint32_t* dbgLast;
     uint32_t* dbgCount;
+    float4* vis;                  // visibility attachment {seq bits, b0, b1, b2} or null
     int32_t clearColor, clearDepth;   // fused clear of this batch
     uint32_t clearRGBA8;
     float clearRGBAF[4];
     float clearDepthValue;
     FrameCtrl* ctrl;
+};
+
+// ---------------------------------------------------------------------------
+// Backward pass (fixed-visibility adjoint)
+// ---------------------------------------------------------------------------
+// One draw of the saved forward batch with its gradient destinations.
+struct GradDesc
+{
+    DrawDesc d;
+    float* gradCol;            // float4 per vertex id (dL/d colour attribute), or null
+    uint64_t gradColCount;     // float4 entries in gradCol
+    float* gradTex;            // RGBA32F gradient of the sampled texture, or null
+    uint64_t gradTexTexels;    // texels in the gradient allocation (padding included)
+};
+
+struct BackwardParams
+{
+    const GradDesc* draws;     // ordered by seqBase
+    uint32_t numDraws;
+    const float4* vis;         // visibility attachment
+    const float4* gradImage;   // dL/d colour attachment 0
+    int32_t W, H;
 };
 
 }  // namespace slg

@@ -9,9 +9,14 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, uint64_t n, hipStream_t s);
 hipError_t launch_setup(const SetupParams& p, uint64_t chunks, hipStream_t s);
 hipError_t launch_clip(const SetupParams& p, uint32_t jobs, hipStream_t s);
-uint32_t bin_blocks(uint32_t entries);
+// Binning splits the entries into at most kMaxBinBlocks blocks of `per`
+// entries (a multiple of 64, at least kMinBinBlock).
+constexpr uint32_t kMaxBinBlocks = 256;
+constexpr uint32_t kMinBinBlock = 1024;
+struct BinPlan { uint32_t blocks; uint32_t per; };
+BinPlan bin_plan(uint32_t entries);
 hipError_t launch_binning(const uint32_t* keys, const uint32_t* vals, const FrameCtrl* ctrl, uint32_t cap,
-                          uint32_t gridBlocks, uint32_t* counts, uint32_t* totals, uint2* range, uint32_t* out,
+                          BinPlan plan, uint32_t* counts, uint32_t* totals, uint2* range, uint32_t* out,
                           int numTiles, hipStream_t s);
 hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s);
 

@@ -58,16 +58,6 @@ __device__ __forceinline__ uint32_t find_draw(const DrawDesc* __restrict__ draws
     return lo;
 }
 
-__device__ __forceinline__ uint64_t fetch_index(const DrawDesc& d, uint64_t e)
-{
-    if (e >= d.iboCount) return 0;
-    switch (d.iboType)
-    {
-        case VERTEX_DATA_BYTE: return gptr(d.ibo)[e];
-        case VERTEX_DATA_SHORT: return gptr(reinterpret_cast<const uint16_t*>(d.ibo))[e];
-        default: return gptr(reinterpret_cast<const uint32_t*>(d.ibo))[e];
-    }
-}
 
 struct TriIn
 {
@@ -97,6 +87,16 @@ __device__ __forceinline__ void load_triangle_in(const SetupParams& P, uint64_t 
     T.seq = D.seqBase + 1u + (uint32_t)t;
 #pragma unroll
     for (int k = 0; k < 3; ++k) T.pos[k] = vertex_program(D, P.uniforms, vid[k], inst, T.var[k]);
+    if (D.visSlot != kNoVisSlot)
+    {
+        // the source triangle's vertex weights travel as one more varying, so
+        // clipping and perspective correction treat them like the others
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+                if ((uint32_t)i == D.visSlot) T.var[k].v[i] = f4(k == 0 ? 1.f : 0.f, k == 1 ? 1.f : 0.f, k == 2 ? 1.f : 0.f, 0.f);
+    }
 
     if (D.cull != SL_CULL_OFF)
     {
@@ -459,29 +459,29 @@ __global__ __launch_bounds__(64) void k_clip(SetupParams P)
 // ---------------------------------------------------------------------------
 // Ordered binning: a stable counting sort of the (tile, record) entries, which
 // k_setup/k_clip emit in record order, into per-tile lists that keep that
-// order.  Entries are split into blocks of kBinBlock; k_bin_count histograms
-// each block per tile, k_bin_colscan turns every tile's column of block counts
-// into block offsets and tile totals, k_bin_tilescan scans the totals into
-// tile ranges, and k_bin_scatter (one wave per block) places each block's
-// entries with ranks from wave ballots, chunk by chunk, so equal tiles keep
-// their entry order.  Entry and block counts are read from the device, so no
+// order.  Entries are split into at most kMaxBinBlocks blocks; k_bin_count
+// histograms each block per tile, k_bin_colscan turns every tile's column of
+// block counts into block offsets and tile totals, k_bin_tilescan scans the
+// totals into tile ranges, and k_bin_scatter (one wave per block) places each
+// block's entries 64 at a time: a bitonic sort of (tile, lane) across the wave
+// groups equal tiles in entry order, and the head of each group reserves its
+// slots with one LDS atomic.  Entry counts are read from the device, so no
 // host round trip is needed between setup and raster.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kBinBlock = 8192;
-
 __device__ __forceinline__ uint32_t bin_entries(const FrameCtrl* ctrl, uint32_t cap)
 {
     return min(ctrl->totalPairs, cap);
 }
 
+// Block b of the binning kernels owns entries [b*per, (b+1)*per).
 __global__ __launch_bounds__(256) void k_bin_count(const uint32_t* __restrict__ keys, const FrameCtrl* __restrict__ ctrl,
-                                                   uint32_t cap, uint32_t* __restrict__ counts, int numTiles)
+                                                   uint32_t cap, uint32_t per, uint32_t* __restrict__ counts, int numTiles)
 {
     extern __shared__ uint32_t sCnt[];
     const uint32_t n = bin_entries(ctrl, cap);
-    const uint32_t start = blockIdx.x * kBinBlock;
+    const uint32_t start = blockIdx.x * per;
     if (start >= n) return;
-    const uint32_t end = min(start + kBinBlock, n);
+    const uint32_t end = min(start + per, n);
     for (int t = threadIdx.x; t < numTiles; t += blockDim.x) sCnt[t] = 0u;
     __syncthreads();
     for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x)
@@ -494,22 +494,43 @@ __global__ __launch_bounds__(256) void k_bin_count(const uint32_t* __restrict__ 
     for (int t = threadIdx.x; t < numTiles; t += blockDim.x) row[t] = sCnt[t];
 }
 
-__global__ __launch_bounds__(256) void k_bin_colscan(const FrameCtrl* __restrict__ ctrl, uint32_t cap,
-                                                     uint32_t* __restrict__ counts, uint32_t* __restrict__ totals, int numTiles)
+// Column scan of the block x tile count matrix: 32 tiles per workgroup, each
+// column split into 8 parts summed independently, then scanned in order.
+constexpr int kColTiles = 32, kColParts = 8;
+
+__global__ __launch_bounds__(kColTiles * kColParts) void k_bin_colscan(const FrameCtrl* __restrict__ ctrl, uint32_t cap, uint32_t per,
+                                                                      uint32_t* __restrict__ counts, uint32_t* __restrict__ totals,
+                                                                      int numTiles)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= numTiles) return;
+    __shared__ uint32_t sPart[kColParts][kColTiles];
+    const int tl = threadIdx.x % kColTiles, part = threadIdx.x / kColTiles;
+    const int t = blockIdx.x * kColTiles + tl;
     const uint32_t n = bin_entries(ctrl, cap);
-    const uint32_t blocks = (n + kBinBlock - 1) / kBinBlock;
-    uint32_t run = 0;
-    for (uint32_t b = 0; b < blocks; ++b)
+    const uint32_t blocks = (n + per - 1) / per;
+    const uint32_t rpp = (blocks + kColParts - 1) / kColParts;
+    const uint32_t r0 = min(blocks, part * rpp), r1 = min(blocks, r0 + rpp);
+    uint32_t sum = 0;
+    if (t < numTiles)
     {
-        uint32_t* c = counts + (uint64_t)b * numTiles + t;
-        const uint32_t v = *c;
-        *c = run;
-        run += v;
+#pragma unroll 8
+        for (uint32_t b = r0; b < r1; ++b) sum += counts[(uint64_t)b * numTiles + t];
     }
-    totals[t] = run;
+    sPart[part][tl] = sum;
+    __syncthreads();
+    uint32_t run = 0;
+    for (int q = 0; q < part; ++q) run += sPart[q][tl];
+    if (t < numTiles)
+    {
+#pragma unroll 8
+        for (uint32_t b = r0; b < r1; ++b)
+        {
+            uint32_t* c = counts + (uint64_t)b * numTiles + t;
+            const uint32_t v = *c;
+            *c = run;
+            run += v;
+        }
+        if (part == kColParts - 1) totals[t] = run;
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_bin_tilescan(const uint32_t* __restrict__ totals, uint2* __restrict__ range, int numTiles)
@@ -547,42 +568,59 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(const uint32_t* __restric
 }
 
 __global__ __launch_bounds__(64) void k_bin_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                    const FrameCtrl* __restrict__ ctrl, uint32_t cap,
+                                                    const FrameCtrl* __restrict__ ctrl, uint32_t cap, uint32_t per,
                                                     const uint32_t* __restrict__ counts, const uint2* __restrict__ range,
                                                     uint32_t* __restrict__ out, int numTiles)
 {
     extern __shared__ uint32_t sBase[];
     const uint32_t n = bin_entries(ctrl, cap);
-    const uint32_t start = blockIdx.x * kBinBlock;
+    const uint32_t start = blockIdx.x * per;
     if (start >= n) return;
-    const uint32_t end = min(start + kBinBlock, n);
+    const uint32_t end = min(start + per, n);
     const uint32_t* row = counts + (uint64_t)blockIdx.x * numTiles;
     for (int t = threadIdx.x; t < numTiles; t += 64) sBase[t] = range[t].x + row[t];
     __syncthreads();
     const int lane = threadIdx.x;
-    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);   // lanes <= lane
     for (uint32_t c = start; c < end; c += 64)
     {
         const uint32_t i = c + lane;
         const uint32_t key = (i < end) ? keys[i] : 0xFFFFFFFFu;
         const bool valid = key < (uint32_t)numTiles;
         const uint32_t val = valid ? vals[i] : 0u;
-        unsigned long long remaining = __ballot(valid);
-        while (remaining)
+        // bitonic sort of (tile << 6 | lane) across the wave: equal tiles end
+        // up adjacent and in entry order; invalid entries sort last
+        uint32_t sv = ((valid ? key : (uint32_t)numTiles) << 6) | (uint32_t)lane;
+#pragma unroll
+        for (int k = 2; k <= 64; k <<= 1)
         {
-            const int leader = __ffsll((long long)remaining) - 1;
-            const uint32_t k = __shfl(key, leader, 64);
-            const unsigned long long m = __ballot(key == k) & remaining;
-            const uint32_t b = sBase[k];
-            if ((m >> lane) & 1ull) out[b + __popcll(m & below)] = val;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            if (lane == leader) sBase[k] = b + (uint32_t)__popcll(m);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            remaining &= ~m;
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1)
+            {
+                const uint32_t o = __shfl_xor(sv, j, 64);
+                const bool up = (lane & k) == 0;
+                const bool lower = (lane & j) == 0;
+                sv = (lower == up) ? min(sv, o) : max(sv, o);
+            }
         }
+        const uint32_t skey = sv >> 6, src = sv & 63u;
+        const uint32_t prev = __shfl_up(skey, 1, 64);
+        const bool head = lane == 0 || prev != skey;
+        const unsigned long long heads = __ballot(head);
+        const int headLane = 63 - __clzll((long long)(heads & upto));
+        // the head of each run of equal tiles reserves the run's slots; runs
+        // have distinct tiles, so the LDS atomics never collide, and chunks
+        // are taken in order by this one wave
+        uint32_t slot = 0;
+        if (head && skey < (uint32_t)numTiles)
+        {
+            const unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1));
+            const uint32_t cnt = after ? (uint32_t)__ffsll((long long)after) : (uint32_t)(64 - lane);
+            slot = atomicAdd(&sBase[skey], cnt);
+        }
+        const uint32_t b = __shfl(slot, headLane, 64);
+        const uint32_t v = __shfl(val, (int)src, 64);
+        if (skey < (uint32_t)numTiles) out[b + (uint32_t)(lane - headLane)] = v;
     }
 }
 
@@ -616,9 +654,13 @@ __device__ __forceinline__ void record_barrier()
     }
 }
 
-template <int CF, bool DBG, int TW, int TH, int SEGS>
+// AUX bit 0: debug buffers (last primitive, depth-pass count); bit 1: the
+// visibility attachment.
+template <int CF, int AUX, int TW, int TH, int SEGS>
 __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
 {
+    constexpr bool DBG = (AUX & 1) != 0;
+    constexpr bool VIS = (AUX & 2) != 0;
     constexpr int NT = TH * 4 * SEGS;       // lanes: rows x segments x 4 phases
     constexpr int SW = TW / SEGS;           // segment width (columns)
     static_assert(SW >= 4 && SW % 4 == 0, "segment width");
@@ -627,6 +669,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     __shared__ float4 sColF[CF == 2 ? TH : 1][TW];
     __shared__ uint32_t sLast[DBG ? TH : 1][TW];
     __shared__ uint32_t sCount[DBG ? TH : 1][TW];
+    __shared__ float4 sVis[VIS ? TH : 1][TW];
 
     const int tile = blockIdx.x;
     const int tx = tile % P.tilesX, ty = tile / P.tilesX;
@@ -650,6 +693,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
             sLast[r][c] = P.clearDepth ? 0u : (inb ? P.dbgLast[gi] : 0u);
             sCount[r][c] = P.clearDepth ? 0u : (inb ? P.dbgCount[gi] : 0u);
         }
+        if (VIS) sVis[r][c] = (P.clearDepth || !inb) ? make_float4(0.f, 0.f, 0.f, 0.f) : P.vis[gi];
     }
     __syncthreads();
 
@@ -729,7 +773,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                         ++tested;
                         if (depth_test(D.dtest, z, sDepth[row][c]))
                         {
-                            if (D.nout > 0)
+                            if (D.nout > 0 || VIS)
                             {
                                 // perspective correction (SL_FragmentProcessor.cpp:486-494)
                                 const float t0 = s0 * R.hw[0], t1 = s1 * R.hw[1], t2 = s2 * R.hw[2];
@@ -751,8 +795,16 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                                         var.v[i] = f4(0.f, 0.f, 0.f, 0.f);
                                     }
                                 }
+                                if (VIS)
+                                {
+                                    float4 wv = f4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                                    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+                                        if ((uint32_t)i == D.visSlot) wv = var.v[i];
+                                    sVis[row][c] = make_float4(__uint_as_float(R.seq), wv.x, wv.y, wv.z);
+                                }
                                 float4 out;
-                                const bool have = fragment_program(D, P.uniforms, var, out);
+                                const bool have = (D.nout > 0) && fragment_program(D, P.uniforms, var, out);
                                 if (have)
                                 {
                                     if (CF == 1)
@@ -823,6 +875,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                 P.dbgLast[gi] = sLast[r][c];
                 P.dbgCount[gi] = sCount[r][c];
             }
+            if (VIS) P.vis[gi] = sVis[r][c];
         }
     }
 
@@ -879,52 +932,70 @@ hipError_t launch_clip(const SetupParams& p, uint32_t jobs, hipStream_t s)
     return hipGetLastError();
 }
 
-uint32_t bin_blocks(uint32_t entries) { return (entries + kBinBlock - 1) / kBinBlock; }
+BinPlan bin_plan(uint32_t entries)
+{
+    BinPlan b;
+    uint64_t per = ((uint64_t)entries + kMaxBinBlocks - 1) / kMaxBinBlocks;
+    per = (per + 63) / 64 * 64;
+    if (per < kMinBinBlock) per = kMinBinBlock;
+    b.per = (uint32_t)per;
+    b.blocks = (uint32_t)(((uint64_t)entries + per - 1) / per);
+    if (b.blocks == 0) b.blocks = 1;
+    return b;
+}
 
 hipError_t launch_binning(const uint32_t* keys, const uint32_t* vals, const FrameCtrl* ctrl, uint32_t cap,
-                          uint32_t gridBlocks, uint32_t* counts, uint32_t* totals, uint2* range, uint32_t* out,
+                          BinPlan plan, uint32_t* counts, uint32_t* totals, uint2* range, uint32_t* out,
                           int numTiles, hipStream_t s)
 {
-    if (gridBlocks == 0) gridBlocks = 1;
     const size_t lds = sizeof(uint32_t) * (size_t)numTiles;
-    hipLaunchKernelGGL(k_bin_count, dim3(gridBlocks), dim3(256), lds, s, keys, ctrl, cap, counts, numTiles);
-    hipLaunchKernelGGL(k_bin_colscan, dim3((numTiles + 255) / 256), dim3(256), 0, s, ctrl, cap, counts, totals, numTiles);
+    hipLaunchKernelGGL(k_bin_count, dim3(plan.blocks), dim3(256), lds, s, keys, ctrl, cap, plan.per, counts, numTiles);
+    hipLaunchKernelGGL(k_bin_colscan, dim3((numTiles + kColTiles - 1) / kColTiles), dim3(kColTiles * kColParts), 0, s,
+                       ctrl, cap, plan.per, counts, totals, numTiles);
     hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), 0, s, totals, range, numTiles);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(gridBlocks), dim3(64), lds, s, keys, vals, ctrl, cap, counts, range, out, numTiles);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(plan.blocks), dim3(64), lds, s, keys, vals, ctrl, cap, plan.per, counts, range, out,
+                       numTiles);
     return hipGetLastError();
 }
 
-template <int CF, bool DBG>
+template <int CF, int AUX>
 static void launch_raster_cfg(const RasterParams& p, int numTiles, int cfg, hipStream_t s)
 {
     const dim3 g(numTiles);
     switch (cfg)
     {
-        case 1: hipLaunchKernelGGL((k_raster<CF, DBG, 32, 8, 2>), g, dim3(64), 0, s, p); break;
-        case 2: hipLaunchKernelGGL((k_raster<CF, DBG, 16, 16, 1>), g, dim3(64), 0, s, p); break;
-        case 3: hipLaunchKernelGGL((k_raster<CF, DBG, 16, 8, 2>), g, dim3(64), 0, s, p); break;
-        case 4: hipLaunchKernelGGL((k_raster<CF, DBG, 64, 16, 2>), g, dim3(128), 0, s, p); break;
-        case 5: hipLaunchKernelGGL((k_raster<CF, DBG, 32, 32, 1>), g, dim3(128), 0, s, p); break;
-        case 6: hipLaunchKernelGGL((k_raster<CF, DBG, 64, 8, 4>), g, dim3(128), 0, s, p); break;
-        case 7: hipLaunchKernelGGL((k_raster<CF, DBG, 64, 32, 2>), g, dim3(256), 0, s, p); break;
-        default: hipLaunchKernelGGL((k_raster<CF, DBG, 32, 16, 1>), g, dim3(64), 0, s, p); break;
+        case 1: hipLaunchKernelGGL((k_raster<CF, AUX, 32, 8, 2>), g, dim3(64), 0, s, p); break;
+        case 2: hipLaunchKernelGGL((k_raster<CF, AUX, 16, 16, 1>), g, dim3(64), 0, s, p); break;
+        case 3: hipLaunchKernelGGL((k_raster<CF, AUX, 16, 8, 2>), g, dim3(64), 0, s, p); break;
+        case 4: hipLaunchKernelGGL((k_raster<CF, AUX, 64, 16, 2>), g, dim3(128), 0, s, p); break;
+        case 5: hipLaunchKernelGGL((k_raster<CF, AUX, 32, 32, 1>), g, dim3(128), 0, s, p); break;
+        case 6: hipLaunchKernelGGL((k_raster<CF, AUX, 64, 8, 4>), g, dim3(128), 0, s, p); break;
+        case 7: hipLaunchKernelGGL((k_raster<CF, AUX, 64, 32, 2>), g, dim3(256), 0, s, p); break;
+        default: hipLaunchKernelGGL((k_raster<CF, AUX, 32, 16, 1>), g, dim3(64), 0, s, p); break;
     }
 }
 
 hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s)
 {
     const int cf = (p.color == nullptr) ? 0 : (p.colorType == SL_COLOR_RGBA_FLOAT ? 2 : 1);
-    if (debug)
+    if (p.vis != nullptr)
     {
-        if (cf == 0) launch_raster_cfg<0, true>(p, numTiles, cfg, s);
-        else if (cf == 1) launch_raster_cfg<1, true>(p, numTiles, cfg, s);
-        else launch_raster_cfg<2, true>(p, numTiles, cfg, s);
+        // the visibility attachment is only built for RGBA32F colour targets
+        if (cf != 2) return hipErrorInvalidValue;
+        if (debug) launch_raster_cfg<2, 3>(p, numTiles, cfg, s);
+        else launch_raster_cfg<2, 2>(p, numTiles, cfg, s);
+    }
+    else if (debug)
+    {
+        if (cf == 0) launch_raster_cfg<0, 1>(p, numTiles, cfg, s);
+        else if (cf == 1) launch_raster_cfg<1, 1>(p, numTiles, cfg, s);
+        else launch_raster_cfg<2, 1>(p, numTiles, cfg, s);
     }
     else
     {
-        if (cf == 0) launch_raster_cfg<0, false>(p, numTiles, cfg, s);
-        else if (cf == 1) launch_raster_cfg<1, false>(p, numTiles, cfg, s);
-        else launch_raster_cfg<2, false>(p, numTiles, cfg, s);
+        if (cf == 0) launch_raster_cfg<0, 0>(p, numTiles, cfg, s);
+        else if (cf == 1) launch_raster_cfg<1, 0>(p, numTiles, cfg, s);
+        else launch_raster_cfg<2, 0>(p, numTiles, cfg, s);
     }
     return hipGetLastError();
 }

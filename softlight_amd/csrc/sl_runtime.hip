@@ -16,6 +16,7 @@
 
 #include "sl_device.h"
 #include "sl_kernels.h"
+#include "sl_backward.h"
 
 using namespace slg;
 
@@ -27,12 +28,27 @@ struct DevBuf
     size_t bytes = 0;
 };
 
-struct Vbo { bool valid = false; DevBuf buf; size_t bytes = 0; };
+struct Vbo { bool valid = false; DevBuf buf; size_t bytes = 0; size_t grad = SL_INVALID_ID; };
 struct Ibo { bool valid = false; DevBuf buf; int type = VERTEX_DATA_INT; uint32_t count = 0; uint32_t bpe = 4; };
 struct Binding { int dims = VERTEX_DIMENSION_4; int type = VERTEX_DATA_FLOAT; ptrdiff_t offset = 0; ptrdiff_t stride = 0; };
 struct Vao { size_t vbo = SL_INVALID_ID; size_t ibo = SL_INVALID_ID; size_t nb = 0; Binding b[SL_VAO_MAX_BINDINGS]; };
-struct Tex { bool valid = false; DevBuf buf; int type = SL_COLOR_RGBA_8U; uint16_t w = 0, h = 0, d = 0; uint32_t bpt = 0; };
-struct Fbo { std::vector<size_t> colors; size_t depth = SL_INVALID_ID; size_t dbgLast = SL_INVALID_ID; size_t dbgCount = SL_INVALID_ID; };
+struct Tex { bool valid = false; DevBuf buf; int type = SL_COLOR_RGBA_8U; uint16_t w = 0, h = 0, d = 0; uint32_t bpt = 0; size_t grad = SL_INVALID_ID; };
+// A draw kept for the backward pass, with the resources its gradients go to.
+struct SavedDraw { DrawDesc d; size_t vbo; size_t tex; };
+struct Fbo
+{
+    std::vector<size_t> colors;
+    size_t depth = SL_INVALID_ID;
+    size_t dbgLast = SL_INVALID_ID;
+    size_t dbgCount = SL_INVALID_ID;
+    size_t vis = SL_INVALID_ID;
+    // draws since the depth attachment was last cleared (visibility attached)
+    std::vector<SavedDraw> saved;
+    bool savedStale = false;         // a resource they read was re-initialised
+    bool savedBlend = false;         // one of them blends: no fixed-visibility adjoint
+    std::vector<GradDesc> gradHost;  // the backward draw table last uploaded ...
+    DevBuf gradDev;                  // ... and its device copy
+};
 struct Ubo { uint8_t bytes[SL_MAX_UNIFORM_BUFFER_SIZE]; };
 struct Shader { uint8_t cull, dtest, dmask, blend, nvary, nout; uint32_t vsProg, fsProg; size_t ubo; };
 
@@ -52,6 +68,10 @@ struct FrameGraph
     FrameCtrl* hCtrl = nullptr;
     uint64_t tris = 0;
     uint64_t pending = 0;                          // replays since the last collection
+    size_t fbo = SL_INVALID_ID;                    // target of the captured batch
+    bool clearDepth = false;
+    std::vector<SavedDraw> draws;                  // its draws, for the backward pass
+    bool timed = false;                            // event nodes around the setup and raster kernels
 };
 
 struct Batch
@@ -61,6 +81,7 @@ struct Batch
     double rgba[4] = {0, 0, 0, 0};
     double depth = 1.0;
     std::vector<DrawDesc> draws;
+    std::vector<size_t> drawVbo, drawTex;          // resource ids behind each draw (backward pass)
     std::vector<uint8_t> uni;
     uint64_t totalTris = 0;
     uint32_t maxVary = 1;
@@ -120,6 +141,9 @@ struct sl_context
     uint64_t lastTotals[3] = {0, 0, 0};            // records, bin entries, clip jobs of the last eager frame
     FrameCtrl* hCtrl = nullptr;                    // pinned readback of the control block
     hipEvent_t ev[8] = {};
+    hipEvent_t bwdEv[2] = {};                      // first / last backward since the last collection
+    hipEvent_t gEv[4] = {};                        // graph replays: setup start/end, raster start/end
+    uint64_t bwdPending = 0;
 };
 
 namespace {
@@ -209,6 +233,7 @@ struct Plan
     Tex* ct;
     Tex* dbgL;
     Tex* dbgC;
+    Tex* vis;
     bool debug;
     uint64_t chunks;
     uint32_t varStride;
@@ -225,9 +250,15 @@ int make_plan(sl_context* c, const Batch& B, Plan& p)
     p.dbgL = tex_at(c, F.dbgLast);
     p.dbgC = tex_at(c, F.dbgCount);
     p.debug = p.dbgL && p.dbgC;
+    p.vis = tex_at(c, F.vis);
     p.W = p.dt->w;
     p.H = p.dt->h;
     if (p.ct && (p.ct->w != p.W || p.ct->h != p.H)) return fail(c, SL_ERR_STATE, "colour and depth attachment sizes differ");
+    if (p.vis)
+    {
+        if (p.vis->type != SL_COLOR_RGBA_FLOAT || p.vis->w != p.W || p.vis->h != p.H) return fail(c, SL_ERR_STATE, "visibility attachment must be RGBA_FLOAT of the framebuffer's size");
+        if (!p.ct || p.ct->type != SL_COLOR_RGBA_FLOAT) return fail(c, SL_ERR_UNSUPPORTED, "visibility needs an RGBA_FLOAT colour attachment 0");
+    }
     p.cfg = (c->tileConfig >= 0 && c->tileConfig < kNumTileConfigs) ? c->tileConfig : kDefaultTileConfig;
     p.tc = kTileConfigs[p.cfg];
     p.tilesX = (p.W + p.tc.tileW - 1) / p.tc.tileW;
@@ -257,7 +288,7 @@ int ensure_frame(sl_context* c, FrameRes& R, const Plan& p, const Batch& B)
     if ((rc = ensure(c, R.vals, sizeof(uint32_t) * (size_t)R.pairCap)) != SL_OK) return rc;
     if ((rc = ensure(c, R.sorted, sizeof(uint32_t) * (size_t)R.pairCap)) != SL_OK) return rc;
     if ((rc = ensure(c, R.clip, sizeof(ClipJob) * (size_t)R.clipCap)) != SL_OK) return rc;
-    if ((rc = ensure(c, R.binCounts, sizeof(uint32_t) * (size_t)bin_blocks(R.pairCap) * p.numTiles)) != SL_OK) return rc;
+    if ((rc = ensure(c, R.binCounts, sizeof(uint32_t) * (size_t)kMaxBinBlocks * p.numTiles)) != SL_OK) return rc;
     if ((rc = ensure(c, R.binTotals, sizeof(uint32_t) * (size_t)p.numTiles)) != SL_OK) return rc;
     return SL_OK;
 }
@@ -302,6 +333,7 @@ RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool have
     rp.depth = reinterpret_cast<float*>(p.dt->buf.p);
     rp.dbgLast = p.debug ? reinterpret_cast<uint32_t*>(p.dbgL->buf.p) : nullptr;
     rp.dbgCount = p.debug ? reinterpret_cast<uint32_t*>(p.dbgC->buf.p) : nullptr;
+    rp.vis = p.vis ? reinterpret_cast<float4*>(p.vis->buf.p) : nullptr;
     rp.clearColor = B.clearColor ? 1 : 0;
     rp.clearDepth = B.clearDepth ? 1 : 0;
     uint32_t packed = 0;
@@ -317,7 +349,8 @@ RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool have
 }
 
 // Control-block reset, look-back status reset and the setup kernel.
-int enqueue_front(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, hipStream_t s)
+int enqueue_front(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, hipStream_t s,
+                  hipEvent_t evSetup0 = nullptr, hipEvent_t evSetup1 = nullptr)
 {
     int rc;
     FrameCtrl* ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
@@ -326,7 +359,10 @@ int enqueue_front(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, hip
     if (B.totalTris == 0) return SL_OK;
     const SetupParams sp = setup_params(p, B, R);
     if ((rc = check(c, launch_fill_u64(sp.status, 0ull, p.chunks, s), "fill status")) != SL_OK) return rc;
-    return check(c, launch_setup(sp, p.chunks, s), "k_setup");
+    if (evSetup0 && (rc = check(c, hipEventRecord(evSetup0, s), "event")) != SL_OK) return rc;
+    if ((rc = check(c, launch_setup(sp, p.chunks, s), "k_setup")) != SL_OK) return rc;
+    if (evSetup1 && (rc = check(c, hipEventRecord(evSetup1, s), "event")) != SL_OK) return rc;
+    return SL_OK;
 }
 
 // Clip, binning and raster.  clipJobs / entries size the grids (a graph uses
@@ -343,7 +379,7 @@ int enqueue_back(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, uint
         if (evClip1) (void)hipEventRecord(evClip1, s);
         if (entries > 0)
         {
-            if ((rc = check(c, launch_binning(sp.pairKeys, sp.pairVals, sp.ctrl, R.pairCap, bin_blocks(entries),
+            if ((rc = check(c, launch_binning(sp.pairKeys, sp.pairVals, sp.ctrl, R.pairCap, bin_plan(entries),
                                               reinterpret_cast<uint32_t*>(R.binCounts.p), reinterpret_cast<uint32_t*>(R.binTotals.p),
                                               reinterpret_cast<uint2*>(R.ranges.p), reinterpret_cast<uint32_t*>(R.sorted.p),
                                               p.numTiles, s), "binning")) != SL_OK) return rc;
@@ -361,6 +397,38 @@ int upload_batch(sl_context* c, FrameRes& R, const Batch& B, hipStream_t s)
     if ((rc = check(c, hipMemcpyAsync(R.draws.p, B.draws.data(), sizeof(DrawDesc) * B.draws.size(), hipMemcpyHostToDevice, s), "upload draws")) != SL_OK) return rc;
     if (!B.uni.empty() && (rc = check(c, hipMemcpyAsync(R.uni.p, B.uni.data(), B.uni.size(), hipMemcpyHostToDevice, s), "upload uniforms")) != SL_OK) return rc;
     return SL_OK;
+}
+
+// Keeps the draws of a batch that rendered with a visibility attachment: the
+// backward pass maps each pixel's sequence number back to its draw.
+void save_draws(sl_context* c, const Batch& B, const Plan& p)
+{
+    Fbo& F = c->fbos[B.fbo];
+    if (!p.vis)
+    {
+        F.saved.clear();
+        F.savedStale = false;
+        F.savedBlend = false;
+        return;
+    }
+    if (B.clearDepth)
+    {
+        F.saved.clear();
+        F.savedStale = false;
+        F.savedBlend = false;
+    }
+    for (size_t i = 0; i < B.draws.size(); ++i)
+    {
+        F.saved.push_back(SavedDraw{B.draws[i], B.drawVbo[i], B.drawTex[i]});
+        if (B.draws[i].blend != SL_BLEND_OFF) F.savedBlend = true;
+    }
+}
+
+// Resource storage moved: saved draws hold stale device pointers.
+void stale_saved_draws(sl_context* c)
+{
+    for (Fbo& F : c->fbos)
+        if (!F.saved.empty()) F.savedStale = true;
 }
 
 int execute_batch(sl_context* c)
@@ -447,6 +515,7 @@ int execute_batch(sl_context* c)
     c->lastTotals[0] = totalRecs;
     c->lastTotals[1] = totalPairs;
     c->lastTotals[2] = jobs;
+    save_draws(c, B, p);
     return SL_OK;
 }
 
@@ -468,6 +537,8 @@ void graph_release(sl_context* c)
     G.hCtrl = nullptr;
     G.valid = false;
     G.pending = 0;
+    G.draws.clear();
+    G.timed = false;
 }
 
 int graph_capture(sl_context* c)
@@ -493,17 +564,42 @@ int graph_capture(sl_context* c)
         return fail(c, SL_ERR_OUT_OF_MEMORY, "pinned control block");
     }
     if ((rc = check(c, hipStreamSynchronize(s), "sync")) != SL_OK) { graph_release(c); return rc; }
-    if ((rc = check(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture")) != SL_OK) { graph_release(c); return rc; }
-    int rc1 = enqueue_front(c, G.res, p, B, s);
-    int rc2 = (rc1 == SL_OK) ? enqueue_back(c, G.res, p, B, G.res.clipCap, G.res.pairCap, s, nullptr, nullptr, nullptr) : rc1;
-    int rc3 = (rc2 == SL_OK) ? check(c, hipMemcpyAsync(G.hCtrl, G.res.ctrl.p, sizeof(FrameCtrl), hipMemcpyDeviceToHost, s), "read ctrl") : rc2;
-    hipGraph_t graph = nullptr;
-    const hipError_t e = hipStreamEndCapture(s, &graph);
-    if (rc3 != SL_OK) { if (graph) (void)hipGraphDestroy(graph); graph_release(c); return rc3; }
-    if ((rc = check(c, e, "end capture")) != SL_OK) { graph_release(c); return rc; }
-    G.graph = graph;
-    if ((rc = check(c, hipGraphInstantiate(&G.exec, graph, nullptr, nullptr, 0), "instantiate graph")) != SL_OK) { graph_release(c); return rc; }
+    // first with event nodes around the setup and raster kernels (per-kernel
+    // device times of the replays), then without them if that capture fails
+    for (int timed = 1; timed >= 0; --timed)
+    {
+        if ((rc = check(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture")) != SL_OK) { graph_release(c); return rc; }
+        int rc1 = timed ? enqueue_front(c, G.res, p, B, s, c->gEv[0], c->gEv[1]) : enqueue_front(c, G.res, p, B, s);
+        int rc2 = (rc1 == SL_OK) ? enqueue_back(c, G.res, p, B, G.res.clipCap, G.res.pairCap, s, nullptr, nullptr,
+                                                timed ? c->gEv[2] : nullptr) : rc1;
+        if (rc2 == SL_OK && timed) rc2 = check(c, hipEventRecord(c->gEv[3], s), "event");
+        int rc3 = (rc2 == SL_OK) ? check(c, hipMemcpyAsync(G.hCtrl, G.res.ctrl.p, sizeof(FrameCtrl), hipMemcpyDeviceToHost, s), "read ctrl") : rc2;
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        if (rc3 == SL_OK && e == hipSuccess && graph)
+        {
+            hipGraphExec_t exec = nullptr;
+            if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess)
+            {
+                G.graph = graph;
+                G.exec = exec;
+                G.timed = timed != 0;
+                break;
+            }
+        }
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        if (!timed)
+        {
+            graph_release(c);
+            return fail(c, SL_ERR_DEVICE, rc3 != SL_OK ? "graph capture" : "graph end capture / instantiate");
+        }
+    }
     G.tris = B.totalTris;
+    G.fbo = B.fbo;
+    G.clearDepth = B.clearDepth;
+    G.draws.clear();
+    for (size_t i = 0; i < B.draws.size(); ++i) G.draws.push_back(SavedDraw{B.draws[i], B.drawVbo[i], B.drawTex[i]});
     G.valid = true;
     return SL_OK;
 }
@@ -523,10 +619,34 @@ int graph_collect(sl_context* c)
     c->stats.tile_pairs += (uint64_t)G.hCtrl->totalPairs * G.pending;
     c->stats.fragments_tested += G.hCtrl->tested * G.pending;
     c->stats.fragments_shaded += G.hCtrl->shaded * G.pending;
+    if (G.timed)
+    {
+        // every replay runs the same work: the last replay's kernel times
+        float ts = 0.f, tr = 0.f;
+        if (hipEventElapsedTime(&ts, c->gEv[0], c->gEv[1]) == hipSuccess && hipEventElapsedTime(&tr, c->gEv[2], c->gEv[3]) == hipSuccess)
+        {
+            c->stats.ms_setup_kernel += (double)ts * G.pending;
+            c->stats.setup_launches += G.pending;
+            c->stats.ms_raster += (double)tr * G.pending;
+            c->stats.raster_launches += G.pending;
+        }
+    }
     G.pending = 0;
     if (G.hCtrl->error) return fail(c, SL_ERR_DEVICE, "graph frame: setup look-back did not complete");
     if (G.hCtrl->overflow) return fail(c, SL_ERR_CAPACITY, "graph frame exceeded its captured capacity; capture again");
     return SL_OK;
+}
+
+// Adds the device time of the backward passes enqueued since the last
+// collection to the counters (waits for the last one).
+void collect_backward(sl_context* c)
+{
+    if (c->bwdPending == 0) return;
+    float t = 0.f;
+    if (hipEventSynchronize(c->bwdEv[1]) == hipSuccess && hipEventElapsedTime(&t, c->bwdEv[0], c->bwdEv[1]) == hipSuccess)
+        c->stats.ms_backward += t;
+    c->stats.backward_launches += c->bwdPending;
+    c->bwdPending = 0;
 }
 
 int flush(sl_context* c)
@@ -549,6 +669,8 @@ int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t s
     const Shader& sh = c->shaders[shaderId];
     if (sh.nout > 1) return fail(c, SL_ERR_UNSUPPORTED, "more than one fragment output");
     Fbo& F = c->fbos[fboId];
+    const bool vis = tex_at(c, F.vis) != nullptr;
+    if (vis && sh.nvary >= SL_SHADER_MAX_VARYING_VECTORS) return fail(c, SL_ERR_UNSUPPORTED, "visibility needs a free varying slot (numVaryings <= 3)");
     Tex* dt = tex_at(c, F.depth);
     if (!dt || dt->type != SL_COLOR_R_FLOAT) return fail(c, SL_ERR_STATE, "framebuffer needs an R_FLOAT depth attachment");
     if (sh.nout == 1)
@@ -609,6 +731,12 @@ int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t s
     d.blend = sh.blend;
     d.nvary = sh.nvary;
     d.nout = sh.nout;
+    d.visSlot = kNoVisSlot;
+    if (vis)
+    {
+        d.visSlot = sh.nvary;
+        d.nvary = (uint8_t)(sh.nvary + 1);
+    }
     d.seqBase = c->triSeq;
     c->triSeq += (uint32_t)numTris;
     viewport_setup(c, (float)dt->w, (float)dt->h, d.scissor, d.vp);
@@ -635,9 +763,18 @@ int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t s
             d.instVboBytes = c->vbos[u.instance_vbo].bytes;
         }
     }
+    size_t texId = SL_INVALID_ID;
+    if (d.hasUniforms)
+    {
+        sl_std_uniforms u;
+        std::memcpy(&u, c->ubos[sh.ubo].bytes, sizeof(u));
+        if (d.tex) texId = u.texture;
+    }
     B.draws.push_back(d);
+    B.drawVbo.push_back(vbo ? vao.vbo : SL_INVALID_ID);
+    B.drawTex.push_back(texId);
     B.totalTris += numTris;
-    B.maxVary = std::max<uint32_t>(B.maxVary, sh.nvary);
+    B.maxVary = std::max<uint32_t>(B.maxVary, d.nvary);
     return SL_OK;
 }
 
@@ -658,7 +795,8 @@ int fill_texture(sl_context* c, Tex& t, const void* value, uint32_t bpt)
             return check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(t.buf.p), v[0], 4 * n, c->stream), "fill texture");
         std::vector<uint8_t> host(n * 16);
         for (size_t i = 0; i < n; ++i) std::memcpy(host.data() + 16 * i, value, 16);
-        hipError_t e = hipMemcpy(t.buf.p, host.data(), host.size(), hipMemcpyHostToDevice);
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpy(t.buf.p, host.data(), host.size(), hipMemcpyHostToDevice);
         return check(c, e, "fill texture");
     }
     return fail(c, SL_ERR_UNSUPPORTED, "clear of this texture format");
@@ -732,6 +870,10 @@ int sl_context_create(sl_context** out)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return SL_ERR_DEVICE; }
     for (auto& e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) { delete c; return SL_ERR_DEVICE; }
+    for (auto& e : c->bwdEv)
+        if (hipEventCreate(&e) != hipSuccess) { delete c; return SL_ERR_DEVICE; }
+    for (auto& e : c->gEv)
+        if (hipEventCreate(&e) != hipSuccess) { delete c; return SL_ERR_DEVICE; }
     if (hipHostMalloc(reinterpret_cast<void**>(&c->hCtrl), sizeof(FrameCtrl), hipHostMallocDefault) != hipSuccess) { delete c; return SL_ERR_DEVICE; }
     std::memset(c->hCtrl, 0, sizeof(FrameCtrl));
     *out = c;
@@ -750,6 +892,9 @@ void sl_context_destroy(sl_context* c)
     for (DevBuf* b : {&c->res.draws, &c->res.uni, &c->res.recs, &c->res.vars, &c->res.keys, &c->res.vals, &c->res.sorted,
                       &c->res.clip, &c->res.status, &c->res.ranges, &c->res.ctrl, &c->res.binCounts, &c->res.binTotals}) release(*b);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    for (auto& f : c->fbos) release(f.gradDev);
+    for (auto& e : c->bwdEv) if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->gEv) if (e) (void)hipEventDestroy(e);
     if (c->hCtrl) (void)hipHostFree(c->hCtrl);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -815,6 +960,7 @@ int sl_vbo_init(sl_context* c, size_t vbo, size_t numBytes, const void* data)
     int rc = flush(c);
     if (rc != SL_OK) return rc;
     if (c->graph.valid) { (void)graph_collect(c); graph_release(c); }   // the graph points at the old buffer
+    stale_saved_draws(c);
     Vbo& v = c->vbos[vbo];
     if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     release(v.buf);
@@ -832,7 +978,21 @@ int sl_vbo_assign(sl_context* c, size_t vbo, const void* data, size_t offset, si
     if (offset + numBytes > v.bytes || !data) return SL_ERR_INVALID_ARG;
     int rc = flush(c);
     if (rc != SL_OK) return rc;
+    // queued graph replays and backward passes may still read the buffer
+    if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     return check(c, hipMemcpy(reinterpret_cast<uint8_t*>(v.buf.p) + offset, data, numBytes, hipMemcpyHostToDevice), "vbo assign");
+}
+
+int sl_vbo_read(sl_context* c, size_t vbo, void* out, size_t offset, size_t numBytes)
+{
+    if (!c || vbo >= c->vbos.size() || !c->vbos[vbo].valid) return SL_ERR_INVALID_HANDLE;
+    Vbo& v = c->vbos[vbo];
+    if (offset + numBytes > v.bytes || !out) return SL_ERR_INVALID_ARG;
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
+    collect_backward(c);
+    return check(c, hipMemcpy(out, reinterpret_cast<const uint8_t*>(v.buf.p) + offset, numBytes, hipMemcpyDeviceToHost), "vbo read");
 }
 
 size_t sl_create_ibo(sl_context* c) { c->ibos.emplace_back(); return c->ibos.size() - 1; }
@@ -846,6 +1006,7 @@ int sl_ibo_init(sl_context* c, size_t ibo, uint32_t numElements, int type, const
     int rc = flush(c);
     if (rc != SL_OK) return rc;
     if (c->graph.valid) { (void)graph_collect(c); graph_release(c); }
+    stale_saved_draws(c);
     if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     Ibo& b = c->ibos[ibo];
     release(b.buf);
@@ -910,6 +1071,7 @@ int sl_texture_init(sl_context* c, size_t tex, int type, uint16_t w, uint16_t h,
     int rc = flush(c);
     if (rc != SL_OK) return rc;
     if (c->graph.valid) { (void)graph_collect(c); graph_release(c); }
+    stale_saved_draws(c);
     if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     Tex& t = c->texs[tex];
     release(t.buf);
@@ -933,6 +1095,7 @@ int sl_texture_set_texels(sl_context* c, size_t tex, const void* data, size_t nu
     if (numBytes != n || !data) return SL_ERR_INVALID_ARG;
     int rc = flush(c);
     if (rc != SL_OK) return rc;
+    if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     return check(c, hipMemcpy(t->buf.p, data, n, hipMemcpyHostToDevice), "texture upload");
 }
 
@@ -946,6 +1109,7 @@ int sl_texture_read(sl_context* c, size_t tex, void* out, size_t numBytes)
     int rc = flush(c);
     if (rc != SL_OK) return rc;
     if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
+    collect_backward(c);
     return check(c, hipMemcpy(out, t->buf.p, n, hipMemcpyDeviceToHost), "texture readback");
 }
 
@@ -1106,7 +1270,9 @@ int sl_finish(sl_context* c)
     if (rc != SL_OK) return rc;
     rc = graph_collect(c);
     if (rc != SL_OK) return rc;
-    return check(c, hipStreamSynchronize(c->stream), "finish");
+    rc = check(c, hipStreamSynchronize(c->stream), "finish");
+    collect_backward(c);
+    return rc;
 }
 
 int sl_frame_graph_capture(sl_context* c)
@@ -1128,6 +1294,26 @@ int sl_frame_graph_launch(sl_context* c)
     if (rc != SL_OK) return rc;
     (void)hipEventRecord(c->ev[7], c->stream);
     c->graph.pending += 1;
+    FrameGraph& G = c->graph;
+    if (G.fbo < c->fbos.size())
+    {
+        Fbo& F = c->fbos[G.fbo];
+        if (tex_at(c, F.vis))
+        {
+            if (G.clearDepth)
+            {
+                F.saved = G.draws;
+                F.savedStale = false;
+                F.savedBlend = false;
+                for (const SavedDraw& d : G.draws)
+                    if (d.d.blend != SL_BLEND_OFF) F.savedBlend = true;
+            }
+            else
+            {
+                F.savedStale = true;   // replays without a depth clear stack up
+            }
+        }
+    }
     return SL_OK;
 }
 
@@ -1137,6 +1323,164 @@ int sl_frame_graph_release(sl_context* c)
     const int rc = graph_collect(c);
     graph_release(c);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// differentiable draws
+// ---------------------------------------------------------------------------
+int sl_framebuffer_attach_visibility(sl_context* c, size_t fbo, size_t tex)
+{
+    if (!c || fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    if (tex != SL_INVALID_ID)
+    {
+        Tex* t = tex_at(c, tex);
+        if (!t) return SL_ERR_INVALID_HANDLE;
+        if (t->type != SL_COLOR_RGBA_FLOAT) return fail(c, SL_ERR_INVALID_ARG, "visibility attachment must be RGBA_FLOAT");
+    }
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    Fbo& F = c->fbos[fbo];
+    F.vis = tex;
+    F.saved.clear();
+    F.savedStale = false;
+    F.savedBlend = false;
+    return SL_OK;
+}
+
+int sl_vbo_attach_gradient(sl_context* c, size_t vbo, size_t grad)
+{
+    if (!c || vbo >= c->vbos.size()) return SL_ERR_INVALID_HANDLE;
+    if (grad != SL_INVALID_ID && (grad >= c->vbos.size() || grad == vbo)) return SL_ERR_INVALID_HANDLE;
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    c->vbos[vbo].grad = grad;
+    return SL_OK;
+}
+
+int sl_texture_attach_gradient(sl_context* c, size_t tex, size_t grad)
+{
+    if (!c || tex >= c->texs.size()) return SL_ERR_INVALID_HANDLE;
+    if (grad != SL_INVALID_ID)
+    {
+        if (grad >= c->texs.size() || grad == tex) return SL_ERR_INVALID_HANDLE;
+        const Tex* g = tex_at(c, grad);
+        const Tex& t = c->texs[tex];
+        if (!g || !t.valid) return SL_ERR_INVALID_HANDLE;
+        if (g->type != SL_COLOR_RGBA_FLOAT || g->w != t.w || g->h != t.h || g->d != t.d)
+            return fail(c, SL_ERR_INVALID_ARG, "texture gradient must be RGBA_FLOAT of the texture's size");
+    }
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    c->texs[tex].grad = grad;
+    return SL_OK;
+}
+
+int sl_gradients_zero(sl_context* c)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    for (const Vbo& v : c->vbos)
+    {
+        if (!v.valid || v.grad >= c->vbos.size() || !c->vbos[v.grad].valid) continue;
+        const Vbo& g = c->vbos[v.grad];
+        if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(g.buf.p), 0u, g.bytes / 4, c->stream), "zero gradient")) != SL_OK) return rc;
+    }
+    for (const Tex& t : c->texs)
+    {
+        if (!t.valid || t.grad >= c->texs.size() || !c->texs[t.grad].valid) continue;
+        const Tex& g = c->texs[t.grad];
+        if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(g.buf.p), 0u, g.buf.bytes / 4, c->stream), "zero gradient")) != SL_OK) return rc;
+    }
+    return SL_OK;
+}
+
+int sl_loss_l2_grad(sl_context* c, size_t colorTex, size_t targetTex, size_t gradTex, float scale)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    Tex* a = tex_at(c, colorTex);
+    Tex* b = tex_at(c, targetTex);
+    Tex* g = tex_at(c, gradTex);
+    if (!a || !b || !g) return SL_ERR_INVALID_HANDLE;
+    for (const Tex* t : {a, b, g})
+        if (t->type != SL_COLOR_RGBA_FLOAT || t->w != a->w || t->h != a->h || t->d != a->d)
+            return fail(c, SL_ERR_INVALID_ARG, "loss textures must be RGBA_FLOAT of one size");
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    const uint64_t n = (uint64_t)a->w * a->h * std::max<uint16_t>(a->d, 1);
+    return check(c, launch_loss_l2_grad(reinterpret_cast<const float4*>(a->buf.p), reinterpret_cast<const float4*>(b->buf.p),
+                                        reinterpret_cast<float4*>(g->buf.p), n, scale, c->stream), "loss gradient");
+}
+
+int sl_framebuffer_backward(sl_context* c, size_t fbo, size_t gradImageTex)
+{
+    if (!c || fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    Fbo& F = c->fbos[fbo];
+    Tex* vis = tex_at(c, F.vis);
+    if (!vis) return fail(c, SL_ERR_STATE, "framebuffer has no visibility attachment");
+    Tex* gi = tex_at(c, gradImageTex);
+    if (!gi) return SL_ERR_INVALID_HANDLE;
+    if (gi->type != SL_COLOR_RGBA_FLOAT || gi->w != vis->w || gi->h != vis->h)
+        return fail(c, SL_ERR_INVALID_ARG, "image gradient must be RGBA_FLOAT of the framebuffer's size");
+    if (F.savedStale) return fail(c, SL_ERR_STATE, "resources changed since the forward pass; draw again");
+    if (F.savedBlend) return fail(c, SL_ERR_UNSUPPORTED, "fixed-visibility backward of blended draws");
+    if (F.saved.empty()) return SL_OK;
+    std::vector<GradDesc> G(F.saved.size());
+    for (size_t i = 0; i < F.saved.size(); ++i)
+    {
+        const SavedDraw& sd = F.saved[i];
+        const bool ct = sd.d.vsProg == SL_PROGRAM_COLOR_TEXTURED && sd.d.fsProg == SL_PROGRAM_COLOR_TEXTURED;
+        const bool pg = sd.d.vsProg == SL_PROGRAM_PASSTHROUGH_GOURAUD && sd.d.fsProg == SL_PROGRAM_PASSTHROUGH_GOURAUD;
+        if (!ct && !pg) return fail(c, SL_ERR_UNSUPPORTED, "backward of this shader program");
+        GradDesc& g = G[i];
+        std::memset(&g, 0, sizeof(g));
+        g.d = sd.d;
+        if (sd.vbo < c->vbos.size())
+        {
+            const size_t gv = c->vbos[sd.vbo].grad;
+            if (gv < c->vbos.size() && c->vbos[gv].valid)
+            {
+                g.gradCol = reinterpret_cast<float*>(c->vbos[gv].buf.p);
+                g.gradColCount = c->vbos[gv].bytes / 16;
+            }
+        }
+        if (ct && sd.tex < c->texs.size())
+        {
+            const Tex& t = c->texs[sd.tex];
+            const Tex* gt = tex_at(c, t.grad);
+            if (gt && gt->type == SL_COLOR_RGBA_FLOAT && gt->w == t.w && gt->h == t.h)
+            {
+                g.gradTex = reinterpret_cast<float*>(gt->buf.p);
+                g.gradTexTexels = gt->buf.bytes / 16;
+            }
+        }
+    }
+    // the table only changes with the draws or the gradient attachments;
+    // re-upload (synchronously, after all queued work) only then
+    hipStream_t s = c->stream;
+    const bool same = F.gradHost.size() == G.size() && F.gradDev.p &&
+                      std::memcmp(F.gradHost.data(), G.data(), sizeof(GradDesc) * G.size()) == 0;
+    if (!same)
+    {
+        if ((rc = ensure(c, F.gradDev, sizeof(GradDesc) * G.size())) != SL_OK) return rc;
+        if ((rc = check(c, hipStreamSynchronize(s), "sync")) != SL_OK) return rc;
+        if ((rc = check(c, hipMemcpy(F.gradDev.p, G.data(), sizeof(GradDesc) * G.size(), hipMemcpyHostToDevice), "upload backward draws")) != SL_OK) return rc;
+        F.gradHost = G;
+    }
+    BackwardParams bp{};
+    bp.draws = reinterpret_cast<const GradDesc*>(F.gradDev.p);
+    bp.numDraws = (uint32_t)G.size();
+    bp.vis = reinterpret_cast<const float4*>(vis->buf.p);
+    bp.gradImage = reinterpret_cast<const float4*>(gi->buf.p);
+    bp.W = vis->w;
+    bp.H = vis->h;
+    if (c->bwdPending == 0) (void)hipEventRecord(c->bwdEv[0], s);
+    if ((rc = check(c, launch_backward(bp, s), "k_backward")) != SL_OK) return rc;
+    (void)hipEventRecord(c->bwdEv[1], s);
+    c->bwdPending += 1;
+    return SL_OK;
 }
 
 int sl_get_stats(const sl_context* c, sl_frame_stats* out)

@@ -120,6 +120,19 @@ __device__ __forceinline__ void model_matrix(const DrawDesc& d, const sl_std_uni
     for (int i = 0; i < 16; ++i) m[i] = u->model[i];
 }
 
+// Index fetch (SL_TriProcessor.cpp:125-234): u8/u16/u32 indices; an index
+// past the buffer reads 0.
+__device__ __forceinline__ uint64_t fetch_index(const DrawDesc& d, uint64_t e)
+{
+    if (e >= d.iboCount) return 0;
+    switch (d.iboType)
+    {
+        case VERTEX_DATA_BYTE: return gptr(d.ibo)[e];
+        case VERTEX_DATA_SHORT: return gptr(reinterpret_cast<const uint16_t*>(d.ibo))[e];
+        default: return gptr(reinterpret_cast<const uint32_t*>(d.ibo))[e];
+    }
+}
+
 // Built-in vertex programs (replace SL_VertexShader::shader, SL_Shader.hpp:58-64)
 __device__ __forceinline__ float4 vertex_program(const DrawDesc& d, const uint8_t* uniArena, uint64_t vertId,
                                                  uint64_t inst, Vary4& out)

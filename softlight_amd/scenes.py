@@ -398,3 +398,192 @@ def build_batched_scene(ctx, seed=3, n_meshes=1000, median_tris=2000, sigma=1.0,
                   triangles=total, median_tris=median_tris, sigma=sigma)
     s.info["_keep"] = keep
     return s
+
+
+# --------------------------------------------------------------------------
+# differentiable views: a colour-and-texture icosphere seen by many cameras
+# --------------------------------------------------------------------------
+def gaussian(seed: int, n: int, stream: int = 0) -> np.ndarray:
+    """Standard normal float64 samples (Box-Muller on two uniform streams)."""
+    u1 = uniform01(seed, n, stream=stream).astype(np.float64)
+    u2 = uniform01(seed, n, stream=stream + 1).astype(np.float64)
+    u1 = np.maximum(u1, 2.0 ** -24)
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+
+
+def look_at(eye, target, up) -> np.ndarray:
+    """View matrix (row-major math convention), camera looking down -z."""
+    eye = np.asarray(eye, dtype=np.float64)
+    f = np.asarray(target, dtype=np.float64) - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, np.asarray(up, dtype=np.float64))
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    V = np.eye(4)
+    V[0, :3], V[1, :3], V[2, :3] = s, u, -f
+    V[0, 3], V[1, 3], V[2, 3] = -s @ eye, -u @ eye, f @ eye
+    return V
+
+
+class DiffScene:
+    """A unit icosphere with per-vertex RGB colours and an RGBA32F texture
+    (SL_PROGRAM_COLOR_TEXTURED), rendered from `views` random cameras into
+    RGBA32F targets with visibility attachments.  Views are packed
+    `views_per_fbo` to a framebuffer as a grid of viewports, so each
+    framebuffer is one batch.  The loss is sum((C - T)^2) over every view;
+    gradients go to an RGBA32F texture and a float4-per-vertex VBO."""
+
+    def __init__(self, ctx: A.SLContext, seed=4, level=5, views=32, size=512, views_per_fbo=8, tex_size=512,
+                 distance=3.0, fov=60.0, edge_views=0):
+        self.ctx = ctx
+        self.seed, self.level, self.views, self.size = seed, level, views, size
+        self.tex_size = tex_size
+        pos, faces = icosphere(level)
+        nv = pos.shape[0]
+        self.n_vertices, self.n_triangles = nv, faces.shape[0]
+        lon = np.arctan2(pos[:, 2], pos[:, 0])
+        lat = np.arccos(np.clip(pos[:, 1], -1.0, 1.0))
+        uv = np.stack([lon / (2 * math.pi) + 0.5, lat / math.pi], axis=1)
+        self.verts = np.zeros((nv, 8), dtype=np.float32)
+        self.verts[:, 0:3] = pos
+        self.verts[:, 3:6] = uniform01(seed, nv * 3, stream=1).reshape(nv, 3)
+        self.verts[:, 6:8] = uv
+        self.texels = uniform01(seed, tex_size * tex_size * 4, stream=2).reshape(tex_size, tex_size, 4).astype(np.float32)
+        self.indices = faces.astype(np.uint32).reshape(-1)
+
+        self.vbo = ctx.create_vbo(self.verts)
+        self.ibo = ctx.create_ibo(self.indices)
+        self.vao = ctx.create_vao(self.vbo, self.ibo, [(0, 32, A.VERTEX_DIMENSION_3, A.VERTEX_DATA_FLOAT),
+                                                       (12, 32, A.VERTEX_DIMENSION_3, A.VERTEX_DATA_FLOAT),
+                                                       (24, 32, A.VERTEX_DIMENSION_2, A.VERTEX_DATA_FLOAT)])
+        self.tex = ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, tex_size, tex_size, 1, self.texels)
+        self.grad_vbo = ctx.create_vbo(np.zeros((nv, 4), dtype=np.float32))
+        self.grad_tex = ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, tex_size, tex_size, 1)
+        ctx.vbo_attach_gradient(self.vbo, self.grad_vbo)
+        ctx.texture_attach_gradient(self.tex, self.grad_tex)
+        self.mesh = A.sl_mesh(self.vao, 0, self.indices.size, A.RENDER_MODE_INDEXED_TRIANGLES, 0)
+
+        # cameras: random directions at `distance`, looking at the centre; the
+        # first edge_views look past the sphere so that the frustum's side
+        # planes cut it (clipped triangles)
+        P = perspective(fov, 1.0, 0.1, 100.0).T.astype(np.float64)
+        d = gaussian(seed, views * 3, stream=3).reshape(views, 3)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        self.shaders = []
+        self.ubos = []
+        for v in range(views):
+            up = (0.0, 1.0, 0.0) if abs(d[v, 1]) < 0.9 else (1.0, 0.0, 0.0)
+            target = np.zeros(3)
+            if v < edge_views:
+                side = np.cross(d[v], up)
+                target = 1.2 * side / np.linalg.norm(side)
+            V = look_at(d[v] * distance, target, up)
+            mvp = P @ V
+            u = A.sl_std_uniforms()
+            u.mvp[:] = mvp.T.astype(np.float32).reshape(-1).tolist()
+            u.model[:] = identity().reshape(-1).tolist()
+            u.tint[:] = [1.0, 1.0, 1.0, 1.0]
+            u.texture = self.tex
+            u.instance_vbo = A.SL_NO_RESOURCE
+            ubo = ctx.create_ubo(u)
+            vs = A.sl_vertex_shader(2, A.SL_CULL_BACK_FACE, A.SL_PROGRAM_COLOR_TEXTURED)
+            fs = A.sl_fragment_shader(2, 1, A.SL_BLEND_OFF, A.SL_DEPTH_TEST_LESS_THAN, A.SL_DEPTH_MASK_ON,
+                                      A.SL_PROGRAM_COLOR_TEXTURED)
+            self.ubos.append(ubo)
+            self.shaders.append(ctx.create_shader(vs, fs, ubo))
+
+        # framebuffers: a gx x gy grid of size x size viewports each
+        self.targets = []
+        per = max(1, min(views_per_fbo, views))
+        gx = 1
+        while gx * gx < per:
+            gx *= 2
+        gy = (per + gx - 1) // gx
+        for f0 in range(0, views, per):
+            vs_ = list(range(f0, min(views, f0 + per)))
+            W, H = gx * size, gy * size
+            color = ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, W, H)
+            depth = ctx.create_texture(A.SL_COLOR_R_FLOAT, W, H)
+            vis = ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, W, H)
+            target = ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, W, H)
+            gimg = ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, W, H)
+            fbo = ctx.create_framebuffer([color], depth)
+            ctx.attach_visibility(fbo, vis)
+            cells = [((i % gx) * size, (i // gx) * size) for i in range(len(vs_))]
+            self.targets.append(dict(fbo=fbo, color=color, depth=depth, vis=vis, target=target, grad_image=gimg,
+                                     W=W, H=H, views=vs_, cells=cells))
+
+    # ---- parameters ----
+    def colors(self) -> np.ndarray:
+        return self.verts[:, 3:6].copy()
+
+    def set_colors(self, rgb: np.ndarray):
+        self.verts[:, 3:6] = rgb
+        self.ctx.vbo_assign(self.vbo, self.verts)
+
+    def set_color_value(self, vertex: int, channel: int, value: float):
+        self.verts[vertex, 3 + channel] = value
+        self.ctx.vbo_assign(self.vbo, self.verts[vertex:vertex + 1, 3 + channel:4 + channel].copy(),
+                            offset=vertex * 32 + 12 + 4 * channel)
+
+    def set_texels(self, texels: np.ndarray):
+        self.texels = np.ascontiguousarray(texels, dtype=np.float32)
+        self.ctx.texture_set_texels(self.tex, self.texels)
+
+    # ---- passes ----
+    def forward(self):
+        for t in self.targets:
+            self.ctx.clear_framebuffer(t["fbo"], 0, (0.0, 0.0, 0.0, 0.0), 1.0)
+            for v, (x, y) in zip(t["views"], t["cells"]):
+                self.ctx.set_viewport(x, y, self.size, self.size)
+                self.ctx.draw(self.mesh, self.shaders[v], t["fbo"])
+        self.ctx.set_viewport(0, 0, 65535, 65535)
+
+    def images(self):
+        self.ctx.finish()
+        return [self.ctx.texture_read(t["color"]) for t in self.targets]
+
+    def visibility(self):
+        self.ctx.finish()
+        return [self.ctx.texture_read(t["vis"]) for t in self.targets]
+
+    def set_targets(self, images):
+        for t, img in zip(self.targets, images):
+            self.ctx.texture_set_texels(t["target"], np.ascontiguousarray(img, dtype=np.float32))
+
+    def make_targets(self, std=0.1):
+        """Targets rendered from the colours and texels perturbed by N(0, std)."""
+        c0, t0 = self.colors(), self.texels.copy()
+        nc, nt = c0.size, t0.size
+        self.set_colors((c0 + std * gaussian(self.seed + 1, nc, stream=5).reshape(c0.shape)).astype(np.float32))
+        self.set_texels((t0 + std * gaussian(self.seed + 1, nt, stream=7).reshape(t0.shape)).astype(np.float32))
+        self.forward()
+        self.set_targets(self.images())
+        self.set_colors(c0)
+        self.set_texels(t0)
+
+    def loss_grad_images(self, scale=1.0):
+        for t in self.targets:
+            self.ctx.loss_l2_grad(t["color"], t["target"], t["grad_image"], scale)
+
+    def backward(self):
+        self.ctx.gradients_zero()
+        for t in self.targets:
+            self.ctx.backward(t["fbo"], t["grad_image"])
+
+    def gradients(self):
+        self.ctx.finish()
+        gc = self.ctx.vbo_read(self.grad_vbo, self.n_vertices * 16, dtype=np.float32).reshape(-1, 4)
+        gt = self.ctx.texture_read(self.grad_tex).reshape(self.tex_size, self.tex_size, 4)
+        return gc, gt
+
+    def loss64(self, images=None):
+        """sum((C - T)^2) in float64 over every view, from the read-back images."""
+        if images is None:
+            images = self.images()
+        tot = 0.0
+        for t, img in zip(self.targets, images):
+            tgt = self.ctx.texture_read(t["target"])
+            d = img.astype(np.float64) - tgt.astype(np.float64)
+            tot += float(np.sum(d * d))
+        return tot

@@ -1,0 +1,336 @@
+"""Differentiable draws: the visibility attachment and the fixed-visibility adjoint.
+
+The reference renders forward only, so nothing in it pins these; the checks
+come from mathematics and from the forward renderer itself:
+
+  visibility rebuild   with no clipped triangle, colour = (sum_k b_k a_k) (.)
+                       bilinear(texture, sum_k b_k uv_k), rebuilt in float32
+                       numpy from each pixel's visibility record, equals the
+                       rendered colour bit-exactly (catches wrong weights,
+                       weight order, or a wrong sequence -> triangle mapping)
+  adjoint vs FD        central finite differences of the float64 loss
+                       sum((C - T)^2) through the forward renderer, on vertex
+                       colours and texels, with views whose triangles are
+                       clipped by the frustum: rel. error <= 1e-3 (the loss is
+                       quadratic in each parameter, so central differences are
+                       exact up to float rounding)
+  directional FD       g . d against (L(x + h d) - L(x - h d)) / 2h for a
+                       random direction over every colour and texel at once
+  autograd wrapper     a torch.autograd.Function around the C ABI (test
+                       harness only) gives the same gradients as the loss
+                       kernel + backward path
+GPU (marked gpu): visibility bit-exact against the oracle, gradients against
+the oracle's scalar adjoint, finite differences on 256 random parameters
+through the GPU forward, and the full 32-view 512x512 configuration.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from softlight_amd import api as A
+from softlight_amd import scenes
+
+BACKENDS = [pytest.param("oracle", id="oracle"), pytest.param("gpu", id="gpu", marks=pytest.mark.gpu)]
+
+
+@pytest.fixture
+def make_ctx(request):
+    def _make(kind):
+        if kind == "oracle":
+            lib = request.getfixturevalue("oracle_lib")
+        else:
+            lib = request.getfixturevalue("gpu_lib")
+        return A.SLContext(lib)
+    return _make
+
+
+# ---------------------------------------------------------------------------
+# float32 restatement of the forward fragment (used only to rebuild pixels)
+# ---------------------------------------------------------------------------
+def _f32(x):
+    return np.float32(x)
+
+
+def _wrap(u):
+    u = np.asarray(u, dtype=np.float32)
+    return (np.where(u < 0, np.float32(1), np.float32(0)) + (u - np.trunc(u))).astype(np.float32)
+
+
+def _bilinear(tex, u, v):
+    """sl_sample_bilinear<RGBA32F, REPEAT> on flat float32 texel rows (with padding)."""
+    H, W = tex.shape[:2]
+    flat = np.zeros(((W + 4) * (H + 4) + 4, 4), dtype=np.float32)
+    flat[:W * H] = tex.reshape(-1, 4)
+    xf = (_wrap(u) * np.float32(W)).astype(np.float32)
+    yf = (_wrap(v) * np.float32(H)).astype(np.float32)
+    xi0 = (xf.astype(np.int64) & 0xFFFF)
+    yi0 = (yf.astype(np.int64) & 0xFFFF)
+    xi1 = np.minimum(xi0 + 1, W)
+    yi1 = np.minimum(yi0 + 1, H)
+    dx = (xf - xi0.astype(np.float32)).astype(np.float32)
+    dy = (yf - yi0.astype(np.float32)).astype(np.float32)
+    omdx = (np.float32(1) - dx).astype(np.float32)
+    omdy = (np.float32(1) - dy).astype(np.float32)
+    p0, p1 = flat[xi0 + W * yi0], flat[xi0 + W * yi1]
+    p2, p3 = flat[xi1 + W * yi0], flat[xi1 + W * yi1]
+    w0 = (p0 * omdx[:, None]) * omdy[:, None]
+    w1 = (p1 * omdx[:, None]) * dy[:, None]
+    w2 = (p2 * dx[:, None]) * omdy[:, None]
+    w3 = (p3 * dx[:, None]) * dy[:, None]
+    return (((w0 + w1) + w2) + w3).astype(np.float32)
+
+
+def rebuild(scene, vis_images):
+    """Colour of every covered pixel rebuilt from its visibility record."""
+    out = []
+    nt = scene.n_triangles
+    faces = scene.indices.reshape(-1, 3).astype(np.int64)
+    for t, vis in zip(scene.targets, vis_images):
+        seq = vis[..., 0].copy().view(np.uint32).astype(np.int64)
+        cov = seq > 0
+        s = seq[cov] - 1
+        view_slot = s // nt              # submission order: views in order, nt triangles each
+        tri = s - view_slot * nt
+        b = vis[..., 1:4][cov].astype(np.float32)
+        vid = faces[tri]
+        a = scene.verts[vid][:, :, 3:6].astype(np.float32)
+        uv = scene.verts[vid][:, :, 6:8].astype(np.float32)
+        col = ((a[:, 0] * b[:, 0:1]) + (a[:, 1] * b[:, 1:2])) + (a[:, 2] * b[:, 2:3])
+        alpha = ((b[:, 0] + b[:, 1]) + b[:, 2]).astype(np.float32)
+        u = ((uv[:, 0, 0] * b[:, 0]) + (uv[:, 1, 0] * b[:, 1])) + (uv[:, 2, 0] * b[:, 2])
+        v = ((uv[:, 0, 1] * b[:, 0]) + (uv[:, 1, 1] * b[:, 1])) + (uv[:, 2, 1] * b[:, 2])
+        tx = _bilinear(scene.texels, u, v)
+        rgba = np.concatenate([col * tx[:, :3], (alpha * tx[:, 3])[:, None]], axis=1).astype(np.float32)
+        out.append((cov, rgba, view_slot))
+    return out
+
+
+def _scene(ctx, **kw):
+    args = dict(level=3, views=2, size=64, views_per_fbo=2, tex_size=32, edge_views=1)
+    args.update(kw)
+    s = scenes.DiffScene(ctx, **args)
+    s.make_targets()
+    return s
+
+
+# ---------------------------------------------------------------------------
+# CPU (oracle) and GPU
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_visibility_rebuilds_the_forward_colour(make_ctx, backend):
+    ctx = make_ctx(backend)
+    s = _scene(ctx, level=3, views=4, size=64, views_per_fbo=4, edge_views=0)
+    ctx.reset_stats()
+    s.forward()
+    imgs, vis = s.images(), s.visibility()
+    n = 0
+    for (cov, rgba, _), img in zip(rebuild(s, vis), imgs):
+        assert cov.sum() > 500
+        np.testing.assert_array_equal(rgba.view(np.uint32), img[cov].view(np.uint32))
+        assert np.all(img[~cov] == 0)
+        n += cov.sum()
+    assert n > 2000
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_visibility_through_clipping(make_ctx, backend):
+    """Clipped triangles carry their source triangle's weights through the
+    clipper; rebuilt colours agree to float rounding."""
+    ctx = make_ctx(backend)
+    s = _scene(ctx, level=3, views=2, size=64, views_per_fbo=2, edge_views=2)
+    ctx.reset_stats()
+    s.forward()
+    imgs, vis = s.images(), s.visibility()
+    for (cov, rgba, _), img, v in zip(rebuild(s, vis), imgs, vis):
+        assert cov.sum() > 300
+        np.testing.assert_allclose(rgba, img[cov], rtol=0, atol=2e-5)
+        bsum = v[..., 1:4][cov].sum(-1)
+        np.testing.assert_allclose(bsum, 1.0, atol=1e-5)
+
+
+def _fd_check(s, params, h=1e-2):
+    s.forward()
+    s.loss_grad_images()
+    s.backward()
+    gc, gt = s.gradients()
+    fd, ad = [], []
+    for kind, idx in params:
+        if kind == "c":
+            v, ch = idx
+            x0 = float(s.verts[v, 3 + ch])
+            s.set_color_value(v, ch, x0 + h); s.forward(); lp = s.loss64()
+            s.set_color_value(v, ch, x0 - h); s.forward(); lm = s.loss64()
+            s.set_color_value(v, ch, x0)
+            ad.append(gc[v, ch])
+        else:
+            y, x, ch = idx
+            t0 = s.texels.copy()
+            tp = t0.copy(); tp[y, x, ch] += h
+            s.set_texels(tp); s.forward(); lp = s.loss64()
+            tm = t0.copy(); tm[y, x, ch] -= h
+            s.set_texels(tm); s.forward(); lm = s.loss64()
+            s.set_texels(t0)
+            ad.append(gt[y, x, ch])
+        fd.append((lp - lm) / (2 * h))
+    return np.array(fd), np.array(ad, dtype=np.float64)
+
+
+def _pick_params(s, n_color, n_texel, seed):
+    """Random colour and texel parameters, drawn among those the views see."""
+    s.forward()
+    s.loss_grad_images()
+    s.backward()
+    gc, gt = s.gradients()
+    rng = np.random.default_rng(seed)
+    cv = np.argwhere(np.abs(gc[:, :3]) > 0)
+    tv = np.argwhere(np.abs(gt) > 0)
+    out = [("c", tuple(cv[i])) for i in rng.choice(len(cv), min(n_color, len(cv)), replace=False)]
+    out += [("t", tuple(tv[i])) for i in rng.choice(len(tv), min(n_texel, len(tv)), replace=False)]
+    return out
+
+
+def _rel_err(fd, ad):
+    floor = 1e-3 * np.abs(fd).max()
+    return np.abs(fd - ad) / np.maximum(np.maximum(np.abs(fd), np.abs(ad)), floor)
+
+
+def test_oracle_adjoint_matches_finite_differences(oracle_lib):
+    ctx = A.SLContext(oracle_lib)
+    s = _scene(ctx, level=3, views=4, size=64, views_per_fbo=4, edge_views=2)
+    params = _pick_params(s, 60, 60, seed=1)
+    fd, ad = _fd_check(s, params)
+    err = _rel_err(fd, ad)
+    assert err.max() <= 1e-3, (err.max(), params[int(err.argmax())])
+
+
+def test_oracle_directional_derivative(oracle_lib):
+    ctx = A.SLContext(oracle_lib)
+    s = _scene(ctx, level=3, views=2, size=64, views_per_fbo=2, edge_views=1)
+    s.forward(); s.loss_grad_images(); s.backward()
+    gc, gt = s.gradients()
+    rng = np.random.default_rng(3)
+    dc = rng.standard_normal((s.n_vertices, 3)).astype(np.float32)
+    dt = rng.standard_normal(s.texels.shape).astype(np.float32)
+    c0, t0 = s.colors(), s.texels.copy()
+    h = 1e-3
+    s.set_colors(c0 + h * dc); s.set_texels(t0 + h * dt); s.forward(); lp = s.loss64()
+    s.set_colors(c0 - h * dc); s.set_texels(t0 - h * dt); s.forward(); lm = s.loss64()
+    s.set_colors(c0); s.set_texels(t0)
+    fd = (lp - lm) / (2 * h)
+    # the perturbed parameters are rounded to float32: use the actual steps
+    ad = float(np.sum(gc[:, :3].astype(np.float64) * dc) + np.sum(gt.astype(np.float64) * dt))
+    assert abs(fd - ad) <= 1e-3 * abs(ad), (fd, ad)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_backward_refuses_stale_draws(make_ctx, backend):
+    ctx = make_ctx(backend)
+    s = _scene(ctx, level=1, views=1, size=32, views_per_fbo=1, tex_size=8, edge_views=0)
+    s.forward(); s.loss_grad_images()
+    # re-initialising a resource the saved draws read invalidates them
+    ctx._check(ctx.lib.vbo_init(ctx.h, s.vbo, s.verts.nbytes, s.verts.ctypes.data_as(A.C.c_void_p)), "vbo_init")
+    with pytest.raises(A.SLError):
+        s.backward()
+    s.forward(); s.loss_grad_images(); s.backward()        # a new forward pass replaces them
+
+
+class RenderViews:
+    """torch.autograd.Function over a DiffScene (test harness only)."""
+
+    @staticmethod
+    def make(torch):
+        class _Fn(torch.autograd.Function):
+            @staticmethod
+            def forward(fctx, scene, colors, texels):
+                scene.set_colors(colors.detach().cpu().numpy().astype(np.float32))
+                scene.set_texels(texels.detach().cpu().numpy().astype(np.float32))
+                scene.forward()
+                fctx.scene = scene
+                return torch.from_numpy(np.stack(scene.images()))
+
+            @staticmethod
+            def backward(fctx, grad_images):
+                s = fctx.scene
+                g = grad_images.detach().cpu().numpy().astype(np.float32)
+                for t, gi in zip(s.targets, g):
+                    s.ctx.texture_set_texels(t["grad_image"], np.ascontiguousarray(gi))
+                s.backward()
+                gc, gt = s.gradients()
+                return None, torch.from_numpy(gc[:, :3].copy()), torch.from_numpy(gt.copy())
+        return _Fn
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_torch_autograd_wrapper(make_ctx, backend):
+    torch = pytest.importorskip("torch")
+    ctx = make_ctx(backend)
+    s = _scene(ctx, level=3, views=2, size=64, views_per_fbo=2, edge_views=1)
+    targets = torch.from_numpy(np.stack([ctx.texture_read(t["target"]) for t in s.targets]))
+    colors = torch.from_numpy(s.colors()).requires_grad_(True)
+    texels = torch.from_numpy(s.texels.copy()).requires_grad_(True)
+    fn = RenderViews.make(torch)
+    img = fn.apply(s, colors, texels)
+    loss = ((img - targets) ** 2).sum()
+    loss.backward()
+    # the same gradients through the library's own loss kernel
+    s.forward(); s.loss_grad_images(); s.backward()
+    gc, gt = s.gradients()
+    np.testing.assert_allclose(colors.grad.numpy(), gc[:, :3], rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(texels.grad.numpy(), gt, rtol=1e-6, atol=1e-6)
+    assert float(np.abs(gc).max()) > 0 and float(np.abs(gt).max()) > 0
+
+
+# ---------------------------------------------------------------------------
+# GPU against the oracle
+# ---------------------------------------------------------------------------
+def _pair(oracle_lib, gpu_lib, **kw):
+    o = _scene(A.SLContext(oracle_lib), **kw)
+    g = _scene(A.SLContext(gpu_lib), **kw)
+    return o, g
+
+
+def _grads(s):
+    s.ctx.reset_stats()
+    s.forward(); s.loss_grad_images(); s.backward()
+    return s.visibility(), s.gradients()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kw", [dict(level=4, views=4, size=128, views_per_fbo=4, tex_size=64, edge_views=1),
+                                dict(level=5, views=8, size=256, views_per_fbo=8, tex_size=512, edge_views=2)],
+                         ids=["level4_4x128", "level5_8x256"])
+def test_gpu_visibility_and_gradients_match_oracle(oracle_lib, gpu_lib, kw):
+    o, g = _pair(oracle_lib, gpu_lib, **kw)
+    vo, (gco, gto) = _grads(o)
+    vg, (gcg, gtg) = _grads(g)
+    for a, b in zip(vo, vg):
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    for ref, got in ((gco, gcg), (gto, gtg)):
+        scale = float(np.abs(ref).max())
+        assert scale > 0
+        assert float(np.abs(ref - got).max()) <= 1e-4 * scale
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_match_finite_differences_256(gpu_lib):
+    ctx = A.SLContext(gpu_lib)
+    s = _scene(ctx, level=5, views=8, size=256, views_per_fbo=8, tex_size=512, edge_views=2)
+    params = _pick_params(s, 128, 128, seed=7)
+    assert len(params) == 256
+    fd, ad = _fd_check(s, params)
+    err = _rel_err(fd, ad)
+    assert err.max() <= 1e-3, (err.max(), params[int(err.argmax())])
+
+
+@pytest.mark.gpu
+def test_gpu_full_size_views_match_oracle(oracle_lib, gpu_lib):
+    """32 views at 512x512 of the level-5 icosphere with a 512x512 texture."""
+    o, g = _pair(oracle_lib, gpu_lib, level=5, views=32, size=512, views_per_fbo=8, tex_size=512, edge_views=0)
+    vo, (gco, gto) = _grads(o)
+    vg, (gcg, gtg) = _grads(g)
+    for a, b in zip(vo, vg):
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    for ref, got in ((gco, gcg), (gto, gtg)):
+        scale = float(np.abs(ref).max())
+        assert float(np.abs(ref - got).max()) <= 1e-4 * scale
