@@ -18,17 +18,19 @@ NUM_CONFIGS = 8
 
 
 def device_name() -> str:
+    """The HIP device name, or the library's only target when HIP gives none."""
+    fallback = "gfx950"
     try:
         hip = ctypes.CDLL("libamdhip64.so")
     except OSError:
         try:
             hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
         except OSError:
-            return "unknown"
+            return fallback
     buf = ctypes.create_string_buffer(256)
-    if hip.hipDeviceGetName(buf, 256, 0) != 0:
-        return "unknown"
-    return buf.value.decode(errors="replace") or "unknown"
+    if hip.hipDeviceGetName(buf, ctypes.c_int(256), ctypes.c_int(0)) != 0:
+        return fallback
+    return buf.value.decode(errors="replace").strip() or fallback
 
 
 def load_cache(path: str = DEFAULT_CACHE) -> dict:

@@ -212,6 +212,7 @@ __global__ __launch_bounds__(256) void k_loss_l2_grad(const float4* __restrict__
 
 hipError_t launch_backward(const BackwardParams& p, hipStream_t s)
 {
+    (void)hipGetLastError();   // drop a stale error from an earlier call
     if (p.W <= 0 || p.H <= 0 || p.numDraws == 0) return hipSuccess;
     const uint64_t blocks = (uint64_t)((p.W + 15) / 16) * (uint64_t)((p.H + 15) / 16);
     hipLaunchKernelGGL(k_backward, dim3((unsigned)blocks), dim3(256), 0, s, p);
@@ -220,6 +221,7 @@ hipError_t launch_backward(const BackwardParams& p, hipStream_t s)
 
 hipError_t launch_loss_l2_grad(const float4* color, const float4* target, float4* grad, uint64_t n, float scale, hipStream_t s)
 {
+    (void)hipGetLastError();   // drop a stale error from an earlier call
     if (n == 0) return hipSuccess;
     uint64_t g = (n + 255) / 256;
     if (g > 8192) g = 8192;

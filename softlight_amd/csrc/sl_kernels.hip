@@ -333,7 +333,10 @@ __global__ __launch_bounds__(kSetupBlock) void k_setup(SetupParams P)
 
     // decoupled look-back by the first wave: each round reads the statuses of
     // the 64 nearest unread predecessors at once (ticket order guarantees every
-    // predecessor is running, so every wait ends)
+    // predecessor is running, so every wait ends).  A status word carries its
+    // own value and nothing else is read through it, so relaxed agent-scope
+    // atomics suffice: no L2 write-back or invalidate per round (the kernel
+    // boundary publishes the records to the next kernels).
     if (threadIdx.x < 64)
     {
         const int lane = threadIdx.x;
@@ -341,19 +344,19 @@ __global__ __launch_bounds__(kSetupBlock) void k_setup(SetupParams P)
         if (chunk == 0)
         {
             if (lane == 0)
-                __hip_atomic_store(&P.status[0], kFlagInc | blockTotal, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&P.status[0], kFlagInc | blockTotal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         else
         {
             if (lane == 0)
-                __hip_atomic_store(&P.status[chunk], kFlagAgg | blockTotal, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&P.status[chunk], kFlagAgg | blockTotal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             int64_t j = (int64_t)chunk - 1;          // nearest predecessor not yet accumulated
             uint32_t spins = 0;
             while (true)
             {
                 const int64_t idx = j - lane;
                 unsigned long long st = kFlagInc;    // before chunk 0: an inclusive zero
-                if (idx >= 0) st = __hip_atomic_load(&P.status[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if (idx >= 0) st = __hip_atomic_load(&P.status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const unsigned long long flag = st & ~kValMask;
                 const unsigned long long incMask = __ballot(flag == kFlagInc);
                 const unsigned long long zeroMask = __ballot(flag == 0);
@@ -377,7 +380,7 @@ __global__ __launch_bounds__(kSetupBlock) void k_setup(SetupParams P)
                 j -= 64;
             }
             if (lane == 0)
-                __hip_atomic_store(&P.status[chunk], kFlagInc | (prefix + blockTotal), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&P.status[chunk], kFlagInc | (prefix + blockTotal), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (lane == 0)
         {
@@ -654,6 +657,86 @@ __device__ __forceinline__ void record_barrier()
     }
 }
 
+// Shading of one fragment from its record and its barycentric values s0..s2
+// (perspective correction SL_FragmentProcessor.cpp:486-494, varyings :315-330,
+// then the fragment program).  `wv` receives the visibility slot's varying.
+__device__ __forceinline__ bool shade(const RasterParams& P, const DrawDesc& D, const float hw[3], float bc3, uint32_t rid,
+                                      float s0, float s1, float s2, float4& out, float4& wv)
+{
+    const float t0 = s0 * hw[0], t1 = s1 * hw[1], t2 = s2 * hw[2];
+    const float sinv = rcpf(((t0 + t1) + t2) + bc3);
+    const float c0 = t0 * sinv, c1 = t1 * sinv, c2 = t2 * sinv;
+    const float4* V = P.vars + (uint64_t)rid * P.varStride;
+    const uint32_t nvMax = P.varStride / 3;
+    Vary4 var;
+#pragma unroll
+    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+    {
+        if (i < D.nvary)
+        {
+            const float4 a = V[i], b = V[nvMax + i], e = V[2 * nvMax + i];
+            var.v[i] = f4(((a.x * c0) + (b.x * c1)) + (e.x * c2), ((a.y * c0) + (b.y * c1)) + (e.y * c2),
+                          ((a.z * c0) + (b.z * c1)) + (e.z * c2), ((a.w * c0) + (b.w * c1)) + (e.w * c2));
+        }
+        else
+        {
+            var.v[i] = f4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+    wv = f4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+        if ((uint32_t)i == D.visSlot) wv = var.v[i];
+    return (D.nout > 0) && fragment_program(D, P.uniforms, var, out);
+}
+
+template <int CF>
+__device__ __forceinline__ void blend_into(uint32_t* col8, float4* colF, int blendMode, float4 out)
+{
+    if (CF == 1)
+    {
+        if (blendMode == SL_BLEND_OFF)
+        {
+            const float o[4] = {out.x, out.y, out.z, out.w};
+            *col8 = pack_rgba8(o);
+        }
+        else
+        {
+            float dcol[4];
+            unpack_rgba8(*col8, dcol);
+            blend(blendMode, out, dcol);
+            *col8 = pack_rgba8(dcol);
+        }
+    }
+    if (CF == 2)
+    {
+        if (blendMode == SL_BLEND_OFF)
+        {
+            *colF = out;
+        }
+        else
+        {
+            const float4 cur = *colF;
+            float dcol[4] = {cur.x, cur.y, cur.z, cur.w};
+            blend(blendMode, out, dcol);
+            *colF = make_float4(dcol[0], dcol[1], dcol[2], dcol[3]);
+        }
+    }
+}
+
+constexpr uint32_t kNoPending = 0xFFFFFFFFu;
+
+// k_raster: one workgroup per screen tile.
+//
+// Shading is deferred where that cannot change the result: a fragment of an
+// unblended draw that passes the depth test only becomes the pixel's pending
+// fragment (its record and barycentric values, in LDS); a later passing
+// fragment replaces it, and it is shaded once -- when the tile ends, or before
+// a blended fragment reads the pixel's colour.  The pending fragment is shaded
+// with exactly the arithmetic an immediate shade would use, so colours are
+// unchanged, but the dependent global loads of varyings and texels are paid
+// once per pixel instead of once per record.
+//
 // AUX bit 0: debug buffers (last primitive, depth-pass count); bit 1: the
 // visibility attachment.
 template <int CF, int AUX, int TW, int TH, int SEGS>
@@ -670,6 +753,8 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     __shared__ uint32_t sLast[DBG ? TH : 1][TW];
     __shared__ uint32_t sCount[DBG ? TH : 1][TW];
     __shared__ float4 sVis[VIS ? TH : 1][TW];
+    __shared__ uint32_t sPend[TH][TW];                  // pending fragment's record, or kNoPending
+    __shared__ float sPendS[3][TH][TW];                 // its barycentric values
 
     const int tile = blockIdx.x;
     const int tx = tile % P.tilesX, ty = tile / P.tilesX;
@@ -694,8 +779,23 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
             sCount[r][c] = P.clearDepth ? 0u : (inb ? P.dbgCount[gi] : 0u);
         }
         if (VIS) sVis[r][c] = (P.clearDepth || !inb) ? make_float4(0.f, 0.f, 0.f, 0.f) : P.vis[gi];
+        sPend[r][c] = kNoPending;
     }
     __syncthreads();
+
+    // shades pixel (r, c)'s pending fragment into the tile (caller owns the pixel)
+    auto resolve = [&](int r, int c) {
+        const uint32_t prid = sPend[r][c];
+        if (prid == kNoPending) return;
+        sPend[r][c] = kNoPending;
+        const SetupRec& Q = P.recs[prid];
+        const DrawDesc& QD = P.draws[Q.draw];
+        const float qhw[3] = {Q.hw[0], Q.hw[1], Q.hw[2]};
+        float4 out, wv;
+        const bool have = shade(P, QD, qhw, Q.bc3, prid, sPendS[0][r][c], sPendS[1][r][c], sPendS[2][r][c], out, wv);
+        if (have) blend_into<CF>(&sCol8[CF == 1 ? r : 0][CF == 1 ? c : 0], &sColF[CF == 2 ? r : 0][CF == 2 ? c : 0], SL_BLEND_OFF, out);
+        if (VIS) sVis[r][c] = make_float4(__uint_as_float(Q.seq), wv.x, wv.y, wv.z);
+    };
 
     const int row = lane / (4 * SEGS);
     const int phase = lane & 3;
@@ -704,7 +804,6 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     const int y = y0 + row;
     const float yf = (float)y;
     const uint2 range = P.tileRange[tile];
-    const uint32_t nvMax = P.varStride / 3;
     uint32_t tested = 0, shaded = 0;
 
     // Records are staged into LDS kBatch at a time by the whole workgroup, so
@@ -742,6 +841,9 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                 if (x < xe)
                 {
                     const DrawDesc& D = P.draws[drawIdx];
+                    // deferred: unblended fragments with outputs, and any
+                    // fragment whose visibility record is wanted
+                    const bool defer = (D.nout > 0 && D.blend == SL_BLEND_OFF) || (VIS && D.nout == 0);
                     // bc(x) = fl(fl(x*dx) + fma(dy, y, c)), then += 4*dx per group of 4 pixels
                     const float fx = (float)xf;
                     float s0 = (fx * R.bx[0]) + fmaf(R.by[0], yf, R.bz[0]);
@@ -773,70 +875,23 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                         ++tested;
                         if (depth_test(D.dtest, z, sDepth[row][c]))
                         {
-                            if (D.nout > 0 || VIS)
+                            if (defer)
                             {
-                                // perspective correction (SL_FragmentProcessor.cpp:486-494)
-                                const float t0 = s0 * R.hw[0], t1 = s1 * R.hw[1], t2 = s2 * R.hw[2];
-                                const float sinv = rcpf(((t0 + t1) + t2) + R.bc3);
-                                const float c0 = t0 * sinv, c1 = t1 * sinv, c2 = t2 * sinv;
-                                const float4* V = P.vars + (uint64_t)rid * P.varStride;
-                                Vary4 var;
-#pragma unroll
-                                for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
-                                {
-                                    if (i < D.nvary)
-                                    {
-                                        const float4 a = V[i], b = V[nvMax + i], e = V[2 * nvMax + i];
-                                        var.v[i] = f4(((a.x * c0) + (b.x * c1)) + (e.x * c2), ((a.y * c0) + (b.y * c1)) + (e.y * c2),
-                                                      ((a.z * c0) + (b.z * c1)) + (e.z * c2), ((a.w * c0) + (b.w * c1)) + (e.w * c2));
-                                    }
-                                    else
-                                    {
-                                        var.v[i] = f4(0.f, 0.f, 0.f, 0.f);
-                                    }
-                                }
-                                if (VIS)
-                                {
-                                    float4 wv = f4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-                                    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
-                                        if ((uint32_t)i == D.visSlot) wv = var.v[i];
-                                    sVis[row][c] = make_float4(__uint_as_float(R.seq), wv.x, wv.y, wv.z);
-                                }
-                                float4 out;
-                                const bool have = (D.nout > 0) && fragment_program(D, P.uniforms, var, out);
-                                if (have)
-                                {
-                                    if (CF == 1)
-                                    {
-                                        if (D.blend == SL_BLEND_OFF)
-                                        {
-                                            const float o[4] = {out.x, out.y, out.z, out.w};
-                                            sCol8[row][c] = pack_rgba8(o);
-                                        }
-                                        else
-                                        {
-                                            float dcol[4];
-                                            unpack_rgba8(sCol8[row][c], dcol);
-                                            blend(D.blend, out, dcol);
-                                            sCol8[row][c] = pack_rgba8(dcol);
-                                        }
-                                    }
-                                    if (CF == 2)
-                                    {
-                                        if (D.blend == SL_BLEND_OFF)
-                                        {
-                                            sColF[row][c] = out;
-                                        }
-                                        else
-                                        {
-                                            const float4 cur = sColF[row][c];
-                                            float dcol[4] = {cur.x, cur.y, cur.z, cur.w};
-                                            blend(D.blend, out, dcol);
-                                            sColF[row][c] = make_float4(dcol[0], dcol[1], dcol[2], dcol[3]);
-                                        }
-                                    }
-                                }
+                                sPend[row][c] = rid;
+                                sPendS[0][row][c] = s0;
+                                sPendS[1][row][c] = s1;
+                                sPendS[2][row][c] = s2;
+                            }
+                            else if (D.nout > 0)
+                            {
+                                // a blended fragment reads the colour: settle the pixel first
+                                resolve(row, c);
+                                const float hw[3] = {R.hw[0], R.hw[1], R.hw[2]};
+                                float4 out, wv;
+                                const bool have = shade(P, D, hw, R.bc3, rid, s0, s1, s2, out, wv);
+                                if (have) blend_into<CF>(&sCol8[CF == 1 ? row : 0][CF == 1 ? c : 0],
+                                                         &sColF[CF == 2 ? row : 0][CF == 2 ? c : 0], D.blend, out);
+                                if (VIS) sVis[row][c] = make_float4(__uint_as_float(R.seq), wv.x, wv.y, wv.z);
                             }
                             if (D.dmask == SL_DEPTH_MASK_ON) sDepth[row][c] = z;
                             if (DBG)
@@ -859,10 +914,11 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     }
     __syncthreads();
 
-    // tile store
+    // shade what is still pending, then store the tile
     for (int k = lane; k < TW * TH; k += NT)
     {
         const int r = k / TW, c = k % TW;
+        resolve(r, c);
         const int gx = x0 + c, gy = y0 + r;
         if (gx < P.W && gy < P.H)
         {
@@ -907,6 +963,7 @@ static inline int grid_for(uint64_t n, int block, int cap = 4096)
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s)
 {
+    (void)hipGetLastError();   // drop a stale error from an earlier call
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
     return hipGetLastError();
@@ -914,6 +971,7 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s)
 
 hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, uint64_t n, hipStream_t s)
 {
+    (void)hipGetLastError();   // drop a stale error from an earlier call
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, s, p, v, n);
     return hipGetLastError();
@@ -921,12 +979,14 @@ hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, uint64_t
 
 hipError_t launch_setup(const SetupParams& p, uint64_t chunks, hipStream_t s)
 {
+    (void)hipGetLastError();   // drop a stale error from an earlier call
     hipLaunchKernelGGL(k_setup, dim3((unsigned)chunks), dim3(kSetupBlock), 0, s, p);
     return hipGetLastError();
 }
 
 hipError_t launch_clip(const SetupParams& p, uint32_t jobs, hipStream_t s)
 {
+    (void)hipGetLastError();   // drop a stale error from an earlier call
     if (jobs == 0) return hipSuccess;
     hipLaunchKernelGGL(k_clip, dim3(grid_for(jobs, 64, 8192)), dim3(64), 0, s, p);
     return hipGetLastError();
@@ -948,6 +1008,7 @@ hipError_t launch_binning(const uint32_t* keys, const uint32_t* vals, const Fram
                           BinPlan plan, uint32_t* counts, uint32_t* totals, uint2* range, uint32_t* out,
                           int numTiles, hipStream_t s)
 {
+    (void)hipGetLastError();   // drop a stale error from an earlier call
     const size_t lds = sizeof(uint32_t) * (size_t)numTiles;
     hipLaunchKernelGGL(k_bin_count, dim3(plan.blocks), dim3(256), lds, s, keys, ctrl, cap, plan.per, counts, numTiles);
     hipLaunchKernelGGL(k_bin_colscan, dim3((numTiles + kColTiles - 1) / kColTiles), dim3(kColTiles * kColParts), 0, s,
@@ -977,6 +1038,7 @@ static void launch_raster_cfg(const RasterParams& p, int numTiles, int cfg, hipS
 
 hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s)
 {
+    (void)hipGetLastError();   // drop a stale error from an earlier call
     const int cf = (p.color == nullptr) ? 0 : (p.colorType == SL_COLOR_RGBA_FLOAT ? 2 : 1);
     if (p.vis != nullptr)
     {
