@@ -125,13 +125,13 @@ def run_gpu(steps, warmup, workload="spheres", raster_config=None, tune=False):
     return scene, st, eager, gst, (t1 - t0)
 
 
-def run_views(steps, warmup):
+def run_views(steps, warmup, tune=False):
     """Differentiable views: 32 cameras at 512x512 of a level-5 icosphere
     (20,480 triangles) with per-vertex colours and a 512x512 RGBA32F texture.
     Forward = clears + 32 draws (4 framebuffers of 8 viewports); backward =
     loss gradient + gradient zeroing + the fixed-visibility adjoint."""
     import softlight_amd
-    from softlight_amd import api, scenes
+    from softlight_amd import api, autotune, scenes
 
     ctx = api.SLContext(softlight_amd.load_library())
     sc = scenes.DiffScene(ctx, **VIEWS_WORKLOAD)
@@ -140,6 +140,16 @@ def run_views(steps, warmup):
     def fwd():
         sc.forward()
         ctx.finish()
+
+    signature = f"views:{sc.views}x{sc.size}:{sc.n_triangles}"
+    cache = os.environ.get("SL_TUNE_CACHE", autotune.DEFAULT_CACHE)
+    if tune:
+        res = autotune.tune(ctx, fwd, signature, path=cache)
+        print(json.dumps({"autotune": signature, **res}), file=sys.stderr, flush=True)
+    else:
+        cached = autotune.lookup(signature, cache)
+        if cached is not None:
+            ctx.set_raster_config(cached)
 
     def fwd_bwd():
         sc.forward()
@@ -159,6 +169,7 @@ def run_views(steps, warmup):
         st = ctx.stats()
         out[name] = dict(ms=(t1 - t0) * 1e3 / steps, views_per_s=sc.views * steps / (t1 - t0),
                          device_ms_forward=st["ms_total"] / steps, device_ms_backward=st["ms_backward"] / steps)
+    out["raster_config"] = ctx.raster_config()
     return sc, out
 
 
@@ -317,11 +328,12 @@ def views_summary(vr):
         "ms_fwd_bwd": round(fb["ms"], 4),
         "device_ms_backward": round(fb["device_ms_backward"], 4),
         "device_ms_forward": round(f["device_ms_forward"], 4),
+        "raster_tile": list(vr["raster_config"][1]),
     }
 
 
 def main_views(args):
-    sc, vr = run_views(args.steps, args.warmup)
+    sc, vr = run_views(args.steps, args.warmup, args.tune)
     fb = vr["fwd_bwd"]
     out = {
         "metric": "Mtri/s + shaded Gfrag/s at 1080p; fwd+bwd views/s vs plain fwd, 1 MI355X",

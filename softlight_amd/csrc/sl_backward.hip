@@ -9,8 +9,7 @@
 // forward computed them, then scatters
 //   dL/d colour[v_k] += b_k * (dL/dC (.) texel)          (per channel)
 //   dL/d texel[tap_j] += w_j * (dL/dC (.) colour)         (bilinear weight w_j)
-// with float atomics.  Within a wave, lanes that hit the same vertex or texel
-// are combined first (one atomic per distinct address and channel).
+// with float atomics, pre-aggregated per 16 x 16 pixel block in LDS.
 //
 // No reference counterpart: the reference renders forward only.
 #include "sl_device.h"
@@ -32,56 +31,87 @@ __device__ __forceinline__ uint32_t find_grad_draw(const GradDesc* D, uint32_t n
     return lo;
 }
 
-// Adds v (c channels) to dst[4*key + ...] for every lane with active set.
-// Lanes of the wave that share a key are summed first with a butterfly over
-// the wave (non-members contribute zero); the lowest such lane issues the
-// atomics.  Every lane of the wave must call this together.
-template <int C>
-__device__ __forceinline__ void wave_scatter_add(float* dst, uint64_t key, bool active, const float (&v)[C])
+// ---------------------------------------------------------------------------
+// Workgroup pre-aggregation: each workgroup (16 x 16 pixels) sums its vertex
+// and texel contributions in LDS hash tables (open addressing, LDS atomics),
+// then adds each distinct entry to HBM once per channel.  A table serves one
+// destination buffer (the first one a lane claims); lanes of other draws, and
+// keys that find no free slot within kProbes, add to HBM directly.
+// ---------------------------------------------------------------------------
+constexpr int kBwdTile = 16;
+constexpr int kVSlots = 512;          // vertex table (a 16 x 16 tile touches at most 768 vertices)
+constexpr int kTSlots = 1024;         // texel table (at most 1024 bilinear taps)
+constexpr int kProbes = 16;
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+
+struct GradTable
 {
-    const int lane = threadIdx.x & 63;
-    unsigned long long remaining = __ballot(active);
-    while (remaining)
+    uint32_t* keys;
+    float4* vals;
+    int bits;                          // log2 of the slot count
+};
+
+// Claims the table for buf (or finds it claimed for buf).
+__device__ __forceinline__ bool claim(unsigned long long* owner, const float* buf)
+{
+    const unsigned long long mine = reinterpret_cast<unsigned long long>(buf);
+    const unsigned long long prev = atomicCAS(owner, 0ull, mine);
+    return prev == 0ull || prev == mine;
+}
+
+template <int C>
+__device__ __forceinline__ void add_global(float* dst, uint32_t key, const float (&v)[C])
+{
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        if (v[c] != 0.f) unsafeAtomicAdd(dst + 4ull * key + c, v[c]);
+}
+
+template <int C>
+__device__ __forceinline__ bool table_add(const GradTable& t, uint32_t key, const float (&v)[C])
+{
+    uint32_t h = (key * 2654435761u) >> (32 - t.bits);
+    const uint32_t mask = (1u << t.bits) - 1u;
+    for (int probe = 0; probe < kProbes; ++probe)
     {
-        const int leader = __ffsll((long long)remaining) - 1;
-        const uint64_t k = __shfl(key, leader, 64);
-        const bool mine = active && key == k;
-        const unsigned long long m = __ballot(mine);
-        if (__popcll(m) == 1)
+        const uint32_t prev = atomicCAS(&t.keys[h], kEmptyKey, key);
+        if (prev == kEmptyKey || prev == key)
         {
-            if (lane == leader)
-            {
+            float* d = reinterpret_cast<float*>(&t.vals[h]);
 #pragma unroll
-                for (int c = 0; c < C; ++c) unsafeAtomicAdd(dst + 4 * k + c, v[c]);
-            }
+            for (int c = 0; c < C; ++c)
+                if (v[c] != 0.f) atomicAdd(d + c, v[c]);
+            return true;
         }
-        else
-        {
-            float s[C];
-#pragma unroll
-            for (int c = 0; c < C; ++c) s[c] = mine ? v[c] : 0.f;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1)
-            {
-#pragma unroll
-                for (int c = 0; c < C; ++c) s[c] += __shfl_xor(s[c], off, 64);
-            }
-            if (lane == leader)
-            {
-#pragma unroll
-                for (int c = 0; c < C; ++c) unsafeAtomicAdd(dst + 4 * k + c, s[c]);
-            }
-        }
-        remaining &= ~m;
+        h = (h + 1u) & mask;
     }
+    return false;
+}
+
+template <int C>
+__device__ __forceinline__ void scatter(const GradTable& t, bool useTable, float* buf, uint32_t key, const float (&v)[C])
+{
+    if (useTable && table_add<C>(t, key, v)) return;
+    add_global<C>(buf, key, v);
 }
 
 // One lane per pixel; a workgroup covers a 16 x 16 block (a wave 16 x 4).
 __global__ __launch_bounds__(256) void k_backward(BackwardParams P)
 {
-    const int bxN = (P.W + 15) >> 4;
-    const int x = (int)(blockIdx.x % bxN) * 16 + (int)(threadIdx.x & 15);
-    const int y = (int)(blockIdx.x / bxN) * 16 + (int)(threadIdx.x >> 4);
+    __shared__ uint32_t sVKey[kVSlots];
+    __shared__ float4 sVVal[kVSlots];
+    __shared__ uint32_t sTKey[kTSlots];
+    __shared__ float4 sTVal[kTSlots];
+    __shared__ unsigned long long sVOwner, sTOwner;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kVSlots; i += 256) { sVKey[i] = kEmptyKey; sVVal[i] = make_float4(0.f, 0.f, 0.f, 0.f); }
+    for (int i = tid; i < kTSlots; i += 256) { sTKey[i] = kEmptyKey; sTVal[i] = make_float4(0.f, 0.f, 0.f, 0.f); }
+    if (tid == 0) { sVOwner = 0ull; sTOwner = 0ull; }
+    __syncthreads();
+
+    const int bxN = (P.W + kBwdTile - 1) / kBwdTile;
+    const int x = (int)(blockIdx.x % bxN) * kBwdTile + (tid & (kBwdTile - 1));
+    const int y = (int)(blockIdx.x / bxN) * kBwdTile + (tid / kBwdTile);
     const bool inb = x < P.W && y < P.H;
     const uint64_t p = (uint64_t)y * P.W + x;
 
@@ -121,10 +151,10 @@ __global__ __launch_bounds__(256) void k_backward(BackwardParams P)
     float4 col = make_float4(0.f, 0.f, 0.f, 0.f), tex = make_float4(1.f, 1.f, 1.f, 1.f);
     BilinearTaps taps{};
     bool textured = false;
+    const bool ct = live && G->d.fsProg == SL_PROGRAM_COLOR_TEXTURED;
     if (live)
     {
         const DrawDesc& D = G->d;
-        const bool ct = D.fsProg == SL_PROGRAM_COLOR_TEXTURED;
         float4 a[3], uv[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k)
@@ -151,49 +181,62 @@ __global__ __launch_bounds__(256) void k_backward(BackwardParams P)
     }
 
     // dL/d colour varying = g (.) texel; colour attribute k gets b_k of it.
-    // COLOR_TEXTURED's alpha varying is the constant 1: no gradient.
-    const bool colGrad = live && G->gradCol != nullptr;
-    const bool ct = live && G->d.fsProg == SL_PROGRAM_COLOR_TEXTURED;
-    const float gc[4] = {g.x * tex.x, g.y * tex.y, g.z * tex.z, ct ? 0.f : g.w * tex.w};
-    float* gradCol = colGrad ? G->gradCol : nullptr;
-    // lanes of a wave may belong to different draws: the scatter key is
-    // per destination buffer, so group by (buffer, index)
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
+    // COLOR_TEXTURED's alpha varying is the constant 1: no gradient there.
+    if (live && G->gradCol != nullptr)
     {
-        const bool act = colGrad && vid[k] < G->gradColCount;
-        const float v[4] = {b[k] * gc[0], b[k] * gc[1], b[k] * gc[2], b[k] * gc[3]};
-        // one pass per distinct destination buffer in the wave
-        unsigned long long pending = __ballot(act);
-        while (pending)
+        const GradTable tv{sVKey, sVVal, 9};
+        const bool useTable = claim(&sVOwner, G->gradCol);
+        const float gc[4] = {g.x * tex.x, g.y * tex.y, g.z * tex.z, ct ? 0.f : g.w * tex.w};
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
         {
-            const int leader = __ffsll((long long)pending) - 1;
-            float* buf = reinterpret_cast<float*>(__shfl(reinterpret_cast<uint64_t>(gradCol), leader, 64));
-            const bool same = act && gradCol == buf;
-            pending &= ~__ballot(same);
-            wave_scatter_add<4>(buf, vid[k], same, v);
+            if (vid[k] >= G->gradColCount) continue;
+            const float v[4] = {b[k] * gc[0], b[k] * gc[1], b[k] * gc[2], b[k] * gc[3]};
+            scatter<4>(tv, useTable, G->gradCol, (uint32_t)vid[k], v);
         }
     }
 
     // dL/d texel = w_j * (g (.) colour varying)
-    const bool texGrad = textured && G->gradTex != nullptr;
-    float* gradTex = texGrad ? G->gradTex : nullptr;
-    const float gt[4] = {g.x * col.x, g.y * col.y, g.z * col.z, g.w * col.w};
-    const uint32_t ti[4] = {taps.i0, taps.i1, taps.i2, taps.i3};
-    const float tw[4] = {taps.omdx * taps.omdy, taps.omdx * taps.dy, taps.dx * taps.omdy, taps.dx * taps.dy};
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
+    if (textured && G->gradTex != nullptr)
     {
-        const bool act = texGrad && ti[j] < G->gradTexTexels;
-        const float v[4] = {tw[j] * gt[0], tw[j] * gt[1], tw[j] * gt[2], tw[j] * gt[3]};
-        unsigned long long pending = __ballot(act);
-        while (pending)
+        const GradTable tt{sTKey, sTVal, 10};
+        const bool useTable = claim(&sTOwner, G->gradTex);
+        const float gt[4] = {g.x * col.x, g.y * col.y, g.z * col.z, g.w * col.w};
+        const uint32_t ti[4] = {taps.i0, taps.i1, taps.i2, taps.i3};
+        const float tw[4] = {taps.omdx * taps.omdy, taps.omdx * taps.dy, taps.dx * taps.omdy, taps.dx * taps.dy};
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
         {
-            const int leader = __ffsll((long long)pending) - 1;
-            float* buf = reinterpret_cast<float*>(__shfl(reinterpret_cast<uint64_t>(gradTex), leader, 64));
-            const bool same = act && gradTex == buf;
-            pending &= ~__ballot(same);
-            wave_scatter_add<4>(buf, ti[j], same, v);
+            if (ti[j] >= G->gradTexTexels) continue;
+            const float v[4] = {tw[j] * gt[0], tw[j] * gt[1], tw[j] * gt[2], tw[j] * gt[3]};
+            scatter<4>(tt, useTable, G->gradTex, ti[j], v);
+        }
+    }
+    __syncthreads();
+
+    // flush the tables: one HBM atomic per distinct entry and non-zero channel
+    float* vbuf = reinterpret_cast<float*>(sVOwner);
+    float* tbuf = reinterpret_cast<float*>(sTOwner);
+    if (vbuf != nullptr)
+    {
+        for (int i = tid; i < kVSlots; i += 256)
+        {
+            const uint32_t key = sVKey[i];
+            if (key == kEmptyKey) continue;
+            const float4 a = sVVal[i];
+            const float v[4] = {a.x, a.y, a.z, a.w};
+            add_global<4>(vbuf, key, v);
+        }
+    }
+    if (tbuf != nullptr)
+    {
+        for (int i = tid; i < kTSlots; i += 256)
+        {
+            const uint32_t key = sTKey[i];
+            if (key == kEmptyKey) continue;
+            const float4 a = sTVal[i];
+            const float v[4] = {a.x, a.y, a.z, a.w};
+            add_global<4>(tbuf, key, v);
         }
     }
 }
@@ -214,7 +257,7 @@ hipError_t launch_backward(const BackwardParams& p, hipStream_t s)
 {
     (void)hipGetLastError();   // drop a stale error from an earlier call
     if (p.W <= 0 || p.H <= 0 || p.numDraws == 0) return hipSuccess;
-    const uint64_t blocks = (uint64_t)((p.W + 15) / 16) * (uint64_t)((p.H + 15) / 16);
+    const uint64_t blocks = (uint64_t)((p.W + kBwdTile - 1) / kBwdTile) * (uint64_t)((p.H + kBwdTile - 1) / kBwdTile);
     hipLaunchKernelGGL(k_backward, dim3((unsigned)blocks), dim3(256), 0, s, p);
     return hipGetLastError();
 }

@@ -113,9 +113,18 @@ struct SetupRec
     int32_t minY, maxY;
     uint32_t draw;
     uint32_t seq;
-    uint32_t pad[2];
+    uint32_t state;            // the draw's raster state, see pack_raster_state
+    uint32_t pad;
 };
 static_assert(sizeof(SetupRec) == 144, "SetupRec layout");
+
+// Depth function, depth mask, blend mode and "has colour outputs" of a draw,
+// packed into each of its records so the raster loop reads no draw table.
+__host__ __device__ inline uint32_t pack_raster_state(const DrawDesc& d)
+{
+    return (uint32_t)(d.dtest & 7u) | ((uint32_t)(d.dmask & 1u) << 3) | ((uint32_t)(d.blend & 7u) << 4) |
+           ((d.nout > 0 ? 1u : 0u) << 7);
+}
 
 struct ClipJob
 {

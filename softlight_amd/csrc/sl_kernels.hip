@@ -63,40 +63,51 @@ struct TriIn
 {
     float4 pos[3];
     Vary4 var[3];
+    uint32_t vid[3];          // vertex ids (element indices < 2^32, checked on the host)
+    uint32_t inst;
     uint32_t draw;
     uint32_t seq;
     int status;               // 0 culled/invisible, 1 partial, 3 full
 };
 
-// Fetch, vertex programs, cull, scissor matrix and clip status of one triangle
-// (SL_TriProcessor::process_verts, SL_TriProcessor.cpp:574-631).
-__device__ __forceinline__ void load_triangle_in(const SetupParams& P, uint64_t g, const DrawDesc& D, uint32_t di, TriIn& T)
+// The three vertex programs' varyings (and, with a visibility attachment, the
+// source-vertex weights, which travel as one more varying so that clipping
+// and perspective correction treat them like the others).
+__device__ __forceinline__ void load_varyings(const SetupParams& P, const DrawDesc& D, TriIn& T)
 {
-    const uint64_t t = g - D.triBase;
-    const uint64_t inst = t / D.trisPerInstance;
-    const uint64_t ti = t - inst * D.trisPerInstance;
-    const uint64_t e = D.elementBegin + 3u * ti;
-    uint64_t vid[3] = {e, e + 1, e + 2};
-    if (D.indexed)
-    {
-        vid[0] = fetch_index(D, e);
-        vid[1] = fetch_index(D, e + 1);
-        vid[2] = fetch_index(D, e + 2);
-    }
-    T.draw = di;
-    T.seq = D.seqBase + 1u + (uint32_t)t;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) T.pos[k] = vertex_program(D, P.uniforms, vid[k], inst, T.var[k]);
+    for (int k = 0; k < 3; ++k) (void)vertex_program(D, P.uniforms, T.vid[k], T.inst, T.var[k]);
     if (D.visSlot != kNoVisSlot)
     {
-        // the source triangle's vertex weights travel as one more varying, so
-        // clipping and perspective correction treat them like the others
 #pragma unroll
         for (int k = 0; k < 3; ++k)
 #pragma unroll
             for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
                 if ((uint32_t)i == D.visSlot) T.var[k].v[i] = f4(k == 0 ? 1.f : 0.f, k == 1 ? 1.f : 0.f, k == 2 ? 1.f : 0.f, 0.f);
     }
+}
+
+// Fetch, vertex positions, cull, scissor matrix and clip status of one
+// triangle (SL_TriProcessor::process_verts, SL_TriProcessor.cpp:574-631).
+// Varyings are left to load_varyings: most triangles never need them.
+__device__ __forceinline__ void load_triangle_pos(const SetupParams& P, uint64_t g, const DrawDesc& D, uint32_t di, TriIn& T)
+{
+    const uint64_t t = g - D.triBase;
+    const uint64_t inst = t / D.trisPerInstance;
+    const uint64_t ti = t - inst * D.trisPerInstance;
+    const uint64_t e = D.elementBegin + 3u * ti;
+    T.vid[0] = (uint32_t)e; T.vid[1] = (uint32_t)(e + 1); T.vid[2] = (uint32_t)(e + 2);
+    if (D.indexed)
+    {
+        T.vid[0] = (uint32_t)fetch_index(D, e);
+        T.vid[1] = (uint32_t)fetch_index(D, e + 1);
+        T.vid[2] = (uint32_t)fetch_index(D, e + 2);
+    }
+    T.inst = (uint32_t)inst;
+    T.draw = di;
+    T.seq = D.seqBase + 1u + (uint32_t)t;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) T.pos[k] = vertex_position(D, P.uniforms, T.vid[k], inst);
 
     if (D.cull != SL_CULL_OFF)
     {
@@ -113,7 +124,9 @@ __device__ __forceinline__ void load_triangle_in(const SetupParams& P, uint64_t 
 __device__ __forceinline__ void load_triangle(const SetupParams& P, uint64_t g, TriIn& T)
 {
     const uint32_t di = find_draw(P.draws, P.numDraws, g);
-    load_triangle_in(P, g, P.draws[di], di, T);
+    const DrawDesc& D = P.draws[di];
+    load_triangle_pos(P, g, D, di, T);
+    load_varyings(P, D, T);
 }
 
 __device__ __forceinline__ float4 clip_edge(int e)
@@ -208,10 +221,12 @@ __device__ int clip_polygon(const float4* inPos, const Vary4* inVar, int nv, flo
 
 __device__ __forceinline__ void write_record(const SetupParams& P, uint32_t recId, uint32_t pairOff,
                                              float4 p0, float4 p1, float4 p2, const Vary4& a, const Vary4& b,
-                                             const Vary4& c, int nv, uint32_t draw, uint32_t seq, const TileRect& tr)
+                                             const Vary4& c, int nv, uint32_t draw, uint32_t seq, uint32_t state,
+                                             const TileRect& tr)
 {
     SetupRec r;
     setup_record(p0, p1, p2, draw, seq, r);
+    r.state = state;
     P.recs[recId] = r;
     float4* V = P.vars + (uint64_t)recId * P.varStride;
     const uint32_t nvMax = P.varStride / 3;
@@ -250,7 +265,8 @@ __device__ __forceinline__ unsigned long long wave_incl_scan(unsigned long long 
     return v;
 }
 
-__global__ __launch_bounds__(kSetupBlock) void k_setup(SetupParams P)
+// 6 waves per SIMD: caps the register budget at 80 VGPRs
+__global__ __launch_bounds__(kSetupBlock, 6) void k_setup(SetupParams P)
 {
     __shared__ uint32_t sChunk;
     __shared__ unsigned long long sWave[kSetupBlock / 64];
@@ -285,7 +301,7 @@ __global__ __launch_bounds__(kSetupBlock) void k_setup(SetupParams P)
     const DrawDesc& DU = sD;
     if (g < P.totalTris && g - DU.triBase < DU.numTris)
     {
-        load_triangle_in(P, g, DU, drawIdx, T);
+        load_triangle_pos(P, g, DU, drawIdx, T);
         if (T.status == 3)
         {
             const DrawDesc& D = DU;
@@ -406,8 +422,9 @@ __global__ __launch_bounds__(kSetupBlock) void k_setup(SetupParams P)
     }
     if (!clipped)
     {
+        load_varyings(P, DU, T);
         write_record(P, recOff, (uint32_t)pairOff, sp[0], sp[1], sp[2], T.var[0], T.var[1], T.var[2], DU.nvary,
-                     T.draw, T.seq, tr);
+                     T.draw, T.seq, pack_raster_state(DU), tr);
     }
     else
     {
@@ -452,7 +469,7 @@ __global__ __launch_bounds__(64) void k_clip(SetupParams P)
             if (!bin_accept(poly[0], poly[i + 1], poly[i + 2])) continue;
             const TileRect r = tile_rect(poly[0], poly[i + 1], poly[i + 2], P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
             write_record(P, rec, pair, poly[0], poly[i + 1], poly[i + 2], pvar[0], pvar[i + 1], pvar[i + 2], nv,
-                         T.draw, T.seq, r);
+                         T.draw, T.seq, pack_raster_state(D), r);
             ++rec;
             pair += tile_count(r);
         }
@@ -840,10 +857,14 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                 int x = xf + 4 * g0;
                 if (x < xe)
                 {
-                    const DrawDesc& D = P.draws[drawIdx];
+                    const uint32_t st = R.state;
+                    const int dtest = (int)(st & 7u);
+                    const bool dmask = ((st >> 3) & 1u) != 0;
+                    const int blendMode = (int)((st >> 4) & 7u);
+                    const bool hasOut = ((st >> 7) & 1u) != 0;
                     // deferred: unblended fragments with outputs, and any
                     // fragment whose visibility record is wanted
-                    const bool defer = (D.nout > 0 && D.blend == SL_BLEND_OFF) || (VIS && D.nout == 0);
+                    const bool defer = (hasOut && blendMode == SL_BLEND_OFF) || (VIS && !hasOut);
                     // bc(x) = fl(fl(x*dx) + fma(dy, y, c)), then += 4*dx per group of 4 pixels
                     const float fx = (float)xf;
                     float s0 = (fx * R.bx[0]) + fmaf(R.by[0], yf, R.bz[0]);
@@ -873,7 +894,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                         const float z = (((R.z[0] * s0) + (R.z[1] * s1)) + (R.z[2] * s2)) + R.bc3;
                         const int c = x - x0;
                         ++tested;
-                        if (depth_test(D.dtest, z, sDepth[row][c]))
+                        if (depth_test(dtest, z, sDepth[row][c]))
                         {
                             if (defer)
                             {
@@ -882,18 +903,19 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                                 sPendS[1][row][c] = s1;
                                 sPendS[2][row][c] = s2;
                             }
-                            else if (D.nout > 0)
+                            else if (hasOut)
                             {
                                 // a blended fragment reads the colour: settle the pixel first
                                 resolve(row, c);
+                                const DrawDesc& D = P.draws[drawIdx];
                                 const float hw[3] = {R.hw[0], R.hw[1], R.hw[2]};
                                 float4 out, wv;
                                 const bool have = shade(P, D, hw, R.bc3, rid, s0, s1, s2, out, wv);
                                 if (have) blend_into<CF>(&sCol8[CF == 1 ? row : 0][CF == 1 ? c : 0],
-                                                         &sColF[CF == 2 ? row : 0][CF == 2 ? c : 0], D.blend, out);
+                                                         &sColF[CF == 2 ? row : 0][CF == 2 ? c : 0], blendMode, out);
                                 if (VIS) sVis[row][c] = make_float4(__uint_as_float(R.seq), wv.x, wv.y, wv.z);
                             }
-                            if (D.dmask == SL_DEPTH_MASK_ON) sDepth[row][c] = z;
+                            if (dmask) sDepth[row][c] = z;
                             if (DBG)
                             {
                                 sLast[row][c] = R.seq;

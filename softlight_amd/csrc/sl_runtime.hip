@@ -694,6 +694,8 @@ int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t s
     Batch& B = c->batch;
     B.fbo = fboId;
 
+    if ((!indexed && m.elementEnd > 0xFFFFFFFFull) || numInstances > 0xFFFFFFFFull)
+        return fail(c, SL_ERR_UNSUPPORTED, "vertex ids and instance counts are limited to 2^32");
     const uint64_t count = (m.elementEnd > m.elementBegin) ? (m.elementEnd - m.elementBegin) : 0;
     const uint64_t tpi = count / 3;
     const uint64_t numTris = tpi * numInstances;

@@ -133,6 +133,24 @@ __device__ __forceinline__ uint64_t fetch_index(const DrawDesc& d, uint64_t e)
     }
 }
 
+// Clip-space position of a vertex: the position part of vertex_program below,
+// with the same operations in the same order (the programs are pure, so a
+// triangle can be culled and binned before its varyings are computed).
+__device__ __forceinline__ float4 vertex_position(const DrawDesc& d, const uint8_t* uniArena, uint64_t vertId, uint64_t inst)
+{
+    if (d.vsProg == SL_PROGRAM_PASSTHROUGH_GOURAUD) return read_attr(d, 0, vertId, 1.f);
+    if (d.vsProg == SL_PROGRAM_LAMBERT_TEXTURED || d.vsProg == SL_PROGRAM_COLOR_TEXTURED)
+    {
+        const sl_std_uniforms* u = reinterpret_cast<const sl_std_uniforms*>(uniArena + d.uniOffset);
+        const float4 p = read_attr(d, 0, vertId, 1.f);
+        float m[16];
+        model_matrix(d, u, inst, m);
+        const float4 world = mat_mul(m, f4(p.x, p.y, p.z, 1.f));
+        return mat_mul(u->mvp, world);
+    }
+    return f4(0.f, 0.f, 0.f, 0.f);
+}
+
 // Built-in vertex programs (replace SL_VertexShader::shader, SL_Shader.hpp:58-64)
 __device__ __forceinline__ float4 vertex_program(const DrawDesc& d, const uint8_t* uniArena, uint64_t vertId,
                                                  uint64_t inst, Vary4& out)
@@ -280,7 +298,8 @@ __device__ __forceinline__ void setup_record(float4 p0, float4 p1, float4 p2, ui
     r.maxY = f2i(fmaxf(fmaxf(p0.y, p1.y), p2.y));
     r.draw = draw;
     r.seq = seq;
-    r.pad[0] = r.pad[1] = 0;
+    r.state = 0;
+    r.pad = 0;
 
     // SL_ScanlineBounds::init: sort by y descending with the reference's swaps
     float ax = p0.x, ay = p0.y, bxx = p1.x, byy = p1.y, cx = p2.x, cy = p2.y, t;
