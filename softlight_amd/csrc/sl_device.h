@@ -28,14 +28,14 @@ constexpr int kMaxFan = 7;             // a 9-gon from 6-plane clipping fans int
 struct TileConfig { int tileW, tileH, segs; };
 constexpr int kNumTileConfigs = 8;
 constexpr TileConfig kTileConfigs[kNumTileConfigs] = {
-    {32, 16, 1},   // 0: 64 lanes
-    {32, 8, 2},    // 1: 64 lanes
-    {16, 16, 1},   // 2: 64 lanes
-    {16, 8, 2},    // 3: 64 lanes
-    {64, 16, 2},   // 4: 128 lanes
-    {32, 32, 1},   // 5: 128 lanes
-    {64, 8, 4},    // 6: 128 lanes
-    {64, 32, 2},   // 7: 256 lanes
+    {32, 16, 1},   // 0: 1 wave
+    {32, 8, 2},    // 1: 1 wave
+    {16, 16, 1},   // 2: 1 wave
+    {16, 8, 2},    // 3: 1 wave
+    {16, 8, 4},    // 4: 2 waves x 4 rows
+    {16, 16, 4},   // 5: 4 waves x 4 rows
+    {32, 16, 4},   // 6: 4 waves x 4 rows
+    {32, 32, 2},   // 7: 4 waves x 8 rows
 };
 constexpr int kDefaultTileConfig = 0;
 
@@ -179,6 +179,7 @@ struct RasterParams
     uint32_t varStride;
     const uint32_t* tileRecs;     // record ids sorted by tile
     const uint2* tileRange;       // [start, end) into tileRecs per tile
+    const uint32_t* tileOrder;    // launch order of the tiles (heaviest first), or null
     int32_t W, H, tilesX, tilesY;
     uint8_t* color;               // attachment 0 texels (RGBA8 or RGBA32F) or null
     int32_t colorType;

@@ -17,7 +17,7 @@ struct BinPlan { uint32_t blocks; uint32_t per; };
 BinPlan bin_plan(uint32_t entries);
 hipError_t launch_binning(const uint32_t* keys, const uint32_t* vals, const FrameCtrl* ctrl, uint32_t cap,
                           BinPlan plan, uint32_t* counts, uint32_t* totals, uint2* range, uint32_t* out,
-                          int numTiles, hipStream_t s);
+                          uint32_t* order, int numTiles, hipStream_t s);
 hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s);
 
 }  // namespace slg

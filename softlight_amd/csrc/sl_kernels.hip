@@ -587,6 +587,38 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(const uint32_t* __restric
     }
 }
 
+// Launch order of the raster tiles, heaviest first: tiles bucketed by
+// floor(log2(records)) + 1 (empty tiles last); the order inside a bucket is
+// arbitrary, which only affects scheduling.
+__global__ __launch_bounds__(1024) void k_tile_order(const uint2* __restrict__ range, uint32_t* __restrict__ order, int numTiles)
+{
+    __shared__ uint32_t sCnt[33];
+    __shared__ uint32_t sBase[33];
+    if (threadIdx.x < 33) sCnt[threadIdx.x] = 0u;
+    __syncthreads();
+    for (int t = threadIdx.x; t < numTiles; t += 1024)
+    {
+        const uint2 r = range[t];
+        atomicAdd(&sCnt[32 - __clz((int)(r.y - r.x))], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        uint32_t run = 0;
+        for (int b = 32; b >= 0; --b)
+        {
+            sBase[b] = run;
+            run += sCnt[b];
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < numTiles; t += 1024)
+    {
+        const uint2 r = range[t];
+        order[atomicAdd(&sBase[32 - __clz((int)(r.y - r.x))], 1u)] = (uint32_t)t;
+    }
+}
+
 __global__ __launch_bounds__(64) void k_bin_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                                                     const FrameCtrl* __restrict__ ctrl, uint32_t cap, uint32_t per,
                                                     const uint32_t* __restrict__ counts, const uint2* __restrict__ range,
@@ -653,7 +685,8 @@ constexpr int kStepLimit = 48;
 
 __device__ __noinline__ float3 recur_advance3(float3 s, float3 d, int k)
 {
-    return make_float3(sl_recur_advance(s.x, d.x, k), sl_recur_advance(s.y, d.y, k), sl_recur_advance(s.z, d.z, k));
+    sl_recur_advance3(s.x, s.y, s.z, d.x, d.y, d.z, k);
+    return s;
 }
 
 // Orders one record's LDS updates before the next record's: a wave barrier
@@ -754,6 +787,11 @@ constexpr uint32_t kNoPending = 0xFFFFFFFFu;
 // unchanged, but the dependent global loads of varyings and texels are paid
 // once per pixel instead of once per record.
 //
+// A tile of more than one wave is split by rows: each wave owns TH/NW rows,
+// stages and walks the tile's record list on its own (skipping records that
+// miss its rows with one uniform test) and never touches another wave's
+// pixels, so no workgroup barrier is needed between records.
+//
 // AUX bit 0: debug buffers (last primitive, depth-pass count); bit 1: the
 // visibility attachment.
 template <int CF, int AUX, int TW, int TH, int SEGS>
@@ -762,8 +800,10 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     constexpr bool DBG = (AUX & 1) != 0;
     constexpr bool VIS = (AUX & 2) != 0;
     constexpr int NT = TH * 4 * SEGS;       // lanes: rows x segments x 4 phases
+    constexpr int NW = NT / 64;             // waves, each owning TH / NW rows
     constexpr int SW = TW / SEGS;           // segment width (columns)
     static_assert(SW >= 4 && SW % 4 == 0, "segment width");
+    static_assert(NT % 64 == 0 && TH % NW == 0, "whole waves of whole rows");
     __shared__ float sDepth[TH][TW];
     __shared__ uint32_t sCol8[CF == 1 ? TH : 1][TW];
     __shared__ float4 sColF[CF == 2 ? TH : 1][TW];
@@ -773,10 +813,13 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     __shared__ uint32_t sPend[TH][TW];                  // pending fragment's record, or kNoPending
     __shared__ float sPendS[3][TH][TW];                 // its barycentric values
 
-    const int tile = blockIdx.x;
+    // heaviest tiles first (the order only affects scheduling)
+    const int tile = (P.tileOrder != nullptr) ? (int)P.tileOrder[blockIdx.x] : (int)blockIdx.x;
     const int tx = tile % P.tilesX, ty = tile / P.tilesX;
     const int x0 = tx * TW, y0 = ty * TH;
     const int lane = threadIdx.x;
+    const int wave = lane >> 6, wl = lane & 63;
+    const int rowLo = y0 + wave * (TH / NW), rowHi = rowLo + TH / NW;
 
     // tile init: fused clear or load
     for (int k = lane; k < TW * TH; k += NT)
@@ -823,27 +866,30 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     const uint2 range = P.tileRange[tile];
     uint32_t tested = 0, shaded = 0;
 
-    // Records are staged into LDS kBatch at a time by the whole workgroup, so
-    // the dependent global loads (entry -> record) are paid once per batch.
+    // Each wave stages records into LDS kBatch at a time, so the dependent
+    // global loads (entry -> record) are paid once per batch.
     constexpr int kBatch = 16;
     constexpr int kRecWords = sizeof(SetupRec) / 4;
-    __shared__ uint32_t sRid[kBatch];
-    __shared__ SetupRec sRec[kBatch];
+    __shared__ uint32_t sRidW[NW][kBatch];
+    __shared__ SetupRec sRecW[NW][kBatch];
+    uint32_t* sRid = sRidW[wave];
+    SetupRec* sRec = sRecW[wave];
     for (uint32_t base = range.x; base < range.y; base += kBatch)
     {
         const int nb = (int)min((uint32_t)kBatch, range.y - base);
-        if (lane < nb) sRid[lane] = P.tileRecs[base + lane];
-        __syncthreads();
-        for (int q = lane; q < nb * kRecWords; q += NT)
+        if (wl < nb) sRid[wl] = P.tileRecs[base + wl];
+        record_barrier<64>();
+        for (int q = wl; q < nb * kRecWords; q += 64)
         {
             const int r = q / kRecWords, w = q - r * kRecWords;
             reinterpret_cast<uint32_t*>(sRec)[q] = reinterpret_cast<const uint32_t*>(P.recs + sRid[r])[w];
         }
-        __syncthreads();
+        record_barrier<64>();
     for (int bi = 0; bi < nb; ++bi)
     {
-        const uint32_t rid = __builtin_amdgcn_readfirstlane(sRid[bi]);
         const SetupRec& R = sRec[bi];
+        if (R.maxY <= rowLo || R.minY >= rowHi) continue;    // misses this wave's rows
+        const uint32_t rid = __builtin_amdgcn_readfirstlane(sRid[bi]);
         const uint32_t drawIdx = __builtin_amdgcn_readfirstlane(R.draw);
         if (y >= R.minY && y < R.maxY && y < P.H)
         {
@@ -930,9 +976,9 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                 }
             }
         }
-        record_barrier<NT>();
+        record_barrier<64>();
     }
-        __syncthreads();
+        record_barrier<64>();
     }
     __syncthreads();
 
@@ -1028,7 +1074,7 @@ BinPlan bin_plan(uint32_t entries)
 
 hipError_t launch_binning(const uint32_t* keys, const uint32_t* vals, const FrameCtrl* ctrl, uint32_t cap,
                           BinPlan plan, uint32_t* counts, uint32_t* totals, uint2* range, uint32_t* out,
-                          int numTiles, hipStream_t s)
+                          uint32_t* order, int numTiles, hipStream_t s)
 {
     (void)hipGetLastError();   // drop a stale error from an earlier call
     const size_t lds = sizeof(uint32_t) * (size_t)numTiles;
@@ -1038,25 +1084,42 @@ hipError_t launch_binning(const uint32_t* keys, const uint32_t* vals, const Fram
     hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), 0, s, totals, range, numTiles);
     hipLaunchKernelGGL(k_bin_scatter, dim3(plan.blocks), dim3(64), lds, s, keys, vals, ctrl, cap, plan.per, counts, range, out,
                        numTiles);
+    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, s, range, order, numTiles);
     return hipGetLastError();
 }
 
+template <int CF, int AUX, int TW, int TH, int SEGS>
+static void launch_raster_shape(const RasterParams& p, int numTiles, hipStream_t s)
+{
+    static_assert(TW * TH > 0, "tile");
+    hipLaunchKernelGGL((k_raster<CF, AUX, TW, TH, SEGS>), dim3(numTiles), dim3(TH * 4 * SEGS), 0, s, p);
+}
+
+// The cases mirror kTileConfigs (checked below).
 template <int CF, int AUX>
 static void launch_raster_cfg(const RasterParams& p, int numTiles, int cfg, hipStream_t s)
 {
-    const dim3 g(numTiles);
     switch (cfg)
     {
-        case 1: hipLaunchKernelGGL((k_raster<CF, AUX, 32, 8, 2>), g, dim3(64), 0, s, p); break;
-        case 2: hipLaunchKernelGGL((k_raster<CF, AUX, 16, 16, 1>), g, dim3(64), 0, s, p); break;
-        case 3: hipLaunchKernelGGL((k_raster<CF, AUX, 16, 8, 2>), g, dim3(64), 0, s, p); break;
-        case 4: hipLaunchKernelGGL((k_raster<CF, AUX, 64, 16, 2>), g, dim3(128), 0, s, p); break;
-        case 5: hipLaunchKernelGGL((k_raster<CF, AUX, 32, 32, 1>), g, dim3(128), 0, s, p); break;
-        case 6: hipLaunchKernelGGL((k_raster<CF, AUX, 64, 8, 4>), g, dim3(128), 0, s, p); break;
-        case 7: hipLaunchKernelGGL((k_raster<CF, AUX, 64, 32, 2>), g, dim3(256), 0, s, p); break;
-        default: hipLaunchKernelGGL((k_raster<CF, AUX, 32, 16, 1>), g, dim3(64), 0, s, p); break;
+        case 1: launch_raster_shape<CF, AUX, 32, 8, 2>(p, numTiles, s); break;
+        case 2: launch_raster_shape<CF, AUX, 16, 16, 1>(p, numTiles, s); break;
+        case 3: launch_raster_shape<CF, AUX, 16, 8, 2>(p, numTiles, s); break;
+        case 4: launch_raster_shape<CF, AUX, 16, 8, 4>(p, numTiles, s); break;
+        case 5: launch_raster_shape<CF, AUX, 16, 16, 4>(p, numTiles, s); break;
+        case 6: launch_raster_shape<CF, AUX, 32, 16, 4>(p, numTiles, s); break;
+        case 7: launch_raster_shape<CF, AUX, 32, 32, 2>(p, numTiles, s); break;
+        default: launch_raster_shape<CF, AUX, 32, 16, 1>(p, numTiles, s); break;
     }
 }
+static_assert(kTileConfigs[0].tileW == 32 && kTileConfigs[0].tileH == 16 && kTileConfigs[0].segs == 1 &&
+              kTileConfigs[1].tileW == 32 && kTileConfigs[1].tileH == 8 && kTileConfigs[1].segs == 2 &&
+              kTileConfigs[2].tileW == 16 && kTileConfigs[2].tileH == 16 && kTileConfigs[2].segs == 1 &&
+              kTileConfigs[3].tileW == 16 && kTileConfigs[3].tileH == 8 && kTileConfigs[3].segs == 2 &&
+              kTileConfigs[4].tileW == 16 && kTileConfigs[4].tileH == 8 && kTileConfigs[4].segs == 4 &&
+              kTileConfigs[5].tileW == 16 && kTileConfigs[5].tileH == 16 && kTileConfigs[5].segs == 4 &&
+              kTileConfigs[6].tileW == 32 && kTileConfigs[6].tileH == 16 && kTileConfigs[6].segs == 4 &&
+              kTileConfigs[7].tileW == 32 && kTileConfigs[7].tileH == 32 && kTileConfigs[7].segs == 2,
+              "launch table and tile configurations disagree");
 
 hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s)
 {

@@ -55,7 +55,7 @@ struct Shader { uint8_t cull, dtest, dmask, blend, nvary, nout; uint32_t vsProg,
 // Device buffers of one frame.
 struct FrameRes
 {
-    DevBuf draws, uni, recs, vars, keys, vals, sorted, clip, status, ranges, ctrl, binCounts, binTotals;
+    DevBuf draws, uni, recs, vars, keys, vals, sorted, clip, status, ranges, ctrl, binCounts, binTotals, order;
     uint32_t recCap = 0, pairCap = 0, clipCap = 0;
 };
 
@@ -277,6 +277,7 @@ int ensure_frame(sl_context* c, FrameRes& R, const Plan& p, const Batch& B)
     int rc;
     if ((rc = ensure(c, R.ctrl, sizeof(FrameCtrl))) != SL_OK) return rc;
     if ((rc = ensure(c, R.ranges, sizeof(uint2) * (size_t)p.numTiles)) != SL_OK) return rc;
+    if ((rc = ensure(c, R.order, sizeof(uint32_t) * (size_t)p.numTiles)) != SL_OK) return rc;
     if (B.totalTris == 0) return SL_OK;
     if (R.recCap == 0) { R.recCap = 1u << 18; R.pairCap = 1u << 20; R.clipCap = 1u << 16; }
     if ((rc = ensure(c, R.draws, sizeof(DrawDesc) * B.draws.size())) != SL_OK) return rc;
@@ -327,6 +328,7 @@ RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool have
     rp.varStride = p.varStride;
     rp.tileRecs = haveEntries ? reinterpret_cast<const uint32_t*>(R.sorted.p) : nullptr;
     rp.tileRange = reinterpret_cast<const uint2*>(R.ranges.p);
+    rp.tileOrder = haveEntries ? reinterpret_cast<const uint32_t*>(R.order.p) : nullptr;
     rp.W = p.W; rp.H = p.H; rp.tilesX = p.tilesX; rp.tilesY = p.tilesY;
     rp.color = p.ct ? reinterpret_cast<uint8_t*>(p.ct->buf.p) : nullptr;
     rp.colorType = p.ct ? p.ct->type : SL_COLOR_RGBA_8U;
@@ -382,7 +384,7 @@ int enqueue_back(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, uint
             if ((rc = check(c, launch_binning(sp.pairKeys, sp.pairVals, sp.ctrl, R.pairCap, bin_plan(entries),
                                               reinterpret_cast<uint32_t*>(R.binCounts.p), reinterpret_cast<uint32_t*>(R.binTotals.p),
                                               reinterpret_cast<uint2*>(R.ranges.p), reinterpret_cast<uint32_t*>(R.sorted.p),
-                                              p.numTiles, s), "binning")) != SL_OK) return rc;
+                                              reinterpret_cast<uint32_t*>(R.order.p), p.numTiles, s), "binning")) != SL_OK) return rc;
         }
     }
     if (evBin1) (void)hipEventRecord(evBin1, s);
@@ -531,7 +533,7 @@ void graph_release(sl_context* c)
     G.exec = nullptr;
     G.graph = nullptr;
     for (DevBuf* b : {&G.res.draws, &G.res.uni, &G.res.recs, &G.res.vars, &G.res.keys, &G.res.vals, &G.res.sorted,
-                      &G.res.clip, &G.res.status, &G.res.ranges, &G.res.ctrl, &G.res.binCounts, &G.res.binTotals}) release(*b);
+                      &G.res.clip, &G.res.status, &G.res.ranges, &G.res.ctrl, &G.res.binCounts, &G.res.binTotals, &G.res.order}) release(*b);
     G.res = FrameRes{};
     if (G.hCtrl) (void)hipHostFree(G.hCtrl);
     G.hCtrl = nullptr;
@@ -892,7 +894,7 @@ void sl_context_destroy(sl_context* c)
     for (auto& t : c->texs) release(t.buf);
     graph_release(c);
     for (DevBuf* b : {&c->res.draws, &c->res.uni, &c->res.recs, &c->res.vars, &c->res.keys, &c->res.vals, &c->res.sorted,
-                      &c->res.clip, &c->res.status, &c->res.ranges, &c->res.ctrl, &c->res.binCounts, &c->res.binTotals}) release(*b);
+                      &c->res.clip, &c->res.status, &c->res.ranges, &c->res.ctrl, &c->res.binCounts, &c->res.binTotals, &c->res.order}) release(*b);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     for (auto& f : c->fbos) release(f.gradDev);
     for (auto& e : c->bwdEv) if (e) (void)hipEventDestroy(e);

@@ -1,10 +1,24 @@
-// Compares sl_recur_advance(s, D, k) with k explicit steps s = fl(s + D) on
-// many seeded (s, D, k) cases; prints "cases mismatches" and exits non-zero on
-// any mismatch.  Built with -ffp-contract=off, like the kernels.
+// Compares sl_recur_advance(s, D, k) and sl_recur_advance3 with k explicit
+// steps s = fl(s + D) on many seeded (s, D, k) cases; prints "cases
+// mismatches" and exits non-zero on any mismatch.  Built with
+// -ffp-contract=off, like the kernels.  With -DSKEWED_RCP the run-length
+// reciprocal is deliberately off by up to 40% either way: the result must not
+// change (the run length is trimmed or split, never trusted).
 #include <cstdio>
 #include <cstdint>
 #include <cstring>
 #include <cmath>
+
+#ifdef SKEWED_RCP
+static uint64_t skew_state = 12345;
+static float skewed_rcp(float x)
+{
+    skew_state = skew_state * 6364136223846793005ull + 1442695040888963407ull;
+    const float f = 0.6f + 0.8f * (float)((skew_state >> 40) * (1.0 / 16777216.0));
+    return f / x;
+}
+#define SL_RECUR_RCP_OVERRIDE skewed_rcp
+#endif
 
 #include "../../softlight_amd/csrc/sl_recurrence.h"
 
@@ -49,7 +63,21 @@ int main(int argc, char** argv)
         }
         float ref = s;
         for (int j = 0; j < k; ++j) ref = ref + D;
-        const float got = sl_recur_advance(s, D, k);
+        float got = sl_recur_advance(s, D, k);
+        if ((i & 3) == 0)
+        {
+            // the three-component form, with this case in the middle slot
+            float s0 = s * 0.75f, s1 = s, s2 = -s;
+            const float D0 = D * 3.f, D2 = D * 0.5f;
+            float r0 = s0, r2 = s2;
+            for (int j = 0; j < k; ++j) { r0 = r0 + D0; r2 = r2 + D2; }
+            sl_recur_advance3(s0, s1, s2, D0, D, D2, k);
+            uint32_t x0, y0, x2, y2;
+            std::memcpy(&x0, &r0, 4); std::memcpy(&y0, &s0, 4);
+            std::memcpy(&x2, &r2, 4); std::memcpy(&y2, &s2, 4);
+            if ((x0 != y0 && !(r0 != r0 && s0 != s0)) || (x2 != y2 && !(r2 != r2 && s2 != s2))) got = s0 * 0.f + NAN;
+            else got = s1;
+        }
         uint32_t a, b;
         std::memcpy(&a, &ref, 4);
         std::memcpy(&b, &got, 4);
