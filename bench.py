@@ -304,7 +304,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": measured_traffic(args.workload),
             "bytes_per_triangle": round(bpt, 3),
         },
     }
@@ -314,6 +314,18 @@ def main():
     if not args.no_cpu and args.workload == "spheres":
         out["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(workload):
+    """HBM bytes per k_setup launch from the committed rocprofv3 counter runs
+    (FETCH_SIZE and WRITE_SIZE scaled by a 256 MiB copy), or None."""
+    if workload != "spheres":
+        return None
+    path = os.path.join(ROOT, "profiles", "k_setup_spheres_1080p_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes_per_launch")
 
 
 def views_summary(vr):

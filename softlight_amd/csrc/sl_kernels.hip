@@ -298,7 +298,17 @@ __global__ __launch_bounds__(kSetupBlock, 6) void k_setup(SetupParams P)
     bool clipped = false;
     float4 sp[3];
     TileRect tr{0, -1, 0, -1};
-    const DrawDesc& DU = sD;
+    // the descriptor is uniform: copy it to scalar registers once (the fields
+    // that are used stay in SGPRs instead of being re-read from LDS)
+    DrawDesc DR;
+    {
+        uint32_t w[sizeof(DrawDesc) / 4];
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(DrawDesc) / 4); ++i)
+            w[i] = __builtin_amdgcn_readfirstlane(reinterpret_cast<const uint32_t*>(&sD)[i]);
+        __builtin_memcpy(&DR, w, sizeof(DrawDesc));
+    }
+    const DrawDesc& DU = DR;
     if (g < P.totalTris && g - DU.triBase < DU.numTris)
     {
         load_triangle_pos(P, g, DU, drawIdx, T);
@@ -866,25 +876,56 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     const uint2 range = P.tileRange[tile];
     uint32_t tested = 0, shaded = 0;
 
-    // Each wave stages records into LDS kBatch at a time, so the dependent
-    // global loads (entry -> record) are paid once per batch.
+    // Each wave stages records into LDS kBatch at a time.  The global loads
+    // of the next batch (its record ids two batches ahead, its record words
+    // one batch ahead) are issued before the current batch is processed, so
+    // their latency hides behind the raster work.
     constexpr int kBatch = 16;
     constexpr int kRecWords = sizeof(SetupRec) / 4;
+    constexpr int kLaneWords = kBatch * kRecWords / 64;
+    static_assert(kBatch * kRecWords % 64 == 0 && kBatch <= 64, "record staging");
     __shared__ uint32_t sRidW[NW][kBatch];
     __shared__ SetupRec sRecW[NW][kBatch];
     uint32_t* sRid = sRidW[wave];
     SetupRec* sRec = sRecW[wave];
+    // record id of entry b + wl (lanes below kBatch)
+    auto fetch_ids = [&](uint32_t b) -> uint32_t {
+        return (wl < kBatch && b + (uint32_t)wl < range.y) ? P.tileRecs[b + wl] : 0u;
+    };
+    // this lane's words of the batch starting at b (ids in lanes 0..kBatch-1)
+    auto fetch_words = [&](uint32_t b, uint32_t ids, uint32_t (&v)[kLaneWords]) {
+        const int nb = (int)min((uint32_t)kBatch, range.y - b);
+#pragma unroll
+        for (int j = 0; j < kLaneWords; ++j)
+        {
+            const int q = wl + 64 * j;
+            const int r = q / kRecWords, w = q - r * kRecWords;
+            const uint32_t id = __shfl(ids, r, 64);
+            v[j] = (r < nb) ? reinterpret_cast<const uint32_t*>(P.recs + id)[w] : 0u;
+        }
+    };
+    uint32_t idsCur = 0, idsNext = 0;
+    uint32_t vCur[kLaneWords] = {};
+    if (range.x < range.y)
+    {
+        idsCur = fetch_ids(range.x);
+        fetch_words(range.x, idsCur, vCur);
+        idsNext = fetch_ids(range.x + kBatch);
+    }
     for (uint32_t base = range.x; base < range.y; base += kBatch)
     {
         const int nb = (int)min((uint32_t)kBatch, range.y - base);
-        if (wl < nb) sRid[wl] = P.tileRecs[base + wl];
+        if (wl < kBatch) sRid[wl] = idsCur;
+#pragma unroll
+        for (int j = 0; j < kLaneWords; ++j) reinterpret_cast<uint32_t*>(sRec)[wl + 64 * j] = vCur[j];
         record_barrier<64>();
-        for (int q = wl; q < nb * kRecWords; q += 64)
+        uint32_t vNext[kLaneWords] = {};
+        uint32_t idsAfter = 0;
+        if (base + kBatch < range.y)
         {
-            const int r = q / kRecWords, w = q - r * kRecWords;
-            reinterpret_cast<uint32_t*>(sRec)[q] = reinterpret_cast<const uint32_t*>(P.recs + sRid[r])[w];
+            fetch_words(base + kBatch, idsNext, vNext);
+            idsAfter = fetch_ids(base + 2 * kBatch);
         }
-        record_barrier<64>();
     for (int bi = 0; bi < nb; ++bi)
     {
         const SetupRec& R = sRec[bi];
@@ -979,6 +1020,10 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
         record_barrier<64>();
     }
         record_barrier<64>();
+        idsCur = idsNext;
+        idsNext = idsAfter;
+#pragma unroll
+        for (int j = 0; j < kLaneWords; ++j) vCur[j] = vNext[j];
     }
     __syncthreads();
 
