@@ -691,7 +691,7 @@ __global__ __launch_bounds__(64) void k_bin_scatter(const uint32_t* __restrict__
 // ---------------------------------------------------------------------------
 // Catch-up distances up to kStepLimit groups are stepped; longer ones use the
 // binade jump (one out-of-line copy for the three barycentric components).
-constexpr int kStepLimit = 48;
+constexpr int kStepLimit = 256;
 
 __device__ __noinline__ float3 recur_advance3(float3 s, float3 d, int k)
 {
@@ -962,7 +962,19 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                     // few, the exact k-step jump when many
                     if (g0 <= kStepLimit)
                     {
-                        for (int gi = 0; gi < g0; ++gi)
+                        // three independent chains, four steps per trip
+                        int gi = 0;
+                        for (; gi + 4 <= g0; gi += 4)
+                        {
+#pragma unroll
+                            for (int u = 0; u < 4; ++u)
+                            {
+                                s0 = s0 + d0;
+                                s1 = s1 + d1;
+                                s2 = s2 + d2;
+                            }
+                        }
+                        for (; gi < g0; ++gi)
                         {
                             s0 = s0 + d0;
                             s1 = s1 + d1;
