@@ -265,6 +265,19 @@ int sl_clear_color_buffer(sl_context* ctx, size_t fbo, unsigned attachment, cons
 int sl_clear_depth_buffer(sl_context* ctx, size_t fbo, double depth);                                  /* SL_Context::clear_depth_buffer */
 int sl_clear_framebuffer(sl_context* ctx, size_t fbo, unsigned attachment, const double rgba[4], double depth); /* SL_Context::clear_framebuffer */
 
+/* ---- blit (SL_Context.cpp:747-875, SL_BlitProcessor.cpp:620-665) --------
+ * Nearest-neighbour copy of a texture region into another texture with
+ * 16.16 fixed-point scaling, the source rows read bottom-up, and the colour
+ * conversion of the reference's blit operators.  Sources: RGBA_8U,
+ * RGBA_FLOAT, R_FLOAT (written as (0, 0, r, 1), SL_BlitProcessor.cpp:303-306);
+ * destinations: RGBA_8U, RGBA_FLOAT.  Where the reference would read or
+ * write outside a texture (a destination region past its height, a source
+ * coordinate past the source), those pixels are skipped. */
+int sl_blit(sl_context* ctx, size_t outTex, size_t inTex);                       /* SL_Context::blit(out, in) */
+int sl_blit_region(sl_context* ctx, size_t outTex, size_t inTex,
+                   uint16_t srcX0, uint16_t srcY0, uint16_t srcX1, uint16_t srcY1,
+                   uint16_t dstX0, uint16_t dstY0, uint16_t dstX1, uint16_t dstY1);  /* SL_Context::blit(out, in, src/dst rects) */
+
 /* ---- HIP-graph frames (no reference counterpart) -------------------------
  * sl_frame_graph_capture executes the pending batch (the clears and draws
  * recorded since the last synchronisation point) once, then captures the same

@@ -1671,6 +1671,75 @@ int slo_frame_graph_capture(sl_context*) { return SL_OK; }
 int slo_frame_graph_launch(sl_context*) { return SL_ERR_UNSUPPORTED; }
 int slo_frame_graph_release(sl_context*) { return SL_OK; }
 
+/* blit: SL_BlitProcessor::blit_nearest (SL_BlitProcessor.cpp:620-665) with the
+ * colour operators of :289-416; pixels where the reference would read or
+ * write outside a texture are skipped */
+int slo_blit_region(sl_context* c, size_t outTex, size_t inTex, uint16_t srcX0, uint16_t srcY0, uint16_t srcX1,
+                    uint16_t srcY1, uint16_t dstX0, uint16_t dstY0, uint16_t dstX1, uint16_t dstY1)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    if (outTex >= c->texs.size() || inTex >= c->texs.size() || !c->texs[outTex].valid || !c->texs[inTex].valid)
+        return SL_ERR_INVALID_HANDLE;
+    Tex& o = c->texs[outTex];
+    const Tex& in = c->texs[inTex];
+    if (in.type != SL_COLOR_RGBA_8U && in.type != SL_COLOR_RGBA_FLOAT && in.type != SL_COLOR_R_FLOAT) return SL_ERR_UNSUPPORTED;
+    if (o.type != SL_COLOR_RGBA_8U && o.type != SL_COLOR_RGBA_FLOAT) return SL_ERR_UNSUPPORTED;
+    if (outTex == inTex) return SL_ERR_INVALID_ARG;
+    // uint_fast32_t is 64 bits wide on the reference's x86-64 targets
+    const uint64_t inW = (uint64_t)srcX1 - (uint64_t)srcX0;
+    const uint64_t inH = (uint64_t)srcY1 - (uint64_t)srcY0;
+    const uint64_t outW = (uint64_t)dstX1 - (uint64_t)dstX0;
+    const uint64_t totalOutW = o.w, totalOutH = o.h;
+    const uint64_t x0 = dstX0;
+    const uint64_t x1 = std::min<uint64_t>(totalOutW, x0 + outW);
+    const uint64_t y1 = std::min<uint64_t>(dstY1, totalOutH);
+    const uint64_t finW = inW << 16, finH = inH << 16;
+    const uint64_t foutW = (finW / totalOutW) + 1u;        // "account for rounding errors"
+    const uint64_t foutH = (finH / totalOutH) + 1u;
+    for (uint64_t y = dstY0; y < y1; ++y)
+    {
+        const uint64_t yf = (y * foutH) >> 16;
+        const uint64_t srcY = (uint64_t)srcY1 - ((uint64_t)srcY0 + yf) - 1u;
+        for (uint64_t x = x0; x < x1; ++x)
+        {
+            const uint64_t srcX = (x * foutW) >> 16;
+            if (srcX >= in.w || srcY >= in.h) continue;
+            const size_t si = srcX + (size_t)in.w * srcY;
+            const size_t di = x + (size_t)o.w * y;
+            float col[4];
+            if (in.type == SL_COLOR_RGBA_8U && o.type == SL_COLOR_RGBA_8U)
+            {
+                std::memcpy(o.data.data() + 4 * di, in.data.data() + 4 * si, 4);
+                continue;
+            }
+            if (in.type == SL_COLOR_RGBA_8U)
+                for (int k = 0; k < 4; ++k) col[k] = (float)in.data[4 * si + k] * 0.00392156862745f;   // color_cast<float, uint8_t>
+            else if (in.type == SL_COLOR_RGBA_FLOAT)
+                std::memcpy(col, in.data.data() + 16 * si, 16);
+            else
+            {
+                // SL_Blit_R_to_RGBA: (min, min, r, max) of the source type (:303-306)
+                col[0] = 0.f; col[1] = 0.f; std::memcpy(&col[2], in.data.data() + 4 * si, 4); col[3] = 1.f;
+            }
+            if (o.type == SL_COLOR_RGBA_8U)
+                for (int k = 0; k < 4; ++k) o.data[4 * di + k] = (uint8_t)(uint32_t)f2i(col[k] * 255.f);   // color_cast<uint8_t, float>
+            else
+                std::memcpy(o.data.data() + 16 * di, col, 16);
+        }
+    }
+    return SL_OK;
+}
+
+int slo_blit(sl_context* c, size_t outTex, size_t inTex)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    if (outTex >= c->texs.size() || inTex >= c->texs.size() || !c->texs[outTex].valid || !c->texs[inTex].valid)
+        return SL_ERR_INVALID_HANDLE;
+    const Tex& o = c->texs[outTex];
+    const Tex& in = c->texs[inTex];
+    return slo_blit_region(c, outTex, inTex, 0, 0, in.w, in.h, 0, 0, o.w, o.h);   // SL_Context.cpp:747-769
+}
+
 /* differentiable draws (no reference counterpart) */
 int slo_framebuffer_attach_visibility(sl_context* c, size_t fbo, size_t tex)
 {

@@ -129,6 +129,8 @@ _SIGS = {
     "frame_graph_capture": (_I, [_P]),
     "frame_graph_launch": (_I, [_P]),
     "frame_graph_release": (_I, [_P]),
+    "blit": (_I, [_P, _SZ, _SZ]),
+    "blit_region": (_I, [_P, _SZ, _SZ] + [C.c_uint16] * 8),
     "framebuffer_attach_visibility": (_I, [_P, _SZ, _SZ]),
     "vbo_attach_gradient": (_I, [_P, _SZ, _SZ]),
     "texture_attach_gradient": (_I, [_P, _SZ, _SZ]),
@@ -299,6 +301,13 @@ class SLContext:
     def attach_debug_buffers(self, fbo, last_tex, count_tex):
         return self._check(self.lib.fbo_attach_debug_buffers(self.h, fbo, last_tex, count_tex),
                            "fbo_attach_debug_buffers")
+
+    # ---- blit ----
+    def blit(self, out_tex, in_tex, src=None, dst=None):
+        """SL_Context::blit; src/dst are (x0, y0, x1, y1) rectangles or None for whole textures."""
+        if src is None and dst is None:
+            return self._check(self.lib.blit(self.h, out_tex, in_tex), "blit")
+        return self._check(self.lib.blit_region(self.h, out_tex, in_tex, *src, *dst), "blit_region")
 
     # ---- differentiable draws ----
     def attach_visibility(self, fbo, tex):

@@ -195,6 +195,19 @@ struct RasterParams
 };
 
 // ---------------------------------------------------------------------------
+// Blit (SL_BlitProcessor::blit_nearest)
+// ---------------------------------------------------------------------------
+struct BlitParams
+{
+    const uint8_t* src;
+    uint8_t* dst;
+    uint32_t srcW, srcH, dstW, dstH;
+    uint32_t srcX0, srcY0, srcX1, srcY1;
+    uint32_t x0, x1, y0, y1;           // destination pixels written: [x0, x1) x [y0, y1)
+    unsigned long long foutW, foutH;   // 16.16 source steps per destination pixel
+};
+
+// ---------------------------------------------------------------------------
 // Backward pass (fixed-visibility adjoint)
 // ---------------------------------------------------------------------------
 // One draw of the saved forward batch with its gradient destinations.

@@ -19,5 +19,6 @@ hipError_t launch_binning(const uint32_t* keys, const uint32_t* vals, const Fram
                           BinPlan plan, uint32_t* counts, uint32_t* totals, uint2* range, uint32_t* out,
                           uint32_t* order, int numTiles, hipStream_t s);
 hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s);
+hipError_t launch_blit(const BlitParams& p, int srcType, int dstType, hipStream_t s);
 
 }  // namespace slg

@@ -1076,6 +1076,56 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
 }
 
 // ---------------------------------------------------------------------------
+// k_blit: nearest-neighbour blit, one destination pixel per lane
+// (SL_BlitProcessor::blit_nearest, SL_BlitProcessor.cpp:620-665; colour
+// operators :289-416)
+// ---------------------------------------------------------------------------
+template <int SRC, int DST>
+__global__ __launch_bounds__(256) void k_blit(BlitParams P)
+{
+    const uint32_t w = P.x1 - P.x0;
+    const uint64_t n = (uint64_t)w * (P.y1 - P.y0);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    {
+        const uint32_t x = P.x0 + (uint32_t)(i % w), y = P.y0 + (uint32_t)(i / w);
+        const unsigned long long yf = ((unsigned long long)y * P.foutH) >> 16;
+        const unsigned long long srcY = (unsigned long long)P.srcY1 - ((unsigned long long)P.srcY0 + yf) - 1ull;
+        const unsigned long long srcX = ((unsigned long long)x * P.foutW) >> 16;
+        if (srcX >= P.srcW || srcY >= P.srcH) continue;      // the reference reads outside the source here
+        const uint64_t si = srcX + (uint64_t)P.srcW * srcY;
+        const uint64_t di = (uint64_t)x + (uint64_t)P.dstW * y;
+        if (SRC == SL_COLOR_RGBA_8U && DST == SL_COLOR_RGBA_8U)
+        {
+            reinterpret_cast<uint32_t*>(P.dst)[di] = reinterpret_cast<const uint32_t*>(P.src)[si];
+        }
+        else if (SRC == SL_COLOR_RGBA_FLOAT && DST == SL_COLOR_RGBA_FLOAT)
+        {
+            reinterpret_cast<float4*>(P.dst)[di] = reinterpret_cast<const float4*>(P.src)[si];
+        }
+        else
+        {
+            float c[4];
+            if (SRC == SL_COLOR_RGBA_8U)
+            {
+                unpack_rgba8(reinterpret_cast<const uint32_t*>(P.src)[si], c);   // color_cast<float, uint8_t>
+            }
+            else if (SRC == SL_COLOR_RGBA_FLOAT)
+            {
+                const float4 v = reinterpret_cast<const float4*>(P.src)[si];
+                c[0] = v.x; c[1] = v.y; c[2] = v.z; c[3] = v.w;
+            }
+            else
+            {
+                // R -> RGBA: (min, min, r, max) of the source type
+                c[0] = 0.f; c[1] = 0.f; c[2] = reinterpret_cast<const float*>(P.src)[si]; c[3] = 1.f;
+            }
+            if (DST == SL_COLOR_RGBA_8U) reinterpret_cast<uint32_t*>(P.dst)[di] = pack_rgba8(c);
+            else reinterpret_cast<float4*>(P.dst)[di] = make_float4(c[0], c[1], c[2], c[3]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
 static inline int grid_for(uint64_t n, int block, int cap = 4096)
@@ -1114,6 +1164,24 @@ hipError_t launch_clip(const SetupParams& p, uint32_t jobs, hipStream_t s)
     (void)hipGetLastError();   // drop a stale error from an earlier call
     if (jobs == 0) return hipSuccess;
     hipLaunchKernelGGL(k_clip, dim3(grid_for(jobs, 64, 8192)), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+template <int SRC, int DST>
+static void launch_blit_t(const BlitParams& p, uint64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_blit<SRC, DST>), dim3(grid_for(n, 256, 16384)), dim3(256), 0, s, p);
+}
+
+hipError_t launch_blit(const BlitParams& p, int srcType, int dstType, hipStream_t s)
+{
+    (void)hipGetLastError();   // drop a stale error from an earlier call
+    const uint64_t n = (uint64_t)(p.x1 - p.x0) * (p.y1 - p.y0);
+    if (n == 0) return hipSuccess;
+    const bool d8 = dstType == SL_COLOR_RGBA_8U;
+    if (srcType == SL_COLOR_RGBA_8U) d8 ? launch_blit_t<SL_COLOR_RGBA_8U, SL_COLOR_RGBA_8U>(p, n, s) : launch_blit_t<SL_COLOR_RGBA_8U, SL_COLOR_RGBA_FLOAT>(p, n, s);
+    else if (srcType == SL_COLOR_RGBA_FLOAT) d8 ? launch_blit_t<SL_COLOR_RGBA_FLOAT, SL_COLOR_RGBA_8U>(p, n, s) : launch_blit_t<SL_COLOR_RGBA_FLOAT, SL_COLOR_RGBA_FLOAT>(p, n, s);
+    else d8 ? launch_blit_t<SL_COLOR_R_FLOAT, SL_COLOR_RGBA_8U>(p, n, s) : launch_blit_t<SL_COLOR_R_FLOAT, SL_COLOR_RGBA_FLOAT>(p, n, s);
     return hipGetLastError();
 }
 

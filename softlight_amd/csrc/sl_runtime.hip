@@ -1330,6 +1330,50 @@ int sl_frame_graph_release(sl_context* c)
 }
 
 // ---------------------------------------------------------------------------
+// blit
+// ---------------------------------------------------------------------------
+int sl_blit_region(sl_context* c, size_t outTex, size_t inTex, uint16_t srcX0, uint16_t srcY0, uint16_t srcX1,
+                   uint16_t srcY1, uint16_t dstX0, uint16_t dstY0, uint16_t dstX1, uint16_t dstY1)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    Tex* o = tex_at(c, outTex);
+    Tex* i = tex_at(c, inTex);
+    if (!o || !i) return SL_ERR_INVALID_HANDLE;
+    if (i->type != SL_COLOR_RGBA_8U && i->type != SL_COLOR_RGBA_FLOAT && i->type != SL_COLOR_R_FLOAT)
+        return fail(c, SL_ERR_UNSUPPORTED, "blit source format");
+    if (o->type != SL_COLOR_RGBA_8U && o->type != SL_COLOR_RGBA_FLOAT) return fail(c, SL_ERR_UNSUPPORTED, "blit destination format");
+    if (o == i) return fail(c, SL_ERR_INVALID_ARG, "blit onto its own source");
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    // SL_BlitProcessor::blit_nearest (SL_BlitProcessor.cpp:620-665), in the
+    // reference's 64-bit unsigned arithmetic
+    const unsigned long long inW = (unsigned long long)srcX1 - srcX0, inH = (unsigned long long)srcY1 - srcY0;
+    const unsigned long long outW = (unsigned long long)dstX1 - dstX0;
+    const unsigned long long totalW = o->w, totalH = o->h;
+    BlitParams bp{};
+    bp.src = reinterpret_cast<const uint8_t*>(i->buf.p);
+    bp.dst = reinterpret_cast<uint8_t*>(o->buf.p);
+    bp.srcW = i->w; bp.srcH = i->h; bp.dstW = o->w; bp.dstH = o->h;
+    bp.srcX0 = srcX0; bp.srcY0 = srcY0; bp.srcX1 = srcX1; bp.srcY1 = srcY1;
+    const unsigned long long x0 = dstX0, x1 = std::min<unsigned long long>(totalW, x0 + outW);
+    const unsigned long long y0 = dstY0, y1 = std::min<unsigned long long>(totalH, dstY1);   // rows past the height are skipped
+    bp.x0 = (uint32_t)x0; bp.x1 = (uint32_t)std::max(x0, x1);
+    bp.y0 = (uint32_t)y0; bp.y1 = (uint32_t)std::max(y0, y1);
+    bp.foutW = ((inW << 16) / totalW) + 1ull;
+    bp.foutH = ((inH << 16) / totalH) + 1ull;
+    return check(c, launch_blit(bp, i->type, o->type, c->stream), "k_blit");
+}
+
+int sl_blit(sl_context* c, size_t outTex, size_t inTex)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    Tex* o = tex_at(c, outTex);
+    Tex* i = tex_at(c, inTex);
+    if (!o || !i) return SL_ERR_INVALID_HANDLE;
+    return sl_blit_region(c, outTex, inTex, 0, 0, i->w, i->h, 0, 0, o->w, o->h);   // SL_Context.cpp:747-769
+}
+
+// ---------------------------------------------------------------------------
 // differentiable draws
 // ---------------------------------------------------------------------------
 int sl_framebuffer_attach_visibility(sl_context* c, size_t fbo, size_t tex)
