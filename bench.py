@@ -173,6 +173,30 @@ def run_views(steps, warmup, tune=False):
     return sc, out
 
 
+def run_blit(reps=50):
+    """Present: blit of a 1920x1080 RGBA8 colour buffer to an RGBA8 back-buffer
+    of the same size (SL_Context::blit), reps times back to back."""
+    import softlight_amd
+    from softlight_amd import api, scenes
+
+    ctx = api.SLContext(softlight_amd.load_library())
+    W, H = 1920, 1080
+    src = ctx.create_texture(api.SL_COLOR_RGBA_8U, W, H, 1, scenes.uniform_u8(11, W * H * 4).reshape(H, W, 4))
+    dst = ctx.create_texture(api.SL_COLOR_RGBA_8U, W, H)
+    ctx.blit(dst, src)
+    ctx.finish()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.blit(dst, src)
+    ctx.finish()
+    dt = (time.perf_counter() - t0) / reps
+    gbs = W * H * 8 / dt / 1e9
+    return {"workload": "1920x1080 RGBA8 -> RGBA8 present blit", "us_per_blit": round(dt * 1e6, 2),
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_blit": W * H * 8},
+            "timing": "wall clock over back-to-back blits (launch overhead included)"}
+
+
 def run_cpu_baseline(frames=3):
     import oracle
     from softlight_amd import api, scenes
@@ -311,6 +335,7 @@ def main():
     if args.workload == "spheres":
         _, vr = run_views(max(3, args.steps // 4), 1)
         out["differentiable"] = views_summary(vr)
+        out["present_blit"] = run_blit()
     if not args.no_cpu and args.workload == "spheres":
         out["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -33,6 +33,9 @@ constexpr uint64_t kFlagInc = 2ull << 62;
 constexpr uint64_t kValMask = (1ull << 62) - 1;
 constexpr uint32_t kRecBits = 30;
 constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr int kVertexSlotBits = 10;                      // per-chunk vertex cache of k_setup
+constexpr int kVertexSlots = 1 << kVertexSlotBits;
+constexpr unsigned long long kNoVertex = ~0ull;
 
 // ---------------------------------------------------------------------------
 // small utilities
@@ -87,10 +90,8 @@ __device__ __forceinline__ void load_varyings(const SetupParams& P, const DrawDe
     }
 }
 
-// Fetch, vertex positions, cull, scissor matrix and clip status of one
-// triangle (SL_TriProcessor::process_verts, SL_TriProcessor.cpp:574-631).
-// Varyings are left to load_varyings: most triangles never need them.
-__device__ __forceinline__ void load_triangle_pos(const SetupParams& P, uint64_t g, const DrawDesc& D, uint32_t di, TriIn& T)
+// Vertex ids, instance and sequence number of global triangle g.
+__device__ __forceinline__ void load_triangle_ids(uint64_t g, const DrawDesc& D, uint32_t di, TriIn& T)
 {
     const uint64_t t = g - D.triBase;
     const uint64_t inst = t / D.trisPerInstance;
@@ -106,9 +107,12 @@ __device__ __forceinline__ void load_triangle_pos(const SetupParams& P, uint64_t
     T.inst = (uint32_t)inst;
     T.draw = di;
     T.seq = D.seqBase + 1u + (uint32_t)t;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) T.pos[k] = vertex_position(D, P.uniforms, T.vid[k], inst);
+}
 
+// Cull, scissor matrix and clip status of a triangle whose clip-space
+// positions are in T.pos.
+__device__ __forceinline__ void classify_triangle(const DrawDesc& D, TriIn& T)
+{
     if (D.cull != SL_CULL_OFF)
     {
         const float det = face_determinant(T.pos[0], T.pos[1], T.pos[2]);
@@ -118,6 +122,17 @@ __device__ __forceinline__ void load_triangle_pos(const SetupParams& P, uint64_t
 #pragma unroll
     for (int k = 0; k < 3; ++k) T.pos[k] = mat_mul(D.scissor, T.pos[k]);
     T.status = clip_status(T.pos[0], T.pos[1], T.pos[2]);
+}
+
+// Fetch, vertex positions, cull, scissor matrix and clip status of one
+// triangle (SL_TriProcessor::process_verts, SL_TriProcessor.cpp:574-631).
+// Varyings are left to load_varyings: most triangles never need them.
+__device__ __forceinline__ void load_triangle_pos(const SetupParams& P, uint64_t g, const DrawDesc& D, uint32_t di, TriIn& T)
+{
+    load_triangle_ids(g, D, di, T);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) T.pos[k] = vertex_position(D, P.uniforms, T.vid[k], T.inst);
+    classify_triangle(D, T);
 }
 
 // Per-lane draw lookup (the clip kernel's jobs come from any draw).
@@ -309,9 +324,63 @@ __global__ __launch_bounds__(kSetupBlock, 6) void k_setup(SetupParams P)
         __builtin_memcpy(&DR, w, sizeof(DrawDesc));
     }
     const DrawDesc& DU = DR;
-    if (g < P.totalTris && g - DU.triBase < DU.numTris)
+    const bool live = g < P.totalTris && g - DU.triBase < DU.numTris;
+    if (live) load_triangle_ids(g, DU, drawIdx, T);
+    if (DU.indexed)
     {
-        load_triangle_pos(P, g, DU, drawIdx, T);
+        // Per-chunk vertex cache: a run of 256 triangles of an indexed mesh
+        // references far fewer distinct vertices than 768, so each distinct
+        // (vertex, instance) position is computed once, by the lane that
+        // claims its slot, and shared through LDS.  The vertex programs are
+        // pure, so the positions are the ones each triangle would compute.
+        __shared__ unsigned long long sKey[kVertexSlots];
+        __shared__ float4 sPos[kVertexSlots];
+        for (int i = threadIdx.x; i < kVertexSlots; i += kSetupBlock) sKey[i] = kNoVertex;
+        __syncthreads();
+        int slot[3] = {-1, -1, -1};
+        bool own[3] = {false, false, false};
+        if (live)
+        {
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+            {
+                const unsigned long long key = (unsigned long long)T.vid[k] | ((unsigned long long)T.inst << 32);
+                uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - kVertexSlotBits));
+                for (int probe = 0; probe < 8; ++probe)
+                {
+                    const unsigned long long prev = atomicCAS(&sKey[h], kNoVertex, key);
+                    if (prev == kNoVertex || prev == key)
+                    {
+                        slot[k] = (int)h;
+                        own[k] = prev == kNoVertex;
+                        break;
+                    }
+                    h = (h + 1u) & (kVertexSlots - 1u);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+            {
+                if (own[k]) sPos[slot[k]] = vertex_position(DU, P.uniforms, T.vid[k], T.inst);
+                else if (slot[k] < 0) T.pos[k] = vertex_position(DU, P.uniforms, T.vid[k], T.inst);   // table crowded
+            }
+        }
+        __syncthreads();
+        if (live)
+        {
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (slot[k] >= 0) T.pos[k] = sPos[slot[k]];
+        }
+    }
+    else if (live)
+    {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) T.pos[k] = vertex_position(DU, P.uniforms, T.vid[k], T.inst);
+    }
+    if (live)
+    {
+        classify_triangle(DU, T);
         if (T.status == 3)
         {
             const DrawDesc& D = DU;
