@@ -143,7 +143,8 @@ struct FrameCtrl
     uint32_t totalPairs;
     uint32_t overflow;         // capacity exceeded (records, pairs or clip jobs)
     uint32_t error;            // look-back spin limit hit
-    uint32_t pad[2];
+    uint32_t emptyRecs;        // records that passed push_bin's tests but cover no pixel (not binned)
+    uint32_t pad;
     unsigned long long tested; // fragments depth-tested
     unsigned long long shaded; // fragments that passed and were shaded
 };

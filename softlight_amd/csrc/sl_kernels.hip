@@ -311,6 +311,7 @@ __global__ __launch_bounds__(kSetupBlock, 6) void k_setup(SetupParams P)
     T.seq = 0;
     uint32_t nrec = 0, npair = 0;
     bool clipped = false;
+    bool emptyRec = false;
     float4 sp[3];
     TileRect tr{0, -1, 0, -1};
     // the descriptor is uniform: copy it to scalar registers once (the fields
@@ -388,9 +389,29 @@ __global__ __launch_bounds__(kSetupBlock, 6) void k_setup(SetupParams P)
             for (int k = 0; k < 3; ++k) sp[k] = divide_and_snap(T.pos[k], D.vp);
             if (bin_accept(sp[0], sp[1], sp[2]))
             {
-                tr = tile_rect(sp[0], sp[1], sp[2], P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
-                nrec = 1;
-                npair = tile_count(tr);
+                const float yMinF = fminf(fminf(sp[0].y, sp[1].y), sp[2].y), yMaxF = fmaxf(fmaxf(sp[0].y, sp[1].y), sp[2].y);
+                if (f2i(yMaxF) - f2i(yMinF) <= kExactRows)
+                {
+                    // small triangle: bin it to the tiles it covers, or
+                    // drop it when it covers no pixel (still counted as binned)
+                    SetupRec r;
+                    setup_record(sp[0], sp[1], sp[2], drawIdx, T.seq, r);
+                    if (exact_tile_rect(r, P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY, tr))
+                    {
+                        nrec = 1;
+                        npair = tile_count(tr);
+                    }
+                    else
+                    {
+                        emptyRec = true;
+                    }
+                }
+                else
+                {
+                    tr = tile_rect(sp[0], sp[1], sp[2], P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
+                    nrec = 1;
+                    npair = tile_count(tr);
+                }
             }
         }
         else if (T.status == 1 && clip_quick(T.pos))
@@ -410,6 +431,12 @@ __global__ __launch_bounds__(kSetupBlock, 6) void k_setup(SetupParams P)
             }
             clipped = nrec > 0;
         }
+    }
+
+    // records dropped as empty still count as binned
+    {
+        const unsigned long long em = __ballot(emptyRec);
+        if ((threadIdx.x & 63) == 0 && em != 0ull) atomicAdd(&P.ctrl->emptyRecs, (uint32_t)__popcll(em));
     }
 
     // block-wide exclusive scan of the packed (pairs << 30 | records) counts

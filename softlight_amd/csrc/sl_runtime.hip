@@ -504,7 +504,7 @@ int execute_batch(sl_context* c)
     }
     (void)hipEventElapsedTime(&t, c->ev[4], c->ev[5]);
     msRaster += t;
-    c->stats.triangles_binned += totalRecs;
+    c->stats.triangles_binned += (uint64_t)totalRecs + c->hCtrl->emptyRecs;
     c->stats.tile_pairs += totalPairs;
     c->stats.fragments_tested += c->hCtrl->tested;
     c->stats.fragments_shaded += c->hCtrl->shaded;
@@ -617,7 +617,7 @@ int graph_collect(sl_context* c)
     c->stats.ms_total += t;
     c->stats.frames += G.pending;
     c->stats.triangles_submitted += G.tris * G.pending;
-    c->stats.triangles_binned += (uint64_t)G.hCtrl->totalRecs * G.pending;
+    c->stats.triangles_binned += ((uint64_t)G.hCtrl->totalRecs + G.hCtrl->emptyRecs) * G.pending;
     c->stats.tile_pairs += (uint64_t)G.hCtrl->totalPairs * G.pending;
     c->stats.fragments_tested += G.hCtrl->tested * G.pending;
     c->stats.fragments_shaded += G.hCtrl->shaded * G.pending;

@@ -327,6 +327,35 @@ __device__ __forceinline__ void scanline_step(const SetupRec& r, float yf, int32
     xMax = (lo >= hi) ? lo : hi;
 }
 
+// The tiles holding the pixels a record actually covers, found by running
+// the raster's own row test (scanline_step, the unsigned span check, the
+// W/H clamps) over the record's rows; false when it covers no pixel.  Used
+// for records of at most kExactRows rows.
+constexpr int kExactRows = 8;
+
+__device__ __forceinline__ bool exact_tile_rect(const SetupRec& r, int W, int H, int tilesX, int tilesY, int shx, int shy,
+                                                TileRect& out)
+{
+    int xlo = 0x7FFFFFFF, xhi = -1, ylo = 0x7FFFFFFF, yhi = -1;
+    const int y0 = max(r.minY, 0), y1 = min(r.maxY, H);
+    for (int y = y0; y < y1; ++y)
+    {
+        int32_t xMin, xMax;
+        scanline_step(r, (float)y, xMin, xMax);
+        if (xMin >= 0 && (uint32_t)xMin < (uint32_t)xMax && xMin < W)
+        {
+            xlo = min(xlo, xMin);
+            xhi = max(xhi, min(xMax, W) - 1);
+            ylo = min(ylo, y);
+            yhi = y;
+        }
+    }
+    if (xhi < 0) return false;
+    out.tx0 = xlo >> shx; out.tx1 = min(xhi >> shx, tilesX - 1);
+    out.ty0 = ylo >> shy; out.ty1 = min(yhi >> shy, tilesY - 1);
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // Samplers (SL_Sampler.hpp:51-218): REPEAT wrap, bilinear with the x+1/y+1
 // neighbours clamped to W/H (not W-1/H-1), ordered texel layout x + W*y.
