@@ -183,7 +183,7 @@ typedef struct sl_frame_stats {
     double   ms_bin;                /* tile binning (device time) */
     double   ms_raster;             /* raster + fragment stage (device time) */
     double   ms_total;              /* whole batched frame (device time) */
-    double   ms_setup_kernel;       /* the setup kernel alone (part of ms_vertex) */
+    double   ms_setup_kernel;       /* setup's counting pass alone (part of ms_vertex) */
     double   ms_clip_kernel;        /* the clip kernel alone (part of ms_vertex) */
     uint64_t setup_launches;        /* setup kernel launches (re-runs after a capacity grow included) */
     uint64_t raster_launches;       /* raster kernel launches */

@@ -134,15 +134,35 @@ struct ClipJob
     uint32_t nrec;             // records reserved by k_setup
 };
 
+// Setup runs as a counting pass, a chunk scan and an emitting pass.  Per
+// chunk of kSetupBlock triangles the counting pass leaves this summary (and
+// one staging word per surviving triangle); the chunk scan fills in the
+// prefixes.
+struct ChunkInfo
+{
+    unsigned long long agg;    // the chunk's packed (bin entries << 30 | records)
+    unsigned long long pre;    // packed totals of all earlier chunks (chunk scan)
+    uint32_t surv;             // triangles of the chunk that yield records (survivors)
+    uint32_t survPre;          // survivors of all earlier chunks (chunk scan)
+    uint32_t draw;             // the chunk's draw (draws start on chunk boundaries)
+    uint32_t empty;            // records dropped because they cover no pixel
+};
+static_assert(sizeof(ChunkInfo) == 32, "ChunkInfo layout");
+
+// Staging word of a survivor (slot = chunk * kSetupBlock + its rank among the
+// chunk's survivors): lane | records before it in the chunk << 8 | bin entries
+// before it in the chunk << 20.
+constexpr int kScanBlock = 1024;          // chunks per chunk-scan workgroup
+
 // Device-side frame control block.
 struct FrameCtrl
 {
-    uint32_t ticket;           // setup chunk ticket (ordered scheduling)
+    uint32_t survivors;        // triangles that yield records (chunk scan)
     uint32_t clipJobs;         // number of clip jobs appended
     uint32_t totalRecs;
     uint32_t totalPairs;
     uint32_t overflow;         // capacity exceeded (records, pairs or clip jobs)
-    uint32_t error;            // look-back spin limit hit
+    uint32_t error;            // the emitting pass found a survivor that yields no record
     uint32_t emptyRecs;        // records that passed push_bin's tests but cover no pixel (not binned)
     uint32_t pad;
     unsigned long long tested; // fragments depth-tested
@@ -167,7 +187,12 @@ struct SetupParams
     uint32_t pairCap;
     ClipJob* clipJobs;
     uint32_t clipCap;
-    unsigned long long* status;   // look-back status per chunk
+    ChunkInfo* chunkInfo;         // per chunk
+    unsigned long long* stage;    // per chunk, kSetupBlock staging words
+    unsigned long long* blockAgg; // packed counts per kScanBlock chunks (summed by atomics)
+    uint32_t* blockSurv;          // survivors per kScanBlock chunks (summed by atomics)
+    uint32_t* survChunk;          // per block of kSetupBlock survivors: the chunk of its first survivor
+    uint32_t numChunks;
     FrameCtrl* ctrl;
 };
 

@@ -1,13 +1,15 @@
 // sl_kernels.hip -- gfx950 kernels of the SoftLight triangle pipeline.
 //
-//   k_setup    vertex programs, culling, clip classification, perspective
-//              divide, viewport snap, bounding-box reject and plane setup for
-//              one triangle per lane; ordered compaction of the surviving
-//              records and their (tile, record) bin entries through a
-//              single-pass decoupled look-back over 256-triangle chunks.
-//              Triangles that need clipping get their record/pair slots
-//              reserved here (position-only clipping to count them) and are
-//              finished by k_clip.
+//   k_setup    counting pass: position part of the vertex programs,
+//              culling, clip classification, perspective divide, viewport
+//              snap and bounding-box reject for one triangle per lane; per
+//              256-triangle chunk, the mask of triangles that yield records
+//              and the packed record / bin-entry total (position-only
+//              clipping counts the straddling triangles).
+//   k_emit     emitting pass: global offsets from the chunk totals, then the
+//              surviving triangles' plane setup, varyings, records and
+//              (tile, record) bin entries, in submission order.  Straddling
+//              triangles get their slots reserved and are finished by k_clip.
 //   k_clip     6-plane Sutherland-Hodgman clipping with varyings and the
 //              reversed fan, writing into the reserved slots.
 //   k_bin_*    ordered binning: stable counting sort of the bin entries by tile.
@@ -28,12 +30,8 @@
 
 namespace slg {
 
-constexpr uint64_t kFlagAgg = 1ull << 62;
-constexpr uint64_t kFlagInc = 2ull << 62;
-constexpr uint64_t kValMask = (1ull << 62) - 1;
-constexpr uint32_t kRecBits = 30;
-constexpr uint32_t kSpinLimit = 1u << 22;
-constexpr int kVertexSlotBits = 10;                      // per-chunk vertex cache of k_setup
+constexpr uint32_t kRecBits = 30;                        // packed counts: bin entries << kRecBits | records
+constexpr int kVertexSlotBits = 9;                       // per-chunk vertex cache of k_setup
 constexpr int kVertexSlots = 1 << kVertexSlotBits;
 constexpr unsigned long long kNoVertex = ~0ull;
 
@@ -78,8 +76,7 @@ struct TriIn
 // and perspective correction treat them like the others).
 __device__ __forceinline__ void load_varyings(const SetupParams& P, const DrawDesc& D, TriIn& T)
 {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) (void)vertex_program(D, P.uniforms, T.vid[k], T.inst, T.var[k]);
+    vertex_varyings3(D, P.uniforms, T.vid, T.inst, T.var);
     if (D.visSlot != kNoVisSlot)
     {
 #pragma unroll
@@ -130,8 +127,8 @@ __device__ __forceinline__ void classify_triangle(const DrawDesc& D, TriIn& T)
 __device__ __forceinline__ void load_triangle_pos(const SetupParams& P, uint64_t g, const DrawDesc& D, uint32_t di, TriIn& T)
 {
     load_triangle_ids(g, D, di, T);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) T.pos[k] = vertex_position(D, P.uniforms, T.vid[k], T.inst);
+    const bool all3[3] = {true, true, true};
+    vertex_positions3(D, P.uniforms, T.vid, T.inst, all3, T.pos);
     classify_triangle(D, T);
 }
 
@@ -266,7 +263,15 @@ __device__ __forceinline__ void write_record(const SetupParams& P, uint32_t recI
 }
 
 // ---------------------------------------------------------------------------
-// k_setup
+// Setup: a counting pass (k_setup) and an emitting pass (k_emit).
+//
+// k_setup runs the position part of the vertex programs, classification and
+// bin reservation for one triangle per lane and leaves, per chunk of
+// kSetupBlock triangles, the mask of triangles that yield records and the
+// chunk's packed (bin entries << kRecBits | records) total, which it also adds
+// to its group and super-group sums.  Nothing waits on another workgroup.
+// k_emit then turns the sums into each group's global offsets and writes the
+// records in submission order.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ unsigned long long wave_incl_scan(unsigned long long v)
 {
@@ -280,22 +285,127 @@ __device__ __forceinline__ unsigned long long wave_incl_scan(unsigned long long 
     return v;
 }
 
-// 6 waves per SIMD: caps the register budget at 80 VGPRs
-__global__ __launch_bounds__(kSetupBlock, 6) void k_setup(SetupParams P)
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v)
 {
-    __shared__ uint32_t sChunk;
-    __shared__ unsigned long long sWave[kSetupBlock / 64];
-    __shared__ unsigned long long sPrefix;
-
-    __shared__ uint32_t sDraw;
-    if (threadIdx.x == 0)
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
     {
-        sChunk = atomicAdd(&P.ctrl->ticket, 1u);
+        const uint32_t o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Last draw whose first triangle is <= g (draws are sorted by triBase and
+// draws[0].triBase == 0), searched 64 candidates per round by the calling
+// wave; every lane returns the result.
+__device__ __forceinline__ uint32_t find_draw_wave(const DrawDesc* __restrict__ draws, uint32_t n, uint64_t g)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t lo = 0, hi = n;   // the answer is in [lo, hi); draws[lo].triBase <= g
+    while (hi - lo > 1)
+    {
+        const uint32_t step = (hi - lo + 63) / 64;
+        const uint32_t idx = lo + lane * step;
+        const bool ok = idx < hi && draws[idx].triBase <= g;
+        const unsigned long long b = __ballot(ok);   // a prefix of lanes, lane 0 always set
+        lo += (uint32_t)(63 - __clzll((long long)b)) * step;
+        hi = min(hi, lo + step);
+    }
+    return lo;
+}
+
+// Records and bin entries a triangle yields, from its classified clip-space
+// positions (T.pos after classify_triangle).  Both setup passes run this.
+struct TriCount
+{
+    float4 sp[3];              // snapped screen positions (unclipped triangles)
+    TileRect tr;               // tiles to bin to (unclipped triangles)
+    uint32_t nrec, npair;
+    bool clipped;              // needs k_clip (nrec fan triangles)
+    bool empty;                // small triangle that covers no pixel: dropped, counted as binned
+};
+
+__device__ __forceinline__ void count_triangle(const SetupParams& P, const DrawDesc& D, uint32_t drawIdx, const TriIn& T,
+                                               TriCount& C)
+{
+    C.nrec = 0;
+    C.npair = 0;
+    C.clipped = false;
+    C.empty = false;
+    C.tr = TileRect{0, -1, 0, -1};
+    if (T.status == 3)
+    {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) C.sp[k] = divide_and_snap(T.pos[k], D.vp);
+        if (bin_accept(C.sp[0], C.sp[1], C.sp[2]))
+        {
+            const float yMinF = fminf(fminf(C.sp[0].y, C.sp[1].y), C.sp[2].y);
+            const float yMaxF = fmaxf(fmaxf(C.sp[0].y, C.sp[1].y), C.sp[2].y);
+            if (f2i(yMaxF) - f2i(yMinF) <= kExactRows)
+            {
+                // small triangle: bin it to the tiles it covers, or drop it
+                // when it covers no pixel (still counted as binned)
+                SetupRec r;
+                setup_record(C.sp[0], C.sp[1], C.sp[2], drawIdx, T.seq, r);
+                if (exact_tile_rect(r, P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY, C.tr))
+                {
+                    C.nrec = 1;
+                    C.npair = tile_count(C.tr);
+                }
+                else
+                {
+                    C.empty = true;
+                }
+            }
+            else
+            {
+                C.tr = tile_rect(C.sp[0], C.sp[1], C.sp[2], P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
+                C.nrec = 1;
+                C.npair = tile_count(C.tr);
+            }
+        }
+    }
+    else if (T.status == 1 && clip_quick(T.pos))
+    {
+        float4 poly[9];
+        const int n = clip_polygon<false>(T.pos, nullptr, 0, poly, nullptr);
+        for (int k = 0; k < n; ++k) poly[k] = divide_and_snap(poly[k], D.vp);
+        for (int i = n - 3; i >= 0; --i)
+        {
+            if (bin_accept(poly[0], poly[i + 1], poly[i + 2]))
+            {
+                const TileRect r = tile_rect(poly[0], poly[i + 1], poly[i + 2], P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
+                ++C.nrec;
+                C.npair += tile_count(r);
+            }
+        }
+        C.clipped = C.nrec > 0;
+    }
+}
+
+// 8 waves per SIMD: the register budget that keeps every wave slot usable
+__global__ __launch_bounds__(kSetupBlock, 8) void k_setup(SetupParams P)
+{
+    const uint32_t chunk = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    __shared__ uint32_t sDraw;
+    if (threadIdx.x < 64)
+    {
         // draws start on chunk boundaries, so one draw covers the whole chunk
-        sDraw = find_draw(P.draws, P.numDraws, (uint64_t)sChunk * kSetupBlock);
+        const uint32_t d = find_draw_wave(P.draws, P.numDraws, (uint64_t)chunk * kSetupBlock);
+        if (threadIdx.x == 0) sDraw = d;
     }
     __syncthreads();
-    const uint32_t chunk = sChunk;
     const uint32_t drawIdx = __builtin_amdgcn_readfirstlane(sDraw);
     const uint64_t g = (uint64_t)chunk * kSetupBlock + threadIdx.x;
     // the chunk's draw descriptor, staged once in LDS for all lanes
@@ -309,11 +419,6 @@ __global__ __launch_bounds__(kSetupBlock, 6) void k_setup(SetupParams P)
     T.status = 0;
     T.draw = 0;
     T.seq = 0;
-    uint32_t nrec = 0, npair = 0;
-    bool clipped = false;
-    bool emptyRec = false;
-    float4 sp[3];
-    TileRect tr{0, -1, 0, -1};
     // the descriptor is uniform: copy it to scalar registers once (the fields
     // that are used stay in SGPRs instead of being re-read from LDS)
     DrawDesc DR;
@@ -329,11 +434,12 @@ __global__ __launch_bounds__(kSetupBlock, 6) void k_setup(SetupParams P)
     if (live) load_triangle_ids(g, DU, drawIdx, T);
     if (DU.indexed)
     {
-        // Per-chunk vertex cache: a run of 256 triangles of an indexed mesh
-        // references far fewer distinct vertices than 768, so each distinct
-        // (vertex, instance) position is computed once, by the lane that
-        // claims its slot, and shared through LDS.  The vertex programs are
-        // pure, so the positions are the ones each triangle would compute.
+        // Per-chunk vertex cache: a run of kSetupBlock triangles of an indexed
+        // mesh references far fewer distinct vertices than 3 * kSetupBlock,
+        // so each distinct (vertex, instance) position is computed once, by
+        // the lane that claims its slot, and shared through LDS.  The vertex
+        // programs are pure, so the positions are the ones each triangle
+        // would compute; a lane that finds no slot computes its own.
         __shared__ unsigned long long sKey[kVertexSlots];
         __shared__ float4 sPos[kVertexSlots];
         for (int i = threadIdx.x; i < kVertexSlots; i += kSetupBlock) sKey[i] = kNoVertex;
@@ -359,12 +465,12 @@ __global__ __launch_bounds__(kSetupBlock, 6) void k_setup(SetupParams P)
                     h = (h + 1u) & (kVertexSlots - 1u);
                 }
             }
+            // owned slots and (table crowded) unslotted vertices
+            const bool need[3] = {own[0] || slot[0] < 0, own[1] || slot[1] < 0, own[2] || slot[2] < 0};
+            vertex_positions3(DU, P.uniforms, T.vid, T.inst, need, T.pos);
 #pragma unroll
             for (int k = 0; k < 3; ++k)
-            {
-                if (own[k]) sPos[slot[k]] = vertex_position(DU, P.uniforms, T.vid[k], T.inst);
-                else if (slot[k] < 0) T.pos[k] = vertex_position(DU, P.uniforms, T.vid[k], T.inst);   // table crowded
-            }
+                if (own[k]) sPos[slot[k]] = T.pos[k];
         }
         __syncthreads();
         if (live)
@@ -376,177 +482,317 @@ __global__ __launch_bounds__(kSetupBlock, 6) void k_setup(SetupParams P)
     }
     else if (live)
     {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) T.pos[k] = vertex_position(DU, P.uniforms, T.vid[k], T.inst);
+        const bool all3[3] = {true, true, true};
+        vertex_positions3(DU, P.uniforms, T.vid, T.inst, all3, T.pos);
     }
+    TriCount C;
+    C.nrec = 0;
+    C.npair = 0;
+    C.empty = false;
     if (live)
     {
         classify_triangle(DU, T);
-        if (T.status == 3)
-        {
-            const DrawDesc& D = DU;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) sp[k] = divide_and_snap(T.pos[k], D.vp);
-            if (bin_accept(sp[0], sp[1], sp[2]))
-            {
-                const float yMinF = fminf(fminf(sp[0].y, sp[1].y), sp[2].y), yMaxF = fmaxf(fmaxf(sp[0].y, sp[1].y), sp[2].y);
-                if (f2i(yMaxF) - f2i(yMinF) <= kExactRows)
-                {
-                    // small triangle: bin it to the tiles it covers, or
-                    // drop it when it covers no pixel (still counted as binned)
-                    SetupRec r;
-                    setup_record(sp[0], sp[1], sp[2], drawIdx, T.seq, r);
-                    if (exact_tile_rect(r, P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY, tr))
-                    {
-                        nrec = 1;
-                        npair = tile_count(tr);
-                    }
-                    else
-                    {
-                        emptyRec = true;
-                    }
-                }
-                else
-                {
-                    tr = tile_rect(sp[0], sp[1], sp[2], P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
-                    nrec = 1;
-                    npair = tile_count(tr);
-                }
-            }
-        }
-        else if (T.status == 1 && clip_quick(T.pos))
-        {
-            const DrawDesc& D = DU;
-            float4 poly[9];
-            const int n = clip_polygon<false>(T.pos, nullptr, 0, poly, nullptr);
-            for (int k = 0; k < n; ++k) poly[k] = divide_and_snap(poly[k], D.vp);
-            for (int i = n - 3; i >= 0; --i)
-            {
-                if (bin_accept(poly[0], poly[i + 1], poly[i + 2]))
-                {
-                    const TileRect r = tile_rect(poly[0], poly[i + 1], poly[i + 2], P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
-                    ++nrec;
-                    npair += tile_count(r);
-                }
-            }
-            clipped = nrec > 0;
-        }
+        count_triangle(P, DU, drawIdx, T, C);
     }
 
-    // records dropped as empty still count as binned
-    {
-        const unsigned long long em = __ballot(emptyRec);
-        if ((threadIdx.x & 63) == 0 && em != 0ull) atomicAdd(&P.ctrl->emptyRecs, (uint32_t)__popcll(em));
-    }
-
-    // block-wide exclusive scan of the packed (pairs << 30 | records) counts
-    const unsigned long long mine = ((unsigned long long)npair << kRecBits) | nrec;
+    // chunk summary: exclusive scan of the packed counts (each survivor's
+    // offset within the chunk), survivor ranks, staging words, totals
+    const unsigned long long mine = ((unsigned long long)C.npair << kRecBits) | C.nrec;
     const unsigned long long incl = wave_incl_scan(mine);
-    const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 63) sWave[wave] = incl;
+    const unsigned long long survivors = __ballot(C.nrec > 0);
+    const unsigned long long empties = __ballot(C.empty);
+    __shared__ unsigned long long sSum[kSetupBlock / 64];
+    __shared__ uint32_t sSurv[kSetupBlock / 64];
+    __shared__ uint32_t sEmpty[kSetupBlock / 64];
+    if (lane == 63) sSum[wave] = incl;
+    if (lane == 0)
+    {
+        sSurv[wave] = (uint32_t)__popcll(survivors);
+        sEmpty[wave] = (uint32_t)__popcll(empties);
+    }
     __syncthreads();
-    unsigned long long waveBase = 0, blockTotal = 0;
+    unsigned long long before = incl - mine, tot = 0;
+    uint32_t rank = (uint32_t)__popcll(survivors & ((1ull << lane) - 1ull)), surv = 0, emp = 0;
 #pragma unroll
     for (int w = 0; w < kSetupBlock / 64; ++w)
     {
-        if (w < wave) waveBase += sWave[w];
-        blockTotal += sWave[w];
+        if (w < wave)
+        {
+            before += sSum[w];
+            rank += sSurv[w];
+        }
+        tot += sSum[w];
+        surv += sSurv[w];
+        emp += sEmpty[w];
     }
-
-    // decoupled look-back by the first wave: each round reads the statuses of
-    // the 64 nearest unread predecessors at once (ticket order guarantees every
-    // predecessor is running, so every wait ends).  A status word carries its
-    // own value and nothing else is read through it, so relaxed agent-scope
-    // atomics suffice: no L2 write-back or invalidate per round (the kernel
-    // boundary publishes the records to the next kernels).
-    if (threadIdx.x < 64)
+    if (C.nrec > 0)
     {
-        const int lane = threadIdx.x;
-        unsigned long long prefix = 0;
-        if (chunk == 0)
+        const unsigned long long recBefore = before & ((1ull << kRecBits) - 1), pairBefore = before >> kRecBits;
+        P.stage[(uint64_t)chunk * kSetupBlock + rank] = (unsigned long long)threadIdx.x | (recBefore << 8) | (pairBefore << 20);
+    }
+    if (threadIdx.x == 0)
+    {
+        ChunkInfo& I = P.chunkInfo[chunk];
+        I.agg = tot;
+        I.surv = surv;
+        I.draw = drawIdx;
+        I.empty = emp;
+        if (tot != 0)
         {
-            if (lane == 0)
-                __hip_atomic_store(&P.status[0], kFlagInc | blockTotal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            atomicAdd(&P.blockAgg[chunk / kScanBlock], tot);
+            atomicAdd(&P.blockSurv[chunk / kScanBlock], surv);
         }
-        else
-        {
-            if (lane == 0)
-                __hip_atomic_store(&P.status[chunk], kFlagAgg | blockTotal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int64_t j = (int64_t)chunk - 1;          // nearest predecessor not yet accumulated
-            uint32_t spins = 0;
-            while (true)
-            {
-                const int64_t idx = j - lane;
-                unsigned long long st = kFlagInc;    // before chunk 0: an inclusive zero
-                if (idx >= 0) st = __hip_atomic_load(&P.status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                const unsigned long long flag = st & ~kValMask;
-                const unsigned long long incMask = __ballot(flag == kFlagInc);
-                const unsigned long long zeroMask = __ballot(flag == 0);
-                const int firstInc = incMask ? (__ffsll((long long)incMask) - 1) : 64;
-                const unsigned long long below = (firstInc >= 64) ? ~0ull : ((2ull << firstInc) - 1ull);
-                if (zeroMask & below)
-                {
-                    if (++spins > kSpinLimit)
-                    {
-                        if (lane == 0) atomicExch(&P.ctrl->error, 1u);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                unsigned long long v = (lane <= firstInc) ? (st & kValMask) : 0ull;
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-                prefix += v;
-                if (firstInc < 64) break;
-                j -= 64;
-            }
-            if (lane == 0)
-                __hip_atomic_store(&P.status[chunk], kFlagInc | (prefix + blockTotal), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (lane == 0)
-        {
-            sPrefix = prefix;
-            if (chunk == gridDim.x - 1)
-            {
-                const unsigned long long tot = prefix + blockTotal;
-                P.ctrl->totalRecs = (uint32_t)(tot & ((1ull << kRecBits) - 1));
-                P.ctrl->totalPairs = (uint32_t)(tot >> kRecBits);
-            }
-        }
+    }
+}
+
+// k_chunk_scan: exclusive prefixes of the chunk totals and survivor counts.
+// Workgroup b owns chunks [b * kScanBlock, (b + 1) * kScanBlock): its offset is
+// the sum of the earlier blocks' totals (which k_setup accumulated), the rest
+// a workgroup scan.  Also records, for every block of kSetupBlock survivors,
+// the chunk that holds its first survivor, and the frame totals.
+__device__ __forceinline__ unsigned long long block_sum_u64(unsigned long long v, unsigned long long* sRed)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    v = wave_sum_u64(v);
+    __syncthreads();
+    if (lane == 0) sRed[wave] = v;
+    __syncthreads();
+    unsigned long long s = 0;
+    for (int w = 0; w < nw; ++w) s += sRed[w];
+    return s;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_chunk_scan(SetupParams P)
+{
+    __shared__ unsigned long long sRed[kScanBlock / 64];
+    __shared__ unsigned long long sA[kScanBlock / 64];
+    __shared__ uint32_t sS[kScanBlock / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint32_t b = blockIdx.x;
+    const uint32_t c = b * kScanBlock + t;
+    unsigned long long pa = 0, ps = 0;
+    for (uint32_t i = t; i < b; i += kScanBlock)
+    {
+        pa += P.blockAgg[i];
+        ps += P.blockSurv[i];
+    }
+    const unsigned long long prefA = block_sum_u64(pa, sRed);
+    const unsigned long long prefS = block_sum_u64(ps, sRed);
+    unsigned long long a = 0;
+    uint32_t sv = 0, em = 0;
+    if (c < P.numChunks)
+    {
+        const ChunkInfo& I = P.chunkInfo[c];
+        a = I.agg;
+        sv = I.surv;
+        em = I.empty;
+    }
+    const unsigned long long emp = block_sum_u64(em, sRed);
+    const unsigned long long ia = wave_incl_scan(a);
+    const uint32_t is = wave_incl_scan_u32(sv);
+    if (lane == 63)
+    {
+        sA[wave] = ia;
+        sS[wave] = is;
     }
     __syncthreads();
-
-    const unsigned long long base = sPrefix + waveBase + (incl - mine);
-    const uint32_t recOff = (uint32_t)(base & ((1ull << kRecBits) - 1));
-    const uint64_t pairOff = base >> kRecBits;
-    if (nrec == 0) return;
-    if ((uint64_t)recOff + nrec > P.recCap || pairOff + npair > P.pairCap)
+    unsigned long long ea = prefA + ia - a, totA = prefA;
+    uint32_t es = (uint32_t)prefS + is - sv, totS = (uint32_t)prefS;
+    for (int w = 0; w < kScanBlock / 64; ++w)
     {
-        atomicExch(&P.ctrl->overflow, 1u);
-        return;
-    }
-    if (!clipped)
-    {
-        load_varyings(P, DU, T);
-        write_record(P, recOff, (uint32_t)pairOff, sp[0], sp[1], sp[2], T.var[0], T.var[1], T.var[2], DU.nvary,
-                     T.draw, T.seq, pack_raster_state(DU), tr);
-    }
-    else
-    {
-        const uint32_t job = atomicAdd(&P.ctrl->clipJobs, 1u);
-        if (job < P.clipCap)
+        if (w < wave)
         {
-            ClipJob J;
-            J.tri = (uint32_t)g;
-            J.recOff = recOff;
-            J.pairOff = (uint32_t)pairOff;
-            J.nrec = nrec;
-            P.clipJobs[job] = J;
+            ea += sA[w];
+            es += sS[w];
+        }
+        totA += sA[w];
+        totS += sS[w];
+    }
+    if (c < P.numChunks)
+    {
+        ChunkInfo& I = P.chunkInfo[c];
+        I.pre = ea;
+        I.survPre = es;
+        // [es, es + sv) holds at most one multiple of kSetupBlock
+        const uint32_t k = (es + kSetupBlock - 1) / kSetupBlock;
+        if (sv > 0 && (uint64_t)k * kSetupBlock < (uint64_t)es + sv) P.survChunk[k] = c;
+    }
+    if (t == 0)
+    {
+        if (emp != 0) atomicAdd(&P.ctrl->emptyRecs, (uint32_t)emp);
+        if (b == gridDim.x - 1)
+        {
+            P.ctrl->totalRecs = (uint32_t)(totA & ((1ull << kRecBits) - 1));
+            P.ctrl->totalPairs = (uint32_t)(totA >> kRecBits);
+            P.ctrl->survivors = totS;
+        }
+    }
+}
+
+// k_emit: one workgroup per block of kEmitBlock survivors (grid-stride).  A
+// lane finds its survivor's chunk by searching the chunks' survivor prefixes
+// (kEmitBlock chunks per window in LDS, starting from the block's first
+// chunk), reads the staging word for its lane and offsets within the chunk,
+// re-fetches and re-classifies the triangle (the vertex programs are pure, so
+// this gives the counting pass's bits) and writes its record, varyings and
+// bin entries at the chunk's prefix plus those offsets.  Straddling triangles
+// become clip jobs with their slots reserved.
+constexpr int kEmitBlock = 256;
+static_assert(kEmitBlock == kSetupBlock, "a block of survivors spans whole staging rows");
+
+__global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
+{
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    __shared__ uint32_t sSP[kEmitBlock];
+    const uint32_t S = P.ctrl->survivors;
+    for (uint32_t k = blockIdx.x; (uint64_t)k * kEmitBlock < S; k += gridDim.x)
+    {
+        const uint32_t s = k * kEmitBlock + t;
+        const bool act = s < S;
+        // the survivor's chunk: the last chunk whose survivor prefix is <= s.
+        // The first window starts at the chunk of survivor k * kEmitBlock, so
+        // its first prefix is <= s for every lane; every lane is resolved by
+        // the window after the one holding its chunk, or by the last window.
+        uint32_t chunk = 0;
+        bool found = !act;
+        const uint32_t maxWindows = P.numChunks / kEmitBlock + 2;
+        uint32_t cw = P.survChunk[k];
+        for (uint32_t it = 0;; ++it, cw += kEmitBlock)
+        {
+            const uint32_t c = cw + t;
+            __syncthreads();
+            sSP[t] = c < P.numChunks ? P.chunkInfo[c].survPre : 0xFFFFFFFFu;
+            __syncthreads();
+            if (!found)
+            {
+                if (sSP[0] > s)
+                {
+                    chunk = cw - 1;   // the previous window's last chunk
+                    found = true;
+                }
+                else
+                {
+                    int j = 0;
+#pragma unroll
+                    for (int step = kEmitBlock / 2; step > 0; step >>= 1)
+                        if (sSP[j + step] <= s) j += step;
+                    if (j < kEmitBlock - 1 || cw + kEmitBlock >= P.numChunks)
+                    {
+                        chunk = cw + (uint32_t)j;
+                        found = true;
+                    }
+                }
+            }
+            if (__syncthreads_and(found)) break;
+            if (it >= maxWindows) break;   // unreachable with consistent prefixes
+        }
+        if (!found || chunk >= P.numChunks)
+        {
+            atomicExch(&P.ctrl->error, 1u);
+            continue;
+        }
+        TriIn T;
+        T.status = 0;
+        T.draw = 0;
+        T.seq = 0;
+        TriCount C;
+        C.nrec = 0;
+        C.npair = 0;
+        C.clipped = false;
+        uint64_t g = 0;
+        uint32_t di = 0;
+        unsigned long long off = 0;
+        bool act2 = act;
+        if (act)
+        {
+            const ChunkInfo& I = P.chunkInfo[chunk];
+            const uint32_t r = s - I.survPre;
+            act2 = r < I.surv;
+            if (!act2) atomicExch(&P.ctrl->error, 1u);
+        }
+        if (act2)
+        {
+            const ChunkInfo& I = P.chunkInfo[chunk];
+            const unsigned long long st = P.stage[(uint64_t)chunk * kSetupBlock + (s - I.survPre)];
+            g = (uint64_t)chunk * kSetupBlock + (st & 0xFFu);
+            di = I.draw;
+            off = I.pre + ((st >> 8) & 0xFFFu) + ((st >> 20) << kRecBits);
+        }
+        // When the wave's survivors share one draw (the usual case) its
+        // descriptor goes to scalar registers: uniform branches, scalar loads.
+        const uint32_t dFirst = __builtin_amdgcn_readfirstlane(di);
+        const bool uni = __ballot(act2 && di != dFirst) == 0ull;
+        DrawDesc DR;
+        if (uni)
+        {
+            constexpr int kDescWords = (int)(sizeof(DrawDesc) / 4);
+            static_assert(sizeof(DrawDesc) % 4 == 0 && kDescWords <= 192, "draw descriptor words");
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(P.draws + dFirst);
+            uint32_t v[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) v[j] = (lane + 64 * j < kDescWords) ? src[lane + 64 * j] : 0u;
+            uint32_t wds[kDescWords];
+#pragma unroll
+            for (int j = 0; j < kDescWords; ++j) wds[j] = __builtin_amdgcn_readlane(v[j / 64], j % 64);
+            __builtin_memcpy(&DR, wds, sizeof(DrawDesc));
+            if (act2)
+            {
+                load_triangle_pos(P, g, DR, di, T);
+                count_triangle(P, DR, di, T, C);
+            }
+        }
+        else if (act2)
+        {
+            const DrawDesc& D = P.draws[di];
+            load_triangle_pos(P, g, D, di, T);
+            count_triangle(P, D, di, T, C);
+        }
+        if (!act2) continue;
+        if (C.nrec == 0)
+        {
+            atomicExch(&P.ctrl->error, 1u);
+            continue;
+        }
+        const uint32_t recOff = (uint32_t)(off & ((1ull << kRecBits) - 1));
+        const uint64_t pairOff = off >> kRecBits;
+        if ((uint64_t)recOff + C.nrec > P.recCap || pairOff + C.npair > P.pairCap)
+        {
+            atomicExch(&P.ctrl->overflow, 1u);
+            continue;
+        }
+        if (!C.clipped)
+        {
+            if (uni)
+            {
+                load_varyings(P, DR, T);
+                write_record(P, recOff, (uint32_t)pairOff, C.sp[0], C.sp[1], C.sp[2], T.var[0], T.var[1], T.var[2], DR.nvary,
+                             T.draw, T.seq, pack_raster_state(DR), C.tr);
+            }
+            else
+            {
+                const DrawDesc& D = P.draws[di];
+                load_varyings(P, D, T);
+                write_record(P, recOff, (uint32_t)pairOff, C.sp[0], C.sp[1], C.sp[2], T.var[0], T.var[1], T.var[2], D.nvary,
+                             T.draw, T.seq, pack_raster_state(D), C.tr);
+            }
         }
         else
         {
-            atomicExch(&P.ctrl->overflow, 1u);
+            const uint32_t job = atomicAdd(&P.ctrl->clipJobs, 1u);
+            if (job < P.clipCap)
+            {
+                ClipJob J;
+                J.tri = (uint32_t)g;
+                J.recOff = recOff;
+                J.pairOff = (uint32_t)pairOff;
+                J.nrec = C.nrec;
+                P.clipJobs[job] = J;
+            }
+            else
+            {
+                atomicExch(&P.ctrl->overflow, 1u);
+            }
         }
     }
 }
@@ -1252,6 +1498,22 @@ hipError_t launch_setup(const SetupParams& p, uint64_t chunks, hipStream_t s)
 {
     (void)hipGetLastError();   // drop a stale error from an earlier call
     hipLaunchKernelGGL(k_setup, dim3((unsigned)chunks), dim3(kSetupBlock), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_chunk_scan(const SetupParams& p, hipStream_t s)
+{
+    (void)hipGetLastError();   // drop a stale error from an earlier call
+    const uint32_t blocks = (p.numChunks + kScanBlock - 1) / kScanBlock;
+    hipLaunchKernelGGL(k_chunk_scan, dim3(blocks), dim3(kScanBlock), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_emit(const SetupParams& p, hipStream_t s)
+{
+    (void)hipGetLastError();   // drop a stale error from an earlier call
+    const uint32_t grid = p.numChunks < 2048u ? p.numChunks : 2048u;   // grid-stride over blocks of survivors
+    hipLaunchKernelGGL(k_emit, dim3(grid), dim3(kEmitBlock), 0, s, p);
     return hipGetLastError();
 }
 

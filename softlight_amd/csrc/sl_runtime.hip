@@ -140,7 +140,7 @@ struct sl_context
     FrameGraph graph;                              // captured frame
     uint64_t lastTotals[3] = {0, 0, 0};            // records, bin entries, clip jobs of the last eager frame
     FrameCtrl* hCtrl = nullptr;                    // pinned readback of the control block
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[10] = {};                        // eager frames: 0-5 stages, 6-7 graph replays, 8-9 counting pass
     hipEvent_t bwdEv[2] = {};                      // first / last backward since the last collection
     hipEvent_t gEv[4] = {};                        // graph replays: setup start/end, raster start/end
     uint64_t bwdPending = 0;
@@ -271,6 +271,16 @@ int make_plan(sl_context* c, const Batch& B, Plan& p)
     return SL_OK;
 }
 
+// Setup's per-chunk buffer: ChunkInfo per chunk, kSetupBlock staging words
+// per chunk, the block sums (packed totals, then survivors; zeroed before
+// every frame) and the first chunk of every block of kSetupBlock survivors.
+uint64_t chunk_blocks(uint64_t chunks) { return (chunks + kScanBlock - 1) / kScanBlock; }
+size_t chunk_buffer_bytes(uint64_t chunks)
+{
+    return (sizeof(ChunkInfo) + sizeof(unsigned long long) * kSetupBlock) * chunks +
+           (sizeof(unsigned long long) + sizeof(uint32_t)) * chunk_blocks(chunks) + sizeof(uint32_t) * chunks;
+}
+
 // Allocates a frame's buffers at the resource set's current capacities.
 int ensure_frame(sl_context* c, FrameRes& R, const Plan& p, const Batch& B)
 {
@@ -282,7 +292,7 @@ int ensure_frame(sl_context* c, FrameRes& R, const Plan& p, const Batch& B)
     if (R.recCap == 0) { R.recCap = 1u << 18; R.pairCap = 1u << 20; R.clipCap = 1u << 16; }
     if ((rc = ensure(c, R.draws, sizeof(DrawDesc) * B.draws.size())) != SL_OK) return rc;
     if ((rc = ensure(c, R.uni, std::max<size_t>(B.uni.size(), 16))) != SL_OK) return rc;
-    if ((rc = ensure(c, R.status, sizeof(unsigned long long) * p.chunks)) != SL_OK) return rc;
+    if ((rc = ensure(c, R.status, chunk_buffer_bytes(p.chunks))) != SL_OK) return rc;
     if ((rc = ensure(c, R.recs, sizeof(SetupRec) * (size_t)R.recCap)) != SL_OK) return rc;
     if ((rc = ensure(c, R.vars, sizeof(float4) * (size_t)p.varStride * R.recCap)) != SL_OK) return rc;
     if ((rc = ensure(c, R.keys, sizeof(uint32_t) * (size_t)R.pairCap)) != SL_OK) return rc;
@@ -313,7 +323,12 @@ SetupParams setup_params(const Plan& p, const Batch& B, FrameRes& R)
     sp.pairCap = R.pairCap;
     sp.clipJobs = reinterpret_cast<ClipJob*>(R.clip.p);
     sp.clipCap = R.clipCap;
-    sp.status = reinterpret_cast<unsigned long long*>(R.status.p);
+    sp.chunkInfo = reinterpret_cast<ChunkInfo*>(R.status.p);
+    sp.stage = reinterpret_cast<unsigned long long*>(sp.chunkInfo + p.chunks);
+    sp.blockAgg = sp.stage + (size_t)kSetupBlock * p.chunks;
+    sp.blockSurv = reinterpret_cast<uint32_t*>(sp.blockAgg + chunk_blocks(p.chunks));
+    sp.survChunk = sp.blockSurv + chunk_blocks(p.chunks);
+    sp.numChunks = (uint32_t)p.chunks;
     sp.ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
     return sp;
 }
@@ -350,7 +365,8 @@ RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool have
     return rp;
 }
 
-// Control-block reset, look-back status reset and the setup kernel.
+// Control-block reset, block-sum reset and the setup passes: counting (the
+// events bracket it), chunk scan, emitting.
 int enqueue_front(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, hipStream_t s,
                   hipEvent_t evSetup0 = nullptr, hipEvent_t evSetup1 = nullptr)
 {
@@ -360,11 +376,12 @@ int enqueue_front(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, hip
     if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(R.ranges.p), 0u, 2ull * p.numTiles, s), "fill ranges")) != SL_OK) return rc;
     if (B.totalTris == 0) return SL_OK;
     const SetupParams sp = setup_params(p, B, R);
-    if ((rc = check(c, launch_fill_u64(sp.status, 0ull, p.chunks, s), "fill status")) != SL_OK) return rc;
+    if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(sp.blockAgg), 0u, 3 * chunk_blocks(p.chunks), s), "fill block sums")) != SL_OK) return rc;
     if (evSetup0 && (rc = check(c, hipEventRecord(evSetup0, s), "event")) != SL_OK) return rc;
     if ((rc = check(c, launch_setup(sp, p.chunks, s), "k_setup")) != SL_OK) return rc;
     if (evSetup1 && (rc = check(c, hipEventRecord(evSetup1, s), "event")) != SL_OK) return rc;
-    return SL_OK;
+    if ((rc = check(c, launch_chunk_scan(sp, s), "k_chunk_scan")) != SL_OK) return rc;
+    return check(c, launch_emit(sp, s), "k_emit");
 }
 
 // Clip, binning and raster.  clipJobs / entries size the grids (a graph uses
@@ -457,12 +474,12 @@ int execute_batch(sl_context* c)
             break;
         }
         (void)hipEventRecord(c->ev[0], s);
-        if ((rc = enqueue_front(c, R, p, B, s)) != SL_OK) return rc;
+        if ((rc = enqueue_front(c, R, p, B, s, c->ev[8], c->ev[9])) != SL_OK) return rc;
         (void)hipEventRecord(c->ev[1], s);
         c->stats.setup_launches += 1;
         if ((rc = check(c, hipMemcpyAsync(c->hCtrl, ctrl, sizeof(FrameCtrl), hipMemcpyDeviceToHost, s), "read ctrl")) != SL_OK) return rc;
         if ((rc = check(c, hipStreamSynchronize(s), "k_setup sync")) != SL_OK) return rc;
-        if (c->hCtrl->error) return fail(c, SL_ERR_DEVICE, "setup look-back did not complete");
+        if (c->hCtrl->error) return fail(c, SL_ERR_DEVICE, "setup emitting pass disagrees with the counting pass");
         if (!c->hCtrl->overflow) break;
         // grow to the exact requirement (+25%) and run setup again; the
         // framebuffer has not been touched yet.
@@ -478,6 +495,7 @@ int execute_batch(sl_context* c)
         float t = 0.f;
         (void)hipEventElapsedTime(&t, c->ev[0], c->ev[1]);
         msVertex += t;
+        (void)hipEventElapsedTime(&t, c->ev[8], c->ev[9]);
         c->stats.ms_setup_kernel += t;
         totalRecs = c->hCtrl->totalRecs;
         totalPairs = c->hCtrl->totalPairs;
@@ -634,7 +652,7 @@ int graph_collect(sl_context* c)
         }
     }
     G.pending = 0;
-    if (G.hCtrl->error) return fail(c, SL_ERR_DEVICE, "graph frame: setup look-back did not complete");
+    if (G.hCtrl->error) return fail(c, SL_ERR_DEVICE, "graph frame: setup emitting pass disagrees with the counting pass");
     if (G.hCtrl->overflow) return fail(c, SL_ERR_CAPACITY, "graph frame exceeded its captured capacity; capture again");
     return SL_OK;
 }

@@ -133,58 +133,99 @@ __device__ __forceinline__ uint64_t fetch_index(const DrawDesc& d, uint64_t e)
     }
 }
 
-// Clip-space position of a vertex: the position part of vertex_program below,
-// with the same operations in the same order (the programs are pure, so a
-// triangle can be culled and binned before its varyings are computed).
-__device__ __forceinline__ float4 vertex_position(const DrawDesc& d, const uint8_t* uniArena, uint64_t vertId, uint64_t inst)
+// The built-in vertex programs (replace SL_VertexShader::shader,
+// SL_Shader.hpp:58-64), split into the clip-space position and the varyings.
+// The programs are pure, so a triangle can be culled and binned from its
+// positions before its varyings are computed; each part performs the same
+// operations in the same order as the whole program would.  Both work on a
+// triangle's three vertices at once and issue every fetch before any
+// arithmetic, so the fetch latencies overlap.
+//
+//   PASSTHROUGH_GOURAUD  position = attribute 0 (w defaults to 1), varying 0 =
+//                        attribute 1 (w defaults to 1)
+//   LAMBERT_TEXTURED     position = mvp * (model * (p.xyz, 1)); varyings
+//                        (uv, 0, 0) and (model * (n.xyz, 0)).xyz
+//   COLOR_TEXTURED       position as above; varyings (c.xyz, 1) and (uv, 0, 0)
+//
+// model = the instance's mat4 when an instance buffer is bound, else the
+// uniform model matrix.  need[k] == false skips vertex k.
+__device__ __forceinline__ void vertex_positions3(const DrawDesc& d, const uint8_t* uniArena, const uint32_t vid[3],
+                                                  uint64_t inst, const bool need[3], float4 out[3])
 {
-    if (d.vsProg == SL_PROGRAM_PASSTHROUGH_GOURAUD) return read_attr(d, 0, vertId, 1.f);
-    if (d.vsProg == SL_PROGRAM_LAMBERT_TEXTURED || d.vsProg == SL_PROGRAM_COLOR_TEXTURED)
-    {
-        const sl_std_uniforms* u = reinterpret_cast<const sl_std_uniforms*>(uniArena + d.uniOffset);
-        const float4 p = read_attr(d, 0, vertId, 1.f);
-        float m[16];
-        model_matrix(d, u, inst, m);
-        const float4 world = mat_mul(m, f4(p.x, p.y, p.z, 1.f));
-        return mat_mul(u->mvp, world);
-    }
-    return f4(0.f, 0.f, 0.f, 0.f);
-}
-
-// Built-in vertex programs (replace SL_VertexShader::shader, SL_Shader.hpp:58-64)
-__device__ __forceinline__ float4 vertex_program(const DrawDesc& d, const uint8_t* uniArena, uint64_t vertId,
-                                                 uint64_t inst, Vary4& out)
-{
-#pragma unroll
-    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i) out.v[i] = f4(0.f, 0.f, 0.f, 0.f);
     if (d.vsProg == SL_PROGRAM_PASSTHROUGH_GOURAUD)
     {
-        out.v[0] = read_attr(d, 1, vertId, 1.f);
-        return read_attr(d, 0, vertId, 1.f);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (need[k]) out[k] = read_attr(d, 0, vid[k], 1.f);
+        return;
     }
     if (d.vsProg == SL_PROGRAM_LAMBERT_TEXTURED || d.vsProg == SL_PROGRAM_COLOR_TEXTURED)
     {
         const sl_std_uniforms* u = reinterpret_cast<const sl_std_uniforms*>(uniArena + d.uniOffset);
-        const float4 p = read_attr(d, 0, vertId, 1.f);
-        const float4 a = read_attr(d, 1, vertId, 0.f);
-        const float4 uv = read_attr(d, 2, vertId, 0.f);
+        float4 p[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) p[k] = need[k] ? read_attr(d, 0, vid[k], 1.f) : f4(0.f, 0.f, 0.f, 1.f);
         float m[16];
         model_matrix(d, u, inst, m);
-        const float4 world = mat_mul(m, f4(p.x, p.y, p.z, 1.f));
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            if (!need[k]) continue;
+            const float4 world = mat_mul(m, f4(p[k].x, p[k].y, p[k].z, 1.f));
+            out[k] = mat_mul(u->mvp, world);
+        }
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (need[k]) out[k] = f4(0.f, 0.f, 0.f, 0.f);
+}
+
+__device__ __forceinline__ void vertex_varyings3(const DrawDesc& d, const uint8_t* uniArena, const uint32_t vid[3],
+                                                 uint64_t inst, Vary4 out[3])
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i) out[k].v[i] = f4(0.f, 0.f, 0.f, 0.f);
+    if (d.vsProg == SL_PROGRAM_PASSTHROUGH_GOURAUD)
+    {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) out[k].v[0] = read_attr(d, 1, vid[k], 1.f);
+        return;
+    }
+    if (d.vsProg == SL_PROGRAM_LAMBERT_TEXTURED || d.vsProg == SL_PROGRAM_COLOR_TEXTURED)
+    {
+        float4 a[3], uv[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            a[k] = read_attr(d, 1, vid[k], 0.f);
+            uv[k] = read_attr(d, 2, vid[k], 0.f);
+        }
         if (d.vsProg == SL_PROGRAM_LAMBERT_TEXTURED)
         {
-            const float4 n = mat_mul(m, f4(a.x, a.y, a.z, 0.f));
-            out.v[0] = f4(uv.x, uv.y, 0.f, 0.f);
-            out.v[1] = f4(n.x, n.y, n.z, 0.f);
+            const sl_std_uniforms* u = reinterpret_cast<const sl_std_uniforms*>(uniArena + d.uniOffset);
+            float m[16];
+            model_matrix(d, u, inst, m);
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+            {
+                const float4 n = mat_mul(m, f4(a[k].x, a[k].y, a[k].z, 0.f));
+                out[k].v[0] = f4(uv[k].x, uv[k].y, 0.f, 0.f);
+                out[k].v[1] = f4(n.x, n.y, n.z, 0.f);
+            }
         }
         else
         {
-            out.v[0] = f4(a.x, a.y, a.z, 1.f);
-            out.v[1] = f4(uv.x, uv.y, 0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+            {
+                out[k].v[0] = f4(a[k].x, a[k].y, a[k].z, 1.f);
+                out[k].v[1] = f4(uv[k].x, uv[k].y, 0.f, 0.f);
+            }
         }
-        return mat_mul(u->mvp, world);
     }
-    return f4(0.f, 0.f, 0.f, 0.f);
 }
 
 // ---------------------------------------------------------------------------
