@@ -1372,11 +1372,12 @@ unsigned slo_set_num_threads(sl_context* c, unsigned n)
     return c->numThreads;
 }
 
-// The oracle has no tiles: the raster configuration is accepted and ignored.
+// The oracle has no tiles: any raster configuration index is accepted and
+// ignored.
 int slo_set_raster_config(sl_context* c, int config)
 {
     if (!c) return SL_ERR_INVALID_ARG;
-    return (config >= 0 && config < 8) ? SL_OK : SL_ERR_INVALID_ARG;
+    return config >= 0 ? SL_OK : SL_ERR_INVALID_ARG;
 }
 
 int slo_get_raster_config(const sl_context* c, int config, int* tileW, int* tileH, int* segments)

@@ -1,7 +1,8 @@
 """Raster tile-configuration autotuner.
 
-Times each of the library's raster tile configurations (sl_get_raster_config
-0..7: tile size and column segments per row) on a caller-supplied frame and
+Times each of the library's raster tile configurations (every index that
+sl_get_raster_config accepts: tile size and column segments per row) on a
+caller-supplied frame and
 keeps the fastest.  Results are cached in a JSON file keyed by the device
 name and a workload signature, so later runs reuse them without timing.
 The choice never changes results, only speed (the parity tests run every
@@ -14,7 +15,14 @@ import json
 import os
 
 DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_configs.json")
-NUM_CONFIGS = 8
+
+
+def num_configs(ctx) -> int:
+    """Number of raster configurations the library offers (indices 0..n-1)."""
+    n = 0
+    while n < 64 and ctx.raster_config(n)[0] == n:
+        n += 1
+    return n
 
 
 def device_name() -> str:
@@ -49,7 +57,7 @@ def tune(ctx, render_frame, signature: str, frames: int = 3, path: str | None = 
     """Time render_frame() under each raster configuration; return the
     per-configuration raster milliseconds and the best index (also cached)."""
     timings = {}
-    for cfg in range(NUM_CONFIGS):
+    for cfg in range(num_configs(ctx)):
         ctx.set_raster_config(cfg)
         render_frame()                     # warm-up (buffer growth, code load)
         ctx.reset_stats()

@@ -26,7 +26,7 @@ constexpr int kMaxFan = 7;             // a 9-gon from 6-plane clipping fans int
 // x = xMin + phase + 4k of its row that fall in its segment of tileW/segs
 // columns (the reference steps its barycentrics in groups of 4 pixels).
 struct TileConfig { int tileW, tileH, segs; };
-constexpr int kNumTileConfigs = 8;
+constexpr int kNumTileConfigs = 10;
 constexpr TileConfig kTileConfigs[kNumTileConfigs] = {
     {32, 16, 1},   // 0: 1 wave
     {32, 8, 2},    // 1: 1 wave
@@ -36,6 +36,8 @@ constexpr TileConfig kTileConfigs[kNumTileConfigs] = {
     {16, 16, 4},   // 5: 4 waves x 4 rows
     {32, 16, 4},   // 6: 4 waves x 4 rows
     {32, 32, 2},   // 7: 4 waves x 8 rows
+    {32, 8, 4},    // 8: 2 waves x 4 rows
+    {32, 8, 8},    // 9: 4 waves x 2 rows
 };
 constexpr int kDefaultTileConfig = 0;
 
@@ -114,7 +116,7 @@ struct SetupRec
     uint32_t draw;
     uint32_t seq;
     uint32_t state;            // the draw's raster state, see pack_raster_state
-    uint32_t pad;
+    uint32_t xbox;             // columns its spans can reach: min x - 1 | (max x - 1) << 16, clamped to 0..65535
 };
 static_assert(sizeof(SetupRec) == 144, "SetupRec layout");
 

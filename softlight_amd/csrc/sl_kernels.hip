@@ -1171,7 +1171,6 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     const int x0 = tx * TW, y0 = ty * TH;
     const int lane = threadIdx.x;
     const int wave = lane >> 6, wl = lane & 63;
-    const int rowLo = y0 + wave * (TH / NW), rowHi = rowLo + TH / NW;
 
     // tile init: fused clear or load
     for (int k = lane; k < TW * TH; k += NT)
@@ -1209,10 +1208,13 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
         if (VIS) sVis[r][c] = make_float4(__uint_as_float(Q.seq), wv.x, wv.y, wv.z);
     };
 
+    // lane = (row, segment, phase): the lane owns the columns x = phase (mod 4)
+    // of its row's segment, whichever record it works on
     const int row = lane / (4 * SEGS);
     const int phase = lane & 3;
     const int seg = (lane >> 2) % SEGS;
     const int sx0 = x0 + seg * SW;
+    const int sxLast = min(sx0 + SW, P.W) - 1;
     const int y = y0 + row;
     const float yf = (float)y;
     const uint2 range = P.tileRange[tile];
@@ -1268,20 +1270,35 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
             fetch_words(base + kBatch, idsNext, vNext);
             idsAfter = fetch_ids(base + 2 * kBatch);
         }
-    for (int bi = 0; bi < nb; ++bi)
-    {
-        const SetupRec& R = sRec[bi];
-        if (R.maxY <= rowLo || R.minY >= rowHi) continue;    // misses this wave's rows
-        const uint32_t rid = __builtin_amdgcn_readfirstlane(sRid[bi]);
-        const uint32_t drawIdx = __builtin_amdgcn_readfirstlane(R.draw);
-        if (y >= R.minY && y < R.maxY && y < P.H)
+        // the batch's records whose rows and column box reach one of this
+        // lane's pixels, as a mask in record order
+        uint32_t cand = 0;
+        if (y < P.H)
         {
+            for (int bi = 0; bi < nb; ++bi)
+            {
+                const SetupRec& R = sRec[bi];
+                const int xlo = max((int)(R.xbox & 0xFFFFu), sx0), xhi = min((int)(R.xbox >> 16), sxLast);
+                const bool hit = y >= R.minY && y < R.maxY && xlo + ((phase - xlo) & 3) <= xhi;
+                cand |= (uint32_t)hit << bi;
+            }
+        }
+        // Each lane walks its own candidates in record order.  A pixel is only
+        // ever touched by the lane that owns it, so lanes may be at different
+        // records at the same time and every pixel still sees its fragments
+        // in submission order.
+        while (cand != 0u)
+        {
+            const int bi = __ffs(cand) - 1;
+            cand &= cand - 1u;
+            const SetupRec& R = sRec[bi];
+            const uint32_t rid = sRid[bi];
             int32_t xMin, xMax;
             scanline_step(R, yf, xMin, xMax);
             if ((uint32_t)xMin < (uint32_t)xMax)
             {
                 const int xs = max(xMin, sx0), xe = min(min(xMax, sx0 + SW), P.W);
-                const int xf = xMin + phase;
+                const int xf = xMin + ((phase - xMin) & 3);   // first span pixel of this lane's column class
                 const int g0 = (xs > xf) ? ((xs - xf + 3) >> 2) : 0;
                 int x = xf + 4 * g0;
                 if (x < xe)
@@ -1348,7 +1365,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                             {
                                 // a blended fragment reads the colour: settle the pixel first
                                 resolve(row, c);
-                                const DrawDesc& D = P.draws[drawIdx];
+                                const DrawDesc& D = P.draws[R.draw];
                                 const float hw[3] = {R.hw[0], R.hw[1], R.hw[2]};
                                 float4 out, wv;
                                 const bool have = shade(P, D, hw, R.bc3, rid, s0, s1, s2, out, wv);
@@ -1371,8 +1388,6 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                 }
             }
         }
-        record_barrier<64>();
-    }
         record_barrier<64>();
         idsCur = idsNext;
         idsNext = idsAfter;
@@ -1591,6 +1606,8 @@ static void launch_raster_cfg(const RasterParams& p, int numTiles, int cfg, hipS
         case 5: launch_raster_shape<CF, AUX, 16, 16, 4>(p, numTiles, s); break;
         case 6: launch_raster_shape<CF, AUX, 32, 16, 4>(p, numTiles, s); break;
         case 7: launch_raster_shape<CF, AUX, 32, 32, 2>(p, numTiles, s); break;
+        case 8: launch_raster_shape<CF, AUX, 32, 8, 4>(p, numTiles, s); break;
+        case 9: launch_raster_shape<CF, AUX, 32, 8, 8>(p, numTiles, s); break;
         default: launch_raster_shape<CF, AUX, 32, 16, 1>(p, numTiles, s); break;
     }
 }
@@ -1601,7 +1618,9 @@ static_assert(kTileConfigs[0].tileW == 32 && kTileConfigs[0].tileH == 16 && kTil
               kTileConfigs[4].tileW == 16 && kTileConfigs[4].tileH == 8 && kTileConfigs[4].segs == 4 &&
               kTileConfigs[5].tileW == 16 && kTileConfigs[5].tileH == 16 && kTileConfigs[5].segs == 4 &&
               kTileConfigs[6].tileW == 32 && kTileConfigs[6].tileH == 16 && kTileConfigs[6].segs == 4 &&
-              kTileConfigs[7].tileW == 32 && kTileConfigs[7].tileH == 32 && kTileConfigs[7].segs == 2,
+              kTileConfigs[7].tileW == 32 && kTileConfigs[7].tileH == 32 && kTileConfigs[7].segs == 2 &&
+              kTileConfigs[8].tileW == 32 && kTileConfigs[8].tileH == 8 && kTileConfigs[8].segs == 4 &&
+              kTileConfigs[9].tileW == 32 && kTileConfigs[9].tileH == 8 && kTileConfigs[9].segs == 8 && kNumTileConfigs == 10,
               "launch table and tile configurations disagree");
 
 hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s)

@@ -340,7 +340,13 @@ __device__ __forceinline__ void setup_record(float4 p0, float4 p1, float4 p2, ui
     r.draw = draw;
     r.seq = seq;
     r.state = 0;
-    r.pad = 0;
+    // Vertices are snapped to whole non-negative pixels, so every span end
+    // (a truncated point of an edge between two vertices) lies in
+    // [min x - 1, max x]: the span's pixels lie in [min x - 1, max x - 1].
+    {
+        const int xa = f2i(fminf(fminf(p0.x, p1.x), p2.x)) - 1, xb = f2i(fmaxf(fmaxf(p0.x, p1.x), p2.x)) - 1;
+        r.xbox = (uint32_t)min(max(xa, 0), 65535) | ((uint32_t)min(max(xb, 0), 65535) << 16);
+    }
 
     // SL_ScanlineBounds::init: sort by y descending with the reference's swaps
     float ax = p0.x, ay = p0.y, bxx = p1.x, byy = p1.y, cx = p2.x, cy = p2.y, t;

@@ -34,7 +34,7 @@ def test_spheres_1080p(gpu_lib, oracle_lib):
     assert rep["triangles_submitted"][0] == 64 * 130560
 
 
-@pytest.mark.parametrize("config", range(8))
+@pytest.mark.parametrize("config", range(10))
 def test_every_raster_configuration(gpu_lib, oracle_lib, config):
     """Results do not depend on the tile configuration."""
     from softlight_amd import api
