@@ -167,9 +167,20 @@ struct FrameCtrl
     uint32_t error;            // the emitting pass found a survivor that yields no record
     uint32_t emptyRecs;        // records that passed push_bin's tests but cover no pixel (not binned)
     uint32_t pad;
-    unsigned long long tested; // fragments depth-tested
-    unsigned long long shaded; // fragments that passed and were shaded
+    // Fragment counters, spread over kCounterSlots cache lines (a workgroup
+    // adds to slot blockIdx % kCounterSlots): tens of thousands of atomics on
+    // one address serialise, on 64 lines they do not.  Readers sum the slots.
+    struct CounterSlot
+    {
+        unsigned long long tested;   // fragments depth-tested
+        unsigned long long shaded;   // fragments that passed and were shaded
+        unsigned long long pad[6];
+    } counters[64];
+    SL_HD uint64_t tested() const { uint64_t t = 0; for (const CounterSlot& c : counters) t += c.tested; return t; }
+    SL_HD uint64_t shaded() const { uint64_t t = 0; for (const CounterSlot& c : counters) t += c.shaded; return t; }
 };
+constexpr int kCounterSlots = 64;
+static_assert(sizeof(FrameCtrl::CounterSlot) == 64, "one cache line per counter slot");
 
 struct SetupParams
 {

@@ -173,23 +173,40 @@ __device__ __forceinline__ int clip_quick(const float4* p)
     return 1;
 }
 
-// SL_TriProcessor::clip_and_process_tris (SL_TriProcessor.cpp:370-499) on
-// positions and (optionally) varyings.  Returns the polygon size (0 if it was
-// clipped away or has fewer than 3 vertices); vertices are left in clip space.
-template <bool kVary>
-__device__ int clip_polygon(const float4* inPos, const Vary4* inVar, int nv, float4* outPos, Vary4* outVar)
+// Polygon buffers of the clipper: entry k of a buffer lives at p[k * stride],
+// so a thread's polygon can sit in LDS (interleaved with other threads'
+// polygons, or in a wave's shared slot) instead of in scratch memory, whose
+// latency every step of the clipper would otherwise pay.
+template <class T>
+struct PolyBuf
 {
-    float4 tpos[9];
-    Vary4 tvar[kVary ? 9 : 1];
+    T* p;
+    int stride;
+    __device__ __forceinline__ T& operator[](int k) const { return p[k * stride]; }
+};
+
+// SL_TriProcessor::clip_and_process_tris (SL_TriProcessor.cpp:370-499) on
+// positions and (kVary) varyings.  Returns the polygon size (0 if it was
+// clipped away or has fewer than 3 vertices); the polygon is left in
+// outPos / outVar in clip space.  tpos / tvar are work buffers.  All buffers
+// hold 9 entries.
+template <bool kVary>
+__device__ int clip_polygon(const float4* inPos, const Vary4* inVar, int nv, PolyBuf<float4> outPos, PolyBuf<Vary4> outVar,
+                            PolyBuf<float4> tpos, PolyBuf<Vary4> tvar)
+{
     int n = 3;
     for (int k = 0; k < 3; ++k)
     {
         outPos[k] = inPos[k];
-        if (kVary) outVar[k] = inVar[k];
+        if constexpr (kVary) outVar[k] = inVar[k];
     }
     for (int e = 0; e < 6; ++e)
     {
         const float4 E = clip_edge(e);
+        // a plane that keeps every vertex leaves the polygon as it is
+        bool allIn = true;
+        for (int k = 0; k < n; ++k) allIn = allIn && !signbit(dot4(outPos[k], E));
+        if (allIn) continue;
         int nn = 0;
         int j = n - 1;
         float4 q0 = outPos[n - 1];
@@ -204,7 +221,7 @@ __device__ int clip_polygon(const float4* inPos, const Vary4* inVar, int nv, flo
             {
                 const float t = t0 * rcpf(t0 - t1);
                 tpos[nn] = mix4(q0, q1, t);
-                if (kVary)
+                if constexpr (kVary)
                 {
 #pragma unroll
                     for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
@@ -215,7 +232,7 @@ __device__ int clip_polygon(const float4* inPos, const Vary4* inVar, int nv, flo
             if (vis1)
             {
                 tpos[nn] = q1;
-                if (kVary) tvar[nn] = outVar[k];
+                if constexpr (kVary) tvar[nn] = outVar[k];
                 ++nn;
             }
             j = k; q0 = q1; t0 = t1; vis0 = vis1;
@@ -225,7 +242,7 @@ __device__ int clip_polygon(const float4* inPos, const Vary4* inVar, int nv, flo
         for (int k = 0; k < n; ++k)
         {
             outPos[k] = tpos[k];
-            if (kVary) outVar[k] = tvar[k];
+            if constexpr (kVary) outVar[k] = tvar[k];
         }
     }
     return n < 3 ? 0 : n;
@@ -332,6 +349,7 @@ struct TriCount
     uint32_t nrec, npair;
     bool clipped;              // needs k_clip (nrec fan triangles)
     bool empty;                // small triangle that covers no pixel: dropped, counted as binned
+    bool needClip;             // straddles a clip plane: counted by count_clipped
 };
 
 __device__ __forceinline__ void count_triangle(const SetupParams& P, const DrawDesc& D, uint32_t drawIdx, const TriIn& T,
@@ -341,6 +359,7 @@ __device__ __forceinline__ void count_triangle(const SetupParams& P, const DrawD
     C.npair = 0;
     C.clipped = false;
     C.empty = false;
+    C.needClip = false;
     C.tr = TileRect{0, -1, 0, -1};
     if (T.status == 3)
     {
@@ -376,8 +395,27 @@ __device__ __forceinline__ void count_triangle(const SetupParams& P, const DrawD
     }
     else if (T.status == 1 && clip_quick(T.pos))
     {
-        float4 poly[9];
-        const int n = clip_polygon<false>(T.pos, nullptr, 0, poly, nullptr);
+        C.needClip = true;   // counted by count_clipped
+    }
+}
+
+// Records and bin entries of the calling wave's straddling triangles
+// (C.needClip): position-only clipping, up to kClipSlots lanes at a time, in
+// the wave's LDS slots (kClipSlots x 18 float4, interleaved).  Call with the
+// wave converged (inactive lanes may be masked off); D is the lane's draw.
+constexpr int kClipSlots = 4;
+
+__device__ __forceinline__ void count_clipped(const SetupParams& P, const DrawDesc& D, const TriIn& T, TriCount& C,
+                                              float4 (*slots)[kClipSlots])
+{
+    const int lane = threadIdx.x & 63;
+    for (unsigned long long m = __ballot(C.needClip); m != 0ull; m = __ballot(C.needClip))
+    {
+        const int rank = (int)__popcll(m & ((1ull << lane) - 1ull));
+        if (!C.needClip || rank >= kClipSlots) continue;
+        const PolyBuf<float4> poly{&slots[0][rank], kClipSlots}, tmp{&slots[9][rank], kClipSlots};
+        const PolyBuf<Vary4> none{nullptr, 0};
+        const int n = clip_polygon<false>(T.pos, nullptr, 0, poly, none, tmp, none);
         for (int k = 0; k < n; ++k) poly[k] = divide_and_snap(poly[k], D.vp);
         for (int i = n - 3; i >= 0; --i)
         {
@@ -389,6 +427,7 @@ __device__ __forceinline__ void count_triangle(const SetupParams& P, const DrawD
             }
         }
         C.clipped = C.nrec > 0;
+        C.needClip = false;
     }
 }
 
@@ -489,11 +528,14 @@ __global__ __launch_bounds__(kSetupBlock, 8) void k_setup(SetupParams P)
     C.nrec = 0;
     C.npair = 0;
     C.empty = false;
+    C.needClip = false;
     if (live)
     {
         classify_triangle(DU, T);
         count_triangle(P, DU, drawIdx, T, C);
     }
+    __shared__ float4 sClip[kSetupBlock / 64][18][kClipSlots];   // each wave's clipper slots
+    count_clipped(P, DU, T, C, sClip[wave]);
 
     // chunk summary: exclusive scan of the packed counts (each survivor's
     // offset within the chunk), survivor ranks, staging words, totals
@@ -644,7 +686,9 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
 {
     const int t = threadIdx.x;
     const int lane = t & 63;
+    const int wave = t >> 6;
     __shared__ uint32_t sSP[kEmitBlock];
+    __shared__ float4 sClip[kEmitBlock / 64][18][kClipSlots];   // each wave's clipper slots
     const uint32_t S = P.ctrl->survivors;
     for (uint32_t k = blockIdx.x; (uint64_t)k * kEmitBlock < S; k += gridDim.x)
     {
@@ -700,6 +744,7 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
         C.nrec = 0;
         C.npair = 0;
         C.clipped = false;
+        C.needClip = false;
         uint64_t g = 0;
         uint32_t di = 0;
         unsigned long long off = 0;
@@ -741,27 +786,41 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
                 load_triangle_pos(P, g, DR, di, T);
                 count_triangle(P, DR, di, T, C);
             }
+            count_clipped(P, DR, T, C, sClip[wave]);
         }
         else if (act2)
         {
             const DrawDesc& D = P.draws[di];
             load_triangle_pos(P, g, D, di, T);
             count_triangle(P, D, di, T, C);
+            count_clipped(P, D, T, C, sClip[wave]);
         }
-        if (!act2) continue;
-        if (C.nrec == 0)
+        bool ok = act2;
+        if (ok && C.nrec == 0)
         {
             atomicExch(&P.ctrl->error, 1u);
-            continue;
+            ok = false;
         }
         const uint32_t recOff = (uint32_t)(off & ((1ull << kRecBits) - 1));
         const uint64_t pairOff = off >> kRecBits;
-        if ((uint64_t)recOff + C.nrec > P.recCap || pairOff + C.npair > P.pairCap)
+        if (ok && ((uint64_t)recOff + C.nrec > P.recCap || pairOff + C.npair > P.pairCap))
         {
             atomicExch(&P.ctrl->overflow, 1u);
-            continue;
+            ok = false;
         }
-        if (!C.clipped)
+        // clip jobs: one reservation per wave
+        const bool wantJob = ok && C.clipped;
+        const unsigned long long jobs = __ballot(wantJob);
+        uint32_t job = 0;
+        if (jobs != 0ull)
+        {
+            const int leader = __ffsll((long long)jobs) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&P.ctrl->clipJobs, (uint32_t)__popcll(jobs));
+            base = __shfl(base, leader, 64);
+            job = base + (uint32_t)__popcll(jobs & ((1ull << lane) - 1ull));
+        }
+        if (ok && !C.clipped)
         {
             if (uni)
             {
@@ -777,9 +836,8 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
                              T.draw, T.seq, pack_raster_state(D), C.tr);
             }
         }
-        else
+        else if (wantJob)
         {
-            const uint32_t job = atomicAdd(&P.ctrl->clipJobs, 1u);
             if (job < P.clipCap)
             {
                 ClipJob J;
@@ -800,8 +858,15 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
 // ---------------------------------------------------------------------------
 // k_clip: finish the clipped triangles into their reserved slots
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_clip(SetupParams P)
+constexpr int kClipBlock = 64;
+
+__global__ __launch_bounds__(kClipBlock) void k_clip(SetupParams P)
 {
+    // each thread's polygon buffers, interleaved across the threads in LDS
+    __shared__ float4 sPoly[9][kClipBlock], sTmp[9][kClipBlock];
+    __shared__ Vary4 sPVar[9][kClipBlock], sTVar[9][kClipBlock];
+    const PolyBuf<float4> poly{&sPoly[0][threadIdx.x], kClipBlock}, tpos{&sTmp[0][threadIdx.x], kClipBlock};
+    const PolyBuf<Vary4> pvar{&sPVar[0][threadIdx.x], kClipBlock}, tvar{&sTVar[0][threadIdx.x], kClipBlock};
     const uint32_t njobs = min(P.ctrl->clipJobs, P.clipCap);
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < njobs; j += gridDim.x * blockDim.x)
     {
@@ -810,9 +875,7 @@ __global__ __launch_bounds__(64) void k_clip(SetupParams P)
         load_triangle(P, J.tri, T);
         const DrawDesc& D = P.draws[T.draw];
         const int nv = D.nvary;
-        float4 poly[9];
-        Vary4 pvar[9];
-        const int n = clip_polygon<true>(T.pos, T.var, nv, poly, pvar);
+        const int n = clip_polygon<true>(T.pos, T.var, nv, poly, pvar, tpos, tvar);
         for (int k = 0; k < n; ++k) poly[k] = divide_and_snap(poly[k], D.vp);
         uint32_t rec = J.recOff, pair = J.pairOff;
         const uint32_t recEnd = J.recOff + J.nrec;
@@ -1427,8 +1490,9 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     }
     if ((lane & 63) == 0)
     {
-        atomicAdd(&P.ctrl->tested, t);
-        atomicAdd(&P.ctrl->shaded, s);
+        FrameCtrl::CounterSlot& slot = P.ctrl->counters[blockIdx.x % kCounterSlots];
+        atomicAdd(&slot.tested, t);
+        atomicAdd(&slot.shaded, s);
     }
 }
 
@@ -1536,7 +1600,7 @@ hipError_t launch_clip(const SetupParams& p, uint32_t jobs, hipStream_t s)
 {
     (void)hipGetLastError();   // drop a stale error from an earlier call
     if (jobs == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_clip, dim3(grid_for(jobs, 64, 8192)), dim3(64), 0, s, p);
+    hipLaunchKernelGGL(k_clip, dim3(grid_for(jobs, kClipBlock, 8192)), dim3(kClipBlock), 0, s, p);
     return hipGetLastError();
 }
 

@@ -164,6 +164,31 @@ int check(sl_context* c, hipError_t e, const char* what)
     return SL_ERR_DEVICE;
 }
 
+// Transfers and zero fills of resources, ordered on the context's stream (a
+// non-blocking stream does not wait for the null stream, so a plain hipMemset
+// could still be running when the next kernel of the stream starts) and
+// complete when they return.
+int upload(sl_context* c, void* dst, const void* src, size_t n, const char* what)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return check(c, e, what);
+}
+
+int download(sl_context* c, void* dst, const void* src, size_t n, const char* what)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return check(c, e, what);
+}
+
+int zero_fill(sl_context* c, void* dst, size_t n, const char* what)
+{
+    hipError_t e = hipMemsetAsync(dst, 0, n, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return check(c, e, what);
+}
+
 int ensure(sl_context* c, DevBuf& b, size_t bytes)
 {
     if (bytes <= b.bytes) return SL_OK;
@@ -524,8 +549,8 @@ int execute_batch(sl_context* c)
     msRaster += t;
     c->stats.triangles_binned += (uint64_t)totalRecs + c->hCtrl->emptyRecs;
     c->stats.tile_pairs += totalPairs;
-    c->stats.fragments_tested += c->hCtrl->tested;
-    c->stats.fragments_shaded += c->hCtrl->shaded;
+    c->stats.fragments_tested += c->hCtrl->tested();
+    c->stats.fragments_shaded += c->hCtrl->shaded();
     c->stats.frames += 1;
     c->stats.raster_launches += 1;
     c->stats.ms_vertex += msVertex;
@@ -637,8 +662,8 @@ int graph_collect(sl_context* c)
     c->stats.triangles_submitted += G.tris * G.pending;
     c->stats.triangles_binned += ((uint64_t)G.hCtrl->totalRecs + G.hCtrl->emptyRecs) * G.pending;
     c->stats.tile_pairs += (uint64_t)G.hCtrl->totalPairs * G.pending;
-    c->stats.fragments_tested += G.hCtrl->tested * G.pending;
-    c->stats.fragments_shaded += G.hCtrl->shaded * G.pending;
+    c->stats.fragments_tested += G.hCtrl->tested() * G.pending;
+    c->stats.fragments_shaded += G.hCtrl->shaded() * G.pending;
     if (G.timed)
     {
         // every replay runs the same work: the last replay's kernel times
@@ -817,9 +842,7 @@ int fill_texture(sl_context* c, Tex& t, const void* value, uint32_t bpt)
             return check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(t.buf.p), v[0], 4 * n, c->stream), "fill texture");
         std::vector<uint8_t> host(n * 16);
         for (size_t i = 0; i < n; ++i) std::memcpy(host.data() + 16 * i, value, 16);
-        hipError_t e = hipStreamSynchronize(c->stream);
-        if (e == hipSuccess) e = hipMemcpy(t.buf.p, host.data(), host.size(), hipMemcpyHostToDevice);
-        return check(c, e, "fill texture");
+        return upload(c, t.buf.p, host.data(), host.size(), "fill texture");
     }
     return fail(c, SL_ERR_UNSUPPORTED, "clear of this texture format");
 }
@@ -989,8 +1012,8 @@ int sl_vbo_init(sl_context* c, size_t vbo, size_t numBytes, const void* data)
     if ((rc = ensure(c, v.buf, numBytes)) != SL_OK) return rc;
     v.bytes = numBytes;
     v.valid = true;
-    if (data) return check(c, hipMemcpy(v.buf.p, data, numBytes, hipMemcpyHostToDevice), "vbo upload");
-    return check(c, hipMemset(v.buf.p, 0, numBytes), "vbo zero");
+    if (data) return upload(c, v.buf.p, data, numBytes, "vbo upload");
+    return zero_fill(c, v.buf.p, numBytes, "vbo zero");
 }
 
 int sl_vbo_assign(sl_context* c, size_t vbo, const void* data, size_t offset, size_t numBytes)
@@ -1002,7 +1025,7 @@ int sl_vbo_assign(sl_context* c, size_t vbo, const void* data, size_t offset, si
     if (rc != SL_OK) return rc;
     // queued graph replays and backward passes may still read the buffer
     if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
-    return check(c, hipMemcpy(reinterpret_cast<uint8_t*>(v.buf.p) + offset, data, numBytes, hipMemcpyHostToDevice), "vbo assign");
+    return upload(c, reinterpret_cast<uint8_t*>(v.buf.p) + offset, data, numBytes, "vbo assign");
 }
 
 int sl_vbo_read(sl_context* c, size_t vbo, void* out, size_t offset, size_t numBytes)
@@ -1014,7 +1037,7 @@ int sl_vbo_read(sl_context* c, size_t vbo, void* out, size_t offset, size_t numB
     if (rc != SL_OK) return rc;
     if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     collect_backward(c);
-    return check(c, hipMemcpy(out, reinterpret_cast<const uint8_t*>(v.buf.p) + offset, numBytes, hipMemcpyDeviceToHost), "vbo read");
+    return download(c, out, reinterpret_cast<const uint8_t*>(v.buf.p) + offset, numBytes, "vbo read");
 }
 
 size_t sl_create_ibo(sl_context* c) { c->ibos.emplace_back(); return c->ibos.size() - 1; }
@@ -1034,8 +1057,8 @@ int sl_ibo_init(sl_context* c, size_t ibo, uint32_t numElements, int type, const
     release(b.buf);
     const size_t bytes = (size_t)numElements * bpe;
     if ((rc = ensure(c, b.buf, bytes + 16)) != SL_OK) return rc;   // padded like SL_IndexBuffer.cpp:155
-    if ((rc = check(c, hipMemset(b.buf.p, 0, bytes + 16), "ibo zero")) != SL_OK) return rc;
-    if (data && (rc = check(c, hipMemcpy(b.buf.p, data, bytes, hipMemcpyHostToDevice), "ibo upload")) != SL_OK) return rc;
+    if ((rc = zero_fill(c, b.buf.p, bytes + 16, "ibo zero")) != SL_OK) return rc;
+    if (data && (rc = upload(c, b.buf.p, data, bytes, "ibo upload")) != SL_OK) return rc;
     b.type = type;
     b.bpe = bpe;
     b.count = numElements;
@@ -1102,7 +1125,7 @@ int sl_texture_init(sl_context* c, size_t tex, int type, uint16_t w, uint16_t h,
     const size_t pw = (size_t)w + (4 - w % 4), ph = (size_t)h + (4 - h % 4), pd = (size_t)d + (4 - d % 4);
     const size_t bytes = pw * ph * pd * bpt + 4 * bpt;
     if ((rc = ensure(c, t.buf, bytes)) != SL_OK) return rc;
-    if ((rc = check(c, hipMemset(t.buf.p, 0, bytes), "texture zero")) != SL_OK) return rc;
+    if ((rc = zero_fill(c, t.buf.p, bytes, "texture zero")) != SL_OK) return rc;
     t.type = type; t.w = w; t.h = h; t.d = d; t.bpt = bpt;
     t.valid = true;
     return SL_OK;
@@ -1118,7 +1141,7 @@ int sl_texture_set_texels(sl_context* c, size_t tex, const void* data, size_t nu
     int rc = flush(c);
     if (rc != SL_OK) return rc;
     if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
-    return check(c, hipMemcpy(t->buf.p, data, n, hipMemcpyHostToDevice), "texture upload");
+    return upload(c, t->buf.p, data, n, "texture upload");
 }
 
 int sl_texture_read(sl_context* c, size_t tex, void* out, size_t numBytes)
@@ -1132,7 +1155,7 @@ int sl_texture_read(sl_context* c, size_t tex, void* out, size_t numBytes)
     if (rc != SL_OK) return rc;
     if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     collect_backward(c);
-    return check(c, hipMemcpy(out, t->buf.p, n, hipMemcpyDeviceToHost), "texture readback");
+    return download(c, out, t->buf.p, n, "texture readback");
 }
 
 size_t sl_create_framebuffer(sl_context* c) { c->fbos.emplace_back(); return c->fbos.size() - 1; }
@@ -1532,7 +1555,7 @@ int sl_framebuffer_backward(sl_context* c, size_t fbo, size_t gradImageTex)
     {
         if ((rc = ensure(c, F.gradDev, sizeof(GradDesc) * G.size())) != SL_OK) return rc;
         if ((rc = check(c, hipStreamSynchronize(s), "sync")) != SL_OK) return rc;
-        if ((rc = check(c, hipMemcpy(F.gradDev.p, G.data(), sizeof(GradDesc) * G.size(), hipMemcpyHostToDevice), "upload backward draws")) != SL_OK) return rc;
+        if ((rc = upload(c, F.gradDev.p, G.data(), sizeof(GradDesc) * G.size(), "upload backward draws")) != SL_OK) return rc;
         F.gradHost = G;
     }
     BackwardParams bp{};
