@@ -326,6 +326,36 @@ int sl_gradients_zero(sl_context* ctx);
 int sl_loss_l2_grad(sl_context* ctx, size_t colorTex, size_t targetTex, size_t gradImageTex, float scale);
 int sl_framebuffer_backward(sl_context* ctx, size_t fboId, size_t gradImageTex);
 
+/* ---- smoothed span coverage (no reference counterpart): gradients to the
+ *      screen positions of the vertices ---------------------------------
+ * The hard coverage rule of SL_ScanlineBounds::step (SL_ScanlineBounds.hpp:
+ * 310-322) puts pixel x of row y inside a triangle when its span ends satisfy
+ * e_min < x + 1 <= e_max.  The smoothed image is the colour attachment with
+ * every horizontal coverage boundary smeared to the right: where the
+ * visibility winners of pixels x-1 and x differ and a span end e of one of
+ * them lies in [x, x + 1) (the right end of the left pixel's triangle, else
+ * the left end of the right pixel's), out(x) = (1 - w) C(x) + w C(x-1) with
+ * w = exp(-(x + 1 - e) / sigma).  w is continuous in e and tends to 0 with
+ * sigma for every such e, so sigma = 0 returns the colour attachment bit for
+ * bit.  Boundaries of clipped triangles are not smoothed.
+ *
+ * sl_framebuffer_attach_positions: buffers of float2 per (saved draw,
+ *   vertex id) -- draw k of the framebuffer's saved batch (submission order
+ *   since the last depth clear) at float2 offset k * numVertices: gradVbo
+ *   receives dL/d(screen x, y) of the vertex, offsetVbo (optional) adds a
+ *   screen-space offset to the snapped vertex position inside the smoothing
+ *   passes.  SL_INVALID_ID detaches either.
+ * sl_smooth_coverage: writes the smoothed image of the framebuffer's last
+ *   batch (visibility attached, RGBA_FLOAT colour attachment 0) to outTex
+ *   (RGBA_FLOAT, framebuffer size, not the colour attachment).
+ * sl_smooth_coverage_backward: given dL/d(smoothed image), writes dL/d(colour
+ *   attachment) to gradImageTex (which sl_framebuffer_backward then takes) and
+ *   adds dL/d(screen position) into the attached position gradients, with the
+ *   visibility winners held fixed.  Queued on the context's stream. */
+int sl_framebuffer_attach_positions(sl_context* ctx, size_t fboId, size_t gradVboId, size_t offsetVboId, uint32_t numVertices);
+int sl_smooth_coverage(sl_context* ctx, size_t fboId, size_t outTexId, float sigma);
+int sl_smooth_coverage_backward(sl_context* ctx, size_t fboId, size_t gradSmoothTexId, float sigma, size_t gradImageTexId);
+
 /* ---- synchronisation and counters (no reference counterpart: the
  *      reference's draw is synchronous, SL_ProcessorPool.cpp:326) ---------- */
 int sl_finish(sl_context* ctx);

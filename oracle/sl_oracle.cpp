@@ -133,6 +133,10 @@ struct SavedPass
     uint32_t seqBase;
     size_t shader;
     size_t tex;               // texture bound through the uniforms at draw time
+    size_t drawIdx;           // the draw call it belongs to (instances of one call share it)
+    std::vector<uint8_t> uni; // uniform snapshot, scissor matrix and viewport at draw time
+    float scissorMat[16];
+    float vpDims[4];
 };
 
 struct Fbo
@@ -143,8 +147,11 @@ struct Fbo
     size_t dbgCount = SL_INVALID_ID;
     size_t vis = SL_INVALID_ID;
     std::vector<SavedPass> saved;   // passes since the depth attachment was last cleared
+    size_t savedDraws = 0;          // draw calls among them
     bool savedStale = false;
     bool savedBlend = false;
+    size_t posGrad = SL_INVALID_ID, posOffset = SL_INVALID_ID;   // smoothed coverage buffers
+    uint32_t posVerts = 0;
 };
 
 struct Ubo { uint8_t bytes[SL_MAX_UNIFORM_BUFFER_SIZE]; };
@@ -1092,7 +1099,12 @@ int run_draws(sl_context* c, const sl_mesh* meshes, size_t numMeshes, size_t num
             Fbo& f = c->fbos[fboId];
             size_t tex = SL_INVALID_ID;
             if (proto.uni) tex = reinterpret_cast<const sl_std_uniforms*>(proto.uni)->texture;
-            f.saved.push_back(SavedPass{m, (numInstances == 1) ? 0 : it, n, c->triSeq, shaderId, tex});
+            SavedPass sp{m, (numInstances == 1) ? 0 : it, n, c->triSeq, shaderId, tex, f.savedDraws, {}, {}, {}};
+            if (numInstances == 1 || it + 1 == nIter) f.savedDraws += 1;
+            if (proto.uni) sp.uni.assign(proto.uni, proto.uni + SL_MAX_UNIFORM_BUFFER_SIZE);
+            std::memcpy(sp.scissorMat, proto.scissorMat, sizeof(sp.scissorMat));
+            std::memcpy(sp.vpDims, proto.vpDims, sizeof(sp.vpDims));
+            f.saved.push_back(std::move(sp));
             if (proto.sh->blend != SL_BLEND_OFF) f.savedBlend = true;
         }
         c->triSeq += (uint32_t)n;
@@ -1170,6 +1182,7 @@ int clear_depth(sl_context* c, size_t fbo, double depth)
     if (f.dbgCount < c->texs.size()) { const uint32_t z = 0; clear_texture_bytes(c->texs[f.dbgCount], &z, 4); }
     if (f.vis < c->texs.size()) { const float z[4] = {0.f, 0.f, 0.f, 0.f}; clear_texture_bytes(c->texs[f.vis], z, 16); }
     f.saved.clear();
+    f.savedDraws = 0;
     f.savedStale = false;
     f.savedBlend = false;
     return SL_OK;
@@ -1416,7 +1429,7 @@ int slo_vbo_init(sl_context* c, size_t vbo, size_t numBytes, const void* data)
 {
     if (vbo >= c->vbos.size()) return SL_ERR_INVALID_HANDLE;
     if (!numBytes) return -1;
-    stale_saved(c);
+    if (c->vbos[vbo].valid) stale_saved(c);
     Vbo& v = c->vbos[vbo];
     v.bytes.assign(numBytes, 0);
     if (data) std::memcpy(v.bytes.data(), data, numBytes);
@@ -1448,7 +1461,7 @@ int slo_ibo_init(sl_context* c, size_t ibo, uint32_t numElements, int type, cons
 {
     if (ibo >= c->ibos.size()) return SL_ERR_INVALID_HANDLE;
     if (!numElements) return -1;
-    stale_saved(c);
+    if (c->ibos[ibo].valid) stale_saved(c);
     const uint32_t bpe = bytes_per_index(type);
     if (!bpe) return SL_ERR_INVALID_ARG;
     Ibo& b = c->ibos[ibo];
@@ -1507,7 +1520,7 @@ int slo_texture_init(sl_context* c, size_t tex, int type, uint16_t w, uint16_t h
     const uint32_t bpt = bytes_per_color(type);
     if (!bpt) return SL_ERR_UNSUPPORTED;
     if (!w || !h || !d) return SL_ERR_INVALID_ARG;
-    stale_saved(c);
+    if (c->texs[tex].valid) stale_saved(c);
     Tex& t = c->texs[tex];
     t.type = type; t.w = w; t.h = h; t.d = d; t.bpt = bpt;
     // SL_Texture.cpp:28-55: each dimension padded by (4 - dim % 4), zero filled
@@ -1753,6 +1766,7 @@ int slo_framebuffer_attach_visibility(sl_context* c, size_t fbo, size_t tex)
     Fbo& f = c->fbos[fbo];
     f.vis = tex;
     f.saved.clear();
+    f.savedDraws = 0;
     f.savedStale = false;
     f.savedBlend = false;
     return SL_OK;
@@ -1808,6 +1822,305 @@ int slo_loss_l2_grad(sl_context* c, size_t colorTex, size_t targetTex, size_t gr
     float* pg = reinterpret_cast<float*>(g.data.data());
     const float s2 = scale * 2.f;
     for (size_t i = 0; i < n; ++i) pg[i] = s2 * (pa[i] - pb[i]);
+    return SL_OK;
+}
+
+} // extern "C"
+
+/*-----------------------------------------------------------------------------
+ * Smoothed span coverage (no reference counterpart; the differentiable-
+ * coverage mode).  Per row, where the visibility winners of pixels x-1 and x
+ * differ and a span end e of one of them (the float SL_ScanlineBounds::step
+ * value before truncation, SL_ScanlineBounds.hpp:310-322) lies in [x, x + 1),
+ * out(x) = (1 - w) C(x) + w C(x-1), w = exp(-(x + 1 - e) / sigma): the right
+ * end of the left pixel's triangle if it lies there, else the left end of the
+ * right pixel's.  Clipped triangles' boundaries are not smoothed; sigma = 0
+ * returns the colour image.  The adjoint holds the winners fixed and sums in
+ * double.
+ *---------------------------------------------------------------------------*/
+struct OSpan
+{
+    bool ok = false;
+    size_t k = 0;             // draw call of the framebuffer's saved batch
+    size_t vid[3] = {0, 0, 0};
+    float sx[3], sy[3];       // snapped screen positions plus the offsets
+    float lo = 0.f, hi = 0.f; // span ends at the row
+    int loP = 0, loQ = 0, hiP = 0, hiQ = 0;   // vertex slots of their edges
+};
+
+OSpan winner_span(sl_context* c, const Fbo& f, uint32_t seq, int y, const float* offsets)
+{
+    OSpan S;
+    if (seq == 0 || f.saved.empty()) return S;
+    size_t lo = 0, hi = f.saved.size();
+    while (hi - lo > 1)
+    {
+        const size_t mid = (lo + hi) / 2;
+        if (f.saved[mid].seqBase < seq) lo = mid;
+        else hi = mid;
+    }
+    const SavedPass& sp = f.saved[lo];
+    if (seq <= sp.seqBase) return S;
+    const size_t t = (size_t)(seq - 1u - sp.seqBase);
+    if (t >= sp.numTris) return S;
+    Draw d;
+    d.ctx = c;
+    d.sh = &c->shaders[sp.shader];
+    d.uni = sp.uni.empty() ? nullptr : sp.uni.data();
+    std::memcpy(d.scissorMat, sp.scissorMat, sizeof(d.scissorMat));
+    std::memcpy(d.vpDims, sp.vpDims, sizeof(d.vpDims));
+    if (bind_mesh(c, sp.mesh, d) != SL_OK) return S;
+    const size_t e = sp.mesh.elementBegin + 3 * t;
+    size_t vid[3] = {e, e + 1, e + 2};
+    if (sp.mesh.mode == RENDER_MODE_INDEXED_TRIANGLES)
+        for (int k = 0; k < 3; ++k) vid[k] = index_at(*d.ibo, e + (size_t)k);
+    V4 pos[3];
+    for (int k = 0; k < 3; ++k)
+    {
+        V4 var[SL_SHADER_MAX_VARYING_VECTORS];
+        pos[k] = mat_mul(d.scissorMat, run_vertex_program(d, vid[k], sp.instance, var));
+    }
+    if (clip_status(pos[0], pos[1], pos[2]) != 3) return S;
+    S.k = sp.drawIdx;
+    for (int k = 0; k < 3; ++k)
+    {
+        divide_and_snap(pos[k], d.vpDims);
+        float ox = 0.f, oy = 0.f;
+        if (offsets && vid[k] < f.posVerts)
+        {
+            ox = offsets[2 * (S.k * f.posVerts + vid[k])];
+            oy = offsets[2 * (S.k * f.posVerts + vid[k]) + 1];
+        }
+        S.sx[k] = pos[k].v[0] + ox;
+        S.sy[k] = pos[k].v[1] + oy;
+        S.vid[k] = vid[k];
+    }
+    // vertices by y descending with SL_ScanlineBounds::init's swaps, then :310-322
+    int a = 0, b = 1, cc = 2;
+    if (S.sy[a] < S.sy[b]) std::swap(a, b);
+    if (S.sy[a] < S.sy[cc]) std::swap(a, cc);
+    if (S.sy[b] < S.sy[cc]) std::swap(b, cc);
+    const float ax = S.sx[a], ay = S.sy[a], bx = S.sx[b], by = S.sy[b], cx = S.sx[cc], cy = S.sy[cc];
+    const float yf = (float)y;
+    const float d0 = yf - ay, d1 = yf - by;
+    const float alpha = d0 * rcpf(cy - ay);
+    S.lo = ((cx - ax) * alpha) + ax;
+    S.loP = a;
+    S.loQ = cc;
+    if (d1 < 0.f)
+    {
+        S.hi = (((cx - bx) * rcpf(cy - by)) * d1) + bx;
+        S.hiP = b;
+        S.hiQ = cc;
+    }
+    else
+    {
+        S.hi = (((bx - ax) * rcpf(by - ay)) * d0) + ax;
+        S.hiP = a;
+        S.hiQ = b;
+    }
+    S.ok = true;
+    return S;
+}
+
+struct OBoundary { bool ok = false; float e = 0.f; OSpan S; int p = 0, q = 0; };
+
+OBoundary boundary_at(sl_context* c, const Fbo& f, const float* vis, int W, int x, int y, const float* offsets)
+{
+    OBoundary B;
+    if (x <= 0 || x >= W) return B;
+    uint32_t sl, sr;
+    std::memcpy(&sl, &vis[4 * ((size_t)y * W + x - 1)], 4);
+    std::memcpy(&sr, &vis[4 * ((size_t)y * W + x)], 4);
+    if (sl == sr) return B;
+    if (sl != 0)
+    {
+        const OSpan L = winner_span(c, f, sl, y, offsets);
+        if (L.ok)
+        {
+            const bool loIsMax = L.lo >= L.hi;
+            const float em = loIsMax ? L.lo : L.hi;
+            if (em >= (float)x && em < (float)(x + 1))
+            {
+                B.ok = true;
+                B.e = em;
+                B.S = L;
+                B.p = loIsMax ? L.loP : L.hiP;
+                B.q = loIsMax ? L.loQ : L.hiQ;
+                return B;
+            }
+        }
+    }
+    if (sr != 0)
+    {
+        const OSpan R = winner_span(c, f, sr, y, offsets);
+        if (R.ok)
+        {
+            const bool loIsMax = R.lo >= R.hi;
+            const float en = loIsMax ? R.hi : R.lo;
+            if (en >= (float)x && en < (float)(x + 1))
+            {
+                B.ok = true;
+                B.e = en;
+                B.S = R;
+                B.p = loIsMax ? R.hiP : R.loP;
+                B.q = loIsMax ? R.hiQ : R.loQ;
+            }
+        }
+    }
+    return B;
+}
+
+// checks shared by both smoothing entry points; fills the images' data
+int smooth_check(sl_context* c, size_t fbo, size_t inTex, size_t outTex, float sigma, Fbo*& F, const Tex*& col,
+                 const Tex*& vis, const Tex*& in, Tex*& out, const float*& offsets, bool backward)
+{
+    if (!c || fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    if (!(sigma >= 0.f) || !std::isfinite(sigma)) return SL_ERR_INVALID_ARG;
+    F = &c->fbos[fbo];
+    if (F->vis >= c->texs.size() || !c->texs[F->vis].valid) return SL_ERR_STATE;
+    if (F->colors.empty() || F->colors[0] >= c->texs.size() || c->texs[F->colors[0]].type != SL_COLOR_RGBA_FLOAT) return SL_ERR_STATE;
+    if (inTex >= c->texs.size() || !c->texs[inTex].valid || outTex >= c->texs.size() || !c->texs[outTex].valid)
+        return SL_ERR_INVALID_HANDLE;
+    vis = &c->texs[F->vis];
+    col = &c->texs[F->colors[0]];
+    in = &c->texs[inTex];
+    out = &c->texs[outTex];
+    for (const Tex* t : {in, (const Tex*)out})
+        if (t->type != SL_COLOR_RGBA_FLOAT || t->w != vis->w || t->h != vis->h) return SL_ERR_INVALID_ARG;
+    if (inTex == outTex) return SL_ERR_INVALID_ARG;
+    if (F->savedStale) return SL_ERR_STATE;
+    const uint64_t need = (uint64_t)F->savedDraws * F->posVerts * 8u;
+    offsets = nullptr;
+    if (F->posOffset != SL_INVALID_ID)
+    {
+        const Vbo& v = c->vbos[F->posOffset];
+        if (!v.valid || v.bytes.size() < need) return SL_ERR_INVALID_ARG;
+        offsets = reinterpret_cast<const float*>(v.bytes.data());
+    }
+    if (backward && F->posGrad != SL_INVALID_ID)
+    {
+        const Vbo& v = c->vbos[F->posGrad];
+        if (!v.valid || v.bytes.size() < need) return SL_ERR_INVALID_ARG;
+    }
+    return SL_OK;
+}
+
+extern "C" {
+
+int slo_framebuffer_attach_positions(sl_context* c, size_t fbo, size_t gradVbo, size_t offsetVbo, uint32_t numVertices)
+{
+    if (!c || fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    if (gradVbo != SL_INVALID_ID && (gradVbo >= c->vbos.size() || !c->vbos[gradVbo].valid)) return SL_ERR_INVALID_HANDLE;
+    if (offsetVbo != SL_INVALID_ID && (offsetVbo >= c->vbos.size() || !c->vbos[offsetVbo].valid)) return SL_ERR_INVALID_HANDLE;
+    if (gradVbo != SL_INVALID_ID && gradVbo == offsetVbo) return SL_ERR_INVALID_ARG;
+    Fbo& f = c->fbos[fbo];
+    f.posGrad = gradVbo;
+    f.posOffset = offsetVbo;
+    f.posVerts = numVertices;
+    return SL_OK;
+}
+
+int slo_smooth_coverage(sl_context* c, size_t fbo, size_t outTex, float sigma)
+{
+    if (!c || fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    const size_t colId = c->fbos[fbo].colors.empty() ? SL_INVALID_ID : c->fbos[fbo].colors[0];
+    Fbo* F; const Tex *col, *vis, *in; Tex* out; const float* offsets;
+    int rc = smooth_check(c, fbo, colId, outTex, sigma, F, col, vis, in, out, offsets, false);
+    if (rc != SL_OK) return rc;
+    const int W = vis->w, H = vis->h;
+    const float* C = reinterpret_cast<const float*>(col->data.data());
+    const float* V = reinterpret_cast<const float*>(vis->data.data());
+    float* O = reinterpret_cast<float*>(out->data.data());
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+        {
+            const size_t i = (size_t)y * W + x;
+            float o[4] = {C[4 * i], C[4 * i + 1], C[4 * i + 2], C[4 * i + 3]};
+            if (sigma > 0.f && !F->saved.empty())
+            {
+                const OBoundary B = boundary_at(c, *F, V, W, x, y, offsets);
+                if (B.ok)
+                {
+                    const float w = std::exp(-((float)(x + 1) - B.e) / sigma);
+                    const float u = 1.f - w;
+                    for (int ch = 0; ch < 4; ++ch) o[ch] = (u * C[4 * i + ch]) + (w * C[4 * (i - 1) + ch]);
+                }
+            }
+            std::memcpy(&O[4 * i], o, 16);
+        }
+    return SL_OK;
+}
+
+// e = Px + (Qx - Px) t, t = (y - Py) / (Qy - Py): partials in double
+static void edge_partials(double px, double py, double qx, double qy, double y, double& dPx, double& dPy, double& dQx, double& dQy)
+{
+    const double t = (y - py) / (qy - py), s = (qx - px) / (qy - py);
+    dPx = 1.0 - t;
+    dQx = t;
+    dPy = -(1.0 - t) * s;
+    dQy = -t * s;
+}
+
+int slo_smooth_coverage_backward(sl_context* c, size_t fbo, size_t gradSmoothTex, float sigma, size_t gradImageTex)
+{
+    Fbo* F; const Tex *col, *vis, *in; Tex* out; const float* offsets;
+    int rc = smooth_check(c, fbo, gradSmoothTex, gradImageTex, sigma, F, col, vis, in, out, offsets, true);
+    if (rc != SL_OK) return rc;
+    const int W = vis->w, H = vis->h;
+    const float* C = reinterpret_cast<const float*>(col->data.data());
+    const float* V = reinterpret_cast<const float*>(vis->data.data());
+    const float* G = reinterpret_cast<const float*>(in->data.data());
+    float* O = reinterpret_cast<float*>(out->data.data());
+    const bool smooth = sigma > 0.f && !F->saved.empty();
+    std::vector<double> pg;
+    if (F->posGrad != SL_INVALID_ID) pg.assign((size_t)F->savedDraws * F->posVerts * 2, 0.0);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+        {
+            const size_t i = (size_t)y * W + x;
+            if (!smooth)
+            {
+                std::memcpy(&O[4 * i], &G[4 * i], 16);
+                continue;
+            }
+            const OBoundary BL = boundary_at(c, *F, V, W, x, y, offsets);
+            const OBoundary BR = boundary_at(c, *F, V, W, x + 1, y, offsets);
+            const double wl = BL.ok ? std::exp(-((double)(x + 1) - (double)BL.e) / (double)sigma) : 0.0;
+            const double wr = BR.ok ? std::exp(-((double)(x + 2) - (double)BR.e) / (double)sigma) : 0.0;
+            for (int ch = 0; ch < 4; ++ch)
+            {
+                double gi = (1.0 - wl) * (double)G[4 * i + ch];
+                if (BR.ok) gi += wr * (double)G[4 * (i + 1) + ch];
+                O[4 * i + ch] = (float)gi;
+            }
+            if (BL.ok && !pg.empty())
+            {
+                double dLdw = 0.0;
+                for (int ch = 0; ch < 4; ++ch) dLdw += (double)G[4 * i + ch] * ((double)C[4 * (i - 1) + ch] - (double)C[4 * i + ch]);
+                const double dLde = dLdw * wl / (double)sigma;
+                const OSpan& S = BL.S;
+                double dPx, dPy, dQx, dQy;
+                edge_partials(S.sx[BL.p], S.sy[BL.p], S.sx[BL.q], S.sy[BL.q], (double)y, dPx, dPy, dQx, dQy);
+                if (S.vid[BL.p] < F->posVerts)
+                {
+                    const size_t o = 2 * (S.k * F->posVerts + S.vid[BL.p]);
+                    pg[o] += dLde * dPx;
+                    pg[o + 1] += dLde * dPy;
+                }
+                if (S.vid[BL.q] < F->posVerts)
+                {
+                    const size_t o = 2 * (S.k * F->posVerts + S.vid[BL.q]);
+                    pg[o] += dLde * dQx;
+                    pg[o + 1] += dLde * dQy;
+                }
+            }
+        }
+    if (!pg.empty())
+    {
+        float* dst = reinterpret_cast<float*>(c->vbos[F->posGrad].bytes.data());
+        for (size_t k = 0; k < pg.size(); ++k) dst[k] += (float)pg[k];
+    }
     return SL_OK;
 }
 

@@ -137,6 +137,9 @@ _SIGS = {
     "gradients_zero": (_I, [_P]),
     "loss_l2_grad": (_I, [_P, _SZ, _SZ, _SZ, C.c_float]),
     "framebuffer_backward": (_I, [_P, _SZ, _SZ]),
+    "framebuffer_attach_positions": (_I, [_P, _SZ, _SZ, _SZ, C.c_uint32]),
+    "smooth_coverage": (_I, [_P, _SZ, _SZ, C.c_float]),
+    "smooth_coverage_backward": (_I, [_P, _SZ, _SZ, C.c_float, _SZ]),
     "get_stats": (_I, [_P, C.POINTER(sl_frame_stats)]),
     "reset_stats": (_I, [_P]),
 }
@@ -327,6 +330,18 @@ class SLContext:
 
     def backward(self, fbo, grad_image_tex):
         return self._check(self.lib.framebuffer_backward(self.h, fbo, grad_image_tex), "framebuffer_backward")
+
+    def attach_positions(self, fbo, grad_vbo=SL_INVALID_ID, offset_vbo=SL_INVALID_ID, num_vertices=0):
+        """float2 per (saved draw, vertex id): screen-position gradients / offsets."""
+        return self._check(self.lib.framebuffer_attach_positions(self.h, fbo, grad_vbo, offset_vbo, num_vertices),
+                           "framebuffer_attach_positions")
+
+    def smooth_coverage(self, fbo, out_tex, sigma):
+        return self._check(self.lib.smooth_coverage(self.h, fbo, out_tex, sigma), "smooth_coverage")
+
+    def smooth_coverage_backward(self, fbo, grad_smooth_tex, sigma, grad_image_tex):
+        return self._check(self.lib.smooth_coverage_backward(self.h, fbo, grad_smooth_tex, sigma, grad_image_tex),
+                           "smooth_coverage_backward")
 
     # ---- uniforms / shaders ----
     def create_ubo(self, data=None):

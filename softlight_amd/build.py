@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT, "libsoftlight_amd.so")
-SOURCES = ["sl_kernels.hip", "sl_backward.hip", "sl_runtime.hip"]
+SOURCES = ["sl_kernels.hip", "sl_backward.hip", "sl_smooth.hip", "sl_runtime.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

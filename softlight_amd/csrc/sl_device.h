@@ -259,6 +259,22 @@ struct GradDesc
     uint64_t gradTexTexels;    // texels in the gradient allocation (padding included)
 };
 
+struct SmoothParams
+{
+    const GradDesc* draws;     // the framebuffer's saved draws, ordered by seqBase (uniOffset into uniforms)
+    uint32_t numDraws;
+    const uint8_t* uniforms;   // the saved draws' uniform snapshots
+    const float4* color;       // hard colour image (colour attachment 0, RGBA32F)
+    const float4* vis;         // visibility attachment
+    const float4* grad;        // backward: dL/d smoothed image
+    float4* out;               // forward: smoothed image; backward: dL/d hard colour image
+    float2* posGrad;           // backward: dL/d screen position per (saved draw, vertex), or null
+    const float2* posOffset;   // screen-space offsets per (saved draw, vertex), or null
+    uint32_t numVerts;         // vertices per saved draw in posGrad / posOffset
+    int32_t W, H;
+    float sigma;
+};
+
 struct BackwardParams
 {
     const GradDesc* draws;     // ordered by seqBase

@@ -9,6 +9,7 @@
 // synchronisation point; uniform blocks are snapshotted at draw time, so the
 // result equals drawing each call in order.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -17,6 +18,7 @@
 #include "sl_device.h"
 #include "sl_kernels.h"
 #include "sl_backward.h"
+#include "sl_smooth.h"
 
 using namespace slg;
 
@@ -33,8 +35,10 @@ struct Ibo { bool valid = false; DevBuf buf; int type = VERTEX_DATA_INT; uint32_
 struct Binding { int dims = VERTEX_DIMENSION_4; int type = VERTEX_DATA_FLOAT; ptrdiff_t offset = 0; ptrdiff_t stride = 0; };
 struct Vao { size_t vbo = SL_INVALID_ID; size_t ibo = SL_INVALID_ID; size_t nb = 0; Binding b[SL_VAO_MAX_BINDINGS]; };
 struct Tex { bool valid = false; DevBuf buf; int type = SL_COLOR_RGBA_8U; uint16_t w = 0, h = 0, d = 0; uint32_t bpt = 0; size_t grad = SL_INVALID_ID; };
-// A draw kept for the backward pass, with the resources its gradients go to.
-struct SavedDraw { DrawDesc d; size_t vbo; size_t tex; };
+// A draw kept for the backward pass, with the resources its gradients go to
+// and its uniform snapshot (the smoothed-coverage passes re-run its vertex
+// positions).
+struct SavedDraw { DrawDesc d; size_t vbo; size_t tex; std::vector<uint8_t> uni; };
 struct Fbo
 {
     std::vector<size_t> colors;
@@ -48,6 +52,13 @@ struct Fbo
     bool savedBlend = false;         // one of them blends: no fixed-visibility adjoint
     std::vector<GradDesc> gradHost;  // the backward draw table last uploaded ...
     DevBuf gradDev;                  // ... and its device copy
+    // smoothed coverage: screen-position gradients and offsets (float2 per
+    // saved draw and vertex), and the draw table / uniform arena last uploaded
+    size_t posGrad = SL_INVALID_ID, posOffset = SL_INVALID_ID;
+    uint32_t posVerts = 0;
+    std::vector<GradDesc> smoothHost;
+    std::vector<uint8_t> smoothUniHost;
+    DevBuf smoothDev, smoothUniDev;
 };
 struct Ubo { uint8_t bytes[SL_MAX_UNIFORM_BUFFER_SIZE]; };
 struct Shader { uint8_t cull, dtest, dmask, blend, nvary, nout; uint32_t vsProg, fsProg; size_t ubo; };
@@ -443,6 +454,14 @@ int upload_batch(sl_context* c, FrameRes& R, const Batch& B, hipStream_t s)
     return SL_OK;
 }
 
+// Draw i's uniform block as the batch snapshotted it (empty without one).
+std::vector<uint8_t> uniform_snapshot(const Batch& B, size_t i)
+{
+    const DrawDesc& d = B.draws[i];
+    if (!d.hasUniforms || (size_t)d.uniOffset + SL_MAX_UNIFORM_BUFFER_SIZE > B.uni.size()) return {};
+    return std::vector<uint8_t>(B.uni.begin() + d.uniOffset, B.uni.begin() + d.uniOffset + SL_MAX_UNIFORM_BUFFER_SIZE);
+}
+
 // Keeps the draws of a batch that rendered with a visibility attachment: the
 // backward pass maps each pixel's sequence number back to its draw.
 void save_draws(sl_context* c, const Batch& B, const Plan& p)
@@ -463,7 +482,7 @@ void save_draws(sl_context* c, const Batch& B, const Plan& p)
     }
     for (size_t i = 0; i < B.draws.size(); ++i)
     {
-        F.saved.push_back(SavedDraw{B.draws[i], B.drawVbo[i], B.drawTex[i]});
+        F.saved.push_back(SavedDraw{B.draws[i], B.drawVbo[i], B.drawTex[i], uniform_snapshot(B, i)});
         if (B.draws[i].blend != SL_BLEND_OFF) F.savedBlend = true;
     }
 }
@@ -644,7 +663,7 @@ int graph_capture(sl_context* c)
     G.fbo = B.fbo;
     G.clearDepth = B.clearDepth;
     G.draws.clear();
-    for (size_t i = 0; i < B.draws.size(); ++i) G.draws.push_back(SavedDraw{B.draws[i], B.drawVbo[i], B.drawTex[i]});
+    for (size_t i = 0; i < B.draws.size(); ++i) G.draws.push_back(SavedDraw{B.draws[i], B.drawVbo[i], B.drawTex[i], uniform_snapshot(B, i)});
     G.valid = true;
     return SL_OK;
 }
@@ -1004,9 +1023,13 @@ int sl_vbo_init(sl_context* c, size_t vbo, size_t numBytes, const void* data)
     if (!numBytes) return -1;                                   // SL_VertexBuffer.cpp:98-101
     int rc = flush(c);
     if (rc != SL_OK) return rc;
-    if (c->graph.valid) { (void)graph_collect(c); graph_release(c); }   // the graph points at the old buffer
-    stale_saved_draws(c);
     Vbo& v = c->vbos[vbo];
+    if (v.valid)
+    {
+        // storage moves: the graph and the saved draws point at the old buffer
+        if (c->graph.valid) { (void)graph_collect(c); graph_release(c); }
+        stale_saved_draws(c);
+    }
     if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     release(v.buf);
     if ((rc = ensure(c, v.buf, numBytes)) != SL_OK) return rc;
@@ -1050,10 +1073,13 @@ int sl_ibo_init(sl_context* c, size_t ibo, uint32_t numElements, int type, const
     if (!bpe) return SL_ERR_INVALID_ARG;
     int rc = flush(c);
     if (rc != SL_OK) return rc;
-    if (c->graph.valid) { (void)graph_collect(c); graph_release(c); }
-    stale_saved_draws(c);
-    if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     Ibo& b = c->ibos[ibo];
+    if (b.valid)
+    {
+        if (c->graph.valid) { (void)graph_collect(c); graph_release(c); }
+        stale_saved_draws(c);
+    }
+    if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     release(b.buf);
     const size_t bytes = (size_t)numElements * bpe;
     if ((rc = ensure(c, b.buf, bytes + 16)) != SL_OK) return rc;   // padded like SL_IndexBuffer.cpp:155
@@ -1115,10 +1141,13 @@ int sl_texture_init(sl_context* c, size_t tex, int type, uint16_t w, uint16_t h,
     if (!w || !h || !d) return SL_ERR_INVALID_ARG;
     int rc = flush(c);
     if (rc != SL_OK) return rc;
-    if (c->graph.valid) { (void)graph_collect(c); graph_release(c); }
-    stale_saved_draws(c);
-    if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     Tex& t = c->texs[tex];
+    if (t.valid)
+    {
+        if (c->graph.valid) { (void)graph_collect(c); graph_release(c); }
+        stale_saved_draws(c);
+    }
+    if ((rc = check(c, hipStreamSynchronize(c->stream), "sync")) != SL_OK) return rc;
     release(t.buf);
     // SL_Texture.cpp:28-55: each dimension padded by (4 - dim % 4), zero filled,
     // so the bilinear sampler's x = W / y = H neighbours read zeros.
@@ -1434,6 +1463,105 @@ int sl_framebuffer_attach_visibility(sl_context* c, size_t fbo, size_t tex)
     F.savedStale = false;
     F.savedBlend = false;
     return SL_OK;
+}
+
+int sl_framebuffer_attach_positions(sl_context* c, size_t fbo, size_t gradVbo, size_t offsetVbo, uint32_t numVertices)
+{
+    if (!c || fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    if (gradVbo != SL_INVALID_ID && (gradVbo >= c->vbos.size() || !c->vbos[gradVbo].valid)) return SL_ERR_INVALID_HANDLE;
+    if (offsetVbo != SL_INVALID_ID && (offsetVbo >= c->vbos.size() || !c->vbos[offsetVbo].valid)) return SL_ERR_INVALID_HANDLE;
+    if (gradVbo != SL_INVALID_ID && gradVbo == offsetVbo) return fail(c, SL_ERR_INVALID_ARG, "position gradients and offsets need separate buffers");
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    Fbo& F = c->fbos[fbo];
+    F.posGrad = gradVbo;
+    F.posOffset = offsetVbo;
+    F.posVerts = numVertices;
+    return SL_OK;
+}
+
+// The smoothing passes' draw table (uniform offsets into the saved draws'
+// snapshots) and arguments; uploads the table when it changed.
+int smooth_setup(sl_context* c, size_t fbo, size_t srcTex, size_t dstTex, float sigma, bool backward, SmoothParams& sp)
+{
+    if (!c || fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    if (!(sigma >= 0.f) || !std::isfinite(sigma)) return fail(c, SL_ERR_INVALID_ARG, "sigma must be finite and >= 0");
+    int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    Fbo& F = c->fbos[fbo];
+    Tex* vis = tex_at(c, F.vis);
+    if (!vis) return fail(c, SL_ERR_STATE, "framebuffer has no visibility attachment");
+    Tex* col = F.colors.empty() ? nullptr : tex_at(c, F.colors[0]);
+    if (!col || col->type != SL_COLOR_RGBA_FLOAT) return fail(c, SL_ERR_STATE, "smoothed coverage needs an RGBA_FLOAT colour attachment 0");
+    Tex* src = tex_at(c, srcTex);
+    Tex* dst = tex_at(c, dstTex);
+    if (!src || !dst) return SL_ERR_INVALID_HANDLE;
+    for (const Tex* t : {src, dst})
+        if (t->type != SL_COLOR_RGBA_FLOAT || t->w != vis->w || t->h != vis->h)
+            return fail(c, SL_ERR_INVALID_ARG, "smoothing images must be RGBA_FLOAT of the framebuffer's size");
+    if (src == dst) return fail(c, SL_ERR_INVALID_ARG, "smoothing input and output must differ");
+    if (F.savedStale) return fail(c, SL_ERR_STATE, "resources changed since the forward pass; draw again");
+    const size_t nd = F.saved.size();
+    const uint64_t need = (uint64_t)nd * F.posVerts * 8u;
+    Vbo* pg = (backward && F.posGrad != SL_INVALID_ID) ? &c->vbos[F.posGrad] : nullptr;
+    Vbo* po = (F.posOffset != SL_INVALID_ID) ? &c->vbos[F.posOffset] : nullptr;
+    if ((pg && (!pg->valid || pg->bytes < need)) || (po && (!po->valid || po->bytes < need)))
+        return fail(c, SL_ERR_INVALID_ARG, "position buffers need 8 bytes per saved draw and vertex");
+    std::vector<GradDesc> G(nd);
+    std::vector<uint8_t> U;
+    for (size_t i = 0; i < nd; ++i)
+    {
+        std::memset(&G[i], 0, sizeof(GradDesc));
+        G[i].d = F.saved[i].d;
+        G[i].d.uniOffset = (uint32_t)U.size();
+        const std::vector<uint8_t>& u = F.saved[i].uni;
+        U.insert(U.end(), u.begin(), u.end());
+        if (u.empty()) U.resize(U.size() + SL_MAX_UNIFORM_BUFFER_SIZE, 0);
+    }
+    if (nd > 0 && (F.smoothHost.size() != nd || F.smoothUniHost != U ||
+                   std::memcmp(F.smoothHost.data(), G.data(), sizeof(GradDesc) * nd) != 0))
+    {
+        if ((rc = ensure(c, F.smoothDev, sizeof(GradDesc) * nd)) != SL_OK) return rc;
+        if ((rc = ensure(c, F.smoothUniDev, U.size())) != SL_OK) return rc;
+        if ((rc = upload(c, F.smoothDev.p, G.data(), sizeof(GradDesc) * nd, "upload smoothing draws")) != SL_OK) return rc;
+        if ((rc = upload(c, F.smoothUniDev.p, U.data(), U.size(), "upload smoothing uniforms")) != SL_OK) return rc;
+        F.smoothHost = G;
+        F.smoothUniHost = U;
+    }
+    sp = SmoothParams{};
+    sp.draws = reinterpret_cast<const GradDesc*>(F.smoothDev.p);
+    sp.numDraws = (uint32_t)nd;
+    sp.uniforms = reinterpret_cast<const uint8_t*>(F.smoothUniDev.p);
+    sp.color = reinterpret_cast<const float4*>(col->buf.p);
+    sp.vis = reinterpret_cast<const float4*>(vis->buf.p);
+    sp.grad = backward ? reinterpret_cast<const float4*>(src->buf.p) : nullptr;
+    sp.out = reinterpret_cast<float4*>(dst->buf.p);
+    sp.posGrad = pg ? reinterpret_cast<float2*>(pg->buf.p) : nullptr;
+    sp.posOffset = po ? reinterpret_cast<const float2*>(po->buf.p) : nullptr;
+    sp.numVerts = F.posVerts;
+    sp.W = vis->w;
+    sp.H = vis->h;
+    sp.sigma = sigma;
+    if (nd == 0) sp.sigma = 0.f;   // nothing drawn: no boundaries
+    return SL_OK;
+}
+
+int sl_smooth_coverage(sl_context* c, size_t fbo, size_t outTex, float sigma)
+{
+    if (!c || fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    const size_t colId = c->fbos[fbo].colors.empty() ? SL_INVALID_ID : c->fbos[fbo].colors[0];
+    SmoothParams sp;
+    int rc = smooth_setup(c, fbo, colId, outTex, sigma, false, sp);
+    if (rc != SL_OK) return rc;
+    return check(c, launch_smooth(sp, c->stream), "k_smooth");
+}
+
+int sl_smooth_coverage_backward(sl_context* c, size_t fbo, size_t gradSmoothTex, float sigma, size_t gradImageTex)
+{
+    SmoothParams sp;
+    int rc = smooth_setup(c, fbo, gradSmoothTex, gradImageTex, sigma, true, sp);
+    if (rc != SL_OK) return rc;
+    return check(c, launch_smooth_backward(sp, c->stream), "k_smooth_backward");
 }
 
 int sl_vbo_attach_gradient(sl_context* c, size_t vbo, size_t grad)
