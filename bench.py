@@ -39,6 +39,9 @@ WORKLOAD = dict(seed=1, n_instances=64, stacks=256, sectors=256, W=1920, H=1080,
 RANDOM_WORKLOAD = dict(seed=0, ntri=4096, W=640, H=480)
 BATCHED_WORKLOAD = dict(seed=3, n_meshes=1000)
 VIEWS_WORKLOAD = dict(seed=4, level=5, views=32, size=512, views_per_fbo=8, tex_size=512)
+SMOOTH_WORKLOAD = dict(seed=6, meshes=8, min_tris=5000, max_tris=60000, views=64, size=512, views_per_fbo=8,
+                       tex_size=512)
+SMOOTH_SIGMAS = (1e-2, 1e-3, 1e-4)
 CALIBRATION_BYTES = 256 << 20
 
 
@@ -171,6 +174,65 @@ def run_views(steps, warmup, tune=False):
                          device_ms_forward=st["ms_total"] / steps, device_ms_backward=st["ms_backward"] / steps)
     out["raster_config"] = ctx.raster_config()
     return sc, out
+
+
+def device_used_bytes():
+    import torch
+    free, total = torch.cuda.mem_get_info()
+    return total - free
+
+
+def run_smooth(steps, warmup):
+    """Smoothed span coverage: 64 views at 512x512 of 8 meshes (5k..60k
+    triangles, log-uniform), per-vertex RGB + 512x512 RGBA32F texture.
+    Forward = clears + 512 draws (8 framebuffers of 8 viewports) + the
+    smoothing pass; backward = loss gradient on the smoothed images, smoothing
+    adjoint (image gradient + vertex screen-position gradients) and the
+    fixed-visibility adjoint to colours and texels.  Per sigma; plus the
+    sigma = 0 check against the hard image."""
+    import numpy as np
+    import softlight_amd
+    from softlight_amd import api, scenes
+
+    base = device_used_bytes()
+    ctx = api.SLContext(softlight_amd.load_library())
+    sc = scenes.SmoothScene(ctx, **SMOOTH_WORKLOAD)
+    sc.forward(0.0)
+    ctx.finish()
+    t0 = sc.targets[0]
+    hard_equal = bool(np.array_equal(ctx.texture_read(t0["smooth"]).view(np.uint32),
+                                     ctx.texture_read(t0["color"]).view(np.uint32)))
+    res = {}
+    peak = device_used_bytes() - base
+    for sigma in SMOOTH_SIGMAS:
+        sc.make_targets(sigma)
+        row = {}
+        for name, fn in (("fwd", lambda: sc.forward(sigma)), ("fwd_bwd", lambda: (sc.forward(sigma), sc.backward(sigma)))):
+            for _ in range(warmup):
+                fn()
+            ctx.finish()
+            t = time.perf_counter()
+            for _ in range(steps):
+                fn()
+            ctx.finish()
+            dt = (time.perf_counter() - t) / steps
+            row[name] = dict(ms=dt * 1e3, views_per_s=sc.views / dt)
+            peak = max(peak, device_used_bytes() - base)
+        ctx.finish()
+        pg = ctx.vbo_read(t0["pos_grad"], len(t0["views"]) * len(sc.meshes) * sc.n_vertices_max * 8, dtype=np.float32)
+        res[f"{sigma:g}"] = {"fwd_views_per_s": round(row["fwd"]["views_per_s"], 1),
+                             "fwd_bwd_views_per_s": round(row["fwd_bwd"]["views_per_s"], 1),
+                             "fwd_bwd_over_fwd_time": round(row["fwd_bwd"]["ms"] / row["fwd"]["ms"], 3),
+                             "ms_fwd": round(row["fwd"]["ms"], 3), "ms_fwd_bwd": round(row["fwd_bwd"]["ms"], 3),
+                             "nonzero_position_grads": int(np.count_nonzero(pg))}
+    ctx.close()
+    return {"workload": f"{sc.views} views x {sc.size}^2 of {len(sc.meshes)} UV spheres "
+                        f"({min(sc.mesh_tris)}..{max(sc.mesh_tris)} triangles, {sc.n_triangles} in all), "
+                        "per-vertex RGB + 512^2 RGBA32F texture, L2 loss on the smoothed images against targets "
+                        "rendered with vertices displaced by N(0, 0.02)",
+            "sigma_0_equals_hard_image": hard_equal,
+            "per_sigma": res,
+            "peak_device_bytes": int(peak)}
 
 
 def run_blit(reps=50):
@@ -335,6 +397,7 @@ def main():
     if args.workload == "spheres":
         _, vr = run_views(max(3, args.steps // 4), 1)
         out["differentiable"] = views_summary(vr)
+        out["smoothed_coverage"] = run_smooth(max(2, args.steps // 8), 1)
         out["present_blit"] = run_blit()
     if not args.no_cpu and args.workload == "spheres":
         out["cpu_baseline"] = run_cpu_baseline()

@@ -309,7 +309,8 @@ int sl_frame_graph_release(sl_context* ctx);
  * sl_texture_attach_gradient: an RGBA32F texture of the same size receiving
  *   dL/d(texel); bilinear taps at x = W / y = H land where the forward read
  *   them (the next row's first texel / the padding).
- * sl_gradients_zero: zeroes every attached gradient buffer.
+ * sl_gradients_zero: zeroes every attached gradient buffer (position
+ *   gradients of the smoothed-coverage mode included).
  * sl_loss_l2_grad: grad = scale * 2 * (color - target) for two RGBA32F
  *   textures of one size (the gradient of scale * sum((color - target)^2)).
  * sl_framebuffer_backward: adds the gradient of the framebuffer's last

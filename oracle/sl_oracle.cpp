@@ -1803,6 +1803,9 @@ int slo_gradients_zero(sl_context* c)
     for (const Tex& t : c->texs)
         if (t.valid && t.grad < c->texs.size() && c->texs[t.grad].valid)
             std::fill(c->texs[t.grad].data.begin(), c->texs[t.grad].data.end(), 0);
+    for (const Fbo& f : c->fbos)
+        if (f.posGrad < c->vbos.size() && c->vbos[f.posGrad].valid)
+            std::fill(c->vbos[f.posGrad].bytes.begin(), c->vbos[f.posGrad].bytes.end(), 0);
     return SL_OK;
 }
 

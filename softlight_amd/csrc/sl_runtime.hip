@@ -1609,6 +1609,12 @@ int sl_gradients_zero(sl_context* c)
         const Tex& g = c->texs[t.grad];
         if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(g.buf.p), 0u, g.buf.bytes / 4, c->stream), "zero gradient")) != SL_OK) return rc;
     }
+    for (const Fbo& F : c->fbos)
+    {
+        if (F.posGrad >= c->vbos.size() || !c->vbos[F.posGrad].valid) continue;
+        const Vbo& g = c->vbos[F.posGrad];
+        if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(g.buf.p), 0u, g.bytes / 4, c->stream), "zero gradient")) != SL_OK) return rc;
+    }
     return SL_OK;
 }
 

@@ -587,3 +587,128 @@ class DiffScene:
             d = img.astype(np.float64) - tgt.astype(np.float64)
             tot += float(np.sum(d * d))
         return tot
+
+
+class SmoothScene:
+    """Multi-mesh views for the smoothed-coverage mode: `meshes` UV spheres with
+    log-uniform triangle counts in [min_tris, max_tris], per-vertex RGB and one
+    RGBA32F texture (SL_PROGRAM_COLOR_TEXTURED), placed around the origin and
+    seen by `views` random cameras.  Views are packed `views_per_fbo` to a
+    framebuffer as a grid of viewports (one batch each); every view draws
+    every mesh.  The loss is sum((S - T)^2) over the smoothed images S;
+    gradients go to the vertex colours, the texels and the vertex screen
+    positions (float2 per saved draw and vertex)."""
+
+    def __init__(self, ctx: A.SLContext, seed=6, meshes=8, min_tris=5000, max_tris=60000, views=64, size=512,
+                 views_per_fbo=8, tex_size=512, distance=4.0, fov=60.0):
+        self.ctx = ctx
+        self.seed, self.views, self.size = seed, views, size
+        lo, hi = math.log(min_tris), math.log(max_tris)
+        u = uniform01(seed, meshes, stream=21).astype(np.float64)
+        self.mesh_tris, self.n_vertices_max, self.n_triangles = [], 0, 0
+        self.meshes, self.vbos, self.pos = [], [], []
+        for m in range(meshes):
+            target = math.exp(lo + (hi - lo) * u[m])
+            st = max(3, int(round(math.sqrt(target / 4.0))))
+            verts, idx = uv_sphere(st, 2 * st)
+            nv = verts.shape[0]
+            v = np.zeros((nv, 8), dtype=np.float32)
+            v[:, 0:3] = verts[:, 0:3]
+            v[:, 3:6] = uniform01(seed + m, nv * 3, stream=22).reshape(nv, 3)
+            v[:, 6:8] = verts[:, 6:8]
+            vbo = ctx.create_vbo(v)
+            ibo = ctx.create_ibo(idx)
+            vao = ctx.create_vao(vbo, ibo, [(0, 32, A.VERTEX_DIMENSION_3, A.VERTEX_DATA_FLOAT),
+                                           (12, 32, A.VERTEX_DIMENSION_3, A.VERTEX_DATA_FLOAT),
+                                           (24, 32, A.VERTEX_DIMENSION_2, A.VERTEX_DATA_FLOAT)])
+            self.meshes.append(A.sl_mesh(vao, 0, idx.size, A.RENDER_MODE_INDEXED_TRIANGLES, 0))
+            self.vbos.append(vbo)
+            self.pos.append(v)
+            self.mesh_tris.append(idx.size // 3)
+            self.n_triangles += idx.size // 3
+            self.n_vertices_max = max(self.n_vertices_max, nv)
+        self.tex = ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, tex_size, tex_size, 1,
+                                      uniform01(seed, tex_size * tex_size * 4, stream=23).reshape(tex_size, tex_size, 4))
+        self.grad_tex = ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, tex_size, tex_size)
+        ctx.texture_attach_gradient(self.tex, self.grad_tex)
+        self.grad_vbos = []
+        for m in range(meshes):
+            g = ctx.create_vbo(np.zeros((self.pos[m].shape[0], 4), dtype=np.float32))
+            ctx.vbo_attach_gradient(self.vbos[m], g)
+            self.grad_vbos.append(g)
+
+        # mesh placements (model matrices, column-major) and cameras
+        place = uniform(seed, meshes * 3, -1.2, 1.2, stream=24).reshape(meshes, 3)
+        models = [translate_scale(p[0], p[1], p[2], 0.55) for p in place]
+        P = perspective(fov, 1.0, 0.1, 100.0).T.astype(np.float64)
+        d = gaussian(seed, views * 3, stream=25).reshape(views, 3)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        self.shaders = []
+        for v in range(views):
+            up = (0.0, 1.0, 0.0) if abs(d[v, 1]) < 0.9 else (1.0, 0.0, 0.0)
+            mvp = P @ look_at(d[v] * distance, np.zeros(3), up)
+            row = []
+            for m in range(meshes):
+                un = A.sl_std_uniforms()
+                un.mvp[:] = mvp.T.astype(np.float32).reshape(-1).tolist()
+                un.model[:] = models[m].reshape(-1).tolist()
+                un.tint[:] = [1.0, 1.0, 1.0, 1.0]
+                un.texture = self.tex
+                un.instance_vbo = A.SL_NO_RESOURCE
+                ubo = ctx.create_ubo(un)
+                vs = A.sl_vertex_shader(2, A.SL_CULL_BACK_FACE, A.SL_PROGRAM_COLOR_TEXTURED)
+                fs = A.sl_fragment_shader(2, 1, A.SL_BLEND_OFF, A.SL_DEPTH_TEST_LESS_THAN, A.SL_DEPTH_MASK_ON,
+                                          A.SL_PROGRAM_COLOR_TEXTURED)
+                row.append(ctx.create_shader(vs, fs, ubo))
+            self.shaders.append(row)
+
+        per = max(1, min(views_per_fbo, views))
+        gx = 1
+        while gx * gx < per:
+            gx *= 2
+        gy = (per + gx - 1) // gx
+        self.targets = []
+        for f0 in range(0, views, per):
+            vs_ = list(range(f0, min(views, f0 + per)))
+            W, H = gx * size, gy * size
+            mk = lambda: ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, W, H)
+            color, vis, smooth, target, gsm, gimg = mk(), mk(), mk(), mk(), mk(), mk()
+            depth = ctx.create_texture(A.SL_COLOR_R_FLOAT, W, H)
+            fbo = ctx.create_framebuffer([color], depth)
+            ctx.attach_visibility(fbo, vis)
+            ndraw = len(vs_) * meshes
+            pg = ctx.create_vbo(np.zeros((ndraw, self.n_vertices_max, 2), dtype=np.float32))
+            ctx.attach_positions(fbo, pg, A.SL_INVALID_ID, self.n_vertices_max)
+            cells = [((i % gx) * size, (i // gx) * size) for i in range(len(vs_))]
+            self.targets.append(dict(fbo=fbo, color=color, vis=vis, smooth=smooth, target=target, grad_smooth=gsm,
+                                     grad_image=gimg, pos_grad=pg, W=W, H=H, views=vs_, cells=cells))
+
+    def forward(self, sigma):
+        for t in self.targets:
+            self.ctx.clear_framebuffer(t["fbo"], 0, (0.0, 0.0, 0.0, 0.0), 1.0)
+            for v, (x, y) in zip(t["views"], t["cells"]):
+                self.ctx.set_viewport(x, y, self.size, self.size)
+                for m, mesh in enumerate(self.meshes):
+                    self.ctx.draw(mesh, self.shaders[v][m], t["fbo"])
+            self.ctx.smooth_coverage(t["fbo"], t["smooth"], sigma)
+        self.ctx.set_viewport(0, 0, 65535, 65535)
+
+    def make_targets(self, sigma, std=0.02):
+        """Targets: the smoothed images of the meshes with vertices displaced by N(0, std)."""
+        for m, v in enumerate(self.pos):
+            d = v.copy()
+            d[:, 0:3] += (std * gaussian(self.seed + 7 + m, d.shape[0] * 3, stream=26).reshape(-1, 3)).astype(np.float32)
+            self.ctx.vbo_assign(self.vbos[m], d)
+        self.forward(sigma)
+        self.ctx.finish()
+        for t in self.targets:
+            self.ctx.texture_set_texels(t["target"], self.ctx.texture_read(t["smooth"]))
+        for m, v in enumerate(self.pos):
+            self.ctx.vbo_assign(self.vbos[m], v)
+
+    def backward(self, sigma):
+        self.ctx.gradients_zero()
+        for t in self.targets:
+            self.ctx.loss_l2_grad(t["smooth"], t["target"], t["grad_smooth"], 1.0)
+            self.ctx.smooth_coverage_backward(t["fbo"], t["grad_smooth"], sigma, t["grad_image"])
+            self.ctx.backward(t["fbo"], t["grad_image"])
