@@ -28,7 +28,11 @@
  *
  * Each part is pinned by a closed-form, exact-arithmetic or invariant check
  * in tests/test_closed_forms.py and tests/test_recurrence.py (list at the top
- * of test_closed_forms.py); the reference ships no golden images.
+ * of test_closed_forms.py); the reference ships no golden images.  The
+ * adjoints and the smoothed-coverage mode (no reference counterpart) are
+ * pinned in tests/test_backward.py and tests/test_smooth.py: finite
+ * differences, the adjoint identity, closed forms of the smoothing model and
+ * its exact sigma = 0 limit.
  *
  * Parity unpinned (no reference output and no mathematical fact fixes them;
  * the pins listed above decide them, and the GPU path shares the choice):
