@@ -249,12 +249,37 @@ struct Draw
     const Ibo* ibo;
     unsigned nvary;           // the shader's varyings, plus the weight slot with a visibility attachment
     int visSlot;              // varying slot of the source-vertex weights, or -1
+    int prim = 0;             // primitive kind of the draw call (kPrimFill, kPrimWire, ...)
     uint64_t tested = 0, shaded = 0, binned = 0;
 };
 
 const sl_std_uniforms* uniforms(const Draw& d)
 {
     return reinterpret_cast<const sl_std_uniforms*>(d.uni);
+}
+
+// Primitive kinds.  A draw call's processor and rasteriser follow the render
+// mode of its first mesh (sl_processor_type_for_draw_mode,
+// SL_ShaderProcessor.cpp:12-37; rasterizer.mMode = mRenderMode,
+// SL_VertexProcessor.cpp:154; ProcessorPool passes meshes[0].mode,
+// SL_ProcessorPool.cpp:286,339); whether a mesh reads its index buffer is the
+// processor's own test of that mesh's mode (SL_TriProcessor.cpp:682,697).
+enum { kPrimFill = 0, kPrimWire = 1, kPrimLine = 2, kPrimPoint = 3, kPrimBad = -1 };
+
+int prim_of(int mode)
+{
+    switch (mode)
+    {
+        case RENDER_MODE_TRIANGLES: case RENDER_MODE_INDEXED_TRIANGLES: return kPrimFill;
+        case RENDER_MODE_TRI_WIRE: case RENDER_MODE_INDEXED_TRI_WIRE: return kPrimWire;
+        default: return kPrimBad;
+    }
+}
+
+bool mesh_indexed(int prim, int mode)
+{
+    if (prim == kPrimFill || prim == kPrimWire) return mode == RENDER_MODE_INDEXED_TRIANGLES || mode == RENDER_MODE_INDEXED_TRI_WIRE;
+    return false;
 }
 
 /*-----------------------------------------------------------------------------
@@ -616,10 +641,75 @@ void shade_fragment(Draw& d, const Bin& b, int x, int y, float z, const float bc
     ++d.shaded;
 }
 
+// d0 / d1 of render_wireframe: max(|a - b|, 1) on int32.  The row above a
+// triangle's first row can have a span end of INT32_MIN (an extrapolated
+// horizontal edge); the reference's int subtraction then wraps, as it does on
+// x86, and |INT32_MIN| stays INT32_MIN, so the result is 1.
+int32_t wire_gap(int32_t a, int32_t b)
+{
+    const int32_t dd = (int32_t)((uint32_t)a - (uint32_t)b);
+    const int32_t ad = (dd < 0) ? (int32_t)(0u - (uint32_t)dd) : dd;
+    return (ad < 1) ? 1 : ad;
+}
+
+// SL_TriRasterizer::render_wireframe (SL_TriRasterizer.cpp:116-202): on each
+// row, the pixels from the span's left end over the distance the left end
+// moved since the row above (at least one), then a jump to the pixels over
+// the distance the right end moved.  The jump target can lie inside the first
+// run, so a pixel may be visited twice.  Barycentrics are evaluated directly
+// per pixel (two fused multiply-adds), not stepped.
+void raster_wire_bin(Draw& d, const Bin& b, int32_t rowThread, int32_t numRowThreads)
+{
+    const V4* P = b.p;
+    const int32_t bboxMinY = f2i(std::min(std::min(P[0].v[1], P[1].v[1]), P[2].v[1]));
+    const int32_t bboxMaxY = f2i(std::max(std::max(P[0].v[1], P[1].v[1]), P[2].v[1]));
+    const int32_t n1 = numRowThreads - 1;
+    const int32_t offset = n1 - ((bboxMinY + (n1 - rowThread)) % numRowThreads);
+    const Scanline sc = scanline_init(P[0], P[1], P[2]);
+    const float dep[3] = {P[0].v[2], P[1].v[2], P[2].v[2]};
+    const int dfunc = d.sh->dtest;
+    const int W = d.tgt.W;
+
+    for (int32_t y = bboxMinY + offset; y < bboxMaxY; y += numRowThreads)
+    {
+        const float yf = (float)y;
+        float bcY[4];
+        for (int k = 0; k < 3; ++k) bcY[k] = fmaf(b.by.v[k], yf, b.bz.v[k]);
+        bcY[3] = fmaf(b.by.v[3], 0.f, b.bz.v[3]);
+        int32_t xMin0, xMax0, xMin1, xMax1;
+        scanline_step(sc, yf, xMin0, xMax0);
+        scanline_step(sc, yf - 1.f, xMin1, xMax1);
+        const int32_t d0 = wire_gap(xMin0, xMin1);
+        const int32_t d1 = wire_gap(xMax0, xMax1);
+        for (int32_t ix = 0, x = xMin0; (uint32_t)x < (uint32_t)xMax0; ++ix, ++x)
+        {
+            if (ix == d0)
+            {
+                const int32_t t = (int32_t)((uint32_t)xMax0 - (uint32_t)d1 - 1u);
+                x = (xMin0 < t) ? t : xMin0;
+                continue;
+            }
+            const float xf = (float)x;
+            float bc[4];
+            for (int k = 0; k < 3; ++k) bc[k] = fmaf(b.bx.v[k], xf, bcY[k]);
+            bc[3] = fmaf(b.bx.v[3], 0.f, bcY[3]);
+            const float z = (((dep[0] * bc[0]) + (dep[1] * bc[1])) + (dep[2] * bc[2])) + (0.f * bc[3]);
+            const float dv = d.tgt.depth[(size_t)x + (size_t)W * (size_t)y];
+            ++d.tested;
+            if (depth_func(dfunc, z, dv)) shade_fragment(d, b, x, y, z, bc);
+        }
+    }
+}
+
 // SL_TriRasterizer::render_triangle_simd, generic branch (:806-923), for the
 // rows y = minY + offset + k*step owned by one thread.
 void raster_bin(Draw& d, const Bin& b, int32_t rowThread, int32_t numRowThreads)
 {
+    if (d.prim == kPrimWire)
+    {
+        raster_wire_bin(d, b, rowThread, numRowThreads);
+        return;
+    }
     const V4* P = b.p;
     const int32_t bboxMinY = f2i(std::min(std::min(P[0].v[1], P[1].v[1]), P[2].v[1]));
     const int32_t bboxMaxY = f2i(std::max(std::max(P[0].v[1], P[1].v[1]), P[2].v[1]));
@@ -968,8 +1058,8 @@ int bind_mesh(sl_context* c, const sl_mesh& m, Draw& d)
     d.vao = &c->vaos[m.vaoId];
     d.vbo = (d.vao->vbo < c->vbos.size()) ? &c->vbos[d.vao->vbo] : nullptr;
     d.ibo = (d.vao->ibo < c->ibos.size()) ? &c->ibos[d.vao->ibo] : nullptr;
-    if (m.mode != RENDER_MODE_TRIANGLES && m.mode != RENDER_MODE_INDEXED_TRIANGLES) return SL_ERR_UNSUPPORTED;
-    if (m.mode == RENDER_MODE_INDEXED_TRIANGLES && d.ibo == nullptr) return SL_ERR_STATE;
+    if (prim_of(m.mode) == kPrimBad) return SL_ERR_UNSUPPORTED;
+    if (mesh_indexed(d.prim, m.mode) && d.ibo == nullptr) return SL_ERR_STATE;
     return SL_OK;
 }
 
@@ -1042,7 +1132,7 @@ void run_draw_threads(sl_context* c, Draw& proto, const std::vector<Pass>& passe
                     vs.out = &perThread[t];
                     const size_t i = P.mesh->elementBegin + 3 * (g - P.triBase);
                     size_t a = i, b = i + 1, e = i + 2;
-                    if (P.mesh->mode == RENDER_MODE_INDEXED_TRIANGLES)
+                    if (mesh_indexed(d.prim, P.mesh->mode))
                     {
                         a = index_at(*d.ibo, i); b = index_at(*d.ibo, i + 1); e = index_at(*d.ibo, i + 2);
                     }
@@ -1087,6 +1177,10 @@ int run_draws(sl_context* c, const sl_mesh* meshes, size_t numMeshes, size_t num
     Draw proto;
     int rc = setup_draw(c, shaderId, fboId, proto);
     if (rc != SL_OK) return rc;
+    proto.prim = prim_of(meshes[0].mode);
+    if (proto.prim == kPrimBad) return SL_ERR_UNSUPPORTED;
+    // visibility records, and what is built on them, cover filled triangles
+    if (proto.tgt.vis && proto.prim != kPrimFill) return SL_ERR_UNSUPPORTED;
     auto t0 = std::chrono::steady_clock::now();
     const size_t nIter = (numInstances == 1) ? numMeshes : numInstances;
     std::vector<Pass> passes;
@@ -1125,7 +1219,7 @@ int run_draws(sl_context* c, const sl_mesh* meshes, size_t numMeshes, size_t num
         {
             bind_mesh(c, *P.mesh, proto);
             VertexStage vs(proto);
-            const bool indexed = (P.mesh->mode == RENDER_MODE_INDEXED_TRIANGLES);
+            const bool indexed = mesh_indexed(proto.prim, P.mesh->mode);
             for (size_t t = 0; t < P.numTris; ++t)
             {
                 const size_t i = P.mesh->elementBegin + 3 * t;
@@ -1284,7 +1378,7 @@ int backward(sl_context* c, size_t fboId, size_t gradImageTex)
         if (bind_mesh(c, sp.mesh, d) != SL_OK) continue;
         const size_t e = sp.mesh.elementBegin + 3 * t;
         size_t vid[3] = {e, e + 1, e + 2};
-        if (sp.mesh.mode == RENDER_MODE_INDEXED_TRIANGLES)
+        if (mesh_indexed(kPrimFill, sp.mesh.mode))
             for (int k = 0; k < 3; ++k) vid[k] = index_at(*d.ibo, e + (size_t)k);
         const float b[3] = {vis[4 * p + 1], vis[4 * p + 2], vis[4 * p + 3]};
         const float* g = &gimg[4 * p];
@@ -1879,7 +1973,7 @@ OSpan winner_span(sl_context* c, const Fbo& f, uint32_t seq, int y, const float*
     if (bind_mesh(c, sp.mesh, d) != SL_OK) return S;
     const size_t e = sp.mesh.elementBegin + 3 * t;
     size_t vid[3] = {e, e + 1, e + 2};
-    if (sp.mesh.mode == RENDER_MODE_INDEXED_TRIANGLES)
+    if (mesh_indexed(kPrimFill, sp.mesh.mode))
         for (int k = 0; k < 3; ++k) vid[k] = index_at(*d.ibo, e + (size_t)k);
     V4 pos[3];
     for (int k = 0; k < 3; ++k)

@@ -30,8 +30,14 @@ SL_CULL_BACK_FACE, SL_CULL_FRONT_FACE, SL_CULL_OFF = 0, 1, 2
  SL_DEPTH_TEST_GREATER_EQUAL, SL_DEPTH_TEST_EQUAL, SL_DEPTH_TEST_NOT_EQUAL) = range(7)
 SL_DEPTH_MASK_OFF, SL_DEPTH_MASK_ON = 0, 1
 SL_BLEND_OFF, SL_BLEND_ALPHA, SL_BLEND_PREMULTIPLED_ALPHA, SL_BLEND_ADDITIVE, SL_BLEND_SCREEN = range(5)
+RENDER_MODE_POINTS = 0x01
+RENDER_MODE_INDEXED_POINTS = 0x03
+RENDER_MODE_LINES = 0x04
+RENDER_MODE_INDEXED_LINES = 0x0C
 RENDER_MODE_TRIANGLES = 0x10
 RENDER_MODE_INDEXED_TRIANGLES = 0x30
+RENDER_MODE_TRI_WIRE = 0x50
+RENDER_MODE_INDEXED_TRI_WIRE = 0x70
 VERTEX_DIMENSION_1, VERTEX_DIMENSION_2, VERTEX_DIMENSION_3, VERTEX_DIMENSION_4 = range(4)
 VERTEX_DATA_BYTE, VERTEX_DATA_SHORT, VERTEX_DATA_INT, VERTEX_DATA_LONG, VERTEX_DATA_FLOAT = range(5)
 SL_COLOR_R_8U, SL_COLOR_R_32U, SL_COLOR_R_FLOAT = 0, 2, 5
@@ -252,6 +258,9 @@ class SLContext:
                  np.dtype(np.uint32): VERTEX_DATA_INT}[arr.dtype]
             self._check(self.lib.ibo_init(self.h, ibo, arr.size, t, arr.ctypes.data_as(_P)), "ibo_init")
         return ibo
+
+    def vao_set_index_buffer(self, vao, ibo):
+        return self._check(self.lib.vao_set_index_buffer(self.h, vao, ibo), "vao_set_index_buffer")
 
     def create_vao(self, vbo=None, ibo=None, bindings=()):
         vao = self.lib.create_vao(self.h)

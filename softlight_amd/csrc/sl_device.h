@@ -73,6 +73,30 @@ struct BindingDesc
 // DrawDesc::visSlot when the target has no visibility attachment.
 constexpr uint32_t kNoVisSlot = 0xFFu;
 
+// Primitive kinds.  A draw call's processor and rasteriser follow the render
+// mode of its first mesh (sl_processor_type_for_draw_mode,
+// SL_ShaderProcessor.cpp:12-37; rasterizer.mMode = mRenderMode,
+// SL_VertexProcessor.cpp:154, set from meshes[0].mode,
+// SL_ProcessorPool.cpp:286,339); whether a mesh reads its index buffer is the
+// processor's own test of that mesh's mode (SL_TriProcessor.cpp:682,697).
+enum : uint32_t { kPrimFill = 0, kPrimWire = 1, kPrimLine = 2, kPrimPoint = 3, kPrimBad = 0xFFu };
+
+SL_HD uint32_t prim_of_mode(int mode)
+{
+    switch (mode)
+    {
+        case RENDER_MODE_TRIANGLES: case RENDER_MODE_INDEXED_TRIANGLES: return kPrimFill;
+        case RENDER_MODE_TRI_WIRE: case RENDER_MODE_INDEXED_TRI_WIRE: return kPrimWire;
+        default: return kPrimBad;
+    }
+}
+
+SL_HD bool mesh_indexed(uint32_t prim, int mode)
+{
+    if (prim == kPrimFill || prim == kPrimWire) return mode == RENDER_MODE_INDEXED_TRIANGLES || mode == RENDER_MODE_INDEXED_TRI_WIRE;
+    return false;
+}
+
 struct DrawDesc
 {
     uint64_t triBase;          // first global triangle of this draw
@@ -94,6 +118,7 @@ struct DrawDesc
     uint32_t numBindings;
     uint32_t vsProg, fsProg;
     uint32_t visSlot;          // varying slot carrying the vertex weights, or kNoVisSlot
+    uint32_t prim;             // primitive kind (kPrimFill, kPrimWire, ...)
     uint8_t  indexed, cull, dtest, dmask, blend, nvary, nout, hasUniforms;
     BindingDesc bind[SL_VAO_MAX_BINDINGS];
     float scissor[16];         // SL_ViewportState::scissor_matrix at draw time
@@ -120,12 +145,26 @@ struct SetupRec
 };
 static_assert(sizeof(SetupRec) == 144, "SetupRec layout");
 
-// Depth function, depth mask, blend mode and "has colour outputs" of a draw,
-// packed into each of its records so the raster loop reads no draw table.
+// Depth function, depth mask, blend mode, "has colour outputs" and the
+// wireframe flag of a draw, packed into each of its records so the raster
+// loop reads no draw table.
+constexpr uint32_t kStateWire = 1u << 8;
+
 __host__ __device__ inline uint32_t pack_raster_state(const DrawDesc& d)
 {
     return (uint32_t)(d.dtest & 7u) | ((uint32_t)(d.dmask & 1u) << 3) | ((uint32_t)(d.blend & 7u) << 4) |
-           ((d.nout > 0 ? 1u : 0u) << 7);
+           ((d.nout > 0 ? 1u : 0u) << 7) | (d.prim == kPrimWire ? kStateWire : 0u);
+}
+
+// d0 / d1 of SL_TriRasterizer::render_wireframe (SL_TriRasterizer.cpp:155-156):
+// max(|a - b|, 1) on int32.  The row above a triangle's first row can have a
+// span end of INT32_MIN (an extrapolated horizontal edge); the int
+// subtraction then wraps and |INT32_MIN| stays INT32_MIN, giving 1.
+SL_HD int32_t wire_gap(int32_t a, int32_t b)
+{
+    const int32_t dd = (int32_t)((uint32_t)a - (uint32_t)b);
+    const int32_t ad = (dd < 0) ? (int32_t)(0u - (uint32_t)dd) : dd;
+    return (ad < 1) ? 1 : ad;
 }
 
 struct ClipJob
@@ -230,6 +269,7 @@ struct RasterParams
     uint32_t clearRGBA8;
     float clearRGBAF[4];
     float clearDepthValue;
+    int32_t wire;                 // some draw of the batch is a wireframe draw
     FrameCtrl* ctrl;
 };
 

@@ -1208,12 +1208,14 @@ constexpr uint32_t kNoPending = 0xFFFFFFFFu;
 // pixels, so no workgroup barrier is needed between records.
 //
 // AUX bit 0: debug buffers (last primitive, depth-pass count); bit 1: the
-// visibility attachment.
+// visibility attachment; bit 2: the batch holds wireframe records (the
+// variants without it carry no wireframe code).
 template <int CF, int AUX, int TW, int TH, int SEGS>
 __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
 {
     constexpr bool DBG = (AUX & 1) != 0;
     constexpr bool VIS = (AUX & 2) != 0;
+    constexpr bool WIRE = (AUX & 4) != 0;
     constexpr int NT = TH * 4 * SEGS;       // lanes: rows x segments x 4 phases
     constexpr int NW = NT / 64;             // waves, each owning TH / NW rows
     constexpr int SW = TW / SEGS;           // segment width (columns)
@@ -1360,94 +1362,153 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
             scanline_step(R, yf, xMin, xMax);
             if ((uint32_t)xMin < (uint32_t)xMax)
             {
-                const int xs = max(xMin, sx0), xe = min(min(xMax, sx0 + SW), P.W);
-                const int xf = xMin + ((phase - xMin) & 3);   // first span pixel of this lane's column class
-                const int g0 = (xs > xf) ? ((xs - xf + 3) >> 2) : 0;
-                int x = xf + 4 * g0;
-                if (x < xe)
+                const uint32_t st = R.state;
+                const int dtest = (int)(st & 7u);
+                const bool dmask = ((st >> 3) & 1u) != 0;
+                const int blendMode = (int)((st >> 4) & 7u);
+                const bool hasOut = ((st >> 7) & 1u) != 0;
+                // deferred: unblended fragments with outputs, and any
+                // fragment whose visibility record is wanted
+                const bool defer = (hasOut && blendMode == SL_BLEND_OFF) || (VIS && !hasOut);
+                // depth test and fragment work of this lane's pixel x of row y
+                auto fragment = [&](int x, float s0, float s1, float s2) {
+                    const float z = (((R.z[0] * s0) + (R.z[1] * s1)) + (R.z[2] * s2)) + R.bc3;
+                    const int c = x - x0;
+                    ++tested;
+                    if (depth_test(dtest, z, sDepth[row][c]))
+                    {
+                        if (defer)
+                        {
+                            sPend[row][c] = rid;
+                            sPendS[0][row][c] = s0;
+                            sPendS[1][row][c] = s1;
+                            sPendS[2][row][c] = s2;
+                        }
+                        else if (hasOut)
+                        {
+                            // a blended fragment reads the colour: settle the pixel first
+                            resolve(row, c);
+                            const DrawDesc& D = P.draws[R.draw];
+                            const float hw[3] = {R.hw[0], R.hw[1], R.hw[2]};
+                            float4 out, wv;
+                            const bool have = shade(P, D, hw, R.bc3, rid, s0, s1, s2, out, wv);
+                            if (have) blend_into<CF>(&sCol8[CF == 1 ? row : 0][CF == 1 ? c : 0],
+                                                     &sColF[CF == 2 ? row : 0][CF == 2 ? c : 0], blendMode, out);
+                            if (VIS) sVis[row][c] = make_float4(__uint_as_float(R.seq), wv.x, wv.y, wv.z);
+                        }
+                        if (dmask) sDepth[row][c] = z;
+                        if (DBG)
+                        {
+                            sLast[row][c] = R.seq;
+                            sCount[row][c] += 1u;
+                        }
+                        ++shaded;
+                    }
+                };
+                // This lane's pixels: x, x + 4, ... below xe, then those of
+                // a second run [xn, xen) (wireframe).  One loop serves both
+                // kinds so the fragment work is inlined once.
+                const bool wire = WIRE && (st & kStateWire) != 0u;
+                int x, xe, xn = 0, xen = 0;
+                float s0, s1, s2, d0, d1, d2;
+                if (wire)
                 {
-                    const uint32_t st = R.state;
-                    const int dtest = (int)(st & 7u);
-                    const bool dmask = ((st >> 3) & 1u) != 0;
-                    const int blendMode = (int)((st >> 4) & 7u);
-                    const bool hasOut = ((st >> 7) & 1u) != 0;
-                    // deferred: unblended fragments with outputs, and any
-                    // fragment whose visibility record is wanted
-                    const bool defer = (hasOut && blendMode == SL_BLEND_OFF) || (VIS && !hasOut);
+                    // wireframe: the run from the left end over the distance
+                    // the left end moved since the row above, then the run up
+                    // to the right end over the distance it moved; the second
+                    // can start inside the first (a pixel visited twice, in
+                    // run order).  Barycentrics are evaluated per pixel from
+                    // the row constants s = fma(dy, y, c) (d = 0).
+                    int32_t xMin1, xMax1;
+                    scanline_step(R, yf - 1.f, xMin1, xMax1);
+                    const int32_t g0 = wire_gap(xMin, xMin1), g1 = wire_gap(xMax, xMax1);
+                    const int64_t firstEnd = (int64_t)xMin + g0;
+                    const int hi = (firstEnd < (int64_t)xMax) ? (int)firstEnd : xMax;
+                    const int xs = max(xMin, sx0);
+                    x = xs + ((phase - xs) & 3);
+                    xe = min(min(hi, sx0 + SW), P.W);
+                    if (firstEnd < (int64_t)xMax)
+                    {
+                        const int32_t t = (int32_t)((uint32_t)xMax - (uint32_t)g1 - 1u);
+                        const int xs2 = max(max(xMin, t) + 1, sx0);
+                        xn = xs2 + ((phase - xs2) & 3);
+                        xen = min(min(xMax, sx0 + SW), P.W);
+                    }
+                    s0 = fmaf(R.by[0], yf, R.bz[0]);
+                    s1 = fmaf(R.by[1], yf, R.bz[1]);
+                    s2 = fmaf(R.by[2], yf, R.bz[2]);
+                    d0 = d1 = d2 = 0.f;
+                }
+                else
+                {
+                    const int xs = max(xMin, sx0);
+                    xe = min(min(xMax, sx0 + SW), P.W);
+                    const int xf = xMin + ((phase - xMin) & 3);   // first span pixel of this lane's column class
+                    const int g0 = (xs > xf) ? ((xs - xf + 3) >> 2) : 0;
+                    x = xf + 4 * g0;
                     // bc(x) = fl(fl(x*dx) + fma(dy, y, c)), then += 4*dx per group of 4 pixels
                     const float fx = (float)xf;
-                    float s0 = (fx * R.bx[0]) + fmaf(R.by[0], yf, R.bz[0]);
-                    float s1 = (fx * R.bx[1]) + fmaf(R.by[1], yf, R.bz[1]);
-                    float s2 = (fx * R.bx[2]) + fmaf(R.by[2], yf, R.bz[2]);
-                    const float d0 = R.bx[0] * 4.f, d1 = R.bx[1] * 4.f, d2 = R.bx[2] * 4.f;
-                    // advance to the segment's first group: plain steps when
-                    // few, the exact k-step jump when many
-                    if (g0 <= kStepLimit)
+                    s0 = (fx * R.bx[0]) + fmaf(R.by[0], yf, R.bz[0]);
+                    s1 = (fx * R.bx[1]) + fmaf(R.by[1], yf, R.bz[1]);
+                    s2 = (fx * R.bx[2]) + fmaf(R.by[2], yf, R.bz[2]);
+                    d0 = R.bx[0] * 4.f;
+                    d1 = R.bx[1] * 4.f;
+                    d2 = R.bx[2] * 4.f;
+                    if (x < xe)
                     {
-                        // three independent chains, four steps per trip
-                        int gi = 0;
-                        for (; gi + 4 <= g0; gi += 4)
+                        // advance to the segment's first group: plain steps when
+                        // few, the exact k-step jump when many
+                        if (g0 <= kStepLimit)
                         {
+                            // three independent chains, four steps per trip
+                            int gi = 0;
+                            for (; gi + 4 <= g0; gi += 4)
+                            {
 #pragma unroll
-                            for (int u = 0; u < 4; ++u)
+                                for (int u = 0; u < 4; ++u)
+                                {
+                                    s0 = s0 + d0;
+                                    s1 = s1 + d1;
+                                    s2 = s2 + d2;
+                                }
+                            }
+                            for (; gi < g0; ++gi)
                             {
                                 s0 = s0 + d0;
                                 s1 = s1 + d1;
                                 s2 = s2 + d2;
                             }
                         }
-                        for (; gi < g0; ++gi)
+                        else
                         {
-                            s0 = s0 + d0;
-                            s1 = s1 + d1;
-                            s2 = s2 + d2;
+                            const float3 a = recur_advance3(make_float3(s0, s1, s2), make_float3(d0, d1, d2), g0);
+                            s0 = a.x;
+                            s1 = a.y;
+                            s2 = a.z;
                         }
                     }
-                    else
-                    {
-                        const float3 a = recur_advance3(make_float3(s0, s1, s2), make_float3(d0, d1, d2), g0);
-                        s0 = a.x;
-                        s1 = a.y;
-                        s2 = a.z;
-                    }
+                }
+                const float c0 = s0, c1 = s1, c2 = s2;
+                for (;;)
+                {
                     for (; x < xe; x += 4)
                     {
-                        const float z = (((R.z[0] * s0) + (R.z[1] * s1)) + (R.z[2] * s2)) + R.bc3;
-                        const int c = x - x0;
-                        ++tested;
-                        if (depth_test(dtest, z, sDepth[row][c]))
+                        if (wire)
                         {
-                            if (defer)
-                            {
-                                sPend[row][c] = rid;
-                                sPendS[0][row][c] = s0;
-                                sPendS[1][row][c] = s1;
-                                sPendS[2][row][c] = s2;
-                            }
-                            else if (hasOut)
-                            {
-                                // a blended fragment reads the colour: settle the pixel first
-                                resolve(row, c);
-                                const DrawDesc& D = P.draws[R.draw];
-                                const float hw[3] = {R.hw[0], R.hw[1], R.hw[2]};
-                                float4 out, wv;
-                                const bool have = shade(P, D, hw, R.bc3, rid, s0, s1, s2, out, wv);
-                                if (have) blend_into<CF>(&sCol8[CF == 1 ? row : 0][CF == 1 ? c : 0],
-                                                         &sColF[CF == 2 ? row : 0][CF == 2 ? c : 0], blendMode, out);
-                                if (VIS) sVis[row][c] = make_float4(__uint_as_float(R.seq), wv.x, wv.y, wv.z);
-                            }
-                            if (dmask) sDepth[row][c] = z;
-                            if (DBG)
-                            {
-                                sLast[row][c] = R.seq;
-                                sCount[row][c] += 1u;
-                            }
-                            ++shaded;
+                            const float fx = (float)x;
+                            s0 = fmaf(R.bx[0], fx, c0);
+                            s1 = fmaf(R.bx[1], fx, c1);
+                            s2 = fmaf(R.bx[2], fx, c2);
                         }
+                        fragment(x, s0, s1, s2);
                         s0 = s0 + d0;
                         s1 = s1 + d1;
                         s2 = s2 + d2;
                     }
+                    if (xn >= xen) break;
+                    x = xn;
+                    xe = xen;
+                    xn = xen;
                 }
             }
         }
@@ -1694,9 +1755,25 @@ hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debu
     if (p.vis != nullptr)
     {
         // the visibility attachment is only built for RGBA32F colour targets
-        if (cf != 2) return hipErrorInvalidValue;
+        // and filled triangles
+        if (cf != 2 || p.wire) return hipErrorInvalidValue;
         if (debug) launch_raster_cfg<2, 3>(p, numTiles, cfg, s);
         else launch_raster_cfg<2, 2>(p, numTiles, cfg, s);
+    }
+    else if (p.wire)
+    {
+        if (debug)
+        {
+            if (cf == 0) launch_raster_cfg<0, 5>(p, numTiles, cfg, s);
+            else if (cf == 1) launch_raster_cfg<1, 5>(p, numTiles, cfg, s);
+            else launch_raster_cfg<2, 5>(p, numTiles, cfg, s);
+        }
+        else
+        {
+            if (cf == 0) launch_raster_cfg<0, 4>(p, numTiles, cfg, s);
+            else if (cf == 1) launch_raster_cfg<1, 4>(p, numTiles, cfg, s);
+            else launch_raster_cfg<2, 4>(p, numTiles, cfg, s);
+        }
     }
     else if (debug)
     {

@@ -397,6 +397,8 @@ RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool have
     }
     rp.clearRGBA8 = packed;
     rp.clearDepthValue = (float)B.depth;
+    rp.wire = 0;
+    for (const DrawDesc& d : B.draws) rp.wire |= (d.prim == kPrimWire) ? 1 : 0;
     rp.ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
     return rp;
 }
@@ -724,17 +726,21 @@ int flush(sl_context* c)
 // ---------------------------------------------------------------------------
 // draw recording
 // ---------------------------------------------------------------------------
-int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t shaderId, size_t fboId)
+// `procMode` is the render mode that picks the primitive kind: the mesh's
+// own for sl_draw / sl_draw_instanced, the first mesh's for sl_draw_multiple.
+int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t shaderId, size_t fboId, int procMode)
 {
     if (shaderId >= c->shaders.size()) return fail(c, SL_ERR_INVALID_HANDLE, "shader handle");
     if (fboId >= c->fbos.size()) return fail(c, SL_ERR_INVALID_HANDLE, "framebuffer handle");
     if (m.vaoId >= c->vaos.size()) return fail(c, SL_ERR_INVALID_HANDLE, "vao handle");
-    if (m.mode != RENDER_MODE_TRIANGLES && m.mode != RENDER_MODE_INDEXED_TRIANGLES) return fail(c, SL_ERR_UNSUPPORTED, "render mode");
+    const uint32_t prim = prim_of_mode(procMode);
+    if (prim == kPrimBad || prim_of_mode(m.mode) == kPrimBad) return fail(c, SL_ERR_UNSUPPORTED, "render mode");
     const Shader& sh = c->shaders[shaderId];
     if (sh.nout > 1) return fail(c, SL_ERR_UNSUPPORTED, "more than one fragment output");
     Fbo& F = c->fbos[fboId];
     const bool vis = tex_at(c, F.vis) != nullptr;
     if (vis && sh.nvary >= SL_SHADER_MAX_VARYING_VECTORS) return fail(c, SL_ERR_UNSUPPORTED, "visibility needs a free varying slot (numVaryings <= 3)");
+    if (vis && prim != kPrimFill) return fail(c, SL_ERR_UNSUPPORTED, "visibility attachments cover filled triangles only");
     Tex* dt = tex_at(c, F.depth);
     if (!dt || dt->type != SL_COLOR_R_FLOAT) return fail(c, SL_ERR_STATE, "framebuffer needs an R_FLOAT depth attachment");
     if (sh.nout == 1)
@@ -745,7 +751,7 @@ int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t s
     const Vao& vao = c->vaos[m.vaoId];
     const Vbo* vbo = (vao.vbo < c->vbos.size() && c->vbos[vao.vbo].valid) ? &c->vbos[vao.vbo] : nullptr;
     const Ibo* ibo = (vao.ibo < c->ibos.size() && c->ibos[vao.ibo].valid) ? &c->ibos[vao.ibo] : nullptr;
-    const bool indexed = m.mode == RENDER_MODE_INDEXED_TRIANGLES;
+    const bool indexed = mesh_indexed(prim, m.mode);
     if (indexed && !ibo) return fail(c, SL_ERR_STATE, "indexed draw without an index buffer");
     const bool needUni = sh.vsProg != SL_PROGRAM_PASSTHROUGH_GOURAUD || sh.fsProg != SL_PROGRAM_PASSTHROUGH_GOURAUD;
     if (needUni && sh.ubo >= c->ubos.size()) return fail(c, SL_ERR_STATE, "program needs a uniform buffer");
@@ -798,6 +804,7 @@ int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t s
     d.nvary = sh.nvary;
     d.nout = sh.nout;
     d.visSlot = kNoVisSlot;
+    d.prim = prim;
     if (vis)
     {
         d.visSlot = sh.nvary;
@@ -1295,7 +1302,7 @@ size_t sl_create_shader(sl_context* c, const sl_vertex_shader* vs, const sl_frag
 int sl_draw(sl_context* c, const sl_mesh* m, size_t shader, size_t fbo)
 {
     if (!c || !m) return SL_ERR_INVALID_ARG;
-    return record_draw(c, *m, 1, shader, fbo);
+    return record_draw(c, *m, 1, shader, fbo, m->mode);
 }
 
 int sl_draw_multiple(sl_context* c, const sl_mesh* m, size_t n, size_t shader, size_t fbo)
@@ -1304,7 +1311,7 @@ int sl_draw_multiple(sl_context* c, const sl_mesh* m, size_t n, size_t shader, s
     if (m == nullptr || n == 0) return SL_OK;                   // SL_Context.cpp:726-730
     for (size_t i = 0; i < n; ++i)
     {
-        const int rc = record_draw(c, m[i], 1, shader, fbo);
+        const int rc = record_draw(c, m[i], 1, shader, fbo, m[0].mode);
         if (rc != SL_OK) return rc;
     }
     return SL_OK;
@@ -1314,7 +1321,7 @@ int sl_draw_instanced(sl_context* c, const sl_mesh* m, size_t numInstances, size
 {
     if (!c || !m) return SL_ERR_INVALID_ARG;
     if (numInstances == 0) return SL_OK;
-    return record_draw(c, *m, numInstances, shader, fbo);
+    return record_draw(c, *m, numInstances, shader, fbo, m->mode);
 }
 
 int sl_clear_color_buffer(sl_context* c, size_t fbo, unsigned att, const double rgba[4])

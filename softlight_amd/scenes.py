@@ -215,20 +215,21 @@ def random_triangle_vertices(seed: int, ntri: int) -> np.ndarray:
 
 
 def build_from_vertex_records(ctx, rec: np.ndarray, W: int, H: int, debug=False, shader_kw=None,
-                              color_type=A.SL_COLOR_RGBA_8U) -> Scene:
+                              color_type=A.SL_COLOR_RGBA_8U, mode=A.RENDER_MODE_TRIANGLES) -> Scene:
     nv = rec.shape[0]
     vbo = ctx.create_vbo(rec)
     vao = ctx.create_vao(vbo, None, [(0, 16, A.VERTEX_DIMENSION_3, A.VERTEX_DATA_FLOAT),
                                      (12, 16, A.VERTEX_DIMENSION_4, A.VERTEX_DATA_BYTE)])
     color, depth, fbo, dbg = make_target(ctx, W, H, debug, color_type)
     shader = gouraud_shader(ctx, **(shader_kw or {}))
-    mesh = A.sl_mesh(vao, 0, nv, A.RENDER_MODE_TRIANGLES, 0)
+    mesh = A.sl_mesh(vao, 0, nv, mode, 0)
     return Scene(ctx, W, H, fbo, color, depth, shader, [("draw", mesh, 1)], n_triangles=nv // 3, debug=dbg)
 
 
-def build_random_triangles(ctx, seed=0, ntri=4096, W=640, H=480, debug=False) -> Scene:
-    s = build_from_vertex_records(ctx, random_triangle_vertices(seed, ntri), W, H, debug)
-    s.info.update(workload="random_triangles", seed=seed, triangles=ntri, W=W, H=H)
+def build_random_triangles(ctx, seed=0, ntri=4096, W=640, H=480, debug=False, wire=False) -> Scene:
+    mode = A.RENDER_MODE_TRI_WIRE if wire else A.RENDER_MODE_TRIANGLES
+    s = build_from_vertex_records(ctx, random_triangle_vertices(seed, ntri), W, H, debug, mode=mode)
+    s.info.update(workload="random_triangles", seed=seed, triangles=ntri, W=W, H=H, mode=mode)
     return s
 
 
@@ -240,7 +241,7 @@ def sphere_instances(seed: int, n: int, extent: float = 20.0, radius: float = 2.
 
 def build_sphere_scene(ctx, seed=1, n_instances=64, stacks=256, sectors=256, W=1920, H=1080,
                        tex_size=1024, radius=2.0, extent=20.0, debug=False, cull=A.SL_CULL_BACK_FACE,
-                       blend=A.SL_BLEND_OFF, alpha=1.0) -> Scene:
+                       blend=A.SL_BLEND_OFF, alpha=1.0, wire=False) -> Scene:
     verts, idx = uv_sphere(stacks, sectors)
     vbo = ctx.create_vbo(verts)
     ibo = ctx.create_ibo(idx)
@@ -268,11 +269,12 @@ def build_sphere_scene(ctx, seed=1, n_instances=64, stacks=256, sectors=256, W=1
     fs = A.sl_fragment_shader(2, 1, blend, A.SL_DEPTH_TEST_LESS_THAN, A.SL_DEPTH_MASK_ON, A.SL_PROGRAM_LAMBERT_TEXTURED)
     shader = ctx.create_shader(vs, fs, ubo)
     color, depth, fbo, dbg = make_target(ctx, W, H, debug)
-    mesh = A.sl_mesh(vao, 0, idx.size, A.RENDER_MODE_INDEXED_TRIANGLES, 0)
+    mode = A.RENDER_MODE_INDEXED_TRI_WIRE if wire else A.RENDER_MODE_INDEXED_TRIANGLES
+    mesh = A.sl_mesh(vao, 0, idx.size, mode, 0)
     s = Scene(ctx, W, H, fbo, color, depth, shader, [("instanced", mesh, n_instances)],
               n_triangles=(idx.size // 3) * n_instances, debug=dbg)
     s.info.update(workload="textured_spheres", seed=seed, instances=n_instances, stacks=stacks, sectors=sectors,
-                  W=W, H=H, tex_size=tex_size, radius=radius, extent=extent,
+                  W=W, H=H, tex_size=tex_size, radius=radius, extent=extent, mode=mode,
                   triangles_per_instance=idx.size // 3, vertices_per_instance=verts.shape[0])
     s.info["_keep"] = (verts, idx, inst)
     return s
@@ -326,7 +328,7 @@ def rotation(axis, angle):
 
 
 def build_batched_scene(ctx, seed=3, n_meshes=1000, median_tris=2000, sigma=1.0, min_tris=12, max_tris=200_000,
-                        W=1920, H=1080, n_textures=16, tex_size=512, blend_fraction=0.2, debug=False):
+                        W=1920, H=1080, n_textures=16, tex_size=512, blend_fraction=0.2, debug=False, wire_every=0):
     """1000 displaced icospheres (log-normal triangle counts), random affine
     model matrices, 80% opaque / 20% alpha-blended Lambert materials over
     sixteen RGBA8 noise textures; one draw call per mesh, in index order."""
@@ -389,7 +391,8 @@ def build_batched_scene(ctx, seed=3, n_meshes=1000, median_tris=2000, sigma=1.0,
         sh = ctx.create_shader(vs_op, fs_bl if blended else fs_op, ubo)
         shaders.append(sh)
         n_blend += int(blended)
-        mesh = A.sl_mesh(vao, 0, 3 * n, A.RENDER_MODE_INDEXED_TRIANGLES, m)
+        wire = wire_every > 0 and m % wire_every == wire_every - 1
+        mesh = A.sl_mesh(vao, 0, 3 * n, A.RENDER_MODE_INDEXED_TRI_WIRE if wire else A.RENDER_MODE_INDEXED_TRIANGLES, m)
         draws.append(("draw_shader", mesh, sh))
         total += n
         keep.append(verts)

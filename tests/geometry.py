@@ -110,3 +110,93 @@ def render(ctx, scene):
     out = scene.read()
     out["stats"] = ctx.stats()
     return out
+
+
+# ---------------------------------------------------------------------------
+# wireframe rows (SL_TriRasterizer.cpp:116-202)
+# ---------------------------------------------------------------------------
+INT32_MIN = -2 ** 31
+_M32 = 0xFFFFFFFF
+
+
+def i32(v: int) -> int:
+    """v wrapped to a two's-complement int32."""
+    v &= _M32
+    return v - (1 << 32) if v >= (1 << 31) else v
+
+
+def scan_sorted(tri):
+    """The vertices ordered by y descending with the scanline setup's three
+    conditional swaps (ties keep input order): v0 top, v1 middle, v2 bottom
+    (SL_ScanlineBounds.hpp:143-166)."""
+    a, b, c = (tuple(p) for p in tri)
+    if a[1] < b[1]:
+        a, b = b, a
+    if a[1] < c[1]:
+        a, c = c, a
+    if b[1] < c[1]:
+        b, c = c, b
+    return a, b, c
+
+
+def line_x(p, q, t):
+    """Exact x of the line through p and q at height t (None if horizontal)."""
+    if p[1] == q[1]:
+        return None
+    return Fr(p[0]) + Fr(q[0] - p[0]) * Fr(t - p[1], q[1] - p[1])
+
+
+def exact_span(tri, t, margin=Fr(1, 1000)):
+    """(xMin, xMax, near) of the span the scanline setup gives at height t,
+    also outside the triangle's rows: the long edge v0-v2 and the short edge
+    v1-v2 below v1, v0-v1 from v1 up, both extended as lines; ends truncate
+    toward zero, a horizontal edge gives INT32_MIN (its slope is infinite).
+    `near` flags an end within `margin` of an integer."""
+    v0, v1, v2 = scan_sorted(tri)
+    lo = line_x(v0, v2, t)
+    hi = line_x(v1, v2, t) if t < v1[1] else line_x(v0, v1, t)
+    ends = [INT32_MIN if e is None else math.trunc(e) for e in (lo, hi)]
+    near = any(e is not None and abs(e - round(e)) < margin for e in (lo, hi))
+    return min(ends), max(ends), near
+
+
+def wire_gap(a: int, b: int) -> int:
+    dd = i32(a - b)
+    ad = i32(-dd) if dd < 0 else dd
+    return 1 if ad < 1 else ad
+
+
+def wire_row_visits(tri, y):
+    """Pixels the wireframe raster visits on row y, in visit order (a pixel
+    may appear twice), and whether an end of this row's or the row above's
+    span lies near an integer."""
+    xMin0, xMax0, n0 = exact_span(tri, y)
+    xMin1, xMax1, n1 = exact_span(tri, y - 1)
+    d0, d1 = wire_gap(xMin0, xMin1), wire_gap(xMax0, xMax1)
+    visits = []
+    ix, x = 0, xMin0
+    while (x & _M32) < (xMax0 & _M32):
+        if ix == d0:
+            x = max(xMin0, i32(xMax0 - d1 - 1))
+        else:
+            visits.append(x)
+        ix += 1
+        x += 1
+    return visits, (n0 or n1)
+
+
+def wire_visits(tri):
+    """{(x, y): visits} of one snapped triangle and the set of rows with a
+    near-integer span end (where float rounding may move an end)."""
+    xs = [p[0] for p in tri]
+    ys = [p[1] for p in tri]
+    counts, amb = {}, set()
+    if max(xs) - min(xs) < 1 or max(ys) - min(ys) < 1:
+        return counts, amb
+    for y in range(min(ys), max(ys)):
+        vis, near = wire_row_visits(tri, y)
+        if near:
+            amb.add(y)
+        for x in vis:
+            counts[(x, y)] = counts.get((x, y), 0) + 1
+    return counts, amb

@@ -1,0 +1,207 @@
+"""Wireframe triangles (RENDER_MODE_TRI_WIRE / INDEXED_TRI_WIRE,
+SL_TriRasterizer::render_wireframe, SL_TriRasterizer.cpp:116-202).
+
+What pins the oracle's restatement, and the mistake each check catches:
+
+  right-triangle outline  the wireframe of an axis-aligned right triangle is
+                          its three edges, each pixel visited once (bottom row
+                          whole, one pixel per row on the vertical and on the
+                          diagonal edge), for all six vertex orders: a wrong
+                          row above (y + 1), a dropped run, the horizontal
+                          edge's infinite slope not giving a whole first row
+  inside the fill         every visited pixel is a pixel of the filled
+                          triangle, and both ends of every filled row are
+                          visited (independent of how the runs are formed)
+  exact row model         visit counts against the span ends computed in
+                          exact rational arithmetic (tests/geometry.py), with
+                          the runs formed from them; covers rows where the
+                          second run starts inside the first (pixels visited
+                          twice) and the extrapolated row above the first row
+  double visit            additive blending with the depth test off adds a
+                          twice-visited pixel's colour twice
+  draw_multiple           the first mesh's mode picks the rasteriser for every
+                          mesh of the call; each mesh's own mode picks whether
+                          it reads indices
+  visibility              refused: visibility records cover filled triangles
+GPU (marked gpu): the same checks, and bit-exact parity with the oracle on
+random triangles, instanced spheres in every tile configuration, and the
+batched scene with every third mesh drawn as wireframe.
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+import pytest
+
+from softlight_amd import api as A
+from softlight_amd import scenes
+from tests.geometry import exact_coverage, render, vertex_records, wire_visits
+from tests.parity import assert_parity, render_both
+
+BACKENDS = [pytest.param("oracle", id="oracle"), pytest.param("gpu", id="gpu", marks=pytest.mark.gpu)]
+
+
+@pytest.fixture
+def make_ctx(request):
+    def _make(kind):
+        return A.SLContext(request.getfixturevalue("oracle_lib" if kind == "oracle" else "gpu_lib"))
+    return _make
+
+
+def wire_scene(ctx, tris, W, H, blend=A.SL_BLEND_OFF, colors=None, mode=A.RENDER_MODE_TRI_WIRE,
+               color_type=A.SL_COLOR_RGBA_8U):
+    rec = vertex_records(tris, W, H, colors=colors)
+    return scenes.build_from_vertex_records(ctx, rec, W, H, debug=True, mode=mode, color_type=color_type,
+                                            shader_kw=dict(cull=A.SL_CULL_OFF, depth_test=A.SL_DEPTH_TEST_OFF,
+                                                           depth_mask=A.SL_DEPTH_MASK_ON, blend=blend))
+
+
+def counts_of(img):
+    ys, xs = np.nonzero(img)
+    return {(int(x), int(y)): int(img[y, x]) for x, y in zip(xs, ys)}
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_right_triangle_outline(kind, make_ctx):
+    ctx = make_ctx(kind)
+    n = 16
+    base = [(4, 4), (4 + n, 4), (4, 4 + n)]
+    expect = {(x, 4): 1 for x in range(4, 4 + n)}                      # bottom edge, whole
+    for y in range(5, 4 + n):
+        expect[(4, y)] = 1                                            # vertical edge
+        expect[(4 + n - (y - 4) - 1, y)] = 1                          # diagonal edge
+    assert len(expect) == n + 2 * (n - 2) + 1
+    for perm in itertools.permutations(base):
+        out = render(ctx, wire_scene(ctx, [list(perm)], 40, 40))
+        assert counts_of(out["count"]) == expect, perm
+        assert out["stats"]["fragments_tested"] == len(expect)
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_rows_match_exact_model_and_lie_inside_the_fill(kind, make_ctx):
+    ctx = make_ctx(kind)
+    rng = np.random.default_rng(1)
+    W, H = 48, 40
+    doubles = 0
+    for trial in range(50):
+        tri = [tuple(int(v) for v in rng.integers(0, [W + 1, H + 1])) for _ in range(3)]
+        got = counts_of(render(ctx, wire_scene(ctx, [tri], W, H))["count"])
+        exp, amb = wire_visits(tri)
+        rows = {y for _, y in got} | {y for _, y in exp}
+        for y in sorted(rows - amb):
+            g = {k: v for k, v in got.items() if k[1] == y}
+            e = {k: v for k, v in exp.items() if k[1] == y}
+            assert g == e, (trial, tri, y, sorted(g.items()), sorted(e.items()))
+        doubles += sum(1 for k, v in exp.items() if v > 1 and k[1] not in amb)
+
+        # against the filled triangle, whatever the runs are
+        fill = render(ctx, wire_scene(ctx, [tri], W, H, mode=A.RENDER_MODE_TRIANGLES))["count"]
+        filled = counts_of(fill)
+        assert set(got) <= set(filled), (trial, tri)
+        for y in {y for _, y in filled}:
+            xs = [x for x, yy in filled if yy == y]
+            assert (min(xs), y) in got and (max(xs), y) in got, (trial, tri, y)
+        cov, camb = exact_coverage(tri, W, H)
+        assert (set(filled) ^ cov) <= camb
+    assert doubles >= 3
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_double_visit_blends_twice(kind, make_ctx):
+    ctx = make_ctx(kind)
+    W, H = 48, 40
+    tri = [(6, 18), (47, 5), (18, 16)]
+    exp, amb = wire_visits(tri)
+    twice = [k for k, v in exp.items() if v == 2 and k[1] not in amb]
+    assert twice
+    col = (40, 30, 20, 255)
+    imgs = []
+    for blend in (A.SL_BLEND_OFF, A.SL_BLEND_ADDITIVE):
+        s = wire_scene(ctx, [tri], W, H, blend=blend, colors=[col] * 3, color_type=A.SL_COLOR_RGBA_FLOAT)
+        s.clear_rgba = (0.0, 0.0, 0.0, 0.0)
+        imgs.append(render(ctx, s))
+    single, added = imgs[0]["color"], imgs[1]["color"]
+    once = [k for k, v in exp.items() if v == 1 and k[1] not in amb]
+    for (x, y) in once:
+        assert np.all(single[y, x, :3] > 0)
+        # additive onto a zero clear: src * alpha (alpha within an ulp of 1)
+        np.testing.assert_allclose(added[y, x, :3], single[y, x, :3], rtol=1e-6, atol=0)
+    for (x, y) in twice:
+        assert imgs[1]["count"][y, x] == 2
+        # the pixel's colour added twice
+        np.testing.assert_allclose(added[y, x, :3], 2 * single[y, x, :3], rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_draw_multiple_uses_the_first_mesh_mode(kind, make_ctx):
+    ctx = make_ctx(kind)
+    W, H = 64, 48
+    tris = [[(2, 3), (40, 9), (12, 40)], [(30, 2), (62, 20), (41, 44)]]
+    rec = vertex_records(tris, W, H)
+
+    def run(modes, multiple):
+        s = scenes.build_from_vertex_records(ctx, rec, W, H, debug=True,
+                                             shader_kw=dict(cull=A.SL_CULL_OFF, depth_test=A.SL_DEPTH_TEST_OFF))
+        vao = s.draws[0][1].vaoId
+        ibo = ctx.create_ibo(np.arange(6, dtype=np.uint32))
+        ctx.vao_set_index_buffer(vao, ibo)
+        meshes = [A.sl_mesh(vao, 3 * i, 3 * i + 3, m, 0) for i, m in enumerate(modes)]
+        s.draws = [("multiple", meshes, 0)] if multiple else [("draw", m, 0) for m in meshes]
+        return counts_of(render(ctx, s)["count"])
+
+    wire2 = run([A.RENDER_MODE_TRI_WIRE, A.RENDER_MODE_TRI_WIRE], False)
+    fill2 = run([A.RENDER_MODE_TRIANGLES, A.RENDER_MODE_TRIANGLES], False)
+    assert wire2 != fill2
+    assert run([A.RENDER_MODE_TRI_WIRE, A.RENDER_MODE_TRIANGLES], True) == wire2
+    assert run([A.RENDER_MODE_TRIANGLES, A.RENDER_MODE_INDEXED_TRI_WIRE], True) == fill2
+    assert run([A.RENDER_MODE_INDEXED_TRI_WIRE, A.RENDER_MODE_INDEXED_TRIANGLES], True) == wire2
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_visibility_refused(kind, make_ctx):
+    ctx = make_ctx(kind)
+    W, H = 32, 32
+    s = wire_scene(ctx, [[(1, 1), (30, 4), (9, 29)]], W, H)
+    vis = ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, W, H)
+    ctx.attach_visibility(s.fbo, vis)
+    with pytest.raises(A.SLError):
+        s.render()
+        ctx.finish()
+
+
+def test_oracle_threads_match_single_thread(oracle_lib):
+    outs = []
+    for n in (1, 4):
+        ctx = A.SLContext(oracle_lib)
+        ctx.num_threads(n)
+        s = scenes.build_random_triangles(ctx, seed=11, ntri=400, W=160, H=120, debug=True, wire=True)
+        outs.append(render(ctx, s))
+    assert_parity(*outs)
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.gpu
+def test_gpu_random_triangles(gpu_lib, oracle_lib):
+    g, o = render_both(gpu_lib, oracle_lib, scenes.build_random_triangles, seed=11, ntri=3000, W=640, H=480,
+                       debug=True, wire=True)
+    assert_parity(g, o)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("config", range(10))
+def test_gpu_spheres_every_configuration(gpu_lib, oracle_lib, config):
+    def build(ctx, **kw):
+        ctx.set_raster_config(config)
+        return scenes.build_sphere_scene(ctx, **kw)
+    g, o = render_both(gpu_lib, oracle_lib, build, seed=2, n_instances=6, stacks=24, sectors=24, W=333, H=199,
+                       tex_size=64, debug=True, wire=True)
+    assert_parity(g, o)
+
+
+@pytest.mark.gpu
+def test_gpu_batched_with_wire_meshes(gpu_lib, oracle_lib):
+    g, o = render_both(gpu_lib, oracle_lib, scenes.build_batched_scene, seed=5, n_meshes=120, median_tris=400,
+                       W=640, H=360, n_textures=4, tex_size=64, debug=True, wire_every=3)
+    assert_parity(g, o)
