@@ -272,15 +272,29 @@ int prim_of(int mode)
     {
         case RENDER_MODE_TRIANGLES: case RENDER_MODE_INDEXED_TRIANGLES: return kPrimFill;
         case RENDER_MODE_TRI_WIRE: case RENDER_MODE_INDEXED_TRI_WIRE: return kPrimWire;
+        case RENDER_MODE_LINES: case RENDER_MODE_INDEXED_LINES: return kPrimLine;
+        case RENDER_MODE_POINTS: case RENDER_MODE_INDEXED_POINTS: return kPrimPoint;
         default: return kPrimBad;
     }
 }
 
+// SL_TriProcessor.cpp:682,697; SL_LineProcessor.cpp:204; SL_PointProcessor.cpp:79
 bool mesh_indexed(int prim, int mode)
 {
     if (prim == kPrimFill || prim == kPrimWire) return mode == RENDER_MODE_INDEXED_TRIANGLES || mode == RENDER_MODE_INDEXED_TRI_WIRE;
-    return false;
+    if (prim == kPrimLine) return mode == RENDER_MODE_INDEXED_LINES;
+    return mode == RENDER_MODE_INDEXED_POINTS;
 }
+
+size_t verts_per_prim(int prim)
+{
+    return prim == kPrimPoint ? 1u : prim == kPrimLine ? 2u : 3u;
+}
+
+// LS_EPSILON (LightSetup, absent).  The line clipper compares differences of
+// snapped screen coordinates (whole numbers) against it, so any value in
+// (0, 1] gives the same results.
+constexpr float kLineEpsilon = 1.0e-5f;
 
 /*-----------------------------------------------------------------------------
  * Textures and samplers (SL_Sampler.hpp:51-218, SL_Texture.hpp:190)
@@ -608,6 +622,29 @@ inline void scanline_step(const Scanline& s, float yf, int32_t& xMin, int32_t& x
     xMax = (lo >= hi) ? lo : hi;
 }
 
+// A fragment that passed the depth test, with its varyings: the fragment
+// program, the colour output and the depth write (SL_FragmentProcessor.cpp:
+// 365-402, 496-530; SL_PointRasterizer.cpp:64-96).
+void finish_fragment(Draw& d, int x, int y, float z, const V4* var, uint32_t seq)
+{
+    V4 out;
+    const bool haveOutputs = run_fragment_program(d, var, out);
+    if (haveOutputs && d.sh->nout > 0) place_color(d, x, y, out);
+
+    const size_t idx = (size_t)x + (size_t)d.tgt.W * (size_t)y;
+    if (d.sh->dmask == SL_DEPTH_MASK_ON) d.tgt.depth[idx] = z;   // regardless of the shader's return
+    if (d.tgt.dbgLast) d.tgt.dbgLast[idx] = seq;
+    if (d.tgt.dbgCount) d.tgt.dbgCount[idx] += 1u;
+    if (d.tgt.vis)
+    {
+        // the source triangle's vertex weights, interpolated like a varying
+        float* v = d.tgt.vis + idx * 4;
+        std::memcpy(&v[0], &seq, 4);
+        for (int k = 0; k < 3; ++k) v[1 + k] = var[d.visSlot].v[k];
+    }
+    ++d.shaded;
+}
+
 void shade_fragment(Draw& d, const Bin& b, int x, int y, float z, const float bc[4])
 {
     // perspective correction, generic branch SL_FragmentProcessor.cpp:486-494
@@ -623,22 +660,7 @@ void shade_fragment(Draw& d, const Bin& b, int x, int y, float z, const float bc
             var[i].v[k] = ((b.var[0][i].v[k] * c0) + (b.var[1][i].v[k] * c1)) + (b.var[2][i].v[k] * c2);
     for (unsigned i = d.nvary; i < SL_SHADER_MAX_VARYING_VECTORS; ++i) var[i] = v4(0.f, 0.f, 0.f, 0.f);
 
-    V4 out;
-    const bool haveOutputs = run_fragment_program(d, var, out);
-    if (haveOutputs && d.sh->nout > 0) place_color(d, x, y, out);
-
-    const size_t idx = (size_t)x + (size_t)d.tgt.W * (size_t)y;
-    if (d.sh->dmask == SL_DEPTH_MASK_ON) d.tgt.depth[idx] = z;   // :526-529, regardless of the shader's return
-    if (d.tgt.dbgLast) d.tgt.dbgLast[idx] = b.seq;
-    if (d.tgt.dbgCount) d.tgt.dbgCount[idx] += 1u;
-    if (d.tgt.vis)
-    {
-        // the source triangle's vertex weights, interpolated like a varying
-        float* v = d.tgt.vis + idx * 4;
-        std::memcpy(&v[0], &b.seq, 4);
-        for (int k = 0; k < 3; ++k) v[1 + k] = var[d.visSlot].v[k];
-    }
-    ++d.shaded;
+    finish_fragment(d, x, y, z, var, b.seq);
 }
 
 // d0 / d1 of render_wireframe: max(|a - b|, 1) on int32.  The row above a
@@ -701,6 +723,96 @@ void raster_wire_bin(Draw& d, const Bin& b, int32_t rowThread, int32_t numRowThr
     }
 }
 
+// Pixel (x, y) of a point or line, addressed as the reference addresses its
+// attachments: texel x + W*y (SL_Framebuffer.cpp:33-38, SL_PointRasterizer.cpp:
+// 56,94).  A snapped coordinate can equal W (NDC +1, or a line clipped to the
+// viewport's right edge): column W of row y is texel 0 of row y + 1.  False
+// for an address at or past W*H, which the reference writes outside the
+// attachment's texels (into its allocation padding); those fragments are
+// dropped here, and so are columns beyond W (only reachable by a line end
+// that float cancellation in the clipper moved past the viewport edge).
+bool linear_pixel(const Draw& d, int x, int y, int& px, int& py)
+{
+    if (x > d.tgt.W) return false;
+    const size_t idx = (size_t)x + (size_t)d.tgt.W * (size_t)y;
+    if (idx >= (size_t)d.tgt.W * (size_t)d.tgt.H) return false;
+    px = (int)(idx % (size_t)d.tgt.W);
+    py = (int)(idx / (size_t)d.tgt.W);
+    return true;
+}
+
+// SL_PointRasterizer::render_point (SL_PointRasterizer.cpp:27-98): varyings
+// pass to the fragment program unchanged.
+void raster_point_bin(Draw& d, const Bin& b, int32_t rowThread, int32_t numRowThreads)
+{
+    const uint16_t x = (uint16_t)f2i(b.p[0].v[0]);
+    const uint16_t y = (uint16_t)f2i(b.p[0].v[1]);
+    int px, py;
+    if (!linear_pixel(d, x, y, px, py)) return;
+    // rows are split over threads by the addressed texel's row (the
+    // reference splits by y, which lets two threads race on a wrapped texel)
+    if (py % numRowThreads != rowThread) return;
+    const float z = b.p[0].v[2];
+    ++d.tested;
+    if (!depth_func(d.sh->dtest, z, d.tgt.depth[(size_t)px + (size_t)d.tgt.W * (size_t)py])) return;
+    V4 var[SL_SHADER_MAX_VARYING_VECTORS];
+    for (unsigned i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i) var[i] = (i < d.nvary) ? b.var[0][i] : v4(0.f, 0.f, 0.f, 0.f);
+    finish_fragment(d, px, py, z, var, b.seq);
+}
+
+// SL_LineRasterizer::render_line (SL_LineRasterizer.cpp:27-80) over
+// sl_draw_line_bresenham (SL_Geometry.hpp:314-366, 16-bit integers): each
+// pixel's position along the line is its distance from the first end times
+// the inverse length; depth and varyings are mixed linearly by it
+// (interpolate_line_varyings, SL_FragmentProcessor.cpp:119-145).
+void raster_line_bin(Draw& d, const Bin& b, int32_t rowThread, int32_t numRowThreads)
+{
+    const float z0 = b.p[0].v[2], z1 = b.p[1].v[2];
+    const float s0x = b.p[0].v[0], s0y = b.p[0].v[1];
+    const float ex = b.p[1].v[0] - s0x, ey = b.p[1].v[1] - s0y;
+    const float dist = 1.f / sqrtf((ex * ex) + (ey * ey));   // inversesqrt(length_squared(sc1 - sc0))
+
+    auto pixel = [&](uint16_t x, uint16_t y) {
+        int qx, qy;
+        if (!linear_pixel(d, x, y, qx, qy)) return;
+        if (qy % numRowThreads != rowThread) return;   // by the addressed texel's row, as for points
+        const float px = (float)x - s0x, py = (float)y - s0y;
+        const float len = sqrtf(((px * px) + (py * py)) + (0.f * 0.f) + (0.f * 0.f));   // length(p - p0)
+        const float interp = len * dist;
+        const float z = z0 + interp * (z1 - z0);                                     // mix(z0, z1, interp)
+        ++d.tested;
+        if (!depth_func(d.sh->dtest, z, d.tgt.depth[(size_t)qx + (size_t)d.tgt.W * (size_t)qy])) return;
+        V4 var[SL_SHADER_MAX_VARYING_VECTORS];
+        for (unsigned i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+            var[i] = (i < d.nvary) ? mix4(b.var[0][i], b.var[1][i], interp) : v4(0.f, 0.f, 0.f, 0.f);
+        finish_fragment(d, qx, qy, z, var, b.seq);
+    };
+
+    // (uint16_t) casts of the screen coordinates, passed as int16_t
+    int16_t x1 = (int16_t)(uint16_t)f2i(s0x), y1 = (int16_t)(uint16_t)f2i(s0y);
+    int16_t x2 = (int16_t)(uint16_t)f2i(b.p[1].v[0]), y2 = (int16_t)(uint16_t)f2i(b.p[1].v[1]);
+    const bool steep = std::abs(x1 - x2) < std::abs(y1 - y2);
+    if (steep) { std::swap(x1, y1); std::swap(x2, y2); }
+    if (x1 > x2) { std::swap(x1, x2); std::swap(y1, y2); }
+    const int16_t dx = (int16_t)(x2 - x1);
+    const int16_t dy = (int16_t)(y2 - y1);
+    const int16_t dErr = (int16_t)(std::abs(dy) * 2);
+    const int16_t yErr = (y2 > y1) ? 1 : -1;
+    int16_t err = 0;
+    int16_t y = y1;
+    for (int16_t x = x1; x <= x2; ++x)
+    {
+        if (steep) pixel((uint16_t)y, (uint16_t)x);
+        else pixel((uint16_t)x, (uint16_t)y);
+        err = (int16_t)(err + dErr);
+        if (err > dx)
+        {
+            y = (int16_t)(y + yErr);
+            err = (int16_t)(err - 2 * dx);
+        }
+    }
+}
+
 // SL_TriRasterizer::render_triangle_simd, generic branch (:806-923), for the
 // rows y = minY + offset + k*step owned by one thread.
 void raster_bin(Draw& d, const Bin& b, int32_t rowThread, int32_t numRowThreads)
@@ -708,6 +820,16 @@ void raster_bin(Draw& d, const Bin& b, int32_t rowThread, int32_t numRowThreads)
     if (d.prim == kPrimWire)
     {
         raster_wire_bin(d, b, rowThread, numRowThreads);
+        return;
+    }
+    if (d.prim == kPrimLine)
+    {
+        raster_line_bin(d, b, rowThread, numRowThreads);
+        return;
+    }
+    if (d.prim == kPrimPoint)
+    {
+        raster_point_bin(d, b, rowThread, numRowThreads);
         return;
     }
     const V4* P = b.p;
@@ -937,6 +1059,129 @@ struct VertexStage
         }
     }
 
+    // SL_PointProcessor::process_verts (SL_PointProcessor.cpp:64-113): one
+    // vertex, visible when w > 0 and every component lies in [-w, w]
+    // (sl_ndc_clip_status, SL_PostVertexTransform.hpp:536-554)
+    void process_point(uint32_t seq, size_t vid, size_t instanceId)
+    {
+        TVert v;
+        for (auto& x : v.var) x = v4(0.f, 0.f, 0.f, 0.f);
+        v.pos = mat_mul(d.scissorMat, run_vertex_program(d, vid, instanceId, v.var));
+        const float w = v.pos.v[3];
+        bool in = w > 0.f;
+        for (int k = 0; k < 4; ++k) in = in && v.pos.v[k] <= w && v.pos.v[k] >= -w;
+        if (!in) return;
+        divide_and_snap(v.pos, d.vpDims);
+        Bin bin;
+        bin.p[0] = v.pos;
+        for (unsigned i = 0; i < d.nvary; ++i) bin.var[0][i] = v.var[i];
+        bin.seq = seq;
+        emit(bin);
+    }
+
+    void push_line(uint32_t seq, const TVert& a, const TVert& b)   // SL_LineProcessor::push_bin (:26-62)
+    {
+        Bin bin;
+        bin.p[0] = a.pos;
+        bin.p[1] = b.pos;
+        for (unsigned i = 0; i < d.nvary; ++i)
+        {
+            bin.var[0][i] = a.var[i];
+            bin.var[1][i] = b.var[i];
+        }
+        bin.seq = seq;
+        emit(bin);
+    }
+
+    // SL_LineProcessor::clip_and_process_lines (:69-169): Liang-Barsky in
+    // screen space against viewportDims read as (xMin, yMin, xMax, yMax),
+    // i.e. the rectangle [x, w] x [y, h] of the viewport rect (x, y, w, h).
+    void clip_line(uint32_t seq, const TVert& a, const TVert& b)
+    {
+        const float dx = b.pos.v[0] - a.pos.v[0];
+        const float dy = b.pos.v[1] - a.pos.v[1];
+        const float xMin = d.vpDims[0], yMin = d.vpDims[1], xMax = d.vpDims[2], yMax = d.vpDims[3];
+        if (fabsf(dx) < kLineEpsilon && fabsf(dy) < kLineEpsilon)
+        {
+            if (a.pos.v[0] >= xMin && a.pos.v[0] <= xMax && a.pos.v[1] >= yMin && a.pos.v[1] <= yMax)
+            {
+                push_line(seq, a, b);
+                return;
+            }
+        }
+        auto clip_segment = [](float num, float denom, float& tE, float& tL) -> bool {
+            if (fabsf(denom) < kLineEpsilon) return num < 0.f;
+            const float t = num / denom;
+            if (denom > 0.f)
+            {
+                if (t > tL) return false;
+                if (t > tE) tE = t;
+            }
+            else
+            {
+                if (t < tE) return false;
+                if (t < tL) tL = t;
+            }
+            return true;
+        };
+        TVert p0 = a, p1 = b;
+        const float ex = p0.pos.v[0] - p1.pos.v[0], ey = p0.pos.v[1] - p1.pos.v[1];
+        const float dist = 1.f / sqrtf((ex * ex) + (ey * ey));   // inversesqrt(length_squared(...))
+        float tE = 0.f, tL = 1.f;
+        const float x1 = p0.pos.v[0], y1 = p0.pos.v[1];
+        if (clip_segment(xMin - x1, dx, tE, tL) && clip_segment(x1 - xMax, -dx, tE, tL) &&
+            clip_segment(yMin - y1, dy, tE, tL) && clip_segment(y1 - yMax, -dy, tE, tL))
+        {
+            if (tL < 1.f)
+            {
+                p1.pos.v[0] = x1 + tL * dx;
+                p1.pos.v[1] = y1 + tL * dy;
+            }
+            if (tE > 0.f)
+            {
+                p0.pos.v[0] = x1 + tE * dx;
+                p0.pos.v[1] = y1 + tE * dy;
+            }
+            V4 d0, d1;
+            for (int k = 0; k < 4; ++k) { d0.v[k] = p0.pos.v[k] - a.pos.v[k]; d1.v[k] = p1.pos.v[k] - b.pos.v[k]; }
+            const float interp0 = sqrtf(dot4(d0, d0)) * dist;   // length(p0 - a) * dist
+            const float interp1 = sqrtf(dot4(d1, d1)) * dist;
+            p0.pos.v[2] = a.pos.v[2] + interp0 * (b.pos.v[2] - a.pos.v[2]);
+            p0.pos.v[3] = a.pos.v[3] + interp0 * (b.pos.v[3] - a.pos.v[3]);
+            p1.pos.v[2] = b.pos.v[2] + interp1 * (a.pos.v[2] - b.pos.v[2]);
+            p1.pos.v[3] = b.pos.v[3] + interp1 * (a.pos.v[3] - b.pos.v[3]);
+            for (unsigned i = 0; i < d.nvary; ++i)
+            {
+                p0.var[i] = mix4(a.var[i], b.var[i], interp0);
+                p1.var[i] = mix4(b.var[i], a.var[i], interp1);
+            }
+            push_line(seq, p0, p1);
+        }
+    }
+
+    // SL_LineProcessor::process_verts (:196-276)
+    void process_line(uint32_t seq, size_t vid0, size_t vid1, size_t instanceId)
+    {
+        TVert v[2];
+        const size_t ids[2] = {vid0, vid1};
+        for (int k = 0; k < 2; ++k)
+        {
+            for (auto& x : v[k].var) x = v4(0.f, 0.f, 0.f, 0.f);
+            v[k].pos = mat_mul(d.scissorMat, run_vertex_program(d, ids[k], instanceId, v[k].var));
+        }
+        if (v[0].pos.v[3] < 0.f || v[1].pos.v[3] < 0.f) return;
+        // sl_ndc_clip_status, two vertices (SL_PostVertexTransform.hpp:560-587)
+        bool all = true;
+        for (int j = 0; j < 2; ++j)
+            for (int k = 0; k < 4; ++k) all = all && v[j].pos.v[k] <= v[j].pos.v[3] && v[j].pos.v[k] >= -v[j].pos.v[3];
+        const int status = (all ? 3 : 0) | ((v[0].pos.v[3] > 0.f || v[1].pos.v[3] > 0.f) ? 1 : 0);
+        if (status == 0) return;
+        divide_and_snap(v[0].pos, d.vpDims);
+        divide_and_snap(v[1].pos, d.vpDims);
+        if (status == 3) push_line(seq, v[0], v[1]);
+        else clip_line(seq, v[0], v[1]);
+    }
+
     void process_triangle(uint32_t seq, size_t vid0, size_t vid1, size_t vid2, size_t instanceId)
     {
         TVert tv[3];
@@ -979,6 +1224,24 @@ size_t index_at(const Ibo& ibo, size_t i)
         case VERTEX_DATA_SHORT: { uint16_t v; std::memcpy(&v, p, 2); return v; }
         default: { uint32_t v; std::memcpy(&v, p, 4); return v; }
     }
+}
+
+// One (mesh, instance) pass of a draw call, as the reference loops them in
+// SL_TriProcessor::execute (SL_TriProcessor.cpp:677-713).
+struct Pass { const sl_mesh* mesh; size_t instance; size_t triBase; size_t numTris; uint32_t seqBase; };
+
+// Primitive g of a pass: its element indices, through the index buffer when
+// the mesh's mode says so, and the processor of the draw's kind.
+void process_prim(VertexStage& vs, const Draw& d, const Pass& P, size_t g)
+{
+    const size_t t = g - P.triBase;
+    const uint32_t seq = P.seqBase + (uint32_t)t + 1u;
+    const bool indexed = mesh_indexed(d.prim, P.mesh->mode);
+    auto vid = [&](size_t e) { return indexed ? index_at(*d.ibo, e) : e; };
+    const size_t i = P.mesh->elementBegin + verts_per_prim(d.prim) * t;
+    if (d.prim == kPrimPoint) vs.process_point(seq, vid(i), P.instance);
+    else if (d.prim == kPrimLine) vs.process_line(seq, vid(i), vid(i + 1), P.instance);
+    else vs.process_triangle(seq, vid(i), vid(i + 1), vid(i + 2), P.instance);
 }
 
 // SL_ViewportState::scissor_matrix / viewport_rect (SL_ViewportState.cpp:17-111)
@@ -1070,9 +1333,6 @@ int bind_mesh(sl_context* c, const sl_mesh& m, Draw& d)
  * SL_VertexProcessor.cpp:84-181).  Bins are merged in submission order before
  * rasterisation, so the output equals the single-threaded one.
 -----------------------------------------------------------------------------*/
-// One (mesh, instance) pass of a draw call, as the reference loops them in
-// SL_TriProcessor::execute (SL_TriProcessor.cpp:677-713).
-struct Pass { const sl_mesh* mesh; size_t instance; size_t triBase; size_t numTris; uint32_t seqBase; };
 
 size_t index_at(const Ibo& ibo, size_t i);
 int bind_mesh(sl_context* c, const sl_mesh& m, Draw& d);
@@ -1130,13 +1390,7 @@ void run_draw_threads(sl_context* c, Draw& proto, const std::vector<Pass>& passe
                     bind_mesh(c, *P.mesh, d);
                     VertexStage vs(d);
                     vs.out = &perThread[t];
-                    const size_t i = P.mesh->elementBegin + 3 * (g - P.triBase);
-                    size_t a = i, b = i + 1, e = i + 2;
-                    if (mesh_indexed(d.prim, P.mesh->mode))
-                    {
-                        a = index_at(*d.ibo, i); b = index_at(*d.ibo, i + 1); e = index_at(*d.ibo, i + 2);
-                    }
-                    vs.process_triangle(P.seqBase + (uint32_t)(g - P.triBase) + 1u, a, b, e, P.instance);
+                    process_prim(vs, d, P, g);
                 }
             }
             bar.wait();
@@ -1190,7 +1444,9 @@ int run_draws(sl_context* c, const sl_mesh* meshes, size_t numMeshes, size_t num
         const sl_mesh& m = (numInstances == 1) ? meshes[it] : meshes[0];
         rc = bind_mesh(c, m, proto);
         if (rc != SL_OK) return rc;
-        const size_t n = (m.elementEnd > m.elementBegin) ? (m.elementEnd - m.elementBegin) / 3 : 0;   // (:569-574)
+        // whole primitives only (SL_TriProcessor.cpp:569-574; a trailing
+        // partial line or triangle is skipped)
+        const size_t n = (m.elementEnd > m.elementBegin) ? (m.elementEnd - m.elementBegin) / verts_per_prim(proto.prim) : 0;
         passes.push_back(Pass{&m, (numInstances == 1) ? 0 : it, total, n, c->triSeq});
         if (proto.tgt.vis && n > 0)
         {
@@ -1219,14 +1475,7 @@ int run_draws(sl_context* c, const sl_mesh* meshes, size_t numMeshes, size_t num
         {
             bind_mesh(c, *P.mesh, proto);
             VertexStage vs(proto);
-            const bool indexed = mesh_indexed(proto.prim, P.mesh->mode);
-            for (size_t t = 0; t < P.numTris; ++t)
-            {
-                const size_t i = P.mesh->elementBegin + 3 * t;
-                size_t a = i, b = i + 1, e = i + 2;
-                if (indexed) { a = index_at(*proto.ibo, i); b = index_at(*proto.ibo, i + 1); e = index_at(*proto.ibo, i + 2); }
-                vs.process_triangle(P.seqBase + (uint32_t)t + 1u, a, b, e, P.instance);
-            }
+            for (size_t t = 0; t < P.numTris; ++t) process_prim(vs, proto, P, P.triBase + t);
         }
     }
     c->stats.fragments_tested += proto.tested;

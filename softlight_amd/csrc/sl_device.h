@@ -87,15 +87,25 @@ SL_HD uint32_t prim_of_mode(int mode)
     {
         case RENDER_MODE_TRIANGLES: case RENDER_MODE_INDEXED_TRIANGLES: return kPrimFill;
         case RENDER_MODE_TRI_WIRE: case RENDER_MODE_INDEXED_TRI_WIRE: return kPrimWire;
+        case RENDER_MODE_LINES: case RENDER_MODE_INDEXED_LINES: return kPrimLine;
+        case RENDER_MODE_POINTS: case RENDER_MODE_INDEXED_POINTS: return kPrimPoint;
         default: return kPrimBad;
     }
 }
 
+// SL_TriProcessor.cpp:682,697; SL_LineProcessor.cpp:204; SL_PointProcessor.cpp:79
 SL_HD bool mesh_indexed(uint32_t prim, int mode)
 {
     if (prim == kPrimFill || prim == kPrimWire) return mode == RENDER_MODE_INDEXED_TRIANGLES || mode == RENDER_MODE_INDEXED_TRI_WIRE;
-    return false;
+    if (prim == kPrimLine) return mode == RENDER_MODE_INDEXED_LINES;
+    return mode == RENDER_MODE_INDEXED_POINTS;
 }
+
+SL_HD uint32_t verts_per_prim(uint32_t prim) { return prim == kPrimPoint ? 1u : prim == kPrimLine ? 2u : 3u; }
+
+// Line draws walk 16-bit Bresenham coordinates (sl_lowp_t); targets up to
+// this size keep every walk's error term inside int16.
+constexpr int kMaxLineTarget = 8192;
 
 struct DrawDesc
 {
@@ -145,15 +155,49 @@ struct SetupRec
 };
 static_assert(sizeof(SetupRec) == 144, "SetupRec layout");
 
+// Point and line records reuse SetupRec's storage:
+//   point  x[0], y[0]  the texel it addresses (column, row); z[0] its depth
+//   line   x[0..1], y[0..1], z[0..1]  the two (clipped) screen ends and depths;
+//          bx[0]  1 / |end1 - end0|;  by[0] by[1] by[2] bz[0] bz[1] bz[2]
+//          (bit patterns of int32) the Bresenham walk: major-axis start,
+//          minor-axis start, major length dx, 2|dminor| (dErr), minor step
+//          (+-1), steep flag (major axis is y).
+// minY / maxY and xbox bound the texels they address, as for triangles.
+struct LineWalk { int32_t X1, Y1, dx, dErr, yErr, steep; };
+
+__device__ __forceinline__ LineWalk line_walk_of(const SetupRec& r)
+{
+    LineWalk w;
+    w.X1 = __float_as_int(r.by[0]);
+    w.Y1 = __float_as_int(r.by[1]);
+    w.dx = __float_as_int(r.by[2]);
+    w.dErr = __float_as_int(r.bz[0]);
+    w.yErr = __float_as_int(r.bz[1]);
+    w.steep = __float_as_int(r.bz[2]);
+    return w;
+}
+
+// Minor-axis offset (in steps of yErr) of step k of the walk: the number of
+// times `err += dErr; if (err > dx) { y += yErr; err -= 2 dx; }` has stepped
+// before pixel k (SL_Geometry.hpp:338-365), in closed form.
+SL_HD int32_t line_minor_steps(const LineWalk& w, int32_t k)
+{
+    const int32_t num = k * w.dErr - w.dx;
+    return (num > 0) ? (num + 2 * w.dx - 1) / (2 * w.dx) : 0;
+}
+
 // Depth function, depth mask, blend mode, "has colour outputs" and the
-// wireframe flag of a draw, packed into each of its records so the raster
+// primitive kind of a draw, packed into each of its records so the raster
 // loop reads no draw table.
 constexpr uint32_t kStateWire = 1u << 8;
+constexpr uint32_t kStateLine = 1u << 9;
+constexpr uint32_t kStatePoint = 1u << 10;
 
 __host__ __device__ inline uint32_t pack_raster_state(const DrawDesc& d)
 {
     return (uint32_t)(d.dtest & 7u) | ((uint32_t)(d.dmask & 1u) << 3) | ((uint32_t)(d.blend & 7u) << 4) |
-           ((d.nout > 0 ? 1u : 0u) << 7) | (d.prim == kPrimWire ? kStateWire : 0u);
+           ((d.nout > 0 ? 1u : 0u) << 7) | (d.prim == kPrimWire ? kStateWire : 0u) |
+           (d.prim == kPrimLine ? kStateLine : 0u) | (d.prim == kPrimPoint ? kStatePoint : 0u);
 }
 
 // d0 / d1 of SL_TriRasterizer::render_wireframe (SL_TriRasterizer.cpp:155-156):
@@ -269,7 +313,7 @@ struct RasterParams
     uint32_t clearRGBA8;
     float clearRGBAF[4];
     float clearDepthValue;
-    int32_t wire;                 // some draw of the batch is a wireframe draw
+    int32_t extra;                // some draw of the batch draws wireframe triangles, lines or points
     FrameCtrl* ctrl;
 };
 

@@ -9,7 +9,8 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, uint64_t n, hipStream_t s);
 hipError_t launch_setup(const SetupParams& p, uint64_t chunks, hipStream_t s);
 hipError_t launch_chunk_scan(const SetupParams& p, hipStream_t s);
-hipError_t launch_emit(const SetupParams& p, hipStream_t s);
+// tris / prims: the batch holds triangle draws / point or line draws.
+hipError_t launch_emit(const SetupParams& p, bool tris, bool prims, hipStream_t s);
 hipError_t launch_clip(const SetupParams& p, uint32_t jobs, hipStream_t s);
 // Binning splits the entries into at most kMaxBinBlocks blocks of `per`
 // entries (a multiple of 64, at least kMinBinBlock).

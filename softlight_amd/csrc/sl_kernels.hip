@@ -93,13 +93,16 @@ __device__ __forceinline__ void load_triangle_ids(uint64_t g, const DrawDesc& D,
     const uint64_t t = g - D.triBase;
     const uint64_t inst = t / D.trisPerInstance;
     const uint64_t ti = t - inst * D.trisPerInstance;
-    const uint64_t e = D.elementBegin + 3u * ti;
-    T.vid[0] = (uint32_t)e; T.vid[1] = (uint32_t)(e + 1); T.vid[2] = (uint32_t)(e + 2);
+    // a point repeats its vertex in all three slots, a line its second
+    const uint64_t per = verts_per_prim(D.prim);
+    const uint64_t e = D.elementBegin + per * ti;
+    const uint64_t e1 = (per > 1) ? e + 1 : e, e2 = (per > 2) ? e + 2 : e1;
+    T.vid[0] = (uint32_t)e; T.vid[1] = (uint32_t)e1; T.vid[2] = (uint32_t)e2;
     if (D.indexed)
     {
         T.vid[0] = (uint32_t)fetch_index(D, e);
-        T.vid[1] = (uint32_t)fetch_index(D, e + 1);
-        T.vid[2] = (uint32_t)fetch_index(D, e + 2);
+        T.vid[1] = (uint32_t)fetch_index(D, e1);
+        T.vid[2] = (uint32_t)fetch_index(D, e2);
     }
     T.inst = (uint32_t)inst;
     T.draw = di;
@@ -107,9 +110,35 @@ __device__ __forceinline__ void load_triangle_ids(uint64_t g, const DrawDesc& D,
 }
 
 // Cull, scissor matrix and clip status of a triangle whose clip-space
-// positions are in T.pos.
+// positions are in T.pos.  Points and lines are not culled; a point is
+// visible (3) when inside the clip volume (sl_ndc_clip_status, one vertex,
+// SL_PostVertexTransform.hpp:536-554); a line is dropped when a vertex has
+// w < 0, else gets the two-vertex status (:560-587; SL_LineProcessor.cpp:
+// 256-265).
+// kKinds: 1 = the draw draws triangles, 2 = points or lines, 3 = either.
+template <int kKinds = 3>
 __device__ __forceinline__ void classify_triangle(const DrawDesc& D, TriIn& T)
 {
+    if ((kKinds & 2) && (D.prim == kPrimPoint || D.prim == kPrimLine))
+    {
+        T.pos[0] = mat_mul(D.scissor, T.pos[0]);
+        T.pos[1] = mat_mul(D.scissor, T.pos[1]);
+        if (D.prim == kPrimPoint)
+        {
+            T.status = (T.pos[0].w > 0.f && clip_inside(T.pos[0])) ? 3 : 0;
+        }
+        else if (T.pos[0].w < 0.f || T.pos[1].w < 0.f)
+        {
+            T.status = 0;
+        }
+        else
+        {
+            T.status = ((clip_inside(T.pos[0]) && clip_inside(T.pos[1])) ? 3 : 0) |
+                       ((T.pos[0].w > 0.f || T.pos[1].w > 0.f) ? 1 : 0);
+        }
+        return;
+    }
+    if (!(kKinds & 1)) return;
     if (D.cull != SL_CULL_OFF)
     {
         const float det = face_determinant(T.pos[0], T.pos[1], T.pos[2]);
@@ -124,12 +153,14 @@ __device__ __forceinline__ void classify_triangle(const DrawDesc& D, TriIn& T)
 // Fetch, vertex positions, cull, scissor matrix and clip status of one
 // triangle (SL_TriProcessor::process_verts, SL_TriProcessor.cpp:574-631).
 // Varyings are left to load_varyings: most triangles never need them.
+template <int kKinds = 3>
 __device__ __forceinline__ void load_triangle_pos(const SetupParams& P, uint64_t g, const DrawDesc& D, uint32_t di, TriIn& T)
 {
     load_triangle_ids(g, D, di, T);
-    const bool all3[3] = {true, true, true};
-    vertex_positions3(D, P.uniforms, T.vid, T.inst, all3, T.pos);
-    classify_triangle(D, T);
+    const uint32_t per = verts_per_prim(D.prim);
+    const bool need[3] = {true, per > 1, per > 2};
+    vertex_positions3(D, P.uniforms, T.vid, T.inst, need, T.pos);
+    classify_triangle<kKinds>(D, T);
 }
 
 // Per-lane draw lookup (the clip kernel's jobs come from any draw).
@@ -137,7 +168,7 @@ __device__ __forceinline__ void load_triangle(const SetupParams& P, uint64_t g, 
 {
     const uint32_t di = find_draw(P.draws, P.numDraws, g);
     const DrawDesc& D = P.draws[di];
-    load_triangle_pos(P, g, D, di, T);
+    load_triangle_pos<1>(P, g, D, di, T);
     load_varyings(P, D, T);
 }
 
@@ -344,14 +375,194 @@ __device__ __forceinline__ uint32_t find_draw_wave(const DrawDesc* __restrict__ 
 // positions (T.pos after classify_triangle).  Both setup passes run this.
 struct TriCount
 {
-    float4 sp[3];              // snapped screen positions (unclipped triangles)
-    TileRect tr;               // tiles to bin to (unclipped triangles)
+    float4 sp[3];              // snapped screen positions (unclipped triangles; a line's two ends; a point's texel)
+    TileRect tr;               // tiles to bin to (unclipped triangles, points, lines)
+    TileRect tr2;              // a line's further tiles: column 0, rows its wrapped texels reach
+    float t0, t1;              // a clipped line's varying mix weights at its two ends
     uint32_t nrec, npair;
     bool clipped;              // needs k_clip (nrec fan triangles)
     bool empty;                // small triangle that covers no pixel: dropped, counted as binned
     bool needClip;             // straddles a clip plane: counted by count_clipped
+    bool lineClip;             // a line clipped to the viewport (varyings mixed by t0, t1)
 };
 
+// ---------------------------------------------------------------------------
+// Points and lines
+// ---------------------------------------------------------------------------
+// Texel (x, y) addresses attachment texel x + W*y (SL_Framebuffer.cpp:33-38):
+// a snapped x of W (NDC +1, or a line clipped to the viewport's right edge)
+// is texel 0 of row y + 1.  False at or past W*H, where the reference writes
+// outside the attachment's texels; those fragments are dropped.
+__device__ __forceinline__ bool linear_texel(int x, int y, int W, int H, int& px, int& py)
+{
+    const int64_t idx = (int64_t)x + (int64_t)W * y;
+    if (x < 0 || y < 0 || idx >= (int64_t)W * H) return false;
+    py = (int)(idx / W);
+    px = (int)(idx - (int64_t)py * W);
+    return true;
+}
+
+// LS_EPSILON (LightSetup, absent).  It is compared against differences of
+// snapped screen coordinates (whole numbers), so any value in (0, 1] gives
+// the same results.
+constexpr float kLineEpsilon = 1.0e-5f;
+
+__device__ __forceinline__ bool line_clip_segment(float num, float denom, float& tE, float& tL)
+{
+    if (fabsf(denom) < kLineEpsilon) return num < 0.f;
+    const float t = num / denom;
+    if (denom > 0.f)
+    {
+        if (t > tL) return false;
+        if (t > tE) tE = t;
+    }
+    else
+    {
+        if (t < tE) return false;
+        if (t < tL) tL = t;
+    }
+    return true;
+}
+
+// SL_LineProcessor::clip_and_process_lines (SL_LineProcessor.cpp:69-169):
+// Liang-Barsky in screen space against viewportDims read as (xMin, yMin,
+// xMax, yMax).  Returns false when nothing is left; a, b become the ends,
+// with depth and w mixed toward the other end by t0 / t1 when clipped.
+__device__ __forceinline__ bool clip_line(float4& a, float4& b, const float vp[4], float& t0, float& t1, bool& clipped)
+{
+    const float dx = b.x - a.x, dy = b.y - a.y;
+    const float xMin = vp[0], yMin = vp[1], xMax = vp[2], yMax = vp[3];
+    clipped = false;
+    t0 = t1 = 0.f;
+    if (fabsf(dx) < kLineEpsilon && fabsf(dy) < kLineEpsilon && a.x >= xMin && a.x <= xMax && a.y >= yMin && a.y <= yMax)
+        return true;
+    const float ex = a.x - b.x, ey = a.y - b.y;
+    const float dist = 1.f / sqrtf((ex * ex) + (ey * ey));
+    float tE = 0.f, tL = 1.f;
+    const float x1 = a.x, y1 = a.y;
+    if (!(line_clip_segment(xMin - x1, dx, tE, tL) && line_clip_segment(x1 - xMax, -dx, tE, tL) &&
+          line_clip_segment(yMin - y1, dy, tE, tL) && line_clip_segment(y1 - yMax, -dy, tE, tL)))
+        return false;
+    float4 p0 = a, p1 = b;
+    if (tL < 1.f)
+    {
+        p1.x = x1 + tL * dx;
+        p1.y = y1 + tL * dy;
+    }
+    if (tE > 0.f)
+    {
+        p0.x = x1 + tE * dx;
+        p0.y = y1 + tE * dy;
+    }
+    const float4 d0 = f4(p0.x - a.x, p0.y - a.y, p0.z - a.z, p0.w - a.w);
+    const float4 d1 = f4(p1.x - b.x, p1.y - b.y, p1.z - b.z, p1.w - b.w);
+    t0 = sqrtf(dot4(d0, d0)) * dist;   // length(p0 - a) * dist
+    t1 = sqrtf(dot4(d1, d1)) * dist;
+    p0.z = a.z + t0 * (b.z - a.z);
+    p0.w = a.w + t0 * (b.w - a.w);
+    p1.z = b.z + t1 * (a.z - b.z);
+    p1.w = b.w + t1 * (a.w - b.w);
+    a = p0;
+    b = p1;
+    clipped = true;
+    return true;
+}
+
+// The walk of sl_draw_line_bresenham (SL_Geometry.hpp:314-366) between the
+// (uint16_t) casts of two screen ends (SL_LineRasterizer.cpp:44-48).
+__device__ __forceinline__ LineWalk make_line_walk(float ax, float ay, float bx, float by)
+{
+    int x1 = (int16_t)(uint16_t)f2i(ax), y1 = (int16_t)(uint16_t)f2i(ay);
+    int x2 = (int16_t)(uint16_t)f2i(bx), y2 = (int16_t)(uint16_t)f2i(by);
+    const bool steep = abs(x1 - x2) < abs(y1 - y2);
+    int t;
+    if (steep)
+    {
+        t = x1; x1 = y1; y1 = t;
+        t = x2; x2 = y2; y2 = t;
+    }
+    if (x1 > x2)
+    {
+        t = x1; x1 = x2; x2 = t;
+        t = y1; y1 = y2; y2 = t;
+    }
+    LineWalk w;
+    w.X1 = x1;
+    w.Y1 = y1;
+    w.dx = x2 - x1;
+    w.dErr = abs(y2 - y1) * 2;
+    w.yErr = (y2 > y1) ? 1 : -1;
+    w.steep = steep ? 1 : 0;
+    return w;
+}
+
+// A line's addressed texels: those of its walk inside the target
+// (columns dx0..dx1, rows dy0..dy1, either range empty when none), and, when
+// the walk reaches column W, the wrapped ones in column 0, rows wy0..wy1.
+struct LineBox { int dx0, dx1, dy0, dy1, wy0, wy1; };
+
+__device__ __forceinline__ bool line_box(const LineWalk& w, int W, int H, LineBox& b)
+{
+    const int majLo = w.X1, majHi = w.X1 + w.dx;
+    const int minEnd = w.Y1 + w.yErr * (w.dErr / 2);
+    const int minLo = min(w.Y1, minEnd), minHi = max(w.Y1, minEnd);
+    const int vx0 = w.steep ? minLo : majLo, vx1 = w.steep ? minHi : majHi;
+    const int vy0 = w.steep ? majLo : minLo, vy1 = w.steep ? majHi : minHi;
+    b.dx0 = max(vx0, 0);
+    b.dx1 = min(vx1, W - 1);
+    b.dy0 = max(vy0, 0);
+    b.dy1 = min(vy1, H - 1);
+    b.wy0 = 1;
+    b.wy1 = 0;
+    if (vx1 >= W)
+    {
+        b.wy0 = max(vy0 + 1, 1);
+        b.wy1 = min(vy1 + 1, H - 1);
+    }
+    return (b.dx0 <= b.dx1 && b.dy0 <= b.dy1) || b.wy0 <= b.wy1;
+}
+
+__device__ __forceinline__ TileRect rect_tiles(int x0, int x1, int y0, int y1, int tilesX, int tilesY, int shx, int shy)
+{
+    TileRect r;
+    if (x0 > x1 || y0 > y1) { r.tx0 = 0; r.tx1 = -1; r.ty0 = 0; r.ty1 = -1; return r; }
+    r.tx0 = x0 >> shx; r.tx1 = min(x1 >> shx, tilesX - 1);
+    r.ty0 = y0 >> shy; r.ty1 = min(y1 >> shy, tilesY - 1);
+    return r;
+}
+
+__device__ __forceinline__ void count_point(const SetupParams& P, const DrawDesc& D, const TriIn& T, TriCount& C)
+{
+    if (T.status != 3) return;
+    const float4 s = divide_and_snap(T.pos[0], D.vp);
+    int px, py;
+    if (!linear_texel((uint16_t)f2i(s.x), (uint16_t)f2i(s.y), P.W, P.H, px, py)) return;
+    C.sp[0] = f4((float)px, (float)py, s.z, s.w);
+    C.tr = rect_tiles(px, px, py, py, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
+    C.nrec = 1;
+    C.npair = 1;
+}
+
+__device__ __forceinline__ void count_line(const SetupParams& P, const DrawDesc& D, const TriIn& T, TriCount& C)
+{
+    if (T.status == 0) return;
+    float4 a = divide_and_snap(T.pos[0], D.vp), b = divide_and_snap(T.pos[1], D.vp);
+    if (T.status != 3 && !clip_line(a, b, D.vp, C.t0, C.t1, C.lineClip)) return;
+    const LineWalk w = make_line_walk(a.x, a.y, b.x, b.y);
+    LineBox box;
+    if (!line_box(w, P.W, P.H, box)) return;
+    C.sp[0] = a;
+    C.sp[1] = b;
+    C.tr = rect_tiles(box.dx0, box.dx1, box.dy0, box.dy1, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
+    // wrapped texels: column 0, minus the tile rows tr already holds there
+    TileRect r2 = rect_tiles(0, 0, box.wy0, box.wy1, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
+    if (tile_count(C.tr) > 0 && C.tr.tx0 == 0) r2.ty0 = max(r2.ty0, C.tr.ty1 + 1);
+    C.tr2 = r2;
+    C.nrec = 1;
+    C.npair = tile_count(C.tr) + tile_count(C.tr2);
+}
+
+template <int kKinds = 3>
 __device__ __forceinline__ void count_triangle(const SetupParams& P, const DrawDesc& D, uint32_t drawIdx, const TriIn& T,
                                                TriCount& C)
 {
@@ -360,7 +571,19 @@ __device__ __forceinline__ void count_triangle(const SetupParams& P, const DrawD
     C.clipped = false;
     C.empty = false;
     C.needClip = false;
+    C.lineClip = false;
     C.tr = TileRect{0, -1, 0, -1};
+    if ((kKinds & 2) && D.prim == kPrimPoint)
+    {
+        count_point(P, D, T, C);
+        return;
+    }
+    if ((kKinds & 2) && D.prim == kPrimLine)
+    {
+        count_line(P, D, T, C);
+        return;
+    }
+    if (!(kKinds & 1)) return;
     if (T.status == 3)
     {
 #pragma unroll
@@ -429,6 +652,89 @@ __device__ __forceinline__ void count_clipped(const SetupParams& P, const DrawDe
         C.clipped = C.nrec > 0;
         C.needClip = false;
     }
+}
+
+// Record, varyings and bin entries of a point or line counted by
+// count_point / count_line (SL_PointProcessor::push_bin, SL_PointProcessor.cpp:
+// 26-58; SL_LineProcessor::push_bin, SL_LineProcessor.cpp:26-62).
+__device__ __forceinline__ void write_prim_record(const SetupParams& P, const DrawDesc& D, uint32_t recId, uint32_t pairOff,
+                                                  const TriIn& T, const TriCount& C)
+{
+    SetupRec r;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        r.x[k] = 0.f; r.y[k] = 0.f; r.z[k] = 0.f; r.hw[k] = 1.f;
+        r.bx[k] = 0.f; r.by[k] = 0.f; r.bz[k] = 0.f;
+    }
+    r.bc3 = 0.f;
+    r.v0y = r.v1y = r.v0x = r.v1x = 0.f;
+    r.p20y = r.p20x = r.p21xy = r.p10xy = 0.f;
+    r.draw = T.draw;
+    r.seq = T.seq;
+    r.state = pack_raster_state(D);
+    float4* V = P.vars + (uint64_t)recId * P.varStride;
+    const uint32_t nvMax = P.varStride / 3;
+    const int nv = D.nvary;
+    if (D.prim == kPrimPoint)
+    {
+        const int px = (int)C.sp[0].x, py = (int)C.sp[0].y;
+        r.x[0] = C.sp[0].x;
+        r.y[0] = C.sp[0].y;
+        r.z[0] = C.sp[0].z;
+        r.minY = py;
+        r.maxY = py + 1;
+        r.xbox = (uint32_t)px | ((uint32_t)px << 16);
+#pragma unroll
+        for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+            if (i < nv) V[i] = T.var[0].v[i];
+    }
+    else
+    {
+        const float4 a = C.sp[0], b = C.sp[1];
+        r.x[0] = a.x; r.y[0] = a.y; r.z[0] = a.z;
+        r.x[1] = b.x; r.y[1] = b.y; r.z[1] = b.z;
+        const float ex = b.x - a.x, ey = b.y - a.y;
+        r.bx[0] = 1.f / sqrtf((ex * ex) + (ey * ey));   // inversesqrt(length_squared(sc1 - sc0)), SL_LineRasterizer.cpp:39
+        const LineWalk w = make_line_walk(a.x, a.y, b.x, b.y);
+        r.by[0] = __int_as_float(w.X1);
+        r.by[1] = __int_as_float(w.Y1);
+        r.by[2] = __int_as_float(w.dx);
+        r.bz[0] = __int_as_float(w.dErr);
+        r.bz[1] = __int_as_float(w.yErr);
+        r.bz[2] = __int_as_float(w.steep);
+        LineBox box;
+        line_box(w, P.W, P.H, box);
+        const bool direct = box.dx0 <= box.dx1 && box.dy0 <= box.dy1, wrap = box.wy0 <= box.wy1;
+        r.minY = direct ? (wrap ? min(box.dy0, box.wy0) : box.dy0) : box.wy0;
+        r.maxY = (direct ? (wrap ? max(box.dy1, box.wy1) : box.dy1) : box.wy1) + 1;
+        const int xl = wrap ? 0 : box.dx0, xh = direct ? box.dx1 : 0;
+        r.xbox = (uint32_t)xl | ((uint32_t)xh << 16);
+        // a clipped end carries its varyings mixed toward the other end
+#pragma unroll
+        for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+        {
+            if (i < nv)
+            {
+                const float4 va = T.var[0].v[i], vb = T.var[1].v[i];
+                V[i] = C.lineClip ? mix4(va, vb, C.t0) : va;
+                V[nvMax + i] = C.lineClip ? mix4(vb, va, C.t1) : vb;
+            }
+        }
+    }
+    P.recs[recId] = r;
+    uint32_t o = pairOff;
+    auto put = [&](const TileRect& tr) {
+        for (int ty = tr.ty0; ty <= tr.ty1; ++ty)
+            for (int tx = tr.tx0; tx <= tr.tx1; ++tx)
+            {
+                P.pairKeys[o] = (uint32_t)(ty * P.tilesX + tx);
+                P.pairVals[o] = recId;
+                ++o;
+            }
+    };
+    put(C.tr);
+    if (D.prim == kPrimLine) put(C.tr2);
 }
 
 // 8 waves per SIMD: the register budget that keeps every wave slot usable
@@ -521,8 +827,9 @@ __global__ __launch_bounds__(kSetupBlock, 8) void k_setup(SetupParams P)
     }
     else if (live)
     {
-        const bool all3[3] = {true, true, true};
-        vertex_positions3(DU, P.uniforms, T.vid, T.inst, all3, T.pos);
+        const uint32_t per = verts_per_prim(DU.prim);
+        const bool need[3] = {true, per > 1, per > 2};
+        vertex_positions3(DU, P.uniforms, T.vid, T.inst, need, T.pos);
     }
     TriCount C;
     C.nrec = 0;
@@ -682,8 +989,15 @@ __global__ __launch_bounds__(kScanBlock) void k_chunk_scan(SetupParams P)
 constexpr int kEmitBlock = 256;
 static_assert(kEmitBlock == kSetupBlock, "a block of survivors spans whole staging rows");
 
+//
+// PRIMS selects the survivors a launch writes: false, those of triangle draws
+// (filled or wireframe); true, those of point and line draws.  A batch with
+// both kinds runs both launches; each writes its own records at the offsets
+// the chunk scan gave them.
+template <bool PRIMS>
 __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
 {
+    constexpr int kKinds = PRIMS ? 2 : 1;
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
@@ -763,6 +1077,8 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
             g = (uint64_t)chunk * kSetupBlock + (st & 0xFFu);
             di = I.draw;
             off = I.pre + ((st >> 8) & 0xFFFu) + ((st >> 20) << kRecBits);
+            const uint32_t pk = P.draws[di].prim;
+            if ((pk == kPrimPoint || pk == kPrimLine) != PRIMS) act2 = false;   // the other launch's survivor
         }
         // When the wave's survivors share one draw (the usual case) its
         // descriptor goes to scalar registers: uniform branches, scalar loads.
@@ -783,17 +1099,17 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
             __builtin_memcpy(&DR, wds, sizeof(DrawDesc));
             if (act2)
             {
-                load_triangle_pos(P, g, DR, di, T);
-                count_triangle(P, DR, di, T, C);
+                load_triangle_pos<kKinds>(P, g, DR, di, T);
+                count_triangle<kKinds>(P, DR, di, T, C);
             }
-            count_clipped(P, DR, T, C, sClip[wave]);
+            if (!PRIMS) count_clipped(P, DR, T, C, sClip[wave]);
         }
         else if (act2)
         {
             const DrawDesc& D = P.draws[di];
-            load_triangle_pos(P, g, D, di, T);
-            count_triangle(P, D, di, T, C);
-            count_clipped(P, D, T, C, sClip[wave]);
+            load_triangle_pos<kKinds>(P, g, D, di, T);
+            count_triangle<kKinds>(P, D, di, T, C);
+            if (!PRIMS) count_clipped(P, D, T, C, sClip[wave]);
         }
         bool ok = act2;
         if (ok && C.nrec == 0)
@@ -825,15 +1141,21 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
             if (uni)
             {
                 load_varyings(P, DR, T);
-                write_record(P, recOff, (uint32_t)pairOff, C.sp[0], C.sp[1], C.sp[2], T.var[0], T.var[1], T.var[2], DR.nvary,
-                             T.draw, T.seq, pack_raster_state(DR), C.tr);
+                if (PRIMS)
+                    write_prim_record(P, DR, recOff, (uint32_t)pairOff, T, C);
+                else
+                    write_record(P, recOff, (uint32_t)pairOff, C.sp[0], C.sp[1], C.sp[2], T.var[0], T.var[1], T.var[2],
+                                 DR.nvary, T.draw, T.seq, pack_raster_state(DR), C.tr);
             }
             else
             {
                 const DrawDesc& D = P.draws[di];
                 load_varyings(P, D, T);
-                write_record(P, recOff, (uint32_t)pairOff, C.sp[0], C.sp[1], C.sp[2], T.var[0], T.var[1], T.var[2], D.nvary,
-                             T.draw, T.seq, pack_raster_state(D), C.tr);
+                if (PRIMS)
+                    write_prim_record(P, D, recOff, (uint32_t)pairOff, T, C);
+                else
+                    write_record(P, recOff, (uint32_t)pairOff, C.sp[0], C.sp[1], C.sp[2], T.var[0], T.var[1], T.var[2],
+                                 D.nvary, T.draw, T.seq, pack_raster_state(D), C.tr);
             }
         }
         else if (wantJob)
@@ -1125,23 +1447,39 @@ __device__ __forceinline__ void record_barrier()
 // Shading of one fragment from its record and its barycentric values s0..s2
 // (perspective correction SL_FragmentProcessor.cpp:486-494, varyings :315-330,
 // then the fragment program).  `wv` receives the visibility slot's varying.
-__device__ __forceinline__ bool shade(const RasterParams& P, const DrawDesc& D, const float hw[3], float bc3, uint32_t rid,
-                                      float s0, float s1, float s2, float4& out, float4& wv)
+// A point's varyings pass unchanged (SL_PointRasterizer.cpp:66-69); a line's
+// are mixed by its position s0 (interpolate_line_varyings,
+// SL_FragmentProcessor.cpp:119-145).
+__device__ __forceinline__ bool shade(const RasterParams& P, const DrawDesc& D, const float hw[3], float bc3, uint32_t state,
+                                      uint32_t rid, float s0, float s1, float s2, float4& out, float4& wv)
 {
     const float t0 = s0 * hw[0], t1 = s1 * hw[1], t2 = s2 * hw[2];
     const float sinv = rcpf(((t0 + t1) + t2) + bc3);
     const float c0 = t0 * sinv, c1 = t1 * sinv, c2 = t2 * sinv;
     const float4* V = P.vars + (uint64_t)rid * P.varStride;
     const uint32_t nvMax = P.varStride / 3;
+    const bool point = (state & kStatePoint) != 0u, line = (state & kStateLine) != 0u;
     Vary4 var;
 #pragma unroll
     for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
     {
         if (i < D.nvary)
         {
-            const float4 a = V[i], b = V[nvMax + i], e = V[2 * nvMax + i];
-            var.v[i] = f4(((a.x * c0) + (b.x * c1)) + (e.x * c2), ((a.y * c0) + (b.y * c1)) + (e.y * c2),
-                          ((a.z * c0) + (b.z * c1)) + (e.z * c2), ((a.w * c0) + (b.w * c1)) + (e.w * c2));
+            const float4 a = V[i];
+            if (point)
+            {
+                var.v[i] = a;
+            }
+            else if (line)
+            {
+                var.v[i] = mix4(a, V[nvMax + i], s0);
+            }
+            else
+            {
+                const float4 b = V[nvMax + i], e = V[2 * nvMax + i];
+                var.v[i] = f4(((a.x * c0) + (b.x * c1)) + (e.x * c2), ((a.y * c0) + (b.y * c1)) + (e.y * c2),
+                              ((a.z * c0) + (b.z * c1)) + (e.z * c2), ((a.w * c0) + (b.w * c1)) + (e.w * c2));
+            }
         }
         else
         {
@@ -1208,14 +1546,14 @@ constexpr uint32_t kNoPending = 0xFFFFFFFFu;
 // pixels, so no workgroup barrier is needed between records.
 //
 // AUX bit 0: debug buffers (last primitive, depth-pass count); bit 1: the
-// visibility attachment; bit 2: the batch holds wireframe records (the
-// variants without it carry no wireframe code).
+// visibility attachment; bit 2: the batch holds wireframe, line or point
+// records (the variants without it carry only the filled-triangle code).
 template <int CF, int AUX, int TW, int TH, int SEGS>
 __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
 {
     constexpr bool DBG = (AUX & 1) != 0;
     constexpr bool VIS = (AUX & 2) != 0;
-    constexpr bool WIRE = (AUX & 4) != 0;
+    constexpr bool EXTRA = (AUX & 4) != 0;
     constexpr int NT = TH * 4 * SEGS;       // lanes: rows x segments x 4 phases
     constexpr int NW = NT / 64;             // waves, each owning TH / NW rows
     constexpr int SW = TW / SEGS;           // segment width (columns)
@@ -1268,7 +1606,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
         const DrawDesc& QD = P.draws[Q.draw];
         const float qhw[3] = {Q.hw[0], Q.hw[1], Q.hw[2]};
         float4 out, wv;
-        const bool have = shade(P, QD, qhw, Q.bc3, prid, sPendS[0][r][c], sPendS[1][r][c], sPendS[2][r][c], out, wv);
+        const bool have = shade(P, QD, qhw, Q.bc3, Q.state, prid, sPendS[0][r][c], sPendS[1][r][c], sPendS[2][r][c], out, wv);
         if (have) blend_into<CF>(&sCol8[CF == 1 ? r : 0][CF == 1 ? c : 0], &sColF[CF == 2 ? r : 0][CF == 2 ? c : 0], SL_BLEND_OFF, out);
         if (VIS) sVis[r][c] = make_float4(__uint_as_float(Q.seq), wv.x, wv.y, wv.z);
     };
@@ -1358,57 +1696,130 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
             cand &= cand - 1u;
             const SetupRec& R = sRec[bi];
             const uint32_t rid = sRid[bi];
+            const uint32_t st = R.state;
+            const int dtest = (int)(st & 7u);
+            const bool dmask = ((st >> 3) & 1u) != 0;
+            const int blendMode = (int)((st >> 4) & 7u);
+            const bool hasOut = ((st >> 7) & 1u) != 0;
+            // deferred: unblended fragments with outputs, and any
+            // fragment whose visibility record is wanted
+            const bool defer = (hasOut && blendMode == SL_BLEND_OFF) || (VIS && !hasOut);
+            // depth test and fragment work of this lane's pixel x of row y;
+            // s0..s2 are a triangle's barycentrics, a line's position
+            // along it (s0)
+            auto fragment = [&](int x, float z, float s0, float s1, float s2) {
+                const int c = x - x0;
+                ++tested;
+                if (depth_test(dtest, z, sDepth[row][c]))
+                {
+                    if (defer)
+                    {
+                        sPend[row][c] = rid;
+                        sPendS[0][row][c] = s0;
+                        sPendS[1][row][c] = s1;
+                        sPendS[2][row][c] = s2;
+                    }
+                    else if (hasOut)
+                    {
+                        // a blended fragment reads the colour: settle the pixel first
+                        resolve(row, c);
+                        const DrawDesc& D = P.draws[R.draw];
+                        const float hw[3] = {R.hw[0], R.hw[1], R.hw[2]};
+                        float4 out, wv;
+                        const bool have = shade(P, D, hw, R.bc3, st, rid, s0, s1, s2, out, wv);
+                        if (have) blend_into<CF>(&sCol8[CF == 1 ? row : 0][CF == 1 ? c : 0],
+                                                 &sColF[CF == 2 ? row : 0][CF == 2 ? c : 0], blendMode, out);
+                        if (VIS) sVis[row][c] = make_float4(__uint_as_float(R.seq), wv.x, wv.y, wv.z);
+                    }
+                    if (dmask) sDepth[row][c] = z;
+                    if (DBG)
+                    {
+                        sLast[row][c] = R.seq;
+                        sCount[row][c] += 1u;
+                    }
+                    ++shaded;
+                }
+            };
+            if (EXTRA && (st & (kStateLine | kStatePoint)) != 0u)
+            {
+                // This lane's texels of a point or line: columns x, x + 4,
+                // ... up to xe of row y, and for a line the walk's texel
+                // (W, y - 1), which is texel 0 of row y, before (pre) or
+                // after (post) them in walk order.  One loop, so the
+                // fragment work is inlined once.
+                const bool isLine = (st & kStateLine) != 0u;
+                int x = 1, xe = 0;
+                bool pre = false, post = false;
+                const LineWalk w = line_walk_of(R);
+                if (!isLine)
+                {
+                    // a point: its one texel, if this lane owns it
+                    const int px = (int)R.x[0];
+                    if ((int)R.y[0] == y && px >= sx0 && px <= sxLast && (px & 3) == phase) x = xe = px;
+                }
+                else
+                {
+                    // a line (SL_LineRasterizer::render_line, SL_LineRasterizer.cpp:
+                    // 27-80): this row's steps of the Bresenham walk in closed form
+                    const int dyAbs = w.dErr >> 1;
+                    int xa = 1, xb = 0;
+                    if (!w.steep)
+                    {
+                        const int n = (y - w.Y1) * w.yErr;
+                        if (n >= 0 && n <= dyAbs)
+                        {
+                            xa = w.X1 + ((n == 0) ? 0 : (w.dx * (2 * n - 1)) / w.dErr + 1);
+                            xb = w.X1 + ((n == dyAbs) ? w.dx : (w.dx * (2 * n + 1)) / w.dErr);
+                        }
+                    }
+                    else
+                    {
+                        const int k = y - w.X1;
+                        if (k >= 0 && k <= w.dx) xa = xb = w.Y1 + w.yErr * line_minor_steps(w, k);
+                    }
+                    if (sx0 == 0 && phase == 0 && y >= 1)
+                    {
+                        const int ka = w.steep ? (y - 1) - w.X1 : P.W - w.X1;
+                        bool hit = false;
+                        if (ka >= 0 && ka <= w.dx)
+                        {
+                            const int m = w.Y1 + w.yErr * line_minor_steps(w, ka);
+                            hit = w.steep ? (m == P.W) : (m == y - 1);
+                        }
+                        const int k0 = w.steep ? y - w.X1 : -w.X1;
+                        pre = hit && !(xa <= 0 && 0 <= xb && k0 < ka);
+                        post = hit && !pre;
+                    }
+                    const int xs = max(xa, sx0);
+                    x = xs + ((phase - xs) & 3);
+                    xe = min(xb, sxLast);
+                }
+                for (;;)
+                {
+                    int vx, vy, px;
+                    if (pre) { vx = P.W; vy = y - 1; px = 0; pre = false; }
+                    else if (x <= xe) { vx = x; vy = y; px = x; x += 4; }
+                    else if (post) { vx = P.W; vy = y - 1; px = 0; post = false; }
+                    else break;
+                    float z = R.z[0], interp = 0.f;
+                    if (isLine)
+                    {
+                        const float fx = (float)vx - R.x[0], fy = (float)vy - R.y[0];
+                        interp = sqrtf((fx * fx) + (fy * fy)) * R.bx[0];   // length(p - p0) * dist
+                        z = R.z[0] + interp * (R.z[1] - R.z[0]);         // mix(z0, z1, interp)
+                    }
+                    fragment(px, z, interp, 0.f, 0.f);
+                }
+                continue;
+            }
             int32_t xMin, xMax;
             scanline_step(R, yf, xMin, xMax);
             if ((uint32_t)xMin < (uint32_t)xMax)
             {
-                const uint32_t st = R.state;
-                const int dtest = (int)(st & 7u);
-                const bool dmask = ((st >> 3) & 1u) != 0;
-                const int blendMode = (int)((st >> 4) & 7u);
-                const bool hasOut = ((st >> 7) & 1u) != 0;
-                // deferred: unblended fragments with outputs, and any
-                // fragment whose visibility record is wanted
-                const bool defer = (hasOut && blendMode == SL_BLEND_OFF) || (VIS && !hasOut);
-                // depth test and fragment work of this lane's pixel x of row y
-                auto fragment = [&](int x, float s0, float s1, float s2) {
-                    const float z = (((R.z[0] * s0) + (R.z[1] * s1)) + (R.z[2] * s2)) + R.bc3;
-                    const int c = x - x0;
-                    ++tested;
-                    if (depth_test(dtest, z, sDepth[row][c]))
-                    {
-                        if (defer)
-                        {
-                            sPend[row][c] = rid;
-                            sPendS[0][row][c] = s0;
-                            sPendS[1][row][c] = s1;
-                            sPendS[2][row][c] = s2;
-                        }
-                        else if (hasOut)
-                        {
-                            // a blended fragment reads the colour: settle the pixel first
-                            resolve(row, c);
-                            const DrawDesc& D = P.draws[R.draw];
-                            const float hw[3] = {R.hw[0], R.hw[1], R.hw[2]};
-                            float4 out, wv;
-                            const bool have = shade(P, D, hw, R.bc3, rid, s0, s1, s2, out, wv);
-                            if (have) blend_into<CF>(&sCol8[CF == 1 ? row : 0][CF == 1 ? c : 0],
-                                                     &sColF[CF == 2 ? row : 0][CF == 2 ? c : 0], blendMode, out);
-                            if (VIS) sVis[row][c] = make_float4(__uint_as_float(R.seq), wv.x, wv.y, wv.z);
-                        }
-                        if (dmask) sDepth[row][c] = z;
-                        if (DBG)
-                        {
-                            sLast[row][c] = R.seq;
-                            sCount[row][c] += 1u;
-                        }
-                        ++shaded;
-                    }
-                };
                 // This lane's pixels: x, x + 4, ... below xe, then those of
                 // a second run [xn, xen) (wireframe).  One loop serves both
                 // kinds so the fragment work is inlined once.
-                const bool wire = WIRE && (st & kStateWire) != 0u;
+                const bool wire = EXTRA && (st & kStateWire) != 0u;
                 int x, xe, xn = 0, xen = 0;
                 float s0, s1, s2, d0, d1, d2;
                 if (wire)
@@ -1500,7 +1911,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                             s1 = fmaf(R.bx[1], fx, c1);
                             s2 = fmaf(R.bx[2], fx, c2);
                         }
-                        fragment(x, s0, s1, s2);
+                        fragment(x, (((R.z[0] * s0) + (R.z[1] * s1)) + (R.z[2] * s2)) + R.bc3, s0, s1, s2);
                         s0 = s0 + d0;
                         s1 = s1 + d1;
                         s2 = s2 + d2;
@@ -1649,11 +2060,12 @@ hipError_t launch_chunk_scan(const SetupParams& p, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_emit(const SetupParams& p, hipStream_t s)
+hipError_t launch_emit(const SetupParams& p, bool tris, bool prims, hipStream_t s)
 {
     (void)hipGetLastError();   // drop a stale error from an earlier call
     const uint32_t grid = p.numChunks < 2048u ? p.numChunks : 2048u;   // grid-stride over blocks of survivors
-    hipLaunchKernelGGL(k_emit, dim3(grid), dim3(kEmitBlock), 0, s, p);
+    if (tris) hipLaunchKernelGGL(k_emit<false>, dim3(grid), dim3(kEmitBlock), 0, s, p);
+    if (prims) hipLaunchKernelGGL(k_emit<true>, dim3(grid), dim3(kEmitBlock), 0, s, p);
     return hipGetLastError();
 }
 
@@ -1756,11 +2168,11 @@ hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debu
     {
         // the visibility attachment is only built for RGBA32F colour targets
         // and filled triangles
-        if (cf != 2 || p.wire) return hipErrorInvalidValue;
+        if (cf != 2 || p.extra) return hipErrorInvalidValue;
         if (debug) launch_raster_cfg<2, 3>(p, numTiles, cfg, s);
         else launch_raster_cfg<2, 2>(p, numTiles, cfg, s);
     }
-    else if (p.wire)
+    else if (p.extra)
     {
         if (debug)
         {

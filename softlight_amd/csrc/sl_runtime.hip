@@ -397,8 +397,8 @@ RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool have
     }
     rp.clearRGBA8 = packed;
     rp.clearDepthValue = (float)B.depth;
-    rp.wire = 0;
-    for (const DrawDesc& d : B.draws) rp.wire |= (d.prim == kPrimWire) ? 1 : 0;
+    rp.extra = 0;
+    for (const DrawDesc& d : B.draws) rp.extra |= (d.prim != kPrimFill) ? 1 : 0;
     rp.ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
     return rp;
 }
@@ -419,7 +419,14 @@ int enqueue_front(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, hip
     if ((rc = check(c, launch_setup(sp, p.chunks, s), "k_setup")) != SL_OK) return rc;
     if (evSetup1 && (rc = check(c, hipEventRecord(evSetup1, s), "event")) != SL_OK) return rc;
     if ((rc = check(c, launch_chunk_scan(sp, s), "k_chunk_scan")) != SL_OK) return rc;
-    return check(c, launch_emit(sp, s), "k_emit");
+    bool tris = false, prims = false;
+    for (const DrawDesc& d : B.draws)
+    {
+        const bool pl = d.prim == kPrimPoint || d.prim == kPrimLine;
+        prims = prims || pl;
+        tris = tris || !pl;
+    }
+    return check(c, launch_emit(sp, tris, prims, s), "k_emit");
 }
 
 // Clip, binning and raster.  clipJobs / entries size the grids (a graph uses
@@ -743,6 +750,8 @@ int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t s
     if (vis && prim != kPrimFill) return fail(c, SL_ERR_UNSUPPORTED, "visibility attachments cover filled triangles only");
     Tex* dt = tex_at(c, F.depth);
     if (!dt || dt->type != SL_COLOR_R_FLOAT) return fail(c, SL_ERR_STATE, "framebuffer needs an R_FLOAT depth attachment");
+    if (prim == kPrimLine && (dt->w > kMaxLineTarget || dt->h > kMaxLineTarget))
+        return fail(c, SL_ERR_UNSUPPORTED, "line draws need a target of at most 8192 x 8192");
     if (sh.nout == 1)
     {
         Tex* ct = F.colors.empty() ? nullptr : tex_at(c, F.colors[0]);
@@ -767,7 +776,7 @@ int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t s
     if ((!indexed && m.elementEnd > 0xFFFFFFFFull) || numInstances > 0xFFFFFFFFull)
         return fail(c, SL_ERR_UNSUPPORTED, "vertex ids and instance counts are limited to 2^32");
     const uint64_t count = (m.elementEnd > m.elementBegin) ? (m.elementEnd - m.elementBegin) : 0;
-    const uint64_t tpi = count / 3;
+    const uint64_t tpi = count / verts_per_prim(prim);   // whole primitives only
     const uint64_t numTris = tpi * numInstances;
     c->stats.triangles_submitted += numTris;
     if (numTris == 0) return SL_OK;

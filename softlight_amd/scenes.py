@@ -233,6 +233,33 @@ def build_random_triangles(ctx, seed=0, ntri=4096, W=640, H=480, debug=False, wi
     return s
 
 
+def build_random_points(ctx, seed=0, npoints=20000, W=640, H=480, debug=False) -> Scene:
+    """npoints Gouraud points uniform in NDC [-1.1, 1.1]^2 x [-1, 1] (some
+    outside the clip volume), depth-tested."""
+    xyz = uniform(seed, npoints * 3, -1.1, 1.1, stream=11).reshape(npoints, 3)
+    xyz[:, 2] = uniform(seed, npoints, -1.0, 1.0, stream=12)
+    rec = np.zeros((npoints, 16), dtype=np.uint8)
+    rec[:, 0:12] = xyz.astype(np.float32).view(np.uint8).reshape(npoints, 12)
+    rec[:, 12:16] = uniform_u8(seed, npoints * 4, stream=13).reshape(npoints, 4)
+    s = build_from_vertex_records(ctx, rec, W, H, debug, mode=A.RENDER_MODE_POINTS)
+    s.info.update(workload="random_points", seed=seed, points=npoints, W=W, H=H)
+    return s
+
+
+def build_random_lines(ctx, seed=0, nlines=3000, W=640, H=480, debug=False) -> Scene:
+    """nlines Gouraud segments with ends uniform in NDC [-1.5, 1.5]^2 x
+    [-1, 1] (many cross the viewport's edges and are clipped), depth-tested."""
+    nv = 2 * nlines
+    xyz = uniform(seed, nv * 3, -1.5, 1.5, stream=14).reshape(nv, 3)
+    xyz[:, 2] = uniform(seed, nv, -1.0, 1.0, stream=15)
+    rec = np.zeros((nv, 16), dtype=np.uint8)
+    rec[:, 0:12] = xyz.astype(np.float32).view(np.uint8).reshape(nv, 12)
+    rec[:, 12:16] = uniform_u8(seed, nv * 4, stream=16).reshape(nv, 4)
+    s = build_from_vertex_records(ctx, rec, W, H, debug, mode=A.RENDER_MODE_LINES)
+    s.info.update(workload="random_lines", seed=seed, lines=nlines, W=W, H=H)
+    return s
+
+
 def sphere_instances(seed: int, n: int, extent: float = 20.0, radius: float = 2.0) -> np.ndarray:
     pos = uniform(seed, n * 3, -extent, extent, stream=7).reshape(n, 3)
     mats = np.stack([translate_scale(p[0], p[1], p[2], radius) for p in pos]).astype(np.float32)
@@ -241,7 +268,7 @@ def sphere_instances(seed: int, n: int, extent: float = 20.0, radius: float = 2.
 
 def build_sphere_scene(ctx, seed=1, n_instances=64, stacks=256, sectors=256, W=1920, H=1080,
                        tex_size=1024, radius=2.0, extent=20.0, debug=False, cull=A.SL_CULL_BACK_FACE,
-                       blend=A.SL_BLEND_OFF, alpha=1.0, wire=False) -> Scene:
+                       blend=A.SL_BLEND_OFF, alpha=1.0, wire=False, prim="triangles") -> Scene:
     verts, idx = uv_sphere(stacks, sectors)
     vbo = ctx.create_vbo(verts)
     ibo = ctx.create_ibo(idx)
@@ -269,7 +296,9 @@ def build_sphere_scene(ctx, seed=1, n_instances=64, stacks=256, sectors=256, W=1
     fs = A.sl_fragment_shader(2, 1, blend, A.SL_DEPTH_TEST_LESS_THAN, A.SL_DEPTH_MASK_ON, A.SL_PROGRAM_LAMBERT_TEXTURED)
     shader = ctx.create_shader(vs, fs, ubo)
     color, depth, fbo, dbg = make_target(ctx, W, H, debug)
-    mode = A.RENDER_MODE_INDEXED_TRI_WIRE if wire else A.RENDER_MODE_INDEXED_TRIANGLES
+    # the same index buffer read as triangles, wireframe triangles, lines or points
+    mode = {"triangles": A.RENDER_MODE_INDEXED_TRI_WIRE if wire else A.RENDER_MODE_INDEXED_TRIANGLES,
+            "lines": A.RENDER_MODE_INDEXED_LINES, "points": A.RENDER_MODE_INDEXED_POINTS}[prim]
     mesh = A.sl_mesh(vao, 0, idx.size, mode, 0)
     s = Scene(ctx, W, H, fbo, color, depth, shader, [("instanced", mesh, n_instances)],
               n_triangles=(idx.size // 3) * n_instances, debug=dbg)

@@ -200,3 +200,45 @@ def wire_visits(tri):
         for x in vis:
             counts[(x, y)] = counts.get((x, y), 0) + 1
     return counts, amb
+
+
+# ---------------------------------------------------------------------------
+# points and lines
+# ---------------------------------------------------------------------------
+def ndc_records(ndc, colors=None) -> np.ndarray:
+    """Records {f32 x, y, z; u8 rgba} for vertices given directly in NDC (w = 1)."""
+    pos = np.asarray(ndc, dtype=np.float32).reshape(-1, 3)
+    rec = np.zeros((pos.shape[0], 16), dtype=np.uint8)
+    rec[:, 0:12] = pos.view(np.uint8).reshape(-1, 12)
+    rec[:, 12:16] = 255 if colors is None else np.asarray(colors, dtype=np.uint8).reshape(-1, 4)
+    return rec
+
+
+def pixel_records(pts, W, H, z=None, colors=None) -> np.ndarray:
+    """Records for vertices at integer screen positions (X, Y) (snapped to
+    exactly those, X == W and Y == H included)."""
+    ndc = [(ndc_for_pixel(X, W), ndc_for_pixel(Y, H), 0.0 if z is None else z[i]) for i, (X, Y) in enumerate(pts)]
+    return ndc_records(ndc, colors)
+
+
+def bresenham_pixels(p, q):
+    """Pixels of the segment between integer points p and q, one per step of
+    its major axis: in each column (row, for a steep segment) the pixel whose
+    centre line is nearest the ideal segment, a tie going to the row (column)
+    of the end with the smaller major coordinate (p first when equal).  The
+    major axis is y only when |dy| > |dx|.  Returned in walk order."""
+    (x1, y1), (x2, y2) = p, q
+    steep = abs(x1 - x2) < abs(y1 - y2)
+    if steep:
+        x1, y1, x2, y2 = y1, x1, y2, x2
+    if x1 > x2:
+        x1, y1, x2, y2 = x2, y2, x1, y1
+    dx, dy = x2 - x1, y2 - y1
+    out = []
+    for k in range(dx + 1):
+        # nearest row to the ideal y1 + k*dy/dx, half-way rounding toward y1
+        q_ = Fr(k * abs(dy), dx) if dx else Fr(0)
+        n = math.ceil(q_ - Fr(1, 2))
+        y = y1 + (n if dy >= 0 else -n)
+        out.append((y, x1 + k) if steep else (x1 + k, y))
+    return out

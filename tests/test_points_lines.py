@@ -1,0 +1,243 @@
+"""Points and lines (RENDER_MODE_POINTS / LINES and their indexed forms;
+SL_PointProcessor, SL_PointRasterizer, SL_LineProcessor, SL_LineRasterizer).
+
+What pins the oracle's restatement, and the mistake each check catches:
+
+  point pixels        a point lands on the pixel its snapped position names,
+                      with its vertex's colour and depth unchanged (no
+                      interpolation): wrong snap, varyings mixed
+  point clipping      points outside [-w, w] or behind the eye are dropped;
+                      NDC x = +1 addresses column W, which is texel 0 of the
+                      next row; row H is outside the attachment: a clamp
+                      instead of the reference's linear addressing
+  last writer         depth test off: a pixel keeps its last point's colour
+  line pixels         every segment's pixels are the column-nearest (row-
+                      nearest when steep) pixels of the ideal segment,
+                      ties toward the end with the smaller major
+                      coordinate, both ends included (exact rational model
+                      in tests/geometry.py): an off-by-one end, a wrong tie
+                      or major axis, y stepping on >= instead of >
+  line depth/colour   depth and colour mixed by the distance from the first
+                      end over the length, against float64
+  line clipping       a segment leaving the viewport ends on the viewport
+                      edge (x = W, which is texel 0 of the next row) and
+                      keeps its depth slope (segments with a vertex behind
+                      the eye are covered by the sphere scenes' parity)
+  draw_multiple       the first mesh's mode picks the processor
+  threads             the row-interleaved threaded oracle equals one thread
+GPU (marked gpu): the same checks, and bit-exact parity with the oracle on
+random point clouds, random line sets (clipped ones included), instanced
+sphere meshes drawn as lines and points, and every tile configuration.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from softlight_amd import api as A
+from softlight_amd import scenes
+from tests.geometry import bresenham_pixels, ndc_records, pixel_records, render
+from tests.parity import assert_parity, render_both
+
+BACKENDS = [pytest.param("oracle", id="oracle"), pytest.param("gpu", id="gpu", marks=pytest.mark.gpu)]
+
+
+@pytest.fixture
+def make_ctx(request):
+    def _make(kind):
+        return A.SLContext(request.getfixturevalue("oracle_lib" if kind == "oracle" else "gpu_lib"))
+    return _make
+
+
+def scene_of(ctx, rec, W, H, mode, depth_test=A.SL_DEPTH_TEST_OFF, color_type=A.SL_COLOR_RGBA_8U,
+             blend=A.SL_BLEND_OFF):
+    return scenes.build_from_vertex_records(ctx, rec, W, H, debug=True, mode=mode, color_type=color_type,
+                                            shader_kw=dict(cull=A.SL_CULL_OFF, depth_test=depth_test,
+                                                           depth_mask=A.SL_DEPTH_MASK_ON, blend=blend))
+
+
+def counts_of(img):
+    ys, xs = np.nonzero(img)
+    return {(int(x), int(y)): int(img[y, x]) for x, y in zip(xs, ys)}
+
+
+def f32(v):
+    return float(np.float32(v))
+
+
+# ---------------------------------------------------------------------------
+# points
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_points_land_on_their_pixels(kind, make_ctx):
+    ctx = make_ctx(kind)
+    W, H = 40, 30
+    rng = np.random.default_rng(2)
+    pts = [(int(x), int(y)) for x, y in zip(rng.integers(0, W, 50), rng.integers(0, H - 1, 50))]
+    pts = list(dict.fromkeys(pts))
+    zs = [f32(v) for v in rng.uniform(-0.9, 0.9, len(pts))]
+    cols = rng.integers(0, 256, (len(pts), 4)).astype(np.uint8)
+    out = render(ctx, scene_of(ctx, pixel_records(pts, W, H, z=zs, colors=cols), W, H, A.RENDER_MODE_POINTS,
+                               color_type=A.SL_COLOR_RGBA_FLOAT))
+    assert counts_of(out["count"]) == {p: 1 for p in pts}
+    for i, (x, y) in enumerate(pts):
+        assert out["depth"][y, x] == np.float32(zs[i])
+        # the vertex colour, unchanged: u8 * (1/255) as the Gouraud program reads it
+        expect = cols[i].astype(np.float32) * np.float32(0.00392156862745)
+        np.testing.assert_array_equal(out["color"][y, x], expect)
+    assert out["stats"]["fragments_tested"] == len(pts)
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_point_clipping_and_edge_addressing(kind, make_ctx):
+    ctx = make_ctx(kind)
+    W, H = 16, 12
+    ndc = [(-1.0, -1.0, 0.0),          # (0, 0)
+           (1.0, -1.0 + 2 * 3.5 / H, 0.0),   # column W of row 3: texel 0 of row 4
+           (1.0, 1.0, 0.0),            # (W, H): past the attachment
+           (0.0, 1.0, 0.0),            # row H: past the attachment
+           (1.01, 0.0, 0.0),           # outside [-w, w]
+           (0.0, 0.0, 1.5),            # z outside
+           (0.0, -1.02, 0.0)]
+    out = render(ctx, scene_of(ctx, ndc_records(ndc), W, H, A.RENDER_MODE_POINTS))
+    assert counts_of(out["count"]) == {(0, 0): 1, (0, 4): 1}
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_points_last_writer_and_indices(kind, make_ctx):
+    ctx = make_ctx(kind)
+    W, H = 20, 20
+    pts = [(3, 4), (7, 8), (3, 4), (10, 1), (7, 8), (3, 4)]
+    cols = [(10 * i, 0, 0, 255) for i in range(len(pts))]
+    s = scene_of(ctx, pixel_records(pts, W, H, colors=cols), W, H, A.RENDER_MODE_INDEXED_POINTS)
+    vao = s.draws[0][1].vaoId
+    order = np.array([5, 4, 3, 2, 1, 0, 1], dtype=np.uint16)
+    ctx.vao_set_index_buffer(vao, ctx.create_ibo(order))
+    s.draws = [("draw", A.sl_mesh(vao, 0, len(order), A.RENDER_MODE_INDEXED_POINTS, 0), 0)]
+    out = render(ctx, s)
+    # vertices 5 4 3 2 1 0 1 land on (3,4) (7,8) (10,1) (3,4) (7,8) (3,4) (7,8)
+    assert counts_of(out["count"]) == {(3, 4): 3, (7, 8): 3, (10, 1): 1}
+    # last writers: index slot 6 (vertex 1) at (7, 8); slot 5 (vertex 0) at (3, 4)
+    assert out["last"][8, 7] == 7 and out["last"][4, 3] == 6
+    assert int(out["color"][8, 7, 0]) == 10 and int(out["color"][4, 3, 0]) == 0
+
+
+# ---------------------------------------------------------------------------
+# lines
+# ---------------------------------------------------------------------------
+def line_pairs(seed, n, W, H):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        p = (int(rng.integers(0, W)), int(rng.integers(0, H)))
+        q = (int(rng.integers(0, W)), int(rng.integers(0, H)))
+        out.append((p, q))
+    return out
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_line_pixels_match_the_segment_model(kind, make_ctx):
+    ctx = make_ctx(kind)
+    W, H = 64, 48
+    cases = [((2, 5), (20, 5)), ((7, 3), (7, 30)), ((1, 1), (30, 30)), ((40, 2), (3, 9)), ((5, 40), (9, 2)),
+             ((10, 10), (10, 10)), ((0, 0), (63, 47)), ((63, 0), (0, 47)), ((20, 20), (26, 23))]
+    cases += line_pairs(4, 40, W, H)
+    for p, q in cases:
+        out = render(ctx, scene_of(ctx, pixel_records([p, q], W, H), W, H, A.RENDER_MODE_LINES))
+        exp = {px: 1 for px in bresenham_pixels(p, q)}
+        assert counts_of(out["count"]) == exp, (p, q)
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_line_depth_and_colour_are_mixed_by_distance(kind, make_ctx):
+    ctx = make_ctx(kind)
+    W, H = 64, 48
+    for (p, q) in [((3, 7), (50, 19)), ((9, 40), (14, 2))]:
+        z = [f32(-0.5), f32(0.75)]
+        cols = np.array([(255, 0, 40, 255), (0, 200, 90, 255)], dtype=np.uint8)
+        out = render(ctx, scene_of(ctx, pixel_records([p, q], W, H, z=z, colors=cols), W, H, A.RENDER_MODE_LINES,
+                                   color_type=A.SL_COLOR_RGBA_FLOAT))
+        L = math.hypot(q[0] - p[0], q[1] - p[1])
+        c0 = cols[0].astype(np.float64) * f32(0.00392156862745)
+        c1 = cols[1].astype(np.float64) * f32(0.00392156862745)
+        for (x, y) in bresenham_pixels(p, q):
+            t = math.hypot(x - p[0], y - p[1]) / L
+            assert abs(float(out["depth"][y, x]) - (z[0] + t * (z[1] - z[0]))) < 2e-6
+            np.testing.assert_allclose(out["color"][y, x], c0 + t * (c1 - c0), atol=2e-6, rtol=0)
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_line_clipping(kind, make_ctx):
+    ctx = make_ctx(kind)
+    W, H = 32, 24
+    y5 = -1.0 + 2 * 5.5 / H
+    ndc = [(-1.0 + 2 * 10.5 / W, y5, -0.5), (3.0, y5, 0.5)]     # leaves through the right edge
+    out = render(ctx, scene_of(ctx, ndc_records(ndc), W, H, A.RENDER_MODE_LINES))
+    got = counts_of(out["count"])
+    # clipped at x = W (the viewport's right edge): columns 10..W-1 of row 5,
+    # and column W, which is texel 0 of row 6
+    assert got == {**{(x, 5): 1 for x in range(10, W)}, (0, 6): 1}
+    # the clipped end keeps the depth slope: z at x from the unclipped segment
+    x0 = (-1.0 + 2 * 10.5 / W + 1) * W / 2
+    x1 = (3.0 + 1) * W / 2
+    for x in (10, 20, 31):
+        t = (x - math.floor(x0)) / (math.floor(x1) - math.floor(x0))
+        assert abs(float(out["depth"][5, x]) - (-0.5 + t * 1.0)) < 1e-5
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_draw_multiple_first_mode_picks_the_processor(kind, make_ctx):
+    ctx = make_ctx(kind)
+    W, H = 32, 32
+    rec = pixel_records([(2, 3), (20, 9), (5, 25), (30, 30)], W, H)
+
+    def run(modes):
+        s = scene_of(ctx, rec, W, H, A.RENDER_MODE_LINES)
+        vao = s.draws[0][1].vaoId
+        meshes = [A.sl_mesh(vao, 0, 2, modes[0], 0), A.sl_mesh(vao, 2, 4, modes[1], 0)]
+        s.draws = [("multiple", meshes, 0)]
+        return counts_of(render(ctx, s)["count"])
+
+    lines = run([A.RENDER_MODE_LINES, A.RENDER_MODE_LINES])
+    assert run([A.RENDER_MODE_LINES, A.RENDER_MODE_POINTS]) == lines
+    assert run([A.RENDER_MODE_POINTS, A.RENDER_MODE_LINES]) == {(2, 3): 1, (20, 9): 1, (5, 25): 1, (30, 30): 1}
+
+
+def test_oracle_threads_match_single_thread(oracle_lib):
+    for build in (lambda c: scenes.build_random_lines(c, seed=3, nlines=300, W=160, H=120, debug=True),
+                  lambda c: scenes.build_random_points(c, seed=3, npoints=2000, W=160, H=120, debug=True)):
+        outs = []
+        for n in (1, 4):
+            ctx = A.SLContext(oracle_lib)
+            ctx.num_threads(n)
+            outs.append(render(ctx, build(ctx)))
+        assert_parity(*outs)
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.gpu
+def test_gpu_random_points(gpu_lib, oracle_lib):
+    g, o = render_both(gpu_lib, oracle_lib, scenes.build_random_points, seed=5, npoints=20000, W=640, H=480,
+                       debug=True)
+    assert_parity(g, o)
+
+
+@pytest.mark.gpu
+def test_gpu_random_lines(gpu_lib, oracle_lib):
+    g, o = render_both(gpu_lib, oracle_lib, scenes.build_random_lines, seed=5, nlines=3000, W=640, H=480,
+                       debug=True)
+    assert_parity(g, o)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("config", range(10))
+@pytest.mark.parametrize("prim", ["lines", "points"])
+def test_gpu_spheres_every_configuration(gpu_lib, oracle_lib, config, prim):
+    def build(ctx, **kw):
+        ctx.set_raster_config(config)
+        return scenes.build_sphere_scene(ctx, **kw)
+    g, o = render_both(gpu_lib, oracle_lib, build, seed=2, n_instances=6, stacks=24, sectors=24, W=333, H=199,
+                       tex_size=64, debug=True, prim=prim)
+    assert_parity(g, o)
