@@ -82,6 +82,8 @@ def run_gpu(steps, warmup, workload="spheres", raster_config=None, tune=False):
     ctx = api.SLContext(softlight_amd.load_library())
     if workload == "spheres":
         scene = scenes.build_sphere_scene(ctx, **WORKLOAD)
+    elif workload in RENDER_MODES:
+        scene = scenes.build_sphere_scene(ctx, **WORKLOAD, **RENDER_MODES[workload])
     elif workload == "batched":
         scene = scenes.build_batched_scene(ctx, **BATCHED_WORKLOAD)
     else:
@@ -235,6 +237,34 @@ def run_smooth(steps, warmup):
             "peak_device_bytes": int(peak)}
 
 
+# the sphere frame's index buffer read in the other render modes
+RENDER_MODES = {"spheres_wire": dict(wire=True), "spheres_lines": dict(prim="lines"),
+                "spheres_points": dict(prim="points")}
+
+
+def run_render_modes(steps, warmup):
+    """The 1080p sphere frame with its index buffer drawn as wireframe
+    triangles, lines (index pairs) and points (every index): per-frame time
+    of captured-graph replays and primitives per second."""
+    out = {}
+    for name, kw in RENDER_MODES.items():
+        scene, st, eager, gst, graph_s = run_gpu(steps, warmup, workload=name)
+        info = scene.info
+        per = {"spheres_wire": 3, "spheres_lines": 2, "spheres_points": 1}[name]
+        prims = info["instances"] * (3 * info["triangles_per_instance"] // per)
+        f = max(1, st["frames"])
+        out[name[len("spheres_"):]] = {
+            "primitives_per_frame": prims,
+            "ms_per_frame": round(graph_s * 1e3 / steps, 4),
+            "Mprim_per_s": round(prims * steps / graph_s / 1e6, 1),
+            "stage_ms": {"setup_kernel": round(st["ms_setup_kernel"] / f, 4), "binning": round(st["ms_bin"] / f, 4),
+                         "raster": round(st["ms_raster"] / f, 4)},
+            "fragments_tested_per_frame": int(st["fragments_tested"] // f),
+            "fragments_shaded_per_frame": int(st["fragments_shaded"] // f),
+        }
+    return out
+
+
 def run_blit(reps=50):
     """Present: blit of a 1920x1080 RGBA8 colour buffer to an RGBA8 back-buffer
     of the same size (SL_Context::blit), reps times back to back."""
@@ -296,10 +326,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--workload", choices=["spheres", "random", "batched", "views"], default="spheres",
+    ap.add_argument("--workload", choices=["spheres", "random", "batched", "views", "modes"], default="spheres",
                     help="spheres: the 1080p textured-sphere frame (default); random: 4096 random triangles at "
                          "640x480; batched: 1000 meshes with mixed opaque/blended materials at 1080p; views: "
-                         "32 differentiable 512x512 views, forward and forward+backward")
+                         "32 differentiable 512x512 views, forward and forward+backward; modes: the sphere frame's "
+                         "index buffer drawn as wireframe triangles, lines and points (its leg only)")
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter calibration)")
     ap.add_argument("--raster-config", type=int, default=None, help="raster tile configuration 0..7 (default: tuned)")
     ap.add_argument("--tune", action="store_true", help="time all raster configurations first and cache the best")
@@ -313,6 +344,9 @@ def main():
         calibration_copy()
     if args.workload == "views":
         main_views(args)
+        return
+    if args.workload == "modes":
+        print(json.dumps({"render_modes": run_render_modes(args.steps, args.warmup)}), flush=True)
         return
     scene, st, eager_s, gst, elapsed = run_gpu(args.steps, args.warmup, args.workload, args.raster_config, args.tune)
     tris = scene.n_triangles
@@ -398,6 +432,7 @@ def main():
         _, vr = run_views(max(3, args.steps // 4), 1)
         out["differentiable"] = views_summary(vr)
         out["smoothed_coverage"] = run_smooth(max(2, args.steps // 8), 1)
+        out["render_modes"] = run_render_modes(max(3, args.steps // 4), 1)
         out["present_blit"] = run_blit()
     if not args.no_cpu and args.workload == "spheres":
         out["cpu_baseline"] = run_cpu_baseline()

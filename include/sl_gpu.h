@@ -255,7 +255,17 @@ int    sl_ubo_assign(sl_context* ctx, size_t ubo, const void* data, size_t offse
 size_t sl_create_shader(sl_context* ctx, const sl_vertex_shader* vs,
                         const sl_fragment_shader* fs, size_t uboId);   /* SL_Context::create_shader; uboId may be SL_INVALID_ID */
 
-/* ---- drawing (SL_Context.cpp:714-740) ------------------------------------ */
+/* ---- drawing (SL_Context.cpp:714-740) ------------------------------------
+ * Every render mode is drawn: filled and wireframe triangles
+ * (SL_TriRasterizer.cpp:116-202, 806-923), lines (SL_LineProcessor.cpp,
+ * SL_LineRasterizer.cpp) and points (SL_PointProcessor.cpp,
+ * SL_PointRasterizer.cpp), indexed or not.  As in the reference, the render
+ * mode of the first mesh of an sl_draw_multiple call picks the processor and
+ * rasteriser for all of its meshes, and each mesh's own mode whether it reads
+ * its index buffer.  A point or line texel is addressed as x + W*y: a snapped
+ * x of W lands on texel 0 of row y + 1; texels at or past W*H are dropped.
+ * Limits: visibility attachments are for filled triangles only; line draws
+ * need a depth target of at most 8192 x 8192 (SL_ERR_UNSUPPORTED). */
 int sl_draw(sl_context* ctx, const sl_mesh* m, size_t shaderId, size_t fboId);                        /* SL_Context::draw */
 int sl_draw_multiple(sl_context* ctx, const sl_mesh* m, size_t numMeshes, size_t shaderId, size_t fboId); /* SL_Context::draw_multiple */
 int sl_draw_instanced(sl_context* ctx, const sl_mesh* m, size_t numInstances, size_t shaderId, size_t fboId); /* SL_Context::draw_instanced */

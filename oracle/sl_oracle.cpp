@@ -18,17 +18,26 @@
  *   mix(a,b,t)    = a + t*(b-a)                      SL_TriProcessor.cpp:410,441
  *   float->int    = truncate; NaN/out of range -> INT32_MIN (x86 cvttss2si)
  *   max(a,b)      = (a < b) ? b : a                  SL_PostVertexTransform.hpp:520-523
+ *   length(v)     = sqrtf(dot(v, v)), inversesqrt(x) = 1/sqrtf(x)   SL_LineProcessor.cpp:132,150; SL_LineRasterizer.cpp:39,61
  * The build runs with -ffp-contract=off so that no other operation fuses.
  *
  * Deviations from the reference (undefined behaviour there):
  *   - a triangle whose snapped screen x or y is not finite is dropped;
  *   - a trailing partial triangle (elementEnd-elementBegin not a multiple of 3)
- *     is skipped instead of reading past elementEnd;
- *   - vertex reads beyond the end of a vertex buffer return zeros.
+ *     or half line is skipped instead of reading past elementEnd;
+ *   - vertex reads beyond the end of a vertex buffer return zeros;
+ *   - point and line texels at or past W*H (written outside the attachment)
+ *     and line texels beyond column W are dropped;
+ *   - the thread-pool mode splits point and line rows by the addressed
+ *     texel's row (the reference's split by y races on wrapped texels).
+ * Every primitive kind keeps submission order per pixel (the reference's
+ * radix sorts of a flush's bins by primIndex or depth are not restated).
  *
  * Each part is pinned by a closed-form, exact-arithmetic or invariant check
  * in tests/test_closed_forms.py and tests/test_recurrence.py (list at the top
- * of test_closed_forms.py); the reference ships no golden images.  The
+ * of test_closed_forms.py), and for the wireframe, point and line modes in
+ * tests/test_wireframe.py and tests/test_points_lines.py; the reference ships
+ * no golden images.  The
  * adjoints and the smoothed-coverage mode (no reference counterpart) are
  * pinned in tests/test_backward.py and tests/test_smooth.py: finite
  * differences, the adjoint identity, closed forms of the smoothing model and
@@ -41,7 +50,9 @@
  *   - truncation (generic build) rather than rounding (AVX build) of span ends;
  *   - rcp being the exact IEEE reciprocal;
  *   - float->u8 conversion of values outside [0,255] (low byte of the
- *     truncated int).
+ *     truncated int);
+ *   - LightMath's length and inversesqrt (pinned above; an approximate
+ *     reciprocal square root would move clipped line ends and line depths).
  */
 #include <algorithm>
 #include <atomic>

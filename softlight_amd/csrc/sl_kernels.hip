@@ -514,7 +514,7 @@ __device__ __forceinline__ bool line_box(const LineWalk& w, int W, int H, LineBo
     b.dy1 = min(vy1, H - 1);
     b.wy0 = 1;
     b.wy1 = 0;
-    if (vx1 >= W)
+    if (vx0 <= W && vx1 >= W)   // the walk passes column W (columns beyond it address nothing kept)
     {
         b.wy0 = max(vy0 + 1, 1);
         b.wy1 = min(vy1 + 1, H - 1);
@@ -536,7 +536,11 @@ __device__ __forceinline__ void count_point(const SetupParams& P, const DrawDesc
     if (T.status != 3) return;
     const float4 s = divide_and_snap(T.pos[0], D.vp);
     int px, py;
-    if (!linear_texel((uint16_t)f2i(s.x), (uint16_t)f2i(s.y), P.W, P.H, px, py)) return;
+    if (!linear_texel((uint16_t)f2i(s.x), (uint16_t)f2i(s.y), P.W, P.H, px, py))
+    {
+        C.empty = true;   // binned by the reference, addresses no kept texel
+        return;
+    }
     C.sp[0] = f4((float)px, (float)py, s.z, s.w);
     C.tr = rect_tiles(px, px, py, py, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
     C.nrec = 1;
@@ -550,7 +554,11 @@ __device__ __forceinline__ void count_line(const SetupParams& P, const DrawDesc&
     if (T.status != 3 && !clip_line(a, b, D.vp, C.t0, C.t1, C.lineClip)) return;
     const LineWalk w = make_line_walk(a.x, a.y, b.x, b.y);
     LineBox box;
-    if (!line_box(w, P.W, P.H, box)) return;
+    if (!line_box(w, P.W, P.H, box))
+    {
+        C.empty = true;   // binned by the reference, addresses no kept texel
+        return;
+    }
     C.sp[0] = a;
     C.sp[1] = b;
     C.tr = rect_tiles(box.dx0, box.dx1, box.dy0, box.dy1, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
