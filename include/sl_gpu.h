@@ -265,7 +265,11 @@ size_t sl_create_shader(sl_context* ctx, const sl_vertex_shader* vs,
  * its index buffer.  A point or line texel is addressed as x + W*y: a snapped
  * x of W lands on texel 0 of row y + 1; texels at or past W*H are dropped.
  * Limits: visibility attachments are for filled triangles only; line draws
- * need a depth target of at most 8192 x 8192 (SL_ERR_UNSUPPORTED). */
+ * need a depth target of at most 8192 x 8192 (SL_ERR_UNSUPPORTED).
+ * Targets: depth R_FLOAT or R_HALF; colour attachment 0 8-bit (R, RG, RGB,
+ * RGBA) or float (R, RG, RGB, RGBA), written as assign_pixel /
+ * assign_alpha_pixel do (SL_Framebuffer.cpp:42-80, 268-370).  Visibility
+ * attachments need RGBA_FLOAT colour and R_FLOAT depth. */
 int sl_draw(sl_context* ctx, const sl_mesh* m, size_t shaderId, size_t fboId);                        /* SL_Context::draw */
 int sl_draw_multiple(sl_context* ctx, const sl_mesh* m, size_t numMeshes, size_t shaderId, size_t fboId); /* SL_Context::draw_multiple */
 int sl_draw_instanced(sl_context* ctx, const sl_mesh* m, size_t numInstances, size_t shaderId, size_t fboId); /* SL_Context::draw_instanced */

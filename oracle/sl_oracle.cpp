@@ -19,6 +19,7 @@
  *   float->int    = truncate; NaN/out of range -> INT32_MIN (x86 cvttss2si)
  *   max(a,b)      = (a < b) ? b : a                  SL_PostVertexTransform.hpp:520-523
  *   length(v)     = sqrtf(dot(v, v)), inversesqrt(x) = 1/sqrtf(x)   SL_LineProcessor.cpp:132,150; SL_LineRasterizer.cpp:39,61
+ *   (half)f       = IEEE binary16, round to nearest even   SL_Context.cpp:946-948, SL_FragmentProcessor.cpp:526-529
  * The build runs with -ffp-contract=off so that no other operation fuses.
  *
  * Deviations from the reference (undefined behaviour there):
@@ -52,7 +53,9 @@
  *   - float->u8 conversion of values outside [0,255] (low byte of the
  *     truncated int);
  *   - LightMath's length and inversesqrt (pinned above; an approximate
- *     reciprocal square root would move clipped line ends and line depths).
+ *     reciprocal square root would move clipped line ends and line depths);
+ *   - LightMath's float -> half rounding (pinned above to the IEEE default;
+ *     pinned against numpy's conversion in tests/test_formats.py).
  */
 #include <algorithm>
 #include <atomic>
@@ -204,12 +207,81 @@ uint32_t bytes_per_color(int type)
     switch (type)
     {
         case SL_COLOR_R_8U: return 1;
-        case SL_COLOR_R_32U: return 4;
-        case SL_COLOR_R_FLOAT: return 4;
+        case SL_COLOR_RG_8U: return 2;
+        case SL_COLOR_RGB_8U: return 3;
         case SL_COLOR_RGBA_8U: return 4;
+        case SL_COLOR_R_32U: return 4;
+        case SL_COLOR_R_HALF: return 2;
+        case SL_COLOR_R_FLOAT: return 4;
+        case SL_COLOR_RG_FLOAT: return 8;
+        case SL_COLOR_RGB_FLOAT: return 12;
         case SL_COLOR_RGBA_FLOAT: return 16;
         default: return 0;
     }
+}
+
+// Colour attachments drawn to: 8-bit unsigned (R, RG, RGB, RGBA) and float
+// (R, RG, RGB, RGBA).  Components per texel, or 0.
+int u8_components(int type)
+{
+    return type == SL_COLOR_R_8U ? 1 : type == SL_COLOR_RG_8U ? 2 : type == SL_COLOR_RGB_8U ? 3 : type == SL_COLOR_RGBA_8U ? 4 : 0;
+}
+
+int float_components(int type)
+{
+    return type == SL_COLOR_R_FLOAT ? 1 : type == SL_COLOR_RG_FLOAT ? 2 : type == SL_COLOR_RGB_FLOAT ? 3 : type == SL_COLOR_RGBA_FLOAT ? 4 : 0;
+}
+
+// IEEE binary16 <-> binary32 (ls::math::half, LightMath absent): the
+// conversion to half rounds to nearest, ties to even; overflow gives
+// infinity, NaN stays NaN.  Written from the two formats' definitions.
+uint16_t float_to_half(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    const uint32_t sign = (u >> 16) & 0x8000u;
+    const uint32_t ex = (u >> 23) & 0xFFu;
+    uint32_t man = u & 0x7FFFFFu;
+    if (ex == 0xFFu) return (uint16_t)(sign | 0x7C00u | (man ? 0x200u | (man >> 13) : 0u));
+    const int e = (int)ex - 127 + 15;
+    if (e >= 31) return (uint16_t)(sign | 0x7C00u);
+    if (e <= 0)
+    {
+        // subnormal half (or zero): value = man' * 2^-24
+        if (e < -10) return (uint16_t)sign;
+        man |= 0x800000u;
+        const int shift = 14 - e;          // 24-bit significand down to the subnormal grid
+        const uint32_t q = man >> shift, rem = man & ((1u << shift) - 1u), half = 1u << (shift - 1);
+        const uint32_t r = q + ((rem > half || (rem == half && (q & 1u))) ? 1u : 0u);
+        return (uint16_t)(sign | r);
+    }
+    const uint32_t q = man >> 13, rem = man & 0x1FFFu;
+    uint32_t r = ((uint32_t)e << 10) | q;
+    if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) ++r;   // may carry into the exponent (to infinity)
+    return (uint16_t)(sign | r);
+}
+
+float half_to_float(uint16_t h)
+{
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    const uint32_t ex = (h >> 10) & 0x1Fu, man = h & 0x3FFu;
+    uint32_t u;
+    if (ex == 0)
+    {
+        if (man == 0) u = sign;
+        else
+        {
+            int e = -1;
+            uint32_t m = man;
+            do { m <<= 1; ++e; } while (!(m & 0x400u));
+            u = sign | ((uint32_t)(127 - 15 - e) << 23) | ((m & 0x3FFu) << 13);
+        }
+    }
+    else if (ex == 0x1Fu) u = sign | 0x7F800000u | (man << 13);
+    else u = sign | ((ex - 15 + 127) << 23) | (man << 13);
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
 }
 
 uint32_t bytes_per_index(int type)
@@ -239,9 +311,11 @@ struct Bin
 struct Target
 {
     int W, H;
-    float* depth;
-    uint8_t* color8;          // RGBA8 attachment 0, or null
-    float* colorF;            // RGBA32F attachment 0, or null
+    float* depth;             // R32F depth attachment, or null
+    uint16_t* depthH;         // R16F depth attachment, or null
+    uint8_t* color8;          // 8-bit attachment 0 (comps components), or null
+    float* colorF;            // float attachment 0 (comps components), or null
+    int comps;                // components per colour texel
     uint32_t* dbgLast;
     uint32_t* dbgCount;
     float* vis;               // visibility attachment {seq bits, b0, b1, b2}, or null
@@ -547,36 +621,70 @@ void blend_rgba(int mode, const V4& s, float d[4])
     for (int c = 0; c < 4; ++c) d[c] = clamp01(d[c]);
 }
 
+// The depth texel as the depth test reads it ((float) of a half texel,
+// _sl_get_depth_texel, SL_TriRasterizer.cpp:94-101), and its write
+// ((depth_type)z, SL_FragmentProcessor.cpp:526-529).
+float depth_read(const Target& t, size_t idx)
+{
+    return t.depthH ? half_to_float(t.depthH[idx]) : t.depth[idx];
+}
+
+void depth_write(Target& t, size_t idx, float z)
+{
+    if (t.depthH) t.depthH[idx] = float_to_half(z);
+    else t.depth[idx] = z;
+}
+
+// assign_pixel / assign_alpha_pixel (SL_Framebuffer.cpp:42-80, 268-370) for
+// 8-bit and float attachments of 1..4 components.  A blend reads the
+// destination as (r, 0, 0, 1), (r, g, 0, 1), (r, r, r, 1) for one, two and
+// three components (the three-component read repeats the first component,
+// :306-307), blends all four, clamps and stores the texel's components.
 void place_color(const Draw& d, int x, int y, const V4& c)
 {
     const Target& t = d.tgt;
     const size_t idx = (size_t)x + (size_t)t.W * (size_t)y;
     const int blend = d.sh->blend;
+    const int n = t.comps;
+    auto read_dst = [&](const float v[4], float dst[4]) {
+        if (n == 1) { dst[0] = v[0]; dst[1] = 0.f; dst[2] = 0.f; dst[3] = 1.f; }
+        else if (n == 2) { dst[0] = v[0]; dst[1] = v[1]; dst[2] = 0.f; dst[3] = 1.f; }
+        else if (n == 3) { dst[0] = v[0]; dst[1] = v[0]; dst[2] = v[0]; dst[3] = 1.f; }
+        else for (int k = 0; k < 4; ++k) dst[k] = v[k];
+    };
     if (t.color8)
     {
-        uint8_t* px = t.color8 + idx * 4;
+        uint8_t* px = t.color8 + idx * (size_t)n;
         if (blend == SL_BLEND_OFF)
         {
-            for (int k = 0; k < 4; ++k) px[k] = (uint8_t)(uint32_t)f2i(c.v[k] * 255.f);
+            for (int k = 0; k < n; ++k) px[k] = (uint8_t)(uint32_t)f2i(c.v[k] * 255.f);
         }
         else
         {
-            float dst[4];
-            for (int k = 0; k < 4; ++k) dst[k] = (float)px[k] * 0.00392156862745f;
+            float v[4] = {0.f, 0.f, 0.f, 0.f}, dst[4];
+            for (int k = 0; k < n; ++k) v[k] = (float)px[k] * 0.00392156862745f;
+            // the missing components are the type's min (0) and max (255) cast to float
+            const float one = 255.f * 0.00392156862745f;
+            read_dst(v, dst);
+            if (n < 4) dst[3] = one;
             blend_rgba(blend, c, dst);
-            for (int k = 0; k < 4; ++k) px[k] = (uint8_t)(uint32_t)f2i(dst[k] * 255.f);
+            for (int k = 0; k < n; ++k) px[k] = (uint8_t)(uint32_t)f2i(dst[k] * 255.f);
         }
     }
     else if (t.colorF)
     {
-        float* px = t.colorF + idx * 4;
+        float* px = t.colorF + idx * (size_t)n;
         if (blend == SL_BLEND_OFF)
         {
-            for (int k = 0; k < 4; ++k) px[k] = c.v[k];
+            for (int k = 0; k < n; ++k) px[k] = c.v[k];
         }
         else
         {
-            blend_rgba(blend, c, px);
+            float v[4] = {0.f, 0.f, 0.f, 0.f}, dst[4];
+            for (int k = 0; k < n; ++k) v[k] = px[k];
+            read_dst(v, dst);
+            blend_rgba(blend, c, dst);
+            for (int k = 0; k < n; ++k) px[k] = dst[k];
         }
     }
 }
@@ -643,7 +751,7 @@ void finish_fragment(Draw& d, int x, int y, float z, const V4* var, uint32_t seq
     if (haveOutputs && d.sh->nout > 0) place_color(d, x, y, out);
 
     const size_t idx = (size_t)x + (size_t)d.tgt.W * (size_t)y;
-    if (d.sh->dmask == SL_DEPTH_MASK_ON) d.tgt.depth[idx] = z;   // regardless of the shader's return
+    if (d.sh->dmask == SL_DEPTH_MASK_ON) depth_write(d.tgt, idx, z);   // regardless of the shader's return
     if (d.tgt.dbgLast) d.tgt.dbgLast[idx] = seq;
     if (d.tgt.dbgCount) d.tgt.dbgCount[idx] += 1u;
     if (d.tgt.vis)
@@ -727,7 +835,7 @@ void raster_wire_bin(Draw& d, const Bin& b, int32_t rowThread, int32_t numRowThr
             for (int k = 0; k < 3; ++k) bc[k] = fmaf(b.bx.v[k], xf, bcY[k]);
             bc[3] = fmaf(b.bx.v[3], 0.f, bcY[3]);
             const float z = (((dep[0] * bc[0]) + (dep[1] * bc[1])) + (dep[2] * bc[2])) + (0.f * bc[3]);
-            const float dv = d.tgt.depth[(size_t)x + (size_t)W * (size_t)y];
+            const float dv = depth_read(d.tgt, (size_t)x + (size_t)W * (size_t)y);
             ++d.tested;
             if (depth_func(dfunc, z, dv)) shade_fragment(d, b, x, y, z, bc);
         }
@@ -765,7 +873,7 @@ void raster_point_bin(Draw& d, const Bin& b, int32_t rowThread, int32_t numRowTh
     if (py % numRowThreads != rowThread) return;
     const float z = b.p[0].v[2];
     ++d.tested;
-    if (!depth_func(d.sh->dtest, z, d.tgt.depth[(size_t)px + (size_t)d.tgt.W * (size_t)py])) return;
+    if (!depth_func(d.sh->dtest, z, depth_read(d.tgt, (size_t)px + (size_t)d.tgt.W * (size_t)py))) return;
     V4 var[SL_SHADER_MAX_VARYING_VECTORS];
     for (unsigned i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i) var[i] = (i < d.nvary) ? b.var[0][i] : v4(0.f, 0.f, 0.f, 0.f);
     finish_fragment(d, px, py, z, var, b.seq);
@@ -792,7 +900,7 @@ void raster_line_bin(Draw& d, const Bin& b, int32_t rowThread, int32_t numRowThr
         const float interp = len * dist;
         const float z = z0 + interp * (z1 - z0);                                     // mix(z0, z1, interp)
         ++d.tested;
-        if (!depth_func(d.sh->dtest, z, d.tgt.depth[(size_t)qx + (size_t)d.tgt.W * (size_t)qy])) return;
+        if (!depth_func(d.sh->dtest, z, depth_read(d.tgt, (size_t)qx + (size_t)d.tgt.W * (size_t)qy))) return;
         V4 var[SL_SHADER_MAX_VARYING_VECTORS];
         for (unsigned i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
             var[i] = (i < d.nvary) ? mix4(b.var[0][i], b.var[1][i], interp) : v4(0.f, 0.f, 0.f, 0.f);
@@ -883,7 +991,7 @@ void raster_bin(Draw& d, const Bin& b, int32_t rowThread, int32_t numRowThreads)
                     {
                         // z = depth * bc (SL_TriRasterizer.cpp:868), 4th term = 0 * bc[3]
                         const float z = (((dep[0] * bc[i][0]) + (dep[1] * bc[i][1])) + (dep[2] * bc[i][2])) + (0.f * bc[i][3]);
-                        const float dv = d.tgt.depth[(size_t)x + (size_t)W * (size_t)y];
+                        const float dv = depth_read(d.tgt, (size_t)x + (size_t)W * (size_t)y);
                         ++d.tested;
                         if (depth_func(dfunc, z, dv)) shade_fragment(d, b, x, y, z, bc[i]);
                     }
@@ -1287,24 +1395,26 @@ int setup_draw(sl_context* c, size_t shaderId, size_t fboId, Draw& d)
     Fbo& f = c->fbos[fboId];
     if (f.depth >= c->texs.size() || !c->texs[f.depth].valid) return SL_ERR_STATE;
     Tex& dt = c->texs[f.depth];
-    if (dt.type != SL_COLOR_R_FLOAT) return SL_ERR_UNSUPPORTED;
+    if (dt.type != SL_COLOR_R_FLOAT && dt.type != SL_COLOR_R_HALF) return SL_ERR_UNSUPPORTED;
     d.ctx = c;
     d.sh = &sh;
     d.uni = (sh.ubo < c->ubos.size()) ? c->ubos[sh.ubo].bytes : nullptr;
     if (d.uni == nullptr && (sh.vsProg != SL_PROGRAM_PASSTHROUGH_GOURAUD || sh.fsProg != SL_PROGRAM_PASSTHROUGH_GOURAUD)) return SL_ERR_STATE;
     d.tgt.W = dt.w;
     d.tgt.H = dt.h;
-    d.tgt.depth = reinterpret_cast<float*>(dt.data.data());
+    d.tgt.depth = (dt.type == SL_COLOR_R_FLOAT) ? reinterpret_cast<float*>(dt.data.data()) : nullptr;
+    d.tgt.depthH = (dt.type == SL_COLOR_R_HALF) ? reinterpret_cast<uint16_t*>(dt.data.data()) : nullptr;
     d.tgt.color8 = nullptr;
     d.tgt.colorF = nullptr;
+    d.tgt.comps = 4;
     if (sh.nout > 1) return SL_ERR_UNSUPPORTED;
     if (sh.nout == 1)
     {
         if (f.colors.empty() || f.colors[0] >= c->texs.size() || !c->texs[f.colors[0]].valid) return SL_ERR_STATE;
         Tex& ct = c->texs[f.colors[0]];
         if (ct.w != dt.w || ct.h != dt.h) return SL_ERR_STATE;
-        if (ct.type == SL_COLOR_RGBA_8U) d.tgt.color8 = ct.data.data();
-        else if (ct.type == SL_COLOR_RGBA_FLOAT) d.tgt.colorF = reinterpret_cast<float*>(ct.data.data());
+        if (u8_components(ct.type)) { d.tgt.color8 = ct.data.data(); d.tgt.comps = u8_components(ct.type); }
+        else if (float_components(ct.type)) { d.tgt.colorF = reinterpret_cast<float*>(ct.data.data()); d.tgt.comps = float_components(ct.type); }
         else return SL_ERR_UNSUPPORTED;
     }
     d.tgt.dbgLast = (f.dbgLast < c->texs.size()) ? reinterpret_cast<uint32_t*>(c->texs[f.dbgLast].data.data()) : nullptr;
@@ -1316,7 +1426,8 @@ int setup_draw(sl_context* c, size_t shaderId, size_t fboId, Draw& d)
     {
         Tex& vt = c->texs[f.vis];
         if (vt.type != SL_COLOR_RGBA_FLOAT || vt.w != dt.w || vt.h != dt.h) return SL_ERR_STATE;
-        if (d.tgt.colorF == nullptr && sh.nout > 0) return SL_ERR_UNSUPPORTED;
+        if ((d.tgt.colorF == nullptr || d.tgt.comps != 4) && sh.nout > 0) return SL_ERR_UNSUPPORTED;
+        if (d.tgt.depthH) return SL_ERR_UNSUPPORTED;
         if (sh.nvary >= SL_SHADER_MAX_VARYING_VECTORS) return SL_ERR_UNSUPPORTED;
         d.tgt.vis = reinterpret_cast<float*>(vt.data.data());
         d.visSlot = sh.nvary;
@@ -1511,17 +1622,18 @@ int clear_color(sl_context* c, size_t fbo, unsigned att, const double rgba[4])
     Fbo& f = c->fbos[fbo];
     if (att >= f.colors.size() || f.colors[att] >= c->texs.size()) return SL_ERR_INVALID_HANDLE;
     Tex& t = c->texs[f.colors[att]];
-    if (t.type == SL_COLOR_RGBA_8U)
+    // sl_match_color_for_type (SL_Color.cpp:137-180): the clear colour cast to the texel type
+    if (const int n = u8_components(t.type))
     {
         uint8_t v[4];
-        for (int k = 0; k < 4; ++k) v[k] = (uint8_t)(int64_t)(rgba[k] * 255.0);   // color_cast<uint8_t,double>, SL_Color.cpp:137
-        clear_texture_bytes(t, v, 4);
+        for (int k = 0; k < 4; ++k) v[k] = (uint8_t)(int64_t)(rgba[k] * 255.0);   // color_cast<uint8_t,double>
+        clear_texture_bytes(t, v, (uint32_t)n);
     }
-    else if (t.type == SL_COLOR_RGBA_FLOAT)
+    else if (const int n = float_components(t.type))
     {
         float v[4];
         for (int k = 0; k < 4; ++k) v[k] = (float)rgba[k];
-        clear_texture_bytes(t, v, 16);
+        clear_texture_bytes(t, v, 4u * (uint32_t)n);
     }
     else return SL_ERR_UNSUPPORTED;
     return SL_OK;
@@ -1533,9 +1645,17 @@ int clear_depth(sl_context* c, size_t fbo, double depth)
     Fbo& f = c->fbos[fbo];
     if (f.depth >= c->texs.size()) return SL_ERR_STATE;
     Tex& t = c->texs[f.depth];
-    if (t.type != SL_COLOR_R_FLOAT) return SL_ERR_UNSUPPORTED;
-    const float v = (float)depth;
-    clear_texture_bytes(t, &v, 4);
+    if (t.type == SL_COLOR_R_FLOAT)
+    {
+        const float v = (float)depth;
+        clear_texture_bytes(t, &v, 4);
+    }
+    else if (t.type == SL_COLOR_R_HALF)
+    {
+        const uint16_t v = float_to_half((float)depth);   // (half)(float)depth, SL_Context.cpp:946-948
+        clear_texture_bytes(t, &v, 2);
+    }
+    else return SL_ERR_UNSUPPORTED;
     if (f.dbgLast < c->texs.size()) { const uint32_t z = 0; clear_texture_bytes(c->texs[f.dbgLast], &z, 4); }
     if (f.dbgCount < c->texs.size()) { const uint32_t z = 0; clear_texture_bytes(c->texs[f.dbgCount], &z, 4); }
     if (f.vis < c->texs.size()) { const float z[4] = {0.f, 0.f, 0.f, 0.f}; clear_texture_bytes(c->texs[f.vis], z, 16); }

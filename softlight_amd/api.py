@@ -40,17 +40,20 @@ RENDER_MODE_TRI_WIRE = 0x50
 RENDER_MODE_INDEXED_TRI_WIRE = 0x70
 VERTEX_DIMENSION_1, VERTEX_DIMENSION_2, VERTEX_DIMENSION_3, VERTEX_DIMENSION_4 = range(4)
 VERTEX_DATA_BYTE, VERTEX_DATA_SHORT, VERTEX_DATA_INT, VERTEX_DATA_LONG, VERTEX_DATA_FLOAT = range(5)
-SL_COLOR_R_8U, SL_COLOR_R_32U, SL_COLOR_R_FLOAT = 0, 2, 5
+SL_COLOR_R_8U, SL_COLOR_R_32U, SL_COLOR_R_HALF, SL_COLOR_R_FLOAT = 0, 2, 4, 5
+SL_COLOR_RG_8U, SL_COLOR_RG_FLOAT = 7, 12
+SL_COLOR_RGB_8U, SL_COLOR_RGB_FLOAT = 14, 19
 SL_COLOR_RGBA_8U, SL_COLOR_RGBA_FLOAT = 21, 26
 
 SL_PROGRAM_PASSTHROUGH_GOURAUD = 1
 SL_PROGRAM_LAMBERT_TEXTURED = 2
 SL_PROGRAM_COLOR_TEXTURED = 3
 
-_BYTES_PER_TEXEL = {SL_COLOR_R_8U: 1, SL_COLOR_R_32U: 4, SL_COLOR_R_FLOAT: 4,
-                    SL_COLOR_RGBA_8U: 4, SL_COLOR_RGBA_FLOAT: 16}
-_TEXEL_DTYPE = {SL_COLOR_R_8U: (np.uint8, 1), SL_COLOR_R_32U: (np.uint32, 1), SL_COLOR_R_FLOAT: (np.float32, 1),
-                SL_COLOR_RGBA_8U: (np.uint8, 4), SL_COLOR_RGBA_FLOAT: (np.float32, 4)}
+_TEXEL_DTYPE = {SL_COLOR_R_8U: (np.uint8, 1), SL_COLOR_RG_8U: (np.uint8, 2), SL_COLOR_RGB_8U: (np.uint8, 3),
+                SL_COLOR_RGBA_8U: (np.uint8, 4), SL_COLOR_R_32U: (np.uint32, 1), SL_COLOR_R_HALF: (np.float16, 1),
+                SL_COLOR_R_FLOAT: (np.float32, 1), SL_COLOR_RG_FLOAT: (np.float32, 2),
+                SL_COLOR_RGB_FLOAT: (np.float32, 3), SL_COLOR_RGBA_FLOAT: (np.float32, 4)}
+_BYTES_PER_TEXEL = {t: np.dtype(dt).itemsize * n for t, (dt, n) in _TEXEL_DTYPE.items()}
 
 
 class sl_mesh(C.Structure):

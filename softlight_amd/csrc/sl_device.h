@@ -303,8 +303,10 @@ struct RasterParams
     const uint2* tileRange;       // [start, end) into tileRecs per tile
     const uint32_t* tileOrder;    // launch order of the tiles (heaviest first), or null
     int32_t W, H, tilesX, tilesY;
-    uint8_t* color;               // attachment 0 texels (RGBA8 or RGBA32F) or null
+    uint8_t* color;               // attachment 0 texels (8-bit or float, colorComps components) or null
     int32_t colorType;
+    int32_t colorComps;           // components per colour texel (1..4)
+    int32_t depthHalf;            // the depth attachment is R16F (else R32F)
     float* depth;
     uint32_t* dbgLast;
     uint32_t* dbgCount;

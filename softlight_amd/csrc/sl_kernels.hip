@@ -23,6 +23,8 @@
 // clip_and_process_tris (SL_TriProcessor.cpp:248-640), SL_TriRasterizer::
 // render_triangle_simd generic branch (SL_TriRasterizer.cpp:806-923),
 // SL_FragmentProcessor::flush_tri_fragments (SL_FragmentProcessor.cpp:416-531).
+#include <hip/hip_fp16.h>
+
 #include "sl_device.h"
 #include "sl_recurrence.h"
 #include "sl_stages.h"
@@ -1501,8 +1503,70 @@ __device__ __forceinline__ bool shade(const RasterParams& P, const DrawDesc& D, 
     return (D.nout > 0) && fragment_program(D, P.uniforms, var, out);
 }
 
+// Attachment texels of P.colorComps components (8-bit: packed into the low
+// bytes of a u32; float: the leading components of a float4), and the
+// depth texel (R32F, or R16F read as float and written rounded to nearest).
+__device__ __forceinline__ uint32_t load_color8(const RasterParams& P, uint64_t gi)
+{
+    const int n = P.colorComps;
+    if (n == 4) return reinterpret_cast<const uint32_t*>(P.color)[gi];
+    const uint8_t* p = P.color + gi * (uint64_t)n;
+    uint32_t v = 0;
+    for (int k = 0; k < n; ++k) v |= (uint32_t)p[k] << (8 * k);
+    return v;
+}
+
+__device__ __forceinline__ void store_color8(const RasterParams& P, uint64_t gi, uint32_t v)
+{
+    const int n = P.colorComps;
+    if (n == 4) { reinterpret_cast<uint32_t*>(P.color)[gi] = v; return; }
+    uint8_t* p = P.color + gi * (uint64_t)n;
+    for (int k = 0; k < n; ++k) p[k] = (uint8_t)(v >> (8 * k));
+}
+
+__device__ __forceinline__ float4 load_colorF(const RasterParams& P, uint64_t gi)
+{
+    const int n = P.colorComps;
+    if (n == 4) return reinterpret_cast<const float4*>(P.color)[gi];
+    const float* p = reinterpret_cast<const float*>(P.color) + gi * (uint64_t)n;
+    return make_float4(p[0], n > 1 ? p[1] : 0.f, n > 2 ? p[2] : 0.f, 0.f);
+}
+
+__device__ __forceinline__ void store_colorF(const RasterParams& P, uint64_t gi, float4 v)
+{
+    const int n = P.colorComps;
+    if (n == 4) { reinterpret_cast<float4*>(P.color)[gi] = v; return; }
+    float* p = reinterpret_cast<float*>(P.color) + gi * (uint64_t)n;
+    p[0] = v.x;
+    if (n > 1) p[1] = v.y;
+    if (n > 2) p[2] = v.z;
+}
+
+__device__ __forceinline__ float load_depth(const RasterParams& P, uint64_t gi)
+{
+    return P.depthHalf ? __half2float(reinterpret_cast<const __half*>(P.depth)[gi]) : P.depth[gi];
+}
+
+__device__ __forceinline__ void store_depth(const RasterParams& P, uint64_t gi, float z)
+{
+    if (P.depthHalf) reinterpret_cast<__half*>(P.depth)[gi] = __float2half_rn(z);
+    else P.depth[gi] = z;
+}
+
+// The destination a blend reads from a texel of n < 4 components
+// (assign_alpha_pixel, SL_Framebuffer.cpp:296-311): (r, min, min, max),
+// (r, g, min, max), and for three components (r, r, r, max).
+__device__ __forceinline__ void blend_dst_read(int n, float one, float d[4])
+{
+    if (n == 4) return;
+    if (n == 3) { d[1] = d[0]; d[2] = d[0]; }
+    if (n == 1) d[1] = 0.f;
+    if (n < 3) d[2] = 0.f;
+    d[3] = one;
+}
+
 template <int CF>
-__device__ __forceinline__ void blend_into(uint32_t* col8, float4* colF, int blendMode, float4 out)
+__device__ __forceinline__ void blend_into(uint32_t* col8, float4* colF, int blendMode, float4 out, int comps)
 {
     if (CF == 1)
     {
@@ -1515,6 +1579,7 @@ __device__ __forceinline__ void blend_into(uint32_t* col8, float4* colF, int ble
         {
             float dcol[4];
             unpack_rgba8(*col8, dcol);
+            blend_dst_read(comps, 255.f * 0.00392156862745f, dcol);   // max (255) cast to float
             blend(blendMode, out, dcol);
             *col8 = pack_rgba8(dcol);
         }
@@ -1529,6 +1594,7 @@ __device__ __forceinline__ void blend_into(uint32_t* col8, float4* colF, int ble
         {
             const float4 cur = *colF;
             float dcol[4] = {cur.x, cur.y, cur.z, cur.w};
+            blend_dst_read(comps, 1.f, dcol);
             blend(blendMode, out, dcol);
             *colF = make_float4(dcol[0], dcol[1], dcol[2], dcol[3]);
         }
@@ -1590,11 +1656,11 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
         const int gx = x0 + c, gy = y0 + r;
         const bool inb = gx < P.W && gy < P.H;
         const uint64_t gi = (uint64_t)gy * P.W + gx;
-        sDepth[r][c] = P.clearDepth ? P.clearDepthValue : (inb ? P.depth[gi] : 0.f);
-        if (CF == 1) sCol8[r][c] = P.clearColor ? P.clearRGBA8 : (inb ? reinterpret_cast<const uint32_t*>(P.color)[gi] : 0u);
+        sDepth[r][c] = P.clearDepth ? P.clearDepthValue : (inb ? load_depth(P, gi) : 0.f);
+        if (CF == 1) sCol8[r][c] = P.clearColor ? P.clearRGBA8 : (inb ? load_color8(P, gi) : 0u);
         if (CF == 2)
             sColF[r][c] = P.clearColor ? make_float4(P.clearRGBAF[0], P.clearRGBAF[1], P.clearRGBAF[2], P.clearRGBAF[3])
-                                       : (inb ? reinterpret_cast<const float4*>(P.color)[gi] : make_float4(0.f, 0.f, 0.f, 0.f));
+                                       : (inb ? load_colorF(P, gi) : make_float4(0.f, 0.f, 0.f, 0.f));
         if (DBG)
         {
             sLast[r][c] = P.clearDepth ? 0u : (inb ? P.dbgLast[gi] : 0u);
@@ -1615,7 +1681,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
         const float qhw[3] = {Q.hw[0], Q.hw[1], Q.hw[2]};
         float4 out, wv;
         const bool have = shade(P, QD, qhw, Q.bc3, Q.state, prid, sPendS[0][r][c], sPendS[1][r][c], sPendS[2][r][c], out, wv);
-        if (have) blend_into<CF>(&sCol8[CF == 1 ? r : 0][CF == 1 ? c : 0], &sColF[CF == 2 ? r : 0][CF == 2 ? c : 0], SL_BLEND_OFF, out);
+        if (have) blend_into<CF>(&sCol8[CF == 1 ? r : 0][CF == 1 ? c : 0], &sColF[CF == 2 ? r : 0][CF == 2 ? c : 0], SL_BLEND_OFF, out, P.colorComps);
         if (VIS) sVis[r][c] = make_float4(__uint_as_float(Q.seq), wv.x, wv.y, wv.z);
     };
 
@@ -1736,10 +1802,10 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                         float4 out, wv;
                         const bool have = shade(P, D, hw, R.bc3, st, rid, s0, s1, s2, out, wv);
                         if (have) blend_into<CF>(&sCol8[CF == 1 ? row : 0][CF == 1 ? c : 0],
-                                                 &sColF[CF == 2 ? row : 0][CF == 2 ? c : 0], blendMode, out);
+                                                 &sColF[CF == 2 ? row : 0][CF == 2 ? c : 0], blendMode, out, P.colorComps);
                         if (VIS) sVis[row][c] = make_float4(__uint_as_float(R.seq), wv.x, wv.y, wv.z);
                     }
-                    if (dmask) sDepth[row][c] = z;
+                    if (dmask) sDepth[row][c] = P.depthHalf ? __half2float(__float2half_rn(z)) : z;
                     if (DBG)
                     {
                         sLast[row][c] = R.seq;
@@ -1948,9 +2014,9 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
         if (gx < P.W && gy < P.H)
         {
             const uint64_t gi = (uint64_t)gy * P.W + gx;
-            P.depth[gi] = sDepth[r][c];
-            if (CF == 1) reinterpret_cast<uint32_t*>(P.color)[gi] = sCol8[r][c];
-            if (CF == 2) reinterpret_cast<float4*>(P.color)[gi] = sColF[r][c];
+            store_depth(P, gi, sDepth[r][c]);
+            if (CF == 1) store_color8(P, gi, sCol8[r][c]);
+            if (CF == 2) store_colorF(P, gi, sColF[r][c]);
             if (DBG)
             {
                 P.dbgLast[gi] = sLast[r][c];
@@ -2171,7 +2237,9 @@ static_assert(kTileConfigs[0].tileW == 32 && kTileConfigs[0].tileH == 16 && kTil
 hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s)
 {
     (void)hipGetLastError();   // drop a stale error from an earlier call
-    const int cf = (p.color == nullptr) ? 0 : (p.colorType == SL_COLOR_RGBA_FLOAT ? 2 : 1);
+    const bool floatColor = p.colorType == SL_COLOR_R_FLOAT || p.colorType == SL_COLOR_RG_FLOAT ||
+                            p.colorType == SL_COLOR_RGB_FLOAT || p.colorType == SL_COLOR_RGBA_FLOAT;
+    const int cf = (p.color == nullptr) ? 0 : (floatColor ? 2 : 1);
     if (p.vis != nullptr)
     {
         // the visibility attachment is only built for RGBA32F colour targets

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -110,8 +111,62 @@ uint32_t bytes_per_color(int type)
         case SL_COLOR_R_FLOAT: return 4;
         case SL_COLOR_RGBA_8U: return 4;
         case SL_COLOR_RGBA_FLOAT: return 16;
+        case SL_COLOR_RG_8U: return 2;
+        case SL_COLOR_RGB_8U: return 3;
+        case SL_COLOR_R_HALF: return 2;
+        case SL_COLOR_RG_FLOAT: return 8;
+        case SL_COLOR_RGB_FLOAT: return 12;
         default: return 0;
     }
+}
+
+// Colour attachments drawn to: 8-bit unsigned and float, 1..4 components.
+int u8_components(int type)
+{
+    return type == SL_COLOR_R_8U ? 1 : type == SL_COLOR_RG_8U ? 2 : type == SL_COLOR_RGB_8U ? 3 : type == SL_COLOR_RGBA_8U ? 4 : 0;
+}
+
+int float_components(int type)
+{
+    return type == SL_COLOR_R_FLOAT ? 1 : type == SL_COLOR_RG_FLOAT ? 2 : type == SL_COLOR_RGB_FLOAT ? 3 : type == SL_COLOR_RGBA_FLOAT ? 4 : 0;
+}
+
+bool depth_format(int type) { return type == SL_COLOR_R_FLOAT || type == SL_COLOR_R_HALF; }
+
+// float -> IEEE binary16, round to nearest even (the depth clear value of a
+// half depth attachment, (half)(float)depth, SL_Context.cpp:946-948), and back.
+uint16_t host_float_to_half(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    const uint32_t sign = (u >> 16) & 0x8000u, ex = (u >> 23) & 0xFFu;
+    uint32_t man = u & 0x7FFFFFu;
+    if (ex == 0xFFu) return (uint16_t)(sign | 0x7C00u | (man ? 0x200u | (man >> 13) : 0u));
+    const int e = (int)ex - 112;
+    if (e >= 31) return (uint16_t)(sign | 0x7C00u);
+    if (e <= 0)
+    {
+        if (e < -10) return (uint16_t)sign;
+        man |= 0x800000u;
+        const int sh = 14 - e;
+        const uint32_t q = man >> sh, rem = man & ((1u << sh) - 1u), hf = 1u << (sh - 1);
+        return (uint16_t)(sign | (q + ((rem > hf || (rem == hf && (q & 1u))) ? 1u : 0u)));
+    }
+    const uint32_t q = man >> 13, rem = man & 0x1FFFu;
+    uint32_t r = ((uint32_t)e << 10) | q;
+    if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) ++r;
+    return (uint16_t)(sign | r);
+}
+
+float host_half_to_float(uint16_t h)
+{
+    const float m = (float)(h & 0x3FFu);
+    const int ex = (h >> 10) & 0x1F;
+    float v;
+    if (ex == 0) v = std::ldexp(m, -24);
+    else if (ex == 31) v = (h & 0x3FFu) ? std::numeric_limits<float>::quiet_NaN() : std::numeric_limits<float>::infinity();
+    else v = std::ldexp(1024.f + m, ex - 25);
+    return (h & 0x8000u) ? -v : v;
 }
 
 uint32_t bytes_per_index(int type)
@@ -280,9 +335,9 @@ int make_plan(sl_context* c, const Batch& B, Plan& p)
     if (B.fbo >= c->fbos.size()) return fail(c, SL_ERR_INVALID_HANDLE, "batch framebuffer");
     Fbo& F = c->fbos[B.fbo];
     p.dt = tex_at(c, F.depth);
-    if (!p.dt || p.dt->type != SL_COLOR_R_FLOAT) return fail(c, SL_ERR_STATE, "framebuffer needs an R_FLOAT depth attachment");
+    if (!p.dt || !depth_format(p.dt->type)) return fail(c, SL_ERR_STATE, "framebuffer needs an R_FLOAT or R_HALF depth attachment");
     p.ct = F.colors.empty() ? nullptr : tex_at(c, F.colors[0]);
-    if (p.ct && p.ct->type != SL_COLOR_RGBA_8U && p.ct->type != SL_COLOR_RGBA_FLOAT) return fail(c, SL_ERR_UNSUPPORTED, "colour attachment format");
+    if (p.ct && !u8_components(p.ct->type) && !float_components(p.ct->type)) return fail(c, SL_ERR_UNSUPPORTED, "colour attachment format");
     p.dbgL = tex_at(c, F.dbgLast);
     p.dbgC = tex_at(c, F.dbgCount);
     p.debug = p.dbgL && p.dbgC;
@@ -294,6 +349,7 @@ int make_plan(sl_context* c, const Batch& B, Plan& p)
     {
         if (p.vis->type != SL_COLOR_RGBA_FLOAT || p.vis->w != p.W || p.vis->h != p.H) return fail(c, SL_ERR_STATE, "visibility attachment must be RGBA_FLOAT of the framebuffer's size");
         if (!p.ct || p.ct->type != SL_COLOR_RGBA_FLOAT) return fail(c, SL_ERR_UNSUPPORTED, "visibility needs an RGBA_FLOAT colour attachment 0");
+        if (p.dt->type != SL_COLOR_R_FLOAT) return fail(c, SL_ERR_UNSUPPORTED, "visibility needs an R_FLOAT depth attachment");
     }
     p.cfg = (c->tileConfig >= 0 && c->tileConfig < kNumTileConfigs) ? c->tileConfig : kDefaultTileConfig;
     p.tc = kTileConfigs[p.cfg];
@@ -383,6 +439,8 @@ RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool have
     rp.W = p.W; rp.H = p.H; rp.tilesX = p.tilesX; rp.tilesY = p.tilesY;
     rp.color = p.ct ? reinterpret_cast<uint8_t*>(p.ct->buf.p) : nullptr;
     rp.colorType = p.ct ? p.ct->type : SL_COLOR_RGBA_8U;
+    rp.colorComps = p.ct ? (u8_components(p.ct->type) ? u8_components(p.ct->type) : float_components(p.ct->type)) : 4;
+    rp.depthHalf = (p.dt->type == SL_COLOR_R_HALF) ? 1 : 0;
     rp.depth = reinterpret_cast<float*>(p.dt->buf.p);
     rp.dbgLast = p.debug ? reinterpret_cast<uint32_t*>(p.dbgL->buf.p) : nullptr;
     rp.dbgCount = p.debug ? reinterpret_cast<uint32_t*>(p.dbgC->buf.p) : nullptr;
@@ -396,7 +454,7 @@ RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool have
         rp.clearRGBAF[k] = (float)B.rgba[k];
     }
     rp.clearRGBA8 = packed;
-    rp.clearDepthValue = (float)B.depth;
+    rp.clearDepthValue = rp.depthHalf ? host_half_to_float(host_float_to_half((float)B.depth)) : (float)B.depth;
     rp.extra = 0;
     for (const DrawDesc& d : B.draws) rp.extra |= (d.prim != kPrimFill) ? 1 : 0;
     rp.ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
@@ -749,7 +807,7 @@ int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t s
     if (vis && sh.nvary >= SL_SHADER_MAX_VARYING_VECTORS) return fail(c, SL_ERR_UNSUPPORTED, "visibility needs a free varying slot (numVaryings <= 3)");
     if (vis && prim != kPrimFill) return fail(c, SL_ERR_UNSUPPORTED, "visibility attachments cover filled triangles only");
     Tex* dt = tex_at(c, F.depth);
-    if (!dt || dt->type != SL_COLOR_R_FLOAT) return fail(c, SL_ERR_STATE, "framebuffer needs an R_FLOAT depth attachment");
+    if (!dt || !depth_format(dt->type)) return fail(c, SL_ERR_STATE, "framebuffer needs an R_FLOAT or R_HALF depth attachment");
     if (prim == kPrimLine && (dt->w > kMaxLineTarget || dt->h > kMaxLineTarget))
         return fail(c, SL_ERR_UNSUPPORTED, "line draws need a target of at most 8192 x 8192");
     if (sh.nout == 1)
@@ -869,17 +927,19 @@ int fill_texture(sl_context* c, Tex& t, const void* value, uint32_t bpt)
         std::memcpy(&v, value, 4);
         return check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(t.buf.p), v, n, c->stream), "fill texture");
     }
-    if (bpt == 16)
+    if (bpt % 4 == 0)
     {
         uint32_t v[4];
-        std::memcpy(v, value, 16);
-        if (v[0] == v[1] && v[1] == v[2] && v[2] == v[3])
-            return check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(t.buf.p), v[0], 4 * n, c->stream), "fill texture");
-        std::vector<uint8_t> host(n * 16);
-        for (size_t i = 0; i < n; ++i) std::memcpy(host.data() + 16 * i, value, 16);
-        return upload(c, t.buf.p, host.data(), host.size(), "fill texture");
+        std::memcpy(v, value, bpt);
+        bool same = true;
+        for (uint32_t k = 1; k < bpt / 4; ++k) same = same && v[k] == v[0];
+        if (same)
+            return check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(t.buf.p), v[0], (bpt / 4) * n, c->stream), "fill texture");
     }
-    return fail(c, SL_ERR_UNSUPPORTED, "clear of this texture format");
+    if (bpt == 0 || bpt > 16) return fail(c, SL_ERR_UNSUPPORTED, "clear of this texture format");
+    std::vector<uint8_t> host(n * bpt);
+    for (size_t i = 0; i < n; ++i) std::memcpy(host.data() + bpt * i, value, bpt);
+    return upload(c, t.buf.p, host.data(), host.size(), "fill texture");
 }
 
 int record_clear(sl_context* c, size_t fboId, int att, const double* rgba, bool depth, double dval)
@@ -890,13 +950,13 @@ int record_clear(sl_context* c, size_t fboId, int att, const double* rgba, bool 
     {
         if (att < 0 || (size_t)att >= F.colors.size() || !tex_at(c, F.colors[att])) return fail(c, SL_ERR_INVALID_HANDLE, "colour attachment");
         const int ty = c->texs[F.colors[att]].type;
-        if (ty != SL_COLOR_RGBA_8U && ty != SL_COLOR_RGBA_FLOAT) return fail(c, SL_ERR_UNSUPPORTED, "clear of this colour format");
+        if (!u8_components(ty) && !float_components(ty)) return fail(c, SL_ERR_UNSUPPORTED, "clear of this colour format");
     }
     if (depth)
     {
         Tex* dt = tex_at(c, F.depth);
         if (!dt) return fail(c, SL_ERR_STATE, "no depth attachment");
-        if (dt->type != SL_COLOR_R_FLOAT) return fail(c, SL_ERR_UNSUPPORTED, "depth format");
+        if (!depth_format(dt->type)) return fail(c, SL_ERR_UNSUPPORTED, "depth format");
     }
     // a clear after draws, or of another framebuffer, ends the current batch
     if (!c->batch.empty() && (c->batch.fbo != fboId || !c->batch.draws.empty()))
@@ -908,15 +968,15 @@ int record_clear(sl_context* c, size_t fboId, int att, const double* rgba, bool 
     {
         // attachments other than 0 are cleared directly
         Tex& t = c->texs[F.colors[att]];
-        if (t.type == SL_COLOR_RGBA_8U)
+        if (const int n = u8_components(t.type))
         {
             uint8_t v[4];
             for (int k = 0; k < 4; ++k) v[k] = (uint8_t)(int64_t)(rgba[k] * 255.0);
-            return fill_texture(c, t, v, 4);
+            return fill_texture(c, t, v, (uint32_t)n);
         }
         float v[4];
         for (int k = 0; k < 4; ++k) v[k] = (float)rgba[k];
-        return fill_texture(c, t, v, 16);
+        return fill_texture(c, t, v, 4u * (uint32_t)float_components(t.type));
     }
     Batch& B = c->batch;
     B.fbo = fboId;

@@ -183,9 +183,10 @@ class Scene:
         return out
 
 
-def make_target(ctx: A.SLContext, W: int, H: int, debug: bool, color_type=A.SL_COLOR_RGBA_8U):
+def make_target(ctx: A.SLContext, W: int, H: int, debug: bool, color_type=A.SL_COLOR_RGBA_8U,
+                depth_type=A.SL_COLOR_R_FLOAT):
     color = ctx.create_texture(color_type, W, H)
-    depth = ctx.create_texture(A.SL_COLOR_R_FLOAT, W, H)
+    depth = ctx.create_texture(depth_type, W, H)
     fbo = ctx.create_framebuffer([color], depth)
     dbg = None
     if debug:
@@ -215,20 +216,23 @@ def random_triangle_vertices(seed: int, ntri: int) -> np.ndarray:
 
 
 def build_from_vertex_records(ctx, rec: np.ndarray, W: int, H: int, debug=False, shader_kw=None,
-                              color_type=A.SL_COLOR_RGBA_8U, mode=A.RENDER_MODE_TRIANGLES) -> Scene:
+                              color_type=A.SL_COLOR_RGBA_8U, mode=A.RENDER_MODE_TRIANGLES,
+                              depth_type=A.SL_COLOR_R_FLOAT) -> Scene:
     nv = rec.shape[0]
     vbo = ctx.create_vbo(rec)
     vao = ctx.create_vao(vbo, None, [(0, 16, A.VERTEX_DIMENSION_3, A.VERTEX_DATA_FLOAT),
                                      (12, 16, A.VERTEX_DIMENSION_4, A.VERTEX_DATA_BYTE)])
-    color, depth, fbo, dbg = make_target(ctx, W, H, debug, color_type)
+    color, depth, fbo, dbg = make_target(ctx, W, H, debug, color_type, depth_type)
     shader = gouraud_shader(ctx, **(shader_kw or {}))
     mesh = A.sl_mesh(vao, 0, nv, mode, 0)
     return Scene(ctx, W, H, fbo, color, depth, shader, [("draw", mesh, 1)], n_triangles=nv // 3, debug=dbg)
 
 
-def build_random_triangles(ctx, seed=0, ntri=4096, W=640, H=480, debug=False, wire=False) -> Scene:
+def build_random_triangles(ctx, seed=0, ntri=4096, W=640, H=480, debug=False, wire=False,
+                           color_type=A.SL_COLOR_RGBA_8U, depth_type=A.SL_COLOR_R_FLOAT, blend=A.SL_BLEND_OFF) -> Scene:
     mode = A.RENDER_MODE_TRI_WIRE if wire else A.RENDER_MODE_TRIANGLES
-    s = build_from_vertex_records(ctx, random_triangle_vertices(seed, ntri), W, H, debug, mode=mode)
+    s = build_from_vertex_records(ctx, random_triangle_vertices(seed, ntri), W, H, debug, mode=mode,
+                                  color_type=color_type, depth_type=depth_type, shader_kw=dict(blend=blend))
     s.info.update(workload="random_triangles", seed=seed, triangles=ntri, W=W, H=H, mode=mode)
     return s
 
@@ -268,7 +272,8 @@ def sphere_instances(seed: int, n: int, extent: float = 20.0, radius: float = 2.
 
 def build_sphere_scene(ctx, seed=1, n_instances=64, stacks=256, sectors=256, W=1920, H=1080,
                        tex_size=1024, radius=2.0, extent=20.0, debug=False, cull=A.SL_CULL_BACK_FACE,
-                       blend=A.SL_BLEND_OFF, alpha=1.0, wire=False, prim="triangles") -> Scene:
+                       blend=A.SL_BLEND_OFF, alpha=1.0, wire=False, prim="triangles",
+                       color_type=A.SL_COLOR_RGBA_8U, depth_type=A.SL_COLOR_R_FLOAT) -> Scene:
     verts, idx = uv_sphere(stacks, sectors)
     vbo = ctx.create_vbo(verts)
     ibo = ctx.create_ibo(idx)
@@ -295,7 +300,7 @@ def build_sphere_scene(ctx, seed=1, n_instances=64, stacks=256, sectors=256, W=1
     vs = A.sl_vertex_shader(2, cull, A.SL_PROGRAM_LAMBERT_TEXTURED)
     fs = A.sl_fragment_shader(2, 1, blend, A.SL_DEPTH_TEST_LESS_THAN, A.SL_DEPTH_MASK_ON, A.SL_PROGRAM_LAMBERT_TEXTURED)
     shader = ctx.create_shader(vs, fs, ubo)
-    color, depth, fbo, dbg = make_target(ctx, W, H, debug)
+    color, depth, fbo, dbg = make_target(ctx, W, H, debug, color_type, depth_type)
     # the same index buffer read as triangles, wireframe triangles, lines or points
     mode = {"triangles": A.RENDER_MODE_INDEXED_TRI_WIRE if wire else A.RENDER_MODE_INDEXED_TRIANGLES,
             "lines": A.RENDER_MODE_INDEXED_LINES, "points": A.RENDER_MODE_INDEXED_POINTS}[prim]
@@ -357,7 +362,8 @@ def rotation(axis, angle):
 
 
 def build_batched_scene(ctx, seed=3, n_meshes=1000, median_tris=2000, sigma=1.0, min_tris=12, max_tris=200_000,
-                        W=1920, H=1080, n_textures=16, tex_size=512, blend_fraction=0.2, debug=False, wire_every=0):
+                        W=1920, H=1080, n_textures=16, tex_size=512, blend_fraction=0.2, debug=False, wire_every=0,
+                        color_type=A.SL_COLOR_RGBA_8U, depth_type=A.SL_COLOR_R_FLOAT):
     """1000 displaced icospheres (log-normal triangle counts), random affine
     model matrices, 80% opaque / 20% alpha-blended Lambert materials over
     sixteen RGBA8 noise textures; one draw call per mesh, in index order."""
@@ -374,7 +380,7 @@ def build_batched_scene(ctx, seed=3, n_meshes=1000, median_tris=2000, sigma=1.0,
     vs_op = A.sl_vertex_shader(2, A.SL_CULL_BACK_FACE, A.SL_PROGRAM_LAMBERT_TEXTURED)
     fs_op = A.sl_fragment_shader(2, 1, A.SL_BLEND_OFF, A.SL_DEPTH_TEST_LESS_THAN, A.SL_DEPTH_MASK_ON, A.SL_PROGRAM_LAMBERT_TEXTURED)
     fs_bl = A.sl_fragment_shader(2, 1, A.SL_BLEND_ALPHA, A.SL_DEPTH_TEST_LESS_THAN, A.SL_DEPTH_MASK_OFF, A.SL_PROGRAM_LAMBERT_TEXTURED)
-    color, depth, fbo, dbg = make_target(ctx, W, H, debug)
+    color, depth, fbo, dbg = make_target(ctx, W, H, debug, color_type, depth_type)
     draws = []
     shaders = []
     keep = []

@@ -25,8 +25,9 @@ def render_both(gpu_lib, oracle_lib, build, **kw):
 
 def diff_report(g, o):
     rep = {}
-    gd = g["depth"].view(np.uint32)
-    od = o["depth"].view(np.uint32)
+    bits = np.uint16 if g["depth"].dtype.itemsize == 2 else np.uint32
+    gd = g["depth"].view(bits)
+    od = o["depth"].view(bits)
     rep["depth_mismatch"] = int((gd != od).sum())
     if "last" in g:
         rep["last_mismatch"] = int((g["last"] != o["last"]).sum())
@@ -42,7 +43,7 @@ def diff_report(g, o):
     return rep
 
 
-def assert_parity(g, o, color_tol=1):
+def assert_parity(g, o, color_tol=1, float_tol=None):
     rep = diff_report(g, o)
     assert rep["depth_mismatch"] == 0, rep
     if "last_mismatch" in rep:
@@ -50,6 +51,8 @@ def assert_parity(g, o, color_tol=1):
         assert rep["count_mismatch"] == 0, rep
     if "color_max_lsb" in rep:
         assert rep["color_max_lsb"] <= color_tol, rep
+    elif float_tol is not None:
+        assert rep["color_max_abs"] <= float_tol, rep
     for k in ("triangles_submitted", "triangles_binned", "fragments_tested", "fragments_shaded"):
         assert rep[k][0] == rep[k][1], (k, rep)
     return rep
