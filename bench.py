@@ -84,6 +84,8 @@ def run_gpu(steps, warmup, workload="spheres", raster_config=None, tune=False):
         scene = scenes.build_sphere_scene(ctx, **WORKLOAD)
     elif workload in RENDER_MODES:
         scene = scenes.build_sphere_scene(ctx, **WORKLOAD, **RENDER_MODES[workload])
+    elif workload == "spheres_rgb8_half":
+        scene = scenes.build_sphere_scene(ctx, **WORKLOAD, color_type=api.SL_COLOR_RGB_8U, depth_type=api.SL_COLOR_R_HALF)
     elif workload == "batched":
         scene = scenes.build_batched_scene(ctx, **BATCHED_WORKLOAD)
     else:
@@ -242,6 +244,17 @@ RENDER_MODES = {"spheres_wire": dict(wire=True), "spheres_lines": dict(prim="lin
                 "spheres_points": dict(prim="points")}
 
 
+def run_demo_targets(steps, warmup):
+    """The 1080p sphere frame into the reference demos' targets: RGB8 colour
+    and R16F depth (sl_large_scene_test.cpp:778-782)."""
+    scene, st, eager, gst, graph_s = run_gpu(steps, warmup, workload="spheres_rgb8_half")
+    f = max(1, st["frames"])
+    return {"targets": "RGB8 colour + R16F depth", "ms_per_frame": round(graph_s * 1e3 / steps, 4),
+            "Mtri_per_s": round(scene.n_triangles * steps / graph_s / 1e6, 1),
+            "raster_ms": round(st["ms_raster"] / f, 4),
+            "fragments_shaded_per_frame": int(st["fragments_shaded"] // f)}
+
+
 def run_render_modes(steps, warmup):
     """The 1080p sphere frame with its index buffer drawn as wireframe
     triangles, lines (index pairs) and points (every index): per-frame time
@@ -346,7 +359,8 @@ def main():
         main_views(args)
         return
     if args.workload == "modes":
-        print(json.dumps({"render_modes": run_render_modes(args.steps, args.warmup)}), flush=True)
+        print(json.dumps({"render_modes": run_render_modes(args.steps, args.warmup),
+                          "demo_targets": run_demo_targets(args.steps, args.warmup)}), flush=True)
         return
     scene, st, eager_s, gst, elapsed = run_gpu(args.steps, args.warmup, args.workload, args.raster_config, args.tune)
     tris = scene.n_triangles
@@ -433,6 +447,7 @@ def main():
         out["differentiable"] = views_summary(vr)
         out["smoothed_coverage"] = run_smooth(max(2, args.steps // 8), 1)
         out["render_modes"] = run_render_modes(max(3, args.steps // 4), 1)
+        out["demo_targets"] = run_demo_targets(max(3, args.steps // 2), 1)
         out["present_blit"] = run_blit()
     if not args.no_cpu and args.workload == "spheres":
         out["cpu_baseline"] = run_cpu_baseline()

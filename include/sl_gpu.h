@@ -121,12 +121,20 @@ enum {
  *   VS: clip = uniforms.mvp*(model*(pos,1)); varying0 = (rgb, 1); varying1 = (u, v, 0, 0).
  *   FS: output0 = varying0 * bilinear REPEAT sample of an RGBA32F texture.
  *       This is the program of the differentiable mode.
+ * SL_PROGRAM_LAMBERT_MRT
+ *   SL_PROGRAM_LAMBERT_TEXTURED with the G-buffer outputs of the reference's
+ *   multiple-render-target test (tests/sl_mrt_test.cpp:105-123):
+ *   VS adds varying2 = (world, 1).  FS: output0 = the Lambert colour,
+ *   output1 = clamp(varying2, 0, 1), output2 = clamp(varying0, 0, 1),
+ *   output3 = clamp((normalize(n), 0), 0, 1); output k goes to colour
+ *   attachment k for k < numOutputs (up to 4).  Use numVaryings = 3.
  * ------------------------------------------------------------------------- */
 enum {
     SL_PROGRAM_NONE = 0,
     SL_PROGRAM_PASSTHROUGH_GOURAUD = 1,
     SL_PROGRAM_LAMBERT_TEXTURED = 2,
-    SL_PROGRAM_COLOR_TEXTURED = 3
+    SL_PROGRAM_COLOR_TEXTURED = 3,
+    SL_PROGRAM_LAMBERT_MRT = 4
 };
 
 /* POD uniform block read by the built-in programs (lives in an SL_UniformBuffer,

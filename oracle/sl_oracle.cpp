@@ -316,6 +316,10 @@ struct Target
     uint8_t* color8;          // 8-bit attachment 0 (comps components), or null
     float* colorF;            // float attachment 0 (comps components), or null
     int comps;                // components per colour texel
+    // attachments 1..3 of a multiple-output fragment program, same layout
+    uint8_t* extra8[3];
+    float* extraF[3];
+    int extraComps[3];
     uint32_t* dbgLast;
     uint32_t* dbgCount;
     float* vis;               // visibility attachment {seq bits, b0, b1, b2}, or null
@@ -513,6 +517,7 @@ V4 run_vertex_program(const Draw& d, size_t vertId, size_t instanceId, V4* var)
             return v4(p[0], p[1], p[2], p[3]);
         }
         case SL_PROGRAM_LAMBERT_TEXTURED:
+        case SL_PROGRAM_LAMBERT_MRT:
         case SL_PROGRAM_COLOR_TEXTURED:
         {
             float p[4], a[4], uv[4], m[16];
@@ -521,11 +526,12 @@ V4 run_vertex_program(const Draw& d, size_t vertId, size_t instanceId, V4* var)
             read_attr(d, 2, vertId, uv, 0.f);
             model_matrix(d, instanceId, m);
             const V4 world = mat_mul(m, v4(p[0], p[1], p[2], 1.f));
-            if (d.sh->vsProg == SL_PROGRAM_LAMBERT_TEXTURED)
+            if (d.sh->vsProg != SL_PROGRAM_COLOR_TEXTURED)
             {
                 const V4 n = mat_mul(m, v4(a[0], a[1], a[2], 0.f));
                 var[0] = v4(uv[0], uv[1], 0.f, 0.f);
                 var[1] = v4(n.v[0], n.v[1], n.v[2], 0.f);
+                if (d.sh->vsProg == SL_PROGRAM_LAMBERT_MRT) var[2] = v4(world.v[0], world.v[1], world.v[2], 1.f);
             }
             else
             {
@@ -539,8 +545,22 @@ V4 run_vertex_program(const Draw& d, size_t vertId, size_t instanceId, V4* var)
     }
 }
 
-bool run_fragment_program(const Draw& d, const V4* var, V4& out)
+// math::clamp(x, lo, hi) as min(max(x, lo), hi) with max(a,b) = (a < b) ? b : a
+// and min(a,b) = (b < a) ? b : a (NaN stays NaN).
+inline float clampf_ref(float x, float lo, float hi)
 {
+    const float m = (x < lo) ? lo : x;
+    return (hi < m) ? hi : m;
+}
+
+inline V4 clamp01(const V4& v)
+{
+    return v4(clampf_ref(v.v[0], 0.f, 1.f), clampf_ref(v.v[1], 0.f, 1.f), clampf_ref(v.v[2], 0.f, 1.f), clampf_ref(v.v[3], 0.f, 1.f));
+}
+
+bool run_fragment_program(const Draw& d, const V4* var, V4* outs)
+{
+    V4& out = outs[0];
     switch (d.sh->fsProg)
     {
         case SL_PROGRAM_PASSTHROUGH_GOURAUD:
@@ -548,6 +568,7 @@ bool run_fragment_program(const Draw& d, const V4* var, V4& out)
             return true;
 
         case SL_PROGRAM_LAMBERT_TEXTURED:
+        case SL_PROGRAM_LAMBERT_MRT:
         {
             const sl_std_uniforms* u = uniforms(d);
             uint8_t texel[4] = {0, 0, 0, 0};
@@ -568,6 +589,13 @@ bool run_fragment_program(const Draw& d, const V4* var, V4& out)
                 out.v[c] = (v < 1.f) ? v : 1.f;
             }
             out.v[3] = albedo[3] * u->tint[3];
+            if (d.sh->fsProg == SL_PROGRAM_LAMBERT_MRT)
+            {
+                // the G-buffer outputs of tests/sl_mrt_test.cpp:105-123: position, uv, normal
+                outs[1] = clamp01(var[2]);
+                outs[2] = clamp01(var[0]);
+                outs[3] = clamp01(v4(mx, my, mz, 0.f));
+            }
             return true;
         }
 
@@ -640,21 +668,27 @@ void depth_write(Target& t, size_t idx, float z)
 // destination as (r, 0, 0, 1), (r, g, 0, 1), (r, r, r, 1) for one, two and
 // three components (the three-component read repeats the first component,
 // :306-307), blends all four, clamps and stores the texel's components.
+void place_color_in(const Draw& d, uint8_t* color8, float* colorF, int n, int x, int y, const V4& c);
+
 void place_color(const Draw& d, int x, int y, const V4& c)
+{
+    place_color_in(d, d.tgt.color8, d.tgt.colorF, d.tgt.comps, x, y, c);
+}
+
+void place_color_in(const Draw& d, uint8_t* color8, float* colorF, int n, int x, int y, const V4& c)
 {
     const Target& t = d.tgt;
     const size_t idx = (size_t)x + (size_t)t.W * (size_t)y;
     const int blend = d.sh->blend;
-    const int n = t.comps;
     auto read_dst = [&](const float v[4], float dst[4]) {
         if (n == 1) { dst[0] = v[0]; dst[1] = 0.f; dst[2] = 0.f; dst[3] = 1.f; }
         else if (n == 2) { dst[0] = v[0]; dst[1] = v[1]; dst[2] = 0.f; dst[3] = 1.f; }
         else if (n == 3) { dst[0] = v[0]; dst[1] = v[0]; dst[2] = v[0]; dst[3] = 1.f; }
         else for (int k = 0; k < 4; ++k) dst[k] = v[k];
     };
-    if (t.color8)
+    if (color8)
     {
-        uint8_t* px = t.color8 + idx * (size_t)n;
+        uint8_t* px = color8 + idx * (size_t)n;
         if (blend == SL_BLEND_OFF)
         {
             for (int k = 0; k < n; ++k) px[k] = (uint8_t)(uint32_t)f2i(c.v[k] * 255.f);
@@ -671,9 +705,9 @@ void place_color(const Draw& d, int x, int y, const V4& c)
             for (int k = 0; k < n; ++k) px[k] = (uint8_t)(uint32_t)f2i(dst[k] * 255.f);
         }
     }
-    else if (t.colorF)
+    else if (colorF)
     {
-        float* px = t.colorF + idx * (size_t)n;
+        float* px = colorF + idx * (size_t)n;
         if (blend == SL_BLEND_OFF)
         {
             for (int k = 0; k < n; ++k) px[k] = c.v[k];
@@ -746,9 +780,15 @@ inline void scanline_step(const Scanline& s, float yf, int32_t& xMin, int32_t& x
 // 365-402, 496-530; SL_PointRasterizer.cpp:64-96).
 void finish_fragment(Draw& d, int x, int y, float z, const V4* var, uint32_t seq)
 {
-    V4 out;
-    const bool haveOutputs = run_fragment_program(d, var, out);
-    if (haveOutputs && d.sh->nout > 0) place_color(d, x, y, out);
+    V4 outs[SL_SHADER_MAX_FRAG_OUTPUTS];
+    const bool haveOutputs = run_fragment_program(d, var, outs);
+    if (haveOutputs && d.sh->nout > 0)
+    {
+        // one output per attachment (SL_FragmentProcessor.cpp:379-395)
+        for (unsigned k = d.sh->nout; k-- > 1;)   // attachments 3, 2, 1, then 0 (the switch falls through)
+            place_color_in(d, d.tgt.extra8[k - 1], d.tgt.extraF[k - 1], d.tgt.extraComps[k - 1], x, y, outs[k]);
+        place_color(d, x, y, outs[0]);
+    }
 
     const size_t idx = (size_t)x + (size_t)d.tgt.W * (size_t)y;
     if (d.sh->dmask == SL_DEPTH_MASK_ON) depth_write(d.tgt, idx, z);   // regardless of the shader's return
@@ -1407,15 +1447,22 @@ int setup_draw(sl_context* c, size_t shaderId, size_t fboId, Draw& d)
     d.tgt.color8 = nullptr;
     d.tgt.colorF = nullptr;
     d.tgt.comps = 4;
-    if (sh.nout > 1) return SL_ERR_UNSUPPORTED;
-    if (sh.nout == 1)
+    for (int k = 0; k < 3; ++k) { d.tgt.extra8[k] = nullptr; d.tgt.extraF[k] = nullptr; d.tgt.extraComps[k] = 4; }
+    // several outputs only from the program that writes them
+    if (sh.nout > 1 && sh.fsProg != SL_PROGRAM_LAMBERT_MRT) return SL_ERR_UNSUPPORTED;
+    for (unsigned k = 0; k < sh.nout; ++k)
     {
-        if (f.colors.empty() || f.colors[0] >= c->texs.size() || !c->texs[f.colors[0]].valid) return SL_ERR_STATE;
-        Tex& ct = c->texs[f.colors[0]];
+        if (f.colors.size() <= k || f.colors[k] >= c->texs.size() || !c->texs[f.colors[k]].valid) return SL_ERR_STATE;
+        Tex& ct = c->texs[f.colors[k]];
         if (ct.w != dt.w || ct.h != dt.h) return SL_ERR_STATE;
-        if (u8_components(ct.type)) { d.tgt.color8 = ct.data.data(); d.tgt.comps = u8_components(ct.type); }
-        else if (float_components(ct.type)) { d.tgt.colorF = reinterpret_cast<float*>(ct.data.data()); d.tgt.comps = float_components(ct.type); }
+        uint8_t* p8 = nullptr;
+        float* pF = nullptr;
+        int n = 0;
+        if (u8_components(ct.type)) { p8 = ct.data.data(); n = u8_components(ct.type); }
+        else if (float_components(ct.type)) { pF = reinterpret_cast<float*>(ct.data.data()); n = float_components(ct.type); }
         else return SL_ERR_UNSUPPORTED;
+        if (k == 0) { d.tgt.color8 = p8; d.tgt.colorF = pF; d.tgt.comps = n; }
+        else { d.tgt.extra8[k - 1] = p8; d.tgt.extraF[k - 1] = pF; d.tgt.extraComps[k - 1] = n; }
     }
     d.tgt.dbgLast = (f.dbgLast < c->texs.size()) ? reinterpret_cast<uint32_t*>(c->texs[f.dbgLast].data.data()) : nullptr;
     d.tgt.dbgCount = (f.dbgCount < c->texs.size()) ? reinterpret_cast<uint32_t*>(c->texs[f.dbgCount].data.data()) : nullptr;
@@ -1427,6 +1474,7 @@ int setup_draw(sl_context* c, size_t shaderId, size_t fboId, Draw& d)
         Tex& vt = c->texs[f.vis];
         if (vt.type != SL_COLOR_RGBA_FLOAT || vt.w != dt.w || vt.h != dt.h) return SL_ERR_STATE;
         if ((d.tgt.colorF == nullptr || d.tgt.comps != 4) && sh.nout > 0) return SL_ERR_UNSUPPORTED;
+        if (sh.nout > 1) return SL_ERR_UNSUPPORTED;
         if (d.tgt.depthH) return SL_ERR_UNSUPPORTED;
         if (sh.nvary >= SL_SHADER_MAX_VARYING_VECTORS) return SL_ERR_UNSUPPORTED;
         d.tgt.vis = reinterpret_cast<float*>(vt.data.data());

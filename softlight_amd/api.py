@@ -48,6 +48,7 @@ SL_COLOR_RGBA_8U, SL_COLOR_RGBA_FLOAT = 21, 26
 SL_PROGRAM_PASSTHROUGH_GOURAUD = 1
 SL_PROGRAM_LAMBERT_TEXTURED = 2
 SL_PROGRAM_COLOR_TEXTURED = 3
+SL_PROGRAM_LAMBERT_MRT = 4
 
 _TEXEL_DTYPE = {SL_COLOR_R_8U: (np.uint8, 1), SL_COLOR_RG_8U: (np.uint8, 2), SL_COLOR_RGB_8U: (np.uint8, 3),
                 SL_COLOR_RGBA_8U: (np.uint8, 4), SL_COLOR_R_32U: (np.uint32, 1), SL_COLOR_R_HALF: (np.float16, 1),

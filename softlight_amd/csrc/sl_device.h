@@ -306,6 +306,9 @@ struct RasterParams
     uint8_t* color;               // attachment 0 texels (8-bit or float, colorComps components) or null
     int32_t colorType;
     int32_t colorComps;           // components per colour texel (1..4)
+    uint8_t* colorX[3];           // colour attachments 1..3 (outputs 1..3 of a multiple-output program), or null
+    int32_t colorXFloat[3];       // their texels are float (else 8-bit)
+    int32_t colorXComps[3];
     int32_t depthHalf;            // the depth attachment is R16F (else R32F)
     float* depth;
     uint32_t* dbgLast;
@@ -316,6 +319,7 @@ struct RasterParams
     float clearRGBAF[4];
     float clearDepthValue;
     int32_t extra;                // some draw of the batch draws wireframe triangles, lines or points
+    int32_t mrt;                  // some draw of the batch has several fragment outputs
     FrameCtrl* ctrl;
 };
 

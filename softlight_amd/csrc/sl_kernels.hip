@@ -1457,11 +1457,71 @@ __device__ __forceinline__ void record_barrier()
 // Shading of one fragment from its record and its barycentric values s0..s2
 // (perspective correction SL_FragmentProcessor.cpp:486-494, varyings :315-330,
 // then the fragment program).  `wv` receives the visibility slot's varying.
+// The destination a blend reads from a texel of n < 4 components
+// (assign_alpha_pixel, SL_Framebuffer.cpp:296-311): (r, min, min, max),
+// (r, g, min, max), and for three components (r, r, r, max).
+__device__ __forceinline__ void blend_dst_read(int n, float one, float d[4])
+{
+    // selects, not conditional stores, so d stays in registers
+    const float r = d[0];
+    const float g = (n == 3) ? r : ((n == 1) ? 0.f : d[1]);
+    const float b = (n == 4) ? d[2] : ((n == 3) ? r : 0.f);
+    const float a = (n == 4) ? d[3] : one;
+    d[1] = g;
+    d[2] = b;
+    d[3] = a;
+}
+
+// Output k (1..3) of a multiple-output program into colour attachment k at
+// texel gi, written or blended as assign_pixel / assign_alpha_pixel do.
+// These attachments are read and written in place: a texel is only ever
+// touched by the lane that owns it.
+__device__ __forceinline__ void write_extra_output(const RasterParams& P, int k, uint64_t gi, float4 o, int blendMode)
+{
+    const int n = P.colorXComps[k];
+    if (P.colorXFloat[k])
+    {
+        float* px = reinterpret_cast<float*>(P.colorX[k]) + gi * (uint64_t)n;
+        float d[4] = {o.x, o.y, o.z, o.w};
+        if (blendMode != SL_BLEND_OFF)
+        {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) d[c] = (c < n) ? px[c] : 0.f;
+            blend_dst_read(n, 1.f, d);
+            blend(blendMode, o, d);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (c < n) px[c] = d[c];
+    }
+    else
+    {
+        uint8_t* px = P.colorX[k] + gi * (uint64_t)n;
+        float d[4] = {o.x, o.y, o.z, o.w};
+        if (blendMode != SL_BLEND_OFF)
+        {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) d[c] = (c < n) ? (float)px[c] * 0.00392156862745f : 0.f;
+            blend_dst_read(n, 255.f * 0.00392156862745f, d);
+            blend(blendMode, o, d);
+        }
+        const uint32_t v = pack_rgba8(d);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (c < n) px[c] = (uint8_t)(v >> (8 * c));
+    }
+}
+
 // A point's varyings pass unchanged (SL_PointRasterizer.cpp:66-69); a line's
 // are mixed by its position s0 (interpolate_line_varyings,
-// SL_FragmentProcessor.cpp:119-145).
+// SL_FragmentProcessor.cpp:119-145).  Outputs 1..3 of a multiple-output
+// program go straight to their attachments at texel gi (attachments 3, 2, 1,
+// as the reference's output switch falls through); output 0 is returned.
+// Only the MRT raster variants carry that code.
+template <bool MRT>
 __device__ __forceinline__ bool shade(const RasterParams& P, const DrawDesc& D, const float hw[3], float bc3, uint32_t state,
-                                      uint32_t rid, float s0, float s1, float s2, float4& out, float4& wv)
+                                      uint32_t rid, float s0, float s1, float s2, float4& out, float4& wv,
+                                      uint64_t gi, int blendMode)
 {
     const float t0 = s0 * hw[0], t1 = s1 * hw[1], t2 = s2 * hw[2];
     const float sinv = rcpf(((t0 + t1) + t2) + bc3);
@@ -1500,7 +1560,12 @@ __device__ __forceinline__ bool shade(const RasterParams& P, const DrawDesc& D, 
 #pragma unroll
     for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
         if ((uint32_t)i == D.visSlot) wv = var.v[i];
-    return (D.nout > 0) && fragment_program(D, P.uniforms, var, out);
+    if (!MRT || D.nout <= 1) return (D.nout > 0) && fragment_program(D, P.uniforms, var, out);
+    float4 extra[3];
+    const bool have = fragment_program(D, P.uniforms, var, out, extra);
+    if (have)
+        for (int k = (int)D.nout - 1; k >= 1; --k) write_extra_output(P, k - 1, gi, extra[k - 1], blendMode);
+    return have;
 }
 
 // Attachment texels of P.colorComps components (8-bit: packed into the low
@@ -1551,18 +1616,6 @@ __device__ __forceinline__ void store_depth(const RasterParams& P, uint64_t gi, 
 {
     if (P.depthHalf) reinterpret_cast<__half*>(P.depth)[gi] = __float2half_rn(z);
     else P.depth[gi] = z;
-}
-
-// The destination a blend reads from a texel of n < 4 components
-// (assign_alpha_pixel, SL_Framebuffer.cpp:296-311): (r, min, min, max),
-// (r, g, min, max), and for three components (r, r, r, max).
-__device__ __forceinline__ void blend_dst_read(int n, float one, float d[4])
-{
-    if (n == 4) return;
-    if (n == 3) { d[1] = d[0]; d[2] = d[0]; }
-    if (n == 1) d[1] = 0.f;
-    if (n < 3) d[2] = 0.f;
-    d[3] = one;
 }
 
 template <int CF>
@@ -1621,13 +1674,16 @@ constexpr uint32_t kNoPending = 0xFFFFFFFFu;
 //
 // AUX bit 0: debug buffers (last primitive, depth-pass count); bit 1: the
 // visibility attachment; bit 2: the batch holds wireframe, line or point
-// records (the variants without it carry only the filled-triangle code).
+// records (the variants without it carry only the filled-triangle code);
+// bit 3: the batch has draws with several fragment outputs (launched with
+// bit 2 set).
 template <int CF, int AUX, int TW, int TH, int SEGS>
 __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
 {
     constexpr bool DBG = (AUX & 1) != 0;
     constexpr bool VIS = (AUX & 2) != 0;
     constexpr bool EXTRA = (AUX & 4) != 0;
+    constexpr bool MRT = (AUX & 8) != 0;
     constexpr int NT = TH * 4 * SEGS;       // lanes: rows x segments x 4 phases
     constexpr int NW = NT / 64;             // waves, each owning TH / NW rows
     constexpr int SW = TW / SEGS;           // segment width (columns)
@@ -1680,7 +1736,8 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
         const DrawDesc& QD = P.draws[Q.draw];
         const float qhw[3] = {Q.hw[0], Q.hw[1], Q.hw[2]};
         float4 out, wv;
-        const bool have = shade(P, QD, qhw, Q.bc3, Q.state, prid, sPendS[0][r][c], sPendS[1][r][c], sPendS[2][r][c], out, wv);
+        const bool have = shade<MRT>(P, QD, qhw, Q.bc3, Q.state, prid, sPendS[0][r][c], sPendS[1][r][c], sPendS[2][r][c], out, wv,
+                                (uint64_t)(y0 + r) * P.W + (x0 + c), SL_BLEND_OFF);
         if (have) blend_into<CF>(&sCol8[CF == 1 ? r : 0][CF == 1 ? c : 0], &sColF[CF == 2 ? r : 0][CF == 2 ? c : 0], SL_BLEND_OFF, out, P.colorComps);
         if (VIS) sVis[r][c] = make_float4(__uint_as_float(Q.seq), wv.x, wv.y, wv.z);
     };
@@ -1800,7 +1857,8 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                         const DrawDesc& D = P.draws[R.draw];
                         const float hw[3] = {R.hw[0], R.hw[1], R.hw[2]};
                         float4 out, wv;
-                        const bool have = shade(P, D, hw, R.bc3, st, rid, s0, s1, s2, out, wv);
+                        const bool have = shade<MRT>(P, D, hw, R.bc3, st, rid, s0, s1, s2, out, wv,
+                                                (uint64_t)y * P.W + x, blendMode);
                         if (have) blend_into<CF>(&sCol8[CF == 1 ? row : 0][CF == 1 ? c : 0],
                                                  &sColF[CF == 2 ? row : 0][CF == 2 ? c : 0], blendMode, out, P.colorComps);
                         if (VIS) sVis[row][c] = make_float4(__uint_as_float(R.seq), wv.x, wv.y, wv.z);
@@ -2244,9 +2302,23 @@ hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debu
     {
         // the visibility attachment is only built for RGBA32F colour targets
         // and filled triangles
-        if (cf != 2 || p.extra) return hipErrorInvalidValue;
+        if (cf != 2 || p.extra || p.mrt) return hipErrorInvalidValue;
         if (debug) launch_raster_cfg<2, 3>(p, numTiles, cfg, s);
         else launch_raster_cfg<2, 2>(p, numTiles, cfg, s);
+    }
+    else if (p.mrt)
+    {
+        if (cf == 0) return hipErrorInvalidValue;   // outputs need colour attachment 0
+        if (debug)
+        {
+            if (cf == 1) launch_raster_cfg<1, 13>(p, numTiles, cfg, s);
+            else launch_raster_cfg<2, 13>(p, numTiles, cfg, s);
+        }
+        else
+        {
+            if (cf == 1) launch_raster_cfg<1, 12>(p, numTiles, cfg, s);
+            else launch_raster_cfg<2, 12>(p, numTiles, cfg, s);
+        }
     }
     else if (p.extra)
     {

@@ -321,6 +321,7 @@ struct Plan
     int W, H, cfg, tilesX, tilesY, numTiles;
     TileConfig tc;
     Tex* dt;
+    Tex* cx[3] = {nullptr, nullptr, nullptr};   // colour attachments 1..3
     Tex* ct;
     Tex* dbgL;
     Tex* dbgC;
@@ -338,6 +339,12 @@ int make_plan(sl_context* c, const Batch& B, Plan& p)
     if (!p.dt || !depth_format(p.dt->type)) return fail(c, SL_ERR_STATE, "framebuffer needs an R_FLOAT or R_HALF depth attachment");
     p.ct = F.colors.empty() ? nullptr : tex_at(c, F.colors[0]);
     if (p.ct && !u8_components(p.ct->type) && !float_components(p.ct->type)) return fail(c, SL_ERR_UNSUPPORTED, "colour attachment format");
+    for (int k = 0; k < 3; ++k)
+    {
+        // attachments 1..3, written by multiple-output programs (validated per draw)
+        Tex* xt = (F.colors.size() > (size_t)k + 1) ? tex_at(c, F.colors[k + 1]) : nullptr;
+        p.cx[k] = (xt && (u8_components(xt->type) || float_components(xt->type)) && xt->w == p.dt->w && xt->h == p.dt->h) ? xt : nullptr;
+    }
     p.dbgL = tex_at(c, F.dbgLast);
     p.dbgC = tex_at(c, F.dbgCount);
     p.debug = p.dbgL && p.dbgC;
@@ -441,6 +448,13 @@ RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool have
     rp.colorType = p.ct ? p.ct->type : SL_COLOR_RGBA_8U;
     rp.colorComps = p.ct ? (u8_components(p.ct->type) ? u8_components(p.ct->type) : float_components(p.ct->type)) : 4;
     rp.depthHalf = (p.dt->type == SL_COLOR_R_HALF) ? 1 : 0;
+    for (int k = 0; k < 3; ++k)
+    {
+        const Tex* xt = p.cx[k];
+        rp.colorX[k] = xt ? reinterpret_cast<uint8_t*>(xt->buf.p) : nullptr;
+        rp.colorXFloat[k] = (xt && float_components(xt->type)) ? 1 : 0;
+        rp.colorXComps[k] = xt ? (float_components(xt->type) ? float_components(xt->type) : u8_components(xt->type)) : 4;
+    }
     rp.depth = reinterpret_cast<float*>(p.dt->buf.p);
     rp.dbgLast = p.debug ? reinterpret_cast<uint32_t*>(p.dbgL->buf.p) : nullptr;
     rp.dbgCount = p.debug ? reinterpret_cast<uint32_t*>(p.dbgC->buf.p) : nullptr;
@@ -456,7 +470,12 @@ RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool have
     rp.clearRGBA8 = packed;
     rp.clearDepthValue = rp.depthHalf ? host_half_to_float(host_float_to_half((float)B.depth)) : (float)B.depth;
     rp.extra = 0;
-    for (const DrawDesc& d : B.draws) rp.extra |= (d.prim != kPrimFill) ? 1 : 0;
+    rp.mrt = 0;
+    for (const DrawDesc& d : B.draws)
+    {
+        rp.extra |= (d.prim != kPrimFill) ? 1 : 0;
+        rp.mrt |= (d.nout > 1) ? 1 : 0;
+    }
     rp.ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
     return rp;
 }
@@ -801,19 +820,22 @@ int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t s
     const uint32_t prim = prim_of_mode(procMode);
     if (prim == kPrimBad || prim_of_mode(m.mode) == kPrimBad) return fail(c, SL_ERR_UNSUPPORTED, "render mode");
     const Shader& sh = c->shaders[shaderId];
-    if (sh.nout > 1) return fail(c, SL_ERR_UNSUPPORTED, "more than one fragment output");
+    if (sh.nout > 1 && sh.fsProg != SL_PROGRAM_LAMBERT_MRT) return fail(c, SL_ERR_UNSUPPORTED, "several fragment outputs need SL_PROGRAM_LAMBERT_MRT");
     Fbo& F = c->fbos[fboId];
     const bool vis = tex_at(c, F.vis) != nullptr;
     if (vis && sh.nvary >= SL_SHADER_MAX_VARYING_VECTORS) return fail(c, SL_ERR_UNSUPPORTED, "visibility needs a free varying slot (numVaryings <= 3)");
     if (vis && prim != kPrimFill) return fail(c, SL_ERR_UNSUPPORTED, "visibility attachments cover filled triangles only");
+    if (vis && sh.nout > 1) return fail(c, SL_ERR_UNSUPPORTED, "visibility attachments need a single fragment output");
     Tex* dt = tex_at(c, F.depth);
     if (!dt || !depth_format(dt->type)) return fail(c, SL_ERR_STATE, "framebuffer needs an R_FLOAT or R_HALF depth attachment");
     if (prim == kPrimLine && (dt->w > kMaxLineTarget || dt->h > kMaxLineTarget))
         return fail(c, SL_ERR_UNSUPPORTED, "line draws need a target of at most 8192 x 8192");
-    if (sh.nout == 1)
+    for (uint32_t k = 0; k < sh.nout; ++k)
     {
-        Tex* ct = F.colors.empty() ? nullptr : tex_at(c, F.colors[0]);
-        if (!ct) return fail(c, SL_ERR_STATE, "framebuffer has no colour attachment 0");
+        Tex* ct = (F.colors.size() > k) ? tex_at(c, F.colors[k]) : nullptr;
+        if (!ct) return fail(c, SL_ERR_STATE, "framebuffer lacks a colour attachment for a fragment output");
+        if (!u8_components(ct->type) && !float_components(ct->type)) return fail(c, SL_ERR_UNSUPPORTED, "colour attachment format");
+        if (ct->w != dt->w || ct->h != dt->h) return fail(c, SL_ERR_STATE, "colour and depth attachment sizes differ");
     }
     const Vao& vao = c->vaos[m.vaoId];
     const Vbo* vbo = (vao.vbo < c->vbos.size() && c->vbos[vao.vbo].valid) ? &c->vbos[vao.vbo] : nullptr;

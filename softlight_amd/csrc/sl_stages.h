@@ -159,7 +159,7 @@ __device__ __forceinline__ void vertex_positions3(const DrawDesc& d, const uint8
             if (need[k]) out[k] = read_attr(d, 0, vid[k], 1.f);
         return;
     }
-    if (d.vsProg == SL_PROGRAM_LAMBERT_TEXTURED || d.vsProg == SL_PROGRAM_COLOR_TEXTURED)
+    if (d.vsProg == SL_PROGRAM_LAMBERT_TEXTURED || d.vsProg == SL_PROGRAM_LAMBERT_MRT || d.vsProg == SL_PROGRAM_COLOR_TEXTURED)
     {
         const sl_std_uniforms* u = reinterpret_cast<const sl_std_uniforms*>(uniArena + d.uniOffset);
         float4 p[3];
@@ -194,7 +194,7 @@ __device__ __forceinline__ void vertex_varyings3(const DrawDesc& d, const uint8_
         for (int k = 0; k < 3; ++k) out[k].v[0] = read_attr(d, 1, vid[k], 1.f);
         return;
     }
-    if (d.vsProg == SL_PROGRAM_LAMBERT_TEXTURED || d.vsProg == SL_PROGRAM_COLOR_TEXTURED)
+    if (d.vsProg == SL_PROGRAM_LAMBERT_TEXTURED || d.vsProg == SL_PROGRAM_LAMBERT_MRT || d.vsProg == SL_PROGRAM_COLOR_TEXTURED)
     {
         float4 a[3], uv[3];
 #pragma unroll
@@ -203,7 +203,7 @@ __device__ __forceinline__ void vertex_varyings3(const DrawDesc& d, const uint8_
             a[k] = read_attr(d, 1, vid[k], 0.f);
             uv[k] = read_attr(d, 2, vid[k], 0.f);
         }
-        if (d.vsProg == SL_PROGRAM_LAMBERT_TEXTURED)
+        if (d.vsProg != SL_PROGRAM_COLOR_TEXTURED)
         {
             const sl_std_uniforms* u = reinterpret_cast<const sl_std_uniforms*>(uniArena + d.uniOffset);
             float m[16];
@@ -214,6 +214,12 @@ __device__ __forceinline__ void vertex_varyings3(const DrawDesc& d, const uint8_
                 const float4 n = mat_mul(m, f4(a[k].x, a[k].y, a[k].z, 0.f));
                 out[k].v[0] = f4(uv[k].x, uv[k].y, 0.f, 0.f);
                 out[k].v[1] = f4(n.x, n.y, n.z, 0.f);
+                if (d.vsProg == SL_PROGRAM_LAMBERT_MRT)
+                {
+                    const float4 p = read_attr(d, 0, vid[k], 1.f);
+                    const float4 world = mat_mul(m, f4(p.x, p.y, p.z, 1.f));
+                    out[k].v[2] = f4(world.x, world.y, world.z, 1.f);
+                }
             }
         }
         else
@@ -476,14 +482,29 @@ __device__ __forceinline__ float4 sample_bilinear_rgbaf(const uint8_t* tex, uint
 }
 
 // Built-in fragment programs (replace SL_FragmentShader::shader, SL_Shader.hpp:89-98)
-__device__ __forceinline__ bool fragment_program(const DrawDesc& d, const uint8_t* uniArena, const Vary4& var, float4& out)
+// math::clamp(x, 0, 1) as min(max(x, 0), 1) with the reference's max / min
+// operand order (NaN stays NaN).
+__device__ __forceinline__ float clamp01_ref(float x)
+{
+    const float m = (x < 0.f) ? 0.f : x;
+    return (1.f < m) ? 1.f : m;
+}
+
+__device__ __forceinline__ float4 clamp01_ref4(float4 v)
+{
+    return f4(clamp01_ref(v.x), clamp01_ref(v.y), clamp01_ref(v.z), clamp01_ref(v.w));
+}
+
+// extra[0..2]: outputs 1..3 (SL_PROGRAM_LAMBERT_MRT only).
+__device__ __forceinline__ bool fragment_program(const DrawDesc& d, const uint8_t* uniArena, const Vary4& var, float4& out,
+                                                 float4* extra = nullptr)
 {
     if (d.fsProg == SL_PROGRAM_PASSTHROUGH_GOURAUD)
     {
         out = var.v[0];
         return true;
     }
-    if (d.fsProg == SL_PROGRAM_LAMBERT_TEXTURED)
+    if (d.fsProg == SL_PROGRAM_LAMBERT_TEXTURED || d.fsProg == SL_PROGRAM_LAMBERT_MRT)
     {
         const sl_std_uniforms* u = reinterpret_cast<const sl_std_uniforms*>(uniArena + d.uniOffset);
         uint32_t texel = 0;
@@ -509,6 +530,13 @@ __device__ __forceinline__ bool fragment_program(const DrawDesc& d, const uint8_
             o[c] = (v < 1.f) ? v : 1.f;
         }
         out = f4(o[0], o[1], o[2], a3 * u->tint[3]);
+        if (extra != nullptr && d.fsProg == SL_PROGRAM_LAMBERT_MRT)
+        {
+            // the G-buffer outputs of tests/sl_mrt_test.cpp:105-123: position, uv, normal
+            extra[0] = clamp01_ref4(var.v[2]);
+            extra[1] = clamp01_ref4(var.v[0]);
+            extra[2] = clamp01_ref4(f4(mx, my, mz, 0.f));
+        }
         return true;
     }
     if (d.fsProg == SL_PROGRAM_COLOR_TEXTURED)
