@@ -339,6 +339,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="spheres: only the headline frame (no differentiable, smoothing, render-mode, target or blit legs)")
     ap.add_argument("--workload", choices=["spheres", "random", "batched", "views", "modes"], default="spheres",
                     help="spheres: the 1080p textured-sphere frame (default); random: 4096 random triangles at "
                          "640x480; batched: 1000 meshes with mixed opaque/blended materials at 1080p; views: "
@@ -442,7 +444,7 @@ def main():
             "bytes_per_triangle": round(bpt, 3),
         },
     }
-    if args.workload == "spheres":
+    if args.workload == "spheres" and not args.headline_only:
         _, vr = run_views(max(3, args.steps // 4), 1)
         out["differentiable"] = views_summary(vr)
         out["smoothed_coverage"] = run_smooth(max(2, args.steps // 8), 1)
