@@ -85,8 +85,11 @@ enum {
     VERTEX_DATA_BYTE = 0, VERTEX_DATA_SHORT, VERTEX_DATA_INT, VERTEX_DATA_LONG,
     VERTEX_DATA_FLOAT, VERTEX_DATA_DOUBLE, VERTEX_DATA_INVALID
 };
-/* SL_Color.hpp:31-73: SL_ColorDataType (the formats this build supports are
- * R_8U, RGBA_8U, R_32U, R_FLOAT, RGBA_FLOAT; the others are rejected). */
+/* SL_Color.hpp:31-73: SL_ColorDataType.  Textures of every format can be
+ * created; draws write 8-bit and float R/RG/RGB/RGBA colour, the packed
+ * RGB_332 / RGB_565 / RGBA_5551 / RGBA_4444 / RGBA_1010102 formats, and
+ * R_FLOAT or R_HALF depth; R_32U holds the debug buffers; samplers read
+ * RGBA_8U and RGBA_FLOAT; the blit reads RGBA_8U, RGBA_FLOAT and R_FLOAT. */
 enum {
     SL_COLOR_R_8U = 0, SL_COLOR_R_16U, SL_COLOR_R_32U, SL_COLOR_R_64U,
     SL_COLOR_R_HALF, SL_COLOR_R_FLOAT, SL_COLOR_R_DOUBLE,
@@ -95,7 +98,8 @@ enum {
     SL_COLOR_RGB_8U, SL_COLOR_RGB_16U, SL_COLOR_RGB_32U, SL_COLOR_RGB_64U,
     SL_COLOR_RGB_HALF, SL_COLOR_RGB_FLOAT, SL_COLOR_RGB_DOUBLE,
     SL_COLOR_RGBA_8U, SL_COLOR_RGBA_16U, SL_COLOR_RGBA_32U, SL_COLOR_RGBA_64U,
-    SL_COLOR_RGBA_HALF, SL_COLOR_RGBA_FLOAT, SL_COLOR_RGBA_DOUBLE
+    SL_COLOR_RGBA_HALF, SL_COLOR_RGBA_FLOAT, SL_COLOR_RGBA_DOUBLE,
+    SL_COLOR_RGB_332, SL_COLOR_RGB_565, SL_COLOR_RGBA_5551, SL_COLOR_RGBA_4444, SL_COLOR_RGBA_1010102
 };
 
 /* ---------------------------------------------------------------------------

@@ -216,6 +216,9 @@ uint32_t bytes_per_color(int type)
         case SL_COLOR_RG_FLOAT: return 8;
         case SL_COLOR_RGB_FLOAT: return 12;
         case SL_COLOR_RGBA_FLOAT: return 16;
+        case SL_COLOR_RGB_332: return 1;
+        case SL_COLOR_RGB_565: case SL_COLOR_RGBA_5551: case SL_COLOR_RGBA_4444: return 2;
+        case SL_COLOR_RGBA_1010102: return 4;
         default: return 0;
     }
 }
@@ -230,6 +233,56 @@ int u8_components(int type)
 int float_components(int type)
 {
     return type == SL_COLOR_R_FLOAT ? 1 : type == SL_COLOR_RG_FLOAT ? 2 : type == SL_COLOR_RGB_FLOAT ? 3 : type == SL_COLOR_RGBA_FLOAT ? 4 : 0;
+}
+
+// Packed formats (SL_ColorCompressed.hpp): bit-field widths of r, g, b, a
+// (r in the lowest bits, as the structs' bit-fields are laid out) and texel
+// bytes.  Zero widths for a format that is not packed.
+struct PackedFmt { int bits[4]; int bytes; };
+
+PackedFmt packed_format(int type)
+{
+    switch (type)
+    {
+        case SL_COLOR_RGB_332: return PackedFmt{{3, 3, 2, 0}, 1};            // :20-28
+        case SL_COLOR_RGB_565: return PackedFmt{{5, 6, 5, 0}, 2};            // :198-206
+        case SL_COLOR_RGBA_5551: return PackedFmt{{5, 5, 5, 1}, 2};          // :376-385
+        case SL_COLOR_RGBA_4444: return PackedFmt{{4, 4, 4, 4}, 2};          // :558-567
+        case SL_COLOR_RGBA_1010102: return PackedFmt{{10, 10, 10, 2}, 4};    // :735-744
+        default: return PackedFmt{{0, 0, 0, 0}, 0};
+    }
+}
+
+// rgba_cast<packed, float> (e.g. SL_ColorCompressed.hpp:300-310): each
+// component times its maximum, converted to the field's integer type (the
+// low bits of the truncated int, as for 8-bit colour) and stored in the
+// bit-field (which keeps its low bits).
+uint32_t packed_encode(const PackedFmt& f, const float c[4])
+{
+    uint32_t w = 0;
+    int shift = 0;
+    for (int k = 0; k < 4; ++k)
+    {
+        if (f.bits[k] == 0) continue;
+        const uint32_t mx = (1u << f.bits[k]) - 1u;
+        w |= ((uint32_t)f2i(c[k] * (float)mx) & mx) << shift;
+        shift += f.bits[k];
+    }
+    return w;
+}
+
+// rgba_cast<float, packed> (e.g. :354-363): field / maximum, alpha 1 when the
+// format has none.
+void packed_decode(const PackedFmt& f, uint32_t w, float c[4])
+{
+    int shift = 0;
+    for (int k = 0; k < 4; ++k)
+    {
+        if (f.bits[k] == 0) { c[k] = (k == 3) ? 1.f : 0.f; continue; }
+        const uint32_t mx = (1u << f.bits[k]) - 1u;
+        c[k] = (float)((w >> shift) & mx) / (float)mx;
+        shift += f.bits[k];
+    }
 }
 
 // IEEE binary16 <-> binary32 (ls::math::half, LightMath absent): the
@@ -315,11 +368,13 @@ struct Target
     uint16_t* depthH;         // R16F depth attachment, or null
     uint8_t* color8;          // 8-bit attachment 0 (comps components), or null
     float* colorF;            // float attachment 0 (comps components), or null
-    int comps;                // components per colour texel
+    int comps;                // components per colour texel (bytes per texel for a packed format)
+    int packed;               // attachment 0's packed format (SL_COLOR_RGB_332 ...), or 0
     // attachments 1..3 of a multiple-output fragment program, same layout
     uint8_t* extra8[3];
     float* extraF[3];
     int extraComps[3];
+    int extraPacked[3];
     uint32_t* dbgLast;
     uint32_t* dbgCount;
     float* vis;               // visibility attachment {seq bits, b0, b1, b2}, or null
@@ -668,18 +723,40 @@ void depth_write(Target& t, size_t idx, float z)
 // destination as (r, 0, 0, 1), (r, g, 0, 1), (r, r, r, 1) for one, two and
 // three components (the three-component read repeats the first component,
 // :306-307), blends all four, clamps and stores the texel's components.
-void place_color_in(const Draw& d, uint8_t* color8, float* colorF, int n, int x, int y, const V4& c);
+void place_color_in(const Draw& d, uint8_t* color8, float* colorF, int n, int packed, int x, int y, const V4& c);
 
 void place_color(const Draw& d, int x, int y, const V4& c)
 {
-    place_color_in(d, d.tgt.color8, d.tgt.colorF, d.tgt.comps, x, y, c);
+    place_color_in(d, d.tgt.color8, d.tgt.colorF, d.tgt.comps, d.tgt.packed, x, y, c);
 }
 
-void place_color_in(const Draw& d, uint8_t* color8, float* colorF, int n, int x, int y, const V4& c)
+void place_color_in(const Draw& d, uint8_t* color8, float* colorF, int n, int packed, int x, int y, const V4& c)
 {
     const Target& t = d.tgt;
     const size_t idx = (size_t)x + (size_t)t.W * (size_t)y;
     const int blend = d.sh->blend;
+    if (packed)
+    {
+        // assign_pixel / assign_alpha_pixel for packed texels (SL_Framebuffer.cpp:82-141, 370-560):
+        // the whole texel decoded, blended, clamped and encoded
+        const PackedFmt f = packed_format(packed);
+        uint8_t* px = color8 + idx * (size_t)f.bytes;
+        uint32_t w = 0;
+        std::memcpy(&w, px, (size_t)f.bytes);
+        if (blend == SL_BLEND_OFF)
+        {
+            w = packed_encode(f, c.v);
+        }
+        else
+        {
+            float dst[4];
+            packed_decode(f, w, dst);
+            blend_rgba(blend, c, dst);
+            w = packed_encode(f, dst);
+        }
+        std::memcpy(px, &w, (size_t)f.bytes);
+        return;
+    }
     auto read_dst = [&](const float v[4], float dst[4]) {
         if (n == 1) { dst[0] = v[0]; dst[1] = 0.f; dst[2] = 0.f; dst[3] = 1.f; }
         else if (n == 2) { dst[0] = v[0]; dst[1] = v[1]; dst[2] = 0.f; dst[3] = 1.f; }
@@ -786,7 +863,8 @@ void finish_fragment(Draw& d, int x, int y, float z, const V4* var, uint32_t seq
     {
         // one output per attachment (SL_FragmentProcessor.cpp:379-395)
         for (unsigned k = d.sh->nout; k-- > 1;)   // attachments 3, 2, 1, then 0 (the switch falls through)
-            place_color_in(d, d.tgt.extra8[k - 1], d.tgt.extraF[k - 1], d.tgt.extraComps[k - 1], x, y, outs[k]);
+            place_color_in(d, d.tgt.extra8[k - 1], d.tgt.extraF[k - 1], d.tgt.extraComps[k - 1], d.tgt.extraPacked[k - 1], x, y,
+                           outs[k]);
         place_color(d, x, y, outs[0]);
     }
 
@@ -1447,7 +1525,8 @@ int setup_draw(sl_context* c, size_t shaderId, size_t fboId, Draw& d)
     d.tgt.color8 = nullptr;
     d.tgt.colorF = nullptr;
     d.tgt.comps = 4;
-    for (int k = 0; k < 3; ++k) { d.tgt.extra8[k] = nullptr; d.tgt.extraF[k] = nullptr; d.tgt.extraComps[k] = 4; }
+    d.tgt.packed = 0;
+    for (int k = 0; k < 3; ++k) { d.tgt.extra8[k] = nullptr; d.tgt.extraF[k] = nullptr; d.tgt.extraComps[k] = 4; d.tgt.extraPacked[k] = 0; }
     // several outputs only from the program that writes them
     if (sh.nout > 1 && sh.fsProg != SL_PROGRAM_LAMBERT_MRT) return SL_ERR_UNSUPPORTED;
     for (unsigned k = 0; k < sh.nout; ++k)
@@ -1457,12 +1536,13 @@ int setup_draw(sl_context* c, size_t shaderId, size_t fboId, Draw& d)
         if (ct.w != dt.w || ct.h != dt.h) return SL_ERR_STATE;
         uint8_t* p8 = nullptr;
         float* pF = nullptr;
-        int n = 0;
+        int n = 0, packed = 0;
         if (u8_components(ct.type)) { p8 = ct.data.data(); n = u8_components(ct.type); }
         else if (float_components(ct.type)) { pF = reinterpret_cast<float*>(ct.data.data()); n = float_components(ct.type); }
+        else if (packed_format(ct.type).bytes) { p8 = ct.data.data(); n = packed_format(ct.type).bytes; packed = ct.type; }
         else return SL_ERR_UNSUPPORTED;
-        if (k == 0) { d.tgt.color8 = p8; d.tgt.colorF = pF; d.tgt.comps = n; }
-        else { d.tgt.extra8[k - 1] = p8; d.tgt.extraF[k - 1] = pF; d.tgt.extraComps[k - 1] = n; }
+        if (k == 0) { d.tgt.color8 = p8; d.tgt.colorF = pF; d.tgt.comps = n; d.tgt.packed = packed; }
+        else { d.tgt.extra8[k - 1] = p8; d.tgt.extraF[k - 1] = pF; d.tgt.extraComps[k - 1] = n; d.tgt.extraPacked[k - 1] = packed; }
     }
     d.tgt.dbgLast = (f.dbgLast < c->texs.size()) ? reinterpret_cast<uint32_t*>(c->texs[f.dbgLast].data.data()) : nullptr;
     d.tgt.dbgCount = (f.dbgCount < c->texs.size()) ? reinterpret_cast<uint32_t*>(c->texs[f.dbgCount].data.data()) : nullptr;
@@ -1682,6 +1762,23 @@ int clear_color(sl_context* c, size_t fbo, unsigned att, const double rgba[4])
         float v[4];
         for (int k = 0; k < 4; ++k) v[k] = (float)rgba[k];
         clear_texture_bytes(t, v, 4u * (uint32_t)n);
+    }
+    else if (packed_format(t.type).bytes)
+    {
+        // rgba_cast<packed, double>: the products in double, then the field's integer type
+        const PackedFmt f = packed_format(t.type);
+        uint32_t w = 0;
+        int shift = 0;
+        for (int k = 0; k < 4; ++k)
+        {
+            if (f.bits[k] == 0) continue;
+            const uint32_t mx = (1u << f.bits[k]) - 1u;
+            const double v = rgba[k] * (double)mx;
+            const int64_t iv = (v >= -9.2e18 && v < 9.2e18) ? (int64_t)v : INT64_MIN;
+            w |= ((uint32_t)iv & mx) << shift;
+            shift += f.bits[k];
+        }
+        clear_texture_bytes(t, &w, (uint32_t)f.bytes);
     }
     else return SL_ERR_UNSUPPORTED;
     return SL_OK;

@@ -44,6 +44,7 @@ SL_COLOR_R_8U, SL_COLOR_R_32U, SL_COLOR_R_HALF, SL_COLOR_R_FLOAT = 0, 2, 4, 5
 SL_COLOR_RG_8U, SL_COLOR_RG_FLOAT = 7, 12
 SL_COLOR_RGB_8U, SL_COLOR_RGB_FLOAT = 14, 19
 SL_COLOR_RGBA_8U, SL_COLOR_RGBA_FLOAT = 21, 26
+SL_COLOR_RGB_332, SL_COLOR_RGB_565, SL_COLOR_RGBA_5551, SL_COLOR_RGBA_4444, SL_COLOR_RGBA_1010102 = 28, 29, 30, 31, 32
 
 SL_PROGRAM_PASSTHROUGH_GOURAUD = 1
 SL_PROGRAM_LAMBERT_TEXTURED = 2
@@ -53,7 +54,9 @@ SL_PROGRAM_LAMBERT_MRT = 4
 _TEXEL_DTYPE = {SL_COLOR_R_8U: (np.uint8, 1), SL_COLOR_RG_8U: (np.uint8, 2), SL_COLOR_RGB_8U: (np.uint8, 3),
                 SL_COLOR_RGBA_8U: (np.uint8, 4), SL_COLOR_R_32U: (np.uint32, 1), SL_COLOR_R_HALF: (np.float16, 1),
                 SL_COLOR_R_FLOAT: (np.float32, 1), SL_COLOR_RG_FLOAT: (np.float32, 2),
-                SL_COLOR_RGB_FLOAT: (np.float32, 3), SL_COLOR_RGBA_FLOAT: (np.float32, 4)}
+                SL_COLOR_RGB_FLOAT: (np.float32, 3), SL_COLOR_RGBA_FLOAT: (np.float32, 4),
+                SL_COLOR_RGB_332: (np.uint8, 1), SL_COLOR_RGB_565: (np.uint16, 1), SL_COLOR_RGBA_5551: (np.uint16, 1),
+                SL_COLOR_RGBA_4444: (np.uint16, 1), SL_COLOR_RGBA_1010102: (np.uint32, 1)}
 _BYTES_PER_TEXEL = {t: np.dtype(dt).itemsize * n for t, (dt, n) in _TEXEL_DTYPE.items()}
 
 

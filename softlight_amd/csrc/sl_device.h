@@ -103,6 +103,61 @@ SL_HD bool mesh_indexed(uint32_t prim, int mode)
 
 SL_HD uint32_t verts_per_prim(uint32_t prim) { return prim == kPrimPoint ? 1u : prim == kPrimLine ? 2u : 3u; }
 
+// Packed colour formats (SL_ColorCompressed.hpp): bit-field widths of r, g,
+// b, a (r lowest, as the structs' bit-fields lie) and texel bytes; all zero
+// for other formats.
+struct PackedFmt { int bits[4]; int bytes; };
+
+SL_HD PackedFmt packed_format(int type)
+{
+    switch (type)
+    {
+        case SL_COLOR_RGB_332: return PackedFmt{{3, 3, 2, 0}, 1};
+        case SL_COLOR_RGB_565: return PackedFmt{{5, 6, 5, 0}, 2};
+        case SL_COLOR_RGBA_5551: return PackedFmt{{5, 5, 5, 1}, 2};
+        case SL_COLOR_RGBA_4444: return PackedFmt{{4, 4, 4, 4}, 2};
+        case SL_COLOR_RGBA_1010102: return PackedFmt{{10, 10, 10, 2}, 4};
+        default: return PackedFmt{{0, 0, 0, 0}, 0};
+    }
+}
+
+// rgba_cast<packed, float> (e.g. SL_ColorCompressed.hpp:300-310): each
+// component times its maximum, truncated to an int (low bits kept, as for
+// 8-bit colour), stored in its bit-field.
+SL_HD uint32_t packed_encode(const PackedFmt& f, const float c[4])
+{
+    uint32_t w = 0;
+    int shift = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        if (f.bits[k] == 0) continue;
+        const uint32_t mx = (1u << f.bits[k]) - 1u;
+        w |= ((uint32_t)f2i(c[k] * (float)mx) & mx) << shift;
+        shift += f.bits[k];
+    }
+    return w;
+}
+
+// rgba_cast<float, packed> (e.g. :354-363): field / maximum; alpha 1 when
+// the format has none.
+SL_HD void packed_decode(const PackedFmt& f, uint32_t w, float c[4])
+{
+    int shift = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+        if (f.bits[k] == 0)
+        {
+            c[k] = (k == 3) ? 1.f : 0.f;
+            continue;
+        }
+        const uint32_t mx = (1u << f.bits[k]) - 1u;
+        c[k] = (float)((w >> shift) & mx) / (float)mx;
+        shift += f.bits[k];
+    }
+}
+
 // Line draws walk 16-bit Bresenham coordinates (sl_lowp_t); targets up to
 // this size keep every walk's error term inside int16.
 constexpr int kMaxLineTarget = 8192;
@@ -305,10 +360,12 @@ struct RasterParams
     int32_t W, H, tilesX, tilesY;
     uint8_t* color;               // attachment 0 texels (8-bit or float, colorComps components) or null
     int32_t colorType;
-    int32_t colorComps;           // components per colour texel (1..4)
+    int32_t colorComps;           // components per colour texel (1..4; bytes per texel when packed)
+    int32_t colorPacked;          // packed format of attachment 0 (SL_COLOR_RGB_332 ...), or 0
     uint8_t* colorX[3];           // colour attachments 1..3 (outputs 1..3 of a multiple-output program), or null
-    int32_t colorXFloat[3];       // their texels are float (else 8-bit)
+    int32_t colorXFloat[3];       // their texels are float (else 8-bit or packed)
     int32_t colorXComps[3];
+    int32_t colorXPacked[3];
     int32_t depthHalf;            // the depth attachment is R16F (else R32F)
     float* depth;
     uint32_t* dbgLast;

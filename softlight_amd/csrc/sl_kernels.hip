@@ -1494,6 +1494,21 @@ __device__ __forceinline__ void write_extra_output(const RasterParams& P, int k,
         for (int c = 0; c < 4; ++c)
             if (c < n) px[c] = d[c];
     }
+    else if (P.colorXPacked[k] != 0)
+    {
+        const PackedFmt f = packed_format(P.colorXPacked[k]);
+        uint8_t* px = P.colorX[k] + gi * (uint64_t)f.bytes;
+        uint32_t w = 0;
+        for (int b = 0; b < f.bytes; ++b) w |= (uint32_t)px[b] << (8 * b);
+        float d[4] = {o.x, o.y, o.z, o.w};
+        if (blendMode != SL_BLEND_OFF)
+        {
+            packed_decode(f, w, d);
+            blend(blendMode, o, d);
+        }
+        w = packed_encode(f, d);
+        for (int b = 0; b < f.bytes; ++b) px[b] = (uint8_t)(w >> (8 * b));
+    }
     else
     {
         uint8_t* px = P.colorX[k] + gi * (uint64_t)n;
@@ -1573,7 +1588,7 @@ __device__ __forceinline__ bool shade(const RasterParams& P, const DrawDesc& D, 
 // depth texel (R32F, or R16F read as float and written rounded to nearest).
 __device__ __forceinline__ uint32_t load_color8(const RasterParams& P, uint64_t gi)
 {
-    const int n = P.colorComps;
+    const int n = P.colorComps;   // bytes per texel for a packed format
     if (n == 4) return reinterpret_cast<const uint32_t*>(P.color)[gi];
     const uint8_t* p = P.color + gi * (uint64_t)n;
     uint32_t v = 0;
@@ -1619,8 +1634,21 @@ __device__ __forceinline__ void store_depth(const RasterParams& P, uint64_t gi, 
 }
 
 template <int CF>
-__device__ __forceinline__ void blend_into(uint32_t* col8, float4* colF, int blendMode, float4 out, int comps)
+__device__ __forceinline__ void blend_into(uint32_t* col8, float4* colF, int blendMode, float4 out, int comps, int packed)
 {
+    if (CF == 1 && packed != 0)
+    {
+        // a packed texel: decoded whole, blended, clamped, encoded (SL_Framebuffer.cpp:370-560)
+        const PackedFmt f = packed_format(packed);
+        float o[4] = {out.x, out.y, out.z, out.w};
+        if (blendMode != SL_BLEND_OFF)
+        {
+            packed_decode(f, *col8, o);
+            blend(blendMode, out, o);
+        }
+        *col8 = packed_encode(f, o);
+        return;
+    }
     if (CF == 1)
     {
         if (blendMode == SL_BLEND_OFF)
@@ -1738,7 +1766,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
         float4 out, wv;
         const bool have = shade<MRT>(P, QD, qhw, Q.bc3, Q.state, prid, sPendS[0][r][c], sPendS[1][r][c], sPendS[2][r][c], out, wv,
                                 (uint64_t)(y0 + r) * P.W + (x0 + c), SL_BLEND_OFF);
-        if (have) blend_into<CF>(&sCol8[CF == 1 ? r : 0][CF == 1 ? c : 0], &sColF[CF == 2 ? r : 0][CF == 2 ? c : 0], SL_BLEND_OFF, out, P.colorComps);
+        if (have) blend_into<CF>(&sCol8[CF == 1 ? r : 0][CF == 1 ? c : 0], &sColF[CF == 2 ? r : 0][CF == 2 ? c : 0], SL_BLEND_OFF, out, P.colorComps, P.colorPacked);
         if (VIS) sVis[r][c] = make_float4(__uint_as_float(Q.seq), wv.x, wv.y, wv.z);
     };
 
@@ -1860,7 +1888,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                         const bool have = shade<MRT>(P, D, hw, R.bc3, st, rid, s0, s1, s2, out, wv,
                                                 (uint64_t)y * P.W + x, blendMode);
                         if (have) blend_into<CF>(&sCol8[CF == 1 ? row : 0][CF == 1 ? c : 0],
-                                                 &sColF[CF == 2 ? row : 0][CF == 2 ? c : 0], blendMode, out, P.colorComps);
+                                                 &sColF[CF == 2 ? row : 0][CF == 2 ? c : 0], blendMode, out, P.colorComps, P.colorPacked);
                         if (VIS) sVis[row][c] = make_float4(__uint_as_float(R.seq), wv.x, wv.y, wv.z);
                     }
                     if (dmask) sDepth[row][c] = P.depthHalf ? __half2float(__float2half_rn(z)) : z;

@@ -116,6 +116,9 @@ uint32_t bytes_per_color(int type)
         case SL_COLOR_R_HALF: return 2;
         case SL_COLOR_RG_FLOAT: return 8;
         case SL_COLOR_RGB_FLOAT: return 12;
+        case SL_COLOR_RGB_332: return 1;
+        case SL_COLOR_RGB_565: case SL_COLOR_RGBA_5551: case SL_COLOR_RGBA_4444: return 2;
+        case SL_COLOR_RGBA_1010102: return 4;
         default: return 0;
     }
 }
@@ -132,6 +135,36 @@ int float_components(int type)
 }
 
 bool depth_format(int type) { return type == SL_COLOR_R_FLOAT || type == SL_COLOR_R_HALF; }
+
+// A colour attachment draws can write: 8-bit, float or packed.
+bool color_format(int type) { return u8_components(type) || float_components(type) || packed_format(type).bytes; }
+
+// Texel units the raster walks per attachment texel: components, or bytes of a packed texel.
+int color_units(int type)
+{
+    if (u8_components(type)) return u8_components(type);
+    if (float_components(type)) return float_components(type);
+    return packed_format(type).bytes;
+}
+
+// The clear colour as a texel of a packed format: rgba_cast<packed, double>
+// (SL_Color.cpp:180-184), the products in double, truncated, low bits kept.
+uint32_t packed_clear_word(int type, const double rgba[4])
+{
+    const PackedFmt f = packed_format(type);
+    uint32_t w = 0;
+    int shift = 0;
+    for (int k = 0; k < 4; ++k)
+    {
+        if (f.bits[k] == 0) continue;
+        const uint32_t mx = (1u << f.bits[k]) - 1u;
+        const double v = rgba[k] * (double)mx;
+        const int64_t iv = (v >= -9.2e18 && v < 9.2e18) ? (int64_t)v : INT64_MIN;
+        w |= ((uint32_t)iv & mx) << shift;
+        shift += f.bits[k];
+    }
+    return w;
+}
 
 // float -> IEEE binary16, round to nearest even (the depth clear value of a
 // half depth attachment, (half)(float)depth, SL_Context.cpp:946-948), and back.
@@ -338,12 +371,12 @@ int make_plan(sl_context* c, const Batch& B, Plan& p)
     p.dt = tex_at(c, F.depth);
     if (!p.dt || !depth_format(p.dt->type)) return fail(c, SL_ERR_STATE, "framebuffer needs an R_FLOAT or R_HALF depth attachment");
     p.ct = F.colors.empty() ? nullptr : tex_at(c, F.colors[0]);
-    if (p.ct && !u8_components(p.ct->type) && !float_components(p.ct->type)) return fail(c, SL_ERR_UNSUPPORTED, "colour attachment format");
+    if (p.ct && !color_format(p.ct->type)) return fail(c, SL_ERR_UNSUPPORTED, "colour attachment format");
     for (int k = 0; k < 3; ++k)
     {
         // attachments 1..3, written by multiple-output programs (validated per draw)
         Tex* xt = (F.colors.size() > (size_t)k + 1) ? tex_at(c, F.colors[k + 1]) : nullptr;
-        p.cx[k] = (xt && (u8_components(xt->type) || float_components(xt->type)) && xt->w == p.dt->w && xt->h == p.dt->h) ? xt : nullptr;
+        p.cx[k] = (xt && color_format(xt->type) && xt->w == p.dt->w && xt->h == p.dt->h) ? xt : nullptr;
     }
     p.dbgL = tex_at(c, F.dbgLast);
     p.dbgC = tex_at(c, F.dbgCount);
@@ -446,14 +479,16 @@ RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool have
     rp.W = p.W; rp.H = p.H; rp.tilesX = p.tilesX; rp.tilesY = p.tilesY;
     rp.color = p.ct ? reinterpret_cast<uint8_t*>(p.ct->buf.p) : nullptr;
     rp.colorType = p.ct ? p.ct->type : SL_COLOR_RGBA_8U;
-    rp.colorComps = p.ct ? (u8_components(p.ct->type) ? u8_components(p.ct->type) : float_components(p.ct->type)) : 4;
+    rp.colorComps = p.ct ? color_units(p.ct->type) : 4;
+    rp.colorPacked = (p.ct && packed_format(p.ct->type).bytes) ? p.ct->type : 0;
     rp.depthHalf = (p.dt->type == SL_COLOR_R_HALF) ? 1 : 0;
     for (int k = 0; k < 3; ++k)
     {
         const Tex* xt = p.cx[k];
         rp.colorX[k] = xt ? reinterpret_cast<uint8_t*>(xt->buf.p) : nullptr;
         rp.colorXFloat[k] = (xt && float_components(xt->type)) ? 1 : 0;
-        rp.colorXComps[k] = xt ? (float_components(xt->type) ? float_components(xt->type) : u8_components(xt->type)) : 4;
+        rp.colorXComps[k] = xt ? color_units(xt->type) : 4;
+        rp.colorXPacked[k] = (xt && packed_format(xt->type).bytes) ? xt->type : 0;
     }
     rp.depth = reinterpret_cast<float*>(p.dt->buf.p);
     rp.dbgLast = p.debug ? reinterpret_cast<uint32_t*>(p.dbgL->buf.p) : nullptr;
@@ -467,7 +502,7 @@ RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool have
         packed |= ((uint32_t)(uint8_t)(int64_t)(B.rgba[k] * 255.0)) << (8 * k);   // color_cast<uint8_t,double>
         rp.clearRGBAF[k] = (float)B.rgba[k];
     }
-    rp.clearRGBA8 = packed;
+    rp.clearRGBA8 = rp.colorPacked ? packed_clear_word(rp.colorPacked, B.rgba) : packed;
     rp.clearDepthValue = rp.depthHalf ? host_half_to_float(host_float_to_half((float)B.depth)) : (float)B.depth;
     rp.extra = 0;
     rp.mrt = 0;
@@ -834,7 +869,7 @@ int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t s
     {
         Tex* ct = (F.colors.size() > k) ? tex_at(c, F.colors[k]) : nullptr;
         if (!ct) return fail(c, SL_ERR_STATE, "framebuffer lacks a colour attachment for a fragment output");
-        if (!u8_components(ct->type) && !float_components(ct->type)) return fail(c, SL_ERR_UNSUPPORTED, "colour attachment format");
+        if (!color_format(ct->type)) return fail(c, SL_ERR_UNSUPPORTED, "colour attachment format");
         if (ct->w != dt->w || ct->h != dt->h) return fail(c, SL_ERR_STATE, "colour and depth attachment sizes differ");
     }
     const Vao& vao = c->vaos[m.vaoId];
@@ -972,7 +1007,7 @@ int record_clear(sl_context* c, size_t fboId, int att, const double* rgba, bool 
     {
         if (att < 0 || (size_t)att >= F.colors.size() || !tex_at(c, F.colors[att])) return fail(c, SL_ERR_INVALID_HANDLE, "colour attachment");
         const int ty = c->texs[F.colors[att]].type;
-        if (!u8_components(ty) && !float_components(ty)) return fail(c, SL_ERR_UNSUPPORTED, "clear of this colour format");
+        if (!color_format(ty)) return fail(c, SL_ERR_UNSUPPORTED, "clear of this colour format");
     }
     if (depth)
     {
@@ -995,6 +1030,11 @@ int record_clear(sl_context* c, size_t fboId, int att, const double* rgba, bool 
             uint8_t v[4];
             for (int k = 0; k < 4; ++k) v[k] = (uint8_t)(int64_t)(rgba[k] * 255.0);
             return fill_texture(c, t, v, (uint32_t)n);
+        }
+        if (packed_format(t.type).bytes)
+        {
+            const uint32_t w = packed_clear_word(t.type, rgba);
+            return fill_texture(c, t, &w, (uint32_t)packed_format(t.type).bytes);
         }
         float v[4];
         for (int k = 0; k < 4; ++k) v[k] = (float)rgba[k];

@@ -172,3 +172,92 @@ def test_gpu_batched_rgb8(gpu_lib, oracle_lib):
                        W=640, H=360, n_textures=4, tex_size=64, debug=True, color_type=A.SL_COLOR_RGB_8U,
                        depth_type=A.SL_COLOR_R_HALF)
     assert_parity(g, o)
+
+
+# ---------------------------------------------------------------------------
+# packed formats (SL_ColorCompressed.hpp): fields r, g, b, a from the lowest bit
+# ---------------------------------------------------------------------------
+PACKED = {A.SL_COLOR_RGB_332: (3, 3, 2, 0), A.SL_COLOR_RGB_565: (5, 6, 5, 0), A.SL_COLOR_RGBA_5551: (5, 5, 5, 1),
+          A.SL_COLOR_RGBA_4444: (4, 4, 4, 4), A.SL_COLOR_RGBA_1010102: (10, 10, 10, 2)}
+
+
+def fields(word, bits):
+    out, shift = [], 0
+    for b in bits:
+        out.append((int(word) >> shift) & ((1 << b) - 1) if b else None)
+        shift += b
+    return out
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+@pytest.mark.parametrize("fmt", list(PACKED))
+def test_packed_texels(kind, make_ctx, fmt):
+    """Drawn texels: each field = trunc(c * max) of the float colour the same
+    draw writes to an RGBA32F target; cleared texels: trunc(clear * max) in
+    double."""
+    ctx = make_ctx(kind)
+    W, H = 64, 48
+    bits = PACKED[fmt]
+    rng = np.random.default_rng(6)
+    cols = rng.integers(0, 256, (6, 4)).astype(np.uint8)
+    out = []
+    for t in (A.SL_COLOR_RGBA_FLOAT, fmt):
+        s = tri_scene(ctx, W, H, TRIS, z=ZS, colors=cols, color_type=t, depth_test=A.SL_DEPTH_TEST_LESS_THAN)
+        s.clear_rgba = (0.3, 0.55, 0.8, 0.6)
+        out.append(render(ctx, s))
+    ref, got = out
+    covered = ref["count"] > 0
+    assert covered.sum() > 500
+    words = got["color"].reshape(H, W)
+    fc = ref["color"].reshape(H, W, 4)
+    ys, xs = np.nonzero(covered)
+    for y, x in list(zip(ys, xs))[::7]:
+        f = fields(words[y, x], bits)
+        for k, b in enumerate(bits):
+            if b:
+                assert f[k] == int(np.float32(fc[y, x, k]) * np.float32((1 << b) - 1)) % (1 << b), (x, y, k)
+    cy, cx = np.nonzero(~covered)
+    f = fields(words[cy[0], cx[0]], bits)
+    for k, (b, c) in enumerate(zip(bits, (0.3, 0.55, 0.8, 0.6))):
+        if b:
+            assert f[k] == int(c * ((1 << b) - 1)), (k, f)
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+@pytest.mark.parametrize("fmt", [A.SL_COLOR_RGB_565, A.SL_COLOR_RGBA_4444])
+def test_packed_blend_decodes_the_texel(kind, make_ctx, fmt):
+    """Additive blend of an opaque black source: the texel decodes to
+    field / max, blends, clamps and encodes back: the cleared colour fields
+    survive unchanged, alpha saturates."""
+    ctx = make_ctx(kind)
+    W, H = 32, 32
+    tri = [(1, 1), (30, 2), (4, 29)]
+    s = tri_scene(ctx, W, H, [tri], colors=[(0, 0, 0, 255)] * 3, color_type=fmt, blend=A.SL_BLEND_ADDITIVE)
+    s.clear_rgba = (0.3, 0.55, 0.8, 0.6)
+    o = render(ctx, s)
+    words = o["color"].reshape(H, W)
+    inside = o["count"] > 0
+    assert inside.sum() > 200
+    bits = PACKED[fmt]
+    want = []
+    for k, (b, c) in enumerate(zip(bits, (0.3, 0.55, 0.8, 0.6))):
+        if b and k == 3:
+            want.append((1 << b) - 1)          # alpha: 0.6 + 1 * 1 clamps to 1
+        elif b:
+            m = (1 << b) - 1
+            v = int(c * m)
+            # decode v / m in float32, add 0, encode: trunc((v / m) * m)
+            want.append(int(np.float32(np.float32(v) / np.float32(m)) * np.float32(m)))
+        else:
+            want.append(None)
+    for y, x in zip(*np.nonzero(inside)):
+        assert [v for v in fields(words[y, x], bits)] == want
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fmt", list(PACKED))
+def test_gpu_packed_parity(gpu_lib, oracle_lib, fmt):
+    g, o = render_both(gpu_lib, oracle_lib, scenes.build_random_triangles, seed=13, ntri=600, W=320, H=240,
+                       debug=True, color_type=fmt, blend=A.SL_BLEND_ALPHA)
+    np.testing.assert_array_equal(g["color"], o["color"])
+    np.testing.assert_array_equal(g["count"], o["count"])
