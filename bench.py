@@ -255,6 +255,35 @@ def run_demo_targets(steps, warmup):
             "fragments_shaded_per_frame": int(st["fragments_shaded"] // f)}
 
 
+def run_scene_file(frames=12, n_objects=120, level=3):
+    """Host ingestion: a scene file of `n_objects` textured icospheres
+    (20*4^level triangles each) loaded by the C++ scene-file loader into a
+    scene graph, then per frame: a root edit, the graph update, the draw-list
+    build, one deferred draw per item and one synchronisation
+    (softlight_amd/examples/scene_graph_draw.cpp)."""
+    import subprocess
+    import tempfile
+
+    import softlight_amd
+    from softlight_amd import scenes
+    exe = os.path.join(os.path.dirname(softlight_amd.LIB_PATH), "scene_graph_draw")
+    if not os.path.exists(exe):
+        return {"skipped": "scene_graph_draw not built"}
+    with tempfile.TemporaryDirectory() as d:
+        obj, counts = scenes.write_icosphere_scene(d, n_objects=n_objects, level=level, extent=12.0)
+        cmds = os.path.join(d, "nodes.txt")
+        with open(cmds, "w") as f:
+            f.write("")
+        r = subprocess.run([exe, "render", obj, cmds, "1920", "1080", os.path.join(d, "c.bin"), os.path.join(d, "d.bin"),
+                            str(frames)], capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            raise RuntimeError(f"scene_graph_draw failed ({r.returncode}): {r.stderr.strip()}")
+        out = json.loads(r.stdout.strip().splitlines()[-1])
+        out["file_MB"] = round(os.path.getsize(obj) / 2 ** 20, 1)
+    out["Mtri_per_s"] = round(out["triangles"] / (out["frame_ms"] * 1e-3) / 1e6, 1)
+    return out
+
+
 def run_render_modes(steps, warmup):
     """The 1080p sphere frame with its index buffer drawn as wireframe
     triangles, lines (index pairs) and points (every index): per-frame time
@@ -341,11 +370,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--headline-only", action="store_true",
                     help="spheres: only the headline frame (no differentiable, smoothing, render-mode, target or blit legs)")
-    ap.add_argument("--workload", choices=["spheres", "random", "batched", "views", "modes"], default="spheres",
+    ap.add_argument("--workload", choices=["spheres", "random", "batched", "views", "modes", "scene_file"], default="spheres",
                     help="spheres: the 1080p textured-sphere frame (default); random: 4096 random triangles at "
                          "640x480; batched: 1000 meshes with mixed opaque/blended materials at 1080p; views: "
                          "32 differentiable 512x512 views, forward and forward+backward; modes: the sphere frame's "
-                         "index buffer drawn as wireframe triangles, lines and points (its leg only)")
+                         "index buffer drawn as wireframe triangles, lines and points (its leg only); scene_file: "
+                         "a 120-object OBJ file loaded into the scene graph and drawn per frame (its leg only)")
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter calibration)")
     ap.add_argument("--raster-config", type=int, default=None, help="raster tile configuration 0..7 (default: tuned)")
     ap.add_argument("--tune", action="store_true", help="time all raster configurations first and cache the best")
@@ -363,6 +393,9 @@ def main():
     if args.workload == "modes":
         print(json.dumps({"render_modes": run_render_modes(args.steps, args.warmup),
                           "demo_targets": run_demo_targets(args.steps, args.warmup)}), flush=True)
+        return
+    if args.workload == "scene_file":
+        print(json.dumps({"scene_file": run_scene_file(max(2, args.steps))}), flush=True)
         return
     scene, st, eager_s, gst, elapsed = run_gpu(args.steps, args.warmup, args.workload, args.raster_config, args.tune)
     tris = scene.n_triangles
@@ -451,6 +484,7 @@ def main():
         out["render_modes"] = run_render_modes(max(3, args.steps // 4), 1)
         out["demo_targets"] = run_demo_targets(max(3, args.steps // 2), 1)
         out["present_blit"] = run_blit()
+        out["scene_file"] = run_scene_file()
     if not args.no_cpu and args.workload == "spheres":
         out["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -58,6 +58,15 @@ def build(verbose: bool = False) -> str:
           os.path.join(HERE, "examples", "facade_draw.cpp"), "-L", OUT, "-lsoftlight_amd",
           f"-Wl,-rpath,$ORIGIN"])
     os.replace(ex + ".tmp", ex)
+    # the scene-file example (include/softlight_amd/SL_SceneFileLoader.hpp):
+    # the full program, and a host-only build of its graph dump
+    inc = ["-std=c++17", "-O2", "-ffp-contract=off", "-I", os.path.join(HERE, "..", "include")]
+    src = os.path.join(HERE, "examples", "scene_graph_draw.cpp")
+    for name, extra in (("scene_graph_draw", ["-L", OUT, "-lsoftlight_amd", "-Wl,-rpath,$ORIGIN"]),
+                        ("scene_graph_dump", ["-DSL_SCENE_DUMP_ONLY"])):
+        exe = os.path.join(OUT, name)
+        _run(["g++", *inc, "-o", exe + ".tmp", src, *extra])
+        os.replace(exe + ".tmp", exe)
     return LIB
 
 

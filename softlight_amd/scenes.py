@@ -750,3 +750,73 @@ class SmoothScene:
             self.ctx.loss_l2_grad(t["smooth"], t["target"], t["grad_smooth"], 1.0)
             self.ctx.smooth_coverage_backward(t["fbo"], t["grad_smooth"], sigma, t["grad_image"])
             self.ctx.backward(t["fbo"], t["grad_image"])
+
+
+# --------------------------------------------------------------------------
+# scene files: Wavefront OBJ + MTL + PPM written from the seeded streams
+# --------------------------------------------------------------------------
+def _fmt(x) -> str:
+    """Shortest text that reads back as the same float32."""
+    return np.format_float_positional(np.float32(x), unique=True, trim="-")
+
+
+def write_ppm(path, rgba: np.ndarray):
+    """Binary PPM (P6) of an RGBA8 image's RGB channels."""
+    h, w = rgba.shape[:2]
+    with open(path, "wb") as f:
+        f.write(b"P6\n%d %d\n255\n" % (w, h))
+        f.write(np.ascontiguousarray(rgba[:, :, :3]).tobytes())
+
+
+def write_icosphere_scene(directory, seed=8, n_objects=6, level=2, n_materials=3, tex_size=64, extent=6.0,
+                          name="spheres.obj"):
+    """`n_objects` displaced icospheres as OBJ objects ('o sphere<i>'), each
+    at its own position in the file's coordinates (a cube of side 2*extent
+    around (0, 0, -2*extent)), with v/vt/vn per corner; materials cycle
+    over `n_materials` entries of which the even ones carry a value-noise
+    diffuse map.  Every other object's second half of faces switches to the
+    next material, so objects hold one or two meshes.  Returns the OBJ path
+    and per-object triangle counts."""
+    import os
+    base_v, faces = icosphere(level)
+    r = uniform01(seed, n_objects * 8, stream=40).astype(np.float64).reshape(n_objects, 8)
+    mtl = []
+    for m in range(n_materials):
+        mtl.append(f"newmtl mat{m}")
+        mtl.append(f"Ka {_fmt(0.1 + 0.05 * m)} {_fmt(0.1)} {_fmt(0.15)}")
+        mtl.append(f"Kd {_fmt(0.9 - 0.1 * m)} {_fmt(0.8)} {_fmt(0.7 + 0.1 * m)}")
+        if m % 2 == 0:
+            img = f"tex{m}.ppm"
+            write_ppm(os.path.join(directory, img), value_noise_rgba8(tex_size, seed * 31 + m, cells=8))
+            mtl.append(f"map_Kd {img}")
+    with open(os.path.join(directory, "spheres.mtl"), "w") as f:
+        f.write("\n".join(mtl) + "\n")
+    lines = ["mtllib spheres.mtl"]
+    counts = []
+    nv = 0
+    for o in range(n_objects):
+        disp = 1.0 + 0.2 * (uniform01(seed * 7919 + o, base_v.shape[0], stream=41).astype(np.float64) - 0.5)
+        rad = 0.6 + 0.8 * r[o, 3]
+        c = np.array([(r[o, 0] - 0.5) * 2 * extent, (r[o, 1] - 0.5) * 2 * extent, -2 * extent + (r[o, 2] - 0.5) * extent])
+        pos = (base_v * (disp[:, None] * rad) + c).astype(np.float32)
+        uv = np.stack([np.arctan2(base_v[:, 2], base_v[:, 0]) / (2 * np.pi) + 0.5,
+                       np.arcsin(np.clip(base_v[:, 1], -1, 1)) / np.pi + 0.5], axis=1).astype(np.float32)
+        nrm = base_v.astype(np.float32)
+        lines.append(f"o sphere{o}")
+        lines += [f"v {_fmt(p[0])} {_fmt(p[1])} {_fmt(p[2])}" for p in pos]
+        lines += [f"vt {_fmt(t[0])} {_fmt(t[1])}" for t in uv]
+        lines += [f"vn {_fmt(n[0])} {_fmt(n[1])} {_fmt(n[2])}" for n in nrm]
+        m0 = o % n_materials
+        lines.append(f"usemtl mat{m0}")
+        half = faces.shape[0] // 2
+        for k, tri in enumerate(faces):
+            if o % 2 == 1 and k == half:
+                lines.append(f"usemtl mat{(m0 + 1) % n_materials}")
+            a, b, cc = (int(i) + 1 + nv for i in tri)
+            lines.append(f"f {a}/{a}/{a} {b}/{b}/{b} {cc}/{cc}/{cc}")
+        nv += base_v.shape[0]
+        counts.append(int(faces.shape[0]))
+    path = os.path.join(directory, name)
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path, counts
