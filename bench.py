@@ -102,7 +102,12 @@ def run_gpu(steps, warmup, workload="spheres", raster_config=None, tune=False):
     elif raster_config is not None:
         ctx.set_raster_config(raster_config)
     else:
-        cached = autotune.lookup(signature, os.environ.get("SL_TUNE_CACHE", autotune.DEFAULT_CACHE))
+        cache = os.environ.get("SL_TUNE_CACHE", autotune.DEFAULT_CACHE)
+        cached = autotune.lookup(signature, cache)
+        if cached is None and workload.startswith("spheres_"):
+            # the sphere frame's variants (other targets, other render modes)
+            # bin the same geometry: start from the sphere frame's tile
+            cached = autotune.lookup(f"spheres:{scene.W}x{scene.H}:{scene.n_triangles}", cache)
         if cached is not None:
             ctx.set_raster_config(cached)
     scene.info["raster_config"] = ctx.raster_config()
@@ -255,7 +260,7 @@ def run_demo_targets(steps, warmup):
             "fragments_shaded_per_frame": int(st["fragments_shaded"] // f)}
 
 
-def run_scene_file(frames=12, n_objects=120, level=3):
+def run_scene_file(frames=12, n_objects=120, level=3, cpu=True):
     """Host ingestion: a scene file of `n_objects` textured icospheres
     (20*4^level triangles each) loaded by the C++ scene-file loader into a
     scene graph, then per frame: a root edit, the graph update, the draw-list
@@ -280,7 +285,17 @@ def run_scene_file(frames=12, n_objects=120, level=3):
             raise RuntimeError(f"scene_graph_draw failed ({r.returncode}): {r.stderr.strip()}")
         out = json.loads(r.stdout.strip().splitlines()[-1])
         out["file_MB"] = round(os.path.getsize(obj) / 2 ** 20, 1)
-    out["Mtri_per_s"] = round(out["triangles"] / (out["frame_ms"] * 1e-3) / 1e6, 1)
+        out["Mtri_per_s"] = round(out["triangles"] / (out["frame_ms"] * 1e-3) / 1e6, 1)
+        if cpu:
+            # the same frame drawn by the oracle's thread pool from the dumped draw list
+            import oracle
+            from softlight_amd import scene_files
+            dump = scene_files.run_dump(scene_files.dump_executable(), obj, "", d)
+            threads = min(16, os.cpu_count() or 1)
+            times = [scene_files.draw_dump(oracle.load(), dump, 1920, 1080, threads=threads)["ms"] for _ in range(3)]
+            ms = sorted(times)[1]
+            out["cpu_baseline"] = {"ms_per_frame": round(ms, 2), "Mtri_per_s": round(out["triangles"] / (ms * 1e-3) / 1e6, 1),
+                                   "cores": threads, "kind": "port", "sample": "the same frame, median of 3"}
     return out
 
 
@@ -395,7 +410,7 @@ def main():
                           "demo_targets": run_demo_targets(args.steps, args.warmup)}), flush=True)
         return
     if args.workload == "scene_file":
-        print(json.dumps({"scene_file": run_scene_file(max(2, args.steps))}), flush=True)
+        print(json.dumps({"scene_file": run_scene_file(max(2, args.steps), cpu=not args.no_cpu)}), flush=True)
         return
     scene, st, eager_s, gst, elapsed = run_gpu(args.steps, args.warmup, args.workload, args.raster_config, args.tune)
     tris = scene.n_triangles
@@ -484,7 +499,7 @@ def main():
         out["render_modes"] = run_render_modes(max(3, args.steps // 4), 1)
         out["demo_targets"] = run_demo_targets(max(3, args.steps // 2), 1)
         out["present_blit"] = run_blit()
-        out["scene_file"] = run_scene_file()
+        out["scene_file"] = run_scene_file(cpu=not args.no_cpu)
     if not args.no_cpu and args.workload == "spheres":
         out["cpu_baseline"] = run_cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -44,6 +44,7 @@ from scipy.spatial.transform import Rotation
 import softlight_amd
 from softlight_amd import api as A
 from softlight_amd import scenes
+from softlight_amd.scene_files import draw_dump, run_dump
 
 BUILD = os.path.dirname(softlight_amd.LIB_PATH)
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -60,35 +61,6 @@ def dump_exe(tmp_path_factory):
                     os.path.join(ROOT, "include"), "-o", exe,
                     os.path.join(ROOT, "softlight_amd", "examples", "scene_graph_draw.cpp")], check=True)
     return exe
-
-
-def run_dump(exe, obj, cmds, tmp_path, check=True):
-    out = tmp_path / "dump"
-    out.mkdir(exist_ok=True)
-    cfile = tmp_path / "nodes.txt"
-    cfile.write_text(cmds)
-    r = subprocess.run([exe, "dump", str(obj), str(cfile), str(out)], capture_output=True, text=True, timeout=60)
-    if not check:
-        return r
-    assert r.returncode == 0, r.stderr
-    d = {"graph": [], "is_child": (out / "is_child.txt").read_text().split()}
-    for line in (out / "graph.txt").read_text().splitlines():
-        i, p, t, tot, imm, name = line.split(" ", 5)
-        d["graph"].append(dict(id=int(i), parent=int(p), type=int(t), total=int(tot), immediate=int(imm), name=name))
-    d["model"] = np.fromfile(out / "model.f32", dtype=np.float32).reshape(-1, 4, 4)    # [node][col][row]
-    d["vertices"] = np.fromfile(out / "vertices.f32", dtype=np.float32).reshape(-1, 8)
-    d["indices"] = np.fromfile(out / "indices.u32", dtype=np.uint32)
-    d["meshes"] = [tuple(int(x) for x in l.split()) for l in (out / "meshes.txt").read_text().splitlines()]
-    d["materials"] = [np.array(l.split(), dtype=np.float64) for l in (out / "materials.txt").read_text().splitlines()]
-    d["draws"] = [tuple(int(x) for x in l.split()) for l in (out / "draws.txt").read_text().splitlines()]
-    d["draw_meshes"] = [tuple(int(x) for x in l.split()) for l in (out / "draw_meshes.txt").read_text().splitlines()]
-    d["images"] = []
-    k = 0
-    while (out / f"image{k}.txt").exists():
-        w, h = (int(x) for x in (out / f"image{k}.txt").read_text().split())
-        d["images"].append(np.fromfile(out / f"image{k}.rgba", dtype=np.uint8).reshape(h, w, 4))
-        k += 1
-    return d
 
 
 def name_ids(d):
@@ -302,64 +274,6 @@ def test_child_queries(dump_exe, tmp_path):
 # ---------------------------------------------------------------------------
 # draw lists on the oracle (and the GPU)
 # ---------------------------------------------------------------------------
-def base_uniforms(W, H):
-    u = A.sl_std_uniforms()
-    u.mvp[:] = scenes.perspective(60.0, W / H, 0.1, 100.0).reshape(-1).tolist()
-    L = np.array([-0.4, -1.0, -0.6], dtype=np.float32)
-    L = L / np.float32(np.sqrt(np.float32((L[0] * L[0] + L[1] * L[1]) + L[2] * L[2])))
-    u.light_dir[:] = [float(L[0]), float(L[1]), float(L[2]), 0.0]
-    u.tint[:] = [1.0, 1.0, 1.0, 1.0]
-    u.instance_vbo = A.SL_NO_RESOURCE
-    return u
-
-
-def draw_dump(lib, d, W, H, base=None, per_mesh=False, vertices=None):
-    """Draws a dumped scene the way sl_submit_draw_list does (per item: the
-    node's model matrix, the material's colours and diffuse map, one
-    draw_multiple), or with per_mesh=True one draw per mesh in the same
-    order (the reference's loop)."""
-    ctx = A.SLContext(lib)
-    color, depth, fbo, _ = scenes.make_target(ctx, W, H, False)
-    white = ctx.create_texture(A.SL_COLOR_RGBA_8U, 4, 4, 1, np.full((4, 4, 4), 255, np.uint8))
-    texs = [ctx.create_texture(A.SL_COLOR_RGBA_8U, im.shape[1], im.shape[0], 1, im) for im in d["images"]]
-    verts = d["vertices"] if vertices is None else vertices
-    vbo = ctx.create_vbo(np.ascontiguousarray(verts))
-    idx = d["indices"]
-    n = idx.size
-    ib = idx.astype(np.uint8 if n <= 0xFF else np.uint16 if n <= 0xFFFF else np.uint32)
-    ibo = ctx.create_ibo(ib)
-    vao = ctx.create_vao(vbo, ibo, [(0, 32, A.VERTEX_DIMENSION_3, A.VERTEX_DATA_FLOAT),
-                                    (20, 32, A.VERTEX_DIMENSION_3, A.VERTEX_DATA_FLOAT),
-                                    (12, 32, A.VERTEX_DIMENSION_2, A.VERTEX_DATA_FLOAT)])
-    base = base_uniforms(W, H) if base is None else base
-    ubo = ctx.create_ubo(base)
-    vs = A.sl_vertex_shader(2, A.SL_CULL_BACK_FACE, A.SL_PROGRAM_LAMBERT_TEXTURED)
-    fs = A.sl_fragment_shader(2, 1, A.SL_BLEND_OFF, A.SL_DEPTH_TEST_LESS_THAN, A.SL_DEPTH_MASK_ON,
-                              A.SL_PROGRAM_LAMBERT_TEXTURED)
-    sh = ctx.create_shader(vs, fs, ubo)
-    ctx.clear_framebuffer(fbo, 0, (0.0, 0.0, 0.0, 0.0), 1.0)
-    for node, first, count, mat in d["draws"]:
-        u = A.sl_std_uniforms.from_buffer_copy(bytes(base))
-        u.model[:] = d["model"][node].reshape(-1).tolist()
-        m = d["materials"][mat].astype(np.float32)
-        u.ambient[:3] = m[0:3].tolist()
-        u.diffuse[:3] = m[4:7].tolist()
-        u.tint[3] = float(np.float32(base.tint[3]) * m[7])
-        u.texture = white if m[13] < 0 else texs[int(m[13])]
-        meshes = [A.sl_mesh(vao, b, e, mode, mi) for b, e, mi, mode in d["draw_meshes"][first:first + count]]
-        if per_mesh:
-            for mesh in meshes:
-                ctx.ubo_assign(ubo, u)
-                ctx.draw(mesh, sh, fbo)
-        else:
-            ctx.ubo_assign(ubo, u)
-            ctx.draw_multiple(meshes, sh, fbo)
-    ctx.finish()
-    out = {"color": ctx.texture_read(color), "depth": ctx.texture_read(depth)}
-    ctx.close()
-    return out
-
-
 def sphere_file(tmp_path, **kw):
     return scenes.write_icosphere_scene(str(tmp_path), **kw)[0]
 
