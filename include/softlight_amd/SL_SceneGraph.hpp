@@ -330,12 +330,12 @@ class SL_SceneGraph
     }
 
     /* SL_SceneGraph::node_is_child (SL_SceneGraph.cpp:1011-1045): whether
-     * parentId is an ancestor of nodeIndex (every node descends from the root). */
+     * parentId is an ancestor of nodeIndex.  As in the reference, the
+     * `parentId >= nodeIndex` rejection comes first, so the root id
+     * (the largest size_t) is never reported as an ancestor. */
     bool node_is_child(size_t nodeIndex, size_t parentId) const noexcept
     {
-        if (nodeIndex == SCENE_NODE_ROOT_ID || nodeIndex >= mNodes.size()) return false;
-        if (parentId == SCENE_NODE_ROOT_ID) return true;
-        if (parentId >= nodeIndex) return false;
+        if (nodeIndex == SCENE_NODE_ROOT_ID || parentId >= nodeIndex || nodeIndex >= mNodes.size()) return false;
         for (size_t p = mNodeParentIds[nodeIndex]; p != SCENE_NODE_ROOT_ID; p = mNodeParentIds[p])
         {
             if (p == parentId) return true;

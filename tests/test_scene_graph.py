@@ -268,7 +268,7 @@ def test_child_queries(dump_exe, tmp_path):
         assert g["immediate"] == sum(1 for j in range(n) if parent[j] == i), g
         row = d["is_child"][i]
         assert row[:n] == "".join("1" if j in ancestors(i) else "0" for j in range(n)), g
-        assert row[n] == "1"
+        assert row[n] == "0"            # the reference rejects parentId >= nodeIndex first, the root id included
 
 
 # ---------------------------------------------------------------------------
