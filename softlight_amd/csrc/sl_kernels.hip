@@ -1726,6 +1726,11 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     __shared__ uint32_t sPend[TH][TW];                  // pending fragment's record, or kNoPending
     __shared__ float sPendS[3][TH][TW];                 // its barycentric values
 
+    // a frame whose setup outgrew the bin capacities (a graph replay, whose
+    // buffers are fixed at capture) leaves its targets untouched rather than
+    // drawing without the records that did not fit; the host reports it
+    if (P.ctrl != nullptr && P.ctrl->overflow != 0u) return;
+
     // heaviest tiles first (the order only affects scheduling)
     const int tile = (P.tileOrder != nullptr) ? (int)P.tileOrder[blockIdx.x] : (int)blockIdx.x;
     const int tx = tile % P.tilesX, ty = tile / P.tilesX;
