@@ -281,6 +281,7 @@ __device__ int clip_polygon(const float4* inPos, const Vary4* inVar, int nv, Pol
     return n < 3 ? 0 : n;
 }
 
+template <bool PAIRS = true>
 __device__ __forceinline__ void write_record(const SetupParams& P, uint32_t recId, uint32_t pairOff,
                                              float4 p0, float4 p1, float4 p2, const Vary4& a, const Vary4& b,
                                              const Vary4& c, int nv, uint32_t draw, uint32_t seq, uint32_t state,
@@ -302,6 +303,7 @@ __device__ __forceinline__ void write_record(const SetupParams& P, uint32_t recI
             V[2 * nvMax + i] = c.v[i];
         }
     }
+    if (!PAIRS) return;
     uint32_t o = pairOff;
     for (int ty = tr.ty0; ty <= tr.ty1; ++ty)
         for (int tx = tr.tx0; tx <= tr.tx1; ++tx)
@@ -310,6 +312,76 @@ __device__ __forceinline__ void write_record(const SetupParams& P, uint32_t recI
             P.pairVals[o] = recId;
             ++o;
         }
+}
+
+// The (tile, record) entries of a wave's records, entry j of a record at
+// pairOff + j in row-major order over its tile rectangle (as write_record
+// writes them).  When some record of the wave covers more than kLanePairs
+// tiles and more than kShareRatio times the wave's mean, the wave writes all
+// of its records' entries together, 64 a step:
+// entry k of the wave's concatenated lists goes to the lane k % 64, which
+// finds its record by a binary search over the lanes' inclusive counts.
+// Called by every lane of the wave; `has` marks the lanes with a record.
+constexpr uint32_t kLanePairs = 8, kShareRatio = 6;
+
+__device__ __forceinline__ void write_pairs_wave(const SetupParams& P, bool has, uint32_t recId, uint32_t pairOff,
+                                                 const TileRect& tr)
+{
+    const int lane = threadIdx.x & 63;
+    const int wi = tr.tx1 - tr.tx0 + 1, hi = tr.ty1 - tr.ty0 + 1;
+    const bool any = has && wi > 0 && hi > 0;
+    const uint32_t w = any ? (uint32_t)wi : 1u;
+    const uint32_t n = any ? (uint32_t)wi * (uint32_t)hi : 0u;
+    uint32_t mx = n, inc = n;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        const uint32_t v = (uint32_t)__shfl_up((int)inc, o, 64);
+        if (lane >= o) inc += v;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)inc, 63, 64);
+    // a lane's own loop costs its count; the shared loop costs the wave's
+    // total / 64 steps of several times the work (the search and the
+    // shuffles): share only when one lane's list dominates the wave
+    if (mx <= kLanePairs || (uint64_t)mx * 64u <= (uint64_t)kShareRatio * total)
+    {
+        if (!any) return;
+        uint32_t o = pairOff;
+        for (int ty = tr.ty0; ty <= tr.ty1; ++ty)
+            for (int tx = tr.tx0; tx <= tr.tx1; ++tx, ++o)
+            {
+                P.pairKeys[o] = (uint32_t)(ty * P.tilesX + tx);
+                P.pairVals[o] = recId;
+            }
+        return;
+    }
+    const int tx0 = any ? tr.tx0 : 0, ty0 = any ? tr.ty0 : 0;
+    for (uint32_t base = 0; base < total; base += 64)
+    {
+        const uint32_t k = base + (uint32_t)lane;
+        // the first lane whose inclusive count exceeds k
+        int L = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+        {
+            const uint32_t v = (uint32_t)__shfl((int)inc, L + step - 1, 64);
+            if (v <= k) L += step;
+        }
+        const uint32_t incL = (uint32_t)__shfl((int)inc, L, 64);
+        const uint32_t nL = (uint32_t)__shfl((int)n, L, 64);
+        const uint32_t wL = (uint32_t)__shfl((int)w, L, 64);
+        const int tx0L = __shfl(tx0, L, 64), ty0L = __shfl(ty0, L, 64);
+        const uint32_t offL = (uint32_t)__shfl((int)pairOff, L, 64);
+        const uint32_t recL = (uint32_t)__shfl((int)recId, L, 64);
+        if (k < total)
+        {
+            const uint32_t j = k - (incL - nL);
+            P.pairKeys[offL + j] = (uint32_t)((ty0L + (int)(j / wL)) * P.tilesX + tx0L + (int)(j % wL));
+            P.pairVals[offL + j] = recL;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1055,10 +1127,11 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
             if (__syncthreads_and(found)) break;
             if (it >= maxWindows) break;   // unreachable with consistent prefixes
         }
+        bool bad = false;
         if (!found || chunk >= P.numChunks)
         {
             atomicExch(&P.ctrl->error, 1u);
-            continue;
+            bad = true;        // the lane goes on without a triangle, with the rest of its wave
         }
         TriIn T;
         T.status = 0;
@@ -1072,8 +1145,8 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
         uint64_t g = 0;
         uint32_t di = 0;
         unsigned long long off = 0;
-        bool act2 = act;
-        if (act)
+        bool act2 = act && !bad;
+        if (act2)
         {
             const ChunkInfo& I = P.chunkInfo[chunk];
             const uint32_t r = s - I.survPre;
@@ -1154,8 +1227,8 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
                 if (PRIMS)
                     write_prim_record(P, DR, recOff, (uint32_t)pairOff, T, C);
                 else
-                    write_record(P, recOff, (uint32_t)pairOff, C.sp[0], C.sp[1], C.sp[2], T.var[0], T.var[1], T.var[2],
-                                 DR.nvary, T.draw, T.seq, pack_raster_state(DR), C.tr);
+                    write_record<false>(P, recOff, (uint32_t)pairOff, C.sp[0], C.sp[1], C.sp[2], T.var[0], T.var[1], T.var[2],
+                                        DR.nvary, T.draw, T.seq, pack_raster_state(DR), C.tr);
             }
             else
             {
@@ -1164,8 +1237,8 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
                 if (PRIMS)
                     write_prim_record(P, D, recOff, (uint32_t)pairOff, T, C);
                 else
-                    write_record(P, recOff, (uint32_t)pairOff, C.sp[0], C.sp[1], C.sp[2], T.var[0], T.var[1], T.var[2],
-                                 D.nvary, T.draw, T.seq, pack_raster_state(D), C.tr);
+                    write_record<false>(P, recOff, (uint32_t)pairOff, C.sp[0], C.sp[1], C.sp[2], T.var[0], T.var[1], T.var[2],
+                                        D.nvary, T.draw, T.seq, pack_raster_state(D), C.tr);
             }
         }
         else if (wantJob)
@@ -1184,6 +1257,8 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
                 atomicExch(&P.ctrl->overflow, 1u);
             }
         }
+        // the records' bin entries, by the whole wave (every lane reaches this)
+        if (!PRIMS) write_pairs_wave(P, ok && !C.clipped, recOff, (uint32_t)pairOff, C.tr);
     }
 }
 
