@@ -32,6 +32,7 @@
 #ifndef SOFTLIGHT_AMD_SL_SCENE_FILE_LOADER_HPP
 #define SOFTLIGHT_AMD_SL_SCENE_FILE_LOADER_HPP
 
+#include <cctype>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -363,7 +364,7 @@ class SL_SceneFileLoader
             {
                 int ch = in.peek();
                 if (ch == '#') { std::string c; std::getline(in, c); }
-                else if (std::isspace(ch)) in.get();
+                else if (ch != EOF && std::isspace(ch)) in.get();
                 else break;
             }
         };
