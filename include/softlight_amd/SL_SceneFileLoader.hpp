@@ -99,6 +99,7 @@ class SL_SceneFileLoader
     {
         mData = SL_SceneFileData{};
         mError.clear();
+        mDefaultMaterial = -1;   // a material index of an earlier file means nothing in this one
         std::ifstream in(path);
         if (!in) return fail(SL_SCENE_LOAD_NO_FILE, "cannot open " + path);
         const std::string dir = dir_of(path);

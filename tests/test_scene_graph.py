@@ -148,6 +148,25 @@ def test_loader_missing_file(dump_exe, tmp_path):
     assert r.returncode == 3 and "cannot open" in r.stderr
 
 
+def test_loader_reused_for_a_second_file(tmp_path):
+    """One loader object loads a file whose faces without usemtl take a
+    default material appended after three named ones, then a file with no
+    mtllib at all: every mesh of the second load must name a material of the
+    second load's table (a default-material index kept from the first file
+    would point past it)."""
+    (tmp_path / "m.mtl").write_text("newmtl a\nKd 1 0 0\nnewmtl b\nKd 0 1 0\nnewmtl c\nKd 0 0 1\n")
+    (tmp_path / "one.obj").write_text("mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nv 1 1 0\n"
+                                      "f 2 4 3\nusemtl b\nf 1 2 3\n")
+    (tmp_path / "two.obj").write_text("v 0 0 0\nv 2 0 0\nv 0 2 0\nf 1 2 3\n")
+    exe = tmp_path / "loader_reuse"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), "-o", str(exe),
+                    os.path.join(HERE, "native", "loader_reuse.cpp")], check=True)
+    r = subprocess.run([str(exe), str(tmp_path / "one.obj"), str(tmp_path / "two.obj")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    first, second = map(int, r.stdout.split()[1:3])
+    assert first == 4 and second == 1, r.stdout
+
+
 # ---------------------------------------------------------------------------
 # transforms and the hierarchy
 # ---------------------------------------------------------------------------
