@@ -116,14 +116,16 @@ enum {
  *   VS: model = uniforms.model, or the instance's mat4 from uniforms.instance_vbo;
  *       world = model*(pos,1); n = model*(normal,0); clip = uniforms.mvp*world.
  *       varying0 = (u, v, 0, 0), varying1 = (n, 0).
- *   FS: albedo = bilinear REPEAT sample of an RGBA8 texture (SL_Sampler.hpp:186-218);
+ *   FS: albedo = bilinear REPEAT sample of an RGBA8 texture (SL_Sampler.hpp:186-218;
+ *       nearest with SL_UNIFORM_SAMPLE_NEAREST);
  *       lambert = max(0, -dot(normalize(n), light_dir));
  *       rgb = min(albedo.rgb * (ambient + diffuse*lambert) * tint.rgb, 1);
  *       a = albedo.a * tint.a.  (Pattern of tests/sl_instancing_test.cpp:105-178.)
  * SL_PROGRAM_COLOR_TEXTURED
  *   VBO vertex = {float pos[3]; float rgb[3]; float uv[2]}.
  *   VS: clip = uniforms.mvp*(model*(pos,1)); varying0 = (rgb, 1); varying1 = (u, v, 0, 0).
- *   FS: output0 = varying0 * bilinear REPEAT sample of an RGBA32F texture.
+ *   FS: output0 = varying0 * bilinear REPEAT sample of an RGBA32F texture
+ *       (nearest with SL_UNIFORM_SAMPLE_NEAREST).
  *       This is the program of the differentiable mode.
  * SL_PROGRAM_LAMBERT_MRT
  *   SL_PROGRAM_LAMBERT_TEXTURED with the G-buffer outputs of the reference's
@@ -141,6 +143,13 @@ enum {
     SL_PROGRAM_LAMBERT_MRT = 4
 };
 
+/* sl_std_uniforms::flags.  SL_UNIFORM_SAMPLE_NEAREST: the textured programs
+ * sample with sl_sample_nearest<..., SL_WrapModeRepeat> (SL_Sampler.hpp:
+ * 122-148: the texel at (uint)(W * wrap(u)), (uint)(H * wrap(v)) in ordered
+ * layout) instead of bilinear filtering; the backward pass then sends each
+ * fragment's texel gradient to that one texel. */
+#define SL_UNIFORM_SAMPLE_NEAREST 0x1u
+
 /* POD uniform block read by the built-in programs (lives in an SL_UniformBuffer,
  * SL_UniformBuffer.hpp:12; matrices column-major like ls::math::mat4). */
 typedef struct sl_std_uniforms {
@@ -152,7 +161,7 @@ typedef struct sl_std_uniforms {
     float tint[4];
     uint32_t texture;       /* texture handle or SL_NO_RESOURCE */
     uint32_t instance_vbo;  /* vbo holding one mat4 per instance, or SL_NO_RESOURCE */
-    uint32_t flags;
+    uint32_t flags;         /* SL_UNIFORM_* bits */
     uint32_t reserved;
 } sl_std_uniforms;
 

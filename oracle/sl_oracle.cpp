@@ -487,6 +487,27 @@ void sample_bilinear_rgba8(const Tex& t, float u, float v, uint8_t out[4])
     }
 }
 
+// sl_sample_nearest<color, REPEAT> (SL_Sampler.hpp:122-148): the texel at
+// xi = (uint32_t)(W * wrap(u)), yi = (uint32_t)(H * wrap(v)), ordered layout
+// xi + W * yi.  wrap(u) can be 1 (a tiny negative u rounds up), so xi = W
+// reads texel 0 of the next row and yi = H the zeroed padding.
+size_t nearest_index(const Tex& t, float u, float v)
+{
+    const uint32_t xi = (uint32_t)f2i((float)t.w * wrap_repeat(u));
+    const uint32_t yi = (uint32_t)f2i((float)t.h * wrap_repeat(v));
+    return (size_t)xi + (size_t)t.w * yi;
+}
+
+void sample_nearest_rgba8(const Tex& t, float u, float v, uint8_t out[4])
+{
+    std::memcpy(out, t.data.data() + 4 * nearest_index(t, u, v), 4);
+}
+
+void sample_nearest_rgbaf(const Tex& t, float u, float v, float out[4])
+{
+    std::memcpy(out, t.data.data() + 16 * nearest_index(t, u, v), 16);
+}
+
 // Same sampler for an RGBA32F texture: color_cast<float,float> is the identity.
 void sample_bilinear_rgbaf(const Tex& t, float u, float v, float out[4])
 {
@@ -628,7 +649,12 @@ bool run_fragment_program(const Draw& d, const V4* var, V4* outs)
             const sl_std_uniforms* u = uniforms(d);
             uint8_t texel[4] = {0, 0, 0, 0};
             if (u->texture < d.ctx->texs.size() && d.ctx->texs[u->texture].type == SL_COLOR_RGBA_8U)
-                sample_bilinear_rgba8(d.ctx->texs[u->texture], var[0].v[0], var[0].v[1], texel);
+            {
+                if (u->flags & SL_UNIFORM_SAMPLE_NEAREST)
+                    sample_nearest_rgba8(d.ctx->texs[u->texture], var[0].v[0], var[0].v[1], texel);
+                else
+                    sample_bilinear_rgba8(d.ctx->texs[u->texture], var[0].v[0], var[0].v[1], texel);
+            }
             float albedo[4];
             for (int c = 0; c < 4; ++c) albedo[c] = (float)texel[c] * 0.00392156862745f;
             const float nx = var[1].v[0], ny = var[1].v[1], nz = var[1].v[2];
@@ -659,7 +685,12 @@ bool run_fragment_program(const Draw& d, const V4* var, V4* outs)
             const sl_std_uniforms* u = uniforms(d);
             float t[4] = {1.f, 1.f, 1.f, 1.f};
             if (u->texture < d.ctx->texs.size() && d.ctx->texs[u->texture].type == SL_COLOR_RGBA_FLOAT)
-                sample_bilinear_rgbaf(d.ctx->texs[u->texture], var[1].v[0], var[1].v[1], t);
+            {
+                if (u->flags & SL_UNIFORM_SAMPLE_NEAREST)
+                    sample_nearest_rgbaf(d.ctx->texs[u->texture], var[1].v[0], var[1].v[1], t);
+                else
+                    sample_bilinear_rgbaf(d.ctx->texs[u->texture], var[1].v[0], var[1].v[1], t);
+            }
             for (int c = 0; c < 4; ++c) out.v[c] = var[0].v[c] * t[c];
             return true;
         }
@@ -1927,10 +1958,24 @@ int backward(sl_context* c, size_t fboId, size_t gradImageTex)
             T = &c->texs[sp.tex];
             const float u = ((uv[0][0] * b[0]) + (uv[1][0] * b[1])) + (uv[2][0] * b[2]);
             const float v = ((uv[0][1] * b[0]) + (uv[1][1] * b[1])) + (uv[2][1] * b[2]);
-            sample_bilinear_rgbaf(*T, u, v, tex);
-            tp = bilinear_taps(*T, u, v);
-            dy = wrap_repeat(v) * (float)T->h;
-            dy = dy - (float)to_u16(dy);
+            sl_std_uniforms su;
+            std::memset(&su, 0, sizeof(su));
+            if (sp.uni.size() >= sizeof(su)) std::memcpy(&su, sp.uni.data(), sizeof(su));
+            if (su.flags & SL_UNIFORM_SAMPLE_NEAREST)
+            {
+                // one tap of weight 1 (SL_Sampler.hpp:122-148)
+                sample_nearest_rgbaf(*T, u, v, tex);
+                tp.i[0] = tp.i[1] = tp.i[2] = tp.i[3] = nearest_index(*T, u, v);
+                tp.w[0] = 1.f; tp.w[1] = tp.w[2] = tp.w[3] = 0.f;
+                dy = 0.f;
+            }
+            else
+            {
+                sample_bilinear_rgbaf(*T, u, v, tex);
+                tp = bilinear_taps(*T, u, v);
+                dy = wrap_repeat(v) * (float)T->h;
+                dy = dy - (float)to_u16(dy);
+            }
         }
         // colour attribute gradients
         const Vbo& V = c->vbos[c->vaos[sp.mesh.vaoId].vbo];

@@ -50,6 +50,7 @@ SL_PROGRAM_PASSTHROUGH_GOURAUD = 1
 SL_PROGRAM_LAMBERT_TEXTURED = 2
 SL_PROGRAM_COLOR_TEXTURED = 3
 SL_PROGRAM_LAMBERT_MRT = 4
+SL_UNIFORM_SAMPLE_NEAREST = 0x1      # sl_std_uniforms.flags: nearest instead of bilinear sampling
 
 _TEXEL_DTYPE = {SL_COLOR_R_8U: (np.uint8, 1), SL_COLOR_RG_8U: (np.uint8, 2), SL_COLOR_RGB_8U: (np.uint8, 3),
                 SL_COLOR_RGBA_8U: (np.uint8, 4), SL_COLOR_R_32U: (np.uint32, 1), SL_COLOR_R_HALF: (np.float16, 1),

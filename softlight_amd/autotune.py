@@ -53,6 +53,19 @@ def lookup(signature: str, path: str = DEFAULT_CACHE):
     return None if entry is None else int(entry["config"])
 
 
+def cached_config(signature: str, path: str = DEFAULT_CACHE):
+    """The cached configuration of `signature` for this device, else the one
+    cached for any device (the cache may have been written on a box whose HIP
+    reported another name), else None."""
+    own = lookup(signature, path)
+    if own is not None:
+        return own
+    for entries in load_cache(path).values():
+        if signature in entries:
+            return int(entries[signature]["config"])
+    return None
+
+
 def tune(ctx, render_frame, signature: str, frames: int = 3, path: str | None = DEFAULT_CACHE) -> dict:
     """Time render_frame() under each raster configuration; return the
     per-configuration raster milliseconds and the best index (also cached)."""

@@ -169,13 +169,22 @@ __global__ __launch_bounds__(256) void k_backward(BackwardParams P)
         {
             const float u = ((uv[0].x * b[0]) + (uv[1].x * b[1])) + (uv[2].x * b[2]);
             const float v = ((uv[0].y * b[0]) + (uv[1].y * b[1])) + (uv[2].y * b[2]);
-            taps = bilinear_taps(D.texW, D.texH, u, v);
-            const auto px = gptr(reinterpret_cast<const float*>(D.tex));
-            float o[4];
+            if (D.nearest)
+            {
+                // one tap of weight 1 (SL_Sampler.hpp:122-148)
+                taps = nearest_taps(D.texW, D.texH, u, v);
+                tex = sample_nearest_rgbaf(D.tex, D.texW, D.texH, u, v);
+            }
+            else
+            {
+                taps = bilinear_taps(D.texW, D.texH, u, v);
+                const auto px = gptr(reinterpret_cast<const float*>(D.tex));
+                float o[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c)
-                o[c] = bilerp(taps, px[4 * taps.i0 + c], px[4 * taps.i1 + c], px[4 * taps.i2 + c], px[4 * taps.i3 + c]);
-            tex = f4(o[0], o[1], o[2], o[3]);
+                for (int c = 0; c < 4; ++c)
+                    o[c] = bilerp(taps, px[4 * taps.i0 + c], px[4 * taps.i1 + c], px[4 * taps.i2 + c], px[4 * taps.i3 + c]);
+                tex = f4(o[0], o[1], o[2], o[3]);
+            }
             textured = true;
         }
     }
@@ -204,9 +213,11 @@ __global__ __launch_bounds__(256) void k_backward(BackwardParams P)
         const float gt[4] = {g.x * col.x, g.y * col.y, g.z * col.z, g.w * col.w};
         const uint32_t ti[4] = {taps.i0, taps.i1, taps.i2, taps.i3};
         const float tw[4] = {taps.omdx * taps.omdy, taps.omdx * taps.dy, taps.dx * taps.omdy, taps.dx * taps.dy};
+        const int ntaps = G->d.nearest ? 1 : 4;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
         {
+            if (j >= ntaps) break;
             if (ti[j] >= G->gradTexTexels) continue;
             const float v[4] = {tw[j] * gt[0], tw[j] * gt[1], tw[j] * gt[2], tw[j] * gt[3]};
             scatter<4>(tt, useTable, G->gradTex, ti[j], v);

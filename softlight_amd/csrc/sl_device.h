@@ -185,6 +185,8 @@ struct DrawDesc
     uint32_t visSlot;          // varying slot carrying the vertex weights, or kNoVisSlot
     uint32_t prim;             // primitive kind (kPrimFill, kPrimWire, ...)
     uint8_t  indexed, cull, dtest, dmask, blend, nvary, nout, hasUniforms;
+    uint8_t  nearest;          // sample with sl_sample_nearest (SL_UNIFORM_SAMPLE_NEAREST)
+    uint8_t  pad8[3];
     BindingDesc bind[SL_VAO_MAX_BINDINGS];
     float scissor[16];         // SL_ViewportState::scissor_matrix at draw time
     float vp[4];               // SL_ViewportState::viewport_rect at draw time
@@ -304,7 +306,11 @@ struct FrameCtrl
     uint32_t overflow;         // capacity exceeded (records, pairs or clip jobs)
     uint32_t error;            // the emitting pass found a survivor that yields no record
     uint32_t emptyRecs;        // records that passed push_bin's tests but cover no pixel (not binned)
-    uint32_t pad;
+    // overflow | error << 1 of every earlier frame since the host last reset
+    // it: the per-frame reset (k_ctrl_reset) folds them in here before it
+    // clears them, so a replayed graph frame that overflowed is still
+    // reported after later replays succeed
+    uint32_t sticky;
     // Fragment counters, spread over kCounterSlots cache lines (a workgroup
     // adds to slot blockIdx % kCounterSlots): tens of thousands of atomics on
     // one address serialise, on 64 lines they do not.  Readers sum the slots.

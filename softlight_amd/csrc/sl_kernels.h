@@ -7,6 +7,9 @@ namespace slg {
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, uint64_t n, hipStream_t s);
+// Clears the control block before a frame, keeping (and adding the last
+// frame's overflow / error flags to) FrameCtrl::sticky.
+hipError_t launch_ctrl_reset(FrameCtrl* ctrl, hipStream_t s);
 hipError_t launch_setup(const SetupParams& p, uint64_t chunks, hipStream_t s);
 hipError_t launch_chunk_scan(const SetupParams& p, hipStream_t s);
 // tris / prims: the batch holds triangle draws / point or line draws.
@@ -22,6 +25,11 @@ hipError_t launch_binning(const uint32_t* keys, const uint32_t* vals, const Fram
                           BinPlan plan, uint32_t* counts, uint32_t* totals, uint2* range, uint32_t* out,
                           uint32_t* order, int numTiles, hipStream_t s);
 hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s);
+// k_raster of one colour format (CF: 0 none, 1 8-bit/packed, 2 float) and
+// variant (AUX bits, see k_raster) on tile configuration `cfg`; defined in
+// sl_raster_impl.h and instantiated in the sl_raster_*.hip units.
+template <int CF, int AUX>
+void launch_raster_cfg(const RasterParams& p, int numTiles, int cfg, hipStream_t s);
 hipError_t launch_blit(const BlitParams& p, int srcType, int dstType, hipStream_t s);
 
 }  // namespace slg

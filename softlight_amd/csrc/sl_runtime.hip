@@ -522,7 +522,7 @@ int enqueue_front(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, hip
 {
     int rc;
     FrameCtrl* ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
-    if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(ctrl), 0u, sizeof(FrameCtrl) / 4, s), "fill ctrl")) != SL_OK) return rc;
+    if ((rc = check(c, launch_ctrl_reset(ctrl, s), "reset ctrl")) != SL_OK) return rc;
     if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(R.ranges.p), 0u, 2ull * p.numTiles, s), "fill ranges")) != SL_OK) return rc;
     if (B.totalTris == 0) return SL_OK;
     const SetupParams sp = setup_params(p, B, R);
@@ -743,6 +743,12 @@ int graph_capture(sl_context* c)
     G.res.clipCap = grow(c->lastTotals[2]);
     if ((rc = ensure_frame(c, G.res, p, B)) != SL_OK) { graph_release(c); return rc; }
     if ((rc = upload_batch(c, G.res, B, s)) != SL_OK) { graph_release(c); return rc; }
+    // the replays' reset keeps FrameCtrl::sticky: start it (and the rest) at zero
+    if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(G.res.ctrl.p), 0u, sizeof(FrameCtrl) / 4, s), "fill ctrl")) != SL_OK)
+    {
+        graph_release(c);
+        return rc;
+    }
     if (hipHostMalloc(reinterpret_cast<void**>(&G.hCtrl), sizeof(FrameCtrl), hipHostMallocDefault) != hipSuccess)
     {
         graph_release(c);
@@ -817,8 +823,20 @@ int graph_collect(sl_context* c)
         }
     }
     G.pending = 0;
-    if (G.hCtrl->error) return fail(c, SL_ERR_DEVICE, "graph frame: setup emitting pass disagrees with the counting pass");
-    if (G.hCtrl->overflow) return fail(c, SL_ERR_CAPACITY, "graph frame exceeded its captured capacity; capture again");
+    // the last replay's flags, and those of every earlier replay since the
+    // last collection (folded into `sticky` by each replay's reset)
+    const bool error = G.hCtrl->error || (G.hCtrl->sticky & 2u);
+    const bool overflow = G.hCtrl->overflow || (G.hCtrl->sticky & 1u);
+    if (G.hCtrl->sticky)
+    {
+        FrameCtrl* dctrl = reinterpret_cast<FrameCtrl*>(G.res.ctrl.p);
+        rc = check(c, launch_fill_u32(&dctrl->sticky, 0u, 1, c->stream), "reset sticky flags");
+        if (rc == SL_OK) rc = check(c, hipStreamSynchronize(c->stream), "sync");
+        G.hCtrl->sticky = 0;
+        if (rc != SL_OK) return rc;
+    }
+    if (error) return fail(c, SL_ERR_DEVICE, "graph frame: setup emitting pass disagrees with the counting pass");
+    if (overflow) return fail(c, SL_ERR_CAPACITY, "a graph frame exceeded its captured capacity (its targets were left untouched); capture again");
     return SL_OK;
 }
 
@@ -946,6 +964,7 @@ int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t s
         d.hasUniforms = 1;
         sl_std_uniforms u;
         std::memcpy(&u, c->ubos[sh.ubo].bytes, sizeof(u));
+        d.nearest = (u.flags & SL_UNIFORM_SAMPLE_NEAREST) ? 1 : 0;
         if (u.texture != SL_NO_RESOURCE && u.texture < c->texs.size() && c->texs[u.texture].valid)
         {
             const Tex& t = c->texs[u.texture];

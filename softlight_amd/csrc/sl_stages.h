@@ -470,6 +470,37 @@ __device__ __forceinline__ uint32_t sample_bilinear_rgba8(const uint8_t* tex, ui
     return out;
 }
 
+// sl_sample_nearest<..., REPEAT> (SL_Sampler.hpp:122-148): the texel at
+// (uint)(W * wrap(u)), (uint)(H * wrap(v)); wrap(u) = 1 (a tiny negative u)
+// reads texel 0 of the next row, or the zeroed padding below the last row.
+__device__ __forceinline__ uint32_t nearest_index(uint32_t W, uint32_t H, float u, float v)
+{
+    const uint32_t xi = (uint32_t)f2i((float)W * wrap_repeat(u));
+    const uint32_t yi = (uint32_t)f2i((float)H * wrap_repeat(v));
+    return xi + W * yi;
+}
+
+__device__ __forceinline__ uint32_t sample_nearest_rgba8(const uint8_t* tex, uint32_t W, uint32_t H, float u, float v)
+{
+    return gptr(reinterpret_cast<const uint32_t*>(tex))[nearest_index(W, H, u, v)];
+}
+
+__device__ __forceinline__ float4 sample_nearest_rgbaf(const uint8_t* tex, uint32_t W, uint32_t H, float u, float v)
+{
+    const auto px = gptr(reinterpret_cast<const float*>(tex)) + 4u * nearest_index(W, H, u, v);
+    return f4(px[0], px[1], px[2], px[3]);
+}
+
+// The nearest sample as bilinear taps (one texel, weight 1) for the adjoint.
+__device__ __forceinline__ BilinearTaps nearest_taps(uint32_t W, uint32_t H, float u, float v)
+{
+    BilinearTaps t;
+    t.i0 = t.i1 = t.i2 = t.i3 = nearest_index(W, H, u, v);
+    t.dx = t.dy = 0.f;
+    t.omdx = t.omdy = 1.f;
+    return t;
+}
+
 __device__ __forceinline__ float4 sample_bilinear_rgbaf(const uint8_t* tex, uint32_t W, uint32_t H, float u, float v)
 {
     const BilinearTaps t = bilinear_taps(W, H, u, v);
@@ -509,7 +540,8 @@ __device__ __forceinline__ bool fragment_program(const DrawDesc& d, const uint8_
         const sl_std_uniforms* u = reinterpret_cast<const sl_std_uniforms*>(uniArena + d.uniOffset);
         uint32_t texel = 0;
         if (d.tex != nullptr && d.texType == SL_COLOR_RGBA_8U)
-            texel = sample_bilinear_rgba8(d.tex, d.texW, d.texH, var.v[0].x, var.v[0].y);
+            texel = d.nearest ? sample_nearest_rgba8(d.tex, d.texW, d.texH, var.v[0].x, var.v[0].y)
+                              : sample_bilinear_rgba8(d.tex, d.texW, d.texH, var.v[0].x, var.v[0].y);
         const float a0 = (float)(texel & 0xFFu) * 0.00392156862745f;
         const float a1 = (float)((texel >> 8) & 0xFFu) * 0.00392156862745f;
         const float a2 = (float)((texel >> 16) & 0xFFu) * 0.00392156862745f;
@@ -543,7 +575,8 @@ __device__ __forceinline__ bool fragment_program(const DrawDesc& d, const uint8_
     {
         float4 t = f4(1.f, 1.f, 1.f, 1.f);
         if (d.tex != nullptr && d.texType == SL_COLOR_RGBA_FLOAT)
-            t = sample_bilinear_rgbaf(d.tex, d.texW, d.texH, var.v[1].x, var.v[1].y);
+            t = d.nearest ? sample_nearest_rgbaf(d.tex, d.texW, d.texH, var.v[1].x, var.v[1].y)
+                          : sample_bilinear_rgbaf(d.tex, d.texW, d.texH, var.v[1].x, var.v[1].y);
         out = f4(var.v[0].x * t.x, var.v[0].y * t.y, var.v[0].z * t.z, var.v[0].w * t.w);
         return true;
     }

@@ -472,7 +472,7 @@ class DiffScene:
     gradients go to an RGBA32F texture and a float4-per-vertex VBO."""
 
     def __init__(self, ctx: A.SLContext, seed=4, level=5, views=32, size=512, views_per_fbo=8, tex_size=512,
-                 distance=3.0, fov=60.0, edge_views=0):
+                 distance=3.0, fov=60.0, edge_views=0, flags=0):
         self.ctx = ctx
         self.seed, self.level, self.views, self.size = seed, level, views, size
         self.tex_size = tex_size
@@ -523,6 +523,7 @@ class DiffScene:
             u.tint[:] = [1.0, 1.0, 1.0, 1.0]
             u.texture = self.tex
             u.instance_vbo = A.SL_NO_RESOURCE
+            u.flags = flags
             ubo = ctx.create_ubo(u)
             vs = A.sl_vertex_shader(2, A.SL_CULL_BACK_FACE, A.SL_PROGRAM_COLOR_TEXTURED)
             fs = A.sl_fragment_shader(2, 1, A.SL_BLEND_OFF, A.SL_DEPTH_TEST_LESS_THAN, A.SL_DEPTH_MASK_ON,

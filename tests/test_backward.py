@@ -204,6 +204,25 @@ def test_oracle_adjoint_matches_finite_differences(oracle_lib):
     assert err.max() <= 1e-3, (err.max(), params[int(err.argmax())])
 
 
+def test_oracle_nearest_adjoint_matches_finite_differences(oracle_lib):
+    """With SL_UNIFORM_SAMPLE_NEAREST each fragment reads one texel, whose
+    gradient takes the whole g (.) colour: the scalar adjoint against central
+    differences on colour and texel parameters."""
+    ctx = A.SLContext(oracle_lib)
+    s = _scene(ctx, level=3, views=4, size=64, views_per_fbo=4, edge_views=2, flags=A.SL_UNIFORM_SAMPLE_NEAREST)
+    params = _pick_params(s, 40, 60, seed=2)
+    fd, ad = _fd_check(s, params)
+    err = _rel_err(fd, ad)
+    assert err.max() <= 1e-3, (err.max(), params[int(err.argmax())])
+    # each visible texel's gradient collects whole fragments: no bilinear
+    # spreading to neighbours, so far fewer texels receive a gradient
+    _, gt_near = s.gradients()
+    b = _scene(A.SLContext(oracle_lib), level=3, views=4, size=64, views_per_fbo=4, edge_views=2)
+    b.forward(); b.loss_grad_images(); b.backward()
+    _, gt_bil = b.gradients()
+    assert 0 < np.count_nonzero(gt_near) < np.count_nonzero(gt_bil)
+
+
 def test_oracle_directional_derivative(oracle_lib):
     ctx = A.SLContext(oracle_lib)
     s = _scene(ctx, level=3, views=2, size=64, views_per_fbo=2, edge_views=1)
@@ -298,8 +317,10 @@ def _grads(s):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("kw", [dict(level=4, views=4, size=128, views_per_fbo=4, tex_size=64, edge_views=1),
-                                dict(level=5, views=8, size=256, views_per_fbo=8, tex_size=512, edge_views=2)],
-                         ids=["level4_4x128", "level5_8x256"])
+                                dict(level=5, views=8, size=256, views_per_fbo=8, tex_size=512, edge_views=2),
+                                dict(level=4, views=4, size=128, views_per_fbo=4, tex_size=64, edge_views=1,
+                                     flags=A.SL_UNIFORM_SAMPLE_NEAREST)],
+                         ids=["level4_4x128", "level5_8x256", "nearest_level4_4x128"])
 def test_gpu_visibility_and_gradients_match_oracle(oracle_lib, gpu_lib, kw):
     o, g = _pair(oracle_lib, gpu_lib, **kw)
     vo, (gco, gto) = _grads(o)

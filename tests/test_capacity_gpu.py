@@ -50,3 +50,58 @@ def test_frame_outgrowing_the_bin_capacity(gpu_lib, oracle_lib):
     assert g0["stats"]["triangles_binned"] > (1 << 18)
     assert_parity(g0, o0)
     assert_parity(g1, o1)
+
+
+def _triangle_records(tris, W, H, seed=5):
+    """Vertex records of triangles given by integer pixel corners."""
+    rng = np.random.default_rng(seed)
+    pts = np.asarray(tris, dtype=np.int64).reshape(-1, 2)
+    ndcx = np.array([ndc_for_pixel(int(v), W) for v in range(W + 1)], dtype=np.float32)
+    ndcy = np.array([ndc_for_pixel(int(v), H) for v in range(H + 1)], dtype=np.float32)
+    rec = np.zeros((pts.shape[0], 16), dtype=np.uint8)
+    pos = np.stack([ndcx[pts[:, 0]], ndcy[pts[:, 1]], np.zeros(pts.shape[0], np.float32)], axis=1).astype(np.float32)
+    rec[:, 0:12] = pos.view(np.uint8).reshape(-1, 12)
+    rec[:, 12:16] = rng.integers(0, 256, size=(pts.shape[0], 4), dtype=np.uint8)
+    return rec
+
+
+def test_graph_replay_overflow_is_reported_after_later_replays(gpu_lib, oracle_lib):
+    """A captured frame whose geometry grows past the captured bin capacity
+    (sl_vbo_assign between replays) skips that replay; a later replay with
+    the original geometry succeeds, and the next synchronisation still
+    reports SL_ERR_CAPACITY for the skipped one (once)."""
+    W = H = 256
+    n = 1000
+    rng = np.random.default_rng(11)
+    x = rng.integers(0, W - 3, size=n)
+    y = rng.integers(0, H - 3, size=n)
+    small = [[(a, b), (a + 3, b), (a, b + 3)] for a, b in zip(x.tolist(), y.tolist())]
+    large = [[(0, 0), (W, 0), (0, H)]] * n
+    rec_small = _triangle_records(small, W, H)
+    rec_large = _triangle_records(large, W, H)
+    ctx = A.SLContext(gpu_lib)
+    s = scenes.build_from_vertex_records(ctx, rec_small, W, H, debug=True,
+                                         shader_kw=dict(cull=A.SL_CULL_OFF, depth_test=A.SL_DEPTH_TEST_OFF))
+    vbo = 0
+    s.render()
+    ctx.frame_graph_capture()
+    ctx.finish()
+    expect = s.read()
+    ctx.vbo_assign(vbo, rec_large)
+    ctx.frame_graph_launch()                  # overflows: bin entries of 1000 screen-sized triangles
+    ctx.vbo_assign(vbo, rec_small)            # (waits for the replay before writing)
+    ctx.frame_graph_launch()                  # fits again
+    with pytest.raises(A.SLError, match=r"\(-105\)"):
+        ctx.finish()
+    ctx.finish()                              # reported once
+    got = s.read()
+    for k in ("color", "depth", "last", "count"):
+        assert np.array_equal(got[k], expect[k]), k
+    o = A.SLContext(oracle_lib)
+    so = scenes.build_from_vertex_records(o, rec_small, W, H, debug=True,
+                                          shader_kw=dict(cull=A.SL_CULL_OFF, depth_test=A.SL_DEPTH_TEST_OFF))
+    so.render()
+    oo = so.read()
+    assert np.array_equal(got["last"], oo["last"])
+    assert np.array_equal(got["depth"].view(np.uint32), oo["depth"].view(np.uint32))
+    ctx.close()

@@ -22,6 +22,23 @@ def test_random_triangles_640x480(gpu_lib, oracle_lib):
     assert rep["fragments_tested"][0] > 90_000_000
 
 
+@pytest.mark.parametrize("scissor,viewport", [((100, 50, 300, 250), None), ((-20, 30, 400, 500), (40, 20, 500, 400)),
+                                               (None, (64, 48, 256, 192))])
+def test_random_triangles_scissor_and_viewport(gpu_lib, oracle_lib, scissor, viewport):
+    """Non-default scissor rectangles and viewports (SL_ViewportState.cpp:
+    17-111): the scissor matrix, the clip against it and the viewport mapping
+    bit-exact against the oracle."""
+    def build(ctx, **kw):
+        if scissor is not None:
+            ctx.set_scissor(*scissor)
+        if viewport is not None:
+            ctx.set_viewport(*viewport)
+        return scenes.build_random_triangles(ctx, **kw)
+    g, o = render_both(gpu_lib, oracle_lib, build, seed=12, ntri=1500, W=640, H=480, debug=True)
+    rep = assert_parity(g, o)
+    assert rep["fragments_tested"][0] > 0
+
+
 def test_spheres_reduced(gpu_lib, oracle_lib):
     g, o = render_both(gpu_lib, oracle_lib, scenes.build_sphere_scene, seed=1, n_instances=8, stacks=48, sectors=48,
                        W=480, H=270, tex_size=128, debug=True)
@@ -74,6 +91,78 @@ def test_batched_meshes_1080p(gpu_lib, oracle_lib):
     """The full batched frame: 1000 meshes (log-normal triangle counts) at 1080p."""
     g, o = render_both(gpu_lib, oracle_lib, scenes.build_batched_scene, debug=True)
     assert_parity(g, o)
+
+
+# ---------------------------------------------------------------------------
+# The raster kernels the benchmark times: no debug attachments (so the
+# non-debug k_raster instances run) and the tuned tile of each workload.
+# ---------------------------------------------------------------------------
+TUNED = {
+    "random": ("random:640x480:4096", scenes.build_random_triangles, dict(seed=0, ntri=4096, W=640, H=480)),
+    "spheres": ("spheres:1920x1080:8355840", scenes.build_sphere_scene, dict(seed=1)),
+    "batched": ("batched:1920x1080:3399137", scenes.build_batched_scene, dict(seed=3, n_meshes=1000)),
+}
+
+
+def _tuned_builder(name):
+    from softlight_amd import autotune
+    signature, build, kw = TUNED[name]
+    cfg = autotune.cached_config(signature)
+    assert cfg is not None, f"no tuned raster configuration cached for {signature}"
+
+    def builder(ctx, **extra):
+        ctx.set_raster_config(cfg)
+        return build(ctx, **kw, **extra)
+    return builder, cfg
+
+
+@pytest.mark.parametrize("name", ["random", "spheres", "batched"])
+def test_tuned_configuration_without_debug_buffers(gpu_lib, oracle_lib, name):
+    """The benchmark's frames (C1 4096 random triangles at 640x480, C2 the
+    1080p sphere frame, C3 the 1000-mesh batched frame) on their tuned raster
+    tiles and without parity attachments: depth bit-exact, RGBA8 within 1 LSB,
+    counters equal."""
+    builder, cfg = _tuned_builder(name)
+    g, o = render_both(gpu_lib, oracle_lib, builder, debug=False)
+    assert "last" not in g
+    assert g["scene"].ctx.raster_config()[0] == cfg
+    rep = assert_parity(g, o)
+    assert rep["fragments_shaded"][0] > 0
+
+
+def test_batched_graph_replays_match_oracle(gpu_lib, oracle_lib):
+    """C3 as the benchmark runs it: the 1000-mesh frame captured as a HIP
+    graph on its tuned tile and replayed three times; before each replay the
+    targets are overwritten with a junk pattern, and every replay's image
+    equals the oracle's frame (depth bit-exact, RGBA8 within 1 LSB, counters
+    equal per frame)."""
+    from softlight_amd import api
+
+    builder, _ = _tuned_builder("batched")
+    ctx_o = api.SLContext(oracle_lib)
+    so = builder(ctx_o, debug=False)
+    ctx_o.reset_stats()
+    so.render()
+    o = so.read()
+    o["stats"] = ctx_o.stats()
+
+    ctx = api.SLContext(gpu_lib)
+    s = builder(ctx, debug=False)
+    s.render()
+    ctx.frame_graph_capture()
+    ctx.finish()
+    junk_c = np.full((s.H, s.W, 4), 0x5A, dtype=np.uint8)
+    junk_d = np.full((s.H, s.W), -0.25, dtype=np.float32)
+    for _ in range(3):
+        ctx.texture_set_texels(s.color, junk_c)
+        ctx.texture_set_texels(s.depth, junk_d)
+        ctx.reset_stats()
+        ctx.frame_graph_launch()
+        ctx.finish()
+        g = s.read()
+        g["stats"] = ctx.stats()
+        assert g["stats"]["frames"] == 1
+        assert_parity(g, o)
 
 
 def test_frame_graph_replays_match(gpu_lib, oracle_lib):
