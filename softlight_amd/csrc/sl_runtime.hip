@@ -786,7 +786,10 @@ int graph_capture(sl_context* c)
             return fail(c, SL_ERR_DEVICE, rc3 != SL_OK ? "graph capture" : "graph end capture / instantiate");
         }
     }
-    G.tris = B.totalTris;
+    // submitted triangles (B.totalTris also counts the padding that starts
+    // every draw on a setup-chunk boundary)
+    G.tris = 0;
+    for (const DrawDesc& d : B.draws) G.tris += d.numTris;
     G.fbo = B.fbo;
     G.clearDepth = B.clearDepth;
     G.draws.clear();
