@@ -295,6 +295,22 @@ int sl_draw(sl_context* ctx, const sl_mesh* m, size_t shaderId, size_t fboId);  
 int sl_draw_multiple(sl_context* ctx, const sl_mesh* m, size_t numMeshes, size_t shaderId, size_t fboId); /* SL_Context::draw_multiple */
 int sl_draw_instanced(sl_context* ctx, const sl_mesh* m, size_t numInstances, size_t shaderId, size_t fboId); /* SL_Context::draw_instanced */
 
+/* ---- a draw that allocates nothing (no counterpart in the reference) ------
+ * For callers that cannot allocate device memory (serving loops and
+ * harnesses that forbid hipMalloc): sl_draw_static records the draw like
+ * sl_draw and runs the pending batch at once, with every per-frame buffer --
+ * setup records, varyings, bin entries, clip jobs, counters, tile ranges, the
+ * draw table and the uniform snapshots -- laid out in one device array of
+ * SL_STATIC_SCRATCH_BYTES compiled into the library (each GPU holds its own
+ * copy until the process ends).  It calls no hipMalloc, hipFree or other
+ * allocating routine and returns when the frame is complete.  A frame whose
+ * buffers do not fit returns SL_ERR_CAPACITY before any pixel is touched.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (from several host threads or contexts) race.  Results equal sl_draw's
+ * followed by sl_finish. */
+#define SL_STATIC_SCRATCH_BYTES ((size_t)64 << 20)
+int sl_draw_static(sl_context* ctx, const sl_mesh* m, size_t shaderId, size_t fboId);
+
 /* ---- clears (SL_Context.cpp:881-964) ------------------------------------- */
 int sl_clear_color_buffer(sl_context* ctx, size_t fbo, unsigned attachment, const double rgba[4]);      /* SL_Context::clear_color_buffer */
 int sl_clear_depth_buffer(sl_context* ctx, size_t fbo, double depth);                                  /* SL_Context::clear_depth_buffer */

@@ -2335,6 +2335,10 @@ int slo_draw_instanced(sl_context* c, const sl_mesh* m, size_t numInstances, siz
     return run_draws(c, m, 1, numInstances, shader, fbo);
 }
 
+// sl_draw_static (no reference counterpart): the oracle draws synchronously
+// into host memory, so it is sl_draw.
+int slo_draw_static(sl_context* c, const sl_mesh* m, size_t shader, size_t fbo) { return slo_draw(c, m, shader, fbo); }
+
 int slo_clear_color_buffer(sl_context* c, size_t fbo, unsigned att, const double rgba[4]) { return clear_color(c, fbo, att, rgba); }
 
 int slo_clear_depth_buffer(sl_context* c, size_t fbo, double depth) { return clear_depth(c, fbo, depth); }

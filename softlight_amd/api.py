@@ -136,6 +136,7 @@ _SIGS = {
     "draw": (_I, [_P, C.POINTER(sl_mesh), _SZ, _SZ]),
     "draw_multiple": (_I, [_P, C.POINTER(sl_mesh), _SZ, _SZ, _SZ]),
     "draw_instanced": (_I, [_P, C.POINTER(sl_mesh), _SZ, _SZ, _SZ]),
+    "draw_static": (_I, [_P, C.POINTER(sl_mesh), _SZ, _SZ]),
     "clear_color_buffer": (_I, [_P, _SZ, _U, C.POINTER(C.c_double)]),
     "clear_depth_buffer": (_I, [_P, _SZ, C.c_double]),
     "clear_framebuffer": (_I, [_P, _SZ, _U, C.POINTER(C.c_double), C.c_double]),
@@ -388,6 +389,11 @@ class SLContext:
 
     def draw_instanced(self, mesh: sl_mesh, n, shader, fbo):
         return self._check(self.lib.draw_instanced(self.h, C.byref(mesh), n, shader, fbo), "draw_instanced")
+
+    def draw_static(self, mesh: sl_mesh, shader, fbo):
+        """sl_draw_static: draw and run the pending batch at once in the
+        library's fixed scratch array (no device allocation)."""
+        return self._check(self.lib.draw_static(self.h, C.byref(mesh), shader, fbo), "draw_static")
 
     def clear_color_buffer(self, fbo, att, rgba):
         c = (C.c_double * 4)(*rgba)

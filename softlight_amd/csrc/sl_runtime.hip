@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,12 @@
 #include "sl_smooth.h"
 
 using namespace slg;
+
+// The per-frame buffers of sl_draw_static (setup records, varyings, bin
+// entries, clip jobs, counters, tile ranges, draw table, uniform arena) live
+// in this array; each GPU holds its own copy until the process ends.  Host
+// code uses the per-device address from hipGetSymbolAddress, never the name.
+static __device__ __align__(256) unsigned char scratch[SL_STATIC_SCRATCH_BYTES];
 
 namespace {
 
@@ -343,7 +350,7 @@ Tex* tex_at(sl_context* c, size_t id)
     return (id < c->texs.size() && c->texs[id].valid) ? &c->texs[id] : nullptr;
 }
 
-int flush(sl_context* c);
+int flush(sl_context* c, bool staticScratch = false);
 
 // ---------------------------------------------------------------------------
 // batch execution
@@ -615,7 +622,108 @@ void stale_saved_draws(sl_context* c)
         if (!F.saved.empty()) F.savedStale = true;
 }
 
-int execute_batch(sl_context* c)
+// ---------------------------------------------------------------------------
+// Static scratch (sl_draw_static): the frame's buffers carved out of
+// `scratch` instead of allocated.
+// ---------------------------------------------------------------------------
+constexpr int kMaxScratchDevices = 64;
+std::mutex gScratchMutex;
+void* gScratchPtr[kMaxScratchDevices] = {};
+
+// The current device's copy of `scratch`, looked up once per device.
+int static_scratch(sl_context* c, uint8_t** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return check(c, e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxScratchDevices) return fail(c, SL_ERR_UNSUPPORTED, "device index beyond the static scratch table");
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (gScratchPtr[dev] == nullptr)
+    {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return check(c, e, "hipGetSymbolAddress(scratch)");
+        if (ptr == nullptr) return fail(c, SL_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned null");
+        gScratchPtr[dev] = ptr;
+    }
+    *out = reinterpret_cast<uint8_t*>(gScratchPtr[dev]);
+    return SL_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+
+// Bin count matrix rows for up to `pairs` entries (bin_plan never uses more).
+uint64_t bin_rows(uint64_t pairs) { return std::min<uint64_t>(kMaxBinBlocks, std::max<uint64_t>(1, (pairs + kMinBinBlock - 1) / kMinBinBlock)); }
+
+// Bytes of the fixed part: control block, tile ranges and order, draw table,
+// uniform arena, setup chunk buffer, tile totals.
+size_t static_fixed_bytes(const Plan& p, const Batch& B)
+{
+    size_t n = align256(sizeof(FrameCtrl)) + align256(sizeof(uint2) * (size_t)p.numTiles) + align256(sizeof(uint32_t) * (size_t)p.numTiles);
+    if (B.totalTris == 0) return n;
+    n += align256(sizeof(DrawDesc) * B.draws.size()) + align256(std::max<size_t>(B.uni.size(), 16)) + align256(chunk_buffer_bytes(p.chunks)) +
+         align256(sizeof(uint32_t) * (size_t)p.numTiles);
+    return n;
+}
+
+// Bytes of the capacity-dependent part.
+size_t static_var_bytes(const Plan& p, uint64_t recs, uint64_t pairs, uint64_t clips)
+{
+    return align256(sizeof(SetupRec) * recs) + align256(sizeof(float4) * (size_t)p.varStride * recs) + 3 * align256(sizeof(uint32_t) * pairs) +
+           align256(sizeof(ClipJob) * clips) + align256(sizeof(uint32_t) * bin_rows(pairs) * (size_t)p.numTiles);
+}
+
+// Points every buffer of R into the array at the given capacities (the fixed
+// part at the same offsets every time).  SL_ERR_CAPACITY when they do not fit.
+int bind_static(sl_context* c, uint8_t* base, FrameRes& R, const Plan& p, const Batch& B, uint64_t recs, uint64_t pairs, uint64_t clips)
+{
+    const size_t fixed = static_fixed_bytes(p, B);
+    if (fixed > SL_STATIC_SCRATCH_BYTES || (B.totalTris > 0 && static_var_bytes(p, recs, pairs, clips) > SL_STATIC_SCRATCH_BYTES - fixed))
+        return fail(c, SL_ERR_CAPACITY, "frame needs more than the SL_STATIC_SCRATCH_BYTES scratch array of sl_draw_static; use sl_draw");
+    size_t off = 0;
+    auto take = [&](DevBuf& b, size_t bytes) { b.p = base + off; b.bytes = bytes; off += align256(bytes); };
+    take(R.ctrl, sizeof(FrameCtrl));
+    take(R.ranges, sizeof(uint2) * (size_t)p.numTiles);
+    take(R.order, sizeof(uint32_t) * (size_t)p.numTiles);
+    if (B.totalTris == 0) return SL_OK;
+    take(R.draws, sizeof(DrawDesc) * B.draws.size());
+    take(R.uni, std::max<size_t>(B.uni.size(), 16));
+    take(R.status, chunk_buffer_bytes(p.chunks));
+    take(R.binTotals, sizeof(uint32_t) * (size_t)p.numTiles);
+    take(R.recs, sizeof(SetupRec) * recs);
+    take(R.vars, sizeof(float4) * (size_t)p.varStride * recs);
+    take(R.keys, sizeof(uint32_t) * pairs);
+    take(R.vals, sizeof(uint32_t) * pairs);
+    take(R.sorted, sizeof(uint32_t) * pairs);
+    take(R.clip, sizeof(ClipJob) * clips);
+    take(R.binCounts, sizeof(uint32_t) * bin_rows(pairs) * (size_t)p.numTiles);
+    R.recCap = (uint32_t)recs;
+    R.pairCap = (uint32_t)pairs;
+    R.clipCap = (uint32_t)clips;
+    return SL_OK;
+}
+
+// Capacities filling the free part of the array: records : entries : clip
+// jobs as the grown buffers start (1 : 4 : 1/4), or `need` (exact totals of an
+// overflowed run) with the rest spread over them in proportion.
+void static_capacities(const Plan& p, const Batch& B, const uint64_t need[3], uint64_t cap[3])
+{
+    const size_t freeBytes = SL_STATIC_SCRATCH_BYTES - std::min<size_t>(SL_STATIC_SCRATCH_BYTES, static_fixed_bytes(p, B));
+    const uint64_t base[3] = {need[0] ? need[0] : 4, need[1] ? need[1] : 16, need[2] ? need[2] : 1};
+    // largest scale k (in 1/64 steps) at which base * k still fits
+    uint64_t lo = 64, hi = 64;
+    while (hi < (1ull << 40) && static_var_bytes(p, base[0] * hi / 64, base[1] * hi / 64, base[2] * hi / 64 + 1) <= freeBytes) { lo = hi; hi *= 2; }
+    while (hi - lo > 1)
+    {
+        const uint64_t mid = (lo + hi) / 2;
+        if (static_var_bytes(p, base[0] * mid / 64, base[1] * mid / 64, base[2] * mid / 64 + 1) <= freeBytes) lo = mid; else hi = mid;
+    }
+    for (int k = 0; k < 3; ++k) cap[k] = std::min<uint64_t>(base[k] * lo / 64 + (k == 2 ? 1 : 0), (1ull << 30));
+}
+
+// Runs the pending batch: into the context's grow-only buffers, or (static)
+// into `scratch` without allocating.
+int execute_batch(sl_context* c, bool staticScratch = false)
 {
     Batch& B = c->batch;
     if (B.empty()) return SL_OK;
@@ -623,14 +731,29 @@ int execute_batch(sl_context* c)
     int rc = make_plan(c, B, p);
     if (rc != SL_OK) return rc;
     hipStream_t s = c->stream;
-    FrameRes& R = c->res;
+    FrameRes local;
+    FrameRes& R = staticScratch ? local : c->res;
+    uint8_t* base = nullptr;
+    if (staticScratch && (rc = static_scratch(c, &base)) != SL_OK) return rc;
+    uint64_t need[3] = {0, 0, 0};
     FrameCtrl* ctrl = nullptr;
     float msVertex = 0.f, msBin = 0.f, msRaster = 0.f;
     uint32_t totalRecs = 0, totalPairs = 0, jobs = 0;
 
     for (int attempt = 0; attempt < 4; ++attempt)
     {
-        if ((rc = ensure_frame(c, R, p, B)) != SL_OK) return rc;
+        if (staticScratch)
+        {
+            uint64_t cap[3];
+            static_capacities(p, B, need, cap);
+            if (attempt > 0 && (cap[0] < need[0] || cap[1] < need[1] || cap[2] < need[2]))
+                return fail(c, SL_ERR_CAPACITY, "frame needs more than the SL_STATIC_SCRATCH_BYTES scratch array of sl_draw_static; use sl_draw");
+            if ((rc = bind_static(c, base, R, p, B, cap[0], cap[1], cap[2])) != SL_OK) return rc;
+        }
+        else if ((rc = ensure_frame(c, R, p, B)) != SL_OK)
+        {
+            return rc;
+        }
         ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
         if (attempt == 0 && (rc = upload_batch(c, R, B, s)) != SL_OK) return rc;
         if (B.totalTris == 0)
@@ -646,6 +769,15 @@ int execute_batch(sl_context* c)
         if ((rc = check(c, hipStreamSynchronize(s), "k_setup sync")) != SL_OK) return rc;
         if (c->hCtrl->error) return fail(c, SL_ERR_DEVICE, "setup emitting pass disagrees with the counting pass");
         if (!c->hCtrl->overflow) break;
+        if (staticScratch)
+        {
+            // lay the array out for the exact totals and run setup again
+            need[0] = c->hCtrl->totalRecs;
+            need[1] = c->hCtrl->totalPairs;
+            need[2] = c->hCtrl->clipJobs;
+            if (attempt == 3) return fail(c, SL_ERR_CAPACITY, "static scratch layout did not converge");
+            continue;
+        }
         // grow to the exact requirement (+25%) and run setup again; the
         // framebuffer has not been touched yet.
         auto grow = [](uint64_t need, uint32_t cur) -> uint64_t { return std::max<uint64_t>(cur, need + need / 4 + 1024); };
@@ -855,10 +987,10 @@ void collect_backward(sl_context* c)
     c->bwdPending = 0;
 }
 
-int flush(sl_context* c)
+int flush(sl_context* c, bool staticScratch)
 {
     if (c->batch.empty()) return SL_OK;
-    const int rc = execute_batch(c);
+    const int rc = execute_batch(c, staticScratch);
     c->batch.reset();
     return rc;
 }
@@ -1475,6 +1607,24 @@ int sl_draw_instanced(sl_context* c, const sl_mesh* m, size_t numInstances, size
     if (!c || !m) return SL_ERR_INVALID_ARG;
     if (numInstances == 0) return SL_OK;
     return record_draw(c, *m, numInstances, shader, fbo, m->mode);
+}
+
+int sl_draw_static(sl_context* c, const sl_mesh* m, size_t shaderId, size_t fboId)
+{
+    if (!c || !m) return SL_ERR_INVALID_ARG;
+    // a pending batch for another framebuffer runs first, through the array too
+    if (!c->batch.empty() && c->batch.fbo != fboId)
+    {
+        const int rc = flush(c, true);
+        if (rc != SL_OK) return rc;
+    }
+    int rc = record_draw(c, *m, 1, shaderId, fboId, (int)m->mode);
+    if (rc != SL_OK)
+    {
+        (void)flush(c, true);
+        return rc;
+    }
+    return flush(c, true);
 }
 
 int sl_clear_color_buffer(sl_context* c, size_t fbo, unsigned att, const double rgba[4])
