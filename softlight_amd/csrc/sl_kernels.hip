@@ -1389,16 +1389,25 @@ __global__ __launch_bounds__(kColTiles * kColParts) void k_bin_colscan(const Fra
     }
 }
 
-__global__ __launch_bounds__(1024) void k_bin_tilescan(const uint32_t* __restrict__ totals, uint2* __restrict__ range, int numTiles)
+// Tile ranges (exclusive scan of the tile totals) and the raster's launch
+// order, heaviest tiles first: tiles bucketed by floor(log2(records)) + 1
+// (empty tiles last); the order inside a bucket is arbitrary, which only
+// affects scheduling.  One workgroup.
+__global__ __launch_bounds__(1024) void k_bin_tilescan(const uint32_t* __restrict__ totals, uint2* __restrict__ range,
+                                                       uint32_t* __restrict__ order, int numTiles)
 {
     __shared__ uint32_t sWave[16];
     __shared__ uint32_t sCarry;
+    __shared__ uint32_t sCnt[33];
+    __shared__ uint32_t sBucket[33];
     if (threadIdx.x == 0) sCarry = 0;
+    if (threadIdx.x < 33) sCnt[threadIdx.x] = 0u;
     __syncthreads();
     for (int base = 0; base < numTiles; base += 1024)
     {
         const int t = base + threadIdx.x;
         const uint32_t v = (t < numTiles) ? totals[t] : 0u;
+        if (t < numTiles) atomicAdd(&sCnt[32 - __clz((int)v)], 1u);
         uint32_t incl = v;
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -1421,38 +1430,18 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(const uint32_t* __restric
         if (threadIdx.x == 0) sCarry += chunkTotal;
         __syncthreads();
     }
-}
-
-// Launch order of the raster tiles, heaviest first: tiles bucketed by
-// floor(log2(records)) + 1 (empty tiles last); the order inside a bucket is
-// arbitrary, which only affects scheduling.
-__global__ __launch_bounds__(1024) void k_tile_order(const uint2* __restrict__ range, uint32_t* __restrict__ order, int numTiles)
-{
-    __shared__ uint32_t sCnt[33];
-    __shared__ uint32_t sBase[33];
-    if (threadIdx.x < 33) sCnt[threadIdx.x] = 0u;
-    __syncthreads();
-    for (int t = threadIdx.x; t < numTiles; t += 1024)
-    {
-        const uint2 r = range[t];
-        atomicAdd(&sCnt[32 - __clz((int)(r.y - r.x))], 1u);
-    }
-    __syncthreads();
     if (threadIdx.x == 0)
     {
         uint32_t run = 0;
         for (int b = 32; b >= 0; --b)
         {
-            sBase[b] = run;
+            sBucket[b] = run;
             run += sCnt[b];
         }
     }
     __syncthreads();
     for (int t = threadIdx.x; t < numTiles; t += 1024)
-    {
-        const uint2 r = range[t];
-        order[atomicAdd(&sBase[32 - __clz((int)(r.y - r.x))], 1u)] = (uint32_t)t;
-    }
+        order[atomicAdd(&sBucket[32 - __clz((int)totals[t])], 1u)] = (uint32_t)t;
 }
 
 __global__ __launch_bounds__(64) void k_bin_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
@@ -1466,16 +1455,39 @@ __global__ __launch_bounds__(64) void k_bin_scatter(const uint32_t* __restrict__
     if (start >= n) return;
     const uint32_t end = min(start + per, n);
     const uint32_t* row = counts + (uint64_t)blockIdx.x * numTiles;
-    for (int t = threadIdx.x; t < numTiles; t += 64) sBase[t] = range[t].x + row[t];
+    // the block's base slot per tile; eight tiles' loads in flight per lane
+    constexpr int kInit = 8;
+    for (int t0 = 0; t0 < numTiles; t0 += 64 * kInit)
+    {
+        uint32_t rx[kInit], rw[kInit];
+#pragma unroll
+        for (int j = 0; j < kInit; ++j)
+        {
+            const int t = t0 + j * 64 + (int)threadIdx.x;
+            rx[j] = (t < numTiles) ? range[t].x : 0u;
+            rw[j] = (t < numTiles) ? row[t] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < kInit; ++j)
+        {
+            const int t = t0 + j * 64 + (int)threadIdx.x;
+            if (t < numTiles) sBase[t] = rx[j] + rw[j];
+        }
+    }
     __syncthreads();
     const int lane = threadIdx.x;
     const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);   // lanes <= lane
+    // the next 64 entries' loads are issued before the current ones are placed
+    uint32_t nkey = (start + lane < end) ? keys[start + lane] : 0xFFFFFFFFu;
+    uint32_t nval = (start + lane < end) ? vals[start + lane] : 0u;
     for (uint32_t c = start; c < end; c += 64)
     {
-        const uint32_t i = c + lane;
-        const uint32_t key = (i < end) ? keys[i] : 0xFFFFFFFFu;
+        const uint32_t key = nkey, rawVal = nval;
+        const uint32_t inext = c + 64 + lane;
+        nkey = (inext < end) ? keys[inext] : 0xFFFFFFFFu;
+        nval = (inext < end) ? vals[inext] : 0u;
         const bool valid = key < (uint32_t)numTiles;
-        const uint32_t val = valid ? vals[i] : 0u;
+        const uint32_t val = valid ? rawVal : 0u;
         // bitonic sort of (tile << 6 | lane) across the wave: equal tiles end
         // up adjacent and in entry order; invalid entries sort last
         uint32_t sv = ((valid ? key : (uint32_t)numTiles) << 6) | (uint32_t)lane;
@@ -1667,10 +1679,9 @@ hipError_t launch_binning(const uint32_t* keys, const uint32_t* vals, const Fram
     hipLaunchKernelGGL(k_bin_count, dim3(plan.blocks), dim3(256), lds, s, keys, ctrl, cap, plan.per, counts, numTiles);
     hipLaunchKernelGGL(k_bin_colscan, dim3((numTiles + kColTiles - 1) / kColTiles), dim3(kColTiles * kColParts), 0, s,
                        ctrl, cap, plan.per, counts, totals, numTiles);
-    hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), 0, s, totals, range, numTiles);
+    hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), 0, s, totals, range, order, numTiles);
     hipLaunchKernelGGL(k_bin_scatter, dim3(plan.blocks), dim3(64), lds, s, keys, vals, ctrl, cap, plan.per, counts, range, out,
                        numTiles);
-    hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(1024), 0, s, range, order, numTiles);
     return hipGetLastError();
 }
 
