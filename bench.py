@@ -42,6 +42,7 @@ VIEWS_WORKLOAD = dict(seed=4, level=5, views=32, size=512, views_per_fbo=8, tex_
 SMOOTH_WORKLOAD = dict(seed=6, meshes=8, min_tris=5000, max_tris=60000, views=64, size=512, views_per_fbo=8,
                        tex_size=512)
 SMOOTH_SIGMAS = (1e-2, 1e-3, 1e-4)
+RASTER_SPLIT = None          # --raster-split: records per part of a split heavy tile (None: the library's default)
 CALIBRATION_BYTES = 256 << 20
 
 
@@ -75,11 +76,15 @@ def frag_bytes(tested, shaded, texel_bytes=16.0) -> float:
     return 4.0 * tested + (4.0 + 4.0 + texel_bytes) * shaded
 
 
-def run_gpu(steps, warmup, workload="spheres", raster_config=None, tune=False):
+def run_gpu(steps, warmup, workload="spheres", raster_config=None, tune=False, raster_split=None):
     import softlight_amd
     from softlight_amd import api, autotune, scenes
 
     ctx = api.SLContext(softlight_amd.load_library())
+    if raster_split is None:
+        raster_split = RASTER_SPLIT
+    if raster_split is not None:
+        ctx.set_raster_split(raster_split)
     if workload == "spheres":
         scene = scenes.build_sphere_scene(ctx, **WORKLOAD)
     elif workload in RENDER_MODES:
@@ -394,7 +399,11 @@ def main():
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter calibration)")
     ap.add_argument("--raster-config", type=int, default=None, help="raster tile configuration 0..7 (default: tuned)")
     ap.add_argument("--tune", action="store_true", help="time all raster configurations first and cache the best")
+    ap.add_argument("--raster-split", type=int, default=None,
+                    help="records per part of a split heavy raster tile (0: no split; default: the library's)")
     args = ap.parse_args()
+    global RASTER_SPLIT
+    RASTER_SPLIT = args.raster_split
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (a single-GPU box)",
               file=sys.stderr)
@@ -412,7 +421,8 @@ def main():
     if args.workload == "scene_file":
         print(json.dumps({"scene_file": run_scene_file(max(2, args.steps), cpu=not args.no_cpu)}), flush=True)
         return
-    scene, st, eager_s, gst, elapsed = run_gpu(args.steps, args.warmup, args.workload, args.raster_config, args.tune)
+    scene, st, eager_s, gst, elapsed = run_gpu(args.steps, args.warmup, args.workload, args.raster_config, args.tune,
+                                               args.raster_split)
     tris = scene.n_triangles
     frames = max(st["frames"], 1)
     ms_step = elapsed * 1e3 / args.steps

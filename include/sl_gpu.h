@@ -230,6 +230,16 @@ int         sl_set_scissor(sl_context* ctx, int32_t x, int32_t y, int32_t w, int
  * sl_get_raster_config(ctx, -1, ...) reports the current one and returns its
  * index; results do not depend on the configuration. */
 int         sl_set_raster_config(sl_context* ctx, int config);
+/* Heavy tiles (no counterpart in the reference): when a batch's draws are
+ * all unblended with one depth function on an R_FLOAT depth target (depth
+ * writes on with LESS / LESS_EQUAL / GREATER / GREATER_EQUAL, or off), a tile
+ * with more than 4 * partRecords records is split into parts of about
+ * partRecords records (at most 1024 parts) rasterised by separate workgroups
+ * and merged exactly.  By default batches with point or line draws split
+ * with parts of 256 records; sl_set_raster_split(n > 0) splits every such
+ * batch with parts of n records, 0 turns the split off.  Results never
+ * depend on it. */
+int         sl_set_raster_split(sl_context* ctx, uint32_t partRecords);
 int         sl_get_raster_config(const sl_context* ctx, int config, int* tileW, int* tileH, int* segments);
 
 /* ---- vertex buffers (SL_VertexBuffer.hpp / .cpp) ------------------------ */

@@ -2061,6 +2061,13 @@ int slo_set_raster_config(sl_context* c, int config)
     return config >= 0 ? SL_OK : SL_ERR_INVALID_ARG;
 }
 
+// Heavy-tile split of the GPU raster: no effect on results, none here.
+int slo_set_raster_split(sl_context* c, uint32_t partRecords)
+{
+    (void)partRecords;
+    return c ? SL_OK : SL_ERR_INVALID_ARG;
+}
+
 int slo_get_raster_config(const sl_context* c, int config, int* tileW, int* tileH, int* segments)
 {
     if (!c) return SL_ERR_INVALID_ARG;

@@ -107,6 +107,7 @@ _SIGS = {
     "set_num_threads": (_U, [_P, _U]),
     "set_raster_config": (_I, [_P, _I]),
     "get_raster_config": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "set_raster_split": (_I, [_P, C.c_uint32]),
     "set_viewport": (_I, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "set_scissor": (_I, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "create_vbo": (_SZ, [_P]),
@@ -230,6 +231,10 @@ class SLContext:
 
     def set_raster_config(self, config):
         return self._check(self.lib.set_raster_config(self.h, config), "set_raster_config")
+
+    def set_raster_split(self, part_records):
+        """Records per part of a split heavy tile (0: no split)."""
+        return self._check(self.lib.set_raster_split(self.h, part_records), "set_raster_split")
 
     def raster_config(self, config=-1):
         w, h, s = _I(), _I(), _I()

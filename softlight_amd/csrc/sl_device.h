@@ -353,6 +353,51 @@ struct SetupParams
     FrameCtrl* ctrl;
 };
 
+// Heavy tiles.  A tile whose record list is long is split into parts of
+// about the same length, each raster workgroup taking one part, when the
+// batch's draws allow an exact merge (splitClass): every draw unblended on a
+// 32-bit float depth target, with one depth function, and either depth
+// writes on with a min-type (LESS / LEQUAL: kSplitMin) or max-type (GREATER /
+// GEQUAL: kSplitMax) function, or depth writes off (kSplitConst).  The depth
+// a pixel holds after a run of fragments is then fold(d, m): the run's own
+// result m (its fragments depth-tested from +inf / -inf) replaces d when it
+// passes the test against d, so part k can start from the exact depth the
+// sequential walk would have (a first pass, mode 1, computes every part's m),
+// count exactly the fragments that pass, and leave its last passing
+// fragment; the tile's last part to finish merges the parts in order.
+// Between the passes a fold pass turns every part's m into the running
+// fold(initial, m_0 .. m_j), so each part reads its starting depth once.
+enum : uint32_t { kSplitNone = 0, kSplitMin = 1, kSplitMax = 2, kSplitConst = 3 };
+constexpr uint32_t kDefaultSplitRecords = 256;  // records per part of a split tile
+constexpr uint32_t kSplitAboveParts = 4;        // tiles of more than kSplitAboveParts parts' records split
+constexpr uint32_t kMaxTileParts = 1024;        // parts of one split tile at most
+
+// One raster workgroup's work: a whole tile (parts == 1) or part `part` of
+// a split tile, records [begin, end) of its list; part results live in slot
+// `slot` (the tile's parts in slots slot - part .. slot - part + parts - 1);
+// `done` counts the split tile's finished parts.
+struct RasterItem
+{
+    uint32_t tile;
+    uint32_t begin, end;
+    uint32_t slot;
+    uint32_t part, parts;
+    uint32_t done;
+    uint32_t pad;
+};
+
+// A part's result for one pixel: its last passing fragment (record or
+// kNoPart, barycentric values) and, with the parity attachments, its pass
+// count and the last passing fragment's sequence number.
+struct PartPixel
+{
+    uint32_t rid;
+    float s0, s1, s2;
+    uint32_t count;
+    uint32_t seq;
+};
+constexpr uint32_t kNoPart = 0xFFFFFFFFu;
+
 struct RasterParams
 {
     const DrawDesc* draws;
@@ -384,6 +429,14 @@ struct RasterParams
     int32_t extra;                // some draw of the batch draws wireframe triangles, lines or points
     int32_t mrt;                  // some draw of the batch has several fragment outputs
     FrameCtrl* ctrl;
+    // heavy-tile split (items == nullptr: one workgroup per tile, tileOrder)
+    const RasterItem* items;      // whole tiles and parts of split tiles, parts first
+    const uint32_t* itemCount;    // [0] items, [1] parts of split tiles (items [0, [1]))
+    float* partDepth;             // per part slot, per tile pixel: the part's own depth result m
+    PartPixel* partRes;           // per part slot, per tile pixel
+    uint32_t* partDone;           // per split tile: parts finished
+    uint32_t splitClass;          // kSplit*
+    int32_t splitDtest;           // the batch's depth function
 };
 
 // ---------------------------------------------------------------------------

@@ -1389,59 +1389,185 @@ __global__ __launch_bounds__(kColTiles * kColParts) void k_bin_colscan(const Fra
     }
 }
 
-// Tile ranges (exclusive scan of the tile totals) and the raster's launch
-// order, heaviest tiles first: tiles bucketed by floor(log2(records)) + 1
-// (empty tiles last); the order inside a bucket is arbitrary, which only
-// affects scheduling.  One workgroup.
-__global__ __launch_bounds__(1024) void k_bin_tilescan(const uint32_t* __restrict__ totals, uint2* __restrict__ range,
-                                                       uint32_t* __restrict__ order, int numTiles)
+// Exclusive prefixes of two values over a 1024-thread workgroup, and their
+// totals.
+__device__ __forceinline__ void block_excl_scan2(uint32_t& a, uint32_t& b, uint32_t& totA, uint32_t& totB, uint2* sWave)
 {
-    __shared__ uint32_t sWave[16];
-    __shared__ uint32_t sCarry;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t ia = a, ib = b;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        const uint32_t oa = __shfl_up(ia, off, 64), ob = __shfl_up(ib, off, 64);
+        if (lane >= off) { ia += oa; ib += ob; }
+    }
+    if (lane == 63) sWave[wave] = make_uint2(ia, ib);
+    __syncthreads();
+    uint32_t ba = 0, bb = 0;
+    totA = totB = 0;
+    for (int w = 0; w < 16; ++w)
+    {
+        const uint2 v = sWave[w];
+        if (w < wave) { ba += v.x; bb += v.y; }
+        totA += v.x;
+        totB += v.y;
+    }
+    a = ba + ia - a;
+    b = bb + ib - b;
+    __syncthreads();
+}
+
+// Tile ranges (exclusive scan of the tile totals), the raster's launch order
+// (heaviest tiles first: tiles bucketed by floor(log2(records)) + 1, empty
+// tiles last; the order inside a bucket only affects scheduling) and, with
+// a split plan (P.items), the raster's work items: a tile of more than
+// splitAbove records is cut into ceil(records / partRecs) parts (at most kMaxTileParts)
+// of equal length while the maxParts part slots last (in tile order); the
+// parts come first, then every tile in launch order, a split tile's entry
+// marked as nothing to do (parts == 0).  One workgroup; each thread owns up
+// to kTilesPerThread consecutive tiles.
+constexpr int kTilesPerThread = 16;   // 16384 tiles at most
+struct SplitPlan
+{
+    RasterItem* items;         // null: no work items
+    uint32_t* itemCount;
+    uint32_t* partDone;
+    uint32_t partRecs, splitAbove, maxParts;
+};
+
+__global__ __launch_bounds__(1024) void k_bin_tilescan(const uint32_t* __restrict__ totals, uint2* __restrict__ range,
+                                                       uint32_t* __restrict__ order, int numTiles, SplitPlan SP)
+{
+    __shared__ uint2 sWave[16];
     __shared__ uint32_t sCnt[33];
     __shared__ uint32_t sBucket[33];
-    if (threadIdx.x == 0) sCarry = 0;
     if (threadIdx.x < 33) sCnt[threadIdx.x] = 0u;
-    __syncthreads();
-    for (int base = 0; base < numTiles; base += 1024)
-    {
-        const int t = base + threadIdx.x;
-        const uint32_t v = (t < numTiles) ? totals[t] : 0u;
-        if (t < numTiles) atomicAdd(&sCnt[32 - __clz((int)v)], 1u);
-        uint32_t incl = v;
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int per = (numTiles + 1023) / 1024;
+    const int t0 = (int)threadIdx.x * per;
+    uint32_t tot[kTilesPerThread];
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1)
+    for (int j = 0; j < kTilesPerThread; ++j) tot[j] = (j < per && t0 + j < numTiles) ? totals[t0 + j] : 0u;
+    auto wanted = [&](uint32_t n) -> uint32_t {
+        return (n > SP.splitAbove) ? min((n + SP.partRecs - 1u) / SP.partRecs, kMaxTileParts) : 1u;
+    };
+    uint32_t se = 0, sw = 0;
+#pragma unroll
+    for (int j = 0; j < kTilesPerThread; ++j)
+    {
+        se += tot[j];
+        if (SP.items != nullptr && j < per)
         {
-            const uint32_t o = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += o;
+            const uint32_t p = wanted(tot[j]);
+            sw += (p > 1u) ? p : 0u;
         }
-        if (lane == 63) sWave[wave] = incl;
-        __syncthreads();
-        uint32_t waveBase = 0, chunkTotal = 0;
-        for (int w = 0; w < 16; ++w)
-        {
-            if (w < wave) waveBase += sWave[w];
-            chunkTotal += sWave[w];
-        }
-        const uint32_t excl = sCarry + waveBase + incl - v;
-        if (t < numTiles) range[t] = make_uint2(excl, excl + v);
-        __syncthreads();
-        if (threadIdx.x == 0) sCarry += chunkTotal;
-        __syncthreads();
     }
+    uint32_t totE, totW;
+    block_excl_scan2(se, sw, totE, totW, sWave);
+    // ranges and the launch-order buckets
+    uint32_t run = se;
+#pragma unroll
+    for (int j = 0; j < kTilesPerThread; ++j)
+    {
+        if (j < per && t0 + j < numTiles)
+        {
+            range[t0 + j] = make_uint2(run, run + tot[j]);
+            run += tot[j];
+            atomicAdd(&sCnt[32 - __clz((int)tot[j])], 1u);
+        }
+    }
+    // split decisions: accepted while the parts wanted so far (in tile order) fit
+    uint32_t splitMask = 0, nParts = 0;
+    if (SP.items != nullptr)
+    {
+        uint32_t rw = sw, sa = 0, sc = 0;
+#pragma unroll
+        for (int j = 0; j < kTilesPerThread; ++j)
+        {
+            if (j < per && t0 + j < numTiles)
+            {
+                const uint32_t p = wanted(tot[j]);
+                if (p > 1u)
+                {
+                    if (rw + p <= SP.maxParts)
+                    {
+                        splitMask |= 1u << j;
+                        sa += p;
+                        sc += 1u;
+                    }
+                    rw += p;
+                }
+            }
+        }
+        uint32_t totA, totC;
+        block_excl_scan2(sa, sc, totA, totC, sWave);
+        nParts = totA;
+        uint32_t ra = sa, rc = sc, rb = se;
+#pragma unroll
+        for (int j = 0; j < kTilesPerThread; ++j)
+        {
+            if (j < per && t0 + j < numTiles && ((splitMask >> j) & 1u))
+            {
+                const uint32_t n = tot[j], p = wanted(n);
+                SP.partDone[rc] = 0u;
+                for (uint32_t k = 0; k < p; ++k)
+                {
+                    RasterItem it;
+                    it.tile = (uint32_t)(t0 + j);
+                    it.begin = rb + (uint32_t)(((uint64_t)n * k) / p);
+                    it.end = rb + (uint32_t)(((uint64_t)n * (k + 1)) / p);
+                    it.slot = ra + k;
+                    it.part = k;
+                    it.parts = p;
+                    it.done = rc;
+                    it.pad = 0u;
+                    SP.items[ra + k] = it;
+                }
+                ra += p;
+                rc += 1u;
+            }
+            if (j < per) rb += tot[j];
+        }
+    }
+    __syncthreads();
     if (threadIdx.x == 0)
     {
-        uint32_t run = 0;
+        uint32_t acc = 0;
         for (int b = 32; b >= 0; --b)
         {
-            sBucket[b] = run;
-            run += sCnt[b];
+            sBucket[b] = acc;
+            acc += sCnt[b];
+        }
+        if (SP.items != nullptr)
+        {
+            SP.itemCount[0] = nParts + (uint32_t)numTiles;
+            SP.itemCount[1] = nParts;
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < numTiles; t += 1024)
-        order[atomicAdd(&sBucket[32 - __clz((int)totals[t])], 1u)] = (uint32_t)t;
+    uint32_t rb = se;
+#pragma unroll
+    for (int j = 0; j < kTilesPerThread; ++j)
+    {
+        if (j < per && t0 + j < numTiles)
+        {
+            const uint32_t pos = atomicAdd(&sBucket[32 - __clz((int)tot[j])], 1u);
+            order[pos] = (uint32_t)(t0 + j);
+            if (SP.items != nullptr)
+            {
+                RasterItem it;
+                it.tile = (uint32_t)(t0 + j);
+                it.begin = rb;
+                it.end = rb + tot[j];
+                it.slot = 0u;
+                it.part = 0u;
+                it.parts = ((splitMask >> j) & 1u) ? 0u : 1u;   // a split tile's parts come first
+                it.done = 0u;
+                it.pad = 0u;
+                SP.items[nParts + pos] = it;
+            }
+            rb += tot[j];
+        }
+    }
 }
 
 __global__ __launch_bounds__(64) void k_bin_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
@@ -1672,16 +1798,27 @@ BinPlan bin_plan(uint32_t entries)
 
 hipError_t launch_binning(const uint32_t* keys, const uint32_t* vals, const FrameCtrl* ctrl, uint32_t cap,
                           BinPlan plan, uint32_t* counts, uint32_t* totals, uint2* range, uint32_t* out,
-                          uint32_t* order, int numTiles, hipStream_t s)
+                          uint32_t* order, int numTiles, hipStream_t s, RasterItem* items, uint32_t* itemCount,
+                          uint32_t* partDone, uint32_t partRecs, uint32_t splitAbove, uint32_t maxParts)
 {
+    SplitPlan sp{items, itemCount, partDone, partRecs, splitAbove, maxParts};
     (void)hipGetLastError();   // drop a stale error from an earlier call
     const size_t lds = sizeof(uint32_t) * (size_t)numTiles;
     hipLaunchKernelGGL(k_bin_count, dim3(plan.blocks), dim3(256), lds, s, keys, ctrl, cap, plan.per, counts, numTiles);
     hipLaunchKernelGGL(k_bin_colscan, dim3((numTiles + kColTiles - 1) / kColTiles), dim3(kColTiles * kColParts), 0, s,
                        ctrl, cap, plan.per, counts, totals, numTiles);
-    hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), 0, s, totals, range, order, numTiles);
+    if (numTiles > 1024 * kTilesPerThread) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_bin_tilescan, dim3(1), dim3(1024), 0, s, totals, range, order, numTiles, sp);
     hipLaunchKernelGGL(k_bin_scatter, dim3(plan.blocks), dim3(64), lds, s, keys, vals, ctrl, cap, plan.per, counts, range, out,
                        numTiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_raster_depth(const RasterParams& p, int grid, int cfg, hipStream_t s)
+{
+    (void)hipGetLastError();   // drop a stale error from an earlier call
+    if (p.extra) launch_raster_depth_cfg<1>(p, grid, cfg, s);
+    else launch_raster_depth_cfg<0>(p, grid, cfg, s);
     return hipGetLastError();
 }
 

@@ -301,7 +301,17 @@ constexpr uint32_t kNoPending = 0xFFFFFFFFu;
 // records (the variants without it carry only the filled-triangle code);
 // bit 3: the batch has draws with several fragment outputs (launched with
 // bit 2 set).
-template <int CF, int AUX, int TW, int TH, int SEGS>
+//
+// Heavy tiles (RasterParams::items): a workgroup takes a work item, a whole
+// tile or one part of a split tile's record list.  MODE 1 walks a part from
+// +inf (-inf) and leaves the depth it ends with, the part's own result m; a
+// MODE 0 part starts from fold(initial depth, m of the earlier parts) --
+// exactly the depth the sequential walk has there -- so it counts the
+// fragments that pass exactly and ends with its last passing fragment
+// pending; it leaves those per pixel, and the tile's last part to finish
+// merges: the tile as it was before the batch, the parts' depths folded in
+// order, the last part with a passing fragment giving the one to shade.
+template <int CF, int AUX, int TW, int TH, int SEGS, int MODE>
 __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
 {
     constexpr bool DBG = (AUX & 1) != 0;
@@ -327,32 +337,87 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     // drawing without the records that did not fit; the host reports it
     if (P.ctrl != nullptr && P.ctrl->overflow != 0u) return;
 
-    // heaviest tiles first (the order only affects scheduling)
-    const int tile = (P.tileOrder != nullptr) ? (int)P.tileOrder[blockIdx.x] : (int)blockIdx.x;
+    // the workgroup's work: a tile in launch order (heaviest first; the order
+    // only affects scheduling), or a work item
+    int tile;
+    uint2 range;
+    RasterItem item{};
+    bool split = false;
+    if (P.items != nullptr)
+    {
+        if (blockIdx.x >= P.itemCount[MODE == 1 ? 1 : 0]) return;
+        item = P.items[blockIdx.x];
+        if (item.parts == 0u) return;          // a split tile's entry in launch order: its parts do the work
+        tile = (int)item.tile;
+        range = make_uint2(item.begin, item.end);
+        split = item.parts > 1u;
+    }
+    else
+    {
+        if (MODE == 1) return;
+        tile = (P.tileOrder != nullptr) ? (int)P.tileOrder[blockIdx.x] : (int)blockIdx.x;
+        range = P.tileRange[tile];
+    }
     const int tx = tile % P.tilesX, ty = tile / P.tilesX;
     const int x0 = tx * TW, y0 = ty * TH;
     const int lane = threadIdx.x;
     const int wave = lane >> 6, wl = lane & 63;
+    constexpr int kTilePix = TW * TH;
+    const uint64_t slot0 = (uint64_t)(item.slot - item.part);   // first part slot of a split tile
 
-    // tile init: fused clear or load
-    for (int k = lane; k < TW * TH; k += NT)
-    {
-        const int r = k / TW, c = k % TW;
-        const int gx = x0 + c, gy = y0 + r;
-        const bool inb = gx < P.W && gy < P.H;
-        const uint64_t gi = (uint64_t)gy * P.W + gx;
-        sDepth[r][c] = P.clearDepth ? P.clearDepthValue : (inb ? load_depth(P, gi) : 0.f);
-        if (CF == 1) sCol8[r][c] = P.clearColor ? P.clearRGBA8 : (inb ? load_color8(P, gi) : 0u);
-        if (CF == 2)
-            sColF[r][c] = P.clearColor ? make_float4(P.clearRGBAF[0], P.clearRGBAF[1], P.clearRGBAF[2], P.clearRGBAF[3])
-                                       : (inb ? load_colorF(P, gi) : make_float4(0.f, 0.f, 0.f, 0.f));
-        if (DBG)
+    // tile init: fused clear or load of the tile as it is before the batch
+    auto init_tile = [&]() {
+        for (int k = lane; k < kTilePix; k += NT)
         {
-            sLast[r][c] = P.clearDepth ? 0u : (inb ? P.dbgLast[gi] : 0u);
-            sCount[r][c] = P.clearDepth ? 0u : (inb ? P.dbgCount[gi] : 0u);
+            const int r = k / TW, c = k % TW;
+            const int gx = x0 + c, gy = y0 + r;
+            const bool inb = gx < P.W && gy < P.H;
+            const uint64_t gi = (uint64_t)gy * P.W + gx;
+            sDepth[r][c] = P.clearDepth ? P.clearDepthValue : (inb ? load_depth(P, gi) : 0.f);
+            if (CF == 1) sCol8[r][c] = P.clearColor ? P.clearRGBA8 : (inb ? load_color8(P, gi) : 0u);
+            if (CF == 2)
+                sColF[r][c] = P.clearColor ? make_float4(P.clearRGBAF[0], P.clearRGBAF[1], P.clearRGBAF[2], P.clearRGBAF[3])
+                                           : (inb ? load_colorF(P, gi) : make_float4(0.f, 0.f, 0.f, 0.f));
+            if (DBG)
+            {
+                sLast[r][c] = P.clearDepth ? 0u : (inb ? P.dbgLast[gi] : 0u);
+                sCount[r][c] = P.clearDepth ? 0u : (inb ? P.dbgCount[gi] : 0u);
+            }
+            if (VIS) sVis[r][c] = (P.clearDepth || !inb) ? make_float4(0.f, 0.f, 0.f, 0.f) : P.vis[gi];
+            sPend[r][c] = kNoPending;
         }
-        if (VIS) sVis[r][c] = (P.clearDepth || !inb) ? make_float4(0.f, 0.f, 0.f, 0.f) : P.vis[gi];
-        sPend[r][c] = kNoPending;
+    };
+    if (MODE == 1)
+    {
+        // a part's own result: its fragments depth-tested from the far end
+        const float far = (P.splitClass == kSplitMax) ? -__builtin_inff() : __builtin_inff();
+        for (int k = lane; k < kTilePix; k += NT) sDepth[k / TW][k % TW] = far;
+    }
+    else if (split)
+    {
+        // a part starts from the depth the sequential walk has there: the
+        // tile's depth before the batch with the earlier parts' results
+        // folded in; its counts start at zero and the merge adds them
+        for (int k = lane; k < kTilePix; k += NT)
+        {
+            const int r = k / TW, c = k % TW;
+            const int gx = x0 + c, gy = y0 + r;
+            const bool inb = gx < P.W && gy < P.H;
+            // the fold pass left fold(initial, m_0 .. m_j) in part j's slot
+            float d = P.clearDepth ? P.clearDepthValue : (inb ? load_depth(P, (uint64_t)gy * P.W + gx) : 0.f);
+            if (P.splitClass != kSplitConst && item.part > 0u) d = P.partDepth[(slot0 + item.part - 1u) * kTilePix + k];
+            sDepth[r][c] = d;
+            if (DBG)
+            {
+                sLast[r][c] = 0u;
+                sCount[r][c] = 0u;
+            }
+            sPend[r][c] = kNoPending;
+        }
+    }
+    else
+    {
+        init_tile();
     }
     __syncthreads();
 
@@ -380,7 +445,6 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     const int sxLast = min(sx0 + SW, P.W) - 1;
     const int y = y0 + row;
     const float yf = (float)y;
-    const uint2 range = P.tileRange[tile];
     uint32_t tested = 0, shaded = 0;
 
     // Each wave stages records into LDS kBatch at a time.  The global loads
@@ -469,6 +533,11 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
             // along it (s0)
             auto fragment = [&](int x, float z, float s0, float s1, float s2) {
                 const int c = x - x0;
+                if (MODE == 1)
+                {
+                    if (depth_test(dtest, z, sDepth[row][c])) sDepth[row][c] = z;
+                    return;
+                }
                 ++tested;
                 if (depth_test(dtest, z, sDepth[row][c]))
                 {
@@ -692,6 +761,90 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     }
     __syncthreads();
 
+    auto add_counters = [&]() {
+        unsigned long long t = tested, sh = shaded;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+        {
+            t += __shfl_xor(t, off, 64);
+            sh += __shfl_xor(sh, off, 64);
+        }
+        if ((lane & 63) == 0)
+        {
+            FrameCtrl::CounterSlot& cs = P.ctrl->counters[blockIdx.x % kCounterSlots];
+            atomicAdd(&cs.tested, t);
+            atomicAdd(&cs.shaded, sh);
+        }
+        tested = 0;
+        shaded = 0;
+    };
+
+    if (MODE == 1)
+    {
+        float* out = P.partDepth + (uint64_t)item.slot * kTilePix;
+        for (int k = lane; k < kTilePix; k += NT) out[k] = sDepth[k / TW][k % TW];
+        return;
+    }
+    if (split)
+    {
+        // this part's results, then the merge by the tile's last part
+        PartPixel* out = P.partRes + (uint64_t)item.slot * kTilePix;
+        for (int k = lane; k < kTilePix; k += NT)
+        {
+            const int r = k / TW, c = k % TW;
+            PartPixel q;
+            q.rid = (sPend[r][c] == kNoPending) ? kNoPart : sPend[r][c];
+            q.s0 = sPendS[0][r][c];
+            q.s1 = sPendS[1][r][c];
+            q.s2 = sPendS[2][r][c];
+            q.count = DBG ? sCount[r][c] : 0u;
+            q.seq = DBG ? sLast[r][c] : 0u;
+            out[k] = q;
+        }
+        add_counters();
+        __shared__ uint32_t sFinished;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (lane == 0) sFinished = atomicAdd(&P.partDone[item.done], 1u);
+        __syncthreads();
+        if (sFinished != item.parts - 1u) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        init_tile();
+        __syncthreads();
+        for (int k = lane; k < kTilePix; k += NT)
+        {
+            const int r = k / TW, c = k % TW;
+            // the depth after every part (the fold pass's last slot)
+            if (P.splitClass != kSplitConst) sDepth[r][c] = P.partDepth[(slot0 + item.parts - 1u) * kTilePix + k];
+            // the fragment to shade: the last part that left one
+            for (uint32_t j = item.parts; j-- > 0u;)
+            {
+                const PartPixel q = P.partRes[(slot0 + j) * kTilePix + k];
+                if (q.rid != kNoPart)
+                {
+                    sPend[r][c] = q.rid;
+                    sPendS[0][r][c] = q.s0;
+                    sPendS[1][r][c] = q.s1;
+                    sPendS[2][r][c] = q.s2;
+                    break;
+                }
+            }
+            if (DBG)
+            {
+                for (uint32_t j = 0; j < item.parts; ++j)
+                {
+                    const PartPixel q = P.partRes[(slot0 + j) * kTilePix + k];
+                    if (q.count != 0u)
+                    {
+                        sCount[r][c] += q.count;
+                        sLast[r][c] = q.seq;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
     // shade what is still pending, then store the tile
     for (int k = lane; k < TW * TH; k += NT)
     {
@@ -713,48 +866,78 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
         }
     }
 
-    // counters
-    unsigned long long t = tested, s = shaded;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-    {
-        t += __shfl_xor(t, off, 64);
-        s += __shfl_xor(s, off, 64);
-    }
-    if ((lane & 63) == 0)
-    {
-        FrameCtrl::CounterSlot& slot = P.ctrl->counters[blockIdx.x % kCounterSlots];
-        atomicAdd(&slot.tested, t);
-        atomicAdd(&slot.shaded, s);
-    }
+    if (!split) add_counters();
 }
 
 }  // namespace
 
-template <int CF, int AUX, int TW, int TH, int SEGS>
-static void launch_raster_shape(const RasterParams& p, int numTiles, hipStream_t s)
+// Between the two passes of the heavy-tile split: for every pixel of a split
+// tile, part j's slot gets fold(initial depth, m_0 .. m_j), the depth the
+// sequential walk has after part j.  One workgroup per split tile (the work
+// item of its part 0), one thread per pixel.
+template <int TW, int TH>
+__global__ __launch_bounds__(256) void k_split_fold(RasterParams P)
+{
+    if (P.ctrl != nullptr && P.ctrl->overflow != 0u) return;
+    if (blockIdx.x >= P.itemCount[1]) return;
+    const RasterItem item = P.items[blockIdx.x];
+    if (item.part != 0u) return;
+    constexpr int kTilePix = TW * TH;
+    const int tx = (int)item.tile % P.tilesX, ty = (int)item.tile / P.tilesX;
+    for (int k = threadIdx.x; k < kTilePix; k += 256)
+    {
+        const int gx = tx * TW + k % TW, gy = ty * TH + k / TW;
+        const bool inb = gx < P.W && gy < P.H;
+        float d = P.clearDepth ? P.clearDepthValue : (inb ? load_depth(P, (uint64_t)gy * P.W + gx) : 0.f);
+        float* m = P.partDepth + (uint64_t)item.slot * kTilePix + k;
+        for (uint32_t j = 0; j < item.parts; ++j)
+        {
+            const float mj = m[(uint64_t)j * kTilePix];
+            if (depth_test(P.splitDtest, mj, d)) d = mj;
+            m[(uint64_t)j * kTilePix] = d;
+        }
+    }
+}
+
+template <int CF, int AUX, int TW, int TH, int SEGS, int MODE>
+static void launch_raster_shape(const RasterParams& p, int grid, hipStream_t s)
 {
     static_assert(TW * TH > 0, "tile");
-    hipLaunchKernelGGL((k_raster<CF, AUX, TW, TH, SEGS>), dim3(numTiles), dim3(TH * 4 * SEGS), 0, s, p);
+    hipLaunchKernelGGL((k_raster<CF, AUX, TW, TH, SEGS, MODE>), dim3(grid), dim3(TH * 4 * SEGS), 0, s, p);
+    if (MODE == 1) hipLaunchKernelGGL((k_split_fold<TW, TH>), dim3(grid), dim3(256), 0, s, p);
 }
 
 // The cases mirror kTileConfigs (checked below).
-template <int CF, int AUX>
-void launch_raster_cfg(const RasterParams& p, int numTiles, int cfg, hipStream_t s)
+template <int CF, int AUX, int MODE>
+static void launch_raster_mode(const RasterParams& p, int grid, int cfg, hipStream_t s)
 {
     switch (cfg)
     {
-        case 1: launch_raster_shape<CF, AUX, 32, 8, 2>(p, numTiles, s); break;
-        case 2: launch_raster_shape<CF, AUX, 16, 16, 1>(p, numTiles, s); break;
-        case 3: launch_raster_shape<CF, AUX, 16, 8, 2>(p, numTiles, s); break;
-        case 4: launch_raster_shape<CF, AUX, 16, 8, 4>(p, numTiles, s); break;
-        case 5: launch_raster_shape<CF, AUX, 16, 16, 4>(p, numTiles, s); break;
-        case 6: launch_raster_shape<CF, AUX, 32, 16, 4>(p, numTiles, s); break;
-        case 7: launch_raster_shape<CF, AUX, 32, 32, 2>(p, numTiles, s); break;
-        case 8: launch_raster_shape<CF, AUX, 32, 8, 4>(p, numTiles, s); break;
-        case 9: launch_raster_shape<CF, AUX, 32, 8, 8>(p, numTiles, s); break;
-        default: launch_raster_shape<CF, AUX, 32, 16, 1>(p, numTiles, s); break;
+        case 1: launch_raster_shape<CF, AUX, 32, 8, 2, MODE>(p, grid, s); break;
+        case 2: launch_raster_shape<CF, AUX, 16, 16, 1, MODE>(p, grid, s); break;
+        case 3: launch_raster_shape<CF, AUX, 16, 8, 2, MODE>(p, grid, s); break;
+        case 4: launch_raster_shape<CF, AUX, 16, 8, 4, MODE>(p, grid, s); break;
+        case 5: launch_raster_shape<CF, AUX, 16, 16, 4, MODE>(p, grid, s); break;
+        case 6: launch_raster_shape<CF, AUX, 32, 16, 4, MODE>(p, grid, s); break;
+        case 7: launch_raster_shape<CF, AUX, 32, 32, 2, MODE>(p, grid, s); break;
+        case 8: launch_raster_shape<CF, AUX, 32, 8, 4, MODE>(p, grid, s); break;
+        case 9: launch_raster_shape<CF, AUX, 32, 8, 8, MODE>(p, grid, s); break;
+        default: launch_raster_shape<CF, AUX, 32, 16, 1, MODE>(p, grid, s); break;
     }
+}
+
+template <int CF, int AUX>
+void launch_raster_cfg(const RasterParams& p, int grid, int cfg, hipStream_t s)
+{
+    launch_raster_mode<CF, AUX, 0>(p, grid, cfg, s);
+}
+
+// The first pass over split tiles' parts (MODE 1); EXTRA: the batch holds
+// wireframe, line or point records.
+template <int EXTRA>
+void launch_raster_depth_cfg(const RasterParams& p, int grid, int cfg, hipStream_t s)
+{
+    launch_raster_mode<0, EXTRA ? 4 : 0, 1>(p, grid, cfg, s);
 }
 static_assert(kTileConfigs[0].tileW == 32 && kTileConfigs[0].tileH == 16 && kTileConfigs[0].segs == 1 &&
               kTileConfigs[1].tileW == 32 && kTileConfigs[1].tileH == 8 && kTileConfigs[1].segs == 2 &&

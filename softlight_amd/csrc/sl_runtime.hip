@@ -76,6 +76,11 @@ struct FrameRes
 {
     DevBuf draws, uni, recs, vars, keys, vals, sorted, clip, status, ranges, ctrl, binCounts, binTotals, order;
     uint32_t recCap = 0, pairCap = 0, clipCap = 0;
+    // heavy-tile split: work items, their counts, per part slot the first
+    // pass's depths and the part results, per split tile a finished-parts
+    // counter (absent: no split)
+    DevBuf items, itemCount, partDepth, partRes, partDone;
+    uint32_t maxParts = 0;
 };
 
 struct FrameGraph
@@ -236,6 +241,8 @@ struct sl_context
     int32_t scissor[4] = {0, 0, 65535, 65535};
     unsigned numThreads = 1;
     int tileConfig = kDefaultTileConfig;
+    uint32_t splitRecs = kDefaultSplitRecords;   // heavy-tile split: records per part (0: off)
+    bool splitAll = false;                       // split triangle-only batches too (sl_set_raster_split)
     uint32_t triSeq = 0;
     sl_frame_stats stats{};
     std::string err;
@@ -420,6 +427,34 @@ size_t chunk_buffer_bytes(uint64_t chunks)
            (sizeof(unsigned long long) + sizeof(uint32_t)) * chunk_blocks(chunks) + sizeof(uint32_t) * chunks;
 }
 
+// Part slots of the heavy-tile split: the per-slot buffers (first-pass
+// depths and part results, 28 B per tile pixel) kept within 64 MB.
+uint32_t split_parts(const Plan& p)
+{
+    const size_t pix = (size_t)p.tc.tileW * p.tc.tileH;
+    return (uint32_t)std::min<size_t>(16384, (64u << 20) / (pix * (sizeof(float) + sizeof(PartPixel))));
+}
+
+// The heavy-tile split applies to a batch whose draws merge exactly
+// (RasterParams::splitClass): all unblended, one depth function, a float
+// depth target, and depth writes on with a min- or max-type function or off.
+uint32_t split_class(const Batch& B, const Plan& p, bool all)
+{
+    if (B.draws.empty() || p.dt->type != SL_COLOR_R_FLOAT) return kSplitNone;
+    // by default only batches with point or line draws, whose records pile
+    // onto few tiles (a distant mesh's vertices); triangle batches when asked
+    bool prims = false;
+    for (const DrawDesc& d : B.draws) prims = prims || d.prim == kPrimPoint || d.prim == kPrimLine;
+    if (!all && !prims) return kSplitNone;
+    const DrawDesc& d0 = B.draws[0];
+    for (const DrawDesc& d : B.draws)
+        if (d.blend != SL_BLEND_OFF || d.dtest != d0.dtest || d.dmask != d0.dmask) return kSplitNone;
+    if (!d0.dmask) return kSplitConst;
+    if (d0.dtest == SL_DEPTH_TEST_LESS_THAN || d0.dtest == SL_DEPTH_TEST_LESS_EQUAL) return kSplitMin;
+    if (d0.dtest == SL_DEPTH_TEST_GREATER_THAN || d0.dtest == SL_DEPTH_TEST_GREATER_EQUAL) return kSplitMax;
+    return kSplitNone;
+}
+
 // Allocates a frame's buffers at the resource set's current capacities.
 int ensure_frame(sl_context* c, FrameRes& R, const Plan& p, const Batch& B)
 {
@@ -440,6 +475,17 @@ int ensure_frame(sl_context* c, FrameRes& R, const Plan& p, const Batch& B)
     if ((rc = ensure(c, R.clip, sizeof(ClipJob) * (size_t)R.clipCap)) != SL_OK) return rc;
     if ((rc = ensure(c, R.binCounts, sizeof(uint32_t) * (size_t)kMaxBinBlocks * p.numTiles)) != SL_OK) return rc;
     if ((rc = ensure(c, R.binTotals, sizeof(uint32_t) * (size_t)p.numTiles)) != SL_OK) return rc;
+    if (c->splitRecs > 0)
+    {
+        const uint32_t parts = split_parts(p);
+        const size_t pix = (size_t)p.tc.tileW * p.tc.tileH;
+        if ((rc = ensure(c, R.items, sizeof(RasterItem) * ((size_t)p.numTiles + parts))) != SL_OK) return rc;
+        if ((rc = ensure(c, R.itemCount, 2 * sizeof(uint32_t))) != SL_OK) return rc;
+        if ((rc = ensure(c, R.partDepth, sizeof(float) * pix * parts)) != SL_OK) return rc;
+        if ((rc = ensure(c, R.partRes, sizeof(PartPixel) * pix * parts)) != SL_OK) return rc;
+        if ((rc = ensure(c, R.partDone, sizeof(uint32_t) * parts)) != SL_OK) return rc;
+        R.maxParts = parts;
+    }
     return SL_OK;
 }
 
@@ -472,7 +518,7 @@ SetupParams setup_params(const Plan& p, const Batch& B, FrameRes& R)
     return sp;
 }
 
-RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool haveEntries)
+RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool haveEntries, bool splitAll)
 {
     RasterParams rp{};
     rp.draws = reinterpret_cast<const DrawDesc*>(R.draws.p);
@@ -519,6 +565,27 @@ RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool have
         rp.mrt |= (d.nout > 1) ? 1 : 0;
     }
     rp.ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
+    rp.items = nullptr;
+    rp.itemCount = nullptr;
+    rp.partDepth = nullptr;
+    rp.partRes = nullptr;
+    rp.partDone = nullptr;
+    rp.splitClass = kSplitNone;
+    rp.splitDtest = 0;
+    if (haveEntries && R.items.p != nullptr && R.maxParts > 0)
+    {
+        const uint32_t cls = split_class(B, p, splitAll);
+        if (cls != kSplitNone)
+        {
+            rp.items = reinterpret_cast<const RasterItem*>(R.items.p);
+            rp.itemCount = reinterpret_cast<const uint32_t*>(R.itemCount.p);
+            rp.partDepth = reinterpret_cast<float*>(R.partDepth.p);
+            rp.partRes = reinterpret_cast<PartPixel*>(R.partRes.p);
+            rp.partDone = reinterpret_cast<uint32_t*>(R.partDone.p);
+            rp.splitClass = cls;
+            rp.splitDtest = B.draws[0].dtest;
+        }
+    }
     return rp;
 }
 
@@ -554,6 +621,7 @@ int enqueue_back(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, uint
                  hipStream_t s, hipEvent_t evClip0, hipEvent_t evClip1, hipEvent_t evBin1)
 {
     int rc;
+    const RasterParams rp = raster_params(p, B, R, B.totalTris > 0 && entries > 0, c->splitAll);
     if (B.totalTris > 0)
     {
         const SetupParams sp = setup_params(p, B, R);
@@ -562,14 +630,27 @@ int enqueue_back(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, uint
         if (evClip1) (void)hipEventRecord(evClip1, s);
         if (entries > 0)
         {
+            // with the heavy-tile split on, the tile scan also writes the
+            // raster's work items
+            const uint32_t partRecs = std::max<uint32_t>(c->splitRecs, 1u);
             if ((rc = check(c, launch_binning(sp.pairKeys, sp.pairVals, sp.ctrl, R.pairCap, bin_plan(entries),
                                               reinterpret_cast<uint32_t*>(R.binCounts.p), reinterpret_cast<uint32_t*>(R.binTotals.p),
                                               reinterpret_cast<uint2*>(R.ranges.p), reinterpret_cast<uint32_t*>(R.sorted.p),
-                                              reinterpret_cast<uint32_t*>(R.order.p), p.numTiles, s), "binning")) != SL_OK) return rc;
+                                              reinterpret_cast<uint32_t*>(R.order.p), p.numTiles, s,
+                                              const_cast<RasterItem*>(rp.items), const_cast<uint32_t*>(rp.itemCount), rp.partDone,
+                                              partRecs, kSplitAboveParts * partRecs, R.maxParts), "binning")) != SL_OK) return rc;
         }
     }
     if (evBin1) (void)hipEventRecord(evBin1, s);
-    const RasterParams rp = raster_params(p, B, R, B.totalTris > 0 && entries > 0);
+    if (rp.items != nullptr)
+    {
+        // heavy tiles: the first pass over split tiles' parts, then one
+        // workgroup per work item (capacity-sized grids: the counts stay on
+        // the device)
+        if (rp.splitClass != kSplitConst && (rc = check(c, launch_raster_depth(rp, (int)R.maxParts, p.cfg, s), "k_raster depth pass")) != SL_OK)
+            return rc;
+        return check(c, launch_raster(rp, p.numTiles + (int)R.maxParts, p.cfg, p.debug, s), "k_raster");
+    }
     return check(c, launch_raster(rp, p.numTiles, p.cfg, p.debug, s), "k_raster");
 }
 
@@ -848,7 +929,8 @@ void graph_release(sl_context* c)
     G.exec = nullptr;
     G.graph = nullptr;
     for (DevBuf* b : {&G.res.draws, &G.res.uni, &G.res.recs, &G.res.vars, &G.res.keys, &G.res.vals, &G.res.sorted,
-                      &G.res.clip, &G.res.status, &G.res.ranges, &G.res.ctrl, &G.res.binCounts, &G.res.binTotals, &G.res.order}) release(*b);
+                      &G.res.clip, &G.res.status, &G.res.ranges, &G.res.ctrl, &G.res.binCounts, &G.res.binTotals, &G.res.order,
+                      &G.res.items, &G.res.itemCount, &G.res.partDepth, &G.res.partRes, &G.res.partDone}) release(*b);
     G.res = FrameRes{};
     if (G.hCtrl) (void)hipHostFree(G.hCtrl);
     G.hCtrl = nullptr;
@@ -1246,7 +1328,8 @@ void sl_context_destroy(sl_context* c)
     for (auto& t : c->texs) release(t.buf);
     graph_release(c);
     for (DevBuf* b : {&c->res.draws, &c->res.uni, &c->res.recs, &c->res.vars, &c->res.keys, &c->res.vals, &c->res.sorted,
-                      &c->res.clip, &c->res.status, &c->res.ranges, &c->res.ctrl, &c->res.binCounts, &c->res.binTotals, &c->res.order}) release(*b);
+                      &c->res.clip, &c->res.status, &c->res.ranges, &c->res.ctrl, &c->res.binCounts, &c->res.binTotals, &c->res.order,
+                      &c->res.items, &c->res.itemCount, &c->res.partDepth, &c->res.partRes, &c->res.partDone}) release(*b);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     for (auto& f : c->fbos) release(f.gradDev);
     for (auto& e : c->bwdEv) if (e) (void)hipEventDestroy(e);
@@ -1274,6 +1357,16 @@ int sl_set_raster_config(sl_context* c, int config)
     const int rc = flush(c);
     c->tileConfig = config;
     return rc;
+}
+
+int sl_set_raster_split(sl_context* c, uint32_t partRecords)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    const int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    c->splitRecs = partRecords;
+    c->splitAll = partRecords > 0;
+    return SL_OK;
 }
 
 int sl_get_raster_config(const sl_context* c, int config, int* tileW, int* tileH, int* segments)
