@@ -1203,6 +1203,57 @@ void divide_and_snap(V4& p, const float vp[4])
     p.v[3] = wInv;
 }
 
+// The polygon loop of SL_TriProcessor::clip_and_process_tris (:370-499):
+// clips the triangle nverts[0..2] (with nv varyings per vertex) against the
+// six planes in place, visiting the planes in the reference's order; returns
+// the polygon's size, 0 when a plane leaves nothing.
+unsigned clip_polygon(V4 nverts[9], V4 nvar[9][SL_SHADER_MAX_VARYING_VECTORS], unsigned nv)
+{
+    const V4 edges[6] = {
+        v4(1.f, 0.f, 0.f, 1.f), v4(-1.f, 0.f, 0.f, 1.f),
+        v4(0.f, 1.f, 0.f, 1.f), v4(0.f, -1.f, 0.f, 1.f),
+        v4(0.f, 0.f, 1.f, 1.f), v4(0.f, 0.f, -1.f, 1.f)};
+    unsigned n = 3;
+    V4 tverts[9];
+    V4 tvar[9][SL_SHADER_MAX_VARYING_VECTORS];
+    for (const V4& edge : edges)
+    {
+        unsigned nn = 0;
+        unsigned j = n - 1;
+        V4 q0 = nverts[n - 1];
+        float t0 = dot4(q0, edge);
+        bool vis0 = !std::signbit(t0);
+        for (unsigned k = 0; k < n; ++k)
+        {
+            const V4 q1 = nverts[k];
+            const float t1 = dot4(q1, edge);
+            const bool vis1 = !std::signbit(t1);
+            if (vis0 != vis1)
+            {
+                const float t = t0 * rcpf(t0 - t1);
+                tverts[nn] = mix4(q0, q1, t);
+                for (unsigned i = 0; i < nv; ++i) tvar[nn][i] = mix4(nvar[j][i], nvar[k][i], t);
+                ++nn;
+            }
+            if (vis1)
+            {
+                tverts[nn] = q1;
+                for (unsigned i = 0; i < nv; ++i) tvar[nn][i] = nvar[k][i];
+                ++nn;
+            }
+            j = k; q0 = q1; t0 = t1; vis0 = vis1;
+        }
+        if (!nn) return 0;
+        n = nn;
+        for (unsigned k = 0; k < n; ++k)
+        {
+            nverts[k] = tverts[k];
+            for (unsigned i = 0; i < nv; ++i) nvar[k][i] = tvar[k][i];
+        }
+    }
+    return n;
+}
+
 struct VertexStage
 {
     Draw& d;
@@ -1266,51 +1317,11 @@ struct VertexStage
     void clip_and_process(uint32_t seq, const TVert& a, const TVert& b, const TVert& c)
     {
         const unsigned nv = d.nvary;
-        const V4 edges[6] = {
-            v4(1.f, 0.f, 0.f, 1.f), v4(-1.f, 0.f, 0.f, 1.f),
-            v4(0.f, 1.f, 0.f, 1.f), v4(0.f, -1.f, 0.f, 1.f),
-            v4(0.f, 0.f, 1.f, 1.f), v4(0.f, 0.f, -1.f, 1.f)};
-        unsigned n = 3;
-        V4 nverts[9], tverts[9];
-        V4 nvar[9][SL_SHADER_MAX_VARYING_VECTORS], tvar[9][SL_SHADER_MAX_VARYING_VECTORS];
+        V4 nverts[9];
+        V4 nvar[9][SL_SHADER_MAX_VARYING_VECTORS];
         nverts[0] = a.pos; nverts[1] = b.pos; nverts[2] = c.pos;
         for (unsigned i = 0; i < nv; ++i) { nvar[0][i] = a.var[i]; nvar[1][i] = b.var[i]; nvar[2][i] = c.var[i]; }
-
-        for (const V4& edge : edges)
-        {
-            unsigned nn = 0;
-            unsigned j = n - 1;
-            V4 q0 = nverts[n - 1];
-            float t0 = dot4(q0, edge);
-            bool vis0 = !std::signbit(t0);
-            for (unsigned k = 0; k < n; ++k)
-            {
-                const V4 q1 = nverts[k];
-                const float t1 = dot4(q1, edge);
-                const bool vis1 = !std::signbit(t1);
-                if (vis0 != vis1)
-                {
-                    const float t = t0 * rcpf(t0 - t1);
-                    tverts[nn] = mix4(q0, q1, t);
-                    for (unsigned i = 0; i < nv; ++i) tvar[nn][i] = mix4(nvar[j][i], nvar[k][i], t);
-                    ++nn;
-                }
-                if (vis1)
-                {
-                    tverts[nn] = q1;
-                    for (unsigned i = 0; i < nv; ++i) tvar[nn][i] = nvar[k][i];
-                    ++nn;
-                }
-                j = k; q0 = q1; t0 = t1; vis0 = vis1;
-            }
-            if (!nn) return;
-            n = nn;
-            for (unsigned k = 0; k < n; ++k)
-            {
-                nverts[k] = tverts[k];
-                for (unsigned i = 0; i < nv; ++i) nvar[k][i] = tvar[k][i];
-            }
-        }
+        const unsigned n = clip_polygon(nverts, nvar, nv);
         if (n < 3) return;
 
         for (unsigned k = 0; k < n; ++k) divide_and_snap(nverts[k], d.vpDims);
@@ -2510,29 +2521,54 @@ int slo_loss_l2_grad(sl_context* c, size_t colorTex, size_t targetTex, size_t gr
 } // extern "C"
 
 /*-----------------------------------------------------------------------------
- * Smoothed span coverage (no reference counterpart; the differentiable-
- * coverage mode).  Per row, where the visibility winners of pixels x-1 and x
- * differ and a span end e of one of them (the float SL_ScanlineBounds::step
- * value before truncation, SL_ScanlineBounds.hpp:310-322) lies in [x, x + 1),
- * out(x) = (1 - w) C(x) + w C(x-1), w = exp(-(x + 1 - e) / sigma): the right
- * end of the left pixel's triangle if it lies there, else the left end of the
- * right pixel's.  Clipped triangles' boundaries are not smoothed; sigma = 0
- * returns the colour image.  The adjoint holds the winners fixed and sums in
- * double.
+ * Smoothed coverage (no reference counterpart; the differentiable-coverage
+ * mode).  Pixel (x, y) samples the point (x + 1, y): the span rule covers it
+ * when e_min < x + 1 <= e_max at row y (SL_ScanlineBounds.hpp:310-322).
+ * Where the visibility winners of two neighbouring pixels differ and an edge
+ * of one of the winners separates their samples, the colour before the edge
+ * leaks into the pixel past it with the weight
+ *     w(d) = 2 s(-d / sigma) = 1 - d / sqrt(d^2 + sigma^2),
+ * s(u) = (1 + u / sqrt(1 + u^2)) / 2 the algebraic sigmoid and d > 0 the
+ * perpendicular distance of the receiving pixel's sample from the edge's
+ * line.  w is 1 on the edge, so the image stays continuous where a pixel's
+ * winner changes, and goes to 0 for every d > 0 as sigma -> 0; its tail
+ * falls as (sigma / d)^2 / 2, so the weight and its derivative stay well
+ * inside float range for sigma far below the pixel spacing.  Two separable
+ * passes over the hard colour image C:
+ *   X(x, y) = (1 - wx) C(x, y) + wx C(x - 1, y): the edge between the samples
+ *     of x - 1 and x is a row-span end in [x, x + 1) -- the right end of the
+ *     left pixel's winner if it lies there, else the left end of the right
+ *     pixel's;
+ *   O(x, y) = X(x, y) + sum wy (X(x, y') - X(x, y)) over the column edges
+ *     that leak the neighbour row y' into the pixel: the edge between the
+ *     samples of rows y and y + 1 in column x + 1 is a column-span end v in
+ *     [y, y + 1] -- the high end of row y's winner if it lies there, else the
+ *     low end of row y + 1's.  Row y receives (distance v - y) unless its
+ *     sample lies on the edge (v = y, a tie the span rule gave to row y);
+ *     then row y + 1 receives (distance 1).  The row pass needs no such
+ *     rule: the span rule puts every tie on the left pixel of a pair.
+ * A clipped winner is the polygon the clipper leaves: its row spans are the
+ * union of the fan pieces the raster drew, its column spans come from its
+ * edges; a vertex the clipper made has no offset and takes no position
+ * gradient (it has no screen position of its own), the edge's other end
+ * does.  sigma = 0 returns C.  The adjoint holds the winners fixed and sums
+ * in double.
  *---------------------------------------------------------------------------*/
-struct OSpan
+struct OShape
 {
     bool ok = false;
+    bool clipped = false;
     size_t k = 0;             // draw call of the framebuffer's saved batch
-    size_t vid[3] = {0, 0, 0};
-    float sx[3], sy[3];       // snapped screen positions plus the offsets
-    float lo = 0.f, hi = 0.f; // span ends at the row
-    int loP = 0, loQ = 0, hiP = 0, hiQ = 0;   // vertex slots of their edges
+    int n = 0;                // vertices: the triangle's 3 or the clipped polygon's
+    float sx[9], sy[9];       // snapped screen positions plus the offsets
+    float rx[9], ry[9];       // snapped screen positions as rasterised
+    size_t vid[9];
+    bool orig[9];             // a vertex of the triangle, not one the clipper made
 };
 
-OSpan winner_span(sl_context* c, const Fbo& f, uint32_t seq, int y, const float* offsets)
+OShape winner_shape(sl_context* c, const Fbo& f, uint32_t seq, const float* offsets)
 {
-    OSpan S;
+    OShape S;
     if (seq == 0 || f.saved.empty()) return S;
     size_t lo = 0, hi = f.saved.size();
     while (hi - lo > 1)
@@ -2562,95 +2598,264 @@ OSpan winner_span(sl_context* c, const Fbo& f, uint32_t seq, int y, const float*
         V4 var[SL_SHADER_MAX_VARYING_VECTORS];
         pos[k] = mat_mul(d.scissorMat, run_vertex_program(d, vid[k], sp.instance, var));
     }
-    if (clip_status(pos[0], pos[1], pos[2]) != 3) return S;
+    const int status = clip_status(pos[0], pos[1], pos[2]);
+    V4 poly[9];
+    for (int k = 0; k < 3; ++k) poly[k] = pos[k];
+    int n = 3;
+    if (status == 1)
+    {
+        V4 var[9][SL_SHADER_MAX_VARYING_VECTORS];
+        n = (int)clip_polygon(poly, var, 0);
+        if (n < 3) return S;
+        S.clipped = true;
+    }
+    else if (status != 3)
+    {
+        return S;
+    }
     S.k = sp.drawIdx;
-    for (int k = 0; k < 3; ++k)
+    S.n = n;
+    for (int k = 0; k < n; ++k)
     {
-        divide_and_snap(pos[k], d.vpDims);
+        // the clipper copies the vertices it keeps unchanged
+        int from = -1;
+        for (int j = 0; j < 3 && from < 0; ++j)
+            if (std::memcmp(&poly[k], &pos[j], sizeof(V4)) == 0) from = j;
+        V4 p = poly[k];
+        divide_and_snap(p, d.vpDims);
+        S.rx[k] = p.v[0];
+        S.ry[k] = p.v[1];
+        S.orig[k] = from >= 0;
+        S.vid[k] = from >= 0 ? vid[from] : 0;
         float ox = 0.f, oy = 0.f;
-        if (offsets && vid[k] < f.posVerts)
+        if (offsets && from >= 0 && vid[from] < f.posVerts)
         {
-            ox = offsets[2 * (S.k * f.posVerts + vid[k])];
-            oy = offsets[2 * (S.k * f.posVerts + vid[k]) + 1];
+            ox = offsets[2 * (S.k * f.posVerts + vid[from])];
+            oy = offsets[2 * (S.k * f.posVerts + vid[from]) + 1];
         }
-        S.sx[k] = pos[k].v[0] + ox;
-        S.sy[k] = pos[k].v[1] + oy;
-        S.vid[k] = vid[k];
-    }
-    // vertices by y descending with SL_ScanlineBounds::init's swaps, then :310-322
-    int a = 0, b = 1, cc = 2;
-    if (S.sy[a] < S.sy[b]) std::swap(a, b);
-    if (S.sy[a] < S.sy[cc]) std::swap(a, cc);
-    if (S.sy[b] < S.sy[cc]) std::swap(b, cc);
-    const float ax = S.sx[a], ay = S.sy[a], bx = S.sx[b], by = S.sy[b], cx = S.sx[cc], cy = S.sy[cc];
-    const float yf = (float)y;
-    const float d0 = yf - ay, d1 = yf - by;
-    const float alpha = d0 * rcpf(cy - ay);
-    S.lo = ((cx - ax) * alpha) + ax;
-    S.loP = a;
-    S.loQ = cc;
-    if (d1 < 0.f)
-    {
-        S.hi = (((cx - bx) * rcpf(cy - by)) * d1) + bx;
-        S.hiP = b;
-        S.hiQ = cc;
-    }
-    else
-    {
-        S.hi = (((bx - ax) * rcpf(by - ay)) * d0) + ax;
-        S.hiP = a;
-        S.hiQ = b;
+        S.sx[k] = p.v[0] + ox;
+        S.sy[k] = p.v[1] + oy;
     }
     S.ok = true;
     return S;
 }
 
-struct OBoundary { bool ok = false; float e = 0.f; OSpan S; int p = 0, q = 0; };
+// A span end and the edge (two vertex slots of the shape) it lies on.
+struct OEnd { float e = 0.f; int p = 0, q = 0; };
 
-OBoundary boundary_at(sl_context* c, const Fbo& f, const float* vis, int W, int x, int y, const float* offsets)
+// Which edge bounds a winner is decided on the positions as rasterised (the
+// same float arithmetic as the raster, so the choice agrees with the hard
+// winners, ties included); only the weights see the offsets.
+//
+// One triangle's span ends at row y with SL_ScanlineBounds::init's vertex
+// order and :310-322's arithmetic: the left and the right end.
+void tri_row_ends(const float* X, const float* Y, const int id[3], float yf, OEnd& left, OEnd& right)
 {
-    OBoundary B;
-    if (x <= 0 || x >= W) return B;
-    uint32_t sl, sr;
-    std::memcpy(&sl, &vis[4 * ((size_t)y * W + x - 1)], 4);
-    std::memcpy(&sr, &vis[4 * ((size_t)y * W + x)], 4);
-    if (sl == sr) return B;
+    int a = id[0], b = id[1], cc = id[2];
+    if (Y[a] < Y[b]) std::swap(a, b);
+    if (Y[a] < Y[cc]) std::swap(a, cc);
+    if (Y[b] < Y[cc]) std::swap(b, cc);
+    const float ax = X[a], ay = Y[a], bx = X[b], by = Y[b], cx = X[cc], cy = Y[cc];
+    const float d0 = yf - ay, d1 = yf - by;
+    const float alpha = d0 * rcpf(cy - ay);
+    OEnd lo{((cx - ax) * alpha) + ax, a, cc}, hi;
+    if (d1 < 0.f) hi = OEnd{(((cx - bx) * rcpf(cy - by)) * d1) + bx, b, cc};
+    else hi = OEnd{(((bx - ax) * rcpf(by - ay)) * d0) + ax, a, b};
+    const bool loIsMax = lo.e >= hi.e;
+    right = loIsMax ? lo : hi;
+    left = loIsMax ? hi : lo;
+}
+
+// The shape's row span at y; a clipped polygon's is the union of the fan
+// pieces (0, j + 1, j + 2) the raster drew whose rows hold y.
+bool row_ends(const OShape& S, int y, OEnd& left, OEnd& right)
+{
+    const float yf = (float)y;
+    if (!S.clipped)
+    {
+        const int id[3] = {0, 1, 2};
+        tri_row_ends(S.rx, S.ry, id, yf, left, right);
+        return true;
+    }
+    bool any = false;
+    for (int j = S.n - 3; j >= 0; --j)
+    {
+        const int id[3] = {0, j + 1, j + 2};
+        const float minX = std::min(std::min(S.rx[id[0]], S.rx[id[1]]), S.rx[id[2]]);
+        const float maxX = std::max(std::max(S.rx[id[0]], S.rx[id[1]]), S.rx[id[2]]);
+        const float minY = std::min(std::min(S.ry[id[0]], S.ry[id[1]]), S.ry[id[2]]);
+        const float maxY = std::max(std::max(S.ry[id[0]], S.ry[id[1]]), S.ry[id[2]]);
+        if ((maxX - minX < 1.f) || (maxY - minY < 1.f)) continue;      // push_bin's reject
+        if (y < (int)minY || y >= (int)maxY) continue;
+        OEnd l, r;
+        tri_row_ends(S.rx, S.ry, id, yf, l, r);
+        if (!any || l.e < left.e) left = l;
+        if (!any || r.e > right.e) right = r;
+        any = true;
+    }
+    return any;
+}
+
+// The crossing of edge (p, q) with the line b = c, along a:
+// Pa + ((Qa - Pa) (c - Pb)) / (Qb - Pb), exact for vertices on the pixel
+// lattice whenever the crossing is.
+float edge_crossing(float pa, float pb, float qa, float qb, float c)
+{
+    return pa + ((qa - pa) * (c - pb)) / (qb - pb);
+}
+
+// The shape's column span at u: the lowest and highest crossing of its edges
+// with the vertical line x = u.
+bool col_ends(const OShape& S, float u, OEnd& low, OEnd& high)
+{
+    bool any = false;
+    for (int i = 0; i < S.n; ++i)
+    {
+        const int p = i, q = (i + 1 == S.n) ? 0 : i + 1;
+        const float px = S.rx[p], qx = S.rx[q];
+        if (px == qx || u < std::min(px, qx) || u > std::max(px, qx)) continue;
+        const float v = edge_crossing(S.ry[p], px, S.ry[q], qx, u);
+        if (!any || v < low.e) low = OEnd{v, p, q};
+        if (!any || v > high.e) high = OEnd{v, p, q};
+        any = true;
+    }
+    return any;
+}
+
+// A smoothed boundary: the edge (P, Q) with the offsets, in pass coordinates
+// -- a along the pass (x for the row pass, y for the column pass), b across
+// it -- and the receiving sample (a = sample, b = cross).  The distance along
+// a is dir (sample - crossing): positive on the rasterised positions, and a
+// smooth function of the offsets (negative values extrapolate the weight).
+struct OBnd
+{
+    bool ok = false;
+    bool gp = false, gq = false;   // the end is a vertex of the triangle (has an offset and a gradient)
+    bool recvLow = false;     // column pass: row y receives (else row y + 1)
+    float sample = 0.f, cross = 0.f, dir = 1.f;
+    float pa = 0.f, pb = 0.f, qa = 0.f, qb = 0.f;
+    size_t k = 0, vp = 0, vq = 0;
+};
+
+OBnd make_bnd(const OShape& S, const OEnd& E, bool rowPass, float sample, float cross, float dir)
+{
+    OBnd B;
+    B.ok = true;
+    B.gp = S.orig[E.p];
+    B.gq = S.orig[E.q];
+    B.sample = sample;
+    B.cross = cross;
+    B.dir = dir;
+    B.pa = rowPass ? S.sx[E.p] : S.sy[E.p];
+    B.pb = rowPass ? S.sy[E.p] : S.sx[E.p];
+    B.qa = rowPass ? S.sx[E.q] : S.sy[E.q];
+    B.qb = rowPass ? S.sy[E.q] : S.sx[E.q];
+    B.k = S.k;
+    B.vp = S.vid[E.p];
+    B.vq = S.vid[E.q];
+    return B;
+}
+
+uint32_t vis_seq(const float* vis, int W, int x, int y)
+{
+    uint32_t s;
+    std::memcpy(&s, &vis[4 * ((size_t)y * W + x)], 4);
+    return s;
+}
+
+// The row-pass boundary between pixels x - 1 and x of row y: pixel x receives.
+OBnd boundary_x(sl_context* c, const Fbo& f, const float* vis, int W, int x, int y, const float* offsets)
+{
+    if (x <= 0 || x >= W) return OBnd{};
+    const uint32_t sl = vis_seq(vis, W, x - 1, y), sr = vis_seq(vis, W, x, y);
+    if (sl == sr) return OBnd{};
+    const float lo = (float)x, hi = (float)(x + 1), yf = (float)y;
+    OEnd l, r;
     if (sl != 0)
     {
-        const OSpan L = winner_span(c, f, sl, y, offsets);
-        if (L.ok)
-        {
-            const bool loIsMax = L.lo >= L.hi;
-            const float em = loIsMax ? L.lo : L.hi;
-            if (em >= (float)x && em < (float)(x + 1))
-            {
-                B.ok = true;
-                B.e = em;
-                B.S = L;
-                B.p = loIsMax ? L.loP : L.hiP;
-                B.q = loIsMax ? L.loQ : L.hiQ;
-                return B;
-            }
-        }
+        const OShape L = winner_shape(c, f, sl, offsets);
+        if (L.ok && row_ends(L, y, l, r) && r.e >= lo && r.e < hi) return make_bnd(L, r, true, hi, yf, 1.f);
     }
     if (sr != 0)
     {
-        const OSpan R = winner_span(c, f, sr, y, offsets);
-        if (R.ok)
-        {
-            const bool loIsMax = R.lo >= R.hi;
-            const float en = loIsMax ? R.hi : R.lo;
-            if (en >= (float)x && en < (float)(x + 1))
-            {
-                B.ok = true;
-                B.e = en;
-                B.S = R;
-                B.p = loIsMax ? R.hiP : R.loP;
-                B.q = loIsMax ? R.hiQ : R.loQ;
-            }
-        }
+        const OShape R = winner_shape(c, f, sr, offsets);
+        if (R.ok && row_ends(R, y, l, r) && l.e >= lo && l.e < hi) return make_bnd(R, l, true, hi, yf, 1.f);
     }
+    return OBnd{};
+}
+
+// The column-pass boundary between rows y and y + 1 of column x (sample
+// x + 1): row y receives unless its sample lies on the edge.
+OBnd col_bnd(const OShape& S, const OEnd& E, int y, int x)
+{
+    const float yl = (float)y, u = (float)(x + 1);
+    const bool low = E.e > yl;
+    OBnd B = make_bnd(S, E, false, low ? yl : yl + 1.f, u, low ? -1.f : 1.f);
+    B.recvLow = low;
     return B;
+}
+
+OBnd boundary_y(sl_context* c, const Fbo& f, const float* vis, int W, int H, int x, int y, const float* offsets)
+{
+    if (y < 0 || y + 1 >= H) return OBnd{};
+    const uint32_t sa = vis_seq(vis, W, x, y), sb = vis_seq(vis, W, x, y + 1);
+    if (sa == sb) return OBnd{};
+    const float lo = (float)y, hi = (float)(y + 1), u = (float)(x + 1);
+    OEnd l, h;
+    if (sa != 0)
+    {
+        const OShape A = winner_shape(c, f, sa, offsets);
+        if (A.ok && col_ends(A, u, l, h) && h.e >= lo && h.e <= hi) return col_bnd(A, h, y, x);
+    }
+    if (sb != 0)
+    {
+        const OShape B = winner_shape(c, f, sb, offsets);
+        if (B.ok && col_ends(B, u, l, h) && l.e >= lo && l.e <= hi) return col_bnd(B, l, y, x);
+    }
+    return OBnd{};
+}
+
+// Signed perpendicular distance of the receiving sample from the edge's line:
+// the distance along a times |sin| of the edge against a.
+float bnd_distance(const OBnd& B)
+{
+    // sample - crossing, measured from P so that no large coordinates cancel
+    const float daxis = B.dir * ((B.sample - B.pa) - ((B.qa - B.pa) * (B.cross - B.pb)) / (B.qb - B.pb));
+    const float ea = B.qa - B.pa, eb = B.qb - B.pb;
+    const float len = std::sqrt(std::fma(ea, ea, eb * eb));
+    return daxis * (std::fabs(eb) / len);
+}
+
+// w(d) = 1 - d / r, r = sqrt(d^2 + sigma^2); for d >= 0 written as
+// sigma^2 / (r (r + d)) so that it keeps its precision where it is small.
+float smooth_weight(float d, float sigma)
+{
+    const float r = std::sqrt(std::fma(d, d, sigma * sigma));
+    return d >= 0.f ? (sigma * sigma) / (r * (r + d)) : 1.f - d / r;
+}
+
+float bnd_weight(const OBnd& B, float sigma) { return B.ok ? smooth_weight(bnd_distance(B), sigma) : 0.f; }
+
+// In double: the signed distance and its partials d/d(Pa, Pb, Qa, Qb), with
+// the crossing ea = Pa + (Qa - Pa) t, t = (c - Pb) / (Qb - Pb), and
+// d = dir (sample - ea) |Qb - Pb| / |Q - P|.
+double bnd_partials(const OBnd& B, double out[4])
+{
+    const double pa = B.pa, pb = B.pb, qa = B.qa, qb = B.qb, dir = B.dir, c = B.cross;
+    const double inv = 1.0 / (qb - pb);
+    const double t = (c - pb) * inv, s = (qa - pa) * inv;
+    const double da = dir * (((double)B.sample - pa) - (qa - pa) * t);
+    const double ePa = dir * (1.0 - t), eQa = dir * t, ePb = -dir * (1.0 - t) * s, eQb = -dir * t * s;
+    const double ea = qa - pa, eb = qb - pb;
+    const double len2 = ea * ea + eb * eb, len = std::sqrt(len2), len3 = len2 * len;
+    const double g = std::fabs(eb) / len, sg = eb > 0.0 ? 1.0 : -1.0;
+    const double gQa = -std::fabs(eb) * ea / len3, gPa = -gQa;
+    const double gQb = sg * ea * ea / len3, gPb = -gQb;
+    out[0] = -ePa * g + da * gPa;
+    out[1] = -ePb * g + da * gPb;
+    out[2] = -eQa * g + da * gQa;
+    out[3] = -eQb * g + da * gQb;
+    return da * g;
 }
 
 // checks shared by both smoothing entry points; fills the images' data
@@ -2711,37 +2916,40 @@ int slo_smooth_coverage(sl_context* c, size_t fbo, size_t outTex, float sigma)
     int rc = smooth_check(c, fbo, colId, outTex, sigma, F, col, vis, in, out, offsets, false);
     if (rc != SL_OK) return rc;
     const int W = vis->w, H = vis->h;
+    const size_t n = (size_t)W * H;
     const float* C = reinterpret_cast<const float*>(col->data.data());
     const float* V = reinterpret_cast<const float*>(vis->data.data());
     float* O = reinterpret_cast<float*>(out->data.data());
+    if (!(sigma > 0.f) || F->saved.empty())
+    {
+        std::memcpy(O, C, n * 16);
+        return SL_OK;
+    }
+    // row pass into X, then the column pass from X
+    std::vector<float> X(n * 4);
     for (int y = 0; y < H; ++y)
         for (int x = 0; x < W; ++x)
         {
             const size_t i = (size_t)y * W + x;
-            float o[4] = {C[4 * i], C[4 * i + 1], C[4 * i + 2], C[4 * i + 3]};
-            if (sigma > 0.f && !F->saved.empty())
+            const OBnd B = boundary_x(c, *F, V, W, x, y, offsets);
+            for (int ch = 0; ch < 4; ++ch) X[4 * i + ch] = C[4 * i + ch];
+            if (B.ok)
             {
-                const OBoundary B = boundary_at(c, *F, V, W, x, y, offsets);
-                if (B.ok)
-                {
-                    const float w = std::exp(-((float)(x + 1) - B.e) / sigma);
-                    const float u = 1.f - w;
-                    for (int ch = 0; ch < 4; ++ch) o[ch] = (u * C[4 * i + ch]) + (w * C[4 * (i - 1) + ch]);
-                }
+                const float w = bnd_weight(B, sigma), u = 1.f - w;
+                for (int ch = 0; ch < 4; ++ch) X[4 * i + ch] = (u * C[4 * i + ch]) + (w * C[4 * (i - 1) + ch]);
             }
-            std::memcpy(&O[4 * i], o, 16);
+        }
+    for (size_t i = 0; i < n * 4; ++i) O[i] = X[i];
+    for (int y = 0; y + 1 < H; ++y)
+        for (int x = 0; x < W; ++x)
+        {
+            const OBnd B = boundary_y(c, *F, V, W, H, x, y, offsets);
+            if (!B.ok) continue;
+            const float w = bnd_weight(B, sigma);
+            const size_t r = (size_t)(B.recvLow ? y : y + 1) * W + x, s = (size_t)(B.recvLow ? y + 1 : y) * W + x;
+            for (int ch = 0; ch < 4; ++ch) O[4 * r + ch] += w * (X[4 * s + ch] - X[4 * r + ch]);
         }
     return SL_OK;
-}
-
-// e = Px + (Qx - Px) t, t = (y - Py) / (Qy - Py): partials in double
-static void edge_partials(double px, double py, double qx, double qy, double y, double& dPx, double& dPy, double& dQx, double& dQy)
-{
-    const double t = (y - py) / (qy - py), s = (qx - px) / (qy - py);
-    dPx = 1.0 - t;
-    dQx = t;
-    dPy = -(1.0 - t) * s;
-    dQy = -t * s;
 }
 
 int slo_smooth_coverage_backward(sl_context* c, size_t fbo, size_t gradSmoothTex, float sigma, size_t gradImageTex)
@@ -2750,59 +2958,158 @@ int slo_smooth_coverage_backward(sl_context* c, size_t fbo, size_t gradSmoothTex
     int rc = smooth_check(c, fbo, gradSmoothTex, gradImageTex, sigma, F, col, vis, in, out, offsets, true);
     if (rc != SL_OK) return rc;
     const int W = vis->w, H = vis->h;
+    const size_t n = (size_t)W * H;
     const float* C = reinterpret_cast<const float*>(col->data.data());
     const float* V = reinterpret_cast<const float*>(vis->data.data());
     const float* G = reinterpret_cast<const float*>(in->data.data());
     float* O = reinterpret_cast<float*>(out->data.data());
-    const bool smooth = sigma > 0.f && !F->saved.empty();
-    std::vector<double> pg;
-    if (F->posGrad != SL_INVALID_ID) pg.assign((size_t)F->savedDraws * F->posVerts * 2, 0.0);
+    if (!(sigma > 0.f) || F->saved.empty())
+    {
+        std::memcpy(O, G, n * 16);
+        return SL_OK;
+    }
+    const double s2 = (double)sigma * (double)sigma;
+    std::vector<OBnd> bx(n), by(n);
+    std::vector<double> wx(n, 0.0), wy(n, 0.0), dx(n, 0.0), dy(n, 0.0);
+    auto dist = [](const OBnd& B) { double p[4]; return bnd_partials(B, p); };
+    auto weight = [&](double d) {
+        const double r = std::sqrt(d * d + s2);
+        return d >= 0.0 ? s2 / (r * (r + d)) : 1.0 - d / r;
+    };
     for (int y = 0; y < H; ++y)
         for (int x = 0; x < W; ++x)
         {
             const size_t i = (size_t)y * W + x;
-            if (!smooth)
-            {
-                std::memcpy(&O[4 * i], &G[4 * i], 16);
-                continue;
-            }
-            const OBoundary BL = boundary_at(c, *F, V, W, x, y, offsets);
-            const OBoundary BR = boundary_at(c, *F, V, W, x + 1, y, offsets);
-            const double wl = BL.ok ? std::exp(-((double)(x + 1) - (double)BL.e) / (double)sigma) : 0.0;
-            const double wr = BR.ok ? std::exp(-((double)(x + 2) - (double)BR.e) / (double)sigma) : 0.0;
+            bx[i] = boundary_x(c, *F, V, W, x, y, offsets);
+            by[i] = boundary_y(c, *F, V, W, H, x, y, offsets);
+            if (bx[i].ok) { dx[i] = dist(bx[i]); wx[i] = weight(dx[i]); }
+            if (by[i].ok) { dy[i] = dist(by[i]); wy[i] = weight(dy[i]); }
+        }
+    // X = row pass of C; GX = dL/dX = column pass transposed; dL/dC = row pass transposed
+    std::vector<double> X(n * 4), GX(n * 4);
+    for (size_t i = 0; i < n; ++i)
+        for (int ch = 0; ch < 4; ++ch)
+            X[4 * i + ch] = (1.0 - wx[i]) * C[4 * i + ch] + (bx[i].ok ? wx[i] * C[4 * (i - 1) + ch] : 0.0);
+    // by[i] is the pair (row y, row y + 1) of column x: it moves w of the
+    // source row's X into the receiving row
+    for (size_t i = 0; i < n * 4; ++i) GX[i] = G[i];
+    for (int y = 0; y + 1 < H; ++y)
+        for (int x = 0; x < W; ++x)
+        {
+            const size_t i = (size_t)y * W + x;
+            if (!by[i].ok) continue;
+            const size_t r = by[i].recvLow ? i : i + W, s = by[i].recvLow ? i + W : i;
             for (int ch = 0; ch < 4; ++ch)
             {
-                double gi = (1.0 - wl) * (double)G[4 * i + ch];
-                if (BR.ok) gi += wr * (double)G[4 * (i + 1) + ch];
-                O[4 * i + ch] = (float)gi;
-            }
-            if (BL.ok && !pg.empty())
-            {
-                double dLdw = 0.0;
-                for (int ch = 0; ch < 4; ++ch) dLdw += (double)G[4 * i + ch] * ((double)C[4 * (i - 1) + ch] - (double)C[4 * i + ch]);
-                const double dLde = dLdw * wl / (double)sigma;
-                const OSpan& S = BL.S;
-                double dPx, dPy, dQx, dQy;
-                edge_partials(S.sx[BL.p], S.sy[BL.p], S.sx[BL.q], S.sy[BL.q], (double)y, dPx, dPy, dQx, dQy);
-                if (S.vid[BL.p] < F->posVerts)
-                {
-                    const size_t o = 2 * (S.k * F->posVerts + S.vid[BL.p]);
-                    pg[o] += dLde * dPx;
-                    pg[o + 1] += dLde * dPy;
-                }
-                if (S.vid[BL.q] < F->posVerts)
-                {
-                    const size_t o = 2 * (S.k * F->posVerts + S.vid[BL.q]);
-                    pg[o] += dLde * dQx;
-                    pg[o + 1] += dLde * dQy;
-                }
+                GX[4 * r + ch] -= wy[i] * G[4 * r + ch];
+                GX[4 * s + ch] += wy[i] * G[4 * r + ch];
             }
         }
-    if (!pg.empty())
-    {
-        float* dst = reinterpret_cast<float*>(c->vbos[F->posGrad].bytes.data());
-        for (size_t k = 0; k < pg.size(); ++k) dst[k] += (float)pg[k];
-    }
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+        {
+            const size_t i = (size_t)y * W + x;
+            for (int ch = 0; ch < 4; ++ch)
+            {
+                double g = (1.0 - wx[i]) * GX[4 * i + ch];
+                if (x + 1 < W && bx[i + 1].ok) g += wx[i + 1] * GX[4 * (i + 1) + ch];
+                O[4 * i + ch] = (float)g;
+            }
+        }
+    if (F->posGrad == SL_INVALID_ID) return SL_OK;
+    std::vector<double> pg((size_t)F->savedDraws * F->posVerts * 2, 0.0);
+    auto scatter = [&](const OBnd& B, double dLdd, bool rowPass) {
+        if (dLdd == 0.0) return;
+        double p[4];
+        bnd_partials(B, p);
+        const int ax = rowPass ? 0 : 1, bxo = rowPass ? 1 : 0;   // a and b as x / y
+        if (B.gp && B.vp < F->posVerts)
+        {
+            const size_t o = 2 * (B.k * F->posVerts + B.vp);
+            pg[o + ax] += dLdd * p[0];
+            pg[o + bxo] += dLdd * p[1];
+        }
+        if (B.gq && B.vq < F->posVerts)
+        {
+            const size_t o = 2 * (B.k * F->posVerts + B.vq);
+            pg[o + ax] += dLdd * p[2];
+            pg[o + bxo] += dLdd * p[3];
+        }
+    };
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+        {
+            const size_t i = (size_t)y * W + x;
+            if (bx[i].ok)
+            {
+                double dLdw = 0.0;
+                for (int ch = 0; ch < 4; ++ch) dLdw += GX[4 * i + ch] * ((double)C[4 * (i - 1) + ch] - (double)C[4 * i + ch]);
+                const double r = std::sqrt(dx[i] * dx[i] + s2);
+                scatter(bx[i], dLdw * (-s2 / (r * r * r)), true);
+            }
+            if (by[i].ok)
+            {
+                const size_t rr = by[i].recvLow ? i : i + W, ss = by[i].recvLow ? i + W : i;
+                double dLdw = 0.0;
+                for (int ch = 0; ch < 4; ++ch) dLdw += (double)G[4 * rr + ch] * (X[4 * ss + ch] - X[4 * rr + ch]);
+                const double r = std::sqrt(dy[i] * dy[i] + s2);
+                scatter(by[i], dLdw * (-s2 / (r * r * r)), false);
+            }
+        }
+    float* dst = reinterpret_cast<float*>(c->vbos[F->posGrad].bytes.data());
+    for (size_t k = 0; k < pg.size(); ++k) dst[k] += (float)pg[k];
+    return SL_OK;
+}
+
+// The smoothed image's inner product with gradTex, sum_i G_i . O_i, with
+// the model evaluated in double (weights, both passes, the sum): the loss
+// whose central differences check the position gradients at sigma far below
+// the pixel spacing, where a float forward would drown them in rounding.
+// Oracle only (test infrastructure).
+int slo_smooth_loss_f64(sl_context* c, size_t fbo, size_t gradTex, float sigma, double* loss)
+{
+    if (!loss) return SL_ERR_INVALID_ARG;
+    if (!c || fbo >= c->fbos.size()) return SL_ERR_INVALID_HANDLE;
+    const size_t colId = c->fbos[fbo].colors.empty() ? SL_INVALID_ID : c->fbos[fbo].colors[0];
+    if (gradTex == colId) return SL_ERR_INVALID_ARG;
+    Fbo* F; const Tex *col, *vis, *in; Tex* out; const float* offsets;
+    int rc = smooth_check(c, fbo, gradTex, colId, sigma, F, col, vis, in, out, offsets, false);
+    if (rc != SL_OK) return rc;
+    const int W = vis->w, H = vis->h;
+    const size_t n = (size_t)W * H;
+    const float* C = reinterpret_cast<const float*>(col->data.data());
+    const float* V = reinterpret_cast<const float*>(vis->data.data());
+    const float* G = reinterpret_cast<const float*>(in->data.data());
+    const bool smooth = sigma > 0.f && !F->saved.empty();
+    const double s2 = (double)sigma * (double)sigma;
+    auto weight = [&](const OBnd& B) {
+        double p[4];
+        const double d = bnd_partials(B, p), r = std::sqrt(d * d + s2);
+        return d >= 0.0 ? s2 / (r * (r + d)) : 1.0 - d / r;
+    };
+    std::vector<double> X(n * 4), O(n * 4);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+        {
+            const size_t i = (size_t)y * W + x;
+            const OBnd B = smooth ? boundary_x(c, *F, V, W, x, y, offsets) : OBnd{};
+            const double w = B.ok ? weight(B) : 0.0;
+            for (int ch = 0; ch < 4; ++ch)
+                X[4 * i + ch] = (1.0 - w) * C[4 * i + ch] + (B.ok ? w * C[4 * (i - 1) + ch] : 0.0);
+        }
+    O = X;
+    for (int y = 0; smooth && y + 1 < H; ++y)
+        for (int x = 0; x < W; ++x)
+        {
+            const OBnd B = boundary_y(c, *F, V, W, H, x, y, offsets);
+            if (!B.ok) continue;
+            const double w = weight(B);
+            const size_t r = (size_t)(B.recvLow ? y : y + 1) * W + x, s = (size_t)(B.recvLow ? y + 1 : y) * W + x;
+            for (int ch = 0; ch < 4; ++ch) O[4 * r + ch] += w * (X[4 * s + ch] - X[4 * r + ch]);
+        }
+    double acc = 0.0;
+    for (size_t i = 0; i < n * 4; ++i) acc += (double)G[i] * O[i];
+    *loss = acc;
     return SL_OK;
 }
 

@@ -263,6 +263,113 @@ __device__ __forceinline__ int clip_status(float4 c0, float4 c1, float4 c2)
     return vis | part;
 }
 
+__device__ __forceinline__ float4 clip_edge(int e)
+{
+    switch (e)
+    {
+        case 0: return f4(1.f, 0.f, 0.f, 1.f);
+        case 1: return f4(-1.f, 0.f, 0.f, 1.f);
+        case 2: return f4(0.f, 1.f, 0.f, 1.f);
+        case 3: return f4(0.f, -1.f, 0.f, 1.f);
+        case 4: return f4(0.f, 0.f, 1.f, 1.f);
+        default: return f4(0.f, 0.f, -1.f, 1.f);
+    }
+}
+
+// Exact shortcut of clip_and_process_tris for the common case: planes are
+// visited in the reference's order; a plane that keeps all three vertices
+// leaves the triangle unchanged, the first plane that rejects all three ends
+// it with no output.  Returns 0 = rejected, 1 = needs the full clipper.
+__device__ __forceinline__ int clip_quick(const float4* p)
+{
+    for (int e = 0; e < 6; ++e)
+    {
+        const float4 E = clip_edge(e);
+        const bool v0 = !signbit(dot4(p[0], E));
+        const bool v1 = !signbit(dot4(p[1], E));
+        const bool v2 = !signbit(dot4(p[2], E));
+        if (v0 && v1 && v2) continue;
+        if (!v0 && !v1 && !v2) return 0;
+        return 1;
+    }
+    return 1;
+}
+
+// Polygon buffers of the clipper: entry k of a buffer lives at p[k * stride],
+// so a thread's polygon can sit in LDS (interleaved with other threads'
+// polygons, or in a wave's shared slot) instead of in scratch memory, whose
+// latency every step of the clipper would otherwise pay.
+template <class T>
+struct PolyBuf
+{
+    T* p;
+    int stride;
+    __device__ __forceinline__ T& operator[](int k) const { return p[k * stride]; }
+};
+
+// SL_TriProcessor::clip_and_process_tris (SL_TriProcessor.cpp:370-499) on
+// positions and (kVary) varyings.  Returns the polygon size (0 if it was
+// clipped away or has fewer than 3 vertices); the polygon is left in
+// outPos / outVar in clip space.  tpos / tvar are work buffers.  All buffers
+// hold 9 entries.
+template <bool kVary>
+__device__ int clip_polygon(const float4* inPos, const Vary4* inVar, int nv, PolyBuf<float4> outPos, PolyBuf<Vary4> outVar,
+                            PolyBuf<float4> tpos, PolyBuf<Vary4> tvar)
+{
+    int n = 3;
+    for (int k = 0; k < 3; ++k)
+    {
+        outPos[k] = inPos[k];
+        if constexpr (kVary) outVar[k] = inVar[k];
+    }
+    for (int e = 0; e < 6; ++e)
+    {
+        const float4 E = clip_edge(e);
+        // a plane that keeps every vertex leaves the polygon as it is
+        bool allIn = true;
+        for (int k = 0; k < n; ++k) allIn = allIn && !signbit(dot4(outPos[k], E));
+        if (allIn) continue;
+        int nn = 0;
+        int j = n - 1;
+        float4 q0 = outPos[n - 1];
+        float t0 = dot4(q0, E);
+        bool vis0 = !signbit(t0);
+        for (int k = 0; k < n; ++k)
+        {
+            const float4 q1 = outPos[k];
+            const float t1 = dot4(q1, E);
+            const bool vis1 = !signbit(t1);
+            if (vis0 != vis1)
+            {
+                const float t = t0 * rcpf(t0 - t1);
+                tpos[nn] = mix4(q0, q1, t);
+                if constexpr (kVary)
+                {
+#pragma unroll
+                    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+                        if (i < nv) tvar[nn].v[i] = mix4(outVar[j].v[i], outVar[k].v[i], t);
+                }
+                ++nn;
+            }
+            if (vis1)
+            {
+                tpos[nn] = q1;
+                if constexpr (kVary) tvar[nn] = outVar[k];
+                ++nn;
+            }
+            j = k; q0 = q1; t0 = t1; vis0 = vis1;
+        }
+        if (nn == 0) return 0;
+        n = nn;
+        for (int k = 0; k < n; ++k)
+        {
+            outPos[k] = tpos[k];
+            if constexpr (kVary) outVar[k] = tvar[k];
+        }
+    }
+    return n < 3 ? 0 : n;
+}
+
 // sl_perspective_divide (:239-249) + sl_ndc_to_screen_coords (:508-524)
 __device__ __forceinline__ float4 divide_and_snap(float4 p, const float vp[4])
 {

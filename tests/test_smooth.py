@@ -25,6 +25,7 @@ gradients and position gradients against the oracle on a 4-view icosphere.
 """
 from __future__ import annotations
 
+import ctypes
 import math
 from fractions import Fraction as Fr
 
@@ -80,39 +81,170 @@ def edge_x(p, q, y):
     return Fr(x0) + Fr(x1 - x0) * Fr(y - y0, y1 - y0)
 
 
+def weight(d, sigma):
+    """Twice the algebraic sigmoid (1 + u / sqrt(1 + u^2)) / 2 at u = -d / sigma."""
+    return 1.0 - d / math.sqrt(d * d + sigma * sigma)
+
+
+SQ317 = math.sqrt(11 ** 2 + 14 ** 2)      # length of the diagonal (13, 1) - (2, 15)
+
+
+def diag_distance(px, py):
+    """Perpendicular distance of the sample (px, py) from the diagonal's line."""
+    return abs(14 * (px - 13) + 11 * (py - 1)) / SQ317
+
+
 def expected_quad(sigma, tris, colors):
-    """The model in float64 from exact edge positions, for the fixtures above
-    (left edge x = 2, the shared diagonal, and for the quad a right edge x = 13)."""
-    img = np.zeros((H, W, 4))
+    """The model in float64 from exact geometry, for the fixtures above: the
+    left edge x = 2, the bottom edge y = 1, the diagonal, and for the quad
+    the right edge x = 13 and the top edge y = 15.  Pixel (x, y) samples the
+    point (x + 1, y)."""
+    hard = np.zeros((H, W, 4))
     cols = [u8f(c) for c in colors]
-    diag = (TRI[1], TRI[2])
+    quad = len(tris) == 2
+    xr = {}
     for y in range(1, 15):
-        e = edge_x(*diag, y)
-        xr = math.trunc(e)
-        covered1 = range(2, xr)
-        for x in covered1:
-            img[y, x] = cols[0]
-        if len(tris) == 2:
-            for x in range(xr, 13):
-                img[y, x] = cols[1]
-        hard = img[y].copy()
-        if len(covered1) > 0:
-            # background leaks into the first pixel of the left triangle (e = 2)
-            w = math.exp(-(3 - 2) / sigma)
-            img[y, 2] = (1 - w) * hard[2] + w * hard[1]
-            # the diagonal: right end of the left triangle leaks into pixel xr
-            if xr < W:
-                w = math.exp(-((xr + 1) - float(e)) / sigma)
-                img[y, xr] = (1 - w) * hard[xr] + w * hard[xr - 1]
-        elif len(tris) == 2 and xr <= 2:
-            # no left-triangle pixel: the right triangle starts at xr with the background before it
-            w = math.exp(-((xr + 1) - float(e)) / sigma)
-            img[y, xr] = (1 - w) * hard[xr] + w * hard[xr - 1]
-        if len(tris) == 2:
-            # right edge x = 13 of the right triangle leaks into pixel 13
-            w = math.exp(-(14 - 13) / sigma)
-            img[y, 13] = (1 - w) * hard[13] + w * hard[12]
-    return img
+        xr[y] = math.trunc(edge_x(TRI[1], TRI[2], y))
+        hard[y, 2:xr[y]] = cols[0]
+        if quad:
+            hard[y, xr[y]:13] = cols[1]
+    # row pass: the colour left of an edge leaks into the pixel right of it
+    X = hard.copy()
+    for y in range(1, 15):
+        e = float(edge_x(TRI[1], TRI[2], y))
+        if xr[y] > 2:
+            w = weight(1.0, sigma)                                  # left edge x = 2 into pixel 2
+            X[y, 2] = (1 - w) * hard[y, 2] + w * hard[y, 1]
+            w = weight(diag_distance(xr[y] + 1, y), sigma)          # diagonal into pixel xr
+            X[y, xr[y]] = (1 - w) * hard[y, xr[y]] + w * hard[y, xr[y] - 1]
+        elif quad:
+            w = weight(diag_distance(3, y), sigma)                  # the quad's second triangle starts at 2
+            X[y, 2] = (1 - w) * hard[y, 2] + w * hard[y, 1]
+        if quad and xr[y] < 13:
+            w = weight(1.0, sigma)                                  # right edge x = 13 into pixel 13
+            X[y, 13] = (1 - w) * hard[y, 13] + w * hard[y, 12]
+    # column pass: the colour above an edge leaks into the row below it
+    O = X.copy()
+    for x in range(2, 13):
+        u = x + 1
+        w = weight(1.0, sigma)                                      # bottom edge y = 1 into row 0
+        O[0, x] += w * (X[1, x] - X[0, x])
+        v = 1 + Fr(14 * (13 - u), 11)                               # the diagonal's crossing of column u
+        y1 = math.floor(v)
+        if v > y1:
+            w = weight(diag_distance(u, y1), sigma)                 # the diagonal into TRI's last row
+            O[y1, x] += w * (X[y1 + 1, x] - X[y1, x])
+        if quad:
+            w = weight(1.0, sigma)                                  # top edge y = 15 into row 14
+            O[14, x] += w * (X[15, x] - X[14, x])
+    return O
+
+
+# A triangle cut by the far plane: vertex (2, 15) at z = 3 (w = 1) leaves the
+# clipper the polygon (2, 6)* (2, 1) (13, 1) (9, 6)* (* made by the clipper,
+# snapped from (2.5, 6.17) and (9.83, 6.17)); the raster draws it as the fan
+# (2,6)-(13,1)-(9,6), (2,6)-(2,1)-(13,1).
+CLIPPED = [(2, 1), (13, 1), (2, 15)]
+CLIP_Z = [0.0, 0.0, 3.0]
+CLIP_POLY = [(2, 6), (2, 1), (13, 1), (9, 6)]
+
+
+def render_clipped(ctx):
+    s = build_screen_triangles(ctx, [CLIPPED], W, H, z=[CLIP_Z], colors=_flat([CLIPPED], [C1]),
+                               color_type=A.SL_COLOR_RGBA_FLOAT)
+    vis = ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, W, H)
+    ctx.attach_visibility(s.fbo, vis)
+    s.render()
+    ctx.finish()
+    return s
+
+
+def expected_clipped(sigma):
+    """The model from exact geometry for CLIPPED: rows 1..5 hold x in
+    [2, trunc(e)), e = 13 - 4 (y - 1) / 5 on the edge (13, 1)-(9, 6).  The
+    edges: x = 2 (clipper-made top end), y = 1, (13,1)-(9,6), y = 6 (both ends
+    clipper-made)."""
+    col = u8f(C1)
+    hard = np.zeros((H, W, 4))
+    L = math.sqrt(4 ** 2 + 5 ** 2)
+    xr = {}
+    for y in range(1, 6):
+        e = Fr(13) - Fr(4 * (y - 1), 5)
+        xr[y] = math.trunc(e)
+        hard[y, 2:xr[y]] = col
+    X = hard.copy()
+    for y in range(1, 6):
+        e = 13 - 4 * (y - 1) / 5
+        w = weight(1.0, sigma)
+        X[y, 2] = (1 - w) * hard[y, 2] + w * hard[y, 1]
+        w = weight((xr[y] + 1 - e) * 5 / L, sigma)
+        X[y, xr[y]] = (1 - w) * hard[y, xr[y]] + w * hard[y, xr[y] - 1]
+    O = X.copy()
+    for x in range(2, 13):
+        u = x + 1
+        O[0, x] += weight(1.0, sigma) * (X[1, x] - X[0, x])          # y = 1 into row 0
+        if u <= 9:
+            O[5, x] += weight(1.0, sigma) * (X[6, x] - X[5, x])      # y = 6 into row 5
+        else:
+            v = 1 + Fr(5 * (13 - u), 4)
+            y1 = math.floor(v)
+            O[y1, x] += weight(float(v - y1) * 4 / L, sigma) * (X[y1 + 1, x] - X[y1, x])
+    return O
+
+
+def near_vertices(points, r=1.5):
+    """Pixels whose sample (x + 1, y) lies within r of a vertex: where span and
+    column ends tie at a vertex and the model breaks the tie by edge order."""
+    m = np.zeros((H, W), dtype=bool)
+    for y in range(H):
+        for x in range(W):
+            m[y, x] = any(math.hypot(x + 1 - px, y - py) <= r for px, py in points)
+    return m
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+@pytest.mark.parametrize("sigma", [0.5, 1e-2])
+def test_closed_form_clipped(kind, make_ctx, sigma):
+    ctx = make_ctx(kind)
+    s = render_clipped(ctx)
+    hard = ctx.texture_read(s.color).astype(np.float64)
+    assert np.count_nonzero(hard[..., 3]) == sum(max(0, math.trunc(13 - 4 * (y - 1) / 5) - 2) for y in range(1, 6))
+    out = smooth(ctx, s, sigma).astype(np.float64)
+    exp = expected_clipped(sigma)
+    m = near_vertices(CLIP_POLY)
+    out[m] = exp[m] = 0
+    np.testing.assert_allclose(out, exp, atol=2e-6, rtol=0)
+
+
+def test_clipped_position_gradients_vs_fp64_finite_differences(oracle_lib):
+    """Offsets on the clipped triangle's three vertices: the clipper-made
+    vertices stay put, the edges through kept vertices move with them."""
+    ctx = A.SLContext(oracle_lib)
+    s = render_clipped(ctx)
+    nv = 3
+    base = np.random.default_rng(2).uniform(-0.35, 0.35, (1, nv, 2))
+    base = (np.round(base / STEP) * STEP).astype(np.float32)
+    off = ctx.create_vbo(base)
+    grad = ctx.create_vbo(np.zeros((1, nv, 2), dtype=np.float32))
+    ctx.attach_positions(s.fbo, grad, off, nv)
+    G = np.random.default_rng(4).uniform(-1, 1, (H, W, 4)).astype(np.float32)
+    g = ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, W, H, 1, G)
+    sigma = 0.1
+    analytic = _analytic(ctx, s, g, grad, nv, sigma)
+    f = oracle_lib.cdll.slo_smooth_loss_f64
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_float, ctypes.POINTER(ctypes.c_double)]
+
+    def loss():
+        out = ctypes.c_double()
+        assert f(ctx.h, s.fbo, g, sigma, ctypes.byref(out)) == 0
+        return out.value
+
+    fd = _central(ctx, off, base, nv, loss, STEP)
+    scale = np.abs(fd).max()
+    # every kept vertex ends an edge: (2,1) and (13,1) the bottom edge, (13,1) also the cut edge
+    assert scale > 0.01 and np.count_nonzero(analytic[:2]) == 4
+    np.testing.assert_allclose(analytic, fd, atol=1e-3 * scale, rtol=0)
 
 
 @pytest.mark.parametrize("kind", BACKENDS)
@@ -126,7 +258,7 @@ def test_sigma_zero_is_the_hard_image(kind, make_ctx):
 
 
 @pytest.mark.parametrize("kind", BACKENDS)
-@pytest.mark.parametrize("sigma", [0.5, 0.1])
+@pytest.mark.parametrize("sigma", [0.5, 0.1, 1e-3])
 @pytest.mark.parametrize("shape", ["triangle", "quad"])
 def test_closed_form(kind, make_ctx, sigma, shape):
     ctx = make_ctx(kind)
@@ -134,9 +266,9 @@ def test_closed_form(kind, make_ctx, sigma, shape):
     s, _ = render_tris(ctx, tris, colors)
     out = smooth(ctx, s, sigma).astype(np.float64)
     exp = expected_quad(sigma, tris, colors)
-    # row 1 holds vertices exactly on pixel samples: its span ends are integers
-    # that float rounding may put on either side of a window edge
-    out[1] = exp[1] = 0
+    # column 12 samples x = 13, where the bottom edge meets the diagonal: which
+    # of the two bounds the column there is a tie the model breaks by edge order
+    out[:, 12] = exp[:, 12] = 0
     np.testing.assert_allclose(out, exp, atol=2e-6, rtol=0)
 
 
@@ -156,45 +288,128 @@ def test_adjoint_identity(oracle_lib):
     assert abs(lhs - rhs) <= 1e-5 * max(1.0, abs(lhs))
 
 
-def test_position_gradients_vs_finite_differences(oracle_lib):
-    ctx = A.SLContext(oracle_lib)
+# vertex offsets off the pixel lattice in x and y (seeded): the weights are
+# smooth functions of the offsets, so central differences apply anywhere.  The
+# offsets sit on a 2^-14 grid and the steps are multiples of it, so the
+# float screen positions (vertex + offset, below 32) move by exactly the step.
+STEP = 2.0 ** -14
+
+
+def _quad_with_offsets(ctx, seed=5):
     s, _ = render_tris(ctx, QUAD, [C1, C2])
     nv = 6
-    base = np.zeros((1, nv, 2), dtype=np.float32)
-    base[..., 0] = 0.3                      # off the pixel grid: no span end on a window edge
+    base = np.random.default_rng(seed).uniform(-0.35, 0.35, (1, nv, 2))
+    base = (np.round(base / STEP) * STEP).astype(np.float32)
     off = ctx.create_vbo(base)
     grad = ctx.create_vbo(np.zeros((1, nv, 2), dtype=np.float32))
     ctx.attach_positions(s.fbo, grad, off, nv)
-    rng = np.random.default_rng(5)
-    G = rng.uniform(-1, 1, (H, W, 4)).astype(np.float32)
-    # row 1 runs through the two bottom vertices: moving one of them in y tilts
-    # the bottom edge through the row's samples (not differentiable there)
-    G[1] = 0
+    G = np.random.default_rng(5).uniform(-1, 1, (H, W, 4)).astype(np.float32)
     g = ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, W, H, 1, G)
+    return s, nv, base, off, grad, G, g
+
+
+def _analytic(ctx, s, g, grad, nv, sigma):
     gi = ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, W, H)
-    sigma = 0.5
     ctx.smooth_coverage_backward(s.fbo, g, sigma, gi)
     ctx.finish()
-    analytic = ctx.vbo_read(grad, nv * 8, dtype=np.float32).reshape(nv, 2).astype(np.float64)
+    return ctx.vbo_read(grad, nv * 8, dtype=np.float32).reshape(nv, 2).astype(np.float64)
 
-    def loss(o):
-        ctx.vbo_assign(off, o)
-        return float(np.sum(G.astype(np.float64) * smooth(ctx, s, sigma)))
 
-    h = 1e-2
+def _central(ctx, off, base, nv, loss, h):
     fd = np.zeros((nv, 2))
     for v in range(nv):
         for a in range(2):
             p = base.copy(); p[0, v, a] += h
             m = base.copy(); m[0, v, a] -= h
-            fd[v, a] = (loss(p) - loss(m)) / (2 * h)
+            step = float(p[0, v, a]) - float(m[0, v, a])       # the float32 offsets actually applied
+            ctx.vbo_assign(off, p)
+            lp = loss()
+            ctx.vbo_assign(off, m)
+            lm = loss()
+            fd[v, a] = (lp - lm) / step
+    ctx.vbo_assign(off, base)
+    return fd
+
+
+@pytest.mark.parametrize("seed", [5, 11])
+@pytest.mark.parametrize("sigma", [0.5, 0.1])
+def test_position_gradients_vs_finite_differences(oracle_lib, sigma, seed):
+    """Every vertex coordinate of the quad (x and y: both passes) against
+    central differences of the float forward."""
+    ctx = A.SLContext(oracle_lib)
+    s, nv, base, off, grad, G, g = _quad_with_offsets(ctx, seed)
+    analytic = _analytic(ctx, s, g, grad, nv, sigma)
+    fd = _central(ctx, off, base, nv, lambda: float(np.sum(G.astype(np.float64) * smooth(ctx, s, sigma))), 32 * STEP)
     scale = np.abs(fd).max()
     assert scale > 0.1
-    np.testing.assert_allclose(analytic, fd, atol=2e-3 * scale, rtol=0)
+    np.testing.assert_allclose(analytic, fd, atol=5e-3 * scale, rtol=0)
+
+
+@pytest.mark.parametrize("sigma", [1e-2, 1e-3, 1e-4])
+def test_position_gradients_vs_fp64_finite_differences(oracle_lib, sigma):
+    """At the workload's sigmas (far below the pixel spacing): non-zero
+    position gradients, within 1e-3 (of the largest) of central differences of
+    the model evaluated in double."""
+    ctx = A.SLContext(oracle_lib)
+    s, nv, base, off, grad, G, g = _quad_with_offsets(ctx)
+    analytic = _analytic(ctx, s, g, grad, nv, sigma)
+    f = oracle_lib.cdll.slo_smooth_loss_f64
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_float, ctypes.POINTER(ctypes.c_double)]
+
+    def loss():
+        out = ctypes.c_double()
+        assert f(ctx.h, s.fbo, g, sigma, ctypes.byref(out)) == 0
+        return out.value
+
+    fd = _central(ctx, off, base, nv, loss, STEP)
+    scale = np.abs(fd).max()
+    assert scale > 0 and np.count_nonzero(analytic) >= 8
+    np.testing.assert_allclose(analytic, fd, atol=1e-3 * scale, rtol=0)
+
+
+@pytest.mark.parametrize("kind", BACKENDS)
+def test_tiny_sigma_is_the_hard_image(kind, make_ctx):
+    """The sigma -> 0 limit holds pixel by pixel, ties included: at a sigma
+    whose square underflows every weight is exactly 0 (a receiver sitting on
+    its edge would give 0/0 instead)."""
+    ctx = make_ctx(kind)
+    s, _ = render_tris(ctx, QUAD, [C1, C2])
+    hard = ctx.texture_read(s.color)
+    np.testing.assert_array_equal(smooth(ctx, s, 1e-25).view(np.uint32), hard.view(np.uint32))
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("sigma", [0.0, 0.5, 0.05])
+@pytest.mark.parametrize("sigma", [0.5, 0.1])
+def test_gpu_position_gradients_vs_finite_differences(gpu_lib, sigma):
+    """The GPU adjoint against central differences of the GPU forward."""
+    ctx = A.SLContext(gpu_lib)
+    s, nv, base, off, grad, G, g = _quad_with_offsets(ctx, 11)
+    analytic = _analytic(ctx, s, g, grad, nv, sigma)
+    fd = _central(ctx, off, base, nv, lambda: float(np.sum(G.astype(np.float64) * smooth(ctx, s, sigma))), 32 * STEP)
+    scale = np.abs(fd).max()
+    assert scale > 0.1
+    np.testing.assert_allclose(analytic, fd, atol=5e-3 * scale, rtol=0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sigma", [1e-2, 1e-3, 1e-4])
+def test_gpu_position_gradients_match_fp64_checked_oracle(oracle_lib, gpu_lib, sigma):
+    """At the workload's sigmas the GPU's position gradients on the quad match
+    the oracle's, which test_position_gradients_vs_fp64_finite_differences
+    checks against FP64 central differences, and are non-zero."""
+    out = []
+    for lib in (oracle_lib, gpu_lib):
+        ctx = A.SLContext(lib)
+        s, nv, base, off, grad, G, g = _quad_with_offsets(ctx)
+        out.append(_analytic(ctx, s, g, grad, nv, sigma))
+    scale = np.abs(out[0]).max()
+    assert scale > 0 and np.count_nonzero(out[1]) >= 8
+    np.testing.assert_allclose(out[1], out[0], atol=1e-4 * scale, rtol=0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sigma", [0.0, 0.5, 0.05, 1e-3])
 def test_gpu_matches_oracle_on_views(oracle_lib, gpu_lib, sigma):
     outs = []
     for lib in (oracle_lib, gpu_lib):
@@ -271,6 +486,9 @@ def _assert_smooth_parity(o, g, sigma):
         for a, b in zip(o[key], g[key]):
             scale = max(1e-6, float(np.abs(a).max()))
             np.testing.assert_allclose(b, a, atol=1e-4 * scale, rtol=0)
+    if sigma > 0.0:
+        # the heavy-tailed weight keeps position gradients alive at every sigma
+        assert all(np.count_nonzero(p) > 100 for p in g["pg"])
     scale = float(np.abs(o["gtex"]).max())
     np.testing.assert_allclose(g["gtex"], o["gtex"], atol=1e-4 * scale, rtol=0)
 
