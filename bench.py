@@ -108,13 +108,18 @@ def run_gpu(steps, warmup, workload="spheres", raster_config=None, tune=False, r
         ctx.set_raster_config(raster_config)
     else:
         cache = os.environ.get("SL_TUNE_CACHE", autotune.DEFAULT_CACHE)
-        cached = autotune.lookup(signature, cache)
-        if cached is None and workload.startswith("spheres_"):
+        entry = autotune.cached_entry(signature, cache)
+        if entry is not None:
+            autotune.apply(ctx, entry)
+            if raster_split is not None:
+                ctx.set_raster_split(raster_split)
+        elif workload.startswith("spheres_"):
             # the sphere frame's variants (other targets, other render modes)
             # bin the same geometry: start from the sphere frame's tile
-            cached = autotune.lookup(f"spheres:{scene.W}x{scene.H}:{scene.n_triangles}", cache)
-        if cached is not None:
-            ctx.set_raster_config(cached)
+            cached = autotune.cached_config(f"spheres:{scene.W}x{scene.H}:{scene.n_triangles}", cache)
+            if cached is not None:
+                ctx.set_raster_config(cached)
+        scene.info["tuned"] = autotune.describe(entry)
     scene.info["raster_config"] = ctx.raster_config()
     for _ in range(warmup):
         frame()
@@ -164,9 +169,9 @@ def run_views(steps, warmup, tune=False):
         res = autotune.tune(ctx, fwd, signature, path=cache)
         print(json.dumps({"autotune": signature, **res}), file=sys.stderr, flush=True)
     else:
-        cached = autotune.lookup(signature, cache)
-        if cached is not None:
-            ctx.set_raster_config(cached)
+        entry = autotune.cached_entry(signature, cache)
+        if entry is not None:
+            autotune.apply(ctx, entry)
 
     def fwd_bwd():
         sc.forward()
@@ -296,11 +301,11 @@ def run_scene_file(frames=12, n_objects=120, level=3, cpu=True):
             import oracle
             from softlight_amd import scene_files
             dump = scene_files.run_dump(scene_files.dump_executable(), obj, "", d)
-            threads = min(16, os.cpu_count() or 1)
-            times = [scene_files.draw_dump(oracle.load(), dump, 1920, 1080, threads=threads)["ms"] for _ in range(3)]
-            ms = sorted(times)[1]
+            threads = host_cpu()["threads"]
+            times = sorted(scene_files.draw_dump(oracle.load(), dump, 1920, 1080, threads=threads)["ms"] for _ in range(5))
+            ms = times[2]
             out["cpu_baseline"] = {"ms_per_frame": round(ms, 2), "Mtri_per_s": round(out["triangles"] / (ms * 1e-3) / 1e6, 1),
-                                   "cores": threads, "kind": "port", "sample": "the same frame, median of 3"}
+                                   "cores": threads, "kind": "port", "sample": "the same frame, median of 5"}
     return out
 
 
@@ -351,34 +356,154 @@ def run_blit(reps=50):
             "timing": "wall clock over back-to-back blits (launch overhead included)"}
 
 
-def run_cpu_baseline(frames=3):
-    import oracle
-    from softlight_amd import api, scenes
-
-    cores = os.cpu_count() or 1
+def host_cpu():
+    """The host's online cores (nproc), the cores this process may run on, the
+    CPU model, and the thread count the CPU legs use: the host-core share of
+    one GPU (16 on these boxes), or fewer when fewer are online."""
+    online = os.cpu_count() or 1
     try:
-        cores = len(os.sched_getaffinity(0))
+        usable = len(os.sched_getaffinity(0))
     except AttributeError:
+        usable = online
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
         pass
-    cores = min(cores, 16)               # the host-core share of one GPU
-    ctx = api.SLContext(oracle.load())
-    ctx.num_threads(cores)
-    scene = scenes.build_sphere_scene(ctx, **WORKLOAD)
-    scene.render()                       # warm-up frame
+    return {"nproc": online, "usable_cores": usable, "model": model, "threads": min(usable, 16)}
+
+
+def _median_frame_ms(ctx, render, frames):
+    render()                             # warm-up frame
+    ctx.finish()
     times = []
     for _ in range(frames):
         t0 = time.perf_counter()
-        scene.render()
+        render()
         ctx.finish()
         times.append(time.perf_counter() - t0)
     times.sort()
-    med = times[len(times) // 2]
+    return times[len(times) // 2] * 1e3
+
+
+def run_cpu_baseline(frames=5):
+    """The oracle (the project's restatement of the reference draw path) on
+    the host, multithreaded: the sphere frame (the headline's CPU figure)."""
+    import oracle
+    from softlight_amd import api, scenes
+
+    cpu = host_cpu()
+    ctx = api.SLContext(oracle.load())
+    ctx.num_threads(cpu["threads"])
+    scene = scenes.build_sphere_scene(ctx, **WORKLOAD)
+    med = _median_frame_ms(ctx, scene.render, frames)
+    ctx.close()
     return {
-        "value": round(scene.n_triangles / med / 1e6, 3),
+        "value": round(scene.n_triangles / (med * 1e-3) / 1e6, 3),
         "unit": "Mtri/s",
-        "cores": cores,
+        "cores": cpu["threads"],
         "kind": "port",
-        "sample": f"{frames} frames (+1 warm-up) of the same 1080p textured-sphere workload, median frame {med * 1e3:.1f} ms",
+        "sample": f"median of {frames} frames (+1 warm-up) of the same 1080p textured-sphere workload: {med:.1f} ms; "
+                  f"{cpu['threads']} threads (the host-core share of one GPU) of {cpu['nproc']} online "
+                  f"({cpu['model']})",
+        "host": cpu,
+    }
+
+
+def run_cpu_baselines(frames=5):
+    """The oracle on the other workloads: 4096 random triangles at 640x480 and
+    the 1000-mesh batched frame (median of `frames` frames, multithreaded),
+    and the differentiable workloads' forward and forward+backward (the scalar
+    adjoint) at one thread and multithreaded: all 32 views of the
+    fixed-visibility workload, and one framebuffer (8 of the 64 views) of the
+    smoothed-coverage workload at sigma = 1e-3."""
+    import oracle
+    from softlight_amd import api, scenes
+
+    cpu = host_cpu()
+    out = {"host": cpu}
+    for name, build, kw in (("random_640x480", scenes.build_random_triangles, RANDOM_WORKLOAD),
+                            ("batched_1080p", scenes.build_batched_scene, BATCHED_WORKLOAD)):
+        ctx = api.SLContext(oracle.load())
+        ctx.num_threads(cpu["threads"])
+        scene = build(ctx, **kw)
+        med = _median_frame_ms(ctx, scene.render, frames)
+        ctx.close()
+        out[name] = {"ms_per_frame": round(med, 2), "Mtri_per_s": round(scene.n_triangles / (med * 1e-3) / 1e6, 3),
+                     "threads": cpu["threads"], "sample": f"median of {frames} frames"}
+
+    def diff_leg(make, fwd, fwd_bwd, views, threads):
+        ctx = api.SLContext(oracle.load())
+        ctx.num_threads(threads)
+        sc = make(ctx)
+        res = {}
+        for name, fn in (("fwd", fwd), ("fwd_bwd", fwd_bwd)):
+            ms = _median_frame_ms(ctx, lambda: fn(sc), 3)
+            res[name] = ms
+        ctx.close()
+        return {"threads": threads, "ms_fwd": round(res["fwd"], 2), "ms_fwd_bwd": round(res["fwd_bwd"], 2),
+                "fwd_views_per_s": round(views / (res["fwd"] * 1e-3), 2),
+                "fwd_bwd_views_per_s": round(views / (res["fwd_bwd"] * 1e-3), 2),
+                "fwd_bwd_over_fwd_time": round(res["fwd_bwd"] / res["fwd"], 3)}
+
+    def make_views(ctx):
+        sc = scenes.DiffScene(ctx, **VIEWS_WORKLOAD)
+        sc.make_targets()
+        return sc
+
+    smooth_kw = dict(SMOOTH_WORKLOAD, views=8)
+
+    def make_smooth(ctx):
+        sc = scenes.SmoothScene(ctx, **smooth_kw)
+        sc.make_targets(1e-3)
+        return sc
+
+    out["views_32x512"] = {
+        "workload": "the fixed-visibility workload: 32 views x 512^2, forward and forward + scalar adjoint, median of 3",
+        "per_threads": [diff_leg(make_views, lambda s: s.forward(),
+                                 lambda s: (s.forward(), s.loss_grad_images(), s.backward()), VIEWS_WORKLOAD["views"], t)
+                        for t in (1, cpu["threads"])]}
+    out["smooth_8_of_64_views"] = {
+        "workload": "one framebuffer (8 of the 64 views) of the smoothed-coverage workload at sigma = 1e-3, forward and "
+                    "forward + scalar adjoint, median of 3",
+        "per_threads": [diff_leg(make_smooth, lambda s: s.forward(1e-3), lambda s: (s.forward(1e-3), s.backward(1e-3)),
+                                 smooth_kw["views"], t)
+                        for t in (1, cpu["threads"])]}
+    return out
+
+
+def run_workload_leg(workload, steps, warmup):
+    """One of the other frames (4096 random triangles at 640x480, the
+    1000-mesh batched frame at 1080p) on its tuned raster knobs: graph-replay
+    frame time, triangles/s, shaded fragments/s and the stage split."""
+    scene, st, eager_s, gst, elapsed = run_gpu(steps, warmup, workload)
+    tris = scene.n_triangles
+    frames = max(st["frames"], 1)
+    ms_raster = st["ms_raster"] / frames
+    if gst["raster_launches"] > 0:
+        ms_raster = gst["ms_raster"] / gst["raster_launches"]
+    shaded = st["fragments_shaded"] / frames
+    tested = st["fragments_tested"] / frames
+    texel_bytes = 0.0 if workload == "random" else 16.0
+    fbytes = frag_bytes(tested, shaded, texel_bytes)
+    frag_gbs = fbytes / (ms_raster * 1e-3) / 1e9
+    return {
+        "triangles_per_frame": tris,
+        "ms_per_frame": round(elapsed * 1e3 / steps, 4),
+        "Mtri_per_s": round(tris * steps / elapsed / 1e6, 3),
+        "shaded_gfrag_per_s": round(shaded / (ms_raster * 1e-3) / 1e9, 4),
+        "fragments_shaded_per_frame": int(shaded),
+        "stage_ms": {"vertex_setup": round(st["ms_vertex"] / frames, 4), "binning": round(st["ms_bin"] / frames, 4),
+                     "raster": round(ms_raster, 4)},
+        "fragment_roofline": {"kernel": "k_raster", "bound": "hbm", "achieved": round(frag_gbs, 2),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(frag_gbs / HBM_PEAK_GBS, 5),
+                              "bytes_per_frame": fbytes},
+        "raster_tile": list(scene.info["raster_config"][1]),
+        "autotuned": scene.info.get("tuned"),
     }
 
 
@@ -397,8 +522,9 @@ def main():
                          "index buffer drawn as wireframe triangles, lines and points (its leg only); scene_file: "
                          "a 120-object OBJ file loaded into the scene graph and drawn per frame (its leg only)")
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter calibration)")
-    ap.add_argument("--raster-config", type=int, default=None, help="raster tile configuration 0..7 (default: tuned)")
-    ap.add_argument("--tune", action="store_true", help="time all raster configurations first and cache the best")
+    ap.add_argument("--raster-config", type=int, default=None, help="raster tile configuration 0..11 (default: tuned)")
+    ap.add_argument("--tune", action="store_true",
+                    help="time the raster knobs first (tile, workgroups per CU, bin capacity) and cache the best")
     ap.add_argument("--raster-split", type=int, default=None,
                     help="records per part of a split heavy raster tile (0: no split; default: the library's)")
     args = ap.parse_args()
@@ -510,8 +636,12 @@ def main():
         out["demo_targets"] = run_demo_targets(max(3, args.steps // 2), 1)
         out["present_blit"] = run_blit()
         out["scene_file"] = run_scene_file(cpu=not args.no_cpu)
+        out["random_640x480"] = run_workload_leg("random", max(3, args.steps // 2), 1)
+        out["batched_1080p"] = run_workload_leg("batched", max(3, args.steps // 2), 1)
     if not args.no_cpu and args.workload == "spheres":
         out["cpu_baseline"] = run_cpu_baseline()
+        if not args.headline_only:
+            out["cpu_baselines"] = run_cpu_baselines()
     print(json.dumps(out), flush=True)
 
 

@@ -225,8 +225,10 @@ int         sl_set_scissor(sl_context* ctx, int32_t x, int32_t y, int32_t w, int
 
 /* Raster tile configuration (no counterpart in the reference, whose raster
  * splits scanlines across CPU threads, SL_ScanlineBounds.hpp:31-50).
- * Configurations 0..7 select the screen tile each GPU workgroup owns and how
- * many column segments a tile row is split into; see sl_get_raster_config.
+ * Configurations 0..11 select the screen tile each GPU workgroup owns (8x8
+ * to 64x64) and how many column segments a tile row is split into; see
+ * sl_get_raster_config.  Configuration 11 (64x64) falls back to 7 (32x32)
+ * for a batch with an RGBA32F colour target or a visibility attachment.
  * sl_get_raster_config(ctx, -1, ...) reports the current one and returns its
  * index; results do not depend on the configuration. */
 int         sl_set_raster_config(sl_context* ctx, int config);
@@ -240,6 +242,12 @@ int         sl_set_raster_config(sl_context* ctx, int config);
  * batch with parts of n records, 0 turns the split off.  Results never
  * depend on it. */
 int         sl_set_raster_split(sl_context* ctx, uint32_t partRecords);
+#define SL_RASTER_SPLIT_DEFAULT 0xFFFFFFFFu   /* sl_set_raster_split: back to the default policy */
+/* Occupancy (no counterpart in the reference): caps the raster workgroups
+ * resident on a CU at workgroupsPerCU (1..32; 0, the default: as many as the
+ * kernel's registers and LDS allow) by giving each workgroup that share of
+ * the CU's LDS.  Results never depend on it. */
+int         sl_set_raster_occupancy(sl_context* ctx, uint32_t workgroupsPerCU);
 int         sl_get_raster_config(const sl_context* ctx, int config, int* tileW, int* tileH, int* segments);
 
 /* ---- vertex buffers (SL_VertexBuffer.hpp / .cpp) ------------------------ */

@@ -2079,6 +2079,13 @@ int slo_set_raster_split(sl_context* c, uint32_t partRecords)
     return c ? SL_OK : SL_ERR_INVALID_ARG;
 }
 
+// Raster occupancy of the GPU path: no effect on results, none here.
+int slo_set_raster_occupancy(sl_context* c, uint32_t workgroupsPerCU)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    return workgroupsPerCU <= 32 ? SL_OK : SL_ERR_INVALID_ARG;
+}
+
 int slo_get_raster_config(const sl_context* c, int config, int* tileW, int* tileH, int* segments)
 {
     if (!c) return SL_ERR_INVALID_ARG;

@@ -51,6 +51,7 @@ SL_PROGRAM_LAMBERT_TEXTURED = 2
 SL_PROGRAM_COLOR_TEXTURED = 3
 SL_PROGRAM_LAMBERT_MRT = 4
 SL_UNIFORM_SAMPLE_NEAREST = 0x1      # sl_std_uniforms.flags: nearest instead of bilinear sampling
+SL_RASTER_SPLIT_DEFAULT = 0xFFFFFFFF  # sl_set_raster_split: the default split policy
 
 _TEXEL_DTYPE = {SL_COLOR_R_8U: (np.uint8, 1), SL_COLOR_RG_8U: (np.uint8, 2), SL_COLOR_RGB_8U: (np.uint8, 3),
                 SL_COLOR_RGBA_8U: (np.uint8, 4), SL_COLOR_R_32U: (np.uint32, 1), SL_COLOR_R_HALF: (np.float16, 1),
@@ -108,6 +109,7 @@ _SIGS = {
     "set_raster_config": (_I, [_P, _I]),
     "get_raster_config": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "set_raster_split": (_I, [_P, C.c_uint32]),
+    "set_raster_occupancy": (_I, [_P, C.c_uint32]),
     "set_viewport": (_I, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "set_scissor": (_I, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "create_vbo": (_SZ, [_P]),
@@ -233,8 +235,13 @@ class SLContext:
         return self._check(self.lib.set_raster_config(self.h, config), "set_raster_config")
 
     def set_raster_split(self, part_records):
-        """Records per part of a split heavy tile (0: no split)."""
+        """Records per part of a split heavy tile (0: no split;
+        SL_RASTER_SPLIT_DEFAULT: the library's default policy)."""
         return self._check(self.lib.set_raster_split(self.h, part_records), "set_raster_split")
+
+    def set_raster_occupancy(self, workgroups_per_cu):
+        """Cap on resident raster workgroups per CU (0: none)."""
+        return self._check(self.lib.set_raster_occupancy(self.h, workgroups_per_cu), "set_raster_occupancy")
 
     def raster_config(self, config=-1):
         w, h, s = _I(), _I(), _I()

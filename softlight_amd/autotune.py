@@ -1,12 +1,16 @@
-"""Raster tile-configuration autotuner.
+"""Raster autotuner.
 
-Times each of the library's raster tile configurations (every index that
-sl_get_raster_config accepts: tile size and column segments per row) on a
-caller-supplied frame and
-keeps the fastest.  Results are cached in a JSON file keyed by the device
-name and a workload signature, so later runs reuse them without timing.
-The choice never changes results, only speed (the parity tests run every
-configuration).
+Times a caller-supplied frame under the library's raster knobs and keeps the
+fastest: the tile configuration (every index sl_get_raster_config accepts:
+tiles from 8x8 to 64x64 and column segments per row), then for that tile the
+occupancy cap (resident raster workgroups per CU, sl_set_raster_occupancy),
+then the bin capacity of a raster workgroup (records per part of a heavy
+tile's bin, sl_set_raster_split: the library's default policy, never split,
+or 64 / 256 / 1024 records).  A knob setting the frame cannot use (a tile
+configuration with more tiles than the binning's 16384) is recorded as such
+and skipped.  Results are cached in a JSON file keyed by the device name and a
+workload signature, so later runs reuse them without timing.  The choice
+never changes results, only speed (the parity tests run every setting).
 """
 from __future__ import annotations
 
@@ -48,45 +52,111 @@ def load_cache(path: str = DEFAULT_CACHE) -> dict:
     return {}
 
 
+OCCUPANCY = (0, 1, 2, 4, 8)            # raster workgroups per CU (0: no cap)
+SPLIT_DEFAULT = 0xFFFFFFFF               # SL_RASTER_SPLIT_DEFAULT
+SPLITS = (SPLIT_DEFAULT, 0, 64, 256, 1024)
+
+
 def lookup(signature: str, path: str = DEFAULT_CACHE):
     entry = load_cache(path).get(device_name(), {}).get(signature)
     return None if entry is None else int(entry["config"])
 
 
+def cached_entry(signature: str, path: str = DEFAULT_CACHE):
+    """The cached knobs of `signature` for this device, else those cached for
+    any device (the cache may have been written on a box whose HIP reported
+    another name), else None: {"config", "wg_per_cu", "split", ...}."""
+    cache = load_cache(path)
+    entry = cache.get(device_name(), {}).get(signature)
+    if entry is None:
+        for entries in cache.values():
+            if signature in entries:
+                entry = entries[signature]
+                break
+    if entry is None:
+        return None
+    out = dict(entry)
+    out["config"] = int(out["config"])
+    out.setdefault("wg_per_cu", 0)
+    out.setdefault("split", SPLIT_DEFAULT)
+    return out
+
+
 def cached_config(signature: str, path: str = DEFAULT_CACHE):
-    """The cached configuration of `signature` for this device, else the one
-    cached for any device (the cache may have been written on a box whose HIP
-    reported another name), else None."""
-    own = lookup(signature, path)
-    if own is not None:
-        return own
-    for entries in load_cache(path).values():
-        if signature in entries:
-            return int(entries[signature]["config"])
-    return None
+    """The cached tile configuration of `signature` (see cached_entry)."""
+    entry = cached_entry(signature, path)
+    return None if entry is None else entry["config"]
+
+
+def apply(ctx, entry) -> None:
+    """Set a cached entry's knobs on a context."""
+    ctx.set_raster_config(int(entry["config"]))
+    ctx.set_raster_occupancy(int(entry.get("wg_per_cu", 0)))
+    ctx.set_raster_split(int(entry.get("split", SPLIT_DEFAULT)))
+
+
+def _time(ctx, render_frame, frames):
+    render_frame()                         # warm-up (buffer growth, code load)
+    ctx.reset_stats()
+    for _ in range(frames):
+        render_frame()
+    st = ctx.stats()
+    n = max(st["frames"], 1)
+    return {"raster_ms": st["ms_raster"] / n, "frame_ms": st["ms_total"] / n}
 
 
 def tune(ctx, render_frame, signature: str, frames: int = 3, path: str | None = DEFAULT_CACHE) -> dict:
-    """Time render_frame() under each raster configuration; return the
-    per-configuration raster milliseconds and the best index (also cached)."""
-    timings = {}
+    """Time render_frame() under each tile configuration, then each occupancy
+    cap and each bin capacity for the best tile; set and return the best
+    knobs with every timing (also cached)."""
+    from softlight_amd.api import SLError
+
+    configs, occupancy, splits = {}, {}, {}
+    ctx.set_raster_occupancy(0)
+    ctx.set_raster_split(SPLIT_DEFAULT)
     for cfg in range(num_configs(ctx)):
         ctx.set_raster_config(cfg)
-        render_frame()                     # warm-up (buffer growth, code load)
-        ctx.reset_stats()
-        for _ in range(frames):
-            render_frame()
-        st = ctx.stats()
-        timings[cfg] = {"raster_ms": st["ms_raster"] / max(st["frames"], 1),
-                        "frame_ms": st["ms_total"] / max(st["frames"], 1),
-                        "tile": ctx.raster_config(cfg)[1]}
-    best = min(timings, key=lambda k: timings[k]["frame_ms"])
+        try:
+            configs[cfg] = {**_time(ctx, render_frame, frames), "tile": ctx.raster_config(cfg)[1]}
+        except SLError as e:
+            configs[cfg] = {"unsupported": str(e), "tile": ctx.raster_config(cfg)[1]}
+    ok = [k for k in configs if "frame_ms" in configs[k]]
+    best = min(ok, key=lambda k: configs[k]["frame_ms"])
     ctx.set_raster_config(best)
-    result = {"config": best, "tile": timings[best]["tile"], "timings": timings}
+    for wg in OCCUPANCY:
+        ctx.set_raster_occupancy(wg)
+        try:
+            occupancy[wg] = _time(ctx, render_frame, frames)
+        except SLError as e:
+            occupancy[wg] = {"unsupported": str(e)}
+    wg_best = min((k for k in occupancy if "frame_ms" in occupancy[k]), key=lambda k: occupancy[k]["frame_ms"])
+    ctx.set_raster_occupancy(wg_best)
+    for sp in SPLITS:
+        ctx.set_raster_split(sp)
+        try:
+            splits[sp] = _time(ctx, render_frame, frames)
+        except SLError as e:
+            splits[sp] = {"unsupported": str(e)}
+    sp_best = min((k for k in splits if "frame_ms" in splits[k]), key=lambda k: splits[k]["frame_ms"])
+    ctx.set_raster_split(sp_best)
+    entry = {"config": best, "tile": configs[best]["tile"], "wg_per_cu": wg_best, "split": sp_best,
+             "frame_ms": splits[sp_best]["frame_ms"]}
+    result = {**entry, "timings": {"config": configs, "wg_per_cu": occupancy,
+                                   "split": {("default" if k == SPLIT_DEFAULT else k): v for k, v in splits.items()}}}
     if path:
         cache = load_cache(path)
-        cache.setdefault(device_name(), {})[signature] = {"config": best, "tile": timings[best]["tile"],
-                                                          "frame_ms": timings[best]["frame_ms"]}
+        cache.setdefault(device_name(), {})[signature] = entry
         with open(path, "w") as f:
             json.dump(cache, f, indent=1, sort_keys=True)
     return result
+
+
+def describe(entry) -> dict:
+    """The knobs of an entry for a report: tile, occupancy cap, bin capacity."""
+    if entry is None:
+        return None
+    sp = int(entry.get("split", SPLIT_DEFAULT))
+    return {"config": int(entry["config"]), "tile": list(entry.get("tile", [])),
+            "workgroups_per_cu": int(entry.get("wg_per_cu", 0)) or "uncapped",
+            "bin_capacity_records_per_workgroup": ("default (split point/line batches at 256)" if sp == SPLIT_DEFAULT
+                                                   else ("unsplit" if sp == 0 else sp))}

@@ -26,7 +26,7 @@ constexpr int kMaxFan = 7;             // a 9-gon from 6-plane clipping fans int
 // x = xMin + phase + 4k of its row that fall in its segment of tileW/segs
 // columns (the reference steps its barycentrics in groups of 4 pixels).
 struct TileConfig { int tileW, tileH, segs; };
-constexpr int kNumTileConfigs = 10;
+constexpr int kNumTileConfigs = 12;
 constexpr TileConfig kTileConfigs[kNumTileConfigs] = {
     {32, 16, 1},   // 0: 1 wave
     {32, 8, 2},    // 1: 1 wave
@@ -38,8 +38,20 @@ constexpr TileConfig kTileConfigs[kNumTileConfigs] = {
     {32, 32, 2},   // 7: 4 waves x 8 rows
     {32, 8, 4},    // 8: 2 waves x 4 rows
     {32, 8, 8},    // 9: 4 waves x 2 rows
+    {8, 8, 2},     // 10: 1 wave
+    {64, 64, 1},   // 11: 4 waves x 16 rows
 };
 constexpr int kDefaultTileConfig = 0;
+
+// Configuration 11's 64x64 tile keeps 4096 pixels' depth, colour and pending
+// fragments in LDS: 112 KB with an 8-bit colour target, more than the 160 KB a
+// workgroup may hold with an RGBA32F target or the visibility attachment, so
+// a batch with either is rasterised on configuration 7 (32x32) instead.
+constexpr int kLargeTileConfig = 11, kLargeTileFallback = 7;
+SL_HD constexpr int effective_tile_config(int cfg, bool floatColor, bool vis)
+{
+    return (cfg == kLargeTileConfig && (floatColor || vis)) ? kLargeTileFallback : cfg;
+}
 
 SL_HD int ilog2i(int v) { int r = 0; while ((1 << r) < v) ++r; return r; }
 
@@ -437,6 +449,7 @@ struct RasterParams
     uint32_t* partDone;           // per split tile: parts finished
     uint32_t splitClass;          // kSplit*
     int32_t splitDtest;           // the batch's depth function
+    uint32_t wgPerCU;             // launch: cap on resident raster workgroups per CU (0: none)
 };
 
 // ---------------------------------------------------------------------------

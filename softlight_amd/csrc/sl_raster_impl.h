@@ -9,6 +9,8 @@
 // SL_FragmentProcessor::flush_tri_fragments (SL_FragmentProcessor.cpp:416-531).
 #pragma once
 
+#include <algorithm>
+
 #include <hip/hip_fp16.h>
 
 #include "sl_device.h"
@@ -899,11 +901,41 @@ __global__ __launch_bounds__(256) void k_split_fold(RasterParams P)
     }
 }
 
+// Dynamic LDS that caps the resident workgroups of a k_raster instance at
+// wgPerCU per CU (0: no cap): each workgroup then holds LDS / wgPerCU of the
+// CU's LDS (its static arrays plus the rest as unused dynamic LDS), within
+// the per-workgroup limit.
+struct LdsLimits { size_t perCU, perBlock; };
+
+static LdsLimits lds_limits()
+{
+    static const LdsLimits L = [] {
+        int dev = 0, cu = 0, blk = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess) cu = 0;
+        if (hipDeviceGetAttribute(&blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) blk = 0;
+        return LdsLimits{(size_t)std::max(cu, 0), (size_t)std::max(blk, 0)};
+    }();
+    return L;
+}
+
+template <class Kernel>
+static size_t occupancy_lds(Kernel kernel, uint32_t wgPerCU)
+{
+    if (wgPerCU == 0) return 0;
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(kernel)) != hipSuccess) return 0;
+    const LdsLimits L = lds_limits();
+    const size_t stat = a.sharedSizeBytes, want = std::min(L.perCU / wgPerCU, L.perBlock);
+    return want > stat ? want - stat : 0;
+}
+
 template <int CF, int AUX, int TW, int TH, int SEGS, int MODE>
 static void launch_raster_shape(const RasterParams& p, int grid, hipStream_t s)
 {
     static_assert(TW * TH > 0, "tile");
-    hipLaunchKernelGGL((k_raster<CF, AUX, TW, TH, SEGS, MODE>), dim3(grid), dim3(TH * 4 * SEGS), 0, s, p);
+    const auto kernel = k_raster<CF, AUX, TW, TH, SEGS, MODE>;
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(TH * 4 * SEGS), occupancy_lds(kernel, p.wgPerCU), s, p);
     if (MODE == 1) hipLaunchKernelGGL((k_split_fold<TW, TH>), dim3(grid), dim3(256), 0, s, p);
 }
 
@@ -922,6 +954,13 @@ static void launch_raster_mode(const RasterParams& p, int grid, int cfg, hipStre
         case 7: launch_raster_shape<CF, AUX, 32, 32, 2, MODE>(p, grid, s); break;
         case 8: launch_raster_shape<CF, AUX, 32, 8, 4, MODE>(p, grid, s); break;
         case 9: launch_raster_shape<CF, AUX, 32, 8, 8, MODE>(p, grid, s); break;
+        case 10: launch_raster_shape<CF, AUX, 8, 8, 2, MODE>(p, grid, s); break;
+        case 11:
+            // the runtime maps configuration 11 to kLargeTileFallback for the
+            // variants whose 64x64 tile does not fit in LDS (effective_tile_config)
+            if constexpr (CF != 2 && (AUX & 2) == 0) launch_raster_shape<CF, AUX, 64, 64, 1, MODE>(p, grid, s);
+            else launch_raster_shape<CF, AUX, 32, 32, 2, MODE>(p, grid, s);
+            break;
         default: launch_raster_shape<CF, AUX, 32, 16, 1, MODE>(p, grid, s); break;
     }
 }
@@ -948,7 +987,10 @@ static_assert(kTileConfigs[0].tileW == 32 && kTileConfigs[0].tileH == 16 && kTil
               kTileConfigs[6].tileW == 32 && kTileConfigs[6].tileH == 16 && kTileConfigs[6].segs == 4 &&
               kTileConfigs[7].tileW == 32 && kTileConfigs[7].tileH == 32 && kTileConfigs[7].segs == 2 &&
               kTileConfigs[8].tileW == 32 && kTileConfigs[8].tileH == 8 && kTileConfigs[8].segs == 4 &&
-              kTileConfigs[9].tileW == 32 && kTileConfigs[9].tileH == 8 && kTileConfigs[9].segs == 8 && kNumTileConfigs == 10,
+              kTileConfigs[9].tileW == 32 && kTileConfigs[9].tileH == 8 && kTileConfigs[9].segs == 8 &&
+              kTileConfigs[10].tileW == 8 && kTileConfigs[10].tileH == 8 && kTileConfigs[10].segs == 2 &&
+              kTileConfigs[11].tileW == 64 && kTileConfigs[11].tileH == 64 && kTileConfigs[11].segs == 1 &&
+              kLargeTileConfig == 11 && kLargeTileFallback == 7 && kNumTileConfigs == 12,
               "launch table and tile configurations disagree");
 
 }  // namespace slg

@@ -243,6 +243,7 @@ struct sl_context
     int tileConfig = kDefaultTileConfig;
     uint32_t splitRecs = kDefaultSplitRecords;   // heavy-tile split: records per part (0: off)
     bool splitAll = false;                       // split triangle-only batches too (sl_set_raster_split)
+    uint32_t wgPerCU = 0;                        // raster occupancy cap (sl_set_raster_occupancy; 0: none)
     uint32_t triSeq = 0;
     sl_frame_stats stats{};
     std::string err;
@@ -406,6 +407,12 @@ int make_plan(sl_context* c, const Batch& B, Plan& p)
         if (p.dt->type != SL_COLOR_R_FLOAT) return fail(c, SL_ERR_UNSUPPORTED, "visibility needs an R_FLOAT depth attachment");
     }
     p.cfg = (c->tileConfig >= 0 && c->tileConfig < kNumTileConfigs) ? c->tileConfig : kDefaultTileConfig;
+    {
+        const int t = p.ct ? p.ct->type : -1;
+        const bool floatColor = t == SL_COLOR_R_FLOAT || t == SL_COLOR_RG_FLOAT || t == SL_COLOR_RGB_FLOAT ||
+                                t == SL_COLOR_RGBA_FLOAT;
+        p.cfg = effective_tile_config(p.cfg, floatColor, p.vis != nullptr);
+    }
     p.tc = kTileConfigs[p.cfg];
     p.tilesX = (p.W + p.tc.tileW - 1) / p.tc.tileW;
     p.tilesY = (p.H + p.tc.tileH - 1) / p.tc.tileH;
@@ -621,7 +628,8 @@ int enqueue_back(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, uint
                  hipStream_t s, hipEvent_t evClip0, hipEvent_t evClip1, hipEvent_t evBin1)
 {
     int rc;
-    const RasterParams rp = raster_params(p, B, R, B.totalTris > 0 && entries > 0, c->splitAll);
+    RasterParams rp = raster_params(p, B, R, B.totalTris > 0 && entries > 0, c->splitAll);
+    rp.wgPerCU = c->wgPerCU;
     if (B.totalTris > 0)
     {
         const SetupParams sp = setup_params(p, B, R);
@@ -1364,8 +1372,24 @@ int sl_set_raster_split(sl_context* c, uint32_t partRecords)
     if (!c) return SL_ERR_INVALID_ARG;
     const int rc = flush(c);
     if (rc != SL_OK) return rc;
+    if (partRecords == SL_RASTER_SPLIT_DEFAULT)
+    {
+        c->splitRecs = kDefaultSplitRecords;
+        c->splitAll = false;
+        return SL_OK;
+    }
     c->splitRecs = partRecords;
     c->splitAll = partRecords > 0;
+    return SL_OK;
+}
+
+int sl_set_raster_occupancy(sl_context* c, uint32_t workgroupsPerCU)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    if (workgroupsPerCU > 32) return fail(c, SL_ERR_INVALID_ARG, "at most 32 raster workgroups per CU");
+    const int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    c->wgPerCU = workgroupsPerCU;
     return SL_OK;
 }
 

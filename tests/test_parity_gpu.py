@@ -51,7 +51,7 @@ def test_spheres_1080p(gpu_lib, oracle_lib):
     assert rep["triangles_submitted"][0] == 64 * 130560
 
 
-@pytest.mark.parametrize("config", range(10))
+@pytest.mark.parametrize("config", range(12))
 def test_every_raster_configuration(gpu_lib, oracle_lib, config):
     """Results do not depend on the tile configuration."""
     from softlight_amd import api
@@ -70,6 +70,36 @@ def test_every_raster_configuration(gpu_lib, oracle_lib, config):
     g, o = render_both(gpu_lib, oracle_lib, _spheres_cfg(config), seed=2, n_instances=6, stacks=24, sectors=24,
                        W=333, H=199, tex_size=64, debug=True)
     assert_parity(g, o)
+
+
+@pytest.mark.parametrize("wg", [1, 2, 8])
+def test_occupancy_cap_does_not_change_results(gpu_lib, oracle_lib, wg):
+    """Capping the raster's resident workgroups per CU (extra LDS per
+    workgroup) changes scheduling only."""
+    def build(ctx, **kw):
+        ctx.set_raster_occupancy(wg)
+        return scenes.build_sphere_scene(ctx, **kw)
+    g, o = render_both(gpu_lib, oracle_lib, build, seed=2, n_instances=6, stacks=24, sectors=24, W=333, H=199,
+                       tex_size=64, debug=True)
+    assert_parity(g, o)
+
+
+def test_large_tile_falls_back_for_float_targets(gpu_lib, oracle_lib):
+    """Configuration 11 (64x64) with an RGBA32F colour target and the
+    visibility attachment runs on the 32x32 tile: visibility bit-exact and
+    colour as the oracle's."""
+    outs = []
+    for lib in (gpu_lib, oracle_lib):
+        ctx = A.SLContext(lib)
+        ctx.set_raster_config(11)
+        s = scenes.DiffScene(ctx, level=3, views=4, size=96, views_per_fbo=4, tex_size=64, edge_views=1)
+        s.forward()
+        ctx.finish()
+        t = s.targets[0]
+        outs.append((ctx.texture_read(t["vis"]), ctx.texture_read(t["color"])))
+    (vg, cg), (vo, co) = outs
+    np.testing.assert_array_equal(vg.view(np.uint32), vo.view(np.uint32))
+    np.testing.assert_allclose(cg, co, atol=1e-6, rtol=0)
 
 
 def _spheres_cfg(config):
@@ -107,13 +137,13 @@ TUNED = {
 def _tuned_builder(name):
     from softlight_amd import autotune
     signature, build, kw = TUNED[name]
-    cfg = autotune.cached_config(signature)
-    assert cfg is not None, f"no tuned raster configuration cached for {signature}"
+    entry = autotune.cached_entry(signature)
+    assert entry is not None, f"no tuned raster configuration cached for {signature}"
 
     def builder(ctx, **extra):
-        ctx.set_raster_config(cfg)
+        autotune.apply(ctx, entry)           # tile, occupancy cap and bin capacity, as the benchmark runs
         return build(ctx, **kw, **extra)
-    return builder, cfg
+    return builder, entry["config"]
 
 
 @pytest.mark.parametrize("name", ["random", "spheres", "batched"])

@@ -232,7 +232,7 @@ def test_gpu_random_lines(gpu_lib, oracle_lib):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("config", range(10))
+@pytest.mark.parametrize("config", range(12))
 @pytest.mark.parametrize("prim", ["lines", "points"])
 def test_gpu_spheres_every_configuration(gpu_lib, oracle_lib, config, prim):
     def build(ctx, **kw):

@@ -85,7 +85,7 @@ def test_split_render_modes(gpu_lib, oracle_lib, kind, part):
     assert_parity(g, o)
 
 
-@pytest.mark.parametrize("config", [0, 2, 5, 7, 9])
+@pytest.mark.parametrize("config", [0, 2, 5, 7, 9, 10, 11])
 def test_split_on_every_tile_shape(gpu_lib, oracle_lib, config):
     def build(ctx, **kw):
         ctx.set_raster_config(config)

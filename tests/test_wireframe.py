@@ -190,7 +190,7 @@ def test_gpu_random_triangles(gpu_lib, oracle_lib):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("config", range(10))
+@pytest.mark.parametrize("config", range(12))
 def test_gpu_spheres_every_configuration(gpu_lib, oracle_lib, config):
     def build(ctx, **kw):
         ctx.set_raster_config(config)
