@@ -626,6 +626,8 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": measured_traffic(args.workload),
             "bytes_per_triangle": round(bpt, 3),
+            "bytes_counted": "per submitted triangle (each instance's own index and attribute fetches)",
+            "unique_input_bytes": unique_input_bytes(scene.info) if args.workload == "spheres" else None,
         },
     }
     if args.workload == "spheres" and not args.headline_only:
@@ -643,6 +645,12 @@ def main():
         if not args.headline_only:
             out["cpu_baselines"] = run_cpu_baselines()
     print(json.dumps(out), flush=True)
+
+
+def unique_input_bytes(info):
+    """The sphere frame's distinct input bytes per launch: the shared vertex
+    and index buffers once plus the per-instance data (64 B a transform)."""
+    return int(info["vertices_per_instance"] * 32 + info["triangles_per_instance"] * 12 + info["instances"] * 64)
 
 
 def measured_traffic(workload):
