@@ -187,6 +187,9 @@ struct DrawDesc
     uint32_t iboCount;
     const uint8_t* instVbo;    // per-instance mat4 array (or null)
     uint64_t instVboBytes;
+    uint64_t posBase;          // first entry of the draw's vertex-pass positions, or kNoPositions
+    uint32_t posVerts;         // vertices per instance in them (ids below it; others run the program)
+    uint32_t pad32;
     const uint8_t* tex;        // sampled texture texels (or null)
     uint32_t texType;
     uint32_t texW, texH;
@@ -338,6 +341,21 @@ struct FrameCtrl
 constexpr int kCounterSlots = 64;
 static_assert(sizeof(FrameCtrl::CounterSlot) == 64, "one cache line per counter slot");
 
+// The vertex pass (k_vertex): an indexed triangle draw whose vertices are
+// shared by several triangles has each (instance, vertex) clip position
+// computed once, into a per-batch array, before setup; setup, emitting and
+// clipping then read positions instead of running the position program per
+// triangle corner.  Work items are blocks of kVertexItem entries, one
+// workgroup each, the draw of a block found in a table of (first block, draw)
+// pairs.
+constexpr uint64_t kNoPositions = ~0ull;
+constexpr uint32_t kVertexBlock = 256;                  // threads of a vertex-pass workgroup
+constexpr uint32_t kVertexPerThread = 4;                // entries per thread and work item
+constexpr uint32_t kVertexItem = kVertexBlock * kVertexPerThread;   // entries per work item
+constexpr uint64_t kMaxPositionEntries = 1ull << 27;   // 2 GiB of positions per batch at most
+constexpr uint64_t kMinPassEntries = 1ull << 16;       // smaller draws keep the setup pass's vertex cache
+struct VtxDraw { uint32_t firstBlock, draw; };
+
 struct SetupParams
 {
     const DrawDesc* draws;
@@ -363,6 +381,10 @@ struct SetupParams
     uint32_t* survChunk;          // per block of kSetupBlock survivors: the chunk of its first survivor
     uint32_t numChunks;
     FrameCtrl* ctrl;
+    float4* positions;            // vertex-pass positions (draws with posBase != kNoPositions)
+    const VtxDraw* vtxDraws;      // by first block
+    uint32_t numVtxDraws;
+    uint32_t vtxBlocks;           // vertex-pass work items (kVertexItem entries each)
 };
 
 // Heavy tiles.  A tile whose record list is long is split into parts of

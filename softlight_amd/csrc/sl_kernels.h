@@ -9,6 +9,7 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
 hipError_t launch_fill_u64(unsigned long long* p, unsigned long long v, uint64_t n, hipStream_t s);
 // Clears the control block before a frame, keeping (and adding the last
 // frame's overflow / error flags to) FrameCtrl::sticky.
+hipError_t launch_vertex(const SetupParams& p, hipStream_t s);
 hipError_t launch_ctrl_reset(FrameCtrl* ctrl, hipStream_t s);
 hipError_t launch_setup(const SetupParams& p, uint64_t chunks, hipStream_t s);
 hipError_t launch_chunk_scan(const SetupParams& p, hipStream_t s);

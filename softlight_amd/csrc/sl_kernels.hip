@@ -166,6 +166,26 @@ __device__ __forceinline__ void classify_triangle(const DrawDesc& D, TriIn& T)
     T.status = clip_status(T.pos[0], T.pos[1], T.pos[2]);
 }
 
+// Clip positions of a triangle's corners from the vertex pass (a draw with
+// posBase != kNoPositions): an id the pass covered is read, any other (an
+// index past the vertex buffer) runs the position program here.  The pass ran
+// the same program on the same inputs, so the bits equal the per-corner
+// evaluation's.
+__device__ __forceinline__ void positions_from_pass(const SetupParams& P, const DrawDesc& D, const uint32_t vid[3],
+                                                    uint64_t inst, const bool want[3], float4 out[3])
+{
+    bool rest[3];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        rest[k] = want[k] && vid[k] >= D.posVerts;
+        any = any || rest[k];
+        if (want[k] && !rest[k]) out[k] = P.positions[D.posBase + inst * D.posVerts + vid[k]];
+    }
+    if (any) vertex_positions3(D, P.uniforms, vid, inst, rest, out);
+}
+
 // Fetch, vertex positions, cull, scissor matrix and clip status of one
 // triangle (SL_TriProcessor::process_verts, SL_TriProcessor.cpp:574-631).
 // Varyings are left to load_varyings: most triangles never need them.
@@ -175,7 +195,8 @@ __device__ __forceinline__ void load_triangle_pos(const SetupParams& P, uint64_t
     load_triangle_ids(g, D, di, T);
     const uint32_t per = verts_per_prim(D.prim);
     const bool need[3] = {true, per > 1, per > 2};
-    vertex_positions3(D, P.uniforms, T.vid, T.inst, need, T.pos);
+    if (D.posBase != kNoPositions) positions_from_pass(P, D, T.vid, T.inst, need, T.pos);
+    else vertex_positions3(D, P.uniforms, T.vid, T.inst, need, T.pos);
     classify_triangle<kKinds>(D, T);
 }
 
@@ -726,6 +747,109 @@ __device__ __forceinline__ void write_prim_record(const SetupParams& P, const Dr
     if (D.prim == kPrimLine) put(C.tr2);
 }
 
+// The vertex pass: one workgroup per work item of kVertexItem (instance,
+// vertex) entries of the batch's qualifying draws; each thread evaluates
+// kVertexPerThread entries (all their loads before any store, so the fetches
+// overlap) and writes each position once.  (A persistent grid claiming items
+// from device-side counters spent its time in the claims: returning atomics
+// on one address from every XCD serialise at the memory side, ~0.4 us each.)
+__global__ __launch_bounds__(kVertexBlock) void k_vertex(SetupParams P)
+{
+    const uint32_t item = blockIdx.x;
+    if (item >= P.vtxBlocks) return;
+    uint32_t lo = 0, hi = P.numVtxDraws;
+    while (hi - lo > 1)
+    {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (P.vtxDraws[mid].firstBlock <= item) lo = mid;
+        else hi = mid;
+    }
+    const VtxDraw vd = P.vtxDraws[lo];
+    // the draw descriptor, staged once in LDS and read into scalar registers
+    __shared__ DrawDesc sD;
+    static_assert(sizeof(DrawDesc) % 4 == 0 && sizeof(DrawDesc) / 4 <= kVertexBlock, "draw descriptor staging");
+    if (threadIdx.x < sizeof(DrawDesc) / 4)
+        reinterpret_cast<uint32_t*>(&sD)[threadIdx.x] = reinterpret_cast<const uint32_t*>(P.draws + vd.draw)[threadIdx.x];
+    __syncthreads();
+    DrawDesc D;
+    {
+        uint32_t w[sizeof(DrawDesc) / 4];
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(DrawDesc) / 4); ++i)
+            w[i] = __builtin_amdgcn_readfirstlane(reinterpret_cast<const uint32_t*>(&sD)[i]);
+        __builtin_memcpy(&D, w, sizeof(DrawDesc));
+    }
+    const uint32_t nv = D.posVerts;
+    const uint64_t total = (uint64_t)nv * (D.numTris / D.trisPerInstance);
+    const uint64_t base = (uint64_t)(item - vd.firstBlock) * kVertexItem;
+    // the item's first entry as (instance, vertex); entries advance by
+    // kVertexBlock, wrapping into the next instance
+    const uint64_t inst0 = base / nv;
+    const uint32_t rem0 = (uint32_t)(base - inst0 * nv);
+    // The transforming programs' matrices (mvp and the model matrix of each
+    // instance the item touches) are staged once in LDS: read per thread,
+    // 32 broadcast loads per entry set this kernel's time.  The products are
+    // the program's own (mat_mul on the same values).
+    constexpr uint32_t kMaxItemInstances = 8;
+    __shared__ float sMvp[16];
+    __shared__ float sModel[kMaxItemInstances][16];
+    const uint32_t span = (uint32_t)((rem0 + kVertexItem - 1u) / nv + 1u);
+    const bool staged = span <= kMaxItemInstances &&
+                        (D.vsProg == SL_PROGRAM_LAMBERT_TEXTURED || D.vsProg == SL_PROGRAM_LAMBERT_MRT ||
+                         D.vsProg == SL_PROGRAM_COLOR_TEXTURED);
+    const sl_std_uniforms* u = reinterpret_cast<const sl_std_uniforms*>(P.uniforms + D.uniOffset);
+    if (staged)
+    {
+        if (threadIdx.x < 16) sMvp[threadIdx.x] = u->mvp[threadIdx.x];
+        if (threadIdx.x < span)
+        {
+            float m[16];
+            model_matrix(D, u, inst0 + threadIdx.x, m);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) sModel[threadIdx.x][i] = m[i];
+        }
+    }
+    __syncthreads();
+    float4 pos[kVertexPerThread];
+#pragma unroll
+    for (int j = 0; j < (int)kVertexPerThread; ++j)
+    {
+        uint64_t inst = inst0;
+        uint32_t vid = rem0 + threadIdx.x + (uint32_t)j * kVertexBlock;
+        while (vid >= nv)
+        {
+            vid -= nv;
+            ++inst;
+        }
+        const bool in = base + threadIdx.x + (uint64_t)j * kVertexBlock < total;
+        pos[j] = f4(0.f, 0.f, 0.f, 0.f);
+        if (staged)
+        {
+            if (in)
+            {
+                const float4 p = read_attr(D, 0, vid, 1.f);
+                const float4 world = mat_mul(sModel[inst - inst0], f4(p.x, p.y, p.z, 1.f));
+                pos[j] = mat_mul(sMvp, world);
+            }
+        }
+        else
+        {
+            const uint32_t v[3] = {vid, vid, vid};
+            const bool one[3] = {in, false, false};
+            float4 t[3];
+            t[0] = f4(0.f, 0.f, 0.f, 0.f);
+            vertex_positions3(D, P.uniforms, v, inst, one, t);
+            pos[j] = t[0];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < (int)kVertexPerThread; ++j)
+    {
+        const uint64_t e = base + threadIdx.x + (uint64_t)j * kVertexBlock;
+        if (e < total) P.positions[D.posBase + e] = pos[j];
+    }
+}
+
 // 8 waves per SIMD: the register budget that keeps every wave slot usable
 __global__ __launch_bounds__(kSetupBlock, 8) void k_setup(SetupParams P)
 {
@@ -766,7 +890,16 @@ __global__ __launch_bounds__(kSetupBlock, 8) void k_setup(SetupParams P)
     const DrawDesc& DU = DR;
     const bool live = g < P.totalTris && g - DU.triBase < DU.numTris;
     if (live) load_triangle_ids(g, DU, drawIdx, T);
-    if (DU.indexed)
+    if (DU.posBase != kNoPositions)
+    {
+        // positions computed once per (instance, vertex) by the vertex pass
+        if (live)
+        {
+            const bool all3[3] = {true, true, true};
+            positions_from_pass(P, DU, T.vid, T.inst, all3, T.pos);
+        }
+    }
+    else if (DU.indexed)
     {
         // Per-chunk vertex cache: a run of kSetupBlock triangles of an indexed
         // mesh references far fewer distinct vertices than 3 * kSetupBlock,
@@ -1616,6 +1749,14 @@ hipError_t launch_ctrl_reset(FrameCtrl* ctrl, hipStream_t s)
 {
     (void)hipGetLastError();   // drop a stale error from an earlier call
     hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(256), 0, s, ctrl);
+    return hipGetLastError();
+}
+
+hipError_t launch_vertex(const SetupParams& p, hipStream_t s)
+{
+    (void)hipGetLastError();   // drop a stale error from an earlier call
+    if (p.vtxBlocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_vertex, dim3(p.vtxBlocks), dim3(kVertexBlock), 0, s, p);
     return hipGetLastError();
 }
 

@@ -81,6 +81,7 @@ struct FrameRes
     // counter (absent: no split)
     DevBuf items, itemCount, partDepth, partRes, partDone;
     uint32_t maxParts = 0;
+    DevBuf positions, vtxDraws;   // the vertex pass's output and work table
 };
 
 struct FrameGraph
@@ -109,6 +110,8 @@ struct Batch
     std::vector<uint8_t> uni;
     uint64_t totalTris = 0;
     uint32_t maxVary = 1;
+    std::vector<VtxDraw> vtxDraws;                 // draws with vertex-pass positions
+    uint64_t posEntries = 0;                       // their (instance, vertex) entries, block-aligned per draw
 
     bool empty() const { return !clearColor && !clearDepth && draws.empty(); }
     void reset() { *this = Batch{}; }
@@ -482,6 +485,11 @@ int ensure_frame(sl_context* c, FrameRes& R, const Plan& p, const Batch& B)
     if ((rc = ensure(c, R.clip, sizeof(ClipJob) * (size_t)R.clipCap)) != SL_OK) return rc;
     if ((rc = ensure(c, R.binCounts, sizeof(uint32_t) * (size_t)kMaxBinBlocks * p.numTiles)) != SL_OK) return rc;
     if ((rc = ensure(c, R.binTotals, sizeof(uint32_t) * (size_t)p.numTiles)) != SL_OK) return rc;
+    if (B.posEntries > 0)
+    {
+        if ((rc = ensure(c, R.positions, sizeof(float4) * (size_t)B.posEntries)) != SL_OK) return rc;
+        if ((rc = ensure(c, R.vtxDraws, sizeof(VtxDraw) * B.vtxDraws.size())) != SL_OK) return rc;
+    }
     if (c->splitRecs > 0)
     {
         const uint32_t parts = split_parts(p);
@@ -522,6 +530,13 @@ SetupParams setup_params(const Plan& p, const Batch& B, FrameRes& R)
     sp.survChunk = sp.blockSurv + chunk_blocks(p.chunks);
     sp.numChunks = (uint32_t)p.chunks;
     sp.ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
+    if (B.posEntries > 0)
+    {
+        sp.positions = reinterpret_cast<float4*>(R.positions.p);
+        sp.vtxDraws = reinterpret_cast<const VtxDraw*>(R.vtxDraws.p);
+        sp.numVtxDraws = (uint32_t)B.vtxDraws.size();
+        sp.vtxBlocks = (uint32_t)(B.posEntries / kVertexItem);
+    }
     return sp;
 }
 
@@ -608,6 +623,7 @@ int enqueue_front(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, hip
     if (B.totalTris == 0) return SL_OK;
     const SetupParams sp = setup_params(p, B, R);
     if ((rc = check(c, launch_fill_u32(reinterpret_cast<uint32_t*>(sp.blockAgg), 0u, 3 * chunk_blocks(p.chunks), s), "fill block sums")) != SL_OK) return rc;
+    if ((rc = check(c, launch_vertex(sp, s), "k_vertex")) != SL_OK) return rc;
     if (evSetup0 && (rc = check(c, hipEventRecord(evSetup0, s), "event")) != SL_OK) return rc;
     if ((rc = check(c, launch_setup(sp, p.chunks, s), "k_setup")) != SL_OK) return rc;
     if (evSetup1 && (rc = check(c, hipEventRecord(evSetup1, s), "event")) != SL_OK) return rc;
@@ -668,6 +684,10 @@ int upload_batch(sl_context* c, FrameRes& R, const Batch& B, hipStream_t s)
     if (B.draws.empty()) return SL_OK;
     if ((rc = check(c, hipMemcpyAsync(R.draws.p, B.draws.data(), sizeof(DrawDesc) * B.draws.size(), hipMemcpyHostToDevice, s), "upload draws")) != SL_OK) return rc;
     if (!B.uni.empty() && (rc = check(c, hipMemcpyAsync(R.uni.p, B.uni.data(), B.uni.size(), hipMemcpyHostToDevice, s), "upload uniforms")) != SL_OK) return rc;
+    if (!B.vtxDraws.empty() &&
+        (rc = check(c, hipMemcpyAsync(R.vtxDraws.p, B.vtxDraws.data(), sizeof(VtxDraw) * B.vtxDraws.size(), hipMemcpyHostToDevice, s),
+                    "upload vertex-pass table")) != SL_OK)
+        return rc;
     return SL_OK;
 }
 
@@ -824,6 +844,14 @@ int execute_batch(sl_context* c, bool staticScratch = false)
     FrameRes& R = staticScratch ? local : c->res;
     uint8_t* base = nullptr;
     if (staticScratch && (rc = static_scratch(c, &base)) != SL_OK) return rc;
+    if (staticScratch)
+    {
+        // the fixed array holds no vertex-pass positions: every corner runs
+        // the position program (the results are the same bits)
+        for (DrawDesc& d : B.draws) d.posBase = kNoPositions;
+        B.vtxDraws.clear();
+        B.posEntries = 0;
+    }
     uint64_t need[3] = {0, 0, 0};
     FrameCtrl* ctrl = nullptr;
     float msVertex = 0.f, msBin = 0.f, msRaster = 0.f;
@@ -1164,6 +1192,29 @@ int record_draw(sl_context* c, const sl_mesh& m, uint64_t numInstances, size_t s
     }
     d.vsProg = sh.vsProg;
     d.fsProg = sh.fsProg;
+    d.posBase = kNoPositions;
+    {
+        // the vertex pass pays where corners share vertices over a large
+        // draw: indexed triangles of a transforming program whose vertex
+        // buffer holds at most two vertices per triangle (a closed mesh has
+        // about a half), with at least kMinPassEntries (instance, vertex)
+        // entries (for small meshes the setup pass's per-chunk vertex cache
+        // is cheaper than an extra pass)
+        const bool transforms = sh.vsProg == SL_PROGRAM_LAMBERT_TEXTURED || sh.vsProg == SL_PROGRAM_LAMBERT_MRT ||
+                                sh.vsProg == SL_PROGRAM_COLOR_TEXTURED;
+        const int64_t stride = vao.nb > 0 ? (int64_t)vao.b[0].stride : 0;
+        const uint64_t verts = (vbo && stride > 0) ? (vbo->bytes + (uint64_t)stride - 1) / (uint64_t)stride : 0;
+        const uint64_t entries = verts * numInstances;
+        const uint64_t blocks = (entries + kVertexItem - 1) / kVertexItem;
+        if (transforms && indexed && (prim == kPrimFill || prim == kPrimWire) && verts > 0 && verts < (1ull << 31) &&
+            entries <= 2 * numTris && entries >= kMinPassEntries && B.posEntries + blocks * kVertexItem <= kMaxPositionEntries)
+        {
+            d.posBase = B.posEntries;
+            d.posVerts = (uint32_t)verts;
+            B.vtxDraws.push_back(VtxDraw{(uint32_t)(B.posEntries / kVertexItem), (uint32_t)B.draws.size()});
+            B.posEntries += blocks * kVertexItem;
+        }
+    }
     d.cull = sh.cull;
     d.dtest = sh.dtest;
     d.dmask = sh.dmask;

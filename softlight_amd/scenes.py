@@ -311,6 +311,7 @@ def build_sphere_scene(ctx, seed=1, n_instances=64, stacks=256, sectors=256, W=1
                   W=W, H=H, tex_size=tex_size, radius=radius, extent=extent, mode=mode,
                   triangles_per_instance=idx.size // 3, vertices_per_instance=verts.shape[0])
     s.info["_keep"] = (verts, idx, inst)
+    s.info["vbo"] = vbo
     return s
 
 
