@@ -286,9 +286,10 @@ SL_HD int32_t wire_gap(int32_t a, int32_t b)
 struct ClipJob
 {
     uint32_t tri;              // global triangle index
-    uint32_t recOff;           // first output record
-    uint32_t pairOff;          // first output (tile, record) pair
+    uint32_t recOff;           // first output record, within the triangle's setup chunk
+    uint32_t pairOff;          // first output (tile, record) pair, within the chunk
     uint32_t nrec;             // records reserved by k_setup
+    uint32_t npair;            // pairs reserved by k_setup
 };
 
 // Setup runs as a counting pass, a chunk scan and an emitting pass.  Per

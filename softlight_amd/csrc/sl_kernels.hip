@@ -186,13 +186,11 @@ __device__ __forceinline__ void positions_from_pass(const SetupParams& P, const 
     if (any) vertex_positions3(D, P.uniforms, vid, inst, rest, out);
 }
 
-// Fetch, vertex positions, cull, scissor matrix and clip status of one
-// triangle (SL_TriProcessor::process_verts, SL_TriProcessor.cpp:574-631).
-// Varyings are left to load_varyings: most triangles never need them.
+// Vertex positions, cull, scissor matrix and clip status of a triangle whose
+// vertex ids and instance are in T.
 template <int kKinds = 3>
-__device__ __forceinline__ void load_triangle_pos(const SetupParams& P, uint64_t g, const DrawDesc& D, uint32_t di, TriIn& T)
+__device__ __forceinline__ void load_positions(const SetupParams& P, const DrawDesc& D, TriIn& T)
 {
-    load_triangle_ids(g, D, di, T);
     const uint32_t per = verts_per_prim(D.prim);
     const bool need[3] = {true, per > 1, per > 2};
     if (D.posBase != kNoPositions) positions_from_pass(P, D, T.vid, T.inst, need, T.pos);
@@ -200,13 +198,14 @@ __device__ __forceinline__ void load_triangle_pos(const SetupParams& P, uint64_t
     classify_triangle<kKinds>(D, T);
 }
 
-// Per-lane draw lookup (the clip kernel's jobs come from any draw).
-__device__ __forceinline__ void load_triangle(const SetupParams& P, uint64_t g, TriIn& T)
+// Fetch, vertex positions, cull, scissor matrix and clip status of one
+// triangle (SL_TriProcessor::process_verts, SL_TriProcessor.cpp:574-631).
+// Varyings are left to load_varyings: most triangles never need them.
+template <int kKinds = 3>
+__device__ __forceinline__ void load_triangle_pos(const SetupParams& P, uint64_t g, const DrawDesc& D, uint32_t di, TriIn& T)
 {
-    const uint32_t di = find_draw(P.draws, P.numDraws, g);
-    const DrawDesc& D = P.draws[di];
-    load_triangle_pos<1>(P, g, D, di, T);
-    load_varyings(P, D, T);
+    load_triangle_ids(g, D, di, T);
+    load_positions<kKinds>(P, D, T);
 }
 
 template <bool PAIRS = true>
@@ -956,6 +955,7 @@ __global__ __launch_bounds__(kSetupBlock, 8) void k_setup(SetupParams P)
     TriCount C;
     C.nrec = 0;
     C.npair = 0;
+    C.clipped = false;
     C.empty = false;
     C.needClip = false;
     if (live)
@@ -965,7 +965,6 @@ __global__ __launch_bounds__(kSetupBlock, 8) void k_setup(SetupParams P)
     }
     __shared__ float4 sClip[kSetupBlock / 64][18][kClipSlots];   // each wave's clipper slots
     count_clipped(P, DU, T, C, sClip[wave]);
-
     // chunk summary: exclusive scan of the packed counts (each survivor's
     // offset within the chunk), survivor ranks, staging words, totals
     const unsigned long long mine = ((unsigned long long)C.npair << kRecBits) | C.nrec;
@@ -996,10 +995,37 @@ __global__ __launch_bounds__(kSetupBlock, 8) void k_setup(SetupParams P)
         surv += sSurv[w];
         emp += sEmpty[w];
     }
-    if (C.nrec > 0)
+    const unsigned long long recBefore = before & ((1ull << kRecBits) - 1), pairBefore = before >> kRecBits;
+    if (C.nrec > 0) P.stage[(uint64_t)chunk * kSetupBlock + rank] = (unsigned long long)threadIdx.x | (recBefore << 8) | (pairBefore << 20);
+    // A straddling triangle's clip job, reserved here (one atomic per wave);
+    // its offsets are within the chunk, k_clip adds the chunk's prefix.
     {
-        const unsigned long long recBefore = before & ((1ull << kRecBits) - 1), pairBefore = before >> kRecBits;
-        P.stage[(uint64_t)chunk * kSetupBlock + rank] = (unsigned long long)threadIdx.x | (recBefore << 8) | (pairBefore << 20);
+        const unsigned long long m = __ballot(C.clipped);
+        if (m != 0ull)
+        {
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t base = 0u;
+            if (lane == leader) base = atomicAdd(&P.ctrl->clipJobs, (uint32_t)__popcll(m));
+            base = __shfl(base, leader, 64);
+            if (C.clipped)
+            {
+                const uint32_t job = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                if (job < P.clipCap)
+                {
+                    ClipJob J;
+                    J.tri = (uint32_t)g;
+                    J.recOff = (uint32_t)recBefore;
+                    J.pairOff = (uint32_t)pairBefore;
+                    J.nrec = C.nrec;
+                    J.npair = C.npair;
+                    P.clipJobs[job] = J;
+                }
+                else
+                {
+                    atomicExch(&P.ctrl->overflow, 1u);
+                }
+            }
+        }
     }
     if (threadIdx.x == 0)
     {
@@ -1124,7 +1150,6 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
     const int lane = t & 63;
     const int wave = t >> 6;
     __shared__ uint32_t sSP[kEmitBlock];
-    __shared__ float4 sClip[kEmitBlock / 64][18][kClipSlots];   // each wave's clipper slots
     const uint32_t S = P.ctrl->survivors;
     for (uint32_t k = blockIdx.x; (uint64_t)k * kEmitBlock < S; k += gridDim.x)
     {
@@ -1225,16 +1250,16 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
                 load_triangle_pos<kKinds>(P, g, DR, di, T);
                 count_triangle<kKinds>(P, DR, di, T, C);
             }
-            if (!PRIMS) count_clipped(P, DR, T, C, sClip[wave]);
         }
         else if (act2)
         {
             const DrawDesc& D = P.draws[di];
             load_triangle_pos<kKinds>(P, g, D, di, T);
             count_triangle<kKinds>(P, D, di, T, C);
-            if (!PRIMS) count_clipped(P, D, T, C, sClip[wave]);
         }
-        bool ok = act2;
+        // a straddling triangle is k_clip's (its job was reserved by the
+        // counting pass); any other survivor yields its record here
+        bool ok = act2 && !C.needClip;
         if (ok && C.nrec == 0)
         {
             atomicExch(&P.ctrl->error, 1u);
@@ -1247,19 +1272,7 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
             atomicExch(&P.ctrl->overflow, 1u);
             ok = false;
         }
-        // clip jobs: one reservation per wave
-        const bool wantJob = ok && C.clipped;
-        const unsigned long long jobs = __ballot(wantJob);
-        uint32_t job = 0;
-        if (jobs != 0ull)
-        {
-            const int leader = __ffsll((long long)jobs) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&P.ctrl->clipJobs, (uint32_t)__popcll(jobs));
-            base = __shfl(base, leader, 64);
-            job = base + (uint32_t)__popcll(jobs & ((1ull << lane) - 1ull));
-        }
-        if (ok && !C.clipped)
+        if (ok)
         {
             if (uni)
             {
@@ -1281,60 +1294,165 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
                                         D.nvary, T.draw, T.seq, pack_raster_state(D), C.tr);
             }
         }
-        else if (wantJob)
-        {
-            if (job < P.clipCap)
-            {
-                ClipJob J;
-                J.tri = (uint32_t)g;
-                J.recOff = recOff;
-                J.pairOff = (uint32_t)pairOff;
-                J.nrec = C.nrec;
-                P.clipJobs[job] = J;
-            }
-            else
-            {
-                atomicExch(&P.ctrl->overflow, 1u);
-            }
-        }
         // the records' bin entries, by the whole wave (every lane reaches this)
-        if (!PRIMS) write_pairs_wave(P, ok && !C.clipped, recOff, (uint32_t)pairOff, C.tr);
+        if (!PRIMS) write_pairs_wave(P, ok, recOff, (uint32_t)pairOff, C.tr);
     }
 }
 
 // ---------------------------------------------------------------------------
 // k_clip: finish the clipped triangles into their reserved slots
 // ---------------------------------------------------------------------------
+// One 16-lane group per job, four jobs per one-wave workgroup.  Lane k of a
+// group holds polygon vertex k: at each plane it tests its edge from vertex
+// k-1 (the last vertex for k = 0) and writes the edge's intersection and its
+// own vertex, when kept, to the next polygon at the position the group's
+// ballots give it, so the output order and every operation on a vertex are
+// those of the serial clipper (clip_polygon).  The fan's triangles are then
+// set up one per lane, their record and bin-entry offsets taken from
+// suffix sums over the group (the reverse fan numbers them from the last).
 constexpr int kClipBlock = 64;
+constexpr int kClipGroup = 16;
+constexpr int kClipGroups = kClipBlock / kClipGroup;
+constexpr int kClipVerts = 9;   // 3 + one per plane
 
 __global__ __launch_bounds__(kClipBlock) void k_clip(SetupParams P)
 {
-    // each thread's polygon buffers, interleaved across the threads in LDS
-    __shared__ float4 sPoly[9][kClipBlock], sTmp[9][kClipBlock];
-    __shared__ Vary4 sPVar[9][kClipBlock], sTVar[9][kClipBlock];
-    const PolyBuf<float4> poly{&sPoly[0][threadIdx.x], kClipBlock}, tpos{&sTmp[0][threadIdx.x], kClipBlock};
-    const PolyBuf<Vary4> pvar{&sPVar[0][threadIdx.x], kClipBlock}, tvar{&sTVar[0][threadIdx.x], kClipBlock};
+    __shared__ float4 sPos[kClipGroups][2][kClipVerts];
+    __shared__ Vary4 sVar[kClipGroups][2][kClipVerts];
+    const int lane = (int)threadIdx.x;   // the workgroup is one wave
+    const int gi = lane / kClipGroup, li = lane % kClipGroup;
+    const unsigned long long gmask = 0xFFFFull << (gi * kClipGroup);
+    const unsigned long long below = gmask & ((1ull << lane) - 1ull);
+    const unsigned long long above = gmask & ~below & ~(1ull << lane);
     const uint32_t njobs = min(P.ctrl->clipJobs, P.clipCap);
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < njobs; j += gridDim.x * blockDim.x)
+    // the loop bound is uniform over the workgroup: every lane reaches the barriers
+    for (uint32_t base = blockIdx.x * kClipGroups; base < njobs; base += gridDim.x * kClipGroups)
     {
-        const ClipJob J = P.clipJobs[j];
-        TriIn T;
-        load_triangle(P, J.tri, T);
-        const DrawDesc& D = P.draws[T.draw];
-        const int nv = D.nvary;
-        const int n = clip_polygon<true>(T.pos, T.var, nv, poly, pvar, tpos, tvar);
-        for (int k = 0; k < n; ++k) poly[k] = divide_and_snap(poly[k], D.vp);
-        uint32_t rec = J.recOff, pair = J.pairOff;
-        const uint32_t recEnd = J.recOff + J.nrec;
-        for (int i = n - 3; i >= 0 && rec < recEnd; --i)
+        const uint32_t j = base + (uint32_t)gi;
+        bool live = j < njobs;
+        ClipJob J = {};
+        uint64_t recOff = 0, pairOff = 0;
+        if (live)
         {
-            if (!bin_accept(poly[0], poly[i + 1], poly[i + 2])) continue;
-            const TileRect r = tile_rect(poly[0], poly[i + 1], poly[i + 2], P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
-            write_record(P, rec, pair, poly[0], poly[i + 1], poly[i + 2], pvar[0], pvar[i + 1], pvar[i + 2], nv,
-                         T.draw, T.seq, pack_raster_state(D), r);
-            ++rec;
-            pair += tile_count(r);
+            J = P.clipJobs[j];
+            // the job's output range: the chunk's prefix (chunk scan) plus the
+            // offsets within the chunk the counting pass reserved
+            const unsigned long long off = P.chunkInfo[J.tri / kSetupBlock].pre + J.recOff +
+                                           ((unsigned long long)J.pairOff << kRecBits);
+            recOff = off & ((1ull << kRecBits) - 1);
+            pairOff = off >> kRecBits;
+            if (recOff + J.nrec > P.recCap || pairOff + J.npair > P.pairCap)
+            {
+                if (li == 0) atomicExch(&P.ctrl->overflow, 1u);   // the host grows the buffers and runs setup again
+                live = false;
+            }
         }
+        TriIn T;
+        int nv = 0;
+        if (live)
+        {
+            // the fetch and vertex programs again; a chunk's triangles are
+            // all of one draw
+            const uint32_t di = P.chunkInfo[J.tri / kSetupBlock].draw;
+            const DrawDesc& D = P.draws[di];
+            load_triangle_pos<1>(P, J.tri, D, di, T);
+            load_varyings(P, D, T);
+            nv = D.nvary;
+            if (li == 0)   // static indices keep the triangle in registers
+            {
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                {
+                    sPos[gi][0][k] = T.pos[k];
+#pragma unroll
+                    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+                        if (i < nv) sVar[gi][0][k].v[i] = T.var[k].v[i];
+                }
+            }
+        }
+        int n = live ? 3 : 0, cur = 0;
+        __syncthreads();
+        for (int e = 0; e < 6; ++e)
+        {
+            const float4 E = clip_edge(e);
+            const bool mine = li < n;
+            const int pj = li == 0 ? n - 1 : li - 1;
+            float4 q0 = f4(0.f, 0.f, 0.f, 0.f), q1 = q0;
+            float t0 = 0.f, t1 = 0.f;
+            bool vis0 = false, vis1 = false;
+            if (mine)
+            {
+                q1 = sPos[gi][cur][li];
+                q0 = sPos[gi][cur][pj];
+                t1 = dot4(q1, E);
+                t0 = dot4(q0, E);
+                vis1 = !signbit(t1);
+                vis0 = !signbit(t0);
+            }
+            const bool cross = mine && vis0 != vis1, keep = mine && vis1;
+            const unsigned long long mc = __ballot(cross) & gmask, mk = __ballot(keep) & gmask;
+            // a plane that keeps every vertex leaves the polygon as it is
+            const bool allIn = __popcll(mk) == n;
+            if (n > 0 && !allIn)
+            {
+                const int nn = __popcll(mc) + __popcll(mk);
+                int b = __popcll(mc & below) + __popcll(mk & below);
+                const int nxt = cur ^ 1;
+                if (cross && b < kClipVerts)
+                {
+                    const float t = t0 * rcpf(t0 - t1);
+                    sPos[gi][nxt][b] = mix4(q0, q1, t);
+#pragma unroll
+                    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+                        if (i < nv) sVar[gi][nxt][b].v[i] = mix4(sVar[gi][cur][pj].v[i], sVar[gi][cur][li].v[i], t);
+                }
+                if (cross) ++b;
+                if (keep && b < kClipVerts)
+                {
+                    sPos[gi][nxt][b] = q1;
+#pragma unroll
+                    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+                        if (i < nv) sVar[gi][nxt][b].v[i] = sVar[gi][cur][li].v[i];
+                }
+                n = min(nn, kClipVerts);
+                cur = nxt;
+            }
+            __syncthreads();
+        }
+        if (n < 3) n = 0;
+        const DrawDesc* D = live ? &P.draws[T.draw] : nullptr;
+        if (li < n) sPos[gi][cur][li] = divide_and_snap(sPos[gi][cur][li], D->vp);
+        __syncthreads();
+        // fan triangle li: vertices 0, li + 1, li + 2
+        bool acc = false;
+        TileRect r = TileRect{0, -1, 0, -1};
+        float4 p0 = f4(0.f, 0.f, 0.f, 0.f), p1 = p0, p2 = p0;
+        if (li <= n - 3)
+        {
+            p0 = sPos[gi][cur][0];
+            p1 = sPos[gi][cur][li + 1];
+            p2 = sPos[gi][cur][li + 2];
+            acc = bin_accept(p0, p1, p2);
+            if (acc) r = tile_rect(p0, p1, p2, P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
+        }
+        // records are numbered from the last fan triangle down, at most the
+        // count the counting pass reserved
+        const uint32_t rank = (uint32_t)__popcll(__ballot(acc) & above);
+        acc = acc && rank < J.nrec;
+        const uint32_t cnt = acc ? tile_count(r) : 0u;
+        uint32_t suf = cnt;   // inclusive sum over this lane and the group's later lanes
+#pragma unroll
+        for (int o = 1; o < kClipGroup; o <<= 1)
+        {
+            const uint32_t v = (uint32_t)__shfl_down((int)suf, o, kClipGroup);
+            if (li + o < kClipGroup) suf += v;
+        }
+        if (acc)
+        {
+            write_record(P, (uint32_t)recOff + rank, (uint32_t)pairOff + (suf - cnt), p0, p1, p2, sVar[gi][cur][0],
+                         sVar[gi][cur][li + 1], sVar[gi][cur][li + 2], nv, T.draw, T.seq, pack_raster_state(*D), r);
+        }
+        __syncthreads();   // the next jobs overwrite the polygon buffers
     }
 }
 
@@ -1796,7 +1914,7 @@ hipError_t launch_clip(const SetupParams& p, uint32_t jobs, hipStream_t s)
 {
     (void)hipGetLastError();   // drop a stale error from an earlier call
     if (jobs == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_clip, dim3(grid_for(jobs, kClipBlock, 8192)), dim3(kClipBlock), 0, s, p);
+    hipLaunchKernelGGL(k_clip, dim3(grid_for(jobs, kClipGroups, 8192)), dim3(kClipBlock), 0, s, p);
     return hipGetLastError();
 }
 

@@ -257,7 +257,7 @@ struct sl_context
     FrameGraph graph;                              // captured frame
     uint64_t lastTotals[3] = {0, 0, 0};            // records, bin entries, clip jobs of the last eager frame
     FrameCtrl* hCtrl = nullptr;                    // pinned readback of the control block
-    hipEvent_t ev[10] = {};                        // eager frames: 0-5 stages, 6-7 graph replays, 8-9 counting pass
+    hipEvent_t ev[11] = {};                        // eager frames: 0-5 stages, 6-7 graph replays, 8-9 counting pass, 10 binning
     hipEvent_t bwdEv[2] = {};                      // first / last backward since the last collection
     hipEvent_t gEv[4] = {};                        // graph replays: setup start/end, raster start/end
     uint64_t bwdPending = 0;
@@ -612,9 +612,10 @@ RasterParams raster_params(const Plan& p, const Batch& B, FrameRes& R, bool have
 }
 
 // Control-block reset, block-sum reset and the setup passes: counting (the
-// events bracket it), chunk scan, emitting.
+// events bracket it), chunk scan, emitting, clipping.
 int enqueue_front(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, hipStream_t s,
-                  hipEvent_t evSetup0 = nullptr, hipEvent_t evSetup1 = nullptr)
+                  hipEvent_t evSetup0 = nullptr, hipEvent_t evSetup1 = nullptr, hipEvent_t evClip0 = nullptr,
+                  hipEvent_t evClip1 = nullptr)
 {
     int rc;
     FrameCtrl* ctrl = reinterpret_cast<FrameCtrl*>(R.ctrl.p);
@@ -635,23 +636,30 @@ int enqueue_front(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, hip
         prims = prims || pl;
         tris = tris || !pl;
     }
-    return check(c, launch_emit(sp, tris, prims, s), "k_emit");
+    if ((rc = check(c, launch_emit(sp, tris, prims, s), "k_emit")) != SL_OK) return rc;
+    // the straddling triangles, whose jobs the counting pass reserved (a
+    // grid for the job capacity: the count stays on the device)
+    if (tris)
+    {
+        if (evClip0 && (rc = check(c, hipEventRecord(evClip0, s), "event")) != SL_OK) return rc;
+        if ((rc = check(c, launch_clip(sp, R.clipCap, s), "k_clip")) != SL_OK) return rc;
+        if (evClip1 && (rc = check(c, hipEventRecord(evClip1, s), "event")) != SL_OK) return rc;
+    }
+    return SL_OK;
 }
 
-// Clip, binning and raster.  clipJobs / entries size the grids (a graph uses
-// the capacities; the kernels read the actual counts from the control block).
-int enqueue_back(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, uint32_t clipJobs, uint32_t entries,
-                 hipStream_t s, hipEvent_t evClip0, hipEvent_t evClip1, hipEvent_t evBin1)
+// Binning and raster.  entries sizes the grids (a graph uses the capacity;
+// the kernels read the actual counts from the control block).
+int enqueue_back(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, uint32_t entries, hipStream_t s,
+                 hipEvent_t evBin0, hipEvent_t evBin1)
 {
     int rc;
     RasterParams rp = raster_params(p, B, R, B.totalTris > 0 && entries > 0, c->splitAll);
     rp.wgPerCU = c->wgPerCU;
+    if (evBin0) (void)hipEventRecord(evBin0, s);
     if (B.totalTris > 0)
     {
         const SetupParams sp = setup_params(p, B, R);
-        if (evClip0) (void)hipEventRecord(evClip0, s);
-        if ((rc = check(c, launch_clip(sp, clipJobs, s), "k_clip")) != SL_OK) return rc;
-        if (evClip1) (void)hipEventRecord(evClip1, s);
         if (entries > 0)
         {
             // with the heavy-tile split on, the tile scan also writes the
@@ -879,7 +887,7 @@ int execute_batch(sl_context* c, bool staticScratch = false)
             break;
         }
         (void)hipEventRecord(c->ev[0], s);
-        if ((rc = enqueue_front(c, R, p, B, s, c->ev[8], c->ev[9])) != SL_OK) return rc;
+        if ((rc = enqueue_front(c, R, p, B, s, c->ev[8], c->ev[9], c->ev[2], c->ev[3])) != SL_OK) return rc;
         (void)hipEventRecord(c->ev[1], s);
         c->stats.setup_launches += 1;
         if ((rc = check(c, hipMemcpyAsync(c->hCtrl, ctrl, sizeof(FrameCtrl), hipMemcpyDeviceToHost, s), "read ctrl")) != SL_OK) return rc;
@@ -915,12 +923,7 @@ int execute_batch(sl_context* c, bool staticScratch = false)
         totalPairs = c->hCtrl->totalPairs;
         jobs = c->hCtrl->clipJobs;
     }
-    else
-    {
-        (void)hipEventRecord(c->ev[3], s);
-    }
-    if ((rc = enqueue_back(c, R, p, B, jobs, totalPairs, s, B.totalTris ? c->ev[2] : nullptr,
-                           B.totalTris ? c->ev[3] : nullptr, c->ev[4])) != SL_OK) return rc;
+    if ((rc = enqueue_back(c, R, p, B, totalPairs, s, c->ev[10], c->ev[4])) != SL_OK) return rc;
     (void)hipEventRecord(c->ev[5], s);
     if ((rc = check(c, hipMemcpyAsync(c->hCtrl, ctrl, sizeof(FrameCtrl), hipMemcpyDeviceToHost, s), "read ctrl")) != SL_OK) return rc;
     if ((rc = check(c, hipStreamSynchronize(s), "frame sync")) != SL_OK) return rc;
@@ -928,12 +931,12 @@ int execute_batch(sl_context* c, bool staticScratch = false)
     float t = 0.f;
     if (B.totalTris > 0)
     {
+        // k_clip's own time, inside the setup span
         (void)hipEventElapsedTime(&t, c->ev[2], c->ev[3]);
-        msVertex += t;
         c->stats.ms_clip_kernel += t;
-        (void)hipEventElapsedTime(&t, c->ev[3], c->ev[4]);
-        msBin += t;
     }
+    (void)hipEventElapsedTime(&t, c->ev[10], c->ev[4]);
+    msBin += t;
     (void)hipEventElapsedTime(&t, c->ev[4], c->ev[5]);
     msRaster += t;
     c->stats.triangles_binned += (uint64_t)totalRecs + c->hCtrl->emptyRecs;
@@ -966,7 +969,8 @@ void graph_release(sl_context* c)
     G.graph = nullptr;
     for (DevBuf* b : {&G.res.draws, &G.res.uni, &G.res.recs, &G.res.vars, &G.res.keys, &G.res.vals, &G.res.sorted,
                       &G.res.clip, &G.res.status, &G.res.ranges, &G.res.ctrl, &G.res.binCounts, &G.res.binTotals, &G.res.order,
-                      &G.res.items, &G.res.itemCount, &G.res.partDepth, &G.res.partRes, &G.res.partDone}) release(*b);
+                      &G.res.items, &G.res.itemCount, &G.res.partDepth, &G.res.partRes, &G.res.partDone,
+                      &G.res.positions, &G.res.vtxDraws}) release(*b);
     G.res = FrameRes{};
     if (G.hCtrl) (void)hipHostFree(G.hCtrl);
     G.hCtrl = nullptr;
@@ -1011,7 +1015,7 @@ int graph_capture(sl_context* c)
     {
         if ((rc = check(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture")) != SL_OK) { graph_release(c); return rc; }
         int rc1 = timed ? enqueue_front(c, G.res, p, B, s, c->gEv[0], c->gEv[1]) : enqueue_front(c, G.res, p, B, s);
-        int rc2 = (rc1 == SL_OK) ? enqueue_back(c, G.res, p, B, G.res.clipCap, G.res.pairCap, s, nullptr, nullptr,
+        int rc2 = (rc1 == SL_OK) ? enqueue_back(c, G.res, p, B, G.res.pairCap, s, nullptr,
                                                 timed ? c->gEv[2] : nullptr) : rc1;
         if (rc2 == SL_OK && timed) rc2 = check(c, hipEventRecord(c->gEv[3], s), "event");
         int rc3 = (rc2 == SL_OK) ? check(c, hipMemcpyAsync(G.hCtrl, G.res.ctrl.p, sizeof(FrameCtrl), hipMemcpyDeviceToHost, s), "read ctrl") : rc2;
@@ -1388,7 +1392,8 @@ void sl_context_destroy(sl_context* c)
     graph_release(c);
     for (DevBuf* b : {&c->res.draws, &c->res.uni, &c->res.recs, &c->res.vars, &c->res.keys, &c->res.vals, &c->res.sorted,
                       &c->res.clip, &c->res.status, &c->res.ranges, &c->res.ctrl, &c->res.binCounts, &c->res.binTotals, &c->res.order,
-                      &c->res.items, &c->res.itemCount, &c->res.partDepth, &c->res.partRes, &c->res.partDone}) release(*b);
+                      &c->res.items, &c->res.itemCount, &c->res.partDepth, &c->res.partRes, &c->res.partDone,
+                      &c->res.positions, &c->res.vtxDraws}) release(*b);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     for (auto& f : c->fbos) release(f.gradDev);
     for (auto& e : c->bwdEv) if (e) (void)hipEventDestroy(e);

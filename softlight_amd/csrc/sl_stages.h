@@ -184,17 +184,18 @@ __device__ __forceinline__ void vertex_positions3(const DrawDesc& d, const uint8
 __device__ __forceinline__ void vertex_varyings3(const DrawDesc& d, const uint8_t* uniArena, const uint32_t vid[3],
                                                  uint64_t inst, Vary4 out[3])
 {
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i) out[k].v[i] = f4(0.f, 0.f, 0.f, 0.f);
+    // Every program's results are gathered in registers and stored once, at
+    // the same places whatever the program: stores to different varyings on
+    // different branches get merged into one store at a computed offset,
+    // which moves the whole triangle into scratch memory.
+    const float4 zero = f4(0.f, 0.f, 0.f, 0.f);
+    float4 r0[3] = {zero, zero, zero}, r1[3] = {zero, zero, zero}, r2[3] = {zero, zero, zero};
     if (d.vsProg == SL_PROGRAM_PASSTHROUGH_GOURAUD)
     {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) out[k].v[0] = read_attr(d, 1, vid[k], 1.f);
-        return;
+        for (int k = 0; k < 3; ++k) r0[k] = read_attr(d, 1, vid[k], 1.f);
     }
-    if (d.vsProg == SL_PROGRAM_LAMBERT_TEXTURED || d.vsProg == SL_PROGRAM_LAMBERT_MRT || d.vsProg == SL_PROGRAM_COLOR_TEXTURED)
+    else if (d.vsProg == SL_PROGRAM_LAMBERT_TEXTURED || d.vsProg == SL_PROGRAM_LAMBERT_MRT || d.vsProg == SL_PROGRAM_COLOR_TEXTURED)
     {
         float4 a[3], uv[3];
 #pragma unroll
@@ -208,17 +209,18 @@ __device__ __forceinline__ void vertex_varyings3(const DrawDesc& d, const uint8_
             const sl_std_uniforms* u = reinterpret_cast<const sl_std_uniforms*>(uniArena + d.uniOffset);
             float m[16];
             model_matrix(d, u, inst, m);
+            const bool mrt = d.vsProg == SL_PROGRAM_LAMBERT_MRT;
 #pragma unroll
             for (int k = 0; k < 3; ++k)
             {
                 const float4 n = mat_mul(m, f4(a[k].x, a[k].y, a[k].z, 0.f));
-                out[k].v[0] = f4(uv[k].x, uv[k].y, 0.f, 0.f);
-                out[k].v[1] = f4(n.x, n.y, n.z, 0.f);
-                if (d.vsProg == SL_PROGRAM_LAMBERT_MRT)
+                r0[k] = f4(uv[k].x, uv[k].y, 0.f, 0.f);
+                r1[k] = f4(n.x, n.y, n.z, 0.f);
+                if (mrt)
                 {
                     const float4 p = read_attr(d, 0, vid[k], 1.f);
                     const float4 world = mat_mul(m, f4(p.x, p.y, p.z, 1.f));
-                    out[k].v[2] = f4(world.x, world.y, world.z, 1.f);
+                    r2[k] = f4(world.x, world.y, world.z, 1.f);
                 }
             }
         }
@@ -227,10 +229,19 @@ __device__ __forceinline__ void vertex_varyings3(const DrawDesc& d, const uint8_
 #pragma unroll
             for (int k = 0; k < 3; ++k)
             {
-                out[k].v[0] = f4(a[k].x, a[k].y, a[k].z, 1.f);
-                out[k].v[1] = f4(uv[k].x, uv[k].y, 0.f, 0.f);
+                r0[k] = f4(a[k].x, a[k].y, a[k].z, 1.f);
+                r1[k] = f4(uv[k].x, uv[k].y, 0.f, 0.f);
             }
         }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        out[k].v[0] = r0[k];
+        out[k].v[1] = r1[k];
+        out[k].v[2] = r2[k];
+#pragma unroll
+        for (int i = 3; i < SL_SHADER_MAX_VARYING_VECTORS; ++i) out[k].v[i] = zero;
     }
 }
 
