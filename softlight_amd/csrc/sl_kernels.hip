@@ -1673,14 +1673,22 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(const uint32_t* __restric
         }
     }
     __syncthreads();
+    if (threadIdx.x < 64)
+    {
+        // bucket b's first launch slot: the tiles of the buckets above it
+        const int l = (int)threadIdx.x;
+        const uint32_t c = (l <= 32) ? sCnt[32 - l] : 0u;
+        uint32_t inc = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1)
+        {
+            const uint32_t v = (uint32_t)__shfl_up((int)inc, o, 64);
+            if (l >= o) inc += v;
+        }
+        if (l <= 32) sBucket[32 - l] = inc - c;
+    }
     if (threadIdx.x == 0)
     {
-        uint32_t acc = 0;
-        for (int b = 32; b >= 0; --b)
-        {
-            sBucket[b] = acc;
-            acc += sCnt[b];
-        }
         if (SP.items != nullptr)
         {
             SP.itemCount[0] = nParts + (uint32_t)numTiles;
@@ -1725,8 +1733,9 @@ __global__ __launch_bounds__(64) void k_bin_scatter(const uint32_t* __restrict__
     if (start >= n) return;
     const uint32_t end = min(start + per, n);
     const uint32_t* row = counts + (uint64_t)blockIdx.x * numTiles;
-    // the block's base slot per tile; eight tiles' loads in flight per lane
-    constexpr int kInit = 8;
+    // the block's base slot per tile; 32 tiles' loads in flight per lane (the
+    // pass is a chain of load round trips: fewer, wider rounds)
+    constexpr int kInit = 32;
     for (int t0 = 0; t0 < numTiles; t0 += 64 * kInit)
     {
         uint32_t rx[kInit], rw[kInit];
@@ -1747,50 +1756,75 @@ __global__ __launch_bounds__(64) void k_bin_scatter(const uint32_t* __restrict__
     __syncthreads();
     const int lane = threadIdx.x;
     const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);   // lanes <= lane
-    // the next 64 entries' loads are issued before the current ones are placed
-    uint32_t nkey = (start + lane < end) ? keys[start + lane] : 0xFFFFFFFFu;
-    uint32_t nval = (start + lane < end) ? vals[start + lane] : 0u;
-    for (uint32_t c = start; c < end; c += 64)
+    // Entries are placed 64 at a time, in order.  A step sorts kAhead chunks
+    // and issues the loads of the next kAhead chunks first.
+    constexpr int kAhead = 4;
+    uint32_t pk[kAhead], pv[kAhead];
+#pragma unroll
+    for (int a = 0; a < kAhead; ++a)
     {
-        const uint32_t key = nkey, rawVal = nval;
-        const uint32_t inext = c + 64 + lane;
-        nkey = (inext < end) ? keys[inext] : 0xFFFFFFFFu;
-        nval = (inext < end) ? vals[inext] : 0u;
-        const bool valid = key < (uint32_t)numTiles;
-        const uint32_t val = valid ? rawVal : 0u;
-        // bitonic sort of (tile << 6 | lane) across the wave: equal tiles end
-        // up adjacent and in entry order; invalid entries sort last
-        uint32_t sv = ((valid ? key : (uint32_t)numTiles) << 6) | (uint32_t)lane;
+        const uint32_t i = start + (uint32_t)(a * 64 + lane);
+        pk[a] = (i < end) ? keys[i] : 0xFFFFFFFFu;
+        pv[a] = (i < end) ? vals[i] : 0u;
+    }
+    for (uint32_t c0 = start; c0 < end; c0 += 64 * kAhead)
+    {
+        // the kAhead chunks are sorted together (independent shuffle chains
+        // interleave) and placed one after the other, in order
+        uint32_t sv[kAhead], val[kAhead];
+#pragma unroll
+        for (int a = 0; a < kAhead; ++a)
+        {
+            const uint32_t c = c0 + (uint32_t)(a * 64);
+            const uint32_t key = pk[a], rawVal = pv[a];
+            const uint32_t inext = c + (uint32_t)(kAhead * 64 + lane);
+            pk[a] = (inext < end) ? keys[inext] : 0xFFFFFFFFu;
+            pv[a] = (inext < end) ? vals[inext] : 0u;
+            const bool valid = key < (uint32_t)numTiles;
+            val[a] = valid ? rawVal : 0u;
+            // bitonic sort of (tile << 6 | lane) across the wave: equal tiles
+            // end up adjacent and in entry order; invalid entries sort last
+            sv[a] = ((valid ? key : (uint32_t)numTiles) << 6) | (uint32_t)lane;
+        }
 #pragma unroll
         for (int k = 2; k <= 64; k <<= 1)
         {
 #pragma unroll
             for (int j = k >> 1; j > 0; j >>= 1)
             {
-                const uint32_t o = __shfl_xor(sv, j, 64);
                 const bool up = (lane & k) == 0;
                 const bool lower = (lane & j) == 0;
-                sv = (lower == up) ? min(sv, o) : max(sv, o);
+#pragma unroll
+                for (int a = 0; a < kAhead; ++a)
+                {
+                    const uint32_t o = __shfl_xor(sv[a], j, 64);
+                    sv[a] = (lower == up) ? min(sv[a], o) : max(sv[a], o);
+                }
             }
         }
-        const uint32_t skey = sv >> 6, src = sv & 63u;
-        const uint32_t prev = __shfl_up(skey, 1, 64);
-        const bool head = lane == 0 || prev != skey;
-        const unsigned long long heads = __ballot(head);
-        const int headLane = 63 - __clzll((long long)(heads & upto));
-        // the head of each run of equal tiles reserves the run's slots; runs
-        // have distinct tiles, so the LDS atomics never collide, and chunks
-        // are taken in order by this one wave
-        uint32_t slot = 0;
-        if (head && skey < (uint32_t)numTiles)
+#pragma unroll
+        for (int a = 0; a < kAhead; ++a)
         {
-            const unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1));
-            const uint32_t cnt = after ? (uint32_t)__ffsll((long long)after) : (uint32_t)(64 - lane);
-            slot = atomicAdd(&sBase[skey], cnt);
+            if (c0 + (uint32_t)(a * 64) >= end) break;
+            const uint32_t skey = sv[a] >> 6, src = sv[a] & 63u;
+            const uint32_t prev = __shfl_up(skey, 1, 64);
+            const bool head = lane == 0 || prev != skey;
+            const unsigned long long heads = __ballot(head);
+            const int headLane = 63 - __clzll((long long)(heads & upto));
+            // the head of each run of equal tiles reserves the run's slots; runs
+            // have distinct tiles, so the LDS atomics never collide, and chunks
+            // are taken in order by this one wave
+            uint32_t slot = 0;
+            if (head && skey < (uint32_t)numTiles)
+            {
+                const unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1));
+                const uint32_t cnt = after ? (uint32_t)__ffsll((long long)after) : (uint32_t)(64 - lane);
+                slot = atomicAdd(&sBase[skey], cnt);
+            }
+            const uint32_t b = __shfl(slot, headLane, 64);
+            const uint32_t v = __shfl(val[a], (int)src, 64);
+            if (skey < (uint32_t)numTiles) out[b + (uint32_t)(lane - headLane)] = v;
         }
-        const uint32_t b = __shfl(slot, headLane, 64);
-        const uint32_t v = __shfl(val, (int)src, 64);
-        if (skey < (uint32_t)numTiles) out[b + (uint32_t)(lane - headLane)] = v;
     }
 }
 
@@ -1914,7 +1948,7 @@ hipError_t launch_clip(const SetupParams& p, uint32_t jobs, hipStream_t s)
 {
     (void)hipGetLastError();   // drop a stale error from an earlier call
     if (jobs == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_clip, dim3(grid_for(jobs, kClipGroups, 8192)), dim3(kClipBlock), 0, s, p);
+    hipLaunchKernelGGL(k_clip, dim3(grid_for(jobs, kClipGroups, 1024)), dim3(kClipBlock), 0, s, p);
     return hipGetLastError();
 }
 
