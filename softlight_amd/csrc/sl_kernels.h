@@ -18,7 +18,7 @@ hipError_t launch_emit(const SetupParams& p, bool tris, bool prims, hipStream_t 
 hipError_t launch_clip(const SetupParams& p, uint32_t jobs, hipStream_t s);
 // Binning splits the entries into at most kMaxBinBlocks blocks of `per`
 // entries (a multiple of 64, at least kMinBinBlock).
-constexpr uint32_t kMaxBinBlocks = 256;
+constexpr uint32_t kMaxBinBlocks = 512;
 constexpr uint32_t kMinBinBlock = 1024;
 struct BinPlan { uint32_t blocks; uint32_t per; };
 BinPlan bin_plan(uint32_t entries);
