@@ -106,7 +106,7 @@ __device__ __forceinline__ SVert make_vert(const SmoothParams& P, uint32_t k, fl
     return v;
 }
 
-__device__ Shape winner_shape(const SmoothParams& P, uint32_t seq)
+__device__ __forceinline__ Shape winner_shape(const SmoothParams& P, uint32_t seq)
 {
     Shape S;
     S.ok = false;
@@ -206,7 +206,7 @@ __device__ __forceinline__ void tri_row_ends(SVert a, SVert b, SVert c, float yf
 
 // The winner's row span at y; a clipped polygon's is the union of the fan
 // pieces (0, j + 1, j + 2) the raster drew whose rows hold y.
-__device__ bool row_ends(const Shape& S, int y, End& left, End& right)
+__device__ __forceinline__ bool row_ends(const Shape& S, int y, End& left, End& right)
 {
     const float yf = (float)y;
     if (!S.clipped)
@@ -250,7 +250,7 @@ __device__ __forceinline__ void col_edge(const SVert& p, const SVert& q, float u
 
 // The winner's column span at u: the lowest and highest crossing of its
 // edges with the vertical line x = u, edges in polygon order.
-__device__ bool col_ends(const Shape& S, float u, End& low, End& high)
+__device__ __forceinline__ bool col_ends(const Shape& S, float u, End& low, End& high)
 {
     bool any = false;
     if (!S.clipped)
@@ -316,7 +316,7 @@ __device__ __forceinline__ uint32_t vis_seq(const SmoothParams& P, int x, int y)
 }
 
 // The row-pass boundary between pixels x - 1 and x of row y: pixel x receives.
-__device__ Bnd boundary_x(const SmoothParams& P, int x, int y)
+__device__ __forceinline__ Bnd boundary_x(const SmoothParams& P, int x, int y)
 {
     if (x <= 0 || x >= P.W || y < 0 || y >= P.H) return no_bnd();
     const uint32_t sl = vis_seq(P, x - 1, y), sr = vis_seq(P, x, y);
@@ -347,7 +347,7 @@ __device__ __forceinline__ Bnd col_bnd(uint32_t k, const End& E, int y, int x)
 
 // The column-pass boundary between rows y and y + 1 of column x (sample
 // x + 1): row y receives unless its sample lies on the edge.
-__device__ Bnd boundary_y(const SmoothParams& P, int x, int y)
+__device__ __forceinline__ Bnd boundary_y(const SmoothParams& P, int x, int y)
 {
     if (x < 0 || x >= P.W || y < 0 || y + 1 >= P.H) return no_bnd();
     const uint32_t sa = vis_seq(P, x, y), sb = vis_seq(P, x, y + 1);
