@@ -201,7 +201,7 @@ def device_used_bytes():
     return total - free
 
 
-def run_smooth(steps, warmup):
+def run_smooth(steps, warmup, tune=False):
     """Smoothed span coverage: 64 views at 512x512 of 8 meshes (5k..60k
     triangles, log-uniform), per-vertex RGB + 512x512 RGBA32F texture.
     Forward = clears + 512 draws (8 framebuffers of 8 viewports) + the
@@ -213,9 +213,29 @@ def run_smooth(steps, warmup):
     import softlight_amd
     from softlight_amd import api, scenes
 
+    from softlight_amd import autotune
+
     base = device_used_bytes()
     ctx = api.SLContext(softlight_amd.load_library())
     sc = scenes.SmoothScene(ctx, **SMOOTH_WORKLOAD)
+    # the raster knobs: timed on the forward at the first sigma (--tune), or
+    # the cached entry of this workload
+    signature = f"smooth:{sc.views}x{sc.size}:{sc.n_triangles}"
+    cache = os.environ.get("SL_TUNE_CACHE", autotune.DEFAULT_CACHE)
+    if tune:
+        sc.make_targets(SMOOTH_SIGMAS[0])
+
+        def fwd():
+            sc.forward(SMOOTH_SIGMAS[0])
+            ctx.finish()
+
+        res = autotune.tune(ctx, fwd, signature, path=cache)
+        print(json.dumps({"autotune": signature, **res}), file=sys.stderr, flush=True)
+        entry = autotune.cached_entry(signature, cache)
+    else:
+        entry = autotune.cached_entry(signature, cache)
+        if entry is not None:
+            autotune.apply(ctx, entry)
     sc.forward(0.0)
     ctx.finish()
     t0 = sc.targets[0]
@@ -244,12 +264,15 @@ def run_smooth(steps, warmup):
                              "fwd_bwd_over_fwd_time": round(row["fwd_bwd"]["ms"] / row["fwd"]["ms"], 3),
                              "ms_fwd": round(row["fwd"]["ms"], 3), "ms_fwd_bwd": round(row["fwd_bwd"]["ms"], 3),
                              "nonzero_position_grads": int(np.count_nonzero(pg))}
+    ctx_tile = ctx.raster_config()[1]
     ctx.close()
     return {"workload": f"{sc.views} views x {sc.size}^2 of {len(sc.meshes)} UV spheres "
                         f"({min(sc.mesh_tris)}..{max(sc.mesh_tris)} triangles, {sc.n_triangles} in all), "
                         "per-vertex RGB + 512^2 RGBA32F texture, L2 loss on the smoothed images against targets "
                         "rendered with vertices displaced by N(0, 0.02)",
             "sigma_0_equals_hard_image": hard_equal,
+            "raster_tile": list(ctx_tile),
+            "autotuned": autotune.describe(entry),
             "per_sigma": res,
             "peak_device_bytes": int(peak)}
 
@@ -515,10 +538,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--headline-only", action="store_true",
                     help="spheres: only the headline frame (no differentiable, smoothing, render-mode, target or blit legs)")
-    ap.add_argument("--workload", choices=["spheres", "random", "batched", "views", "modes", "scene_file"], default="spheres",
+    ap.add_argument("--workload", choices=["spheres", "random", "batched", "views", "smooth", "modes", "scene_file"],
+                    default="spheres",
                     help="spheres: the 1080p textured-sphere frame (default); random: 4096 random triangles at "
                          "640x480; batched: 1000 meshes with mixed opaque/blended materials at 1080p; views: "
-                         "32 differentiable 512x512 views, forward and forward+backward; modes: the sphere frame's "
+                         "32 differentiable 512x512 views, forward and forward+backward; smooth: the smoothed-"
+                         "coverage leg (64 views of 8 meshes, each sigma); modes: the sphere frame's "
                          "index buffer drawn as wireframe triangles, lines and points (its leg only); scene_file: "
                          "a 120-object OBJ file loaded into the scene graph and drawn per frame (its leg only)")
     ap.add_argument("--calibrate", action="store_true", help="also run a 256 MiB device copy (counter calibration)")
@@ -539,6 +564,9 @@ def main():
         calibration_copy()
     if args.workload == "views":
         main_views(args)
+        return
+    if args.workload == "smooth":
+        print(json.dumps({"smoothed_coverage": run_smooth(max(2, args.steps // 8), 1, args.tune)}), flush=True)
         return
     if args.workload == "modes":
         print(json.dumps({"render_modes": run_render_modes(args.steps, args.warmup),
