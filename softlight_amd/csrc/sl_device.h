@@ -515,6 +515,11 @@ struct SmoothParams
     uint32_t numVerts;         // vertices per saved draw in posGrad / posOffset
     int32_t W, H;
     float sigma;
+    // per pixel, written by the boundary pass: .x the row-pass weight of the
+    // boundary between x - 1 and x, .y the column-pass weight of the boundary
+    // between rows y and y + 1 with its sign bit set when row y receives; NaN
+    // where there is no boundary
+    float2* bounds;
 };
 
 struct BackwardParams

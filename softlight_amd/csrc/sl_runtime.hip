@@ -261,6 +261,7 @@ struct sl_context
     hipEvent_t bwdEv[2] = {};                      // first / last backward since the last collection
     hipEvent_t gEv[4] = {};                        // graph replays: setup start/end, raster start/end
     uint64_t bwdPending = 0;
+    DevBuf smoothBounds;                           // the smoothing passes' per-pixel boundary weights (float2)
 };
 
 namespace {
@@ -1395,7 +1396,9 @@ void sl_context_destroy(sl_context* c)
                       &c->res.items, &c->res.itemCount, &c->res.partDepth, &c->res.partRes, &c->res.partDone,
                       &c->res.positions, &c->res.vtxDraws}) release(*b);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
-    for (auto& f : c->fbos) release(f.gradDev);
+    for (auto& f : c->fbos)
+        for (DevBuf* b : {&f.gradDev, &f.smoothDev, &f.smoothUniDev}) release(*b);
+    release(c->smoothBounds);
     for (auto& e : c->bwdEv) if (e) (void)hipEventDestroy(e);
     for (auto& e : c->gEv) if (e) (void)hipEventDestroy(e);
     if (c->hCtrl) (void)hipHostFree(c->hCtrl);
@@ -2026,6 +2029,11 @@ int smooth_setup(sl_context* c, size_t fbo, size_t srcTex, size_t dstTex, float 
     sp.H = vis->h;
     sp.sigma = sigma;
     if (nd == 0) sp.sigma = 0.f;   // nothing drawn: no boundaries
+    if (sp.sigma > 0.f)
+    {
+        if ((rc = ensure(c, c->smoothBounds, sizeof(float2) * (size_t)vis->w * vis->h)) != SL_OK) return rc;
+        sp.bounds = reinterpret_cast<float2*>(c->smoothBounds.p);
+    }
     return SL_OK;
 }
 

@@ -5,6 +5,7 @@
 
 namespace slg {
 
+// Each launcher runs the boundary pass first (p.bounds: W * H float2).
 hipError_t launch_smooth(const SmoothParams& p, hipStream_t s);
 hipError_t launch_smooth_backward(const SmoothParams& p, hipStream_t s);
 

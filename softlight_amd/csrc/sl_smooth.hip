@@ -423,12 +423,54 @@ __device__ __forceinline__ float4 add_scaled_diff(float4 o, float w, float4 a, f
     return make_float4(o.x + w * (a.x - b.x), o.y + w * (a.y - b.y), o.z + w * (a.z - b.z), o.w + w * (a.w - b.w));
 }
 
+// The boundary pass: each pixel's row-pass boundary (x - 1 | x) and
+// column-pass boundary (y | y + 1), as weights (NaN: none; the column
+// weight's sign bit: row y receives).  Every pass below reads them, so each
+// boundary's winners are fetched once instead of by every pixel that uses it.
+__global__ __launch_bounds__(256) void k_smooth_bounds(SmoothParams P)
+{
+    const uint64_t n = (uint64_t)P.W * P.H;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    {
+        const int x = (int)(i % (uint64_t)P.W), y = (int)(i / (uint64_t)P.W);
+        const Bnd Bx = boundary_x(P, x, y);
+        const Bnd By = boundary_y(P, x, y);
+        const float wx = Bx.ok ? bnd_weight(Bx, P.sigma) : __builtin_nanf("");
+        float wy = __builtin_nanf("");
+        if (By.ok)
+        {
+            const float w = bnd_weight(By, P.sigma);
+            wy = By.recvLow ? -w : w;
+        }
+        P.bounds[i] = make_float2(wx, wy);
+    }
+}
+
+// A stored boundary: present, its weight and (column pass) whether row y receives.
+struct StoredBnd
+{
+    bool ok, recvLow;
+    float w;
+};
+
+__device__ __forceinline__ StoredBnd stored_x(const SmoothParams& P, uint64_t i)
+{
+    const float v = P.bounds[i].x;
+    return StoredBnd{!isnan(v), false, v};
+}
+
+__device__ __forceinline__ StoredBnd stored_y(const SmoothParams& P, uint64_t i)
+{
+    const float v = P.bounds[i].y;
+    return StoredBnd{!isnan(v), (bool)signbit(v), fabsf(v)};
+}
+
 // X(x, y): the row pass at one pixel.
 __device__ __forceinline__ float4 row_pass(const SmoothParams& P, int x, int y)
 {
     const uint64_t i = (uint64_t)y * P.W + x;
-    const Bnd B = boundary_x(P, x, y);
-    return B.ok ? lerp_px(P.color[i], P.color[i - 1], bnd_weight(B, P.sigma)) : P.color[i];
+    const StoredBnd B = stored_x(P, i);
+    return B.ok ? lerp_px(P.color[i], P.color[i - 1], B.w) : P.color[i];
 }
 
 __global__ __launch_bounds__(256) void k_smooth(SmoothParams P)
@@ -444,10 +486,14 @@ __global__ __launch_bounds__(256) void k_smooth(SmoothParams P)
         }
         const float4 X = row_pass(P, x, y);
         float4 o = X;
-        const Bnd Bd = boundary_y(P, x, y - 1);   // rows y - 1, y: row y receives when the tie rule says so
-        if (Bd.ok && !Bd.recvLow) o = add_scaled_diff(o, bnd_weight(Bd, P.sigma), row_pass(P, x, y - 1), X);
-        const Bnd Bu = boundary_y(P, x, y);       // rows y, y + 1
-        if (Bu.ok && Bu.recvLow) o = add_scaled_diff(o, bnd_weight(Bu, P.sigma), row_pass(P, x, y + 1), X);
+        if (y > 0)
+        {
+            // rows y - 1, y: row y receives when the tie rule says so
+            const StoredBnd Bd = stored_y(P, i - P.W);
+            if (Bd.ok && !Bd.recvLow) o = add_scaled_diff(o, Bd.w, row_pass(P, x, y - 1), X);
+        }
+        const StoredBnd Bu = stored_y(P, i);   // rows y, y + 1
+        if (Bu.ok && Bu.recvLow) o = add_scaled_diff(o, Bu.w, row_pass(P, x, y + 1), X);
         P.out[i] = o;
     }
 }
@@ -458,19 +504,20 @@ __device__ __forceinline__ float4 grad_x_image(const SmoothParams& P, int x, int
     const uint64_t i = (uint64_t)y * P.W + x;
     const float4 g = P.grad[i];
     float4 gx = g;
-    const Bnd Bd = boundary_y(P, x, y - 1);
-    if (Bd.ok)
+    if (y > 0)
     {
-        const float w = bnd_weight(Bd, P.sigma);
-        if (Bd.recvLow) gx = add_scaled_diff(gx, w, P.grad[i - P.W], make_float4(0.f, 0.f, 0.f, 0.f));   // this row is the source
-        else gx = add_scaled_diff(gx, -w, g, make_float4(0.f, 0.f, 0.f, 0.f));                          // this row receives
+        const StoredBnd Bd = stored_y(P, i - P.W);
+        if (Bd.ok)
+        {
+            if (Bd.recvLow) gx = add_scaled_diff(gx, Bd.w, P.grad[i - P.W], make_float4(0.f, 0.f, 0.f, 0.f));   // this row is the source
+            else gx = add_scaled_diff(gx, -Bd.w, g, make_float4(0.f, 0.f, 0.f, 0.f));                          // this row receives
+        }
     }
-    const Bnd Bu = boundary_y(P, x, y);
+    const StoredBnd Bu = stored_y(P, i);
     if (Bu.ok)
     {
-        const float w = bnd_weight(Bu, P.sigma);
-        if (Bu.recvLow) gx = add_scaled_diff(gx, -w, g, make_float4(0.f, 0.f, 0.f, 0.f));
-        else gx = add_scaled_diff(gx, w, P.grad[i + P.W], make_float4(0.f, 0.f, 0.f, 0.f));
+        if (Bu.recvLow) gx = add_scaled_diff(gx, -Bu.w, g, make_float4(0.f, 0.f, 0.f, 0.f));
+        else gx = add_scaled_diff(gx, Bu.w, P.grad[i + P.W], make_float4(0.f, 0.f, 0.f, 0.f));
     }
     return gx;
 }
@@ -522,22 +569,27 @@ __global__ __launch_bounds__(256) void k_smooth_backward(SmoothParams P)
         }
         // dL/dC = the row pass transposed over dL/dX
         const float4 gx = grad_x_image(P, x, y);
-        const Bnd Bx = boundary_x(P, x, y);
-        const float wx = bnd_weight(Bx, P.sigma);
+        const StoredBnd Sx = stored_x(P, i);
+        const float wx = Sx.ok ? Sx.w : 0.f;
         float4 gc = make_float4((1.f - wx) * gx.x, (1.f - wx) * gx.y, (1.f - wx) * gx.z, (1.f - wx) * gx.w);
-        const Bnd Bn = boundary_x(P, x + 1, y);
-        if (Bn.ok) gc = add_scaled_diff(gc, bnd_weight(Bn, P.sigma), grad_x_image(P, x + 1, y), make_float4(0.f, 0.f, 0.f, 0.f));
+        if (x + 1 < P.W)
+        {
+            const StoredBnd Sn = stored_x(P, i + 1);
+            if (Sn.ok) gc = add_scaled_diff(gc, Sn.w, grad_x_image(P, x + 1, y), make_float4(0.f, 0.f, 0.f, 0.f));
+        }
         P.out[i] = gc;
         if (P.posGrad == nullptr) continue;
-        // this pixel's row-pass boundary (x - 1 | x) and the column-pass boundary (y | y + 1)
-        if (Bx.ok)
+        // this pixel's row-pass boundary (x - 1 | x) and the column-pass
+        // boundary (y | y + 1), rebuilt for their edges' vertices
+        if (Sx.ok)
         {
+            const Bnd Bx = boundary_x(P, x, y);
             const float dLdw = dot4f(gx, add_scaled_diff(make_float4(0.f, 0.f, 0.f, 0.f), 1.f, P.color[i - 1], P.color[i]));
             scatter_position(P, Bx, dL_dd(Bx, P.sigma, dLdw), true);
         }
-        const Bnd By = boundary_y(P, x, y);
-        if (By.ok)
+        if (stored_y(P, i).ok)
         {
+            const Bnd By = boundary_y(P, x, y);
             const float4 X0 = row_pass(P, x, y), X1 = row_pass(P, x, y + 1);
             const float dLdw = By.recvLow ? dot4f(P.grad[i], add_scaled_diff(make_float4(0.f, 0.f, 0.f, 0.f), 1.f, X1, X0))
                                           : dot4f(P.grad[i + P.W], add_scaled_diff(make_float4(0.f, 0.f, 0.f, 0.f), 1.f, X0, X1));
@@ -557,6 +609,7 @@ hipError_t launch_smooth(const SmoothParams& p, hipStream_t s)
 {
     (void)hipGetLastError();   // drop a stale error from an earlier call
     if (p.W <= 0 || p.H <= 0) return hipSuccess;
+    if (p.sigma > 0.f) hipLaunchKernelGGL(k_smooth_bounds, dim3(smooth_grid(p)), dim3(256), 0, s, p);
     hipLaunchKernelGGL(k_smooth, dim3(smooth_grid(p)), dim3(256), 0, s, p);
     return hipGetLastError();
 }
@@ -565,6 +618,7 @@ hipError_t launch_smooth_backward(const SmoothParams& p, hipStream_t s)
 {
     (void)hipGetLastError();   // drop a stale error from an earlier call
     if (p.W <= 0 || p.H <= 0) return hipSuccess;
+    if (p.sigma > 0.f) hipLaunchKernelGGL(k_smooth_bounds, dim3(smooth_grid(p)), dim3(256), 0, s, p);
     hipLaunchKernelGGL(k_smooth_backward, dim3(smooth_grid(p)), dim3(256), 0, s, p);
     return hipGetLastError();
 }
