@@ -266,6 +266,7 @@ def run_smooth(steps, warmup, tune=False):
                              "nonzero_position_grads": int(np.count_nonzero(pg))}
     ctx_tile = ctx.raster_config()[1]
     ctx.close()
+    shared = run_smooth_shared(steps, warmup, entry)
     return {"workload": f"{sc.views} views x {sc.size}^2 of {len(sc.meshes)} UV spheres "
                         f"({min(sc.mesh_tris)}..{max(sc.mesh_tris)} triangles, {sc.n_triangles} in all), "
                         "per-vertex RGB + 512^2 RGBA32F texture, L2 loss on the smoothed images against targets "
@@ -274,6 +275,45 @@ def run_smooth(steps, warmup, tune=False):
             "raster_tile": list(ctx_tile),
             "autotuned": autotune.describe(entry),
             "per_sigma": res,
+            "peak_device_bytes": int(peak),
+            "shared_visibility": shared}
+
+
+def run_smooth_shared(steps, warmup, entry):
+    """The smoothed-coverage leg at the first sigma with one visibility
+    attachment shared by the 8 framebuffers: visibility recomputed (each
+    backward pass draws its framebuffer again) instead of kept per
+    framebuffer; views/s and the device-memory delta beside the kept case."""
+    import softlight_amd
+    from softlight_amd import api, autotune, scenes
+
+    sigma = SMOOTH_SIGMAS[0]
+    base = device_used_bytes()
+    ctx = api.SLContext(softlight_amd.load_library())
+    if entry is not None:
+        autotune.apply(ctx, entry)
+    sc = scenes.SmoothScene(ctx, **SMOOTH_WORKLOAD, shared_vis=True)
+    sc.make_targets(sigma)
+    peak = device_used_bytes() - base
+    row = {}
+    for name, fn in (("fwd", lambda: sc.forward(sigma)), ("fwd_bwd", lambda: (sc.forward(sigma), sc.backward(sigma)))):
+        for _ in range(warmup):
+            fn()
+        ctx.finish()
+        ctx.reset_stats()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        ctx.finish()
+        dt = (time.perf_counter() - t) / steps
+        row[name] = dict(ms=dt * 1e3, views_per_s=sc.views / dt, rebuilds=ctx.stats()["visibility_rebuilds"] / steps)
+        peak = max(peak, device_used_bytes() - base)
+    ctx.close()
+    return {"sigma": sigma, "fwd_views_per_s": round(row["fwd"]["views_per_s"], 1),
+            "fwd_bwd_views_per_s": round(row["fwd_bwd"]["views_per_s"], 1),
+            "fwd_bwd_over_fwd_time": round(row["fwd_bwd"]["ms"] / row["fwd"]["ms"], 3),
+            "visibility_rebuilds_per_step": row["fwd_bwd"]["rebuilds"],
+            "visibility_bytes": 16 * sc.targets[0]["W"] * sc.targets[0]["H"],
             "peak_device_bytes": int(peak)}
 
 

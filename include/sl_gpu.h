@@ -210,6 +210,7 @@ typedef struct sl_frame_stats {
     uint64_t raster_launches;       /* raster kernel launches */
     double   ms_backward;           /* backward kernels (device time, counted at the next synchronisation) */
     uint64_t backward_launches;     /* backward passes executed (counted likewise) */
+    uint64_t visibility_rebuilds;   /* framebuffers redrawn because a shared visibility attachment held another's draws */
 } sl_frame_stats;
 
 typedef struct sl_context sl_context;
@@ -372,7 +373,13 @@ int sl_frame_graph_release(sl_context* ctx);
  *   perspective-correct weights of the source triangle's three vertices,
  *   carried through clipping like a varying.  Needs numVaryings <= 3 in the
  *   shaders drawn (the weights use the next varying slot).  Clears write 0.
- *   texId = SL_NO_RESOURCE detaches.
+ *   texId = SL_NO_RESOURCE detaches.  Several framebuffers may share one
+ *   attachment (visibility recomputed instead of kept): when it holds another
+ *   framebuffer's draws, the backward passes of this one (smoothing and
+ *   fixed-visibility) first draw its saved draws again from their depth clear
+ *   -- the same visibility, depth and colour bit for bit -- and count it in
+ *   sl_frame_stats::visibility_rebuilds.  Draws that do not start from a depth
+ *   clear return SL_ERR_STATE there, blended draws SL_ERR_UNSUPPORTED.
  * sl_vbo_attach_gradient: a VBO receiving dL/d(colour attribute) as one
  *   float4 per vertex id (binding 1 of the draws that read the VBO).
  * sl_texture_attach_gradient: an RGBA32F texture of the same size receiving

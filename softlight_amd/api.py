@@ -90,7 +90,8 @@ class sl_frame_stats(C.Structure):
                 ("ms_vertex", C.c_double), ("ms_bin", C.c_double), ("ms_raster", C.c_double),
                 ("ms_total", C.c_double), ("ms_setup_kernel", C.c_double), ("ms_clip_kernel", C.c_double),
                 ("setup_launches", C.c_uint64), ("raster_launches", C.c_uint64),
-                ("ms_backward", C.c_double), ("backward_launches", C.c_uint64)]
+                ("ms_backward", C.c_double), ("backward_launches", C.c_uint64),
+                ("visibility_rebuilds", C.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -42,7 +42,19 @@ struct Vbo { bool valid = false; DevBuf buf; size_t bytes = 0; size_t grad = SL_
 struct Ibo { bool valid = false; DevBuf buf; int type = VERTEX_DATA_INT; uint32_t count = 0; uint32_t bpe = 4; };
 struct Binding { int dims = VERTEX_DIMENSION_4; int type = VERTEX_DATA_FLOAT; ptrdiff_t offset = 0; ptrdiff_t stride = 0; };
 struct Vao { size_t vbo = SL_INVALID_ID; size_t ibo = SL_INVALID_ID; size_t nb = 0; Binding b[SL_VAO_MAX_BINDINGS]; };
-struct Tex { bool valid = false; DevBuf buf; int type = SL_COLOR_RGBA_8U; uint16_t w = 0, h = 0, d = 0; uint32_t bpt = 0; size_t grad = SL_INVALID_ID; };
+struct Tex
+{
+    bool valid = false;
+    DevBuf buf;
+    int type = SL_COLOR_RGBA_8U;
+    uint16_t w = 0, h = 0, d = 0;
+    uint32_t bpt = 0;
+    size_t grad = SL_INVALID_ID;
+    // as a visibility attachment: the framebuffer whose draws it last
+    // recorded, and that recording's generation
+    size_t visFbo = SL_INVALID_ID;
+    uint64_t visGen = 0;
+};
 // A draw kept for the backward pass, with the resources its gradients go to
 // and its uniform snapshot (the smoothed-coverage passes re-run its vertex
 // positions).
@@ -56,6 +68,9 @@ struct Fbo
     size_t vis = SL_INVALID_ID;
     // draws since the depth attachment was last cleared (visibility attached)
     std::vector<SavedDraw> saved;
+    bool savedFromClear = false;     // they start at a depth clear (of savedClearDepth) ...
+    double savedClearDepth = 1.0;
+    uint64_t visGen = 0;             // ... and recorded this generation of the visibility attachment
     bool savedStale = false;         // a resource they read was re-initialised
     bool savedBlend = false;         // one of them blends: no fixed-visibility adjoint
     std::vector<GradDesc> gradHost;  // the backward draw table last uploaded ...
@@ -95,6 +110,7 @@ struct FrameGraph
     uint64_t pending = 0;                          // replays since the last collection
     size_t fbo = SL_INVALID_ID;                    // target of the captured batch
     bool clearDepth = false;
+    double depth = 1.0;                            // its depth clear value
     std::vector<SavedDraw> draws;                  // its draws, for the backward pass
     bool timed = false;                            // event nodes around the setup and raster kernels
 };
@@ -262,6 +278,7 @@ struct sl_context
     hipEvent_t gEv[4] = {};                        // graph replays: setup start/end, raster start/end
     uint64_t bwdPending = 0;
     DevBuf smoothBounds;                           // the smoothing passes' per-pixel boundary weights (float2)
+    uint64_t visCounter = 0;                       // generations of visibility recordings
 };
 
 namespace {
@@ -710,6 +727,19 @@ std::vector<uint8_t> uniform_snapshot(const Batch& B, size_t i)
 
 // Keeps the draws of a batch that rendered with a visibility attachment: the
 // backward pass maps each pixel's sequence number back to its draw.
+// The framebuffer's draws now fill its visibility attachment: a later
+// backward pass of another framebuffer that shares the attachment rebuilds
+// its own first (rebuild_visibility).
+void mark_visibility(sl_context* c, size_t fbo)
+{
+    Fbo& F = c->fbos[fbo];
+    Tex* V = tex_at(c, F.vis);
+    if (!V) return;
+    V->visFbo = fbo;
+    V->visGen = ++c->visCounter;
+    F.visGen = V->visGen;
+}
+
 void save_draws(sl_context* c, const Batch& B, const Plan& p)
 {
     Fbo& F = c->fbos[B.fbo];
@@ -718,6 +748,7 @@ void save_draws(sl_context* c, const Batch& B, const Plan& p)
         F.saved.clear();
         F.savedStale = false;
         F.savedBlend = false;
+        F.savedFromClear = false;
         return;
     }
     if (B.clearDepth)
@@ -725,12 +756,19 @@ void save_draws(sl_context* c, const Batch& B, const Plan& p)
         F.saved.clear();
         F.savedStale = false;
         F.savedBlend = false;
+        F.savedFromClear = true;
+        F.savedClearDepth = B.depth;
+    }
+    else if (F.saved.empty())
+    {
+        F.savedFromClear = false;
     }
     for (size_t i = 0; i < B.draws.size(); ++i)
     {
         F.saved.push_back(SavedDraw{B.draws[i], B.drawVbo[i], B.drawTex[i], uniform_snapshot(B, i)});
         if (B.draws[i].blend != SL_BLEND_OFF) F.savedBlend = true;
     }
+    mark_visibility(c, B.fbo);
 }
 
 // Resource storage moved: saved draws hold stale device pointers.
@@ -1047,6 +1085,7 @@ int graph_capture(sl_context* c)
     for (const DrawDesc& d : B.draws) G.tris += d.numTris;
     G.fbo = B.fbo;
     G.clearDepth = B.clearDepth;
+    G.depth = B.depth;
     G.draws.clear();
     for (size_t i = 0; i < B.draws.size(); ++i) G.draws.push_back(SavedDraw{B.draws[i], B.drawVbo[i], B.drawTex[i], uniform_snapshot(B, i)});
     G.valid = true;
@@ -1865,6 +1904,8 @@ int sl_frame_graph_launch(sl_context* c)
                 F.saved = G.draws;
                 F.savedStale = false;
                 F.savedBlend = false;
+                F.savedFromClear = true;
+                F.savedClearDepth = G.depth;
                 for (const SavedDraw& d : G.draws)
                     if (d.d.blend != SL_BLEND_OFF) F.savedBlend = true;
             }
@@ -1872,6 +1913,7 @@ int sl_frame_graph_launch(sl_context* c)
             {
                 F.savedStale = true;   // replays without a depth clear stack up
             }
+            mark_visibility(c, G.fbo);
         }
     }
     return SL_OK;
@@ -1948,6 +1990,8 @@ int sl_framebuffer_attach_visibility(sl_context* c, size_t fbo, size_t tex)
     F.saved.clear();
     F.savedStale = false;
     F.savedBlend = false;
+    F.savedFromClear = false;
+    F.visGen = 0;
     return SL_OK;
 }
 
@@ -1964,6 +2008,54 @@ int sl_framebuffer_attach_positions(sl_context* c, size_t fbo, size_t gradVbo, s
     F.posOffset = offsetVbo;
     F.posVerts = numVertices;
     return SL_OK;
+}
+
+// Visibility recomputed instead of kept: framebuffers may share one
+// visibility attachment (one W x H buffer for any number of them). When the
+// attachment holds another framebuffer's draws, the backward passes of this
+// one first draw its saved draws again from their depth clear: the same
+// draws, depth tests and sequence numbers give the same visibility, depth and
+// (unblended draws over the finished image) colour, bit for bit. Call after
+// flush(); the rebuild is executed before returning.
+int rebuild_visibility(sl_context* c, size_t fbo)
+{
+    Fbo& F = c->fbos[fbo];
+    Tex* V = tex_at(c, F.vis);
+    if (!V || F.saved.empty()) return SL_OK;
+    if (V->visFbo == fbo && V->visGen == F.visGen) return SL_OK;
+    if (!F.savedFromClear)
+        return fail(c, SL_ERR_STATE, "the visibility attachment holds another framebuffer's draws and this framebuffer's draws "
+                                     "do not start from a depth clear: draw again");
+    if (F.savedBlend)
+        return fail(c, SL_ERR_UNSUPPORTED, "the visibility attachment holds another framebuffer's draws and blended draws cannot "
+                                           "be redrawn over the finished image: give the framebuffer its own attachment");
+    const std::vector<SavedDraw> saved = F.saved;   // the batch's execution records them again
+    Batch& B = c->batch;
+    B.reset();
+    B.fbo = fbo;
+    B.clearDepth = true;
+    B.depth = F.savedClearDepth;
+    for (const SavedDraw& sd : saved)
+    {
+        DrawDesc d = sd.d;
+        // the draws may come from several batches: lay them out as one
+        B.totalTris = (B.totalTris + kSetupBlock - 1) / kSetupBlock * kSetupBlock;
+        d.triBase = B.totalTris;
+        d.posBase = kNoPositions;   // the position program per corner: the same bits as the vertex pass
+        if (d.hasUniforms)
+        {
+            if (sd.uni.size() != SL_MAX_UNIFORM_BUFFER_SIZE) return fail(c, SL_ERR_STATE, "saved draw without its uniform snapshot");
+            d.uniOffset = (uint32_t)B.uni.size();
+            B.uni.insert(B.uni.end(), sd.uni.begin(), sd.uni.end());
+        }
+        B.draws.push_back(d);
+        B.drawVbo.push_back(sd.vbo);
+        B.drawTex.push_back(sd.tex);
+        B.totalTris += d.numTris;
+        B.maxVary = std::max<uint32_t>(B.maxVary, d.nvary);
+    }
+    c->stats.visibility_rebuilds += 1;
+    return flush(c);
 }
 
 // The smoothing passes' draw table (uniform offsets into the saved draws'
@@ -1987,6 +2079,7 @@ int smooth_setup(sl_context* c, size_t fbo, size_t srcTex, size_t dstTex, float 
             return fail(c, SL_ERR_INVALID_ARG, "smoothing images must be RGBA_FLOAT of the framebuffer's size");
     if (src == dst) return fail(c, SL_ERR_INVALID_ARG, "smoothing input and output must differ");
     if (F.savedStale) return fail(c, SL_ERR_STATE, "resources changed since the forward pass; draw again");
+    if ((rc = rebuild_visibility(c, fbo)) != SL_OK) return rc;
     const size_t nd = F.saved.size();
     const uint64_t need = (uint64_t)nd * F.posVerts * 8u;
     Vbo* pg = (backward && F.posGrad != SL_INVALID_ID) ? &c->vbos[F.posGrad] : nullptr;
@@ -2141,6 +2234,7 @@ int sl_framebuffer_backward(sl_context* c, size_t fbo, size_t gradImageTex)
     if (F.savedStale) return fail(c, SL_ERR_STATE, "resources changed since the forward pass; draw again");
     if (F.savedBlend) return fail(c, SL_ERR_UNSUPPORTED, "fixed-visibility backward of blended draws");
     if (F.saved.empty()) return SL_OK;
+    if ((rc = rebuild_visibility(c, fbo)) != SL_OK) return rc;
     std::vector<GradDesc> G(F.saved.size());
     for (size_t i = 0; i < F.saved.size(); ++i)
     {

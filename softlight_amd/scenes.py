@@ -637,10 +637,12 @@ class SmoothScene:
     framebuffer as a grid of viewports (one batch each); every view draws
     every mesh.  The loss is sum((S - T)^2) over the smoothed images S;
     gradients go to the vertex colours, the texels and the vertex screen
-    positions (float2 per saved draw and vertex)."""
+    positions (float2 per saved draw and vertex).  With shared_vis every
+    framebuffer uses one visibility attachment: the backward passes then draw
+    each framebuffer's views again to recompute its visibility."""
 
     def __init__(self, ctx: A.SLContext, seed=6, meshes=8, min_tris=5000, max_tris=60000, views=64, size=512,
-                 views_per_fbo=8, tex_size=512, distance=4.0, fov=60.0):
+                 views_per_fbo=8, tex_size=512, distance=4.0, fov=60.0, shared_vis=False):
         self.ctx = ctx
         self.seed, self.views, self.size = seed, views, size
         lo, hi = math.log(min_tris), math.log(max_tris)
@@ -708,11 +710,13 @@ class SmoothScene:
             gx *= 2
         gy = (per + gx - 1) // gx
         self.targets = []
+        shared = ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, gx * size, gy * size) if shared_vis else None
         for f0 in range(0, views, per):
             vs_ = list(range(f0, min(views, f0 + per)))
             W, H = gx * size, gy * size
             mk = lambda: ctx.create_texture(A.SL_COLOR_RGBA_FLOAT, W, H)
-            color, vis, smooth, target, gsm, gimg = mk(), mk(), mk(), mk(), mk(), mk()
+            color, smooth, target, gsm, gimg = mk(), mk(), mk(), mk(), mk()
+            vis = shared if shared is not None else mk()
             depth = ctx.create_texture(A.SL_COLOR_R_FLOAT, W, H)
             fbo = ctx.create_framebuffer([color], depth)
             ctx.attach_visibility(fbo, vis)

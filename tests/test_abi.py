@@ -48,7 +48,7 @@ def test_struct_sizes():
     assert ctypes.sizeof(A.sl_mesh) == 32                    # SL_Mesh.hpp: "32 bytes"
     assert ctypes.sizeof(A.sl_std_uniforms) == 16 * 4 * 2 + 16 * 4 + 16
     assert ctypes.sizeof(A.sl_std_uniforms) <= 1024          # fits an SL_UniformBuffer
-    assert ctypes.sizeof(A.sl_frame_stats) == 6 * 8 + 6 * 8 + 2 * 8 + 8 + 8
+    assert ctypes.sizeof(A.sl_frame_stats) == 6 * 8 + 6 * 8 + 2 * 8 + 8 + 8 + 8
 
 
 def test_product_loader_refuses_missing_library(monkeypatch, tmp_path):

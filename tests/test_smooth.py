@@ -467,6 +467,7 @@ def _smooth_workload(lib, sigma, threads=8, **kw):
     out["gcol"] = [ctx.vbo_read(g, sc.pos[m].shape[0] * 16, dtype=np.float32) for m, g in enumerate(sc.grad_vbos)]
     out["gtex"] = ctx.texture_read(sc.grad_tex)
     out["tris"] = sc.mesh_tris
+    out["rebuilds"] = ctx.stats()["visibility_rebuilds"]
     ctx.close()
     return out
 
@@ -512,3 +513,29 @@ def test_gpu_matches_oracle_on_full_smooth_workload(oracle_lib, gpu_lib):
     o = _smooth_workload(oracle_lib, 1e-2, views=64)
     g = _smooth_workload(gpu_lib, 1e-2, views=64)
     _assert_smooth_parity(o, g, 1e-2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sigma", [0.0, 1e-2])
+def test_gpu_shared_visibility_attachment_is_recomputed(gpu_lib, sigma):
+    """Three framebuffers sharing one visibility attachment (visibility
+    recomputed, not kept): each backward pass first draws its framebuffer's
+    views again, so the hard and smoothed images and the image gradients equal
+    those of separate attachments bit for bit, and the position, colour and
+    texel gradients (summed by atomics in either order) within 1e-5 of the
+    largest."""
+    kw = dict(seed=6, meshes=3, min_tris=2000, max_tris=8000, views=6, size=128, views_per_fbo=2, tex_size=64)
+    sep = _smooth_workload(gpu_lib, sigma, **kw)
+    shr = _smooth_workload(gpu_lib, sigma, shared_vis=True, **kw)
+    assert sep["rebuilds"] == 0 and shr["rebuilds"] >= 3
+    for key in ("hard", "smooth", "gimg"):
+        for a, b in zip(sep[key], shr[key]):
+            np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    for key in ("pg", "gcol"):
+        for a, b in zip(sep[key], shr[key]):
+            scale = max(1e-6, float(np.abs(a).max()))
+            np.testing.assert_allclose(b, a, atol=1e-5 * scale, rtol=0)
+    scale = max(1e-6, float(np.abs(sep["gtex"]).max()))
+    np.testing.assert_allclose(shr["gtex"], sep["gtex"], atol=1e-5 * scale, rtol=0)
+    if sigma > 0.0:
+        assert all(np.count_nonzero(p) > 10 for p in shr["pg"])
