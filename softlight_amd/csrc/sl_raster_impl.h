@@ -499,18 +499,41 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
             fetch_words(base + kBatch, idsNext, vNext);
             idsAfter = fetch_ids(base + 2 * kBatch);
         }
-        // the batch's records whose rows and column box reach one of this
-        // lane's pixels, as a mask in record order
-        uint32_t cand = 0;
-        if (y < P.H)
+        // The batch's records whose rows and column box reach one of this
+        // lane's pixels, as a mask in record order.  Lane r works out record
+        // r's mask over the wave's 64 lanes -- the wave's rows it spans times
+        // the lanes whose columns (x = phase mod 4 within their segment, left
+        // of the image's edge) meet its column box -- and every lane takes
+        // its own bit of each record's mask (read from lane r's registers).
+        uint64_t hitMask = 0;
+        if (wl < nb)
         {
-            for (int bi = 0; bi < nb; ++bi)
-            {
-                const SetupRec& R = sRec[bi];
-                const int xlo = max((int)(R.xbox & 0xFFFFu), sx0), xhi = min((int)(R.xbox >> 16), sxLast);
-                const bool hit = y >= R.minY && y < R.maxY && xlo + ((phase - xlo) & 3) <= xhi;
-                cand |= (uint32_t)hit << bi;
-            }
+            const SetupRec& R = sRec[wl];
+            constexpr int LPR = 4 * SEGS, RPW = 64 / LPR;   // lanes per row, rows per wave
+            const int wy0 = y0 + (wave * 64) / LPR;
+            const int r0 = max(R.minY - wy0, 0), r1 = min(min(R.maxY, P.H) - wy0, RPW) - 1;
+            uint64_t rowsel = 0;
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+                if (r >= r0 && r <= r1) rowsel |= 1ull << (r * LPR);
+            const int jlo = max((int)(R.xbox & 0xFFFFu) - x0, 0);
+            const int jhi = min(min((int)(R.xbox >> 16) - x0, TW - 1), P.W - 1 - x0);
+            const uint64_t cm = (jlo <= jhi) ? ((~0ull >> (63 - jhi)) & (~0ull << jlo)) : 0ull;
+            uint64_t g = cm;
+#pragma unroll
+            for (int k = 1; k < SW / 4; ++k) g |= cm >> (4 * k);
+            uint64_t colBits = 0;
+#pragma unroll
+            for (int sg = 0; sg < SEGS; ++sg) colBits |= ((g >> (sg * SW)) & 0xFull) << (sg * 4);
+            hitMask = colBits * rowsel;   // colBits < 2^LPR: the row copies do not overlap
+        }
+        uint32_t cand = 0;
+        for (int bi = 0; bi < nb; ++bi)
+        {
+            const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hitMask, bi);
+            const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(hitMask >> 32), bi);
+            const uint64_t m = ((uint64_t)mhi << 32) | mlo;
+            cand |= (uint32_t)((m >> wl) & 1ull) << bi;
         }
         // Each lane walks its own candidates in record order.  A pixel is only
         // ever touched by the lane that owns it, so lanes may be at different
