@@ -379,7 +379,7 @@ struct SetupParams
     unsigned long long* stage;    // per chunk, kSetupBlock staging words
     unsigned long long* blockAgg; // packed counts per kScanBlock chunks (summed by atomics)
     uint32_t* blockSurv;          // survivors per kScanBlock chunks (summed by atomics)
-    uint32_t* survChunk;          // per block of kSetupBlock survivors: the chunk of its first survivor
+    uint32_t* survChunk;          // per survivor (in submission order): its chunk, written by the chunk scan
     uint32_t numChunks;
     FrameCtrl* ctrl;
     float4* positions;            // vertex-pass positions (draws with posBase != kNoPositions)

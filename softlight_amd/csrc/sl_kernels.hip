@@ -1127,9 +1127,6 @@ __global__ __launch_bounds__(kScanBlock) void k_chunk_scan(SetupParams P)
         ChunkInfo& I = P.chunkInfo[c];
         I.pre = ea;
         I.survPre = es;
-        // [es, es + sv) holds at most one multiple of kSetupBlock
-        const uint32_t k = (es + kSetupBlock - 1) / kSetupBlock;
-        if (sv > 0 && (uint64_t)k * kSetupBlock < (uint64_t)es + sv) P.survChunk[k] = c;
     }
     if (t == 0)
     {
@@ -1141,6 +1138,21 @@ __global__ __launch_bounds__(kScanBlock) void k_chunk_scan(SetupParams P)
             P.ctrl->survivors = totS;
         }
     }
+}
+
+// The chunk of every survivor (in submission order), one wave per chunk:
+// the emitting pass reads its lanes' chunks directly, where runs of chunks
+// without survivors (culled or off-screen meshes) would otherwise have to be
+// walked.
+constexpr int kSurvMapChunks = 4;   // chunks per workgroup, one per wave
+
+__global__ __launch_bounds__(64 * kSurvMapChunks) void k_surv_map(SetupParams P)
+{
+    const uint32_t c = blockIdx.x * kSurvMapChunks + (threadIdx.x >> 6);
+    if (c >= P.numChunks) return;
+    const ChunkInfo& I = P.chunkInfo[c];
+    const uint32_t pre = I.survPre, n = I.surv;
+    for (uint32_t k = threadIdx.x & 63; k < n; k += 64) P.survChunk[pre + k] = c;
 }
 
 // k_emit: one workgroup per block of kEmitBlock survivors (grid-stride).  A
@@ -1166,51 +1178,15 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int wave = t >> 6;
-    __shared__ uint32_t sSP[kEmitBlock];
     const uint32_t S = P.ctrl->survivors;
     for (uint32_t k = blockIdx.x; (uint64_t)k * kEmitBlock < S; k += gridDim.x)
     {
         const uint32_t s = k * kEmitBlock + t;
         const bool act = s < S;
-        // the survivor's chunk: the last chunk whose survivor prefix is <= s.
-        // The first window starts at the chunk of survivor k * kEmitBlock, so
-        // its first prefix is <= s for every lane; every lane is resolved by
-        // the window after the one holding its chunk, or by the last window.
-        uint32_t chunk = 0;
-        bool found = !act;
-        const uint32_t maxWindows = P.numChunks / kEmitBlock + 2;
-        uint32_t cw = P.survChunk[k];
-        for (uint32_t it = 0;; ++it, cw += kEmitBlock)
-        {
-            const uint32_t c = cw + t;
-            __syncthreads();
-            sSP[t] = c < P.numChunks ? P.chunkInfo[c].survPre : 0xFFFFFFFFu;
-            __syncthreads();
-            if (!found)
-            {
-                if (sSP[0] > s)
-                {
-                    chunk = cw - 1;   // the previous window's last chunk
-                    found = true;
-                }
-                else
-                {
-                    int j = 0;
-#pragma unroll
-                    for (int step = kEmitBlock / 2; step > 0; step >>= 1)
-                        if (sSP[j + step] <= s) j += step;
-                    if (j < kEmitBlock - 1 || cw + kEmitBlock >= P.numChunks)
-                    {
-                        chunk = cw + (uint32_t)j;
-                        found = true;
-                    }
-                }
-            }
-            if (__syncthreads_and(found)) break;
-            if (it >= maxWindows) break;   // unreachable with consistent prefixes
-        }
+        // the survivor's chunk (the chunk scan wrote it per survivor)
+        const uint32_t chunk = act ? P.survChunk[s] : 0u;
         bool bad = false;
-        if (!found || chunk >= P.numChunks)
+        if (chunk >= P.numChunks)
         {
             atomicExch(&P.ctrl->error, 1u);
             bad = true;        // the lane goes on without a triangle, with the rest of its wave
@@ -1949,6 +1925,7 @@ hipError_t launch_chunk_scan(const SetupParams& p, hipStream_t s)
     (void)hipGetLastError();   // drop a stale error from an earlier call
     const uint32_t blocks = (p.numChunks + kScanBlock - 1) / kScanBlock;
     hipLaunchKernelGGL(k_chunk_scan, dim3(blocks), dim3(kScanBlock), 0, s, p);
+    hipLaunchKernelGGL(k_surv_map, dim3((p.numChunks + kSurvMapChunks - 1) / kSurvMapChunks), dim3(64 * kSurvMapChunks), 0, s, p);
     return hipGetLastError();
 }
 

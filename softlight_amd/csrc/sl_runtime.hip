@@ -451,8 +451,8 @@ int make_plan(sl_context* c, const Batch& B, Plan& p)
 uint64_t chunk_blocks(uint64_t chunks) { return (chunks + kScanBlock - 1) / kScanBlock; }
 size_t chunk_buffer_bytes(uint64_t chunks)
 {
-    return (sizeof(ChunkInfo) + sizeof(unsigned long long) * kSetupBlock) * chunks +
-           (sizeof(unsigned long long) + sizeof(uint32_t)) * chunk_blocks(chunks) + sizeof(uint32_t) * chunks;
+    return (sizeof(ChunkInfo) + (sizeof(unsigned long long) + sizeof(uint32_t)) * kSetupBlock) * chunks +
+           (sizeof(unsigned long long) + sizeof(uint32_t)) * chunk_blocks(chunks);
 }
 
 // Part slots of the heavy-tile split: the per-slot buffers (first-pass
