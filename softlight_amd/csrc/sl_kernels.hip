@@ -631,35 +631,160 @@ __device__ __forceinline__ void count_triangle(const SetupParams& P, const DrawD
     }
 }
 
-// Records and bin entries of the calling wave's straddling triangles
-// (C.needClip): position-only clipping, up to kClipSlots lanes at a time, in
-// the wave's LDS slots (kClipSlots x 18 float4, interleaved).  Call with the
-// wave converged (inactive lanes may be masked off); D is the lane's draw.
-constexpr int kClipSlots = 4;
+// The straddling-triangle clipper shared by the counting pass and k_clip: one
+// 16-lane group per triangle, lane k holding polygon vertex k.  At each plane
+// every lane tests the edge from vertex k - 1 (the last vertex for k = 0) and
+// writes the edge's intersection and its own vertex, when kept, into the
+// group's next polygon at the position the group's ballots give it, so the
+// output order and every operation on a vertex are those of the serial
+// clipper (clip_polygon).  pos / var: the group's two polygon buffers in LDS
+// (var only with kVary); the triangle is in pos[0][0..2] (var[0][0..2]) on
+// entry, n = 3 (0 for a group without one).  Every lane of the wave calls it
+// (the planes end in wave-wide syncs).  Returns the polygon size (0 when
+// fewer than 3 vertices are left) and its buffer in `cur`.
+constexpr int kClipGroup = 16;
+constexpr int kClipVerts = 9;   // 3 + one per plane
 
+__device__ __forceinline__ void wave_sync()
+{
+    // LDS operations of one wave take effect in program order: fences keep
+    // the compiler from moving them across
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool kVary>
+__device__ __forceinline__ int group_clip(float4 (*pos)[kClipVerts], Vary4 (*var)[kClipVerts], int n, int nv, int& cur)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    const int gi = lane / kClipGroup, li = lane % kClipGroup;
+    const unsigned long long gmask = 0xFFFFull << (gi * kClipGroup);
+    const unsigned long long below = gmask & ((1ull << lane) - 1ull);
+    cur = 0;
+    wave_sync();
+    for (int e = 0; e < 6; ++e)
+    {
+        const float4 E = clip_edge(e);
+        const bool mine = li < n;
+        const int pj = li == 0 ? n - 1 : li - 1;
+        float4 q0 = f4(0.f, 0.f, 0.f, 0.f), q1 = q0;
+        float t0 = 0.f, t1 = 0.f;
+        bool vis0 = false, vis1 = false;
+        if (mine)
+        {
+            q1 = pos[cur][li];
+            q0 = pos[cur][pj];
+            t1 = dot4(q1, E);
+            t0 = dot4(q0, E);
+            vis1 = !signbit(t1);
+            vis0 = !signbit(t0);
+        }
+        const bool cross = mine && vis0 != vis1, keep = mine && vis1;
+        const unsigned long long mc = __ballot(cross) & gmask, mk = __ballot(keep) & gmask;
+        // a plane that keeps every vertex leaves the polygon as it is
+        const bool allIn = __popcll(mk) == n;
+        if (n > 0 && !allIn)
+        {
+            const int nn = __popcll(mc) + __popcll(mk);
+            int b = __popcll(mc & below) + __popcll(mk & below);
+            const int nxt = cur ^ 1;
+            if (cross && b < kClipVerts)
+            {
+                const float t = t0 * rcpf(t0 - t1);
+                pos[nxt][b] = mix4(q0, q1, t);
+                if constexpr (kVary)
+                {
+#pragma unroll
+                    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+                        if (i < nv) var[nxt][b].v[i] = mix4(var[cur][pj].v[i], var[cur][li].v[i], t);
+                }
+            }
+            if (cross) ++b;
+            if (keep && b < kClipVerts)
+            {
+                pos[nxt][b] = q1;
+                if constexpr (kVary)
+                {
+#pragma unroll
+                    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
+                        if (i < nv) var[nxt][b].v[i] = var[cur][li].v[i];
+                }
+            }
+            n = min(nn, kClipVerts);
+            cur = nxt;
+        }
+        wave_sync();
+    }
+    return n < 3 ? 0 : n;
+}
+
+// Records and bin entries of the calling wave's straddling triangles
+// (C.needClip), four at a time: group g of 16 lanes clips the g-th pending
+// straddler (its positions read from its lane) with group_clip, then its
+// lanes set up one fan triangle each and the group's sums go back to the
+// straddler's lane.  Call with every lane of the wave (inactive lanes with
+// needClip false); D is the wave's draw; slots: the wave's polygon buffers.
 __device__ __forceinline__ void count_clipped(const SetupParams& P, const DrawDesc& D, const TriIn& T, TriCount& C,
-                                              float4 (*slots)[kClipSlots])
+                                              float4 (*slots)[2][kClipVerts])
 {
     const int lane = threadIdx.x & 63;
+    const int gi = lane / kClipGroup, li = lane % kClipGroup;
     for (unsigned long long m = __ballot(C.needClip); m != 0ull; m = __ballot(C.needClip))
     {
-        const int rank = (int)__popcll(m & ((1ull << lane) - 1ull));
-        if (!C.needClip || rank >= kClipSlots) continue;
-        const PolyBuf<float4> poly{&slots[0][rank], kClipSlots}, tmp{&slots[9][rank], kClipSlots};
-        const PolyBuf<Vary4> none{nullptr, 0};
-        const int n = clip_polygon<false>(T.pos, nullptr, 0, poly, none, tmp, none);
-        for (int k = 0; k < n; ++k) poly[k] = divide_and_snap(poly[k], D.vp);
-        for (int i = n - 3; i >= 0; --i)
+        // this group's straddler: the gi-th pending lane
+        unsigned long long mm = m;
+        for (int g = 0; g < gi && mm != 0ull; ++g) mm &= mm - 1ull;
+        const int src = mm ? __ffsll((long long)mm) - 1 : -1;
+        const int from = src < 0 ? lane : src;
+        float4 p[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            p[k] = f4(__shfl(T.pos[k].x, from, 64), __shfl(T.pos[k].y, from, 64), __shfl(T.pos[k].z, from, 64),
+                      __shfl(T.pos[k].w, from, 64));
+        float4 (*pos)[kClipVerts] = slots[gi];
+        if (src >= 0 && li == 0)
         {
-            if (bin_accept(poly[0], poly[i + 1], poly[i + 2]))
+#pragma unroll
+            for (int k = 0; k < 3; ++k) pos[0][k] = p[k];
+        }
+        int cur = 0;
+        const int n = group_clip<false>(pos, nullptr, src >= 0 ? 3 : 0, 0, cur);
+        if (li < n) pos[cur][li] = divide_and_snap(pos[cur][li], D.vp);
+        wave_sync();
+        uint32_t nrec = 0, npair = 0;
+        if (li <= n - 3)
+        {
+            const float4 a = pos[cur][0], b = pos[cur][li + 1], c = pos[cur][li + 2];
+            if (bin_accept(a, b, c))
             {
-                const TileRect r = tile_rect(poly[0], poly[i + 1], poly[i + 2], P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
-                ++C.nrec;
-                C.npair += tile_count(r);
+                const TileRect r = tile_rect(a, b, c, P.W, P.H, P.tilesX, P.tilesY, P.tileShiftX, P.tileShiftY);
+                nrec = 1;
+                npair = tile_count(r);
             }
         }
-        C.clipped = C.nrec > 0;
-        C.needClip = false;
+#pragma unroll
+        for (int o = kClipGroup / 2; o > 0; o >>= 1)
+        {
+            nrec += (uint32_t)__shfl_xor((int)nrec, o, kClipGroup);
+            npair += (uint32_t)__shfl_xor((int)npair, o, kClipGroup);
+        }
+        wave_sync();   // the next round rewrites the buffers
+        // each straddler lane takes its group's sums
+#pragma unroll
+        for (int g = 0; g < 64 / kClipGroup; ++g)
+        {
+            const int sg = __shfl(src, g * kClipGroup, 64);
+            const uint32_t rg = (uint32_t)__shfl((int)nrec, g * kClipGroup, 64);
+            const uint32_t pg = (uint32_t)__shfl((int)npair, g * kClipGroup, 64);
+            if (sg == lane)
+            {
+                C.nrec = rg;
+                C.npair = pg;
+                C.clipped = rg > 0;
+                C.needClip = false;
+            }
+        }
     }
 }
 
@@ -980,7 +1105,7 @@ __global__ __launch_bounds__(kSetupBlock, 8) void k_setup(SetupParams P)
         classify_triangle(DU, T);
         count_triangle(P, DU, drawIdx, T, C);
     }
-    __shared__ float4 sClip[kSetupBlock / 64][18][kClipSlots];   // each wave's clipper slots
+    __shared__ float4 sClip[kSetupBlock / 64][64 / kClipGroup][2][kClipVerts];   // each wave's clipper buffers
     count_clipped(P, DU, T, C, sClip[wave]);
     // chunk summary: exclusive scan of the packed counts (each survivor's
     // offset within the chunk), survivor ranks, staging words, totals
@@ -1304,9 +1429,7 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
 // set up one per lane, their record and bin-entry offsets taken from
 // suffix sums over the group (the reverse fan numbers them from the last).
 constexpr int kClipBlock = 64;
-constexpr int kClipGroup = 16;
 constexpr int kClipGroups = kClipBlock / kClipGroup;
-constexpr int kClipVerts = 9;   // 3 + one per plane
 
 __global__ __launch_bounds__(kClipBlock) void k_clip(SetupParams P)
 {
@@ -1363,56 +1486,8 @@ __global__ __launch_bounds__(kClipBlock) void k_clip(SetupParams P)
                 }
             }
         }
-        int n = live ? 3 : 0, cur = 0;
-        __syncthreads();
-        for (int e = 0; e < 6; ++e)
-        {
-            const float4 E = clip_edge(e);
-            const bool mine = li < n;
-            const int pj = li == 0 ? n - 1 : li - 1;
-            float4 q0 = f4(0.f, 0.f, 0.f, 0.f), q1 = q0;
-            float t0 = 0.f, t1 = 0.f;
-            bool vis0 = false, vis1 = false;
-            if (mine)
-            {
-                q1 = sPos[gi][cur][li];
-                q0 = sPos[gi][cur][pj];
-                t1 = dot4(q1, E);
-                t0 = dot4(q0, E);
-                vis1 = !signbit(t1);
-                vis0 = !signbit(t0);
-            }
-            const bool cross = mine && vis0 != vis1, keep = mine && vis1;
-            const unsigned long long mc = __ballot(cross) & gmask, mk = __ballot(keep) & gmask;
-            // a plane that keeps every vertex leaves the polygon as it is
-            const bool allIn = __popcll(mk) == n;
-            if (n > 0 && !allIn)
-            {
-                const int nn = __popcll(mc) + __popcll(mk);
-                int b = __popcll(mc & below) + __popcll(mk & below);
-                const int nxt = cur ^ 1;
-                if (cross && b < kClipVerts)
-                {
-                    const float t = t0 * rcpf(t0 - t1);
-                    sPos[gi][nxt][b] = mix4(q0, q1, t);
-#pragma unroll
-                    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
-                        if (i < nv) sVar[gi][nxt][b].v[i] = mix4(sVar[gi][cur][pj].v[i], sVar[gi][cur][li].v[i], t);
-                }
-                if (cross) ++b;
-                if (keep && b < kClipVerts)
-                {
-                    sPos[gi][nxt][b] = q1;
-#pragma unroll
-                    for (int i = 0; i < SL_SHADER_MAX_VARYING_VECTORS; ++i)
-                        if (i < nv) sVar[gi][nxt][b].v[i] = sVar[gi][cur][li].v[i];
-                }
-                n = min(nn, kClipVerts);
-                cur = nxt;
-            }
-            __syncthreads();
-        }
-        if (n < 3) n = 0;
+        int cur = 0;
+        const int n = group_clip<true>(sPos[gi], sVar[gi], live ? 3 : 0, nv, cur);
         const DrawDesc* D = live ? &P.draws[T.draw] : nullptr;
         if (li < n) sPos[gi][cur][li] = divide_and_snap(sPos[gi][cur][li], D->vp);
         __syncthreads();
