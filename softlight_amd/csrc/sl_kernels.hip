@@ -637,7 +637,7 @@ __device__ __forceinline__ void count_triangle(const SetupParams& P, const DrawD
 // writes the edge's intersection and its own vertex, when kept, into the
 // group's next polygon at the position the group's ballots give it, so the
 // output order and every operation on a vertex are those of the serial
-// clipper (clip_polygon).  pos / var: the group's two polygon buffers in LDS
+// polygon loop (SL_TriProcessor.cpp:370-499).  pos / var: the group's two polygon buffers in LDS
 // (var only with kVary); the triangle is in pos[0][0..2] (var[0][0..2]) on
 // entry, n = 3 (0 for a group without one).  Every lane of the wave calls it
 // (the planes end in wave-wide syncs).  Returns the polygon size (0 when
@@ -1425,7 +1425,7 @@ __global__ __launch_bounds__(kEmitBlock, 4) void k_emit(SetupParams P)
 // k-1 (the last vertex for k = 0) and writes the edge's intersection and its
 // own vertex, when kept, to the next polygon at the position the group's
 // ballots give it, so the output order and every operation on a vertex are
-// those of the serial clipper (clip_polygon).  The fan's triangles are then
+// those of the serial polygon loop (SL_TriProcessor.cpp:370-499).  The fan's triangles are then
 // set up one per lane, their record and bin-entry offsets taken from
 // suffix sums over the group (the reverse fan numbers them from the last).
 constexpr int kClipBlock = 64;
