@@ -249,6 +249,14 @@ int         sl_set_raster_split(sl_context* ctx, uint32_t partRecords);
  * kernel's registers and LDS allow) by giving each workgroup that share of
  * the CU's LDS.  Results never depend on it. */
 int         sl_set_raster_occupancy(sl_context* ctx, uint32_t workgroupsPerCU);
+/* Depth rejection (no counterpart in the reference): with enable non-zero,
+ * batches of filled triangles without a visibility attachment use the
+ * raster variant that skips, for each wave of a tile holding 64 or more
+ * records, the records whose depth provably fails a LESS / LESS_EQUAL test
+ * at every pixel the wave owns (R_FLOAT depth targets only; such records
+ * only count their depth tests).  Off by default: it pays off where depth
+ * complexity is high.  Results never depend on it. */
+int         sl_set_raster_depth_reject(sl_context* ctx, int enable);
 int         sl_get_raster_config(const sl_context* ctx, int config, int* tileW, int* tileH, int* segments);
 
 /* ---- vertex buffers (SL_VertexBuffer.hpp / .cpp) ------------------------ */

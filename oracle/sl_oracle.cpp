@@ -2086,6 +2086,13 @@ int slo_set_raster_occupancy(sl_context* c, uint32_t workgroupsPerCU)
     return workgroupsPerCU <= 32 ? SL_OK : SL_ERR_INVALID_ARG;
 }
 
+// GPU raster variant choice: nothing to do on the CPU (results never depend on it)
+int slo_set_raster_depth_reject(sl_context* c, int enable)
+{
+    (void)enable;
+    return c ? SL_OK : SL_ERR_INVALID_ARG;
+}
+
 int slo_get_raster_config(const sl_context* c, int config, int* tileW, int* tileH, int* segments)
 {
     if (!c) return SL_ERR_INVALID_ARG;

@@ -111,6 +111,7 @@ _SIGS = {
     "get_raster_config": (_I, [_P, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "set_raster_split": (_I, [_P, C.c_uint32]),
     "set_raster_occupancy": (_I, [_P, C.c_uint32]),
+    "set_raster_depth_reject": (_I, [_P, _I]),
     "set_viewport": (_I, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "set_scissor": (_I, [_P, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),
     "create_vbo": (_SZ, [_P]),
@@ -243,6 +244,10 @@ class SLContext:
     def set_raster_occupancy(self, workgroups_per_cu):
         """Cap on resident raster workgroups per CU (0: none)."""
         return self._check(self.lib.set_raster_occupancy(self.h, workgroups_per_cu), "set_raster_occupancy")
+
+    def set_raster_depth_reject(self, enable):
+        """Raster variant with per-wave depth rejection on (True) or off."""
+        return self._check(self.lib.set_raster_depth_reject(self.h, 1 if enable else 0), "set_raster_depth_reject")
 
     def raster_config(self, config=-1):
         w, h, s = _I(), _I(), _I()

@@ -6,7 +6,8 @@ tiles from 8x8 to 64x64 and column segments per row), then for that tile the
 occupancy cap (resident raster workgroups per CU, sl_set_raster_occupancy),
 then the bin capacity of a raster workgroup (records per part of a heavy
 tile's bin, sl_set_raster_split: the library's default policy, never split,
-or 64 / 256 / 1024 records).  A knob setting the frame cannot use (a tile
+or 64 / 256 / 1024 records), then the raster variant with per-wave depth
+rejection on or off (sl_set_raster_depth_reject).  A knob setting the frame cannot use (a tile
 configuration with more tiles than the binning's 16384) is recorded as such
 and skipped.  Results are cached in a JSON file keyed by the device name and a
 workload signature, so later runs reuse them without timing.  The choice
@@ -79,6 +80,7 @@ def cached_entry(signature: str, path: str = DEFAULT_CACHE):
     out["config"] = int(out["config"])
     out.setdefault("wg_per_cu", 0)
     out.setdefault("split", SPLIT_DEFAULT)
+    out.setdefault("depth_reject", False)
     return out
 
 
@@ -93,6 +95,7 @@ def apply(ctx, entry) -> None:
     ctx.set_raster_config(int(entry["config"]))
     ctx.set_raster_occupancy(int(entry.get("wg_per_cu", 0)))
     ctx.set_raster_split(int(entry.get("split", SPLIT_DEFAULT)))
+    ctx.set_raster_depth_reject(bool(entry.get("depth_reject", False)))
 
 
 def _time(ctx, render_frame, frames):
@@ -107,13 +110,14 @@ def _time(ctx, render_frame, frames):
 
 def tune(ctx, render_frame, signature: str, frames: int = 3, path: str | None = DEFAULT_CACHE) -> dict:
     """Time render_frame() under each tile configuration, then each occupancy
-    cap and each bin capacity for the best tile; set and return the best
-    knobs with every timing (also cached)."""
+    cap, each bin capacity and depth rejection off / on for the best tile;
+    set and return the best knobs with every timing (also cached)."""
     from softlight_amd.api import SLError
 
-    configs, occupancy, splits = {}, {}, {}
+    configs, occupancy, splits, rejects = {}, {}, {}, {}
     ctx.set_raster_occupancy(0)
     ctx.set_raster_split(SPLIT_DEFAULT)
+    ctx.set_raster_depth_reject(False)
     for cfg in range(num_configs(ctx)):
         ctx.set_raster_config(cfg)
         try:
@@ -139,10 +143,16 @@ def tune(ctx, render_frame, signature: str, frames: int = 3, path: str | None = 
             splits[sp] = {"unsupported": str(e)}
     sp_best = min((k for k in splits if "frame_ms" in splits[k]), key=lambda k: splits[k]["frame_ms"])
     ctx.set_raster_split(sp_best)
+    rejects[False] = splits[sp_best]
+    ctx.set_raster_depth_reject(True)
+    rejects[True] = _time(ctx, render_frame, frames)
+    rj_best = rejects[True]["frame_ms"] < rejects[False]["frame_ms"]
+    ctx.set_raster_depth_reject(rj_best)
     entry = {"config": best, "tile": configs[best]["tile"], "wg_per_cu": wg_best, "split": sp_best,
-             "frame_ms": splits[sp_best]["frame_ms"]}
+             "depth_reject": rj_best, "frame_ms": rejects[rj_best]["frame_ms"]}
     result = {**entry, "timings": {"config": configs, "wg_per_cu": occupancy,
-                                   "split": {("default" if k == SPLIT_DEFAULT else k): v for k, v in splits.items()}}}
+                                   "split": {("default" if k == SPLIT_DEFAULT else k): v for k, v in splits.items()},
+                                   "depth_reject": {("on" if k else "off"): v for k, v in rejects.items()}}}
     if path:
         cache = load_cache(path)
         cache.setdefault(device_name(), {})[signature] = entry
@@ -152,11 +162,13 @@ def tune(ctx, render_frame, signature: str, frames: int = 3, path: str | None = 
 
 
 def describe(entry) -> dict:
-    """The knobs of an entry for a report: tile, occupancy cap, bin capacity."""
+    """The knobs of an entry for a report: tile, occupancy cap, bin capacity,
+    depth rejection."""
     if entry is None:
         return None
     sp = int(entry.get("split", SPLIT_DEFAULT))
     return {"config": int(entry["config"]), "tile": list(entry.get("tile", [])),
             "workgroups_per_cu": int(entry.get("wg_per_cu", 0)) or "uncapped",
             "bin_capacity_records_per_workgroup": ("default (split point/line batches at 256)" if sp == SPLIT_DEFAULT
-                                                   else ("unsplit" if sp == 0 else sp))}
+                                                   else ("unsplit" if sp == 0 else sp)),
+            "depth_reject": bool(entry.get("depth_reject", False))}

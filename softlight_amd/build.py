@@ -19,7 +19,7 @@ LIB = os.path.join(OUT, "libsoftlight_amd.so")
 SOURCES = ["sl_kernels.hip", "sl_backward.hip", "sl_smooth.hip", "sl_runtime.hip",
            # k_raster instances, split so that they compile in parallel
            "sl_raster_plain.hip", "sl_raster_plain_dbg.hip", "sl_raster_modes.hip", "sl_raster_modes_dbg.hip",
-           "sl_raster_mrt.hip", "sl_raster_vis.hip"]
+           "sl_raster_mrt.hip", "sl_raster_vis.hip", "sl_raster_reject.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -30,7 +30,8 @@ hipError_t launch_binning(const uint32_t* keys, const uint32_t* vals, const Fram
                           uint32_t* order, int numTiles, hipStream_t s, RasterItem* items = nullptr,
                           uint32_t* itemCount = nullptr, uint32_t* partDone = nullptr, uint32_t partRecs = 1,
                           uint32_t splitAbove = 0, uint32_t maxParts = 0);
-hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s);
+// reject: the filled-triangle variants (plain and debug) with depth rejection (k_raster AUX bit 4)
+hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s, bool reject = false);
 // k_raster of one colour format (CF: 0 none, 1 8-bit/packed, 2 float) and
 // variant (AUX bits, see k_raster) on tile configuration `cfg`; defined in
 // sl_raster_impl.h and instantiated in the sl_raster_*.hip units.

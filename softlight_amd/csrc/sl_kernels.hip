@@ -2077,7 +2077,7 @@ hipError_t launch_raster_depth(const RasterParams& p, int grid, int cfg, hipStre
     return hipGetLastError();
 }
 
-hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s)
+hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debug, hipStream_t s, bool reject)
 {
     (void)hipGetLastError();   // drop a stale error from an earlier call
     const bool floatColor = p.colorType == SL_COLOR_R_FLOAT || p.colorType == SL_COLOR_RG_FLOAT ||
@@ -2120,11 +2120,23 @@ hipError_t launch_raster(const RasterParams& p, int numTiles, int cfg, bool debu
             else launch_raster_cfg<2, 4>(p, numTiles, cfg, s);
         }
     }
+    else if (debug && reject)
+    {
+        if (cf == 0) launch_raster_cfg<0, 17>(p, numTiles, cfg, s);
+        else if (cf == 1) launch_raster_cfg<1, 17>(p, numTiles, cfg, s);
+        else launch_raster_cfg<2, 17>(p, numTiles, cfg, s);
+    }
     else if (debug)
     {
         if (cf == 0) launch_raster_cfg<0, 1>(p, numTiles, cfg, s);
         else if (cf == 1) launch_raster_cfg<1, 1>(p, numTiles, cfg, s);
         else launch_raster_cfg<2, 1>(p, numTiles, cfg, s);
+    }
+    else if (reject)
+    {
+        if (cf == 0) launch_raster_cfg<0, 16>(p, numTiles, cfg, s);
+        else if (cf == 1) launch_raster_cfg<1, 16>(p, numTiles, cfg, s);
+        else launch_raster_cfg<2, 16>(p, numTiles, cfg, s);
     }
     else
     {

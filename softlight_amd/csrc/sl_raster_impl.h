@@ -27,11 +27,36 @@ namespace {
 // Catch-up distances up to kStepLimit groups are stepped; longer ones use the
 // binade jump (one out-of-line copy for the three barycentric components).
 constexpr int kStepLimit = 256;
+// Depth rejection (AUX bit 4) runs on tiles (or parts) of at least this many records.
+constexpr int kHizMinRecs = 64;
 
 __device__ __noinline__ float3 recur_advance3(float3 s, float3 d, int k)
 {
     sl_recur_advance3(s.x, s.y, s.z, d.x, d.y, d.z, k);
     return s;
+}
+
+// Greatest value over the wave's 64 lanes (every lane active): pairs and
+// quads with DPP quad permutes, the 16-lane row by rotating it 4 and 8 lanes,
+// then the four rows' results read from lanes 0, 16, 32 and 48.  NaN inputs
+// are ignored (fmaxf) unless every lane holds one.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float wave_max_f32(float v)
+{
+    v = fmaxf(v, dpp_f32<0xB1>(v));    // quad_perm [1,0,3,2]: lane ^ 1
+    v = fmaxf(v, dpp_f32<0x4E>(v));    // quad_perm [2,3,0,1]: lane ^ 2
+    v = fmaxf(v, dpp_f32<0x124>(v));   // row_ror:4
+    v = fmaxf(v, dpp_f32<0x128>(v));   // row_ror:8
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
 }
 
 // Orders one record's LDS updates before the next record's: a wave barrier
@@ -302,7 +327,9 @@ constexpr uint32_t kNoPending = 0xFFFFFFFFu;
 // visibility attachment; bit 2: the batch holds wireframe, line or point
 // records (the variants without it carry only the filled-triangle code);
 // bit 3: the batch has draws with several fragment outputs (launched with
-// bit 2 set).
+// bit 2 set); bit 4: depth rejection of records that fail every depth test
+// in a wave (filled triangles, LESS / LESS_EQUAL; results are identical with
+// it and without it).
 //
 // Heavy tiles (RasterParams::items): a workgroup takes a work item, a whole
 // tile or one part of a split tile's record list.  MODE 1 walks a part from
@@ -320,6 +347,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     constexpr bool VIS = (AUX & 2) != 0;
     constexpr bool EXTRA = (AUX & 4) != 0;
     constexpr bool MRT = (AUX & 8) != 0;
+    constexpr bool HIZ = (AUX & 16) != 0;
     constexpr int NT = TH * 4 * SEGS;       // lanes: rows x segments x 4 phases
     constexpr int NW = NT / 64;             // waves, each owning TH / NW rows
     constexpr int SW = TW / SEGS;           // segment width (columns)
@@ -454,6 +482,9 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     // one batch ahead) are issued before the current batch is processed, so
     // their latency hides behind the raster work.
     constexpr int kBatch = 16;
+    // depth rejection (below) on tiles of at least kHizMinRecs records:
+    // lighter tiles reject too little to pay for the bound's reduction
+    const bool hiz = HIZ && !P.depthHalf && range.y - range.x >= (uint32_t)kHizMinRecs;
     constexpr int kRecWords = sizeof(SetupRec) / 4;
     constexpr int kLaneWords = kBatch * kRecWords / 64;
     static_assert(kBatch * kRecWords % 64 == 0 && kBatch <= 64, "record staging");
@@ -506,6 +537,23 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
         // of the image's edge) meet its column box -- and every lane takes
         // its own bit of each record's mask (read from lane r's registers).
         uint64_t hitMask = 0;
+        // Depth rejection.  zTop: the greatest depth the wave's pixels hold at
+        // the batch's start (float depth only: an R16F write rounds, which can
+        // land above a clear value that is not a half).  Depth tests LESS and
+        // LESS_EQUAL only lower a pixel's depth, so zTop bounds it for every
+        // record of the batch before the first one that can raise it (depth
+        // writes with the test off, GREATER(_EQUAL) or NOT_EQUAL).  A filled
+        // triangle with a LESS(_EQUAL) test whose computed depth provably
+        // exceeds zTop at every pixel of the wave it can reach fails each of
+        // its depth tests; its lanes only count them (rejMask).
+        float zTop = -INFINITY;
+        if (HIZ && hiz)
+        {
+#pragma unroll
+            for (int k = 0; k < SW / 4; ++k) zTop = fmaxf(zTop, sDepth[row][seg * SW + phase + 4 * k]);
+            zTop = wave_max_f32(zTop);
+        }
+        bool rej = false, raise = false;
         if (wl < nb)
         {
             const SetupRec& R = sRec[wl];
@@ -526,6 +574,51 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
 #pragma unroll
             for (int sg = 0; sg < SEGS; ++sg) colBits |= ((g >> (sg * SW)) & 0xFull) << (sg * 4);
             hitMask = colBits * rowsel;   // colBits < 2^LPR: the row copies do not overlap
+            const uint32_t rst = R.state;
+            const int rdt = (int)(rst & 7u);
+            raise = HIZ && ((rst >> 3) & 1u) != 0u &&
+                    (rdt == SL_DEPTH_TEST_OFF || rdt == SL_DEPTH_TEST_GREATER_THAN ||
+                     rdt == SL_DEPTH_TEST_GREATER_EQUAL || rdt == SL_DEPTH_TEST_NOT_EQUAL);
+            if (HIZ && hiz && hitMask != 0ull && (rst & (kStateWire | kStateLine | kStatePoint)) == 0u &&
+                (rdt == SL_DEPTH_TEST_LESS_THAN || rdt == SL_DEPTH_TEST_LESS_EQUAL))
+            {
+                // The lane computes s_i = fl(x bx_i + fma(by_i, y, bz_i)), then
+                // s_i += 4 bx_i per group up to its pixel (at most X / 4 + 1
+                // steps from the span's start, x >= 0), and z = sum z_i s_i.
+                // With A_i = X |bx_i| + Y |by_i| + |bz_i| (>= |S_i| and each
+                // product term over 0 <= x <= X, 0 <= y <= Y) every rounding
+                // of that chain and of the corner values below is at most
+                // 2^-24 A_i, so the computed depth of any pixel in the
+                // rectangle lies within sum |z_i| A_i 2^-24 (X / 4 + 13) of
+                // the plane's exact value, whose minimum is at a corner.
+                const float xa = (float)(x0 + jlo), xb = (float)(x0 + jhi);
+                const float ya = (float)(wy0 + r0), yb = (float)(wy0 + r1);
+                float zlo = INFINITY, mag = 0.f;
+                float zc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                {
+                    const float bxi = R.bx[i], byi = R.by[i], bzi = R.bz[i], zi = R.z[i];
+                    const float v = fmaf(byi, ya, bzi), w = fmaf(byi, yb, bzi);
+                    zc[0] = fmaf(zi, fmaf(bxi, xa, v), zc[0]);
+                    zc[1] = fmaf(zi, fmaf(bxi, xb, v), zc[1]);
+                    zc[2] = fmaf(zi, fmaf(bxi, xa, w), zc[2]);
+                    zc[3] = fmaf(zi, fmaf(bxi, xb, w), zc[3]);
+                    mag = fmaf(fabsf(zi), fmaf(xb, fabsf(bxi), fmaf(yb, fabsf(byi), fabsf(bzi))), mag);
+                }
+                zlo = fminf(fminf(zc[0], zc[1]), fminf(zc[2], zc[3]));
+                // twice the bound above, and NaN / infinite terms never reject
+                const float margin = mag * (5.9604645e-8f * (2.f * (0.25f * xb + 13.f))) * 1.0625f;
+                rej = (zlo - margin) > zTop;
+            }
+        }
+        uint32_t rejMask = 0;
+        if constexpr (HIZ)
+        {
+            // records before the batch's first depth-raising record
+            const uint64_t rb = __ballot(raise);
+            const uint64_t before = (rb != 0ull) ? ((1ull << (__ffsll((unsigned long long)rb) - 1)) - 1ull) : ~0ull;
+            rejMask = (uint32_t)(__ballot(rej) & before);
         }
         uint32_t cand = 0;
         for (int bi = 0; bi < nb; ++bi)
@@ -712,6 +805,12 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                     const int xf = xMin + ((phase - xMin) & 3);   // first span pixel of this lane's column class
                     const int g0 = (xs > xf) ? ((xs - xf + 3) >> 2) : 0;
                     x = xf + 4 * g0;
+                    if (HIZ && ((rejMask >> bi) & 1u))
+                    {
+                        // every depth test of this record fails here: count them
+                        if (MODE != 1 && x < xe) tested += (uint32_t)((xe - x + 3) >> 2);
+                        continue;
+                    }
                     // bc(x) = fl(fl(x*dx) + fma(dy, y, c)), then += 4*dx per group of 4 pixels
                     const float fx = (float)xf;
                     s0 = (fx * R.bx[0]) + fmaf(R.by[0], yf, R.bz[0]);

@@ -263,6 +263,7 @@ struct sl_context
     uint32_t splitRecs = kDefaultSplitRecords;   // heavy-tile split: records per part (0: off)
     bool splitAll = false;                       // split triangle-only batches too (sl_set_raster_split)
     uint32_t wgPerCU = 0;                        // raster occupancy cap (sl_set_raster_occupancy; 0: none)
+    bool depthReject = false;                    // sl_set_raster_depth_reject
     uint32_t triSeq = 0;
     sl_frame_stats stats{};
     std::string err;
@@ -699,9 +700,9 @@ int enqueue_back(sl_context* c, FrameRes& R, const Plan& p, const Batch& B, uint
         // the device)
         if (rp.splitClass != kSplitConst && (rc = check(c, launch_raster_depth(rp, (int)R.maxParts, p.cfg, s), "k_raster depth pass")) != SL_OK)
             return rc;
-        return check(c, launch_raster(rp, p.numTiles + (int)R.maxParts, p.cfg, p.debug, s), "k_raster");
+        return check(c, launch_raster(rp, p.numTiles + (int)R.maxParts, p.cfg, p.debug, s, c->depthReject), "k_raster");
     }
-    return check(c, launch_raster(rp, p.numTiles, p.cfg, p.debug, s), "k_raster");
+    return check(c, launch_raster(rp, p.numTiles, p.cfg, p.debug, s, c->depthReject), "k_raster");
 }
 
 int upload_batch(sl_context* c, FrameRes& R, const Batch& B, hipStream_t s)
@@ -1488,6 +1489,15 @@ int sl_set_raster_occupancy(sl_context* c, uint32_t workgroupsPerCU)
     const int rc = flush(c);
     if (rc != SL_OK) return rc;
     c->wgPerCU = workgroupsPerCU;
+    return SL_OK;
+}
+
+int sl_set_raster_depth_reject(sl_context* c, int enable)
+{
+    if (!c) return SL_ERR_INVALID_ARG;
+    const int rc = flush(c);
+    if (rc != SL_OK) return rc;
+    c->depthReject = enable != 0;
     return SL_OK;
 }
 
