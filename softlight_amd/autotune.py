@@ -9,13 +9,12 @@ tile's bin, sl_set_raster_split: the library's default policy, never split,
 or 64 / 256 / 1024 records), then the raster variant with per-wave depth
 rejection on or off (sl_set_raster_depth_reject).  A knob setting the frame cannot use (a tile
 configuration with more tiles than the binning's 16384) is recorded as such
-and skipped.  Results are cached in a JSON file keyed by the device name and a
+and skipped.  Results are cached in a JSON file keyed by the target (gfx950) and a
 workload signature, so later runs reuse them without timing.  The choice
 never changes results, only speed (the parity tests run every setting).
 """
 from __future__ import annotations
 
-import ctypes
 import json
 import os
 
@@ -31,19 +30,10 @@ def num_configs(ctx) -> int:
 
 
 def device_name() -> str:
-    """The HIP device name, or the library's only target when HIP gives none."""
-    fallback = "gfx950"
-    try:
-        hip = ctypes.CDLL("libamdhip64.so")
-    except OSError:
-        try:
-            hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
-        except OSError:
-            return fallback
-    buf = ctypes.create_string_buffer(256)
-    if hip.hipDeviceGetName(buf, ctypes.c_int(256), ctypes.c_int(0)) != 0:
-        return fallback
-    return buf.value.decode(errors="replace").strip() or fallback
+    """The cache's device key: the library's only target (HIP's device name
+    is the generic "AMD Radeon Graphics" on some boxes and unavailable on
+    others, which would split one device's entries over two keys)."""
+    return "gfx950"
 
 
 def load_cache(path: str = DEFAULT_CACHE) -> dict:
@@ -146,7 +136,9 @@ def tune(ctx, render_frame, signature: str, frames: int = 3, path: str | None = 
     rejects[False] = splits[sp_best]
     ctx.set_raster_depth_reject(True)
     rejects[True] = _time(ctx, render_frame, frames)
-    rj_best = rejects[True]["frame_ms"] < rejects[False]["frame_ms"]
+    # kept only for a clear gain: batches the variant does not cover (the
+    # visibility attachment) time the same either way
+    rj_best = rejects[True]["frame_ms"] < 0.98 * rejects[False]["frame_ms"]
     ctx.set_raster_depth_reject(rj_best)
     entry = {"config": best, "tile": configs[best]["tile"], "wg_per_cu": wg_best, "split": sp_best,
              "depth_reject": rj_best, "frame_ms": rejects[rj_best]["frame_ms"]}
