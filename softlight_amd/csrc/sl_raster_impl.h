@@ -545,7 +545,8 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
         // writes with the test off, GREATER(_EQUAL) or NOT_EQUAL).  A filled
         // triangle with a LESS(_EQUAL) test whose computed depth provably
         // exceeds zTop at every pixel of the wave it can reach fails each of
-        // its depth tests; its lanes only count them (rejMask).
+        // its depth tests: it leaves every lane's walk (rejMask), and lane r
+        // counts record r's tests over the wave instead.
         float zTop = -INFINITY;
         if (HIZ && hiz)
         {
@@ -554,6 +555,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
             zTop = wave_max_f32(zTop);
         }
         bool rej = false, raise = false;
+        uint32_t rejTested = 0;   // depth tests of this lane's record in the wave, if it is rejected
         if (wl < nb)
         {
             const SetupRec& R = sRec[wl];
@@ -610,6 +612,23 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                 // twice the bound above, and NaN / infinite terms never reject
                 const float margin = mag * (5.9604645e-8f * (2.f * (0.25f * xb + 13.f))) * 1.0625f;
                 rej = (zlo - margin) > zTop;
+                if (rej && MODE != 1)
+                {
+                    // its depth tests in the wave: per row, the span's pixels
+                    // in the tile and the image (every lane's share of the
+                    // row's columns, as the walk would count them)
+                    const int xr = min(x0 + TW, P.W);
+                    for (int r = r0; r <= r1; ++r)
+                    {
+                        int32_t xMin, xMax;
+                        scanline_step(R, (float)(wy0 + r), xMin, xMax);
+                        if ((uint32_t)xMin < (uint32_t)xMax)
+                        {
+                            const int a = max((int)xMin, x0), b = min((int)xMax, xr);
+                            if (a < b) rejTested += (uint32_t)(b - a);
+                        }
+                    }
+                }
             }
         }
         uint32_t rejMask = 0;
@@ -619,6 +638,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
             const uint64_t rb = __ballot(raise);
             const uint64_t before = (rb != 0ull) ? ((1ull << (__ffsll((unsigned long long)rb) - 1)) - 1ull) : ~0ull;
             rejMask = (uint32_t)(__ballot(rej) & before);
+            if (wl < kBatch && ((rejMask >> wl) & 1u)) tested += rejTested;
         }
         uint32_t cand = 0;
         for (int bi = 0; bi < nb; ++bi)
@@ -628,6 +648,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
             const uint64_t m = ((uint64_t)mhi << 32) | mlo;
             cand |= (uint32_t)((m >> wl) & 1ull) << bi;
         }
+        if constexpr (HIZ) cand &= ~rejMask;   // rejected records are counted above
         // Each lane walks its own candidates in record order.  A pixel is only
         // ever touched by the lane that owns it, so lanes may be at different
         // records at the same time and every pixel still sees its fragments
@@ -805,12 +826,6 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
                     const int xf = xMin + ((phase - xMin) & 3);   // first span pixel of this lane's column class
                     const int g0 = (xs > xf) ? ((xs - xf + 3) >> 2) : 0;
                     x = xf + 4 * g0;
-                    if (HIZ && ((rejMask >> bi) & 1u))
-                    {
-                        // every depth test of this record fails here: count them
-                        if (MODE != 1 && x < xe) tested += (uint32_t)((xe - x + 3) >> 2);
-                        continue;
-                    }
                     // bc(x) = fl(fl(x*dx) + fma(dy, y, c)), then += 4*dx per group of 4 pixels
                     const float fx = (float)xf;
                     s0 = (fx * R.bx[0]) + fmaf(R.by[0], yf, R.bz[0]);
