@@ -196,9 +196,18 @@ def run_views(steps, warmup, tune=False):
 
 
 def device_used_bytes():
-    import torch
-    free, total = torch.cuda.mem_get_info()
-    return total - free
+    """Device memory in use on device 0 (hipMemGetInfo total - free), from the
+    HIP runtime the library itself runs on."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    free, total = ctypes.c_size_t(), ctypes.c_size_t()
+    rc = hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total))
+    if rc != 0:
+        raise RuntimeError(f"hipMemGetInfo failed ({rc})")
+    return total.value - free.value
 
 
 def run_smooth(steps, warmup, tune=False):

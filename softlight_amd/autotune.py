@@ -100,8 +100,9 @@ def _time(ctx, render_frame, frames):
 
 def tune(ctx, render_frame, signature: str, frames: int = 3, path: str | None = DEFAULT_CACHE) -> dict:
     """Time render_frame() under each tile configuration, then each occupancy
-    cap, each bin capacity and depth rejection off / on for the best tile;
-    set and return the best knobs with every timing (also cached)."""
+    cap, each bin capacity and depth rejection off / on for the best tile
+    (with rejection kept, every tile again with it on); set and return the
+    best knobs with every timing (also cached)."""
     from softlight_amd.api import SLError
 
     configs, occupancy, splits, rejects = {}, {}, {}, {}
@@ -140,11 +141,27 @@ def tune(ctx, render_frame, signature: str, frames: int = 3, path: str | None = 
     # visibility attachment) time the same either way
     rj_best = rejects[True]["frame_ms"] < 0.98 * rejects[False]["frame_ms"]
     ctx.set_raster_depth_reject(rj_best)
+    frame_ms = rejects[rj_best]["frame_ms"]
+    configs_rj = {}
+    if rj_best:
+        # rejection changes which tile is fastest: the tiles again with it on
+        for cfg in range(num_configs(ctx)):
+            ctx.set_raster_config(cfg)
+            try:
+                configs_rj[cfg] = _time(ctx, render_frame, frames)
+            except SLError as e:
+                configs_rj[cfg] = {"unsupported": str(e)}
+        ok = [k for k in configs_rj if "frame_ms" in configs_rj[k]]
+        cfg_rj = min(ok, key=lambda k: configs_rj[k]["frame_ms"])
+        if configs_rj[cfg_rj]["frame_ms"] < frame_ms:
+            best, frame_ms = cfg_rj, configs_rj[cfg_rj]["frame_ms"]
+        ctx.set_raster_config(best)
     entry = {"config": best, "tile": configs[best]["tile"], "wg_per_cu": wg_best, "split": sp_best,
-             "depth_reject": rj_best, "frame_ms": rejects[rj_best]["frame_ms"]}
+             "depth_reject": rj_best, "frame_ms": frame_ms}
     result = {**entry, "timings": {"config": configs, "wg_per_cu": occupancy,
                                    "split": {("default" if k == SPLIT_DEFAULT else k): v for k, v in splits.items()},
-                                   "depth_reject": {("on" if k else "off"): v for k, v in rejects.items()}}}
+                                   "depth_reject": {("on" if k else "off"): v for k, v in rejects.items()},
+                                   "config_with_depth_reject": configs_rj}}
     if path:
         cache = load_cache(path)
         cache.setdefault(device_name(), {})[signature] = entry
