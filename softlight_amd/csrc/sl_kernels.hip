@@ -992,7 +992,11 @@ __device__ __forceinline__ uint32_t xcd_block_order(uint32_t i, uint32_t n, uint
 
 constexpr uint32_t kSetupXcdBlock = 128;   // setup chunks per XCD run (measured: 64-512 within 2%, 128 best)
 
-__global__ __launch_bounds__(kSetupBlock, 8) void k_setup(SetupParams P)
+// Seven workgroups per CU (72 VGPRs): at eight the 64-VGPR cap spilled the
+// clip-counting loop (20 B per lane, 50 MB of scratch writes on the sphere
+// frame); at six nothing spills but the pass was slower (0.1067 ms at seven,
+// 0.1081 at six, 0.1102 at eight).
+__global__ __launch_bounds__(kSetupBlock, 7) void k_setup(SetupParams P)
 {
     const uint32_t chunk = xcd_block_order(blockIdx.x, gridDim.x, kSetupXcdBlock);
     const int lane = threadIdx.x & 63;
