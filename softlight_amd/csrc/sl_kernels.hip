@@ -1679,8 +1679,10 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(const uint32_t* __restric
     }
     uint32_t totE, totW;
     block_excl_scan2(se, sw, totE, totW, sWave);
-    // ranges and the launch-order buckets
-    uint32_t run = se;
+    // ranges and the launch-order buckets: a thread's consecutive tiles
+    // mostly share a bucket, so each run of equal buckets is counted with one
+    // LDS atomic (same-address atomics serialise)
+    uint32_t run = se, runCount = 0;
 #pragma unroll
     for (int j = 0; j < kTilesPerThread; ++j)
     {
@@ -1688,7 +1690,15 @@ __global__ __launch_bounds__(1024) void k_bin_tilescan(const uint32_t* __restric
         {
             range[t0 + j] = make_uint2(run, run + tot[j]);
             run += tot[j];
-            atomicAdd(&sCnt[32 - __clz((int)tot[j])], 1u);
+            const int b = 32 - __clz((int)tot[j]);
+            ++runCount;
+            const bool last = !(j + 1 < kTilesPerThread && j + 1 < per && t0 + j + 1 < numTiles) ||
+                              32 - __clz((int)tot[j + 1 < kTilesPerThread ? j + 1 : j]) != b;
+            if (last)
+            {
+                atomicAdd(&sCnt[b], runCount);
+                runCount = 0;
+            }
         }
     }
     // split decisions: accepted while the parts wanted so far (in tile order) fit
