@@ -487,7 +487,7 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
     const bool hiz = HIZ && !P.depthHalf && range.y - range.x >= (uint32_t)kHizMinRecs;
     constexpr int kRecWords = sizeof(SetupRec) / 4;
     constexpr int kLaneWords = kBatch * kRecWords / 64;
-    static_assert(kBatch * kRecWords % 64 == 0 && kBatch <= 64, "record staging");
+    static_assert(kBatch * kRecWords % 64 == 0 && kBatch < 32, "record staging; 32-bit record masks");
     __shared__ uint32_t sRidW[NW][kBatch];
     __shared__ SetupRec sRecW[NW][kBatch];
     uint32_t* sRid = sRidW[wave];
@@ -641,14 +641,28 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
             if (wl < kBatch && ((rejMask >> wl) & 1u)) tested += rejTested;
         }
         uint32_t cand = 0;
-        for (int bi = 0; bi < nb; ++bi)
+        if constexpr (HIZ)
         {
-            const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hitMask, bi);
-            const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(hitMask >> 32), bi);
-            const uint64_t m = ((uint64_t)mhi << 32) | mlo;
-            cand |= (uint32_t)((m >> wl) & 1ull) << bi;
+            // the records left after rejection (rejected ones are counted above)
+            for (uint32_t todo = ((1u << nb) - 1u) & ~rejMask; todo != 0u; todo &= todo - 1u)
+            {
+                const int bi = __ffs(todo) - 1;
+                const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hitMask, bi);
+                const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(hitMask >> 32), bi);
+                const uint64_t m = ((uint64_t)mhi << 32) | mlo;
+                cand |= (uint32_t)((m >> wl) & 1ull) << bi;
+            }
         }
-        if constexpr (HIZ) cand &= ~rejMask;   // rejected records are counted above
+        else
+        {
+            for (int bi = 0; bi < nb; ++bi)
+            {
+                const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hitMask, bi);
+                const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(hitMask >> 32), bi);
+                const uint64_t m = ((uint64_t)mhi << 32) | mlo;
+                cand |= (uint32_t)((m >> wl) & 1ull) << bi;
+            }
+        }
         // Each lane walks its own candidates in record order.  A pixel is only
         // ever touched by the lane that owns it, so lanes may be at different
         // records at the same time and every pixel still sees its fragments
