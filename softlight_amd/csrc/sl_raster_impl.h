@@ -640,28 +640,18 @@ __global__ __launch_bounds__(TH * 4 * SEGS) void k_raster(RasterParams P)
             rejMask = (uint32_t)(__ballot(rej) & before);
             if (wl < kBatch && ((rejMask >> wl) & 1u)) tested += rejTested;
         }
+        // only the records that reach some lane of the wave (and, with
+        // rejection, are left after it: rejected ones are counted above)
         uint32_t cand = 0;
-        if constexpr (HIZ)
+        uint32_t todo = (uint32_t)__ballot(hitMask != 0ull);   // lanes nb.. hold no mask
+        if constexpr (HIZ) todo &= ~rejMask;
+        for (; todo != 0u; todo &= todo - 1u)
         {
-            // the records left after rejection (rejected ones are counted above)
-            for (uint32_t todo = ((1u << nb) - 1u) & ~rejMask; todo != 0u; todo &= todo - 1u)
-            {
-                const int bi = __ffs(todo) - 1;
-                const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hitMask, bi);
-                const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(hitMask >> 32), bi);
-                const uint64_t m = ((uint64_t)mhi << 32) | mlo;
-                cand |= (uint32_t)((m >> wl) & 1ull) << bi;
-            }
-        }
-        else
-        {
-            for (int bi = 0; bi < nb; ++bi)
-            {
-                const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hitMask, bi);
-                const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(hitMask >> 32), bi);
-                const uint64_t m = ((uint64_t)mhi << 32) | mlo;
-                cand |= (uint32_t)((m >> wl) & 1ull) << bi;
-            }
+            const int bi = __ffs(todo) - 1;
+            const uint32_t mlo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)hitMask, bi);
+            const uint32_t mhi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(hitMask >> 32), bi);
+            const uint64_t m = ((uint64_t)mhi << 32) | mlo;
+            cand |= (uint32_t)((m >> wl) & 1ull) << bi;
         }
         // Each lane walks its own candidates in record order.  A pixel is only
         // ever touched by the lane that owns it, so lanes may be at different
