@@ -351,7 +351,7 @@ static_assert(sizeof(FrameCtrl::CounterSlot) == 64, "one cache line per counter 
 // pairs.
 constexpr uint64_t kNoPositions = ~0ull;
 constexpr uint32_t kVertexBlock = 256;                  // threads of a vertex-pass workgroup
-constexpr uint32_t kVertexPerThread = 4;                // entries per thread and work item
+constexpr uint32_t kVertexPerThread = 2;                // entries per thread and work item (1, 4, 8 measured slower on the sphere frame)
 constexpr uint32_t kVertexItem = kVertexBlock * kVertexPerThread;   // entries per work item
 constexpr uint64_t kMaxPositionEntries = 1ull << 27;   // 2 GiB of positions per batch at most
 constexpr uint64_t kMinPassEntries = 1ull << 16;       // smaller draws keep the setup pass's vertex cache
